@@ -1,0 +1,214 @@
+/*
+ * weldform_hip.h -- C-ABI of the MI355X explicit SPH step (libweldform_hip.so).
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every call returns 0 on
+ * success or a negative WF_E* code; wf_last_error() gives the message.  No
+ * C++ exception crosses this boundary.  One host thread per context; each
+ * context owns its device memory and one HIP stream (default priority).
+ *
+ * Particle data crosses the boundary in the reference's layout and in the
+ * caller's particle order: Vec3_t fields as 3 doubles per particle (x,y,z),
+ * Mat3_t fields as 9 doubles per particle (row-major 3x3).  Internally the
+ * state is kept in a Morton-sorted structure-of-arrays; the library maps
+ * indices both ways.
+ *
+ * Reference interfaces replaced (luchete80/WeldForm):
+ *   wf_create/wf_set_*       <- SPH::Domain / SPH::Particle public members
+ *                               (Source/Domain.h:113-537, Source/Particle.h:50-265)
+ *   wf_add_velocity_bc etc.  <- Domain::bConds / amps and the UserAcc hook
+ *                               (Source/Domain.h:69-111, Source/WeldForm.cpp:51-120)
+ *   wf_init                  <- set-up part of SolveDiffUpdateLeapFrog
+ *                               (Source/SolverLeapfrog.cpp:46-103)
+ *   wf_step(WF_LEAPFROG)     <- one iteration of SolveDiffUpdateLeapFrog's loop
+ *                               (Source/SolverLeapfrog.cpp:124-362)
+ *   wf_step(WF_KICKDRIFT)    <- one iteration of SolveDiffUpdateKickDrift's loop
+ *                               (Source/SolverKickDrift.cpp:117-330)
+ *   wf_rebuild_neighbours    <- ClearNbData + MainNeighbourSearch + CalcPairPosList
+ *                               (Source/Neighbour.cpp:346-359,479-497,617-694)
+ *   wf_eval(WF_EVAL_ACCEL)   <- CalcAccel + AccelReduction (Source/InteractionAlt.cpp:12-260)
+ *   wf_eval(WF_EVAL_DENSITY) <- CalcDensInc + DensReduction (Source/InteractionAlt.cpp:483-640)
+ *   wf_eval(WF_EVAL_RATES)   <- CalcRateTensors + RateTensorsReduction
+ *                               (Source/InteractionAlt.cpp:263-479)
+ *   wf_eval(WF_EVAL_STRESS)  <- Particle::CalcStressStrain for every particle
+ *                               (Source/Particle.cpp:960-1067)
+ *   wf_gen_cylinder          <- Domain::AddCylinderLength (Source/Domain.cpp:708-860)
+ */
+#ifndef WELDFORM_HIP_H
+#define WELDFORM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_ABI_VERSION 1
+
+/* error codes */
+#define WF_OK 0
+#define WF_EINVAL (-1)      /* bad argument                                  */
+#define WF_EHIP (-2)        /* HIP runtime error                             */
+#define WF_ESTATE (-3)      /* call out of order (e.g. step before init)     */
+#define WF_ENBOVERFLOW (-4) /* a particle has more neighbours than max_nb    */
+#define WF_ENAN (-5)        /* non-finite value detected in the state        */
+#define WF_EUNSUPPORTED (-6)/* feature outside the implemented path          */
+#define WF_ENOMEM (-7)
+
+/* precision of the pair passes */
+#define WF_FP64 0           /* double everywhere                              */
+#define WF_MIXED 1          /* pair arithmetic in float; integration, return
+                               map and global reductions in double           */
+
+/* integration schemes */
+#define WF_LEAPFROG 0
+#define WF_KICKDRIFT 1
+
+/* material models (Source/Particle.h:34-37) */
+#define WF_BILINEAR 0
+#define WF_HOLLOMON 1
+#define WF_JOHNSON_COOK 2
+
+/* double fields (wf_set_field / wf_get_field); width per particle in [] */
+#define WF_F_X 0            /* [3] position            Particle::x          */
+#define WF_F_V 1            /* [3] velocity            Particle::v          */
+#define WF_F_A 2            /* [3] acceleration        Particle::a          */
+#define WF_F_DISP 3         /* [3] Particle::Displacement                     */
+#define WF_F_DENSITY 4      /* [1] Particle::Density                          */
+#define WF_F_MASS 5         /* [1] Particle::Mass                             */
+#define WF_F_H 6            /* [1] Particle::h                                */
+#define WF_F_SIGMA 7        /* [9] Particle::Sigma                            */
+#define WF_F_SHEAR 8        /* [9] Particle::ShearStress                      */
+#define WF_F_STRAIN_RATE 9  /* [9] Particle::StrainRate                       */
+#define WF_F_ROT_RATE 10    /* [9] Particle::RotationRate                     */
+#define WF_F_STRAIN 11      /* [9] Particle::Strain                           */
+#define WF_F_STRAIN_PL 12   /* [9] Particle::Strain_pl                        */
+#define WF_F_PL_STRAIN 13   /* [1] Particle::pl_strain                        */
+#define WF_F_DPL_STRAIN 14  /* [1] Particle::delta_pl_strain                  */
+#define WF_F_SIGMAY 15      /* [1] Particle::Sigmay                           */
+#define WF_F_EFF_SR 16      /* [1] Particle::eff_strain_rate                  */
+#define WF_F_PRESSURE 17    /* [1] Particle::Pressure                         */
+#define WF_F_T 18           /* [1] Particle::T                                */
+#define WF_F_DTDT 19        /* [1] Particle::dTdt                             */
+#define WF_F_DDENSITY 20    /* [1] Particle::dDensity                         */
+#define WF_F_Q_PLHEAT 21    /* [1] Particle::q_plheat                         */
+#define WF_F_STRAIN_PL_INCR 22 /* [9] Particle::Strain_pl_incr                */
+#define WF_F_Q_CONV 23      /* [1] Particle::q_conv                           */
+#define WF_F_H_CONV 24      /* [1] Particle::h_conv                           */
+#define WF_F_T_INF 25       /* [1] Particle::T_inf                            */
+#define WF_F_Q_SOURCE 26    /* [1] Particle::q_source                         */
+#define WF_F_PS_ENERGY 27   /* [1] Particle::ps_energy                        */
+#define WF_NFIELDS 28
+
+/* int fields */
+#define WF_I_ID 0           /* Particle::ID (zone id used by BCs)            */
+#define WF_I_MATERIAL 1     /* index into the wf_set_material table; pairs
+                               form only between particles of one index,
+                               as Particle::Material gates SMPairs          */
+#define WF_I_FREE 2         /* Particle::IsFree (only free-free pairs)        */
+#define WF_I_THERMAL_BC 3   /* Particle::Thermal_BC (1 = convection)          */
+#define WF_I_NB 4           /* read-only: Particle::Nb (neighbour count)      */
+#define WF_NIFIELDS 5
+
+/* single-pass evaluations (wf_eval) */
+#define WF_EVAL_ACCEL 0     /* a   <- sum over pairs, from x, v, Density, Sigma */
+#define WF_EVAL_DENSITY 1   /* dDensity <- from x, v, Density                  */
+#define WF_EVAL_RATES 2     /* StrainRate, RotationRate <- from x, v, Density  */
+#define WF_EVAL_STRESS 3    /* CalcStressStrain(dt) of every particle using the
+                               current StrainRate / RotationRate               */
+
+typedef struct wf_config {
+  int32_t precision;        /* WF_FP64 or WF_MIXED                             */
+  int32_t ts_nb_inc;        /* Domain::ts_nb_inc (neighbour search period, 5)  */
+  int32_t max_nb;           /* neighbour capacity per particle (1..128)       */
+  int32_t thermal_solver;   /* Domain::thermal_solver                          */
+  int32_t pl_heating;       /* Domain::pl_heating                              */
+  int32_t device;           /* HIP device ordinal                              */
+  int32_t reserved[2];
+  double cellfac;           /* Domain::Cellfac (2 for the cubic spline)       */
+  double pl_work_heat_frac; /* Domain::pl_work_heat_frac (0.9)                */
+  double dom_max[3];        /* Domain::DomMax (default -1e11)                  */
+  double dom_min[3];        /* Domain::DomMin (default +1e11)                  */
+} wf_config;
+
+typedef struct wf_material {
+  int32_t model;            /* WF_BILINEAR / WF_HOLLOMON / WF_JOHNSON_COOK     */
+  int32_t reserved;
+  double E, nu;             /* Elastic_ (Source/Material.h:64-74)              */
+  double G;                 /* Particle::G                                     */
+  double Cs;                /* Particle::Cs                                    */
+  double ref_density;       /* Particle::RefDensity                            */
+  double P0;                /* Particle::P0                                    */
+  double alpha, beta;       /* Particle::Alpha / Beta (artificial viscosity)   */
+  double Ep;                /* Particle::Ep (bilinear plastic modulus)         */
+  double hol_K, hol_m, hol_sy0;                 /* Hollomon                     */
+  double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt; /* Johnson-Cook  */
+  double k_T, cp_T, th_exp; /* Particle::k_T, cp_T, th_exp                     */
+} wf_material;
+
+typedef struct wf_scalars {
+  double time;              /* Domain::Time                                    */
+  double dt;                /* Domain::deltat                                  */
+  int64_t steps;
+  int64_t nb_searches;      /* neighbour searches performed                    */
+  int64_t npairs;           /* unordered free-free pairs (SMPairs size)        */
+  int32_t max_nb_count;     /* largest per-particle neighbour count            */
+  int32_t reserved;
+  double kin_energy_sum;    /* Domain::kin_energy_sum                          */
+  double int_energy_sum;    /* Domain::int_energy_sum                          */
+  double plastic_work;      /* Domain::plastic_work                            */
+  double max_pl_strain;     /* max over particles of pl_strain                 */
+  double max_T, min_T;      /* Domain::m_maxT / m_minT                         */
+} wf_scalars;
+
+int wf_abi_version(void);
+const char *wf_last_error(const void *ctx); /* ctx may be NULL (last global) */
+int wf_device_count(int32_t *count);
+
+/* life cycle */
+int wf_create(const wf_config *cfg, int64_t n_particles, void **ctx);
+int wf_destroy(void *ctx);
+
+/* state */
+int wf_set_material(void *ctx, int32_t index, const wf_material *m);
+int wf_set_field(void *ctx, int32_t field, const double *host);
+int wf_get_field(void *ctx, int32_t field, double *host);
+int wf_set_int_field(void *ctx, int32_t field, const int32_t *host);
+int wf_get_int_field(void *ctx, int32_t field, int32_t *host);
+
+/* boundary conditions, applied where the reference calls GeneralAfter */
+int wf_add_amplitude(void *ctx, int32_t id, int32_t npts, const double *time, const double *value);
+int wf_add_velocity_bc(void *ctx, int32_t zone, int32_t value_type, const double value[3],
+                       int32_t amp_id, double amp_factor);
+int wf_add_temperature_bc(void *ctx, int32_t zone, double T);
+
+/* solve */
+int wf_init(void *ctx, double dt);
+int wf_step(void *ctx, int32_t scheme, int64_t nsteps); /* enqueues; see wf_sync */
+int wf_sync(void *ctx);                                 /* waits, reports device errors */
+int wf_get_scalars(void *ctx, wf_scalars *out);
+/* reaction sum(m*a) of every particle of a velocity-BC zone, taken where
+ * the reference first calls GeneralAfter in a step (after AccelReduction,
+ * before the BC zeroes a).  Copies the last min(cap, steps) entries. */
+int wf_get_reaction_history(void *ctx, int32_t zone, double *out3, int64_t cap, int64_t *count);
+
+/* neighbours */
+int wf_rebuild_neighbours(void *ctx);
+int wf_get_neighbour_pairs(void *ctx, int32_t *pairs2, int64_t cap, int64_t *count);
+
+/* single passes for parity checks (no time advance) */
+int wf_eval(void *ctx, int32_t what, double dt);
+
+/* kernel timing with HIP events on the context's stream.
+ * kernel: 0 accel, 1 density+drift, 2 rates+stress, 3 neighbour build,
+ *         4 whole step.  avg_ms is the mean duration per launch. */
+int wf_set_timing(void *ctx, int32_t enable);
+int wf_get_kernel_time(void *ctx, int32_t kernel, double *avg_ms, int64_t *launches);
+int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_count);
+
+/* host-only geometry helper: AddCylinderLength lattice (3 doubles per
+ * particle).  Returns the particle count; writes at most cap particles. */
+int64_t wf_gen_cylinder(const double V[3], double Rxy, double Lz, double r, double *out, int64_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1067 @@
+/*
+ * wf_oracle.c -- CPU oracle (TEST INFRASTRUCTURE, see wf_oracle.h).
+ *
+ * Plain C restatement of the reference's explicit SPH step with Nproc=1
+ * pair order.  Arithmetic expressions keep the reference's operand order
+ * (the 3x3 helpers of External/matvec.h:549-850 are element-wise, so every
+ * matrix expression below is written per element in the same order).
+ * Build with -ffp-contract=off so each line rounds as written.
+ */
+#include "wf_oracle.h"
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MIN_PS_FOR_NBSEARCH 1.e-6 /* Source/Domain.cpp:28 */
+
+static char g_err[512];
+static int g_threads = 1;
+static int fail(const char *msg) {
+  snprintf(g_err, sizeof g_err, "%s", msg);
+  return -1;
+}
+const char *wfo_last_error(void) { return g_err; }
+void wfo_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+static const int k_fdim[WFO_NFIELDS] = {
+    3, 3, 3, 3, 3,        /* X V A XPREV DISP */
+    1, 1, 1, 1, 1, 1,     /* RHO DRHO MASS H REFRHO P */
+    1, 1, 1, 1, 1,        /* CS P0 G ALPHA BETA */
+    9, 9, 9, 9, 9, 9, 9,  /* SIGMA SHEAR SR RR STRAIN EPL EPLINC */
+    1, 1, 1, 1,           /* PLSTRAIN DPLSTRAIN SIGMAY EFFSR */
+    1, 1, 1, 1, 1, 1,     /* EP ET T DTDT KT CPT */
+    1, 1, 1, 1, 1,        /* THEXP QCONV QSRC QPL QCONT */
+    1, 1, 1,              /* HCONV TINF PSENERGY */
+    3                     /* NORMAL */
+};
+
+typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T; } bc_t;
+typedef struct { int id, n; double t[WFO_MAX_AMP_PTS], v[WFO_MAX_AMP_PTS]; } amp_t;
+
+struct wfo_domain {
+  int n, solid_count;
+  double *f[WFO_NFIELDS];
+  int *fi[WFO_NIFIELDS];
+  wfo_material mats[256];
+  /* domain flags / parameters (Source/Domain.cpp:78-175) */
+  double Cellfac, DomMax[3], DomMin[3], Time, deltat, pl_work_heat_frac;
+  int ts_nb_inc, thermal_solver, pl_heating;
+  /* cell grid (Source/Neighbour.cpp:10-139) */
+  double TRPR[3], BLPF[3], CellSize[3], hmax;
+  int CellNo[3];
+  int *HOC, *LL;
+  /* pairs: SMPairs in Nproc=1 order, (temp1, temp2) */
+  int *pairs;
+  int64_t npairs, cap_pairs, n_other_pairs;
+  /* Nishimura tables (Source/Neighbour.cpp:570-694) */
+  int *ipair, *jpair, *Anei, *Aref;
+  double *pair_force, *pair_densinc, *pair_sr, *pair_rr;
+  /* solver state (Source/SolverLeapfrog.cpp:71-77) */
+  int isfirst, isyielding, ts_i, nb_cleared, initialized;
+  double max_pl_strain, kin_energy_sum, int_energy_sum, plastic_work;
+  double m_maxT, m_minT;
+  int64_t steps, nb_searches;
+  /* BCs */
+  bc_t bcs[WFO_MAX_BCS];
+  int nbcs;
+  amp_t amps[WFO_MAX_AMPS];
+  int namps;
+  double react[WFO_MAX_ZONES][3];
+  int react_pending;
+  double *temp_work, *dkin_work;
+};
+
+static void *xcalloc(size_t n, size_t s) {
+  void *p = calloc(n ? n : 1, s);
+  if (!p) { fprintf(stderr, "wf_oracle: out of memory\n"); abort(); }
+  return p;
+}
+
+wfo_domain *wfo_create(int n) {
+  wfo_domain *d = (wfo_domain *)xcalloc(1, sizeof(wfo_domain));
+  d->n = n;
+  d->solid_count = n;
+  for (int k = 0; k < WFO_NFIELDS; ++k) d->f[k] = (double *)xcalloc((size_t)n * k_fdim[k], sizeof(double));
+  for (int k = 0; k < WFO_NIFIELDS; ++k) d->fi[k] = (int *)xcalloc((size_t)n, sizeof(int));
+  /* Particle constructor defaults (Source/Particle.cpp:28-157) */
+  for (int i = 0; i < n; ++i) {
+    d->fi[WFO_FREE][i] = 1;
+    d->f[WFO_H][i] = 0.0;
+  }
+  /* Domain constructor defaults (Source/Domain.cpp:78-175) */
+  d->Cellfac = 2.0;
+  for (int k = 0; k < 3; ++k) { d->DomMax[k] = -100000000000.0; d->DomMin[k] = 100000000000.0; }
+  d->ts_nb_inc = 5;
+  d->pl_work_heat_frac = 0.9;
+  d->LL = (int *)xcalloc((size_t)n, sizeof(int));
+  d->ipair = (int *)xcalloc((size_t)n, sizeof(int));
+  d->jpair = (int *)xcalloc((size_t)n, sizeof(int));
+  d->Anei = (int *)xcalloc((size_t)n * WFO_MAX_NB, sizeof(int));
+  d->Aref = (int *)xcalloc((size_t)n * WFO_MAX_NB, sizeof(int));
+  d->temp_work = (double *)xcalloc((size_t)n, sizeof(double));
+  d->dkin_work = (double *)xcalloc((size_t)n, sizeof(double));
+  d->isfirst = 1;
+  return d;
+}
+
+void wfo_destroy(wfo_domain *d) {
+  if (!d) return;
+  for (int k = 0; k < WFO_NFIELDS; ++k) free(d->f[k]);
+  for (int k = 0; k < WFO_NIFIELDS; ++k) free(d->fi[k]);
+  free(d->HOC); free(d->LL); free(d->pairs); free(d->ipair); free(d->jpair);
+  free(d->Anei); free(d->Aref); free(d->pair_force); free(d->pair_densinc);
+  free(d->pair_sr); free(d->pair_rr); free(d->temp_work); free(d->dkin_work);
+  free(d);
+}
+
+int wfo_n(const wfo_domain *d) { return d->n; }
+double *wfo_f64(wfo_domain *d, int f) { return (f >= 0 && f < WFO_NFIELDS) ? d->f[f] : NULL; }
+int *wfo_i32(wfo_domain *d, int f) { return (f >= 0 && f < WFO_NIFIELDS) ? d->fi[f] : NULL; }
+
+int wfo_set_material(wfo_domain *d, int idx, const wfo_material *m) {
+  if (idx < 0 || idx >= 256) return fail("material index out of range");
+  d->mats[idx] = *m;
+  if (m->model == WFO_HOLLOMON) {
+    /* Hollomon ctor checks eps0 <= eps1 (Source/Material.cpp:48-57) */
+    double eps0 = m->sy0 / m->E, eps1 = pow(m->sy0 / m->K, 1. / m->m);
+    if (eps0 > eps1) return fail("Hollomon material bad definition");
+  }
+  return 0;
+}
+
+int wfo_set_param(wfo_domain *d, const char *s, double v) {
+  if (!strcmp(s, "Cellfac")) d->Cellfac = v;
+  else if (!strcmp(s, "ts_nb_inc")) d->ts_nb_inc = (int)v;
+  else if (!strcmp(s, "thermal_solver")) d->thermal_solver = (int)v;
+  else if (!strcmp(s, "pl_heating")) d->pl_heating = (int)v;
+  else if (!strcmp(s, "pl_work_heat_frac")) d->pl_work_heat_frac = v;
+  else if (!strcmp(s, "solid_count")) d->solid_count = (int)v;
+  else if (!strcmp(s, "Time")) d->Time = v;
+  else if (!strcmp(s, "deltat")) d->deltat = v;
+  else if (!strncmp(s, "DomMax", 6) && s[6] >= '0' && s[6] <= '2') d->DomMax[s[6] - '0'] = v;
+  else if (!strncmp(s, "DomMin", 6) && s[6] >= '0' && s[6] <= '2') d->DomMin[s[6] - '0'] = v;
+  else return fail("unknown parameter");
+  return 0;
+}
+
+double wfo_get_param(const wfo_domain *d, const char *s) {
+  if (!strcmp(s, "Time")) return d->Time;
+  if (!strcmp(s, "deltat")) return d->deltat;
+  if (!strcmp(s, "kin_energy_sum")) return d->kin_energy_sum;
+  if (!strcmp(s, "int_energy_sum")) return d->int_energy_sum;
+  if (!strcmp(s, "plastic_work")) return d->plastic_work;
+  if (!strcmp(s, "max_pl_strain")) return d->max_pl_strain;
+  if (!strcmp(s, "steps")) return (double)d->steps;
+  if (!strcmp(s, "nb_searches")) return (double)d->nb_searches;
+  if (!strcmp(s, "hmax")) return d->hmax;
+  if (!strcmp(s, "m_maxT")) return d->m_maxT;
+  if (!strcmp(s, "m_minT")) return d->m_minT;
+  if (!strcmp(s, "n_other_pairs")) return (double)d->n_other_pairs;
+  if (!strncmp(s, "CellNo", 6)) return d->CellNo[s[6] - '0'];
+  if (!strncmp(s, "CellSize", 8)) return d->CellSize[s[8] - '0'];
+  if (!strncmp(s, "BLPF", 4)) return d->BLPF[s[4] - '0'];
+  if (!strncmp(s, "TRPR", 4)) return d->TRPR[s[4] - '0'];
+  return NAN;
+}
+
+/* ------------------------------------------------------------------ */
+/* Cubic-spline kernel, 3D (Source/Functions.cpp:25-37, 74-87)         */
+double wfo_kernel(double q, double h) {
+  double C = 1.0 / (h * h * h * M_PI);
+  if (q < 1.0) return C * (1.0 - (3.0 / 2.0) * q * q + (3.0 / 4.0) * q * q * q);
+  else if (q < 2.0) return C * ((1.0 / 4.0) * (2.0 - q) * (2.0 - q) * (2.0 - q));
+  return 0.0;
+}
+double wfo_grad_kernel(double q, double h) {
+  double C = 1.0 / (h * h * h * h * M_PI);
+  if (q == 0.0) return C / h * (-3.0 + (9.0 / 2.0) * q);
+  else if (q < 1.0) return C / (q * h) * (-3.0 * q + (9.0 / 4.0) * q * q);
+  else if (q < 2.0) return C / (q * h) * ((-3.0 / 4.0) * (2.0 - q) * (2.0 - q));
+  return 0.0;
+}
+
+static inline double norm3(const double *a) {
+  return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+}
+static inline double dot3(const double *a, const double *b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* ------------------------------------------------------------------ */
+/* Geometry: calcHalfPartCount + AddCylinderLength, 3D, no ghost rows  */
+/* (Source/Domain.cpp:674-689, 708-860)                                 */
+static int half_part_count(double r, double R, int xinc) {
+  int ypartcount = -1;
+  if (xinc > 0) {
+    ypartcount = 1;
+    double yp = r;
+    double xp = r + (double)(xinc - 1) * 2. * r;
+    double rad = sqrt(yp * yp + xp * xp);
+    while (rad <= R - r) {
+      yp += 2. * r;
+      rad = sqrt(yp * yp + xp * xp);
+      ypartcount++;
+    }
+    ypartcount -= 1;
+  }
+  return ypartcount;
+}
+
+int64_t wfo_gen_cylinder(const double V[3], double Rxy, double Lz, double r, double *out, int64_t cap) {
+  int64_t cnt = 0;
+  int numpartxy = half_part_count(r, Rxy, 1);
+  double zp = V[2] + r;
+  while (zp <= (V[2] + Lz - r)) {
+    double yp = V[1] - r - (2. * r * (numpartxy - 1));
+    int numypart = 2 * numpartxy;
+    int yinc = numpartxy, yinc_sign = -1;
+    for (int j = 0; j < numypart; j++) {
+      int numxpart = half_part_count(r, Rxy, yinc);
+      double xp = V[0] - r - (2. * r * (numxpart - 1));
+      for (int i = 0; i < 2 * numxpart; i++) {
+        if (out && cnt < cap) { out[3 * cnt] = xp; out[3 * cnt + 1] = yp; out[3 * cnt + 2] = zp; }
+        cnt++;
+        xp += 2. * r;
+      }
+      yp += 2. * r;
+      yinc += yinc_sign;
+      if (yinc < 1) { yinc = 1; yinc_sign = 1; }
+    }
+    zp += 2. * r;
+  }
+  return cnt;
+}
+
+/* AssignZone (Source/Domain.cpp:691-706) */
+int wfo_assign_zone(wfo_domain *d, const double start[3], const double end[3], int id) {
+  int cnt = 0;
+  for (int a = 0; a < d->n; a++) {
+    int inc = 1;
+    for (int i = 0; i < 3; i++)
+      if (d->f[WFO_X][3 * a + i] < start[i] || d->f[WFO_X][3 * a + i] > end[i]) inc = 0;
+    if (inc) { d->fi[WFO_ID][a] = id; cnt++; }
+  }
+  return cnt;
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary conditions: amplitude::getValAtTime (Source/Domain.h:75-87)*/
+static double amp_value(const amp_t *a, double t) {
+  int i = a->n - 1;
+  int end = 0;
+  while (!end) {
+    i--;
+    if (i == 0 || t > a->t[i]) end = 1;
+  }
+  return a->v[i] + (a->v[i + 1] - a->v[i]) / (a->t[i + 1] - a->t[i]) * (t - a->t[i]);
+}
+
+int wfo_add_velocity_bc(wfo_domain *d, int zone, int value_type, const double value[3], int amp_id, double amp_factor) {
+  if (d->nbcs >= WFO_MAX_BCS) return fail("too many BCs");
+  bc_t *b = &d->bcs[d->nbcs++];
+  memset(b, 0, sizeof *b);
+  b->zone = zone; b->type = 0; b->value_type = value_type; b->amp_id = amp_id; b->amp_factor = amp_factor;
+  for (int k = 0; k < 3; k++) b->value[k] = value[k];
+  return 0;
+}
+int wfo_add_temperature_bc(wfo_domain *d, int zone, double T) {
+  if (d->nbcs >= WFO_MAX_BCS) return fail("too many BCs");
+  bc_t *b = &d->bcs[d->nbcs++];
+  memset(b, 0, sizeof *b);
+  b->zone = zone; b->type = 2; b->T = T;
+  return 0;
+}
+int wfo_add_amplitude(wfo_domain *d, int id, int npts, const double *t, const double *v) {
+  if (d->namps >= WFO_MAX_AMPS || npts < 2 || npts > WFO_MAX_AMP_PTS) return fail("bad amplitude");
+  amp_t *a = &d->amps[d->namps++];
+  a->id = id; a->n = npts;
+  for (int k = 0; k < npts; k++) { a->t[k] = t[k]; a->v[k] = v[k]; }
+  return 0;
+}
+
+/* UserAcc (Source/WeldForm.cpp:51-92): velocity BC zeroes a and imposes v;
+ * temperature BC imposes T.  The first call of each step also records the
+ * reaction sum(m*a) of every velocity-BC zone before a is zeroed. */
+int wfo_general_after(wfo_domain *d) {
+  double *x_a = d->f[WFO_A], *v = d->f[WFO_V], *m = d->f[WFO_MASS], *T = d->f[WFO_T];
+  int *id = d->fi[WFO_ID];
+  if (d->react_pending) {
+    memset(d->react, 0, sizeof d->react);
+    for (int b = 0; b < d->nbcs; b++) {
+      if (d->bcs[b].type != 0) continue;
+      int z = d->bcs[b].zone;
+      if (z < 0 || z >= WFO_MAX_ZONES) continue;
+      double s[3] = {0, 0, 0};
+      for (int i = 0; i < d->n; i++)
+        if (id[i] == z)
+          for (int k = 0; k < 3; k++) s[k] += m[i] * x_a[3 * i + k];
+      for (int k = 0; k < 3; k++) d->react[z][k] = s[k];
+    }
+    d->react_pending = 0;
+  }
+  for (int i = 0; i < d->n; i++) {
+    for (int b = 0; b < d->nbcs; b++) {
+      const bc_t *bc = &d->bcs[b];
+      if (id[i] != bc->zone) continue;
+      if (bc->type == 0) {
+        if (bc->value_type == 0) {
+          for (int k = 0; k < 3; k++) { x_a[3 * i + k] = 0.0; v[3 * i + k] = bc->value[k]; }
+        } else if (bc->value_type == 1) {
+          for (int j = 0; j < d->namps; j++) {
+            if (d->amps[j].id != bc->amp_id) continue;
+            double val = bc->amp_factor * amp_value(&d->amps[j], d->Time);
+            for (int k = 0; k < 3; k++) { x_a[3 * i + k] = 0.0; v[3 * i + k] = val * bc->value[k]; }
+          }
+        }
+      } else if (bc->type == 2) {
+        T[i] = bc->T;
+      }
+    }
+  }
+  return 0;
+}
+
+int wfo_reaction(const wfo_domain *d, int zone, double out[3]) {
+  if (zone < 0 || zone >= WFO_MAX_ZONES) return fail("zone out of range");
+  for (int k = 0; k < 3; k++) out[k] = d->react[zone][k];
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* InitialChecks (Source/Domain.cpp:2097-2113): EOS pressure (PresEq 0) */
+static void initial_checks(wfo_domain *d) {
+  for (int i = 0; i < d->n; i++)
+    d->f[WFO_P][i] = d->f[WFO_P0][i] + (d->f[WFO_CS][i] * d->f[WFO_CS][i]) * (d->f[WFO_RHO][i] - d->f[WFO_REFRHO][i]);
+}
+
+/* CellInitiate, 3D non-periodic (Source/Neighbour.cpp:10-139) */
+static int cell_initiate(wfo_domain *d) {
+  const double *x = d->f[WFO_X];
+  if (!(norm3(d->TRPR) > 0.0) && !(norm3(d->BLPF) > 0.0)) {
+    for (int k = 0; k < 3; k++) d->BLPF[k] = d->TRPR[k] = x[k];
+    d->hmax = d->f[WFO_H][0];
+    for (int i = 0; i < d->n; i++) {
+      for (int k = 0; k < 3; k++) {
+        if (x[3 * i + k] > d->TRPR[k]) d->TRPR[k] = x[3 * i + k];
+        if (x[3 * i + k] < d->BLPF[k]) d->BLPF[k] = x[3 * i + k];
+      }
+      if (d->f[WFO_H][i] > d->hmax) d->hmax = d->f[WFO_H][i];
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    if (d->DomMax[k] > d->TRPR[k]) d->TRPR[k] = d->DomMax[k];
+    if (d->DomMin[k] < d->BLPF[k]) d->BLPF[k] = d->DomMin[k];
+  }
+  for (int k = 0; k < 3; k++) { d->TRPR[k] += d->hmax / 2; d->BLPF[k] -= d->hmax / 2; }
+  for (int k = 0; k < 3; k++) {
+    double e = (d->TRPR[k] - d->BLPF[k]) / (d->Cellfac * d->hmax);
+    if ((double)(ceil(e) - e) < (d->hmax / 10.0)) d->CellNo[k] = (int)ceil(e);
+    else d->CellNo[k] = (int)floor(e);
+    if (d->CellNo[k] <= 0) return fail("CellInitiate: zero cells along an axis");
+  }
+  for (int k = 0; k < 3; k++) d->CellSize[k] = (d->TRPR[k] - d->BLPF[k]) / d->CellNo[k];
+  int64_t nc = (int64_t)d->CellNo[0] * d->CellNo[1] * d->CellNo[2];
+  if (nc > (int64_t)1 << 30) return fail("CellInitiate: too many cells");
+  free(d->HOC);
+  d->HOC = (int *)xcalloc((size_t)nc, sizeof(int));
+  for (int64_t c = 0; c < nc; c++) d->HOC[c] = -1;
+  return 0;
+}
+
+#define HOC_AT(d, i, j, k) (d)->HOC[((int64_t)(i) * (d)->CellNo[1] + (j)) * (d)->CellNo[2] + (k)]
+
+/* CellReset + ListGenerate, 3D (Source/Neighbour.cpp:141-284) */
+static void list_generate(wfo_domain *d) {
+  int64_t nc = (int64_t)d->CellNo[0] * d->CellNo[1] * d->CellNo[2];
+  for (int64_t c = 0; c < nc; c++) d->HOC[c] = -1;
+  const double *x = d->f[WFO_X];
+  for (int a = 0; a < d->n; a++) {
+    int c[3];
+    for (int k = 0; k < 3; k++) {
+      c[k] = (int)(floor((x[3 * a + k] - d->BLPF[k]) / d->CellSize[k]));
+      if (c[k] < 0) c[k] = 0;
+      if (c[k] >= d->CellNo[k]) c[k] = d->CellNo[k] - 1;
+    }
+    int temp = HOC_AT(d, c[0], c[1], c[2]);
+    HOC_AT(d, c[0], c[1], c[2]) = a;
+    d->LL[a] = temp;
+  }
+}
+
+/* CheckRadius (Source/Neighbour.cpp:361-372) */
+static inline int check_radius(const wfo_domain *d, int p1, int p2) {
+  const double *x = d->f[WFO_X];
+  double h = (d->f[WFO_H][p1] + d->f[WFO_H][p2]) / 2;
+  double xij[3] = {x[3 * p1] - x[3 * p2], x[3 * p1 + 1] - x[3 * p2 + 1], x[3 * p1 + 2] - x[3 * p2 + 2]};
+  double rij = norm3(xij);
+  return (rij / h) <= d->Cellfac;
+}
+
+typedef struct { int *p; int64_t n, cap; int64_t other; } pairbuf_t;
+static void pb_push(pairbuf_t *b, int a, int c) {
+  if (b->n == b->cap) {
+    b->cap = b->cap ? 2 * b->cap : 4096;
+    b->p = (int *)realloc(b->p, (size_t)b->cap * 2 * sizeof(int));
+    if (!b->p) { fprintf(stderr, "wf_oracle: out of memory\n"); abort(); }
+  }
+  b->p[2 * b->n] = a; b->p[2 * b->n + 1] = c; b->n++;
+}
+
+/* AllocateNbPair without contact (Source/Neighbour.cpp:374-417): both free
+ * and same Material -> SMPairs.  Free/fixed and cross-material pairs feed
+ * paths this oracle does not model; they are only counted. */
+static inline void allocate_pair(const wfo_domain *d, int t1, int t2, pairbuf_t *b) {
+  if (!check_radius(d, t1, t2)) return;
+  const int *fr = d->fi[WFO_FREE], *mt = d->fi[WFO_MATERIAL];
+  if (fr[t1] || fr[t2]) {
+    if (mt[t1] == mt[t2] && fr[t1] && fr[t2]) pb_push(b, t1, t2);
+    else b->other++;
+  }
+}
+
+/* YZPlaneCellsNeighbourSearch (Source/Neighbour.cpp:418-477) */
+static void yz_plane_search(const wfo_domain *d, int q1, pairbuf_t *b) {
+  const int *CN = d->CellNo;
+  const int *LL = d->LL;
+  for (int q3 = 0; q3 < CN[2]; q3++)
+    for (int q2 = 0; q2 < CN[1]; q2++) {
+      int temp1 = HOC_AT(d, q1, q2, q3);
+      if (temp1 == -1) continue;
+      while (temp1 != -1) {
+        int temp2 = LL[temp1];
+        while (temp2 != -1) { allocate_pair(d, temp1, temp2, b); temp2 = LL[temp2]; }
+        if (q1 + 1 < CN[0]) {
+          temp2 = HOC_AT(d, q1 + 1, q2, q3);
+          while (temp2 != -1) { allocate_pair(d, temp1, temp2, b); temp2 = LL[temp2]; }
+        }
+        if (q2 + 1 < CN[1]) {
+          for (int i = q1 - 1; i <= q1 + 1; i++)
+            if (i < CN[0] && i >= 0) {
+              temp2 = HOC_AT(d, i, q2 + 1, q3);
+              while (temp2 != -1) { allocate_pair(d, temp1, temp2, b); temp2 = LL[temp2]; }
+            }
+        }
+        if (q3 + 1 < CN[2]) {
+          for (int j = q2 - 1; j <= q2 + 1; j++)
+            for (int i = q1 - 1; i <= q1 + 1; i++)
+              if (i < CN[0] && i >= 0 && j < CN[1] && j >= 0) {
+                temp2 = HOC_AT(d, i, j, q3 + 1);
+                while (temp2 != -1) { allocate_pair(d, temp1, temp2, b); temp2 = LL[temp2]; }
+              }
+        }
+        temp1 = LL[temp1];
+      }
+    }
+}
+
+/* MainNeighbourSearch (Source/Neighbour.cpp:346-359).  Planes are searched
+ * in parallel into per-plane buffers that are concatenated in q1 order,
+ * which is exactly the Nproc=1 SMPairs order. */
+static void main_neighbour_search(wfo_domain *d) {
+  int nx = d->CellNo[0];
+  pairbuf_t *bufs = (pairbuf_t *)xcalloc((size_t)nx, sizeof(pairbuf_t));
+#pragma omp parallel for schedule(dynamic) num_threads(g_threads)
+  for (int q1 = 0; q1 < nx; q1++) yz_plane_search(d, q1, &bufs[q1]);
+  int64_t tot = 0, other = 0;
+  for (int q1 = 0; q1 < nx; q1++) { tot += bufs[q1].n; other += bufs[q1].other; }
+  if (tot > d->cap_pairs) {
+    free(d->pairs);
+    d->cap_pairs = tot;
+    d->pairs = (int *)xcalloc((size_t)tot * 2, sizeof(int));
+  }
+  int64_t off = 0;
+  for (int q1 = 0; q1 < nx; q1++) {
+    if (bufs[q1].n) memcpy(d->pairs + 2 * off, bufs[q1].p, (size_t)bufs[q1].n * 2 * sizeof(int));
+    off += bufs[q1].n;
+    free(bufs[q1].p);
+  }
+  free(bufs);
+  d->npairs = tot;
+  d->n_other_pairs = other;
+  d->nb_cleared = 0;
+  d->nb_searches++;
+}
+
+/* CalcPairPosList (Source/Neighbour.cpp:617-694) with an overflow check
+ * the reference lacks; SaveNeighbourData (:499-532). */
+static int calc_pair_pos_list(wfo_domain *d) {
+  int64_t P = d->npairs;
+  free(d->pair_force); free(d->pair_densinc); free(d->pair_sr); free(d->pair_rr);
+  d->pair_force = (double *)xcalloc((size_t)P * 3, sizeof(double));
+  d->pair_densinc = (double *)xcalloc((size_t)P, sizeof(double));
+  d->pair_sr = (double *)xcalloc((size_t)P * 9, sizeof(double));
+  d->pair_rr = (double *)xcalloc((size_t)P * 9, sizeof(double));
+  memset(d->ipair, 0, sizeof(int) * d->n);
+  memset(d->jpair, 0, sizeof(int) * d->n);
+  memset(d->fi[WFO_NB], 0, sizeof(int) * d->n);
+  for (int64_t pp = 0; pp < P; pp++) {
+    int a = d->pairs[2 * pp], b = d->pairs[2 * pp + 1];
+    int i = a < b ? a : b, j = a < b ? b : a;
+    if (d->ipair[i] + d->jpair[i] >= WFO_MAX_NB || d->ipair[j] + d->jpair[j] >= WFO_MAX_NB)
+      return fail("MAX_NB_PER_PART exceeded");
+    d->Anei[(int64_t)i * WFO_MAX_NB + d->ipair[i]] = j;
+    d->Anei[(int64_t)j * WFO_MAX_NB + WFO_MAX_NB - 1 - d->jpair[j]] = i;
+    d->Aref[(int64_t)i * WFO_MAX_NB + d->ipair[i]] = (int)pp;
+    d->Aref[(int64_t)j * WFO_MAX_NB + WFO_MAX_NB - 1 - d->jpair[j]] = (int)pp;
+    d->ipair[i]++;
+    d->jpair[j]++;
+    d->fi[WFO_NB][a]++;
+    d->fi[WFO_NB][b]++;
+  }
+  return 0;
+}
+
+/* ClearNbData (Source/Neighbour.cpp:479-497) */
+static void clear_nb_data(wfo_domain *d) {
+  d->npairs = 0;
+  list_generate(d);
+  d->nb_cleared = 1;
+}
+
+int wfo_neighbour_search(wfo_domain *d) {
+  if (!d->HOC && cell_initiate(d)) return -1;
+  clear_nb_data(d);
+  main_neighbour_search(d);
+  return calc_pair_pos_list(d);
+}
+int64_t wfo_npairs(const wfo_domain *d) { return d->npairs; }
+const int *wfo_pairs(const wfo_domain *d) { return d->pairs; }
+
+/* ------------------------------------------------------------------ */
+/* StartAcceleration with a = 0, TI = 0 (Source/Domain.cpp:1519-1597)  */
+int wfo_start_acceleration(wfo_domain *d) {
+  for (int i = 0; i < d->n; i++) {
+    if (!d->fi[WFO_FREE][i]) {
+      d->f[WFO_P][i] = 0.0;
+      memset(d->f[WFO_SIGMA] + 9 * (int64_t)i, 0, 9 * sizeof(double));
+      memset(d->f[WFO_SHEAR] + 9 * (int64_t)i, 0, 9 * sizeof(double));
+    }
+    for (int k = 0; k < 3; k++) d->f[WFO_A][3 * i + k] = 0.0;
+    d->f[WFO_DRHO][i] = 0.0;
+    memset(d->f[WFO_SR] + 9 * (int64_t)i, 0, 9 * sizeof(double));
+    memset(d->f[WFO_RR] + 9 * (int64_t)i, 0, 9 * sizeof(double));
+  }
+  return 0;
+}
+
+/* pair geometry shared by the three passes: P1 = min index, P2 = max */
+typedef struct { int p1, p2; double h, xij[3], vij[3], rij, GK; } pairgeo_t;
+static inline void pair_geo(const wfo_domain *d, int64_t p, pairgeo_t *g) {
+  int a = d->pairs[2 * p], b = d->pairs[2 * p + 1];
+  g->p1 = a < b ? a : b;
+  g->p2 = a < b ? b : a;
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V];
+  g->h = (d->f[WFO_H][g->p1] + d->f[WFO_H][g->p2]) / 2;
+  for (int k = 0; k < 3; k++) {
+    g->xij[k] = x[3 * g->p1 + k] - x[3 * g->p2 + k];
+    g->vij[k] = v[3 * g->p1 + k] - v[3 * g->p2 + k];
+  }
+  g->rij = norm3(g->xij);
+  g->GK = wfo_grad_kernel(g->rij / g->h, g->h);
+}
+
+/* CalcAccel, Nishimura branch, GradientType 0, TI = 0
+ * (Source/InteractionAlt.cpp:12-235) */
+static void calc_accel_pairs(wfo_domain *d) {
+  const double *rho = d->f[WFO_RHO], *sig = d->f[WFO_SIGMA];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo(d, p, &g);
+    int p1 = g.p1, p2 = g.p2;
+    double Alpha = (d->f[WFO_ALPHA][p1] + d->f[WFO_ALPHA][p2]) / 2.0;
+    double Beta = (d->f[WFO_BETA][p1] + d->f[WFO_BETA][p2]) / 2.0;
+    double di = rho[p1], dj = rho[p2];
+    double PI_s = 0.0; /* PIij = PI_s * I */
+    if (Alpha != 0.0 || Beta != 0.0) {
+      double MUij = g.h * dot3(g.vij, g.xij) / (g.rij * g.rij + 0.01 * g.h * g.h);
+      double Ci = d->f[WFO_CS][p1], Cj = d->f[WFO_CS][p2];
+      double Cij = 0.5 * (Ci + Cj);
+      if (dot3(g.vij, g.xij) < 0) PI_s = (Alpha * Cij * MUij + Beta * MUij * MUij) / (0.5 * (di + dj));
+    }
+    double si = 1.0 / (di * di), sj = 1.0 / (dj * dj);
+    double M[9];
+    for (int a = 0; a < 3; a++)
+      for (int c = 0; c < 3; c++) {
+        double piac = (a == c) ? 1.0 * PI_s : 0.0 * PI_s;
+        M[3 * a + c] = ((sig[9 * (int64_t)p1 + 3 * a + c] * si + sig[9 * (int64_t)p2 + 3 * a + c] * sj) + piac) + 0.0;
+      }
+    double gv[3] = {g.GK * g.xij[0], g.GK * g.xij[1], g.GK * g.xij[2]};
+    for (int c = 0; c < 3; c++)
+      d->pair_force[3 * p + c] = gv[0] * M[c] + gv[1] * M[3 + c] + gv[2] * M[6 + c];
+  }
+}
+
+/* AccelReduction (Source/InteractionAlt.cpp:237-260) */
+static void accel_reduction(wfo_domain *d) {
+  const double *m = d->f[WFO_MASS];
+  double *acc = d->f[WFO_A];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->solid_count; i++) {
+    double a[3] = {0., 0., 0.};
+    const int *an = d->Anei + (int64_t)i * WFO_MAX_NB, *ar = d->Aref + (int64_t)i * WFO_MAX_NB;
+    for (int n = 0; n < d->ipair[i]; n++)
+      for (int k = 0; k < 3; k++) a[k] = a[k] + m[an[n]] * d->pair_force[3 * (int64_t)ar[n] + k];
+    for (int n = 0; n < d->jpair[i]; n++)
+      for (int k = 0; k < 3; k++)
+        a[k] = a[k] - m[an[WFO_MAX_NB - 1 - n]] * d->pair_force[3 * (int64_t)ar[WFO_MAX_NB - 1 - n] + k];
+    for (int k = 0; k < 3; k++) acc[3 * i + k] = a[k];
+  }
+}
+
+int wfo_calc_accel(wfo_domain *d) {
+  calc_accel_pairs(d);
+  accel_reduction(d);
+  return 0;
+}
+
+/* CalcDensInc + DensReduction (Source/InteractionAlt.cpp:483-640) */
+int wfo_calc_dens(wfo_domain *d) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo(d, p, &g);
+    double gv[3] = {g.GK * g.xij[0], g.GK * g.xij[1], g.GK * g.xij[2]};
+    d->pair_densinc[p] = dot3(g.vij, gv);
+  }
+  const double *m = d->f[WFO_MASS], *rho = d->f[WFO_RHO];
+  double *drho = d->f[WFO_DRHO];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->solid_count; i++) {
+    const int *an = d->Anei + (int64_t)i * WFO_MAX_NB, *ar = d->Aref + (int64_t)i * WFO_MAX_NB;
+    double s = 0.;
+    for (int n = 0; n < d->ipair[i]; n++) s += m[an[n]] / rho[an[n]] * d->pair_densinc[ar[n]];
+    for (int n = 0; n < d->jpair[i]; n++) {
+      int j = an[WFO_MAX_NB - 1 - n];
+      s += m[j] / rho[j] * d->pair_densinc[ar[WFO_MAX_NB - 1 - n]];
+    }
+    s *= rho[i];
+    drho[i] = s;
+  }
+  return 0;
+}
+
+/* CalcRateTensors + RateTensorsReduction (Source/InteractionAlt.cpp:263-479) */
+int wfo_calc_rates(wfo_domain *d) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo(d, p, &g);
+    const double *va = g.vij, *xi = g.xij;
+    double SR[9], RR[9];
+    SR[0] = 2.0 * va[0] * xi[0];
+    SR[1] = va[0] * xi[1] + va[1] * xi[0];
+    SR[2] = va[0] * xi[2] + va[2] * xi[0];
+    SR[3] = SR[1];
+    SR[4] = 2.0 * va[1] * xi[1];
+    SR[5] = va[1] * xi[2] + va[2] * xi[1];
+    SR[6] = SR[2];
+    SR[7] = SR[5];
+    SR[8] = 2.0 * va[2] * xi[2];
+    RR[0] = RR[4] = RR[8] = 0.0;
+    RR[1] = va[0] * xi[1] - va[1] * xi[0];
+    RR[2] = va[0] * xi[2] - va[2] * xi[0];
+    RR[5] = va[1] * xi[2] - va[2] * xi[1];
+    RR[3] = -RR[1];
+    RR[6] = -RR[2];
+    RR[7] = -RR[5];
+    double s = -0.5 * g.GK;
+    for (int k = 0; k < 9; k++) {
+      d->pair_sr[9 * p + k] = SR[k] * s;
+      d->pair_rr[9 * p + k] = RR[k] * s;
+    }
+  }
+  const double *m = d->f[WFO_MASS], *rho = d->f[WFO_RHO];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->solid_count; i++) {
+    const int *an = d->Anei + (int64_t)i * WFO_MAX_NB, *ar = d->Aref + (int64_t)i * WFO_MAX_NB;
+    double *SRi = d->f[WFO_SR] + 9 * (int64_t)i, *RRi = d->f[WFO_RR] + 9 * (int64_t)i;
+    for (int n = 0; n < d->ipair[i]; n++) {
+      double mjdj = m[an[n]] / rho[an[n]];
+      for (int k = 0; k < 9; k++) {
+        SRi[k] = SRi[k] + d->pair_sr[9 * (int64_t)ar[n] + k] * mjdj;
+        RRi[k] = RRi[k] + d->pair_rr[9 * (int64_t)ar[n] + k] * mjdj;
+      }
+    }
+    for (int n = 0; n < d->jpair[i]; n++) {
+      int j = an[WFO_MAX_NB - 1 - n];
+      int64_t pr = ar[WFO_MAX_NB - 1 - n];
+      double mjdj = m[j] / rho[j];
+      for (int k = 0; k < 9; k++) {
+        SRi[k] = SRi[k] + d->pair_sr[9 * pr + k] * mjdj;
+        RRi[k] = RRi[k] + d->pair_rr[9 * pr + k] * mjdj;
+      }
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Materials (Source/Material.cpp:6-72)                                 */
+static double jc_yield(const wfo_material *M, double strain, double sr, double T) {
+  double T_h = (T - M->Tt) / (M->Tm - M->Tt);
+  double f = 1.0;
+  if (sr > M->eps_0) f = (1.0 + M->C * log(sr / M->eps_0));
+  return (M->A + M->B * pow(strain, M->n)) * f * (1.0 - pow(T_h, M->jm));
+}
+static double jc_tangent(const wfo_material *M, double pl, double sr, double T) {
+  double T_h = (T - M->Tt) / (M->Tm - M->Tt);
+  double f = 1.0;
+  if (sr > M->eps_0) f = (1.0 + M->C * log(sr / M->eps_0));
+  if (pl > 0.) return M->n * M->B * pow(pl, M->n - 1.) * f * (1.0 - pow(T_h, M->jm));
+  return M->E;
+}
+static double hol_yield(const wfo_material *M, double strain) {
+  double eps0 = M->sy0 / M->E, eps1 = pow(M->sy0 / M->K, 1. / M->m);
+  if (strain + eps0 > eps1) return M->K * pow(strain + eps0, M->m);
+  return M->sy0;
+}
+static double hol_tangent(const wfo_material *M, double strain) {
+  double eps0 = M->sy0 / M->E, eps1 = pow(M->sy0 / M->K, 1. / M->m);
+  if (strain + eps0 > eps1) return M->K * M->m * pow(strain + eps0, (M->m - 1.0));
+  return 0.;
+}
+
+/* Mult(Mat,Mat) term order of External/matvec.h:845-850 */
+static inline void mat_mult(const double *A, const double *B, double *M) {
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++)
+      M[3 * a + b] = A[3 * a + 2] * B[6 + b] + A[3 * a + 1] * B[3 + b] + A[3 * a] * B[b];
+}
+
+/* Particle::CalcStressStrain with Fail = 1 (Source/Particle.cpp:960-1067) */
+static void stress_strain(wfo_domain *d, int i, double dt) {
+  double *S = d->f[WFO_SHEAR] + 9 * (int64_t)i, *SR = d->f[WFO_SR] + 9 * (int64_t)i;
+  double *RR = d->f[WFO_RR] + 9 * (int64_t)i, *Sig = d->f[WFO_SIGMA] + 9 * (int64_t)i;
+  const wfo_material *M = &d->mats[d->fi[WFO_MATIDX][i]];
+  int model = d->fi[WFO_MODEL][i];
+  double rho = d->f[WFO_RHO][i];
+  double Cs = d->f[WFO_CS][i];
+  double P = d->f[WFO_P0][i] + (Cs * Cs) * (rho - d->f[WFO_REFRHO][i]);
+  d->f[WFO_P][i] = P;
+  double RT[9], SRT[9], RS[9];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) RT[3 * a + b] = RR[3 * b + a];
+  mat_mult(S, RT, SRT);
+  mat_mult(RR, S, RS);
+  double dep = 0., sig_trial = 0.;
+  double G = d->f[WFO_G][i];
+  double c3 = 1.0 / 3.0 * (SR[0] + SR[4] + SR[8]);
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      int k = 3 * a + b;
+      double Iab = (a == b) ? 1.0 : 0.0;
+      S[k] = dt * ((2.0 * G) * (SR[k] - Iab * c3) + SRT[k] + RS[k]) + S[k];
+    }
+  d->f[WFO_EFFSR][i] = sqrt(0.5 * ((SR[0] - SR[4]) * (SR[0] - SR[4]) + (SR[4] - SR[8]) * (SR[4] - SR[8]) +
+                                   (SR[8] - SR[0]) * (SR[8] - SR[0])) +
+                            3.0 * (SR[1] * SR[1] + SR[5] * SR[5] + SR[6] * SR[6]));
+  double *Sy = &d->f[WFO_SIGMAY][i], *pl = &d->f[WFO_PLSTRAIN][i], *Ep = &d->f[WFO_EP][i];
+  double J2 = 0.5 * (S[0] * S[0] + 2.0 * S[1] * S[3] + 2.0 * S[2] * S[6] + S[4] * S[4] + 2.0 * S[5] * S[7] + S[8] * S[8]);
+  sig_trial = sqrt(3.0 * J2);
+  double ratio = *Sy / sig_trial;
+  double fac = (1.0 < ratio) ? 1.0 : ratio; /* std::min(ratio, 1.0) */
+  for (int k = 0; k < 9; k++) S[k] = S[k] * fac;
+  if (model == WFO_HOLLOMON) *Sy = hol_yield(M, *pl);
+  else if (model == WFO_JOHNSON_COOK) *Sy = jc_yield(M, *pl, d->f[WFO_EFFSR][i], d->f[WFO_T][i]);
+  if (sig_trial > *Sy) {
+    if (model == WFO_HOLLOMON) d->f[WFO_ET][i] = hol_tangent(M, *pl);
+    else if (model == WFO_JOHNSON_COOK) d->f[WFO_ET][i] = jc_tangent(M, *pl, d->f[WFO_EFFSR][i], d->f[WFO_T][i]);
+    if (*Ep < 0) *Ep = 1. * M->E;
+    dep = (sig_trial - *Sy) / (3. * G + *Ep);
+    *pl += dep;
+    d->f[WFO_DPLSTRAIN][i] = dep;
+    if (model == WFO_BILINEAR) *Sy += dep * *Ep;
+  }
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      double Iab = (a == b) ? 1.0 : 0.0;
+      Sig[3 * a + b] = Iab * (-P) + S[3 * a + b];
+    }
+  if (dep > 0.0) {
+    double f = dep / *Sy;
+    double *inc = d->f[WFO_EPLINC] + 9 * (int64_t)i, *epl = d->f[WFO_EPL] + 9 * (int64_t)i;
+    inc[0] = f * (Sig[0] - 0.5 * (Sig[4] + Sig[8]));
+    inc[4] = f * (Sig[4] - 0.5 * (Sig[0] + Sig[8]));
+    inc[8] = f * (Sig[8] - 0.5 * (Sig[0] + Sig[4]));
+    inc[1] = inc[3] = 1.5 * f * (Sig[1]);
+    inc[2] = inc[6] = 1.5 * f * (Sig[2]);
+    inc[5] = inc[7] = 1.5 * f * (Sig[5]);
+    for (int k = 0; k < 9; k++) epl[k] = epl[k] + inc[k];
+  }
+  double *E = d->f[WFO_STRAIN] + 9 * (int64_t)i;
+  for (int k = 0; k < 9; k++) E[k] = dt * SR[k] + E[k];
+}
+
+int wfo_stress_all(wfo_domain *d, double dt) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->solid_count; i++) stress_strain(d, i, dt);
+  return 0;
+}
+
+/* Domain::CalcPlasticWorkHeat (Source/ThermalStuct.cpp:14-39) with
+ * Particle::CalcPlasticWorkHeat (Source/Particle.cpp:922-939) */
+static void plastic_work_heat(wfo_domain *d, double dt) {
+  if (!d->pl_heating) return;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int p = 0; p < d->solid_count; p++) {
+    if (d->f[WFO_DPLSTRAIN][p] > 0.0) {
+      const double *Sig = d->f[WFO_SIGMA] + 9 * (int64_t)p, *inc = d->f[WFO_EPLINC] + 9 * (int64_t)p;
+      double dp[9];
+      for (int k = 0; k < 9; k++) dp[k] = inc[k] * (1. / dt);
+      d->f[WFO_QPL][p] = Sig[0] * dp[0] + 2.0 * Sig[1] * dp[3] + 2.0 * Sig[2] * dp[6] + Sig[4] * dp[4] +
+                         2.0 * Sig[5] * dp[7] + Sig[8] * dp[8];
+    }
+    d->f[WFO_PSENERGY][p] += d->f[WFO_QPL][p] * dt;
+  }
+  double pl_sum = 0.;
+  for (int p = 0; p < d->solid_count; p++) pl_sum += d->f[WFO_MASS][p] / d->f[WFO_RHO][p] * d->f[WFO_QPL][p];
+  d->plastic_work += pl_sum * dt;
+}
+
+/* per-particle merge of the forward and backward Nishimura lists back into
+ * global pair order p, i.e. the order an Nproc=1 pair loop visits them */
+#define MERGE_BEGIN(d, i)                                                     \
+  const int *ar_ = (d)->Aref + (int64_t)(i) * WFO_MAX_NB;                     \
+  int nf_ = (d)->ipair[i], nb_ = (d)->jpair[i], f_ = 0, b_ = 0;              \
+  while (f_ < nf_ || b_ < nb_) {                                             \
+    int64_t p_;                                                               \
+    if (b_ >= nb_ || (f_ < nf_ && ar_[f_] < ar_[WFO_MAX_NB - 1 - b_])) {      \
+      p_ = ar_[f_++];                                                         \
+    } else {                                                                  \
+      p_ = ar_[WFO_MAX_NB - 1 - b_++];                                        \
+    }
+#define MERGE_END }
+
+/* CalcConvHeat, 3D (Source/Thermal.cpp:339-389) */
+static void conv_heat(wfo_domain *d) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->n; i++) {
+    double dens = d->f[WFO_RHO][i];
+    double dS2 = pow(d->f[WFO_MASS][i] / dens, 2.0 / 3.0);
+    if (d->fi[WFO_THBC][i] == 1)
+      d->f[WFO_QCONV][i] = dens * d->f[WFO_HCONV][i] * dS2 * (d->f[WFO_TINF][i] - d->f[WFO_T][i]) / d->f[WFO_MASS][i];
+  }
+}
+
+/* CalcTempInc without gradient correction (Source/Thermal.cpp:53-219):
+ * temp[first] += m, temp[second] -= m with m using the second particle's
+ * m/rho; accumulated per particle in pair order. */
+static void temp_inc(wfo_domain *d) {
+  const double *x = d->f[WFO_X], *T = d->f[WFO_T], *kT = d->f[WFO_KT];
+  const double *ms = d->f[WFO_MASS], *rho = d->f[WFO_RHO], *hh = d->f[WFO_H];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->n; i++) {
+    double acc = 0.0;
+    MERGE_BEGIN(d, i)
+      int p1 = d->pairs[2 * p_], p2 = d->pairs[2 * p_ + 1];
+      double xij[3] = {x[3 * p1] - x[3 * p2], x[3 * p1 + 1] - x[3 * p2 + 1], x[3 * p1 + 2] - x[3 * p2 + 2]};
+      double h = (hh[p1] + hh[p2]) / 2.0;
+      double GK = wfo_grad_kernel(norm3(xij) / h, h);
+      double mj = ms[p2], dj = rho[p2];
+      double gv[3] = {GK * xij[0], GK * xij[1], GK * xij[2]};
+      double m = mj / dj * 4. * (kT[p1] * kT[p2]) / (kT[p1] + kT[p2]) * (T[p1] - T[p2]) * dot3(xij, gv) /
+                 (norm3(xij) * norm3(xij));
+      if (p1 == i) acc += m; else acc -= m;
+    MERGE_END
+    d->temp_work[i] = acc;
+  }
+  for (int i = 0; i < d->solid_count; i++) {
+    double dd = d->f[WFO_RHO][i];
+    double f = 1. / (dd * d->f[WFO_CPT][i]);
+    d->f[WFO_DTDT][i] = f * (d->temp_work[i] + d->f[WFO_QCONV][i] + d->f[WFO_QSRC][i] +
+                             d->f[WFO_QPL][i] * d->pl_work_heat_frac + d->f[WFO_QCONT][i]);
+  }
+  for (int i = d->solid_count; i < d->n; i++) d->f[WFO_DTDT][i] = 0.;
+}
+
+/* ThermalCalcs (Source/Thermal.cpp:721-748) with CalcThermalExpStrainRate
+ * (Source/ThermalStuct.cpp:7-12, Source/Particle.cpp:953-957) */
+int wfo_thermal_calcs(wfo_domain *d, double dt) {
+  if (!d->thermal_solver) return 0;
+  d->m_maxT = 0.;
+  d->m_minT = 1000.;
+  for (int i = 0; i < d->solid_count; i++) {
+    d->f[WFO_T][i] += dt * d->f[WFO_DTDT][i];
+    if (d->f[WFO_T][i] > d->m_maxT) d->m_maxT = d->f[WFO_T][i];
+    if (d->f[WFO_T][i] < d->m_minT) d->m_minT = d->f[WFO_T][i];
+  }
+  conv_heat(d);
+  temp_inc(d);
+  for (int p = 0; p < d->n; p++) {
+    double s = d->f[WFO_THEXP][p] * d->f[WFO_DTDT][p];
+    double *SR = d->f[WFO_SR] + 9 * (int64_t)p;
+    for (int a = 0; a < 3; a++) SR[4 * a] = SR[4 * a] + 1.0 * s;
+  }
+  return 0;
+}
+
+/* CalcKinEnergyEqn (Source/Domain.cpp:1752-1808): the pair terms are
+ * antisymmetric, so the sum is zero up to rounding; kept for fidelity. */
+static void kin_energy(wfo_domain *d) {
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V], *P = d->f[WFO_P];
+  const double *ms = d->f[WFO_MASS], *rho = d->f[WFO_RHO], *hh = d->f[WFO_H];
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->n; i++) {
+    double acc = 0.0;
+    MERGE_BEGIN(d, i)
+      int p1 = d->pairs[2 * p_], p2 = d->pairs[2 * p_ + 1];
+      double xij[3], vij[3];
+      for (int k = 0; k < 3; k++) { xij[k] = x[3 * p1 + k] - x[3 * p2 + k]; vij[k] = v[3 * p1 + k] - v[3 * p2 + k]; }
+      double h = (hh[p1] + hh[p2]) / 2.0;
+      double GK = wfo_grad_kernel(norm3(xij) / h, h);
+      double di = rho[p1], dj = rho[p2], mj = ms[p2];
+      double gv[3] = {GK * xij[0], GK * xij[1], GK * xij[2]};
+      double temp = 0.5 * mj * (P[p1] / (di * di) + P[p2] / (dj * dj)) * dot3(vij, gv);
+      if (p1 == i) acc += temp; else acc -= temp;
+    MERGE_END
+    d->dkin_work[i] = acc;
+  }
+  double inc = 0.;
+  for (int i = 0; i < d->n; i++) inc += d->dkin_work[i];
+  d->kin_energy_sum += inc * d->deltat;
+}
+
+/* CalcIntEnergyEqn (Source/Domain.cpp:1810-1829, Source/Particle.cpp:941-950) */
+static void int_energy(wfo_domain *d) {
+  double inc = 0.;
+  for (int i = 0; i < d->n; i++) {
+    const double *S = d->f[WFO_SHEAR] + 9 * (int64_t)i, *SR = d->f[WFO_SR] + 9 * (int64_t)i;
+    double de = d->f[WFO_MASS][i] / d->f[WFO_RHO][i] *
+                (S[0] * SR[0] + 2.0 * S[1] * SR[3] + 2.0 * S[2] * SR[6] + S[4] * SR[4] + 2.0 * S[5] * SR[7] + S[8] * SR[8]);
+    inc += de;
+  }
+  d->int_energy_sum += inc * d->deltat;
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver set-up (Source/SolverLeapfrog.cpp:24-103)                     */
+int wfo_solve_init(wfo_domain *d, double dt) {
+  if (d->n <= 0) return fail("Particle Count is Null");
+  d->deltat = dt;
+  initial_checks(d);
+  memset(d->TRPR, 0, sizeof d->TRPR);
+  memset(d->BLPF, 0, sizeof d->BLPF);
+  if (cell_initiate(d)) return -1;
+  list_generate(d);
+  clear_nb_data(d);
+  d->isfirst = 1; d->isyielding = 0; d->ts_i = 0;
+  d->initialized = 1;
+  return 0;
+}
+
+static double max_pl_strain(const wfo_domain *d) {
+  double mx = 0;
+  for (int i = 0; i < d->n; i++)
+    if (d->f[WFO_PLSTRAIN][i] > mx) mx = d->f[WFO_PLSTRAIN][i];
+  return mx;
+}
+
+/* neighbour schedule at the top of a step (SolverLeapfrog.cpp:160-201) */
+static int schedule_search(wfo_domain *d, double mx) {
+  if (mx > MIN_PS_FOR_NBSEARCH && !d->isyielding) {
+    clear_nb_data(d);
+    main_neighbour_search(d);
+    if (calc_pair_pos_list(d)) return -1;
+    d->isyielding = 1;
+  }
+  if (mx > MIN_PS_FOR_NBSEARCH || d->isfirst) {
+    if (d->ts_i == 0 && d->nb_cleared) {
+      main_neighbour_search(d);
+      if (calc_pair_pos_list(d)) return -1;
+    }
+  }
+  return 0;
+}
+
+/* neighbour schedule at the end of a step (SolverLeapfrog.cpp:347-356) */
+static void schedule_end(wfo_domain *d, double mx) {
+  if (mx > MIN_PS_FOR_NBSEARCH) {
+    if (d->ts_i == (d->ts_nb_inc - 1)) clear_nb_data(d);
+    d->ts_i++;
+    if (d->ts_i > (d->ts_nb_inc - 1)) d->ts_i = 0;
+  }
+}
+
+static void density_update(wfo_domain *d) {
+  for (int i = 0; i < d->n; i++) d->f[WFO_RHO][i] += d->deltat * d->f[WFO_DRHO][i];
+}
+
+static void drift(wfo_domain *d) {
+  double *x = d->f[WFO_X], *v = d->f[WFO_V], *xp = d->f[WFO_XPREV], *u = d->f[WFO_DISP];
+  for (int i = 0; i < d->n; i++)
+    for (int k = 0; k < 3; k++) {
+      xp[3 * i + k] = x[3 * i + k];
+      double du = (v[3 * i + k] + 0.0) * d->deltat;
+      u[3 * i + k] += du;
+      x[3 * i + k] += du;
+    }
+}
+
+static void step_tail(wfo_domain *d, double mx) {
+  plastic_work_heat(d, d->deltat);
+  wfo_thermal_calcs(d, d->deltat);
+  kin_energy(d);
+  int_energy(d);
+  d->steps++;
+  d->Time += d->deltat;
+  d->isfirst = 0;
+  schedule_end(d, mx);
+}
+
+/* One iteration of SolveDiffUpdateLeapFrog's loop
+ * (Source/SolverLeapfrog.cpp:124-362), fixed time step, no contact. */
+int wfo_step_leapfrog(wfo_domain *d) {
+  if (!d->initialized) return fail("wfo_solve_init not called");
+  wfo_start_acceleration(d);
+  double mx = max_pl_strain(d);
+  d->max_pl_strain = mx;
+  if (schedule_search(d, mx)) return -1;
+  wfo_calc_accel(d);
+  d->react_pending = 1;
+  wfo_general_after(d);
+  double *v = d->f[WFO_V], *a = d->f[WFO_A];
+  if (d->isfirst)
+    for (int i = 0; i < 3 * d->n; i++) v[i] -= a[i] * 0.5 * d->deltat;
+  wfo_general_after(d);
+  double prev = d->deltat;
+  for (int i = 0; i < 3 * d->n; i++) v[i] += a[i] * 0.5 * (prev + d->deltat);
+  wfo_general_after(d);
+  wfo_calc_dens(d);
+  density_update(d);
+  drift(d);
+  wfo_calc_rates(d);
+  wfo_stress_all(d, d->deltat);
+  step_tail(d, mx);
+  return 0;
+}
+
+/* One iteration of SolveDiffUpdateKickDrift's loop
+ * (Source/SolverKickDrift.cpp:117-330), fixed time step, no contact. */
+int wfo_step_kickdrift(wfo_domain *d) {
+  if (!d->initialized) return fail("wfo_solve_init not called");
+  wfo_start_acceleration(d);
+  double mx = max_pl_strain(d);
+  d->max_pl_strain = mx;
+  if (schedule_search(d, mx)) return -1;
+  wfo_calc_accel(d);
+  d->react_pending = 1;
+  wfo_general_after(d);
+  double *v = d->f[WFO_V], *a = d->f[WFO_A];
+  double prev = d->deltat;
+  for (int i = 0; i < 3 * d->n; i++) v[i] += a[i] * 0.5 * prev * 1.;
+  wfo_general_after(d);
+  wfo_calc_dens(d);
+  for (int i = 0; i < d->n; i++) d->f[WFO_RHO][i] += d->deltat * d->f[WFO_DRHO][i] * 1.;
+  drift(d);
+  for (int i = 0; i < 3 * d->n; i++) v[i] += a[i] * d->deltat / 2. * 1.;
+  wfo_general_after(d);
+  wfo_calc_rates(d);
+  wfo_stress_all(d, d->deltat);
+  step_tail(d, mx);
+  return 0;
+}

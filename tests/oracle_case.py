@@ -1,0 +1,84 @@
+"""Load a weldform_amd.cases.Case into the CPU oracle (test helper)."""
+import numpy as np
+
+import pyoracle as po
+from weldform_amd import capi
+
+
+def oracle_from_case(case, threads=1):
+    po.set_threads(threads)
+    d = po.Domain(case.n)
+    d.f("X")[:] = case.x
+    d.f("H")[:] = case.h
+    d.f("MASS")[:] = case.mass
+    d.f("RHO")[:] = case.density
+    d.f("SIGMAY")[:] = case.sigmay
+    if case.T is not None:
+        d.f("T")[:] = case.T
+    d.i("ID")[:] = case.ids
+    d.i("MATERIAL")[:] = case.material
+    d.i("MATIDX")[:] = case.material
+    d.i("FREE")[:] = 1
+    for idx, m in case.materials.items():
+        sel = case.material == idx
+        d.f("REFRHO")[sel] = m["ref_density"]
+        d.f("CS")[sel] = m["Cs"]
+        d.f("G")[sel] = m["G"]
+        d.f("P0")[sel] = m.get("P0", 0.0)
+        d.f("ALPHA")[sel] = m.get("alpha", 0.0)
+        d.f("BETA")[sel] = m.get("beta", 0.0)
+        d.f("EP")[sel] = m.get("Ep", 0.0)
+        d.f("KT")[sel] = m.get("k_T", 0.0)
+        d.f("CPT")[sel] = m.get("cp_T", 0.0)
+        d.f("THEXP")[sel] = m.get("th_exp", 0.0)
+        d.i("MODEL")[sel] = m["model"]
+        d.set_material(idx, model=m["model"], E=m["E"], nu=m.get("nu", 0.3),
+                       K=m.get("hol_K", 0.0), m=m.get("hol_m", 1.0), sy0=m.get("hol_sy0", 0.0),
+                       A=m.get("jc_A", 0.0), B=m.get("jc_B", 0.0), n=m.get("jc_n", 1.0),
+                       C=m.get("jc_C", 0.0), eps_0=m.get("jc_eps0", 1.0), jm=m.get("jc_m", 1.0),
+                       Tm=m.get("jc_Tm", 1.0), Tt=m.get("jc_Tt", 0.0))
+    for k in range(3):
+        d.set_param(f"DomMax{k}", case.dom_max[k])
+        d.set_param(f"DomMin{k}", case.dom_min[k])
+    d.set_param("thermal_solver", int(case.thermal))
+    d.set_param("pl_heating", int(case.pl_heating))
+    for a in case.amps:
+        d.add_amplitude(a["id"], a["t"], a["v"])
+    for b in case.bcs:
+        d.add_velocity_bc(b["zone"], b["value"], b.get("value_type", 0), b.get("amp_id", 0),
+                          b.get("amp_factor", 1.0))
+    for b in case.tbcs:
+        d.add_temperature_bc(b["zone"], b["T"])
+    return d
+
+
+def unordered(pairs):
+    """(temp1, temp2) pairs -> sorted unique (min, max) rows."""
+    p = np.sort(np.asarray(pairs, dtype=np.int64), axis=1)
+    p = p[np.lexsort((p[:, 1], p[:, 0]))]
+    return p
+
+
+def brute_pairs(x, h, cellfac=2.0, chunk=512):
+    """O(N^2) neighbour set with CheckRadius' predicate (Source/Neighbour.cpp:361-372)."""
+    n = len(x)
+    out = []
+    for s in range(0, n, chunk):
+        xi = x[s:s + chunk]
+        d = xi[:, None, :] - x[None, :, :]
+        r = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2])
+        hh = (h[s:s + chunk, None] + h[None, :]) / 2
+        ok = (r / hh) <= cellfac
+        ii, jj = np.nonzero(ok)
+        ii = ii + s
+        keep = ii < jj
+        out.append(np.stack([ii[keep], jj[keep]], axis=1))
+    p = np.concatenate(out)
+    return p[np.lexsort((p[:, 1], p[:, 0]))]
+
+
+def normwise_err(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    s = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / s) if s > 0 else float(np.max(np.abs(a - b)))

@@ -1,0 +1,259 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU oracle.
+
+Bars (stated per test): neighbour sets bit-exact; single FP64 passes within
+1e-10 normwise (max|gpu-ref| / max|ref|); 1000-step displacement and
+reaction histories within 1e-8 normwise.
+"""
+import numpy as np
+import pytest
+
+from oracle_case import normwise_err, oracle_from_case, unordered
+from weldform_amd import capi, cases
+
+pytestmark = pytest.mark.gpu
+
+TOL_PASS = 1e-10
+TOL_RUN = 1e-8
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return cases.config1()
+
+
+def _random_state(case, seed):
+    rng = np.random.default_rng(seed)
+    n = case.n
+    v = rng.normal(0.0, 1.0, size=(n, 3))
+    rho = case.density * (1.0 + 1e-3 * rng.uniform(-1, 1, n))
+    s = rng.normal(0.0, 1e8, size=(n, 3, 3))
+    sig = 0.5 * (s + np.transpose(s, (0, 2, 1)))
+    return v, rho, sig
+
+
+def test_neighbour_sets_bit_exact(c1):
+    _log("case", c1.name, "n", c1.n)
+    s = cases.make_solver(c1)
+    s.rebuild_neighbours()
+    g = s.neighbour_pairs()
+    d = oracle_from_case(c1)
+    d.neighbour_search()
+    o = unordered(d.pairs())
+    assert g.shape == o.shape
+    assert np.array_equal(g.astype(np.int64), o)
+    nb = s.get_int("nb")
+    assert np.array_equal(nb, d.i("NB"))
+
+
+def test_neighbour_sets_jittered():
+    c = cases.small_jc(steps=1)
+    _log("case", c.name, "n", c.n)
+    s = cases.make_solver(c)
+    s.rebuild_neighbours()
+    d = oracle_from_case(c)
+    d.neighbour_search()
+    assert np.array_equal(s.neighbour_pairs().astype(np.int64), unordered(d.pairs()))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_single_passes_fp64(c1, seed):
+    v, rho, sig = _random_state(c1, seed)
+    s = cases.make_solver(c1)
+    s.set("v", v)
+    s.set("density", rho)
+    s.set("sigma", sig)
+    s.rebuild_neighbours()
+    d = oracle_from_case(c1)
+    d.f("V")[:] = v
+    d.f("RHO")[:] = rho
+    d.f("SIGMA")[:] = sig
+    d.neighbour_search()
+    d.start_acceleration()
+    # acceleration pass
+    s.eval("accel")
+    d.calc_accel()
+    e = normwise_err(s.get("a"), d.f("A"))
+    _log("accel err", e)
+    assert e < TOL_PASS
+    # density rate
+    s.eval("density")
+    d.calc_dens()
+    e = normwise_err(s.get("ddensity"), d.f("DRHO"))
+    _log("density err", e)
+    assert e < TOL_PASS
+    # strain and rotation rates
+    s.eval("rates")
+    d.calc_rates()
+    e1 = normwise_err(s.get("strain_rate"), d.f("SR"))
+    e2 = normwise_err(s.get("rot_rate"), d.f("RR"))
+    _log("rates err", e1, e2)
+    assert e1 < TOL_PASS and e2 < TOL_PASS
+
+
+@pytest.mark.parametrize("model", ["bilinear", "jc", "hollomon"])
+def test_stress_update_fp64(model):
+    rng = np.random.default_rng(5)
+    if model == "jc":
+        c = cases.small_jc(steps=1)
+    else:
+        c = cases.config1(steps=1)
+    if model == "hollomon":
+        m = dict(c.materials[0])
+        m.update(model=capi.HOLLOMON, hol_K=500e6, hol_m=0.2, hol_sy0=150e6)
+        c.materials[0] = m
+        c.sigmay[:] = 150e6
+    n = c.n
+    srate = rng.normal(0, 2e3, size=(n, 3, 3))
+    srate = 0.5 * (srate + np.transpose(srate, (0, 2, 1)))
+    w = rng.normal(0, 2e3, size=(n, 3))
+    rr = np.zeros((n, 3, 3))
+    rr[:, 0, 1], rr[:, 0, 2], rr[:, 1, 2] = w[:, 0], w[:, 1], w[:, 2]
+    rr -= np.transpose(rr, (0, 2, 1))
+    S0 = rng.normal(0, 1e8, size=(n, 3, 3))
+    S0 = 0.5 * (S0 + np.transpose(S0, (0, 2, 1)))
+    S0 -= np.eye(3)[None] * np.trace(S0, axis1=1, axis2=2)[:, None, None] / 3
+    pl = np.abs(rng.normal(0, 0.05, n))
+    rho = c.density * (1 + 1e-3 * rng.uniform(-1, 1, n))
+    T = np.full(n, 350.0)
+    dt = 1e-6
+    s = cases.make_solver(c)
+    for name, val in [("strain_rate", srate), ("rot_rate", rr), ("shear", S0), ("pl_strain", pl),
+                      ("density", rho), ("T", T)]:
+        s.set(name, val)
+    s.eval("stress", dt)
+    d = oracle_from_case(c)
+    d.f("SR")[:] = srate
+    d.f("RR")[:] = rr
+    d.f("SHEAR")[:] = S0
+    d.f("PLSTRAIN")[:] = pl
+    d.f("RHO")[:] = rho
+    d.f("T")[:] = T
+    d.stress_all(dt)
+    for gname, oname in [("shear", "SHEAR"), ("pl_strain", "PLSTRAIN"), ("sigmay", "SIGMAY"),
+                         ("eff_sr", "EFFSR"), ("pressure", "P"), ("strain_pl", "EPL"),
+                         ("dpl_strain", "DPLSTRAIN"), ("sigma", "SIGMA")]:
+        e = normwise_err(s.get(gname), d.f(oname))
+        _log(model, gname, e)
+        assert e < TOL_PASS, (gname, e)
+    yielded = (d.f("DPLSTRAIN") > 0).sum()
+    _log(model, "yielded particles", yielded)
+    assert yielded > 0
+
+
+def _run_pair(case, steps, scheme):
+    s = cases.make_solver(case)
+    d = oracle_from_case(case, threads=4)
+    d.solve_init(case.dt)
+    R_o = []
+    for _ in range(steps):
+        d.step("leapfrog" if scheme == capi.LEAPFROG else "kickdrift")
+        R_o.append(d.reaction(case.top_zone).copy())
+    s.step(steps, scheme)
+    s.sync()
+    return s, d, np.array(R_o)
+
+
+@pytest.mark.parametrize("scheme", [capi.LEAPFROG, capi.KICKDRIFT])
+def test_run_config1(c1, scheme):
+    steps = 1000 if scheme == capi.LEAPFROG else 300
+    _log("config1 run", "scheme", scheme, "steps", steps)
+    s, d, R_o = _run_pair(c1, steps, scheme)
+    R_g = s.reaction_history(c1.top_zone)
+    assert R_g.shape == R_o.shape
+    e_r = normwise_err(R_g[:, 2], R_o[:, 2])
+    e_u = normwise_err(s.get("disp"), d.f("DISP"))
+    e_x = normwise_err(s.get("x") - c1.x, d.f("X") - c1.x)
+    e_s = normwise_err(s.get("sigma"), d.f("SIGMA"))
+    sc = s.scalars()
+    _log("reaction err", e_r, "disp err", e_u, "x err", e_x, "sigma err", e_s)
+    _log("scalars gpu", sc)
+    _log("oracle nb", d.param("nb_searches"), "int", d.param("int_energy_sum"), "maxpl", d.param("max_pl_strain"))
+    assert e_r < TOL_RUN and e_u < TOL_RUN and e_x < TOL_RUN
+    assert e_s < 1e-7
+    assert sc["nb_searches"] == d.param("nb_searches")
+    assert abs(sc["int_energy_sum"] - d.param("int_energy_sum")) <= 1e-8 * abs(d.param("int_energy_sum"))
+    assert np.array_equal(s.neighbour_pairs().astype(np.int64), unordered(d.pairs()))
+
+
+def _fast_jc(steps):
+    c = cases.small_jc(steps=steps)
+    # compress at 20 m/s so the Johnson-Cook branch and rebuild schedule engage
+    c.bcs[1]["value"] = (0.0, 0.0, -20.0)
+    return c
+
+
+def test_run_jc_jittered():
+    """Stable window (200 steps, plastic flow and ~40 neighbour rebuilds):
+    reaction and displacement within 1e-8 of the oracle."""
+    c = _fast_jc(200)
+    _log("case", c.name, "steps", c.steps)
+    s, d, R_o = _run_pair(c, c.steps, capi.LEAPFROG)
+    R_g = s.reaction_history(c.top_zone)
+    e_r = normwise_err(R_g[:, 2], R_o[:, 2])
+    e_u = normwise_err(s.get("disp"), d.f("DISP"))
+    e_p = normwise_err(s.get("pl_strain"), d.f("PLSTRAIN"))
+    sc = s.scalars()
+    _log("reaction err", e_r, "disp err", e_u, "pl err", e_p, "maxpl", sc["max_pl_strain"],
+         "nb", sc["nb_searches"], d.param("nb_searches"))
+    assert sc["max_pl_strain"] > 1e-2
+    assert sc["nb_searches"] == d.param("nb_searches") > 20
+    assert e_r < TOL_RUN and e_u < TOL_RUN and e_p < 1e-7
+
+
+def test_run_jc_unstable_window():
+    """Past step ~230 this case is dynamically unstable: the oracle itself,
+    with its initial positions perturbed by 1e-15 (relative), drifts by ~1e-6
+    by step 400.  The HIP path must stay within that sensitivity band."""
+    steps = 400
+    c = _fast_jc(steps)
+    s, d, R_o = _run_pair(c, steps, capi.LEAPFROG)
+    cp = _fast_jc(steps)
+    rng = np.random.default_rng(0)
+    cp.x = cp.x * (1 + 1e-15 * rng.uniform(-1, 1, cp.x.shape))
+    dp = oracle_from_case(cp, threads=4)
+    dp.solve_init(cp.dt)
+    R_p = []
+    for _ in range(steps):
+        dp.step()
+        R_p.append(dp.reaction(cp.top_zone)[2])
+    band_r = normwise_err(np.array(R_p), R_o[:, 2])
+    band_u = normwise_err(dp.f("DISP"), d.f("DISP"))
+    e_r = normwise_err(s.reaction_history(c.top_zone)[:, 2], R_o[:, 2])
+    e_u = normwise_err(s.get("disp"), d.f("DISP"))
+    _log("gpu err", e_r, e_u, "oracle self-sensitivity", band_r, band_u)
+    assert e_r <= 10 * max(band_r, 1e-8) and e_u <= 10 * max(band_u, 1e-8)
+
+
+def test_determinism_and_conservation(c1):
+    v, rho, sig = _random_state(c1, 9)
+    out = []
+    for _ in range(2):
+        s = cases.make_solver(c1)
+        s.set("v", v)
+        s.set("sigma", sig)
+        s.rebuild_neighbours()
+        s.eval("accel")
+        out.append(s.get("a"))
+    assert np.array_equal(out[0], out[1])
+    ma = out[0] * c1.mass[:, None]
+    # pair forces are antisymmetric: total momentum rate vanishes to rounding
+    assert np.abs(ma.sum(axis=0)).max() <= 1e-12 * np.abs(ma).sum()
+
+
+def test_errors_reported(c1):
+    s = capi.Solver(c1.n)
+    with pytest.raises(capi.WFError) as e:
+        s.step(1)
+    assert e.value.code == capi.ESTATE
+    s2 = capi.Solver(c1.n, max_nb=8)
+    s2.set_material(0, **c1.materials[0])
+    for name, val in [("x", c1.x), ("h", c1.h), ("mass", c1.mass), ("density", c1.density)]:
+        s2.set(name, val)
+    s2.init(c1.dt)
+    with pytest.raises(capi.WFError) as e:
+        s2.rebuild_neighbours()
+    assert e.value.code == capi.ENBOVERFLOW

@@ -1,0 +1,277 @@
+"""ctypes binding of the C-ABI in include/weldform_hip.h.
+
+Mirrors the reference's SPH::Domain operator interface at the level the
+Python tests and the benchmark need: particle fields in and out, boundary
+conditions, SolveDiffUpdateLeapFrog / SolveDiffUpdateKickDrift steps.
+The library is the compiled HIP path; there is no CPU fallback: if
+libweldform_hip.so is missing, loading fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libweldform_hip.so")
+
+OK, EINVAL, EHIP, ESTATE, ENBOVERFLOW, ENAN, EUNSUPPORTED, ENOMEM = 0, -1, -2, -3, -4, -5, -6, -7
+FP64, MIXED = 0, 1
+LEAPFROG, KICKDRIFT = 0, 1
+BILINEAR, HOLLOMON, JOHNSON_COOK = 0, 1, 2
+
+FIELDS = {
+    "x": (0, 3), "v": (1, 3), "a": (2, 3), "disp": (3, 3), "density": (4, 1),
+    "mass": (5, 1), "h": (6, 1), "sigma": (7, 9), "shear": (8, 9),
+    "strain_rate": (9, 9), "rot_rate": (10, 9), "strain": (11, 9),
+    "strain_pl": (12, 9), "pl_strain": (13, 1), "dpl_strain": (14, 1),
+    "sigmay": (15, 1), "eff_sr": (16, 1), "pressure": (17, 1), "T": (18, 1),
+    "dTdt": (19, 1), "ddensity": (20, 1), "q_plheat": (21, 1),
+    "strain_pl_incr": (22, 9), "q_conv": (23, 1), "h_conv": (24, 1),
+    "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1),
+}
+INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4}
+EVAL = {"accel": 0, "density": 1, "rates": 2, "stress": 3}
+KERNELS = {"accel": 0, "density": 1, "rates": 2, "neighbours": 3, "step": 4}
+
+
+class WFError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"weldform_hip error {code}: {msg}")
+        self.code = code
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("ts_nb_inc", ctypes.c_int32),
+                ("max_nb", ctypes.c_int32), ("thermal_solver", ctypes.c_int32),
+                ("pl_heating", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 2), ("cellfac", ctypes.c_double),
+                ("pl_work_heat_frac", ctypes.c_double), ("dom_max", ctypes.c_double * 3),
+                ("dom_min", ctypes.c_double * 3)]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in
+        ["E", "nu", "G", "Cs", "ref_density", "P0", "alpha", "beta", "Ep",
+         "hol_K", "hol_m", "hol_sy0", "jc_A", "jc_B", "jc_n", "jc_C", "jc_eps0",
+         "jc_m", "jc_Tm", "jc_Tt", "k_T", "cp_T", "th_exp"]]
+
+
+class Scalars(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("steps", ctypes.c_int64), ("nb_searches", ctypes.c_int64),
+                ("npairs", ctypes.c_int64), ("max_nb_count", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in
+        ["kin_energy_sum", "int_energy_sum", "plastic_work", "max_pl_strain", "max_T", "min_T"]]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+EXPORTS = [
+    "wf_abi_version", "wf_last_error", "wf_device_count", "wf_create", "wf_destroy",
+    "wf_set_material", "wf_set_field", "wf_get_field", "wf_set_int_field",
+    "wf_get_int_field", "wf_add_amplitude", "wf_add_velocity_bc",
+    "wf_add_temperature_bc", "wf_init", "wf_step", "wf_sync", "wf_get_scalars",
+    "wf_get_reaction_history", "wf_rebuild_neighbours", "wf_get_neighbour_pairs",
+    "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats",
+    "wf_gen_cylinder",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libweldform_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m weldform_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, d = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int32)
+    I64P = ctypes.POINTER(ctypes.c_int64)
+    sig = {
+        "wf_abi_version": ([], ctypes.c_int),
+        "wf_last_error": ([vp], ctypes.c_char_p),
+        "wf_device_count": ([ip], ctypes.c_int),
+        "wf_create": ([ctypes.POINTER(Config), i64, ctypes.POINTER(vp)], ctypes.c_int),
+        "wf_destroy": ([vp], ctypes.c_int),
+        "wf_set_material": ([vp, i32, ctypes.POINTER(Material)], ctypes.c_int),
+        "wf_set_field": ([vp, i32, dp], ctypes.c_int),
+        "wf_get_field": ([vp, i32, dp], ctypes.c_int),
+        "wf_set_int_field": ([vp, i32, ip], ctypes.c_int),
+        "wf_get_int_field": ([vp, i32, ip], ctypes.c_int),
+        "wf_add_amplitude": ([vp, i32, i32, dp, dp], ctypes.c_int),
+        "wf_add_velocity_bc": ([vp, i32, i32, dp, i32, d], ctypes.c_int),
+        "wf_add_temperature_bc": ([vp, i32, d], ctypes.c_int),
+        "wf_init": ([vp, d], ctypes.c_int),
+        "wf_step": ([vp, i32, i64], ctypes.c_int),
+        "wf_sync": ([vp], ctypes.c_int),
+        "wf_get_scalars": ([vp, ctypes.POINTER(Scalars)], ctypes.c_int),
+        "wf_get_reaction_history": ([vp, i32, dp, i64, I64P], ctypes.c_int),
+        "wf_rebuild_neighbours": ([vp], ctypes.c_int),
+        "wf_get_neighbour_pairs": ([vp, ip, i64, I64P], ctypes.c_int),
+        "wf_eval": ([vp, i32, d], ctypes.c_int),
+        "wf_set_timing": ([vp, i32], ctypes.c_int),
+        "wf_get_kernel_time": ([vp, i32, dp, I64P], ctypes.c_int),
+        "wf_get_neighbour_stats": ([vp, I64P, ip], ctypes.c_int),
+        "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def gen_cylinder(V, R, L, r):
+    """Domain::AddCylinderLength lattice positions, shape (n, 3)."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    n = lib().wf_gen_cylinder(_dp(V), R, L, r, None, 0)
+    if n < 0:
+        raise WFError(n, "bad cylinder parameters")
+    out = np.zeros((n, 3))
+    lib().wf_gen_cylinder(_dp(V), R, L, r, _dp(out), n)
+    return out
+
+
+def device_count():
+    c = ctypes.c_int32(0)
+    rc = lib().wf_device_count(ctypes.byref(c))
+    return c.value if rc == 0 else 0
+
+
+class Solver:
+    """One device context (one GPU, one HIP stream)."""
+
+    def __init__(self, n, precision=FP64, ts_nb_inc=5, max_nb=128, thermal_solver=False,
+                 pl_heating=False, device=0, cellfac=2.0, pl_work_heat_frac=0.9,
+                 dom_max=(-1e11, -1e11, -1e11), dom_min=(1e11, 1e11, 1e11)):
+        cfg = Config(precision=precision, ts_nb_inc=ts_nb_inc, max_nb=max_nb,
+                     thermal_solver=int(thermal_solver), pl_heating=int(pl_heating),
+                     device=device, cellfac=cellfac, pl_work_heat_frac=pl_work_heat_frac)
+        cfg.dom_max[:] = list(dom_max)
+        cfg.dom_min[:] = list(dom_min)
+        self.n = int(n)
+        self.ctx = ctypes.c_void_p()
+        self._check(lib().wf_create(ctypes.byref(cfg), self.n, ctypes.byref(self.ctx)), glob=True)
+
+    def close(self):
+        if getattr(self, "ctx", None) and self.ctx.value:
+            lib().wf_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, glob=False):
+        if rc != 0:
+            msg = lib().wf_last_error(None if glob else self.ctx)
+            raise WFError(rc, msg.decode() if msg else "")
+
+    # fields -------------------------------------------------------------
+    def set(self, name, arr):
+        fid, dim = FIELDS[name]
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        if a.size != dim * self.n:
+            raise ValueError(f"{name}: expected {dim * self.n} values, got {a.size}")
+        self._check(lib().wf_set_field(self.ctx, fid, _dp(a)))
+
+    def get(self, name):
+        fid, dim = FIELDS[name]
+        a = np.zeros(dim * self.n)
+        self._check(lib().wf_get_field(self.ctx, fid, _dp(a)))
+        return a.reshape(self.n, 3, 3) if dim == 9 else (a.reshape(self.n, 3) if dim == 3 else a)
+
+    def set_int(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.int32).reshape(-1)
+        if a.size != self.n:
+            raise ValueError(f"{name}: expected {self.n} values")
+        self._check(lib().wf_set_int_field(self.ctx, INT_FIELDS[name], _ip(a)))
+
+    def get_int(self, name):
+        a = np.zeros(self.n, dtype=np.int32)
+        self._check(lib().wf_get_int_field(self.ctx, INT_FIELDS[name], _ip(a)))
+        return a
+
+    def set_material(self, idx, **kw):
+        m = Material(**kw)
+        self._check(lib().wf_set_material(self.ctx, int(idx), ctypes.byref(m)))
+
+    # boundary conditions ---------------------------------------------------
+    def add_amplitude(self, aid, t, v):
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._check(lib().wf_add_amplitude(self.ctx, aid, len(t), _dp(t), _dp(v)))
+
+    def add_velocity_bc(self, zone, value, value_type=0, amp_id=0, amp_factor=1.0):
+        v = np.ascontiguousarray(value, dtype=np.float64)
+        self._check(lib().wf_add_velocity_bc(self.ctx, zone, value_type, _dp(v), amp_id, amp_factor))
+
+    def add_temperature_bc(self, zone, T):
+        self._check(lib().wf_add_temperature_bc(self.ctx, zone, float(T)))
+
+    # solve -----------------------------------------------------------------
+    def init(self, dt):
+        self._check(lib().wf_init(self.ctx, float(dt)))
+
+    def step(self, nsteps=1, scheme=LEAPFROG):
+        self._check(lib().wf_step(self.ctx, scheme, int(nsteps)))
+
+    def sync(self):
+        self._check(lib().wf_sync(self.ctx))
+
+    def scalars(self):
+        s = Scalars()
+        self._check(lib().wf_get_scalars(self.ctx, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reaction_history(self, zone, cap=1 << 16):
+        out = np.zeros(3 * cap)
+        cnt = ctypes.c_int64(0)
+        self._check(lib().wf_get_reaction_history(self.ctx, zone, _dp(out), cap, ctypes.byref(cnt)))
+        return out[:3 * cnt.value].reshape(-1, 3)
+
+    def rebuild_neighbours(self):
+        self._check(lib().wf_rebuild_neighbours(self.ctx))
+
+    def neighbour_pairs(self):
+        cnt = ctypes.c_int64(0)
+        self._check(lib().wf_get_neighbour_pairs(self.ctx, None, 0, ctypes.byref(cnt)))
+        out = np.zeros(2 * max(cnt.value, 1), dtype=np.int32)
+        self._check(lib().wf_get_neighbour_pairs(self.ctx, _ip(out), cnt.value, ctypes.byref(cnt)))
+        return out[:2 * cnt.value].reshape(-1, 2)
+
+    def eval(self, what, dt=0.0):
+        self._check(lib().wf_eval(self.ctx, EVAL[what], float(dt)))
+
+    def set_timing(self, on):
+        self._check(lib().wf_set_timing(self.ctx, int(bool(on))))
+
+    def kernel_time(self, kernel):
+        ms = ctypes.c_double(0)
+        cnt = ctypes.c_int64(0)
+        self._check(lib().wf_get_kernel_time(self.ctx, KERNELS[kernel], ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def neighbour_stats(self):
+        p = ctypes.c_int64(0)
+        m = ctypes.c_int32(0)
+        self._check(lib().wf_get_neighbour_stats(self.ctx, ctypes.byref(p), ctypes.byref(m)))
+        return p.value, m.value

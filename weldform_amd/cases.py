@@ -1,0 +1,156 @@
+"""Workload definitions: cylinder upsetting cases built the way the reference's
+JSON driver builds them (Source/WeldForm.cpp:224-369, 884-921).
+
+A Case holds plain numpy arrays in the reference's particle order plus the
+material table and boundary conditions, so the same case can be loaded into
+the device solver (weldform_amd.capi.Solver) or into any checker.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+
+@dataclass
+class Case:
+    name: str
+    x: np.ndarray
+    h: np.ndarray
+    mass: np.ndarray
+    density: np.ndarray
+    ids: np.ndarray
+    material: np.ndarray
+    materials: dict            # idx -> dict of wf_material fields
+    sigmay: np.ndarray
+    dt: float
+    steps: int
+    T: np.ndarray = None
+    bcs: list = field(default_factory=list)     # dicts: zone, value, value_type, amp_id, amp_factor
+    tbcs: list = field(default_factory=list)    # dicts: zone, T
+    amps: list = field(default_factory=list)    # dicts: id, t, v
+    dom_max: tuple = (-1e11, -1e11, -1e11)
+    dom_min: tuple = (1e11, 1e11, 1e11)
+    thermal: bool = False
+    pl_heating: bool = False
+    top_zone: int = 3
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self):
+        return len(self.h)
+
+
+def elastic(E, nu, rho):
+    """Derived constants as in Source/WeldForm.cpp:338-343."""
+    K = E / (3. * (1. - 2 * nu))
+    G = E / (2. * (1. + nu))
+    Cs = math.sqrt(K / rho)
+    return K, G, Cs
+
+
+def _zones(x, r):
+    z = x[:, 2]
+    ids = np.zeros(len(z), dtype=np.int32)
+    zlo, zhi = z.min(), z.max()
+    ids[z <= zlo + 0.5 * r] = 2
+    ids[z >= zhi - 0.5 * r] = 3
+    return ids
+
+
+def cylinder_case(name, R, L, dx, rho, E, nu, mat, dt_cfl=0.4, hfac=1.2, steps=1000,
+                  jitter=0.0, seed=0, v_top=-1.0, dommax_quirk=True, T0=None):
+    r = dx / 2.
+    x = capi.gen_cylinder([0.0, 0.0, 0.0], R, L, r)
+    if jitter > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        x = x + rng.uniform(-jitter * dx, jitter * dx, size=x.shape)
+    n = len(x)
+    K, G, Cs = elastic(E, nu, rho)
+    h = 2. * r * hfac
+    # AddCylinderLength mass (Source/Domain.cpp:840-855)
+    vol = math.pi * R * R * L
+    m = vol * rho / n
+    dt = dt_cfl * h / Cs
+    ids = _zones(x, r)
+    mat = dict(mat)
+    mat.update(E=E, nu=nu, G=G, Cs=Cs, ref_density=rho, P0=0.0)
+    if mat["model"] == capi.JOHNSON_COOK:
+        # initial yield = JC(0, 0, T) (Source/WeldForm.cpp:912-913)
+        Th = (T0 - mat["jc_Tt"]) / (mat["jc_Tm"] - mat["jc_Tt"])
+        sy0 = (mat["jc_A"] + mat["jc_B"] * 0.0 ** mat["jc_n"]) * 1.0 * (1.0 - Th ** mat["jc_m"])
+    elif mat["model"] == capi.HOLLOMON:
+        sy0 = mat["hol_K"] * 0 + mat["hol_sy0"]
+    else:
+        sy0 = mat.pop("sy0")
+    mat.pop("sy0", None)
+    case = Case(
+        name=name, x=x, h=np.full(n, h), mass=np.full(n, m), density=np.full(n, rho),
+        ids=ids, material=np.zeros(n, dtype=np.int32), materials={0: mat},
+        sigmay=np.full(n, sy0), dt=dt, steps=steps,
+        T=None if T0 is None else np.full(n, float(T0)),
+        bcs=[dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, 0.0, v_top))],
+        meta=dict(R=R, L=L, dx=dx, h=h, K=K, jitter=jitter, seed=seed))
+    if dommax_quirk:
+        # WeldForm.cpp:408-411 writes DomMax(0)=L[i], DomMin(0)=-L[i] for i=0..2,
+        # leaving index 0 = the block length along z
+        case.dom_max = (L, -1e11, -1e11)
+        case.dom_min = (-L, 1e11, 1e11)
+    return case
+
+
+AL = dict(rho=2700.0, E=68.9e9, nu=0.3)
+
+
+def config1(steps=1000):
+    """Plain FP64 cylinder upsetting: R=7.5 mm, L=30 mm, dx=1 mm (4,524 particles),
+    bilinear Sy=300 MPa, Et=1.5 GPa, alpha=1, beta=0, top row v_z=-1 m/s."""
+    E = AL["E"]
+    Et = 1.5e9
+    mat = dict(model=capi.BILINEAR, alpha=1.0, beta=0.0, Ep=E * Et / (E - Et), sy0=300e6)
+    return cylinder_case("cyl_upset_4k_bilinear", 7.5e-3, 30e-3, 1.0e-3, AL["rho"], E, AL["nu"], mat, steps=steps)
+
+
+JC_AL = dict(model=capi.JOHNSON_COOK, alpha=1.0, beta=0.0, jc_A=175e6, jc_B=380e6, jc_n=0.32,
+             jc_C=0.0015, jc_eps0=1.0, jc_m=1.0, jc_Tm=775.0, jc_Tt=300.0)
+
+
+def config2(steps=1000, dx=0.5537e-3, R=30e-3, L=60e-3, seed=2026):
+    """Cylinder upsetting at ~1M particles: R=30 mm, L=60 mm, dx=0.5537 mm
+    (976,320 particles; dx=0.42 mm would give 2,248,144), +-1e-3 dx seeded
+    lattice jitter, Johnson-Cook hardening with eps_0=1/s and T=T_t=300 K."""
+    return cylinder_case("cyl_upset_1M_jc", R, L, dx, AL["rho"], AL["E"], AL["nu"], JC_AL,
+                         steps=steps, jitter=1e-3, seed=seed, T0=300.0)
+
+
+def small_jc(steps=200, dx=1.0e-3, R=7.5e-3, L=30e-3, seed=7):
+    """Config-2 physics (JC, jitter) at config-1 size, for parity tests."""
+    return cylinder_case("cyl_upset_4k_jc", R, L, dx, AL["rho"], AL["E"], AL["nu"], JC_AL,
+                         steps=steps, jitter=1e-3, seed=seed, T0=300.0)
+
+
+def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5):
+    s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
+                    thermal_solver=case.thermal, pl_heating=case.pl_heating,
+                    dom_max=case.dom_max, dom_min=case.dom_min)
+    for idx, m in case.materials.items():
+        s.set_material(idx, **m)
+    s.set("x", case.x)
+    s.set("h", case.h)
+    s.set("mass", case.mass)
+    s.set("density", case.density)
+    s.set("sigmay", case.sigmay)
+    if case.T is not None:
+        s.set("T", case.T)
+    s.set_int("id", case.ids)
+    s.set_int("material", case.material)
+    for a in case.amps:
+        s.add_amplitude(a["id"], a["t"], a["v"])
+    for b in case.bcs:
+        s.add_velocity_bc(b["zone"], b["value"], b.get("value_type", 0), b.get("amp_id", 0),
+                          b.get("amp_factor", 1.0))
+    for b in case.tbcs:
+        s.add_temperature_bc(b["zone"], b["T"])
+    s.init(case.dt)
+    return s

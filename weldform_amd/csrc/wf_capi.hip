@@ -1,0 +1,1128 @@
+// wf_capi.hip -- C-ABI of libweldform_hip.so (declared in include/weldform_hip.h).
+//
+// Host side of the device path: staging of particle fields in the caller's
+// order, the reference's cell grid (CellInitiate), the Morton permutation,
+// device allocation, and the per-step launch sequence of the Leapfrog and
+// KickDrift loops.  No host<->device synchronisation happens inside
+// wf_step: the neighbour-search schedule lives on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wf_internal.h"
+
+namespace {
+
+using namespace wf;
+
+thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(call)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      throw Error(WF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+
+const int kFieldDim[WF_NFIELDS] = {3, 3, 3, 3, 1, 1, 1, 9, 9, 9, 9, 9, 9, 1,
+                                   1, 1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1};
+
+struct BcHost {
+  int zone, type, value_type, amp_id;
+  double value[3], amp_factor, T;
+};
+struct AmpHost {
+  int id;
+  std::vector<double> t, v;
+  // amplitude::getValAtTime (Source/Domain.h:75-87)
+  double at(double time) const {
+    int i = (int)t.size() - 1;
+    bool end = false;
+    while (!end) {
+      i--;
+      if (i == 0 || time > t[i]) end = true;
+    }
+    return v[i] + (v[i + 1] - v[i]) / (t[i + 1] - t[i]) * (time - t[i]);
+  }
+};
+
+enum { T_ACCEL = 0, T_DENS, T_RATES, T_NBR, T_STEP, T_COUNT };
+
+struct Ctx {
+  wf_config cfg{};
+  int64_t n = 0;
+  long long npad = 0;
+  hipStream_t st = nullptr;
+  std::vector<double> hf[WF_NFIELDS];
+  std::vector<int32_t> hi[WF_NIFIELDS];
+  std::vector<wf_material> mats;
+  std::vector<char> mat_set;
+  std::vector<BcHost> bcs;
+  std::vector<AmpHost> amps;
+  bool inited = false;
+  bool built = false;
+  std::vector<int32_t> perm, inv;  // perm[internal] = original
+  Dev d{};
+  std::vector<void *> allocs;
+  DevMat *dmats = nullptr;
+  int cur = 0;
+  double time = 0.0, dt = 0.0;
+  long long steps = 0;
+  std::vector<int> rzones;
+  std::string err;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Rec {
+    int k;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  double tsum[T_COUNT] = {0};
+  long long tcnt[T_COUNT] = {0};
+  size_t ev_next = 0;
+
+  template <class T>
+  T *alloc(size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc(&p, count * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, count * sizeof(T), st));
+    allocs.push_back(p);
+    return (T *)p;
+  }
+  ~Ctx() {
+    for (auto &e : ev_pool) (void)hipEventDestroy(e);
+    for (void *p : allocs) (void)hipFree(p);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+int fail(Ctx *c, int code, const std::string &m) {
+  g_err = m;
+  if (c) c->err = m;
+  return code;
+}
+
+template <class F>
+int guard(Ctx *c, F &&f) {
+  try {
+    return f();
+  } catch (const Error &e) {
+    return fail(c, e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(c, WF_ENOMEM, "host out of memory");
+  } catch (const std::exception &e) {
+    return fail(c, WF_EINVAL, e.what());
+  }
+}
+
+Ctx *as_ctx(void *p) {
+  if (!p) throw Error(WF_EINVAL, "null context");
+  return (Ctx *)p;
+}
+
+// ---------------------------------------------------------------------------
+// device <-> host helpers (internal order <-> caller's order)
+
+template <class T>
+std::vector<T> dget(Ctx *c, const T *p, size_t count) {
+  std::vector<T> h(count);
+  HIPCHK(hipMemcpyAsync(h.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return h;
+}
+template <class T>
+void dput(Ctx *c, T *p, const std::vector<T> &h) {
+  HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// symmetric storage index of (a,b)
+inline int sym_index(int a, int b) {
+  if (a == b) return a;
+  if (a > b) std::swap(a, b);
+  return a == 0 ? (b == 1 ? XY : XZ) : YZ;
+}
+
+// component-major [6][npad] from 9-per-particle caller data
+void put_sym(Ctx *c, double *dev, const double *h9) {
+  std::vector<double> t((size_t)6 * c->npad, 0.0);
+  for (int64_t k = 0; k < c->n; ++k) {
+    const double *m = h9 + 9 * (size_t)c->perm[k];
+    const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) t[(size_t)q * c->npad + k] = m[3 * ia[q] + ib[q]];
+  }
+  dput(c, dev, t);
+}
+void get_sym(Ctx *c, const double *dev, double *h9) {
+  auto t = dget(c, dev, (size_t)6 * c->npad);
+  for (int64_t k = 0; k < c->n; ++k) {
+    double *m = h9 + 9 * (size_t)c->perm[k];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) m[3 * a + b] = t[(size_t)sym_index(a, b) * c->npad + k];
+  }
+}
+void put_scalar(Ctx *c, double *dev, const double *h) {
+  std::vector<double> t((size_t)c->npad, 0.0);
+  for (int64_t k = 0; k < c->n; ++k) t[k] = h[c->perm[k]];
+  dput(c, dev, t);
+}
+void get_scalar(Ctx *c, const double *dev, double *h) {
+  auto t = dget(c, dev, (size_t)c->npad);
+  for (int64_t k = 0; k < c->n; ++k) h[c->perm[k]] = t[k];
+}
+// double4 arrays: replace components [c0, c0+nc) from caller data of width w
+void put_d4(Ctx *c, double4 *dev, const double *h, int c0, int nc, int w) {
+  auto t = dget(c, dev, (size_t)c->npad);
+  for (int64_t k = 0; k < c->n; ++k) {
+    double *e = &t[k].x;
+    for (int q = 0; q < nc; ++q) e[c0 + q] = h[(size_t)w * c->perm[k] + q];
+  }
+  dput(c, dev, t);
+}
+void get_d4(Ctx *c, const double4 *dev, double *h, int c0, int nc, int w) {
+  auto t = dget(c, dev, (size_t)c->npad);
+  for (int64_t k = 0; k < c->n; ++k) {
+    const double *e = &t[k].x;
+    for (int q = 0; q < nc; ++q) h[(size_t)w * c->perm[k] + q] = e[c0 + q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// set-up: CellInitiate (Source/Neighbour.cpp:10-139), 3D, non-periodic
+
+void cell_initiate(Ctx *c) {
+  const std::vector<double> &x = c->hf[WF_F_X], &h = c->hf[WF_F_H];
+  double TRPR[3], BLPF[3], hmax = h[0];
+  for (int k = 0; k < 3; ++k) TRPR[k] = BLPF[k] = x[k];
+  for (int64_t i = 0; i < c->n; ++i) {
+    for (int k = 0; k < 3; ++k) {
+      if (x[3 * i + k] > TRPR[k]) TRPR[k] = x[3 * i + k];
+      if (x[3 * i + k] < BLPF[k]) BLPF[k] = x[3 * i + k];
+    }
+    if (h[i] > hmax) hmax = h[i];
+  }
+  for (int k = 0; k < 3; ++k) {
+    if (c->cfg.dom_max[k] > TRPR[k]) TRPR[k] = c->cfg.dom_max[k];
+    if (c->cfg.dom_min[k] < BLPF[k]) BLPF[k] = c->cfg.dom_min[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    TRPR[k] += hmax / 2;
+    BLPF[k] -= hmax / 2;
+  }
+  long long ncells = 1;
+  for (int k = 0; k < 3; ++k) {
+    const double e = (TRPR[k] - BLPF[k]) / (c->cfg.cellfac * hmax);
+    int cn = ((double)(std::ceil(e) - e) < (hmax / 10.0)) ? (int)std::ceil(e) : (int)std::floor(e);
+    if (cn <= 0) throw Error(WF_EINVAL, "CellInitiate: zero cells along an axis");
+    c->d.cellno[k] = cn;
+    c->d.blpf[k] = BLPF[k];
+    ncells *= cn;
+  }
+  if (ncells > (1LL << 30)) throw Error(WF_EINVAL, "CellInitiate: too many cells");
+  for (int k = 0; k < 3; ++k) c->d.cellsize[k] = (TRPR[k] - BLPF[k]) / c->d.cellno[k];
+  c->d.ncells = (int)ncells;
+}
+
+uint64_t spread3(uint64_t v) {
+  v &= 0x1fffff;
+  v = (v | v << 32) & 0x1f00000000ffffULL;
+  v = (v | v << 16) & 0x1f0000ff0000ffULL;
+  v = (v | v << 8) & 0x100f00f00f00f00fULL;
+  v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+  v = (v | v << 2) & 0x1249249249249249ULL;
+  return v;
+}
+
+int host_cell_coord(double x, double blpf, double cs, int cn) {
+  double f = std::floor((x - blpf) / cs);
+  if (!(f >= 0.0)) return 0;
+  if (f >= (double)cn) return cn - 1;
+  return (int)f;
+}
+
+// internal order: Morton order of the particles' initial cells (finer grid
+// of half a cell so neighbouring lanes share cache lines), ties by index
+void make_permutation(Ctx *c) {
+  const std::vector<double> &x = c->hf[WF_F_X];
+  std::vector<std::pair<uint64_t, int32_t>> key(c->n);
+  for (int64_t i = 0; i < c->n; ++i) {
+    uint64_t m = 0;
+    for (int k = 0; k < 3; ++k) {
+      const int cc = host_cell_coord(x[3 * i + k], c->d.blpf[k], c->d.cellsize[k] * 0.5, 2 * c->d.cellno[k]);
+      m |= spread3((uint64_t)cc) << k;
+    }
+    key[i] = {m, (int32_t)i};
+  }
+  std::sort(key.begin(), key.end());
+  c->perm.resize(c->n);
+  c->inv.resize(c->n);
+  for (int64_t k = 0; k < c->n; ++k) {
+    c->perm[k] = key[k].second;
+    c->inv[key[k].second] = (int32_t)k;
+  }
+}
+
+DevMat to_dev(const wf_material &m) {
+  DevMat d{};
+  d.model = m.model;
+  d.E = m.E;
+  d.G = m.G;
+  d.Cs = m.Cs;
+  d.ref_density = m.ref_density;
+  d.P0 = m.P0;
+  d.alpha = m.alpha;
+  d.beta = m.beta;
+  // CalcStressStrain replaces a negative Ep by E on the plastic branch
+  // (Source/Particle.cpp:1020), the only place Ep is read
+  d.Ep = m.Ep < 0 ? 1. * m.E : m.Ep;
+  d.hol_K = m.hol_K;
+  d.hol_m = m.hol_m;
+  d.hol_sy0 = m.hol_sy0;
+  if (m.model == WF_HOLLOMON) {
+    d.hol_eps0 = m.hol_sy0 / m.E;
+    d.hol_eps1 = std::pow(m.hol_sy0 / m.hol_K, 1. / m.hol_m);
+  }
+  d.jc_A = m.jc_A;
+  d.jc_B = m.jc_B;
+  d.jc_n = m.jc_n;
+  d.jc_C = m.jc_C;
+  d.jc_eps0 = m.jc_eps0;
+  d.jc_m = m.jc_m;
+  d.jc_Tm = m.jc_Tm;
+  d.jc_Tt = m.jc_Tt;
+  d.k_T = m.k_T;
+  d.cp_T = m.cp_T;
+  d.th_exp = m.th_exp;
+  return d;
+}
+
+void allocate(Ctx *c) {
+  Dev &d = c->d;
+  const size_t np = (size_t)c->npad;
+  d.n = (int)c->n;
+  d.npad = c->npad;
+  d.max_nb = c->cfg.max_nb;
+  d.cellfac = c->cfg.cellfac;
+  d.pl_frac = c->cfg.pl_work_heat_frac;
+  d.thermal = c->cfg.thermal_solver;
+  d.pl_heating = c->cfg.pl_heating;
+  d.mixed = c->cfg.precision == WF_MIXED;
+  d.nblocks = (int)((c->n + kBlock - 1) / kBlock);
+  for (int b = 0; b < 2; ++b) {
+    d.xh[b] = c->alloc<double4>(np);
+    d.rho[b] = c->alloc<double>(np);
+    d.vol[b] = c->alloc<double>(np);
+  }
+  d.vm = c->alloc<double4>(np);
+  d.acc = c->alloc<double4>(np);
+  d.disp = c->alloc<double4>(np);
+  d.srec = c->alloc<StressRec>(np);
+  d.S = c->alloc<double>(6 * np);
+  d.strain = c->alloc<double>(6 * np);
+  d.epl = c->alloc<double>(6 * np);
+  d.eplinc = c->alloc<double>(6 * np);
+  d.sr = c->alloc<double>(6 * np);
+  d.rr = c->alloc<double>(3 * np);
+  double **scal[] = {&d.pl, &d.dpl, &d.sigy, &d.effsr, &d.pres, &d.drho, &d.T, &d.dTdt,
+                     &d.qpl, &d.psen, &d.qconv, &d.hconv, &d.tinf, &d.qsrc};
+  for (double **p : scal) *p = c->alloc<double>(np);
+  int **ints[] = {&d.id, &d.mat, &d.group, &d.thbc, &d.perm, &d.cell, &d.sorted, &d.nnb};
+  for (int **p : ints) *p = c->alloc<int>(np);
+  d.nbr = c->alloc<int>((size_t)c->cfg.max_nb * np);
+  d.cell_count = c->alloc<int>((size_t)d.ncells + 1);
+  d.cell_start = c->alloc<int>((size_t)d.ncells + 1);
+  d.cell_cursor = c->alloc<int>((size_t)d.ncells + 1);
+  d.scan_tmp = c->alloc<int>((size_t)(d.ncells + 1023) / 1024 + 1);
+  c->dmats = c->alloc<DevMat>(kMaxMat);
+  d.mats = c->dmats;
+  d.flags = c->alloc<DevFlags>(1);
+  d.partials = c->alloc<double>((size_t)d.nblocks * P_COUNT);
+  d.scalars = c->alloc<double>(S_COUNT);
+  d.hist_cap = 1 << 16;
+  d.react_hist = c->alloc<double>((size_t)d.hist_cap * kMaxZones * 3);
+}
+
+void upload_all(Ctx *c) {
+  Dev &d = c->d;
+  const int64_t n = c->n;
+  const size_t np = (size_t)c->npad;
+  auto &F = c->hf;
+  auto field = [&](int f) -> const double * { return F[f].empty() ? nullptr : F[f].data(); };
+  // positions, h, velocities, mass
+  std::vector<double4> xh(np), vm(np), z4(np);
+  for (int64_t k = 0; k < n; ++k) {
+    const int o = c->perm[k];
+    xh[k] = make_double4(F[WF_F_X][3 * o], F[WF_F_X][3 * o + 1], F[WF_F_X][3 * o + 2], F[WF_F_H][o]);
+    const double *v = field(WF_F_V);
+    vm[k] = make_double4(v ? v[3 * o] : 0.0, v ? v[3 * o + 1] : 0.0, v ? v[3 * o + 2] : 0.0, F[WF_F_MASS][o]);
+  }
+  dput(c, d.xh[0], xh);
+  dput(c, d.xh[1], xh);
+  dput(c, d.vm, vm);
+  if (field(WF_F_A)) put_d4(c, d.acc, field(WF_F_A), 0, 3, 3);
+  if (field(WF_F_DISP)) put_d4(c, d.disp, field(WF_F_DISP), 0, 3, 3);
+  // density, volumes, stress records, pressure (InitialChecks,
+  // Source/Domain.cpp:2097-2113: Pressure = EOS(Density))
+  const std::vector<int32_t> &mt = c->hi[WF_I_MATERIAL];
+  std::vector<double> rho(np, 0.0), vol(np, 0.0), pres(np, 0.0);
+  std::vector<StressRec> rec(np);
+  const double *sig = field(WF_F_SIGMA);
+  for (int64_t k = 0; k < n; ++k) {
+    const int o = c->perm[k];
+    const double r = F[WF_F_DENSITY][o];
+    const wf_material &M = c->mats[mt[o]];
+    rho[k] = r;
+    vol[k] = F[WF_F_MASS][o] / r;
+    pres[k] = M.P0 + (M.Cs * M.Cs) * (r - M.ref_density);
+    StressRec &R = rec[k];
+    const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    for (int q = 0; q < 6; ++q) R.s[q] = sig ? sig[9 * (size_t)o + 3 * ia[q] + ib[q]] : 0.0;
+    R.rho = r;
+    R.irho2 = 1.0 / (r * r);
+  }
+  dput(c, d.rho[0], rho);
+  dput(c, d.rho[1], rho);
+  dput(c, d.vol[0], vol);
+  dput(c, d.vol[1], vol);
+  dput(c, d.pres, pres);
+  dput(c, d.srec, rec);
+  // tensors
+  if (field(WF_F_SHEAR)) put_sym(c, d.S, field(WF_F_SHEAR));
+  if (field(WF_F_STRAIN)) put_sym(c, d.strain, field(WF_F_STRAIN));
+  if (field(WF_F_STRAIN_PL)) put_sym(c, d.epl, field(WF_F_STRAIN_PL));
+  if (field(WF_F_STRAIN_PL_INCR)) put_sym(c, d.eplinc, field(WF_F_STRAIN_PL_INCR));
+  if (field(WF_F_STRAIN_RATE)) put_sym(c, d.sr, field(WF_F_STRAIN_RATE));
+  // scalars
+  struct {
+    int f;
+    double *p;
+  } sc[] = {{WF_F_PL_STRAIN, d.pl}, {WF_F_DPL_STRAIN, d.dpl}, {WF_F_SIGMAY, d.sigy}, {WF_F_EFF_SR, d.effsr},
+            {WF_F_T, d.T},          {WF_F_DTDT, d.dTdt},     {WF_F_Q_PLHEAT, d.qpl}, {WF_F_Q_CONV, d.qconv},
+            {WF_F_H_CONV, d.hconv}, {WF_F_T_INF, d.tinf},    {WF_F_Q_SOURCE, d.qsrc}, {WF_F_PS_ENERGY, d.psen}};
+  for (auto &s : sc)
+    if (field(s.f)) put_scalar(c, s.p, field(s.f));
+  // ints
+  std::vector<int> id(np, 0), mat(np, 0), grp(np, -1), thbc(np, 0), perm(np, 0);
+  for (int64_t k = 0; k < n; ++k) {
+    const int o = c->perm[k];
+    id[k] = c->hi[WF_I_ID][o];
+    mat[k] = c->hi[WF_I_MATERIAL][o];
+    grp[k] = c->hi[WF_I_FREE][o] ? mat[k] : -1;
+    thbc[k] = c->hi[WF_I_THERMAL_BC][o];
+    perm[k] = o;
+  }
+  dput(c, d.id, id);
+  dput(c, d.mat, mat);
+  dput(c, d.group, grp);
+  dput(c, d.thbc, thbc);
+  dput(c, d.perm, perm);
+  std::vector<DevMat> dm(kMaxMat);
+  for (size_t k = 0; k < c->mats.size() && k < (size_t)kMaxMat; ++k)
+    if (c->mat_set[k]) dm[k] = to_dev(c->mats[k]);
+  dput(c, c->dmats, dm);
+  // device schedule state: the pre-loop ClearNbData leaves the data cleared
+  DevFlags fl{};
+  fl.cleared = 1;
+  fl.ts_nb_inc = c->cfg.ts_nb_inc;
+  const double *pl = field(WF_F_PL_STRAIN);
+  if (pl)
+    for (int64_t i = 0; i < n; ++i)
+      if (pl[i] > 1.e-6) fl.yielded = 1;
+  std::vector<DevFlags> fv(1, fl);
+  dput(c, d.flags, fv);
+}
+
+// ---------------------------------------------------------------------------
+// timing
+
+hipEvent_t next_event(Ctx *c) {
+  if (c->ev_next == c->ev_pool.size()) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_next++];
+}
+
+struct Timed {
+  Ctx *c;
+  int k;
+  hipEvent_t a{}, b{};
+  Timed(Ctx *cc, int kk) : c(cc), k(kk) {
+    if (c->timing) {
+      a = next_event(c);
+      b = next_event(c);
+      HIPCHK(hipEventRecord(a, c->st));
+    }
+  }
+  ~Timed() noexcept(false) {
+    if (c->timing) {
+      HIPCHK(hipEventRecord(b, c->st));
+      c->recs.push_back({k, a, b});
+    }
+  }
+};
+
+void harvest_timing(Ctx *c) {
+  if (c->recs.empty()) return;
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (auto &r : c->recs) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    c->tsum[r.k] += ms;
+    c->tcnt[r.k] += 1;
+  }
+  c->recs.clear();
+  c->ev_next = 0;
+}
+
+BcTable make_bc(Ctx *c) {
+  BcTable t{};
+  for (const BcHost &b : c->bcs) {
+    if (t.n >= kMaxBc) throw Error(WF_EUNSUPPORTED, "more than 8 boundary conditions");
+    const int k = t.n++;
+    t.zone[k] = b.zone;
+    t.type[k] = b.type;
+    t.T[k] = b.T;
+    if (b.type == 0) {
+      if (b.value_type == 0) {
+        for (int q = 0; q < 3; ++q) t.v[k][q] = b.value[q];
+      } else {
+        // UserAcc amplitude branch (Source/WeldForm.cpp:73-81)
+        bool found = false;
+        for (const AmpHost &a : c->amps)
+          if (a.id == b.amp_id) {
+            const double val = b.amp_factor * a.at(c->time);
+            for (int q = 0; q < 3; ++q) t.v[k][q] = val * b.value[q];
+            found = true;
+          }
+        if (!found) t.zone[k] = -0x7ffffff;  // no matching amplitude: BC inactive
+      }
+    }
+  }
+  t.nz = (int)c->rzones.size();
+  for (int z = 0; z < t.nz; ++z) t.rzone[z] = c->rzones[z];
+  return t;
+}
+
+void check_flags(Ctx *c) {
+  auto f = dget(c, c->d.flags, 1)[0];
+  if (f.overflow)
+    throw Error(WF_ENBOVERFLOW, "a particle has more neighbours than max_nb (" + std::to_string(c->cfg.max_nb) +
+                                    "); largest count " + std::to_string(f.max_count));
+  if (f.nan) throw Error(WF_ENAN, "non-finite value in the particle state");
+}
+
+void one_step(Ctx *c, int scheme) {
+  Dev &d = c->d;
+  const int isfirst = c->steps == 0;
+  Timed whole(c, T_STEP);
+  launch_schedule(d, c->st, isfirst, 0);
+  {
+    Timed t(c, T_NBR);
+    launch_neighbours(d, c->st, c->cur);
+  }
+  c->built = true;
+  {
+    Timed t(c, T_ACCEL);
+    launch_accel(d, c->st, c->cur);
+  }
+  const BcTable bc = make_bc(c);
+  launch_kick(d, c->st, bc, scheme, isfirst, c->dt, c->dt);
+  {
+    Timed t(c, T_DENS);
+    launch_dens_drift(d, c->st, c->cur, c->dt, 1);
+  }
+  if (scheme == WF_KICKDRIFT) launch_kick2(d, c->st, c->dt);
+  const int nxt = c->cur ^ 1;
+  {
+    Timed t(c, T_RATES);
+    launch_rates_stress(d, c->st, nxt, c->dt, 1, 1);
+  }
+  if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, c->dt);
+  launch_finalize(d, c->st, c->dt, c->steps, bc.nz, 1);
+  HIPCHK(hipGetLastError());
+  c->cur = nxt;
+  c->time += c->dt;
+  c->steps += 1;
+}
+
+}  // namespace
+
+// ===========================================================================
+// exported C-ABI
+
+extern "C" {
+
+int wf_abi_version(void) { return WF_ABI_VERSION; }
+
+const char *wf_last_error(const void *ctx) {
+  if (ctx) return ((const Ctx *)ctx)->err.c_str();
+  return g_err.c_str();
+}
+
+int wf_device_count(int32_t *count) {
+  return guard(nullptr, [&] {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    *count = n;
+    return WF_OK;
+  });
+}
+
+int wf_create(const wf_config *cfg, int64_t n, void **out) {
+  return guard(nullptr, [&] {
+    if (!cfg || !out) throw Error(WF_EINVAL, "null argument");
+    if (n <= 0 || n > (1LL << 31) - 1024) throw Error(WF_EINVAL, "particle count out of range");
+    if (cfg->max_nb < 1 || cfg->max_nb > kMaxNb) throw Error(WF_EINVAL, "max_nb must be in 1..128");
+    if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED) throw Error(WF_EINVAL, "bad precision");
+    if (cfg->precision == WF_MIXED) throw Error(WF_EUNSUPPORTED, "mixed precision is not available in this build");
+    if (!(cfg->cellfac > 0)) throw Error(WF_EINVAL, "cellfac must be positive");
+    if (cfg->ts_nb_inc < 1) throw Error(WF_EINVAL, "ts_nb_inc must be >= 1");
+    Ctx *c = new Ctx();
+    c->cfg = *cfg;
+    c->n = n;
+    c->npad = (n + kBlock - 1) / kBlock * kBlock;
+    c->mats.resize(kMaxMat);
+    c->mat_set.assign(kMaxMat, 0);
+    for (int f = 0; f < WF_NIFIELDS; ++f) c->hi[f].assign(n, 0);
+    c->hi[WF_I_FREE].assign(n, 1);
+    try {
+      HIPCHK(hipSetDevice(cfg->device));
+      HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+    return WF_OK;
+  });
+}
+
+int wf_destroy(void *ctx) {
+  if (!ctx) return WF_OK;
+  Ctx *c = (Ctx *)ctx;
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  delete c;
+  return WF_OK;
+}
+
+int wf_set_material(void *ctx, int32_t index, const wf_material *m) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!m || index < 0 || index >= kMaxMat) throw Error(WF_EINVAL, "material index out of range");
+    if (m->model < WF_BILINEAR || m->model > WF_JOHNSON_COOK) throw Error(WF_EINVAL, "unknown material model");
+    if (m->model == WF_HOLLOMON) {
+      // Hollomon constructor check (Source/Material.cpp:48-57)
+      const double eps0 = m->hol_sy0 / m->E, eps1 = std::pow(m->hol_sy0 / m->hol_K, 1. / m->hol_m);
+      if (eps0 > eps1)
+        throw Error(WF_EINVAL,
+                    "Hollomon material bad definition, please correct Yield Stress, Elastic Modulus or Material "
+                    "hardening constants.");
+    }
+    c->mats[index] = *m;
+    c->mat_set[index] = 1;
+    if (c->inited) {
+      std::vector<DevMat> one(1, to_dev(*m));
+      HIPCHK(hipMemcpyAsync(c->dmats + index, one.data(), sizeof(DevMat), hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
+    return WF_OK;
+  });
+}
+
+int wf_set_field(void *ctx, int32_t field, const double *h) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (field < 0 || field >= WF_NFIELDS || !h) throw Error(WF_EINVAL, "bad field");
+    const size_t cnt = (size_t)kFieldDim[field] * c->n;
+    c->hf[field].assign(h, h + cnt);
+    if (!c->inited) return WF_OK;
+    Dev &d = c->d;
+    switch (field) {
+      case WF_F_X:
+        put_d4(c, d.xh[c->cur], h, 0, 3, 3);
+        break;
+      case WF_F_H:
+        put_d4(c, d.xh[0], h, 3, 1, 1);
+        put_d4(c, d.xh[1], h, 3, 1, 1);
+        break;
+      case WF_F_V:
+        put_d4(c, d.vm, h, 0, 3, 3);
+        break;
+      case WF_F_A:
+        put_d4(c, d.acc, h, 0, 3, 3);
+        break;
+      case WF_F_DISP:
+        put_d4(c, d.disp, h, 0, 3, 3);
+        break;
+      case WF_F_MASS:
+      case WF_F_DENSITY:
+      case WF_F_SIGMA: {
+        // mass, density and stress feed vol = m/rho and the stress record
+        if (field == WF_F_MASS) put_d4(c, d.vm, h, 3, 1, 1);
+        auto vm = dget(c, d.vm, (size_t)c->npad);
+        std::vector<double> rho((size_t)c->npad);
+        if (field == WF_F_DENSITY)
+          for (int64_t k = 0; k < c->n; ++k) rho[k] = h[c->perm[k]];
+        else
+          rho = dget(c, d.rho[c->cur], (size_t)c->npad);
+        auto rec = dget(c, d.srec, (size_t)c->npad);
+        std::vector<double> vol((size_t)c->npad, 0.0);
+        for (int64_t k = 0; k < c->n; ++k) {
+          vol[k] = vm[k].w / rho[k];
+          rec[k].rho = rho[k];
+          rec[k].irho2 = 1.0 / (rho[k] * rho[k]);
+          if (field == WF_F_SIGMA) {
+            const double *m = h + 9 * (size_t)c->perm[k];
+            const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+            for (int q = 0; q < 6; ++q) rec[k].s[q] = m[3 * ia[q] + ib[q]];
+          }
+        }
+        dput(c, d.rho[c->cur], rho);
+        dput(c, d.vol[c->cur], vol);
+        dput(c, d.srec, rec);
+        break;
+      }
+      case WF_F_SHEAR: put_sym(c, d.S, h); break;
+      case WF_F_STRAIN_RATE: put_sym(c, d.sr, h); break;
+      case WF_F_ROT_RATE: {
+        std::vector<double> t((size_t)3 * c->npad, 0.0);
+        for (int64_t k = 0; k < c->n; ++k) {
+          const double *m = h + 9 * (size_t)c->perm[k];
+          t[k] = m[1];
+          t[(size_t)c->npad + k] = m[2];
+          t[(size_t)2 * c->npad + k] = m[5];
+        }
+        dput(c, d.rr, t);
+        break;
+      }
+      case WF_F_STRAIN: put_sym(c, d.strain, h); break;
+      case WF_F_STRAIN_PL: put_sym(c, d.epl, h); break;
+      case WF_F_STRAIN_PL_INCR: put_sym(c, d.eplinc, h); break;
+      case WF_F_PL_STRAIN: put_scalar(c, d.pl, h); break;
+      case WF_F_DPL_STRAIN: put_scalar(c, d.dpl, h); break;
+      case WF_F_SIGMAY: put_scalar(c, d.sigy, h); break;
+      case WF_F_EFF_SR: put_scalar(c, d.effsr, h); break;
+      case WF_F_PRESSURE: put_scalar(c, d.pres, h); break;
+      case WF_F_T: put_scalar(c, d.T, h); break;
+      case WF_F_DTDT: put_scalar(c, d.dTdt, h); break;
+      case WF_F_DDENSITY: put_scalar(c, d.drho, h); break;
+      case WF_F_Q_PLHEAT: put_scalar(c, d.qpl, h); break;
+      case WF_F_Q_CONV: put_scalar(c, d.qconv, h); break;
+      case WF_F_H_CONV: put_scalar(c, d.hconv, h); break;
+      case WF_F_T_INF: put_scalar(c, d.tinf, h); break;
+      case WF_F_Q_SOURCE: put_scalar(c, d.qsrc, h); break;
+      case WF_F_PS_ENERGY: put_scalar(c, d.psen, h); break;
+      default: throw Error(WF_EINVAL, "field not settable");
+    }
+    return WF_OK;
+  });
+}
+
+int wf_get_field(void *ctx, int32_t field, double *h) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (field < 0 || field >= WF_NFIELDS || !h) throw Error(WF_EINVAL, "bad field");
+    if (!c->inited) {
+      if (c->hf[field].empty()) throw Error(WF_ESTATE, "field not set");
+      std::memcpy(h, c->hf[field].data(), c->hf[field].size() * sizeof(double));
+      return WF_OK;
+    }
+    Dev &d = c->d;
+    switch (field) {
+      case WF_F_X: get_d4(c, d.xh[c->cur], h, 0, 3, 3); break;
+      case WF_F_H: get_d4(c, d.xh[c->cur], h, 3, 1, 1); break;
+      case WF_F_V: get_d4(c, d.vm, h, 0, 3, 3); break;
+      case WF_F_MASS: get_d4(c, d.vm, h, 3, 1, 1); break;
+      case WF_F_A: get_d4(c, d.acc, h, 0, 3, 3); break;
+      case WF_F_DISP: get_d4(c, d.disp, h, 0, 3, 3); break;
+      case WF_F_DENSITY: get_scalar(c, d.rho[c->cur], h); break;
+      case WF_F_SIGMA: {
+        auto rec = dget(c, d.srec, (size_t)c->npad);
+        for (int64_t k = 0; k < c->n; ++k) {
+          double *m = h + 9 * (size_t)c->perm[k];
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) m[3 * a + b] = rec[k].s[sym_index(a, b)];
+        }
+        break;
+      }
+      case WF_F_SHEAR: get_sym(c, d.S, h); break;
+      case WF_F_STRAIN_RATE: get_sym(c, d.sr, h); break;
+      case WF_F_ROT_RATE: {
+        auto t = dget(c, d.rr, (size_t)3 * c->npad);
+        for (int64_t k = 0; k < c->n; ++k) {
+          double *m = h + 9 * (size_t)c->perm[k];
+          const double w01 = t[k], w02 = t[(size_t)c->npad + k], w12 = t[(size_t)2 * c->npad + k];
+          const double f[9] = {0.0, w01, w02, -w01, 0.0, w12, -w02, -w12, 0.0};
+          std::memcpy(m, f, sizeof f);
+        }
+        break;
+      }
+      case WF_F_STRAIN: get_sym(c, d.strain, h); break;
+      case WF_F_STRAIN_PL: get_sym(c, d.epl, h); break;
+      case WF_F_STRAIN_PL_INCR: get_sym(c, d.eplinc, h); break;
+      case WF_F_PL_STRAIN: get_scalar(c, d.pl, h); break;
+      case WF_F_DPL_STRAIN: get_scalar(c, d.dpl, h); break;
+      case WF_F_SIGMAY: get_scalar(c, d.sigy, h); break;
+      case WF_F_EFF_SR: get_scalar(c, d.effsr, h); break;
+      case WF_F_PRESSURE: get_scalar(c, d.pres, h); break;
+      case WF_F_T: get_scalar(c, d.T, h); break;
+      case WF_F_DTDT: get_scalar(c, d.dTdt, h); break;
+      case WF_F_DDENSITY: get_scalar(c, d.drho, h); break;
+      case WF_F_Q_PLHEAT: get_scalar(c, d.qpl, h); break;
+      case WF_F_Q_CONV: get_scalar(c, d.qconv, h); break;
+      case WF_F_H_CONV: get_scalar(c, d.hconv, h); break;
+      case WF_F_T_INF: get_scalar(c, d.tinf, h); break;
+      case WF_F_Q_SOURCE: get_scalar(c, d.qsrc, h); break;
+      case WF_F_PS_ENERGY: get_scalar(c, d.psen, h); break;
+      default: throw Error(WF_EINVAL, "bad field");
+    }
+    return WF_OK;
+  });
+}
+
+int wf_set_int_field(void *ctx, int32_t field, const int32_t *h) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (field < 0 || field >= WF_NIFIELDS || field == WF_I_NB || !h) throw Error(WF_EINVAL, "bad int field");
+    if (c->inited) throw Error(WF_ESTATE, "int fields are fixed after wf_init");
+    if (field == WF_I_MATERIAL)
+      for (int64_t i = 0; i < c->n; ++i)
+        if (h[i] < 0 || h[i] >= kMaxMat) throw Error(WF_EINVAL, "material index out of range");
+    c->hi[field].assign(h, h + c->n);
+    return WF_OK;
+  });
+}
+
+int wf_get_int_field(void *ctx, int32_t field, int32_t *h) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (field < 0 || field >= WF_NIFIELDS || !h) throw Error(WF_EINVAL, "bad int field");
+    if (field == WF_I_NB) {
+      if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+      auto t = dget(c, c->d.nnb, (size_t)c->npad);
+      for (int64_t k = 0; k < c->n; ++k) h[c->perm[k]] = t[k];
+      return WF_OK;
+    }
+    std::memcpy(h, c->hi[field].data(), (size_t)c->n * sizeof(int32_t));
+    return WF_OK;
+  });
+}
+
+int wf_add_amplitude(void *ctx, int32_t id, int32_t npts, const double *t, const double *v) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (npts < 2 || !t || !v) throw Error(WF_EINVAL, "amplitude needs at least 2 points");
+    AmpHost a;
+    a.id = id;
+    a.t.assign(t, t + npts);
+    a.v.assign(v, v + npts);
+    c->amps.push_back(a);
+    return WF_OK;
+  });
+}
+
+int wf_add_velocity_bc(void *ctx, int32_t zone, int32_t value_type, const double value[3], int32_t amp_id,
+                       double amp_factor) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!value || (value_type != 0 && value_type != 1)) throw Error(WF_EINVAL, "bad velocity BC");
+    if ((int)c->bcs.size() >= kMaxBc) throw Error(WF_EUNSUPPORTED, "more than 8 boundary conditions");
+    BcHost b{};
+    b.zone = zone;
+    b.type = 0;
+    b.value_type = value_type;
+    b.amp_id = amp_id;
+    b.amp_factor = amp_factor;
+    for (int k = 0; k < 3; ++k) b.value[k] = value[k];
+    c->bcs.push_back(b);
+    if (std::find(c->rzones.begin(), c->rzones.end(), zone) == c->rzones.end()) {
+      if ((int)c->rzones.size() >= kMaxZones) throw Error(WF_EUNSUPPORTED, "more than 8 velocity-BC zones");
+      c->rzones.push_back(zone);
+    }
+    return WF_OK;
+  });
+}
+
+int wf_add_temperature_bc(void *ctx, int32_t zone, double T) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if ((int)c->bcs.size() >= kMaxBc) throw Error(WF_EUNSUPPORTED, "more than 8 boundary conditions");
+    BcHost b{};
+    b.zone = zone;
+    b.type = 2;
+    b.T = T;
+    c->bcs.push_back(b);
+    return WF_OK;
+  });
+}
+
+int wf_init(void *ctx, double dt) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_init already called");
+    if (!(dt > 0)) throw Error(WF_EINVAL, "dt must be positive");
+    for (int f : {WF_F_X, WF_F_DENSITY, WF_F_MASS, WF_F_H})
+      if (c->hf[f].empty()) throw Error(WF_ESTATE, "x, density, mass and h must be set before wf_init");
+    for (int64_t i = 0; i < c->n; ++i)
+      if (!c->mat_set[c->hi[WF_I_MATERIAL][i]]) throw Error(WF_ESTATE, "particle references an unset material");
+    if (c->cfg.thermal_solver)
+      for (int64_t i = 0; i < c->n; ++i)
+        if (!(c->mats[c->hi[WF_I_MATERIAL][i]].cp_T > 0)) throw Error(WF_EINVAL, "thermal solver needs cp_T > 0");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    cell_initiate(c);
+    make_permutation(c);
+    allocate(c);
+    upload_all(c);
+    c->dt = dt;
+    c->time = 0.0;
+    c->steps = 0;
+    c->cur = 0;
+    c->inited = true;
+    return WF_OK;
+  });
+}
+
+int wf_step(void *ctx, int32_t scheme, int64_t nsteps) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (scheme != WF_LEAPFROG && scheme != WF_KICKDRIFT) throw Error(WF_EINVAL, "unknown scheme");
+    for (int64_t s = 0; s < nsteps; ++s) one_step(c, scheme);
+    return WF_OK;
+  });
+}
+
+int wf_sync(void *ctx) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (c->inited) {
+      harvest_timing(c);
+      check_flags(c);
+    }
+    return WF_OK;
+  });
+}
+
+int wf_get_scalars(void *ctx, wf_scalars *out) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!out) throw Error(WF_EINVAL, "null argument");
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    launch_max_pl(c->d, c->st);
+    auto s = dget(c, c->d.scalars, S_COUNT);
+    auto f = dget(c, c->d.flags, 1)[0];
+    std::memset(out, 0, sizeof *out);
+    out->time = c->time;
+    out->dt = c->dt;
+    out->steps = c->steps;
+    out->nb_searches = f.nb_searches;
+    out->npairs = f.directed_pairs / 2;
+    out->max_nb_count = f.max_count;
+    out->kin_energy_sum = s[S_KIN];
+    out->int_energy_sum = s[S_INT];
+    out->plastic_work = s[S_PLW];
+    out->max_pl_strain = s[S_MAXPL];
+    out->max_T = s[S_MAXT];
+    out->min_T = s[S_MINT];
+    return WF_OK;
+  });
+}
+
+int wf_get_reaction_history(void *ctx, int32_t zone, double *out3, int64_t cap, int64_t *count) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    auto it = std::find(c->rzones.begin(), c->rzones.end(), zone);
+    if (it == c->rzones.end()) throw Error(WF_EINVAL, "zone has no velocity BC");
+    const int z = (int)(it - c->rzones.begin());
+    const long long hc = c->d.hist_cap;
+    long long avail = std::min<long long>(c->steps, hc);
+    long long m = std::min<long long>(avail, cap);
+    auto h = dget(c, c->d.react_hist, (size_t)hc * kMaxZones * 3);
+    const long long first = c->steps - m;
+    for (long long s = 0; s < m; ++s) {
+      const long long slot = (first + s) % hc;
+      for (int k = 0; k < 3; ++k) out3[3 * s + k] = h[(slot * kMaxZones + z) * 3 + k];
+    }
+    if (count) *count = m;
+    return WF_OK;
+  });
+}
+
+int wf_rebuild_neighbours(void *ctx) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    launch_schedule(c->d, c->st, 0, 1);
+    launch_neighbours(c->d, c->st, c->cur);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->built = true;
+    check_flags(c);
+    return WF_OK;
+  });
+}
+
+int wf_get_neighbour_pairs(void *ctx, int32_t *pairs2, int64_t cap, int64_t *count) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited || !c->built) throw Error(WF_ESTATE, "no neighbour list built yet");
+    auto nnb = dget(c, c->d.nnb, (size_t)c->npad);
+    auto nbr = dget(c, c->d.nbr, (size_t)c->cfg.max_nb * c->npad);
+    std::vector<std::pair<int32_t, int32_t>> p;
+    p.reserve((size_t)c->n * 32);
+    for (int64_t i = 0; i < c->n; ++i)
+      for (int k = 0; k < nnb[i]; ++k) {
+        const int j = nbr[(size_t)k * c->npad + i];
+        const int oi = c->perm[i], oj = c->perm[j];
+        if (oi < oj) p.push_back({oi, oj});
+      }
+    std::sort(p.begin(), p.end());
+    const int64_t m = std::min<int64_t>((int64_t)p.size(), cap);
+    for (int64_t k = 0; k < m; ++k) {
+      pairs2[2 * k] = p[k].first;
+      pairs2[2 * k + 1] = p[k].second;
+    }
+    if (count) *count = (int64_t)p.size();
+    return WF_OK;
+  });
+}
+
+int wf_eval(void *ctx, int32_t what, double dt) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (!c->built && what != WF_EVAL_STRESS) throw Error(WF_ESTATE, "call wf_rebuild_neighbours first");
+    switch (what) {
+      case WF_EVAL_ACCEL: launch_accel(c->d, c->st, c->cur); break;
+      case WF_EVAL_DENSITY: launch_dens_drift(c->d, c->st, c->cur, dt, 0); break;
+      case WF_EVAL_RATES: launch_rates_stress(c->d, c->st, c->cur, dt, 1, 0); break;
+      case WF_EVAL_STRESS: launch_rates_stress(c->d, c->st, c->cur, dt, 0, 1); break;
+      default: throw Error(WF_EINVAL, "unknown evaluation");
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return WF_OK;
+  });
+}
+
+int wf_set_timing(void *ctx, int32_t enable) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    harvest_timing(c);
+    c->timing = enable != 0;
+    for (int k = 0; k < T_COUNT; ++k) {
+      c->tsum[k] = 0.0;
+      c->tcnt[k] = 0;
+    }
+    return WF_OK;
+  });
+}
+
+int wf_get_kernel_time(void *ctx, int32_t kernel, double *avg_ms, int64_t *launches) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (kernel < 0 || kernel >= T_COUNT) throw Error(WF_EINVAL, "bad kernel id");
+    harvest_timing(c);
+    if (avg_ms) *avg_ms = c->tcnt[kernel] ? c->tsum[kernel] / c->tcnt[kernel] : 0.0;
+    if (launches) *launches = c->tcnt[kernel];
+    return WF_OK;
+  });
+}
+
+int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_count) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    auto f = dget(c, c->d.flags, 1)[0];
+    if (directed_pairs) *directed_pairs = f.directed_pairs;
+    if (max_count) *max_count = f.max_count;
+    return WF_OK;
+  });
+}
+
+// AddCylinderLength lattice, 3D without ghost rows (Source/Domain.cpp:674-860)
+static int half_part_count(double r, double R, int xinc) {
+  int ypartcount = -1;
+  if (xinc > 0) {
+    ypartcount = 1;
+    double yp = r;
+    double xp = r + (double)(xinc - 1) * 2. * r;
+    double rad = std::sqrt(yp * yp + xp * xp);
+    while (rad <= R - r) {
+      yp += 2. * r;
+      rad = std::sqrt(yp * yp + xp * xp);
+      ypartcount++;
+    }
+    ypartcount -= 1;
+  }
+  return ypartcount;
+}
+
+int64_t wf_gen_cylinder(const double V[3], double Rxy, double Lz, double r, double *out, int64_t cap) {
+#pragma clang fp contract(off)
+  if (!V || !(r > 0) || !(Rxy > 0) || !(Lz > 0)) return WF_EINVAL;
+  int64_t cnt = 0;
+  const int numpartxy = half_part_count(r, Rxy, 1);
+  double zp = V[2] + r;
+  while (zp <= (V[2] + Lz - r)) {
+    double yp = V[1] - r - (2. * r * (numpartxy - 1));
+    const int numypart = 2 * numpartxy;
+    int yinc = numpartxy, yinc_sign = -1;
+    for (int j = 0; j < numypart; j++) {
+      const int numxpart = half_part_count(r, Rxy, yinc);
+      double xp = V[0] - r - (2. * r * (numxpart - 1));
+      for (int i = 0; i < 2 * numxpart; i++) {
+        if (out && cnt < cap) {
+          out[3 * cnt] = xp;
+          out[3 * cnt + 1] = yp;
+          out[3 * cnt + 2] = zp;
+        }
+        cnt++;
+        xp += 2. * r;
+      }
+      yp += 2. * r;
+      yinc += yinc_sign;
+      if (yinc < 1) {
+        yinc = 1;
+        yinc_sign = 1;
+      }
+    }
+    zp += 2. * r;
+  }
+  return cnt;
+}
+
+}  // extern "C"

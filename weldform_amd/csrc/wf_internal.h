@@ -1,0 +1,148 @@
+// wf_internal.h -- device data layout and launcher declarations shared by
+// the HIP kernels (wf_kernels.hip) and the C-ABI layer (wf_capi.hip).
+//
+// HBM layout (internal particle order = Morton order of the initial cell
+// grid, fixed for the lifetime of a context):
+//   xh[2]   double4  (x, y, z, h)          positions, double-buffered
+//   vm      double4  (vx, vy, vz, mass)
+//   acc     double4  (ax, ay, az, 0)
+//   srec    StressRec (64 B) = (sigma [6], rho, 1/rho^2): the stress snapshot
+//           the acceleration gather reads for every neighbour
+//   rho[2], vol[2]   double (vol = mass/rho), double-buffered
+//   per-particle own state as component-major arrays (coalesced, never
+//   gathered): S[6], strain[6], epl[6], eplinc[6], sr[6], rr[3], scalars
+//   nbr     int32 [max_nb][npad] neighbour list, column-major (lane i of a
+//           wavefront reads entry k of particle i: one coalesced 256 B load)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/weldform_hip.h"
+
+namespace wf {
+
+constexpr int kBlock = 256;
+constexpr int kMaxBc = 8;        // velocity / temperature BC slots per context
+constexpr int kMaxZones = 8;     // velocity-BC zones with a reaction history
+constexpr int kMaxMat = 256;
+constexpr int kMaxNb = 128;
+
+// symmetric 3x3 as (xx, yy, zz, xy, xz, yz)
+enum { XX = 0, YY = 1, ZZ = 2, XY = 3, XZ = 4, YZ = 5 };
+
+struct __align__(64) StressRec {
+  double s[6];   // Cauchy stress Sigma (symmetric)
+  double rho;    // density the record was formed with
+  double irho2;  // 1 / (rho * rho)
+};
+
+struct __align__(32) StressRecF {
+  float s[6];
+  float rho;
+  float irho2;
+};
+
+struct DevMat {
+  int model;
+  int pad;
+  double E, G, Cs, ref_density, P0, alpha, beta, Ep;
+  double hol_K, hol_m, hol_sy0, hol_eps0, hol_eps1;
+  double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt;
+  double k_T, cp_T, th_exp;
+};
+
+// device-side solver state (one per context)
+struct DevFlags {
+  int do_search;     // neighbour search in this step
+  int yielded;       // sticky: some particle has pl_strain > 1e-6
+  int isyielding;    // reference's isyielding
+  int ts_i;          // reference's ts_i
+  int cleared;       // reference's m_isNbDataCleared
+  int step_yield;    // yielded value sampled at the top of the step
+  int overflow;      // a neighbour row was truncated
+  int nan;           // non-finite acceleration seen
+  int max_count;     // largest neighbour count of the last build
+  int ts_nb_inc;     // reference's ts_nb_inc (search period after yield)
+  long long nb_searches;
+  long long directed_pairs;
+};
+
+// partial-sum slots per block
+enum {
+  P_INT = 0,         // sum vol * S:SR       (internal energy rate)
+  P_KIN,             // sum of kinetic energy pair terms
+  P_PLW,             // sum vol * q_plheat   (plastic work rate)
+  P_MAXT, P_MINT,    // max / min temperature
+  P_REACT,           // + 3*zone + comp
+  P_COUNT = P_REACT + 3 * kMaxZones
+};
+
+enum { S_KIN = 0, S_INT, S_PLW, S_MAXT, S_MINT, S_MAXPL, S_COUNT };
+
+struct BcTable {
+  int n;
+  int zone[kMaxBc];
+  int type[kMaxBc];            // 0 velocity, 2 temperature
+  double v[kMaxBc][3];         // velocity value at the current time
+  double T[kMaxBc];
+  int nz;                      // reaction zones
+  int rzone[kMaxZones];
+};
+
+struct Dev {
+  int n;
+  long long npad;
+  int max_nb;
+  int ncells;
+  int cellno[3];
+  double blpf[3];
+  double cellsize[3];
+  double cellfac;
+  double pl_frac;
+  int thermal;
+  int pl_heating;
+  int mixed;
+  int nblocks;
+  // state
+  double4 *xh[2];
+  double4 *vm;
+  double4 *acc;
+  double4 *disp;
+  StressRec *srec;
+  double *rho[2];
+  double *vol[2];
+  double *S, *strain, *epl, *eplinc, *sr, *rr;   // [6|3][npad]
+  double *pl, *dpl, *sigy, *effsr, *pres, *drho;
+  double *T, *dTdt, *qpl, *psen, *qconv, *hconv, *tinf, *qsrc;
+  int *id, *mat, *group, *thbc, *perm;
+  // mixed-precision shadows of the gathered inputs
+  float4 *xhf[2];
+  float4 *vmf;
+  float *volf[2];
+  StressRecF *srecf;
+  // neighbours
+  int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *scan_tmp;
+  int *nnb, *nbr;
+  // materials, flags, reductions
+  const DevMat *mats;
+  DevFlags *flags;
+  double *partials;            // [nblocks][P_COUNT]
+  double *scalars;             // [S_COUNT]
+  double *react_hist;          // [hist_cap][kMaxZones][3]
+  long long hist_cap;
+};
+
+// launchers (wf_kernels.hip); all enqueue on `st`
+void launch_schedule(const Dev &d, hipStream_t st, int isfirst, int force);
+void launch_neighbours(const Dev &d, hipStream_t st, int cur);
+void launch_accel(const Dev &d, hipStream_t st, int cur);
+void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, int isfirst, double dt_prev,
+                 double dt);
+void launch_kick2(const Dev &d, hipStream_t st, double dt);
+void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update);
+void launch_rates_stress(const Dev &d, hipStream_t st, int buf, double dt, int do_rates, int do_stress);
+void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt);
+void launch_finalize(const Dev &d, hipStream_t st, double dt, long long step, int nz, int full_step);
+void launch_max_pl(const Dev &d, hipStream_t st);
+
+}  // namespace wf
