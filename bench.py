@@ -30,17 +30,40 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kind, n, nbar):
-    """Algorithmic HBM bytes per particle for one launch (FP64, SoA, each input
-    read once, each output written once, int32 neighbour list read once)."""
+def algorithmic_bytes(kind, n, nbar, mixed):
+    """Algorithmic HBM bytes of one launch: every particle's gathered inputs
+    read once (8 B per value in FP64, 4 B in mixed mode), its own state read
+    and written once in FP64, the int32 neighbour row (count + n_bar indices)
+    read once."""
+    w = 4.0 if mixed else 8.0
     lst = 4.0 * nbar + 4.0
-    if kind == "accel":          # x,h,v,m,rho,sigma in (120) ; a out (24)
-        return n * (120.0 + 24.0 + lst)
-    if kind == "density":        # x,h,v,m,rho in (72) ; rho,x,disp out (8+24+24) + m/rho (8)
-        return n * (72.0 + 64.0 + lst)
-    if kind == "rates":          # rates pass 72 in + 72 out, stress update ~370 B
-        return n * (72.0 + 72.0 + 370.0 + lst)
+    if kind == "accel":    # gathered x,h,v,m,rho,sigma(6) = 15 ; out a (3 FP64)
+        return n * (15 * w + 24.0 + lst)
+    if kind == "density":  # gathered x,h,v,m,m/rho = 9 ; own rho, disp in ; rho, m/rho, x, disp out
+        return n * (9 * w + 32.0 + 64.0 + (24.0 if mixed else 0.0) + lst)
+    if kind == "rates":    # gathered x,h,v,m,m/rho,rho = 10 ; stress update in/out (FP64):
+        # S, strain, strain_pl (3x6) + pl, Sy, T in; the same + sigma record,
+        # rates (9), eff. strain rate, pressure out
+        return n * (10 * w + 8.0 * (18 + 3) + 8.0 * (18 + 3 + 8 + 9 + 2) + (32.0 if mixed else 0.0) + lst)
     raise ValueError(kind)
+
+
+def timed_run(case, precision, steps, warmup):
+    from weldform_amd import cases
+
+    w = cases.make_solver(case, precision=precision)
+    w.step(warmup)
+    w.sync()
+    w.close()
+    s = cases.make_solver(case, precision=precision)
+    s.sync()
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    times = {k: s.kernel_time(k) for k in ("accel", "density", "rates", "neighbours", "step")}
+    return s, wall, times
 
 
 def cpu_baseline(case, steps):
@@ -64,8 +87,8 @@ def cpu_baseline(case, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -74,55 +97,75 @@ def main():
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
+
     from weldform_amd import capi, cases
 
-    case = cases.config2(steps=args.steps + args.warmup)
+    case = cases.config2(steps=args.steps)
     log(f"bench: {case.name} n={case.n} dt={case.dt:.4e} warmup={args.warmup} steps={args.steps}")
-    s = cases.make_solver(case, precision=capi.FP64)
-    s.step(args.warmup)
-    s.sync()
-    s.set_timing(True)
-    t0 = time.perf_counter()
-    s.step(args.steps)
-    s.sync()
-    wall = time.perf_counter() - t0
-    times = {k: s.kernel_time(k) for k in ("accel", "density", "rates", "neighbours", "step")}
-    pairs, maxnb = s.neighbour_stats()
-    sc = s.scalars()
+    runs = {}
+    for prec, name in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
+        s, wall, times = timed_run(case, prec, args.steps, args.warmup)
+        runs[name] = dict(solver=s, wall=wall, times=times, R=s.reaction_history(case.top_zone)[:, 2],
+                          disp=s.get("disp"), ps=case.n * args.steps / wall)
+        log(f"bench: {name}: {1e3 * wall / args.steps:.3f} ms/step  {runs[name]['ps']:.4e} particle-steps/s  "
+            "kernel avg ms " + ", ".join(f"{k}={v[0]:.4f}" for k, v in times.items()))
+    f64, mx = runs["fp64"], runs["mixed"]
+    pairs, maxnb = mx["solver"].neighbour_stats()
+    sc = mx["solver"].scalars()
     nbar = pairs / case.n
-    ps = case.n * args.steps / wall
-    log(f"bench: wall {wall:.3f} s  {1e3 * wall / args.steps:.3f} ms/step  {ps:.4e} particle-steps/s")
-    log("bench: kernel avg ms " + ", ".join(f"{k}={v[0]:.4f} ({v[1]})" for k, v in times.items()))
+
+    def nerr(a, b):
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+    err = {"mixed_vs_fp64_gpu_reaction_history": nerr(mx["R"], f64["R"]),
+           "mixed_vs_fp64_gpu_displacement": nerr(mx["disp"], f64["disp"]),
+           "mixed_vs_fp64_gpu_final_reaction": abs(mx["R"][-1] - f64["R"][-1]) / abs(f64["R"][-1])}
+    gold = os.path.join(ROOT, "tests", "golden", "config2_oracle.npz")
+    if os.path.exists(gold) and args.steps == 1000:
+        g = np.load(gold)
+        if int(g["n"]) == case.n and int(g["steps"]) == args.steps:
+            for name, r in runs.items():
+                err[f"{name}_vs_oracle_reaction_history"] = nerr(r["R"], g["R3"][:, 2])
+                err[f"{name}_vs_oracle_displacement_subsample"] = nerr(r["disp"][g["idx"]], g["disp"])
+    log("bench: errors " + json.dumps(err))
+    times = mx["times"]
     dom = max(("accel", "density", "rates"), key=lambda k: times[k][0])
     dom_ms = times[dom][0]
-    ach = algorithmic_bytes(dom, case.n, nbar) / (dom_ms * 1e-3) / 1e9
+    ach = algorithmic_bytes(dom, case.n, nbar, True) / (dom_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": ps,
+        "value": mx["ps"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * mx["wall"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 pair passes + f64 integration/return map/reductions",
         "data": "synthetic",
         "config": {
-            "workload": f"{case.name}: cylinder upsetting R=30mm L=60mm dx=0.5537mm, JC, v_top=-1m/s",
+            "workload": f"{case.name}: cylinder upsetting R=30mm L=60mm dx=0.5537mm (976,320 particles), "
+                        f"JC, v_top=-1m/s, {args.steps} Leapfrog steps from t=0",
             "particles": case.n,
-            "precision": "fp64",
+            "precision": "mixed",
+            "fp64_particle_steps_per_s": f64["ps"],
+            "fp64_ms_per_step": 1e3 * f64["wall"] / args.steps,
+            "mixed_speedup_vs_fp64": mx["ps"] / f64["ps"],
+            "errors": err,
             "pairs_unordered": pairs // 2,
             "mean_neighbours": nbar,
             "max_neighbours": maxnb,
-            "pair_evaluations_per_s": (pairs // 2) * args.steps / wall,
-            "kernel_ms": {k: v[0] for k, v in times.items()},
+            "pair_evaluations_per_s": (pairs // 2) * args.steps / mx["wall"],
+            "kernel_ms_mixed": {k: v[0] for k, v in times.items()},
+            "kernel_ms_fp64": {k: v[0] for k, v in f64["times"].items()},
             "nb_searches": sc["nb_searches"],
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": dom,
+            "kernel": dom + " (mixed)",
             "achieved": ach,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

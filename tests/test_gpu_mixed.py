@@ -1,0 +1,81 @@
+"""Mixed precision (FP32 pair arithmetic, FP64 integration, return map and
+reductions): errors stated against the FP64 oracle.
+
+Tolerances: single pair passes within 2e-5 normwise of the FP64 oracle on the
+same inputs; reaction-force history within 1e-4 normwise after 1000 steps
+(the target for the mixed step)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle_case import normwise_err, oracle_from_case
+from weldform_amd import capi, cases
+
+pytestmark = pytest.mark.gpu
+
+TOL_PASS_MIXED = 2e-5
+TOL_RUN_MIXED = 1e-4
+
+
+def _log(*a):
+    print(*a, flush=True)
+
+
+def test_mixed_single_passes():
+    c = cases.config1()
+    rng = np.random.default_rng(3)
+    v = rng.normal(0.0, 1.0, size=(c.n, 3))
+    s = rng.normal(0.0, 1e8, size=(c.n, 3, 3))
+    sig = 0.5 * (s + np.transpose(s, (0, 2, 1)))
+    g = cases.make_solver(c, precision=capi.MIXED)
+    g.set("v", v)
+    g.set("sigma", sig)
+    g.rebuild_neighbours()
+    d = oracle_from_case(c)
+    d.f("V")[:] = v
+    d.f("SIGMA")[:] = sig
+    d.neighbour_search()
+    d.start_acceleration()
+    g.eval("accel")
+    d.calc_accel()
+    e_a = normwise_err(g.get("a"), d.f("A"))
+    g.eval("density")
+    d.calc_dens()
+    e_d = normwise_err(g.get("ddensity"), d.f("DRHO"))
+    g.eval("rates")
+    d.calc_rates()
+    e_s = normwise_err(g.get("strain_rate"), d.f("SR"))
+    e_w = normwise_err(g.get("rot_rate"), d.f("RR"))
+    _log("mixed pass errors accel", e_a, "density", e_d, "strain rate", e_s, "rotation", e_w)
+    assert max(e_a, e_d, e_s, e_w) < TOL_PASS_MIXED
+
+
+def test_mixed_config1_run():
+    c = cases.config1()
+    g = cases.make_solver(c, precision=capi.MIXED)
+    g.step(c.steps)
+    g.sync()
+    d = oracle_from_case(c, threads=4)
+    d.solve_init(c.dt)
+    R = []
+    for _ in range(c.steps):
+        d.step()
+        R.append(d.reaction(c.top_zone)[2])
+    e_r = normwise_err(g.reaction_history(c.top_zone)[:, 2], np.array(R))
+    e_u = normwise_err(g.get("disp"), d.f("DISP"))
+    _log("mixed config1 reaction err", e_r, "disp err", e_u)
+    assert e_r < TOL_RUN_MIXED and e_u < TOL_RUN_MIXED
+
+
+def test_mixed_config2_vs_oracle_golden():
+    gz = np.load(os.path.join(os.path.dirname(__file__), "golden", "config2_oracle.npz"))
+    c = cases.config2(steps=int(gz["steps"]))
+    g = cases.make_solver(c, precision=capi.MIXED)
+    g.step(c.steps)
+    g.sync()
+    e_r = normwise_err(g.reaction_history(c.top_zone)[:, 2], gz["R3"][:, 2])
+    e_f = abs(g.reaction_history(c.top_zone)[-1, 2] - gz["R3"][-1, 2]) / abs(gz["R3"][-1, 2])
+    e_u = normwise_err(g.get("disp")[gz["idx"]], gz["disp"])
+    _log("mixed config2 reaction history err", e_r, "final reaction err", e_f, "disp err", e_u)
+    assert e_r < TOL_RUN_MIXED and e_f < TOL_RUN_MIXED and e_u < TOL_RUN_MIXED

@@ -257,3 +257,29 @@ def test_errors_reported(c1):
     with pytest.raises(capi.WFError) as e:
         s2.rebuild_neighbours()
     assert e.value.code == capi.ENBOVERFLOW
+
+
+def test_config2_full_size_vs_oracle_golden():
+    """976,320 particles, 1000 FP64 steps against the oracle's own run of the
+    same case (tests/golden/make_config2_golden.py): reaction history and the
+    displacement / density of a fixed particle subsample within 1e-8."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "config2_oracle.npz"))
+    c = cases.config2(steps=int(g["steps"]))
+    assert c.n == int(g["n"]) and c.dt == float(g["dt"])
+    _log("case", c.name, "n", c.n, "steps", c.steps)
+    s = cases.make_solver(c)
+    s.step(c.steps)
+    s.sync()
+    R = s.reaction_history(c.top_zone)
+    e_r = normwise_err(R[:, 2], g["R3"][:, 2])
+    e_r2 = normwise_err(s.reaction_history(2)[:, 2], g["R2"][:, 2])
+    disp = s.get("disp")
+    e_u = normwise_err(disp[g["idx"]], g["disp"])
+    e_rho = normwise_err(s.get("density")[g["idx"]] - 2700.0, g["density"] - 2700.0)
+    sc = s.scalars()
+    _log("reaction err", e_r, e_r2, "disp err", e_u, "rho err", e_rho, "pairs", sc["npairs"], int(g["npairs"]))
+    assert sc["npairs"] == int(g["npairs"])
+    assert sc["nb_searches"] == int(g["nb_searches"][-1])
+    assert e_r < TOL_RUN and e_r2 < TOL_RUN and e_u < TOL_RUN and e_rho < 1e-6
+    assert abs(sc["int_energy_sum"] - float(g["int_energy_sum"])) <= 1e-8 * abs(float(g["int_energy_sum"]))

@@ -211,6 +211,8 @@ void cell_initiate(Ctx *c) {
     }
     if (h[i] > hmax) hmax = h[i];
   }
+  // mixed-precision shadows store positions relative to the body's centre
+  for (int k = 0; k < 3; ++k) c->d.xc[k] = 0.5 * (TRPR[k] + BLPF[k]);
   for (int k = 0; k < 3; ++k) {
     if (c->cfg.dom_max[k] > TRPR[k]) TRPR[k] = c->cfg.dom_max[k];
     if (c->cfg.dom_min[k] < BLPF[k]) BLPF[k] = c->cfg.dom_min[k];
@@ -350,6 +352,14 @@ void allocate(Ctx *c) {
   d.scalars = c->alloc<double>(S_COUNT);
   d.hist_cap = 1 << 16;
   d.react_hist = c->alloc<double>((size_t)d.hist_cap * kMaxZones * 3);
+  if (d.mixed) {
+    for (int b = 0; b < 2; ++b) {
+      d.xhf[b] = c->alloc<float4>(np);
+      d.vrf[b] = c->alloc<float2>(np);
+    }
+    d.vmf = c->alloc<float4>(np);
+    d.srecf = c->alloc<StressRecF>(np);
+  }
 }
 
 void upload_all(Ctx *c) {
@@ -586,7 +596,6 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
     if (n <= 0 || n > (1LL << 31) - 1024) throw Error(WF_EINVAL, "particle count out of range");
     if (cfg->max_nb < 1 || cfg->max_nb > kMaxNb) throw Error(WF_EINVAL, "max_nb must be in 1..128");
     if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED) throw Error(WF_EINVAL, "bad precision");
-    if (cfg->precision == WF_MIXED) throw Error(WF_EUNSUPPORTED, "mixed precision is not available in this build");
     if (!(cfg->cellfac > 0)) throw Error(WF_EINVAL, "cellfac must be positive");
     if (cfg->ts_nb_inc < 1) throw Error(WF_EINVAL, "ts_nb_inc must be >= 1");
     Ctx *c = new Ctx();
@@ -728,6 +737,9 @@ int wf_set_field(void *ctx, int32_t field, const double *h) {
       case WF_F_PS_ENERGY: put_scalar(c, d.psen, h); break;
       default: throw Error(WF_EINVAL, "field not settable");
     }
+    launch_shadows(d, c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
     return WF_OK;
   });
 }
@@ -894,6 +906,9 @@ int wf_init(void *ctx, double dt) {
     make_permutation(c);
     allocate(c);
     upload_all(c);
+    launch_shadows(c->d, c->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
     c->dt = dt;
     c->time = 0.0;
     c->steps = 0;

@@ -115,10 +115,12 @@ struct Dev {
   double *pl, *dpl, *sigy, *effsr, *pres, *drho;
   double *T, *dTdt, *qpl, *psen, *qconv, *hconv, *tinf, *qsrc;
   int *id, *mat, *group, *thbc, *perm;
-  // mixed-precision shadows of the gathered inputs
-  float4 *xhf[2];
-  float4 *vmf;
-  float *volf[2];
+  // mixed-precision shadows of the gathered inputs (written by the FP64
+  // integrators); positions are stored relative to xc to halve their error
+  double xc[3];
+  float4 *xhf[2];              // (x - xc, y - yc, z - zc, h)
+  float4 *vmf;                 // (vx, vy, vz, mass)
+  float2 *vrf[2];              // (mass/rho, rho)
   StressRecF *srecf;
   // neighbours
   int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *scan_tmp;
@@ -144,5 +146,6 @@ void launch_rates_stress(const Dev &d, hipStream_t st, int buf, double dt, int d
 void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt);
 void launch_finalize(const Dev &d, hipStream_t st, double dt, long long step, int nz, int full_step);
 void launch_max_pl(const Dev &d, hipStream_t st);
+void launch_shadows(const Dev &d, hipStream_t st);
 
 }  // namespace wf

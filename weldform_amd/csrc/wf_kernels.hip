@@ -19,6 +19,29 @@ namespace wf {
 
 static inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// grid of the neighbour-build kernels: they run every step but exit at once
+// unless a search is due, so they use a modest grid with grid-stride loops
+static inline int nblk_small(long long n) {
+  const int b = nblk(n);
+  return b < 1024 ? b : 1024;
+}
+
+// XCD-aware tile index.  The dispatcher hands consecutive workgroups to the
+// 8 XCDs in turn and each XCD has its own L2; remapping makes XCD x walk one
+// contiguous run of tiles, so a particle and its (Morton-adjacent)
+// neighbours are fetched into the same L2.  Bijective for any grid size.
+constexpr int kXcd = 8;
+__device__ inline int xcd_tile(int b, int nb) {
+  const int q = nb / kXcd, r = nb % kXcd;
+  const int x = b % kXcd, k = b / kXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// block partial of reduction slot `slot` (slot-major: coalesced in finalize)
+__device__ inline double &part(const Dev &d, int slot, int block) {
+  return d.partials[(long long)slot * d.nblocks + block];
+}
+
 // ---------------------------------------------------------------------------
 // reductions
 
@@ -55,14 +78,125 @@ __device__ inline double block_reduce(double v, double *sh) {
 // ---------------------------------------------------------------------------
 // kernel functions (Source/Functions.cpp:74-87) and the neighbour predicate
 
-// cubic spline gradient factor: GK * x_ij = grad_i W_ij (3D)
-__device__ inline double grad_kernel(double q, double h) {
-  const double C = 1.0 / (h * h * h * h * M_PI);
-  if (q == 0.0) return C / h * (-3.0);
-  if (q < 1.0) return C / (q * h) * (-3.0 * q + 2.25 * q * q);
-  if (q < 2.0) return C / (q * h) * (-0.75 * (2.0 - q) * (2.0 - q));
+// cubic spline gradient factor GK with GK * x_ij = grad_i W_ij (3D),
+// Source/Functions.cpp:74-87:  GK = C/(q h) * P(q), C = 1/(pi h^4), so for
+// q < 1: GK = c5 (-3 + 9/4 q) and for 1 <= q < 2: GK = c5 (-3/4 (2-q)^2) / q
+// with c5 = 1/(pi h^5).  Reciprocals use the hardware estimate refined by two
+// Newton steps (a few ulp from the correctly rounded quotient).
+__device__ inline double rcp_d(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ inline double rsqrt_d(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+struct KernH {
+  double h, ih, c5;  // pair smoothing length, 1/h, 1/(pi h^5)
+};
+__device__ inline KernH kern_h(double h) {
+  KernH k;
+  k.h = h;
+  k.ih = rcp_d(h);
+  const double ih2 = k.ih * k.ih;
+  k.c5 = ih2 * ih2 * k.ih * (1.0 / M_PI);
+  return k;
+}
+// h of the pair (h_i + h_j)/2; reuses particle i's constants when h_j == h_i
+__device__ inline KernH pair_h(const KernH &ki, double hi, double hj) {
+  if (hj == hi) return ki;
+  return kern_h((hi + hj) / 2);
+}
+__device__ inline double grad_kernel_r2(double r2, const KernH &k) {
+  if (r2 == 0.0) return -3.0 * k.c5;
+  const double ri = rsqrt_d(r2);
+  const double q = r2 * ri * k.ih;
+  if (q < 1.0) return k.c5 * (-3.0 + 2.25 * q);
+  if (q < 2.0) {
+    const double t = 2.0 - q;
+    return k.c5 * (-0.75 * t * t) * (k.h * ri);
+  }
   return 0.0;
 }
+
+// the same kernel math for either pair precision R (double or float):
+// float uses the 1-ulp hardware reciprocal / reciprocal square root
+__device__ inline double rcp_t(double x) { return rcp_d(x); }
+__device__ inline float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline double rsqrt_t(double x) { return rsqrt_d(x); }
+__device__ inline float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
+
+template <typename R>
+struct KH {
+  R h, ih, c5;
+};
+template <typename R>
+__device__ inline KH<R> kern_h_t(R h) {
+  KH<R> k;
+  k.h = h;
+  k.ih = rcp_t(h);
+  const R ih2 = k.ih * k.ih;
+  k.c5 = ih2 * ih2 * k.ih * (R)(1.0 / M_PI);
+  return k;
+}
+template <typename R>
+__device__ inline KH<R> pair_h_t(const KH<R> &ki, R hi, R hj) {
+  if (hj == hi) return ki;
+  return kern_h_t<R>((hi + hj) / (R)2);
+}
+template <typename R>
+__device__ inline R gk_t(R r2, const KH<R> &k) {
+  if (r2 == (R)0) return (R)-3 * k.c5;
+  const R ri = rsqrt_t(r2);
+  const R q = r2 * ri * k.ih;
+  if (q < (R)1) return k.c5 * ((R)-3 + (R)2.25 * q);
+  if (q < (R)2) {
+    const R t = (R)2 - q;
+    return k.c5 * ((R)-0.75 * t * t) * (k.h * ri);
+  }
+  return (R)0;
+}
+
+// what a gather reads of particle j, from the FP64 state or its FP32 shadows
+template <bool MX>
+struct Src;
+template <>
+struct Src<false> {
+  using R = double;
+  using R4 = double4;
+  __device__ static R4 x(const Dev &d, int b, int j) { return d.xh[b][j]; }
+  __device__ static R4 v(const Dev &d, int j) { return d.vm[j]; }
+  __device__ static R vol(const Dev &d, int b, int j) { return d.vol[b][j]; }
+  __device__ static R rho(const Dev &d, int b, int j) { return d.rho[b][j]; }
+  __device__ static void srec(const Dev &d, int j, R s[6], R &rho, R &irho2) {
+    const StressRec r = d.srec[j];
+    for (int k = 0; k < 6; ++k) s[k] = r.s[k];
+    rho = r.rho;
+    irho2 = r.irho2;
+  }
+};
+template <>
+struct Src<true> {
+  using R = float;
+  using R4 = float4;
+  __device__ static R4 x(const Dev &d, int b, int j) { return d.xhf[b][j]; }
+  __device__ static R4 v(const Dev &d, int j) { return d.vmf[j]; }
+  __device__ static R vol(const Dev &d, int b, int j) { return d.vrf[b][j].x; }
+  __device__ static R rho(const Dev &d, int b, int j) { return d.vrf[b][j].y; }
+  __device__ static void srec(const Dev &d, int j, R s[6], R &rho, R &irho2) {
+    const StressRecF r = d.srecf[j];
+    for (int k = 0; k < 6; ++k) s[k] = r.s[k];
+    rho = r.rho;
+    irho2 = r.irho2;
+  }
+};
 
 // CheckRadius (Source/Neighbour.cpp:361-372), evaluated without contraction
 // so it rounds exactly as the host predicate does.
@@ -115,23 +249,19 @@ WF_KERNEL void k_schedule(Dev d, int isfirst, int force) {
 // neighbour build: counting sort of particles by reference cell, then one
 // lane per particle scans the 27 surrounding cells with the exact predicate.
 
-WF_KERNEL void k_zero_int(Dev d, int *p, int n) {
-  if (!d.flags->do_search) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
+// cell_count and cell_cursor are zero on entry (allocation, then cleared by
+// k_cell_sort once a search has consumed them)
 WF_KERNEL void k_cell_hist(Dev d, int cur) {
   if (!d.flags->do_search) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= d.n) return;
-  const double4 x = d.xh[cur][i];
-  const int cx = cell_coord(x.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
-  const int cy = cell_coord(x.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
-  const int cz = cell_coord(x.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
-  const int c = (cx * d.cellno[1] + cy) * d.cellno[2] + cz;
-  d.cell[i] = c;
-  atomicAdd(&d.cell_count[c], 1);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+    const double4 x = d.xh[cur][i];
+    const int cx = cell_coord(x.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
+    const int cy = cell_coord(x.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
+    const int cz = cell_coord(x.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
+    const int c = (cx * d.cellno[1] + cy) * d.cellno[2] + cz;
+    d.cell[i] = c;
+    atomicAdd(&d.cell_count[c], 1);
+  }
 }
 
 // inclusive scan of one value per lane across a wavefront
@@ -197,19 +327,20 @@ __global__ __launch_bounds__(1024) void k_scan3(Dev d) {
 
 WF_KERNEL void k_scatter(Dev d) {
   if (!d.flags->do_search) return;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= d.n) return;
-  const int c = d.cell[i];
-  const int pos = d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1);
-  d.sorted[pos] = i;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+    const int c = d.cell[i];
+    const int pos = d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1);
+    d.sorted[pos] = i;
+  }
 }
 
 // cells hold ~10-30 particles: an insertion sort makes the within-cell
 // order (and with it every neighbour row) independent of atomic order
 WF_KERNEL void k_cell_sort(Dev d) {
   if (!d.flags->do_search) return;
-  int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= d.ncells) return;
+  for (int c = blockIdx.x * kBlock + threadIdx.x; c < d.ncells; c += gridDim.x * kBlock) {
+  d.cell_count[c] = 0;
+  d.cell_cursor[c] = 0;
   const int s = d.cell_start[c], e = d.cell_start[c + 1];
   for (int a = s + 1; a < e; ++a) {
     const int v = d.sorted[a];
@@ -220,14 +351,15 @@ WF_KERNEL void k_cell_sort(Dev d) {
     }
     d.sorted[b + 1] = v;
   }
+  }
 }
 
 WF_KERNEL void k_build_nbr(Dev d, int cur) {
   if (!d.flags->do_search) return;
   __shared__ double sh[kBlock / 64];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  int cnt = 0;
-  if (i < d.n) {
+  double tot = 0.0, mx = 0.0;
+  for (int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+    int cnt = 0;
     const int g = d.group[i];
     if (g >= 0) {
       const double4 xi = d.xh[cur][i];
@@ -238,30 +370,31 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
         if (ax < 0 || ax >= nx) continue;
         for (int ay = cy - 1; ay <= cy + 1; ++ay) {
           if (ay < 0 || ay >= ny) continue;
-          for (int az = cz - 1; az <= cz + 1; ++az) {
-            if (az < 0 || az >= nz) continue;
-            const int cc = (ax * ny + ay) * nz + az;
-            const int s = d.cell_start[cc], e = d.cell_start[cc + 1];
-            for (int p = s; p < e; ++p) {
-              const int j = d.sorted[p];
-              if (j == i || d.group[j] != g) continue;
-              if (!in_radius(xi, d.xh[cur][j], d.cellfac)) continue;
-              if (cnt < d.max_nb) d.nbr[(long long)cnt * d.npad + i] = j;
-              ++cnt;
-            }
+          // cells (ax, ay, cz-1..cz+1) are consecutive in the cell order
+          const int c0 = (ax * ny + ay) * nz;
+          const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < nz ? cz + 1 : nz - 1;
+          const int s0 = d.cell_start[c0 + zlo], e0 = d.cell_start[c0 + zhi + 1];
+          for (int p = s0; p < e0; ++p) {
+            const int j = d.sorted[p];
+            if (j == i || d.group[j] != g) continue;
+            if (!in_radius(xi, d.xh[cur][j], d.cellfac)) continue;
+            if (cnt < d.max_nb) d.nbr[(long long)cnt * d.npad + i] = j;
+            ++cnt;
           }
         }
       }
     }
     d.nnb[i] = cnt < d.max_nb ? cnt : d.max_nb;
     if (cnt > d.max_nb) d.flags->overflow = 1;
+    tot += cnt;
+    mx = cnt > mx ? cnt : mx;
   }
   // deterministic totals via block reduction + integer atomics
-  const double tot = block_reduce<0>((double)cnt, sh);
-  const double mx = block_reduce<1>((double)cnt, sh);
+  const double bt = block_reduce<0>(tot, sh);
+  const double bm = block_reduce<1>(mx, sh);
   if (threadIdx.x == 0) {
-    atomicAdd((unsigned long long *)&d.flags->directed_pairs, (unsigned long long)tot);
-    atomicMax(&d.flags->max_count, (int)mx);
+    atomicAdd((unsigned long long *)&d.flags->directed_pairs, (unsigned long long)bt);
+    atomicMax(&d.flags->max_count, (int)bm);
   }
 }
 
@@ -270,72 +403,78 @@ void launch_schedule(const Dev &d, hipStream_t st, int isfirst, int force) {
 }
 
 void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
-  const int nbc = nblk(d.ncells);
-  k_zero_int<<<nbc, kBlock, 0, st>>>(d, d.cell_count, d.ncells);
-  k_zero_int<<<nbc, kBlock, 0, st>>>(d, d.cell_cursor, d.ncells);
-  k_cell_hist<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  k_cell_hist<<<nblk_small(d.n), kBlock, 0, st>>>(d, cur);
   const int ns = (d.ncells + 1023) / 1024;
   k_scan1<<<ns, 1024, 0, st>>>(d);
   k_scan2<<<1, 1024, 0, st>>>(d, ns);
   k_scan3<<<ns, 1024, 0, st>>>(d);
-  k_scatter<<<nblk(d.n), kBlock, 0, st>>>(d);
-  k_cell_sort<<<nbc, kBlock, 0, st>>>(d);
-  k_build_nbr<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  k_scatter<<<nblk_small(d.n), kBlock, 0, st>>>(d);
+  k_cell_sort<<<nblk_small(d.ncells), kBlock, 0, st>>>(d);
+  k_build_nbr<<<nblk_small(d.n) * 2, kBlock, 0, st>>>(d, cur);
 }
 
 // ---------------------------------------------------------------------------
 // pass 1: acceleration (CalcAccel + AccelReduction, GradientType 0, TI = 0)
 
+template <bool MX>
 WF_KERNEL void k_accel(Dev d, int cur) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  using S = Src<MX>;
+  using R = typename S::R;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= d.n) return;
-  double ax = 0.0, ay = 0.0, az = 0.0;
+  R ax = 0, ay = 0, az = 0;
   if (d.group[i] >= 0) {
     const DevMat &M = d.mats[d.mat[i]];
-    const double alpha = M.alpha, beta = M.beta, cs = M.Cs;
-    const bool av = (alpha != 0.0 || beta != 0.0);
-    const double4 xi = d.xh[cur][i];
-    const double4 vi = d.vm[i];
-    const StressRec ri = d.srec[i];
-    const double sxx = ri.s[XX] * ri.irho2, syy = ri.s[YY] * ri.irho2, szz = ri.s[ZZ] * ri.irho2;
-    const double sxy = ri.s[XY] * ri.irho2, sxz = ri.s[XZ] * ri.irho2, syz = ri.s[YZ] * ri.irho2;
+    const R alpha = (R)M.alpha, beta = (R)M.beta, cs = (R)M.Cs;
+    const bool av = (M.alpha != 0.0 || M.beta != 0.0);
+    const auto xi = S::x(d, cur, i);
+    const auto vi = S::v(d, i);
+    R si[6], rhoi, irhoi;
+    S::srec(d, i, si, rhoi, irhoi);
+    R ti[6];
+    for (int k = 0; k < 6; ++k) ti[k] = si[k] * irhoi;
+    const KH<R> ki = kern_h_t<R>(xi.w);
     const int cnt = d.nnb[i];
     const int *nb = d.nbr + i;
     for (int k = 0; k < cnt; ++k) {
       const int j = nb[(long long)k * d.npad];
-      const double4 xj = d.xh[cur][j];
-      const double4 vj = d.vm[j];
-      const StressRec rj = d.srec[j];
-      const double h = (xi.w + xj.w) / 2;
-      const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      const double GK = grad_kernel(r / h, h);
-      double pi = 0.0;
+      const auto xj = S::x(d, cur, j);
+      const auto vj = S::v(d, j);
+      R sj[6], rhoj, irhoj;
+      S::srec(d, j, sj, rhoj, irhoj);
+      const KH<R> kh = pair_h_t<R>(ki, xi.w, xj.w);
+      const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const R r2 = dx * dx + dy * dy + dz * dz;
+      const R GK = gk_t<R>(r2, kh);
+      R pi = 0;
       if (av) {
-        const double vdot = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-        if (vdot < 0.0) {
-          const double mu = h * vdot / (r * r + 0.01 * h * h);
-          pi = (alpha * cs * mu + beta * mu * mu) / (0.5 * (ri.rho + rj.rho));
+        const R vdot = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+        if (vdot < (R)0) {
+          const R mu = kh.h * vdot * rcp_t(r2 + (R)0.01 * kh.h * kh.h);
+          pi = (alpha * cs * mu + beta * mu * mu) * rcp_t((R)0.5 * (rhoi + rhoj));
         }
       }
-      const double Mxx = sxx + rj.s[XX] * rj.irho2 + pi;
-      const double Myy = syy + rj.s[YY] * rj.irho2 + pi;
-      const double Mzz = szz + rj.s[ZZ] * rj.irho2 + pi;
-      const double Mxy = sxy + rj.s[XY] * rj.irho2;
-      const double Mxz = sxz + rj.s[XZ] * rj.irho2;
-      const double Myz = syz + rj.s[YZ] * rj.irho2;
-      const double gx = GK * dx, gy = GK * dy, gz = GK * dz;
-      const double mj = vj.w;
+      const R Mxx = ti[XX] + sj[XX] * irhoj + pi;
+      const R Myy = ti[YY] + sj[YY] * irhoj + pi;
+      const R Mzz = ti[ZZ] + sj[ZZ] * irhoj + pi;
+      const R Mxy = ti[XY] + sj[XY] * irhoj;
+      const R Mxz = ti[XZ] + sj[XZ] * irhoj;
+      const R Myz = ti[YZ] + sj[YZ] * irhoj;
+      const R gx = GK * dx, gy = GK * dy, gz = GK * dz;
+      const R mj = vj.w;
       ax += mj * (gx * Mxx + gy * Mxy + gz * Mxz);
       ay += mj * (gx * Mxy + gy * Myy + gz * Myz);
       az += mj * (gx * Mxz + gy * Myz + gz * Mzz);
     }
   }
-  d.acc[i] = make_double4(ax, ay, az, 0.0);
+  d.acc[i] = make_double4((double)ax, (double)ay, (double)az, 0.0);
 }
 
 void launch_accel(const Dev &d, hipStream_t st, int cur) {
-  k_accel<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  if (d.mixed)
+    k_accel<true><<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  else
+    k_accel<false><<<nblk(d.n), kBlock, 0, st>>>(d, cur);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,6 +521,7 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, int isfirst, double dt_prev
       }
     }
     d.vm[i] = make_double4(v[0], v[1], v[2], vm.w);
+    if (d.mixed) d.vmf[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)vm.w);
     d.acc[i] = make_double4(av[0], av[1], av[2], 0.0);
   }
   for (int z = 0; z < bc.nz; ++z) {
@@ -390,10 +530,9 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, int isfirst, double dt_prev
     const double fy = block_reduce<0>(in ? m * a.y : 0.0, sh);
     const double fz = block_reduce<0>(in ? m * a.z : 0.0, sh);
     if (threadIdx.x == 0) {
-      double *p = d.partials + (long long)blockIdx.x * P_COUNT + P_REACT + 3 * z;
-      p[0] = fx;
-      p[1] = fy;
-      p[2] = fz;
+      part(d, P_REACT + 3 * z, blockIdx.x) = fx;
+      part(d, P_REACT + 3 * z + 1, blockIdx.x) = fy;
+      part(d, P_REACT + 3 * z + 2, blockIdx.x) = fz;
     }
   }
 }
@@ -408,6 +547,7 @@ WF_KERNEL void k_kick2(Dev d, double dt) {
   v.y += a.y * dt / 2.;
   v.z += a.z * dt / 2.;
   d.vm[i] = v;
+  if (d.mixed) d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
 }
 
 void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, int isfirst, double dt_prev,
@@ -423,47 +563,60 @@ void launch_kick2(const Dev &d, hipStream_t st, double dt) { k_kick2<<<nblk(d.n)
 //  Source/SolverLeapfrog.cpp:280-301).  Reads x_n, v_{n+1/2}, m/rho_n and
 // writes rho_{n+1}, m/rho_{n+1}, x_{n+1} into the other buffer.
 
+template <bool MX>
 WF_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  using S = Src<MX>;
+  using R = typename S::R;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= d.n) return;
-  const double4 xi = d.xh[cur][i];
-  const double4 vi = d.vm[i];
-  double s = 0.0;
+  R s = 0;
   if (d.group[i] >= 0) {
+    const auto xi = S::x(d, cur, i);
+    const auto vi = S::v(d, i);
+    const KH<R> ki = kern_h_t<R>(xi.w);
     const int cnt = d.nnb[i];
     const int *nb = d.nbr + i;
     for (int k = 0; k < cnt; ++k) {
       const int j = nb[(long long)k * d.npad];
-      const double4 xj = d.xh[cur][j];
-      const double4 vj = d.vm[j];
-      const double volj = d.vol[cur][j];
-      const double h = (xi.w + xj.w) / 2;
-      const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      const double GK = grad_kernel(r / h, h);
-      const double t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
+      const auto xj = S::x(d, cur, j);
+      const auto vj = S::v(d, j);
+      const R volj = S::vol(d, cur, j);
+      const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, pair_h_t<R>(ki, xi.w, xj.w));
+      const R t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
       s += volj * t;
     }
   }
   const double rhoi = d.rho[cur][i];
-  const double drho = s * rhoi;
+  const double drho = (double)s * rhoi;
   d.drho[i] = drho;
   if (!do_update) return;
   const int nxt = cur ^ 1;
+  const double4 xd = d.xh[cur][i];
+  const double4 vd = d.vm[i];
   const double rn = rhoi + dt * drho;
+  const double vn = vd.w / rn;
   d.rho[nxt][i] = rn;
-  d.vol[nxt][i] = vi.w / rn;
-  const double ux = vi.x * dt, uy = vi.y * dt, uz = vi.z * dt;
+  d.vol[nxt][i] = vn;
+  const double ux = vd.x * dt, uy = vd.y * dt, uz = vd.z * dt;
   double4 u = d.disp[i];
   u.x += ux;
   u.y += uy;
   u.z += uz;
   d.disp[i] = u;
-  d.xh[nxt][i] = make_double4(xi.x + ux, xi.y + uy, xi.z + uz, xi.w);
+  const double4 xn = make_double4(xd.x + ux, xd.y + uy, xd.z + uz, xd.w);
+  d.xh[nxt][i] = xn;
+  if (MX) {
+    d.vrf[nxt][i] = make_float2((float)vn, (float)rn);
+    d.xhf[nxt][i] = make_float4((float)(xn.x - d.xc[0]), (float)(xn.y - d.xc[1]), (float)(xn.z - d.xc[2]), (float)xn.w);
+  }
 }
 
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update) {
-  k_dens_drift<<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
+  if (d.mixed)
+    k_dens_drift<true><<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
+  else
+    k_dens_drift<false><<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
 }
 
 // ---------------------------------------------------------------------------
@@ -486,56 +639,63 @@ __device__ inline double hol_yield(const DevMat &M, double strain) {
   return M.hol_sy0;
 }
 
+template <bool MX>
 WF_KERNEL void k_rates_stress(Dev d, int buf, double dt, int do_rates, int do_stress) {
+  using Sr = Src<MX>;
+  using R = typename Sr::R;
   __shared__ double sh[kBlock / 64];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const long long np = d.npad;
   double e_int = 0.0, e_kin = 0.0, e_plw = 0.0, tmax = -INFINITY, tmin = INFINITY;
   if (i < d.n) {
     const int g = d.group[i];
     double SR[6], W[3];  // W = (w01, w02, w12)
     if (do_rates) {
-      for (int k = 0; k < 6; ++k) SR[k] = 0.0;
-      W[0] = W[1] = W[2] = 0.0;
+      R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};
+      R ekin = 0;
       if (g >= 0) {
         const DevMat &M = d.mats[d.mat[i]];
-        const double4 xi = d.xh[buf][i];
-        const double4 vi = d.vm[i];
-        const double rhoi = d.rho[buf][i];
-        const double Pi = M.P0 + (M.Cs * M.Cs) * (rhoi - M.ref_density);
-        const double pri = Pi / (rhoi * rhoi);
-        const double mi = vi.w;
+        const auto xi = Sr::x(d, buf, i);
+        const auto vi = Sr::v(d, i);
+        const R rhoi = Sr::rho(d, buf, i);
+        const R P0 = (R)M.P0, cs2 = (R)(M.Cs * M.Cs), rho0 = (R)M.ref_density;
+        const R irhoi = rcp_t(rhoi);
+        const R pri = (P0 + cs2 * (rhoi - rho0)) * irhoi * irhoi;
+        const R mi = vi.w;
+        const KH<R> ki = kern_h_t<R>(xi.w);
         const int cnt = d.nnb[i];
         const int *nb = d.nbr + i;
         for (int k = 0; k < cnt; ++k) {
           const int j = nb[(long long)k * np];
-          const double4 xj = d.xh[buf][j];
-          const double4 vj = d.vm[j];
-          const double volj = d.vol[buf][j];
-          const double rhoj = d.rho[buf][j];
-          const double h = (xi.w + xj.w) / 2;
-          const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-          const double r = sqrt(dx * dx + dy * dy + dz * dz);
-          const double GK = grad_kernel(r / h, h);
-          const double vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-          const double c = volj * (-0.5 * GK);
-          SR[XX] += c * (2.0 * vx * dx);
-          SR[YY] += c * (2.0 * vy * dy);
-          SR[ZZ] += c * (2.0 * vz * dz);
-          SR[XY] += c * (vx * dy + vy * dx);
-          SR[XZ] += c * (vx * dz + vz * dx);
-          SR[YZ] += c * (vy * dz + vz * dy);
-          W[0] += c * (vx * dy - vy * dx);
-          W[1] += c * (vx * dz - vz * dx);
-          W[2] += c * (vy * dz - vz * dy);
+          const auto xj = Sr::x(d, buf, j);
+          const auto vj = Sr::v(d, j);
+          const R volj = Sr::vol(d, buf, j);
+          const R rhoj = Sr::rho(d, buf, j);
+          const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+          const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, pair_h_t<R>(ki, xi.w, xj.w));
+          const R vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+          const R c = volj * ((R)-0.5 * GK);
+          sr[XX] += c * ((R)2 * vx * dx);
+          sr[YY] += c * ((R)2 * vy * dy);
+          sr[ZZ] += c * ((R)2 * vz * dz);
+          sr[XY] += c * (vx * dy + vy * dx);
+          sr[XZ] += c * (vx * dz + vz * dx);
+          sr[YZ] += c * (vy * dz + vz * dy);
+          w[0] += c * (vx * dy - vy * dx);
+          w[1] += c * (vx * dz - vz * dx);
+          w[2] += c * (vy * dz - vz * dy);
           // kinetic-energy pair term (Source/Domain.cpp:1776-1792): the pair
           // is oriented lower-index first; its sum over particles vanishes
-          const double Pj = M.P0 + (M.Cs * M.Cs) * (rhoj - M.ref_density);
-          const double msec = i < j ? vj.w : mi;
-          const double term = 0.5 * msec * (pri + Pj / (rhoj * rhoj)) * (vx * (GK * dx) + vy * (GK * dy) + vz * (GK * dz));
-          e_kin += i < j ? term : -term;
+          const R irj = rcp_t(rhoj);
+          const R Pj = P0 + cs2 * (rhoj - rho0);
+          const R msec = i < j ? vj.w : mi;
+          const R term = (R)0.5 * msec * (pri + Pj * irj * irj) * (vx * (GK * dx) + vy * (GK * dy) + vz * (GK * dz));
+          ekin += i < j ? term : -term;
         }
       }
+      for (int k = 0; k < 6; ++k) SR[k] = (double)sr[k];
+      for (int k = 0; k < 3; ++k) W[k] = (double)w[k];
+      e_kin = (double)ekin;
       for (int k = 0; k < 6; ++k) d.sr[k * np + i] = SR[k];
       for (int k = 0; k < 3; ++k) d.rr[k * np + i] = W[k];
     } else {
@@ -609,6 +769,13 @@ WF_KERNEL void k_rates_stress(Dev d, int buf, double dt, int do_rates, int do_st
       rec.rho = rho;
       rec.irho2 = 1.0 / (rho * rho);
       d.srec[i] = rec;
+      if (MX) {
+        StressRecF rf;
+        for (int k = 0; k < 6; ++k) rf.s[k] = (float)sig[k];
+        rf.rho = (float)rho;
+        rf.irho2 = (float)rec.irho2;
+        d.srecf[i] = rf;
+      }
       const double vol = d.vol[buf][i];
       // plastic work heat, Source/ThermalStuct.cpp:14-39 (delta_pl_strain is
       // never reset by the reference, so the last increment keeps heating)
@@ -641,17 +808,19 @@ WF_KERNEL void k_rates_stress(Dev d, int buf, double dt, int do_rates, int do_st
   const double r3 = block_reduce<1>(tmax, sh);
   const double r4 = block_reduce<2>(tmin, sh);
   if (threadIdx.x == 0) {
-    double *p = d.partials + (long long)blockIdx.x * P_COUNT;
-    p[P_INT] = r0;
-    p[P_KIN] = r1;
-    p[P_PLW] = r2;
-    p[P_MAXT] = r3;
-    p[P_MINT] = r4;
+    part(d, P_INT, blockIdx.x) = r0;
+    part(d, P_KIN, blockIdx.x) = r1;
+    part(d, P_PLW, blockIdx.x) = r2;
+    part(d, P_MAXT, blockIdx.x) = r3;
+    part(d, P_MINT, blockIdx.x) = r4;
   }
 }
 
 void launch_rates_stress(const Dev &d, hipStream_t st, int buf, double dt, int do_rates, int do_stress) {
-  k_rates_stress<<<d.nblocks, kBlock, 0, st>>>(d, buf, dt, do_rates, do_stress);
+  if (d.mixed)
+    k_rates_stress<true><<<d.nblocks, kBlock, 0, st>>>(d, buf, dt, do_rates, do_stress);
+  else
+    k_rates_stress<false><<<d.nblocks, kBlock, 0, st>>>(d, buf, dt, do_rates, do_stress);
 }
 
 // ---------------------------------------------------------------------------
@@ -672,7 +841,7 @@ __device__ inline bool pair_first(int ci, int cj, int oi, int oj, int ny, int nz
 
 WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
   __shared__ double sh[kBlock / 64];
-  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const long long np = d.npad;
   double e_int = 0.0;
   if (i < d.n && d.group[i] >= 0) {
@@ -689,18 +858,19 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
     const double kk = 4. * (M.k_T * M.k_T) / (M.k_T + M.k_T);
     const int ci = d.cell[i], oi = d.perm[i];
     double temp = 0.0;
+    const KernH ki = kern_h(xi.w);
     const int cnt = d.nnb[i];
     const int *nb = d.nbr + i;
     for (int k = 0; k < cnt; ++k) {
       const int j = nb[(long long)k * np];
       const double4 xj = d.xh[buf][j];
-      const double h = (xi.w + xj.w) / 2.0;
       const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      const double GK = grad_kernel(r / h, h);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double GK = grad_kernel_r2(r2, pair_h(ki, xi.w, xj.w));
       const bool first = pair_first(ci, d.cell[j], oi, d.perm[j], d.cellno[1], d.cellno[2]);
       const double vsec = first ? d.vol[buf][j] : voli;
-      temp += vsec * kk * (Ti - d.T[j]) * (GK * (dx * dx + dy * dy + dz * dz)) / (r * r);
+      // dot(x_ij, GK x_ij) / |x_ij|^2 = GK
+      temp += vsec * kk * (Ti - d.T[j]) * GK;
     }
     const double f = 1. / (rho * M.cp_T);
     const double dTdt = f * (temp + d.qconv[i] + d.qsrc[i] + d.qpl[i] * d.pl_frac + 0.0);
@@ -718,7 +888,7 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
                     2.0 * S[YZ] * SR[YZ] + S[ZZ] * SR[ZZ]);
   }
   const double r0 = block_reduce<0>(e_int, sh);
-  if (threadIdx.x == 0) d.partials[(long long)blockIdx.x * P_COUNT + P_INT] = r0;
+  if (threadIdx.x == 0) part(d, P_INT, blockIdx.x) = r0;
 }
 
 void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
@@ -729,19 +899,19 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
 // end of step: fixed-order reduction of the block partials, scalar updates,
 // reaction history, and the end-of-step half of the neighbour schedule.
 
-__global__ __launch_bounds__(256) void k_finalize(Dev d, double dt, long long step, int nz, int full_step) {
-  __shared__ double sh[4];
+__global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, long long step, int nz, int full_step) {
+  __shared__ double sh[16];
   const int nb = d.nblocks;
   double res[P_COUNT];
   for (int s = 0; s < P_COUNT; ++s) {
-    if (s >= P_REACT + 3 * nz) {
-      res[s] = 0.0;
-      continue;
-    }
+    res[s] = 0.0;
+    const bool active = s == P_INT || s == P_KIN || (s == P_PLW && d.pl_heating) ||
+                        ((s == P_MAXT || s == P_MINT) && d.thermal) || (s >= P_REACT && s < P_REACT + 3 * nz);
+    if (!active) continue;
     const int op = (s == P_MAXT) ? 1 : (s == P_MINT ? 2 : 0);
     double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
     for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      const double x = d.partials[(long long)b * P_COUNT + s];
+      const double x = part(d, s, b);
       v = op == 0 ? v + x : (op == 1 ? fmax(v, x) : fmin(v, x));
     }
     res[s] = op == 0 ? block_reduce<0>(v, sh) : (op == 1 ? block_reduce<1>(v, sh) : block_reduce<2>(v, sh));
@@ -767,7 +937,7 @@ __global__ __launch_bounds__(256) void k_finalize(Dev d, double dt, long long st
 }
 
 void launch_finalize(const Dev &d, hipStream_t st, double dt, long long step, int nz, int full_step) {
-  k_finalize<<<1, 256, 0, st>>>(d, dt, step, nz, full_step);
+  k_finalize<<<1, 1024, 0, st>>>(d, dt, step, nz, full_step);
 }
 
 // ---------------------------------------------------------------------------
@@ -778,15 +948,38 @@ WF_KERNEL void k_max_pl(Dev d) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const double v = i < d.n ? d.pl[i] : 0.0;
   const double r = block_reduce<1>(v, sh);
-  if (threadIdx.x == 0) d.partials[(long long)blockIdx.x * P_COUNT + P_MAXT] = r;
+  if (threadIdx.x == 0) part(d, P_MAXT, blockIdx.x) = r;
 }
 
 __global__ __launch_bounds__(256) void k_max_pl_final(Dev d) {
-  __shared__ double sh[4];
+  __shared__ double sh[16];
   double v = 0.0;
-  for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) v = fmax(v, d.partials[(long long)b * P_COUNT + P_MAXT]);
+  for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) v = fmax(v, part(d, P_MAXT, b));
   const double r = block_reduce<1>(v, sh);
   if (threadIdx.x == 0) d.scalars[S_MAXPL] = r;
+}
+
+// rebuild every FP32 shadow from the FP64 state (set-up and host writes)
+WF_KERNEL void k_shadows(Dev d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= d.n) return;
+  for (int b = 0; b < 2; ++b) {
+    const double4 x = d.xh[b][i];
+    d.xhf[b][i] = make_float4((float)(x.x - d.xc[0]), (float)(x.y - d.xc[1]), (float)(x.z - d.xc[2]), (float)x.w);
+    d.vrf[b][i] = make_float2((float)d.vol[b][i], (float)d.rho[b][i]);
+  }
+  const double4 v = d.vm[i];
+  d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+  const StressRec r = d.srec[i];
+  StressRecF f;
+  for (int k = 0; k < 6; ++k) f.s[k] = (float)r.s[k];
+  f.rho = (float)r.rho;
+  f.irho2 = (float)r.irho2;
+  d.srecf[i] = f;
+}
+
+void launch_shadows(const Dev &d, hipStream_t st) {
+  if (d.mixed) k_shadows<<<nblk(d.n), kBlock, 0, st>>>(d);
 }
 
 void launch_max_pl(const Dev &d, hipStream_t st) {
