@@ -1,0 +1,338 @@
+"""Pins of the CPU oracle against what the mathematics fixes.
+
+The reference ships no automated tests or golden vectors for this path, so
+each oracle function is checked against closed forms, invariants, special
+cases and brute force, each chosen so that a plausible slip (a dropped term,
+a sign, a transposed index, a factor 2) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from oracle_case import brute_pairs, oracle_from_case, unordered
+from weldform_amd import cases
+
+H = 1.2e-3
+
+
+# ---------------------------------------------------------------- kernel
+def test_kernel_closed_form_and_normalisation():
+    h = H
+    assert po.kernel(0.0, h) == pytest.approx(1.0 / (math.pi * h ** 3), rel=1e-15)
+    # continuity at q = 1 and support end at q = 2
+    assert po.kernel(1.0 - 1e-12, h) == pytest.approx(po.kernel(1.0, h), rel=1e-9)
+    assert po.kernel(2.0, h) == 0.0 and po.kernel(1.999999, h) > 0.0
+    # 3D normalisation: 4 pi int_0^{2h} W r^2 dr = 1
+    r = np.linspace(0, 2 * h, 200001)
+    w = np.array([po.kernel(x / h, h) for x in r])
+    integral = 4 * math.pi * np.trapezoid(w * r * r, r)
+    assert integral == pytest.approx(1.0, abs=1e-8)
+
+
+def test_grad_kernel_is_dW_dr_over_r():
+    h = H
+    for q in [1e-3, 0.3, 0.7, 0.999, 1.0, 1.3, 1.7, 1.99]:
+        r = q * h
+        eps = 1e-7 * h
+        dwdr = (po.kernel((r + eps) / h, h) - po.kernel((r - eps) / h, h)) / (2 * eps)
+        # GK * x_ij = grad_i W  =>  GK * r = dW/dr
+        assert po.grad_kernel(q, h) * r == pytest.approx(dwdr, rel=1e-6)
+    # q == 0 branch: limit of (dW/dr)/r = -3 C / h^2 with C = 1/(pi h^3)
+    assert po.grad_kernel(0.0, h) == pytest.approx(-3.0 / (math.pi * h ** 5), rel=1e-15)
+    assert po.grad_kernel(2.0, h) == 0.0
+
+
+# ---------------------------------------------------------------- geometry
+def test_cylinder_counts_and_symmetry():
+    # counts quoted for the reference's generator
+    assert len(po.gen_cylinder([0, 0, 0], 7.5e-3, 30e-3, 0.5e-3)) == 4524
+    x = po.gen_cylinder([0, 0, 0], 7.5e-3, 30e-3, 0.5e-3)
+    # quadrant symmetric lattice, centres within R - r, 29 rows
+    assert np.unique(np.round(x[:, 2], 12)).size == 29
+    assert np.max(np.hypot(x[:, 0], x[:, 1])) <= 7.5e-3 - 0.5e-3 + 1e-12
+    key = lambda a: set(map(tuple, np.round(a, 12)))  # noqa: E731
+    assert key(x) == key(x * np.array([-1, 1, 1])) == key(x * np.array([1, -1, 1]))
+
+
+# ---------------------------------------------------------------- neighbours
+@pytest.mark.parametrize("jitter", [False, True])
+def test_neighbour_set_equals_brute_force(jitter):
+    c = cases.small_jc(steps=1) if jitter else cases.config1(steps=1)
+    d = oracle_from_case(c)
+    d.neighbour_search()
+    p = d.pairs()
+    u = unordered(p)
+    # each unordered pair once, never a self pair
+    assert np.all(u[:, 0] < u[:, 1])
+    assert len(np.unique(u, axis=0)) == len(u)
+    assert np.array_equal(u, brute_pairs(c.x, c.h))
+    if not jitter:
+        assert len(u) == 107490
+    # per-particle neighbour counts (SaveNeighbourData) match the pair list
+    nb = np.bincount(p.ravel(), minlength=c.n)
+    assert np.array_equal(nb, d.i("NB"))
+
+
+def test_pair_orientation_follows_half_stencil():
+    """temp1 is the particle whose cell scans the other (same cell: the higher
+    index, since ListGenerate pushes to the list head)."""
+    c = cases.config1(steps=1)
+    d = oracle_from_case(c)
+    d.solve_init(c.dt)
+    d.neighbour_search()
+    p = d.pairs()
+    cs = np.array([d.param(f"CellSize{k}") for k in range(3)])
+    bl = np.array([d.param(f"BLPF{k}") for k in range(3)])
+    cn = np.array([d.param(f"CellNo{k}") for k in range(3)]).astype(int)
+    cc = np.clip(np.floor((c.x - bl) / cs).astype(int), 0, cn - 1)
+    a, b = p[:, 0], p[:, 1]
+    dd = cc[b] - cc[a]
+    same = np.all(dd == 0, axis=1)
+    assert np.all(a[same] > b[same])
+    half = (dd[:, 2] == 1) | ((dd[:, 2] == 0) & (dd[:, 1] == 1)) | ((dd[:, 2] == 0) & (dd[:, 1] == 0) & (dd[:, 0] == 1))
+    assert np.all(half[~same])
+
+
+# ---------------------------------------------------------------- pair passes
+def _interior(c, d):
+    nb = d.i("NB")
+    return nb == nb.max()
+
+
+def _lattice_state(L=None, sig=None):
+    c = cases.config1(steps=1)
+    d = oracle_from_case(c)
+    if L is not None:
+        d.f("V")[:] = c.x @ np.asarray(L).T
+    if sig is not None:
+        d.f("SIGMA")[:] = sig(c.x)
+    d.neighbour_search()
+    d.start_acceleration()
+    return c, d
+
+
+def test_uniform_fields_give_zero_rates():
+    c = cases.config1(steps=1)
+    d = oracle_from_case(c)
+    d.f("V")[:] = np.array([0.3, -1.2, 2.0])
+    d.neighbour_search()
+    d.start_acceleration()
+    d.calc_dens()
+    d.calc_rates()
+    assert np.all(d.f("DRHO") == 0) and np.all(d.f("SR") == 0) and np.all(d.f("RR") == 0)
+
+
+def _lattice_moment(c):
+    """mu with sum_j V_j GK_ij x_ij (x) x_ij = -mu I on the full cubic-lattice
+    neighbourhood, computed from the kernel function and the lattice offsets
+    alone (V = m/rho, the interior neighbourhood is every offset within 2h)."""
+    dx, h = c.meta["dx"], c.meta["h"]
+    V = c.mass[0] / c.density[0]
+    r = np.arange(-3, 4)
+    k = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    k2 = (k * k).sum(1)
+    k2 = k2[(k2 > 0) & (np.sqrt(k2) * dx / h <= 2.0)]
+    assert len(k2) == 56
+    return -V / 3.0 * sum(po.grad_kernel(math.sqrt(q) * dx / h, h) * q * dx * dx for q in k2)
+
+
+def test_linear_velocity_field():
+    """v = L x on the lattice interior: SR = mu sym(L), Omega = mu skew(L),
+    drho/dt = -rho mu tr(L), with one lattice moment mu (close to 1)."""
+    L = np.array([[1.0, 2.0, -0.5], [0.3, -1.5, 0.7], [-0.2, 0.4, 0.8]]) * 100.0
+    c, d = _lattice_state(L=L)
+    mu = _lattice_moment(c)
+    assert 0.9 < mu < 1.3
+    d.calc_rates()
+    d.calc_dens()
+    m = _interior(c, d)
+    scale = np.abs(L).max()
+    assert np.abs(d.f("SR")[m] - mu * 0.5 * (L + L.T)).max() < 1e-10 * scale
+    assert np.abs(d.f("RR")[m] - mu * 0.5 * (L - L.T)).max() < 1e-10 * scale
+    assert np.abs(d.f("DRHO")[m] + 2700.0 * mu * np.trace(L)).max() < 1e-10 * 2700.0 * scale
+    # a rigid rotation has no strain rate on the symmetric lattice interior
+    W = np.array([[0, 1.0, 2.0], [-1.0, 0, -0.5], [-2.0, 0.5, 0]]) * 100.0
+    c, d = _lattice_state(L=W)
+    d.calc_rates()
+    m = _interior(c, d)
+    assert np.abs(d.f("SR")[m]).max() < 1e-10 * 100.0
+    assert np.abs(d.f("RR")[m] - mu * W).max() < 1e-10 * 200.0
+
+
+def test_accel_uniform_and_linear_pressure():
+    s0 = np.array([[1.0, 0.2, -0.3], [0.2, -2.0, 0.5], [-0.3, 0.5, 0.7]]) * 1e8
+    c, d = _lattice_state(sig=lambda x: np.broadcast_to(s0, (len(x), 3, 3)).copy())
+    d.calc_accel()
+    m = _interior(c, d)
+    a = d.f("A")
+    assert np.abs(a[m]).max() < 1e-9 * np.abs(a).max()
+    g = np.array([3.0, -1.0, 2.0]) * 1e10     # pressure gradient, Pa/m
+    c, d = _lattice_state(sig=lambda x: -(1e8 + x @ g)[:, None, None] * np.eye(3)[None])
+    mu = _lattice_moment(c)
+    d.calc_accel()
+    m = _interior(c, d)
+    # momentum equation: a = div(sigma)/rho = -mu grad(p)/rho on the lattice
+    assert np.abs(d.f("A")[m] + mu * g / 2700.0).max() < 1e-8 * np.abs(g).max() / 2700.0
+
+
+def test_momentum_conservation():
+    c = cases.small_jc(steps=1)
+    d = oracle_from_case(c)
+    rng = np.random.default_rng(4)
+    d.f("V")[:] = rng.normal(size=(c.n, 3))
+    s = rng.normal(0, 1e8, size=(c.n, 3, 3))
+    d.f("SIGMA")[:] = 0.5 * (s + np.transpose(s, (0, 2, 1)))
+    d.neighbour_search()
+    d.start_acceleration()
+    d.calc_accel()
+    ma = d.f("A") * c.mass[:, None]
+    assert np.abs(ma.sum(axis=0)).max() < 1e-12 * np.abs(ma).sum()
+
+
+# ---------------------------------------------------------------- stress update
+def _one_particle(model, **mat):
+    c = cases.config1(steps=1)
+    d = po.Domain(1)
+    E, nu, rho = 68.9e9, 0.3, 2700.0
+    G = E / (2 * (1 + nu))
+    d.f("RHO")[:] = rho
+    d.f("REFRHO")[:] = rho
+    d.f("CS")[:] = math.sqrt(E / (3 * (1 - 2 * nu)) / rho)
+    d.f("G")[:] = G
+    d.f("MASS")[:] = 1.0
+    d.i("MODEL")[:] = model
+    m = dict(model=model, E=E, nu=nu)
+    m.update(mat)
+    d.set_material(0, **m)
+    return d, G, E, c
+
+
+def test_elastic_increment_closed_form():
+    d, G, E, _ = _one_particle(po.BILINEAR)
+    d.f("SIGMAY")[:] = 1e12
+    e = 2.0e3
+    d.f("SR")[0] = np.diag([e, -0.3 * e, 0.1 * e])
+    d.f("SR")[0, 0, 1] = d.f("SR")[0, 1, 0] = 0.4 * e
+    dt = 1e-7
+    sr = d.f("SR")[0].copy()
+    d.stress_all(dt)
+    dev = sr - np.trace(sr) / 3 * np.eye(3)
+    assert np.allclose(d.f("SHEAR")[0], 2 * G * dt * dev, rtol=1e-14, atol=0)
+    # effective strain rate of the reference (Particle.cpp:983-987) = sqrt(3/2 dev:dev)
+    assert d.f("EFFSR")[0] == pytest.approx(math.sqrt(1.5 * np.sum(dev * dev)), rel=1e-12)
+
+
+def test_jaumann_rotation_closed_form():
+    d, G, E, _ = _one_particle(po.BILINEAR)
+    d.f("SIGMAY")[:] = 1e12
+    s, w, dt = 1e8, 5e3, 1e-7
+    d.f("SHEAR")[0] = np.diag([s, -s, 0.0])
+    d.f("RR")[0, 0, 1], d.f("RR")[0, 1, 0] = w, -w
+    d.stress_all(dt)
+    S = d.f("SHEAR")[0]
+    # S W^T + W S for S = diag(s,-s,0), W_01 = w: off-diagonal -2 s w
+    assert S[0, 1] == pytest.approx(-2 * s * w * dt, rel=1e-12)
+    assert S[1, 0] == pytest.approx(-2 * s * w * dt, rel=1e-12)
+    assert S[0, 0] == s and S[1, 1] == -s
+
+
+def test_radial_return_bilinear_closed_form():
+    Et = 1.5e9
+    d, G, E, _ = _one_particle(po.BILINEAR)
+    Ep = E * Et / (E - Et)
+    Sy = 300e6
+    d.f("SIGMAY")[:] = Sy
+    d.f("EP")[:] = Ep
+    e, dt = 5e4, 1e-6      # one large uniaxial deviatoric increment
+    d.f("SR")[0] = np.diag([e, -0.5 * e, -0.5 * e])
+    d.stress_all(dt)
+    S = d.f("SHEAR")[0]
+    sig_tr = 2 * G * dt * e * 1.5           # sqrt(3 J2) of the trial deviator
+    assert math.sqrt(1.5 * np.sum(S * S)) == pytest.approx(Sy, rel=1e-12)   # scaled to the old Sy
+    dep = (sig_tr - Sy) / (3 * G + Ep)
+    assert d.f("PLSTRAIN")[0] == pytest.approx(dep, rel=1e-12)
+    assert d.f("SIGMAY")[0] == pytest.approx(Sy + dep * Ep, rel=1e-12)
+    # plastic strain increment is deviatoric and coaxial with S (Prandtl-Reuss)
+    epl = d.f("EPL")[0]
+    assert abs(np.trace(epl)) < 1e-12 * np.abs(epl).max()
+    assert np.allclose(epl / np.abs(epl).max(), S / np.abs(S).max(), atol=1e-12)
+
+
+def test_johnson_cook_and_hollomon_yield_closed_form():
+    A, B, n, C, eps0, m, Tm, Tt = 175e6, 380e6, 0.32, 0.0015, 1.0, 1.0, 775.0, 300.0
+    d, G, E, _ = _one_particle(po.JOHNSON_COOK, A=A, B=B, n=n, C=C, eps_0=eps0, jm=m, Tm=Tm, Tt=Tt)
+    pl, T = 0.1, 400.0
+    d.f("PLSTRAIN")[:] = pl
+    d.f("T")[:] = T
+    d.f("SIGMAY")[:] = 1e12      # keeps the step elastic; Sy is re-evaluated
+    e = 100.0
+    d.f("SR")[0] = np.diag([e, -0.5 * e, -0.5 * e])
+    d.stress_all(1e-12)
+    sr_eq = 1.5 * e              # eff. strain rate of a uniaxial deviatoric rate
+    want = (A + B * pl ** n) * (1 + C * math.log(sr_eq / eps0)) * (1 - ((T - Tt) / (Tm - Tt)) ** m)
+    assert d.f("SIGMAY")[0] == pytest.approx(want, rel=1e-12)
+    K, mh, sy0 = 500e6, 0.2, 150e6
+    d, G, E, _ = _one_particle(po.HOLLOMON, K=K, m=mh, sy0=sy0)
+    d.f("PLSTRAIN")[:] = 0.05
+    d.f("SIGMAY")[:] = 1e12
+    d.stress_all(1e-12)
+    assert d.f("SIGMAY")[0] == pytest.approx(K * (0.05 + sy0 / E) ** mh, rel=1e-12)
+
+
+# ---------------------------------------------------------------- solver loop
+def test_free_particles_drift_and_bc():
+    """Particles farther apart than 2h have no pairs: Leapfrog keeps v and
+    moves x by v dt per step; a velocity-BC zone follows its amplitude."""
+    n = 3
+    d = po.Domain(n)
+    d.f("X")[:] = np.array([[0, 0, 0], [1.0, 0, 0.5], [0, 1.0, 1.0]])
+    d.f("H")[:] = 1e-3
+    d.f("MASS")[:] = 1.0
+    d.f("RHO")[:] = d.f("REFRHO")[:] = 1000.0
+    d.f("SIGMAY")[:] = 1e12
+    d.f("V")[:] = np.array([[1.0, 2.0, 3.0], [0, 0, 0], [0, 0, 0]])
+    d.i("ID")[:] = [0, 0, 5]
+    d.set_material(0, model=po.BILINEAR, E=1e9, nu=0.3)
+    d.add_amplitude(1, [0.0, 1e-6, 1.0], [0.0, 2.0, 2.0])
+    d.add_velocity_bc(5, [0.0, 0.0, -1.0], value_type=1, amp_id=1, amp_factor=0.5)
+    dt = 1e-7
+    d.solve_init(dt)
+    for k in range(7):
+        d.step()
+    assert np.allclose(d.f("X")[0], np.array([1.0, 2.0, 3.0]) * 7 * dt, rtol=1e-13)
+    # amplitude at t = 6 dt: 2 * 6e-7 / 1e-6 = 1.2 ; velocity = 0.5 * 1.2 * (0,0,-1)
+    assert d.f("V")[2] == pytest.approx([0.0, 0.0, -0.6], rel=1e-12)
+
+
+def test_neighbour_schedule():
+    """Searches run on step 1, on the first step whose start-of-step max
+    plastic strain exceeds 1e-6, and every ts_nb_inc = 5 steps after."""
+    c = cases.small_jc(steps=120)
+    c.bcs[1]["value"] = (0.0, 0.0, -20.0)
+    d = oracle_from_case(c, threads=4)
+    d.solve_init(c.dt)
+    searches, mx = [], []
+    for _ in range(c.steps):
+        before = d.param("nb_searches")
+        d.step()
+        mx.append(d.param("max_pl_strain"))
+        searches.append(d.param("nb_searches") - before)
+    s0 = next(k for k, v in enumerate(mx) if v > 1e-6)
+    want = [1 if (k == 0 or (k >= s0 and (k - s0) % 5 == 0)) else 0 for k in range(c.steps)]
+    assert searches == want
+
+
+def test_threads_do_not_change_results():
+    c = cases.small_jc(steps=30)
+    c.bcs[1]["value"] = (0.0, 0.0, -20.0)
+    out = []
+    for th in (1, 4):
+        d = oracle_from_case(c, threads=th)
+        d.solve_init(c.dt)
+        for _ in range(c.steps):
+            d.step()
+        out.append((d.f("X").copy(), d.f("SIGMA").copy(), d.param("int_energy_sum")))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
