@@ -49,6 +49,9 @@ def algorithmic_bytes(kind, n, nbar, mixed):
 
 
 def timed_run(case, precision, steps, warmup):
+    """Graph-replay timed region for the throughput, then the same steps again
+    launched one by one with HIP events around each pair kernel (per-kernel
+    durations for the roofline)."""
     from weldform_amd import cases
 
     w = cases.make_solver(case, precision=precision)
@@ -57,14 +60,20 @@ def timed_run(case, precision, steps, warmup):
     w.close()
     s = cases.make_solver(case, precision=precision)
     s.sync()
-    s.set_timing(True)
     t0 = time.perf_counter()
     s.step(steps)
     s.sync()
     wall = time.perf_counter() - t0
-    times = {k: s.kernel_time(k) for k in ("accel", "density", "rates", "neighbours", "step")}
-    return s, wall, times
-
+    e = cases.make_solver(case, precision=precision)
+    e.sync()
+    e.set_timing(True)
+    t1 = time.perf_counter()
+    e.step(steps)
+    e.sync()
+    wall_eager = time.perf_counter() - t1
+    times = {k: e.kernel_time(k) for k in ("accel", "density", "rates", "neighbours", "step")}
+    e.close()
+    return s, wall, wall_eager, times
 
 def cpu_baseline(case, steps):
     sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
@@ -91,7 +100,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of 1 GiB (for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = 1 << 30
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built", file=sys.stderr)
@@ -105,10 +126,12 @@ def main():
     log(f"bench: {case.name} n={case.n} dt={case.dt:.4e} warmup={args.warmup} steps={args.steps}")
     runs = {}
     for prec, name in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
-        s, wall, times = timed_run(case, prec, args.steps, args.warmup)
-        runs[name] = dict(solver=s, wall=wall, times=times, R=s.reaction_history(case.top_zone)[:, 2],
-                          disp=s.get("disp"), ps=case.n * args.steps / wall)
-        log(f"bench: {name}: {1e3 * wall / args.steps:.3f} ms/step  {runs[name]['ps']:.4e} particle-steps/s  "
+        s, wall, wall_eager, times = timed_run(case, prec, args.steps, args.warmup)
+        runs[name] = dict(solver=s, wall=wall, wall_eager=wall_eager, times=times,
+                          R=s.reaction_history(case.top_zone)[:, 2], disp=s.get("disp"),
+                          ps=case.n * args.steps / wall)
+        log(f"bench: {name}: {1e3 * wall / args.steps:.3f} ms/step (graph), "
+            f"{1e3 * wall_eager / args.steps:.3f} ms/step (eager, timed)  {runs[name]['ps']:.4e} particle-steps/s  "
             "kernel avg ms " + ", ".join(f"{k}={v[0]:.4f}" for k, v in times.items()))
     f64, mx = runs["fp64"], runs["mixed"]
     pairs, maxnb = mx["solver"].neighbour_stats()
@@ -161,6 +184,10 @@ def main():
             "pair_evaluations_per_s": (pairs // 2) * args.steps / mx["wall"],
             "kernel_ms_mixed": {k: v[0] for k, v in times.items()},
             "kernel_ms_fp64": {k: v[0] for k, v in f64["times"].items()},
+            "eager_ms_per_step_mixed": 1e3 * mx["wall_eager"] / args.steps,
+            "eager_ms_per_step_fp64": 1e3 * f64["wall_eager"] / args.steps,
+            "timing_note": "value: steps replayed from one captured HIP graph per step; kernel_ms: "
+                           "the same steps launched one by one with HIP events on the solver's stream",
             "nb_searches": sc["nb_searches"],
         },
         "roofline": {

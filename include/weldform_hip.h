@@ -122,7 +122,9 @@ typedef struct wf_config {
   int32_t thermal_solver;   /* Domain::thermal_solver                          */
   int32_t pl_heating;       /* Domain::pl_heating                              */
   int32_t device;           /* HIP device ordinal                              */
-  int32_t reserved[2];
+  int32_t no_graphs;        /* 1: launch kernels one by one instead of replaying
+                               a captured HIP graph per step                  */
+  int32_t reserved;
   double cellfac;           /* Domain::Cellfac (2 for the cubic spline)       */
   double pl_work_heat_frac; /* Domain::pl_work_heat_frac (0.9)                */
   double dom_max[3];        /* Domain::DomMax (default -1e11)                  */

@@ -77,6 +77,10 @@ struct Ctx {
   double time = 0.0, dt = 0.0;
   long long steps = 0;
   std::vector<int> rzones;
+  BcTable bc{};
+  bool bc_dirty = true;
+  DevAmp *damps = nullptr;
+  hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [scheme][cur]
   std::string err;
   // timing
   bool timing = false;
@@ -99,7 +103,16 @@ struct Ctx {
     allocs.push_back(p);
     return (T *)p;
   }
+  void drop_graphs() {
+    for (auto &g : gexec)
+      for (auto &e : g)
+        if (e) {
+          (void)hipGraphExecDestroy(e);
+          e = nullptr;
+        }
+  }
   ~Ctx() {
+    drop_graphs();
     for (auto &e : ev_pool) (void)hipEventDestroy(e);
     for (void *p : allocs) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
@@ -347,6 +360,8 @@ void allocate(Ctx *c) {
   d.scan_tmp = c->alloc<int>((size_t)(d.ncells + 1023) / 1024 + 1);
   c->dmats = c->alloc<DevMat>(kMaxMat);
   d.mats = c->dmats;
+  c->damps = c->alloc<DevAmp>(kMaxAmp);
+  d.amps = c->damps;
   d.flags = c->alloc<DevFlags>(1);
   d.partials = c->alloc<double>((size_t)d.nblocks * P_COUNT);
   d.scalars = c->alloc<double>(S_COUNT);
@@ -496,33 +511,41 @@ void harvest_timing(Ctx *c) {
   c->ev_next = 0;
 }
 
-BcTable make_bc(Ctx *c) {
+// BC and amplitude tables for the device; amplitudes are evaluated there
+void refresh_bc(Ctx *c) {
+  if (!c->bc_dirty) return;
+  if ((int)c->amps.size() > kMaxAmp) throw Error(WF_EUNSUPPORTED, "more than 8 amplitude tables");
+  std::vector<DevAmp> da(kMaxAmp);
+  for (size_t k = 0; k < c->amps.size(); ++k) {
+    const AmpHost &a = c->amps[k];
+    if ((int)a.t.size() > kMaxAmpPts) throw Error(WF_EUNSUPPORTED, "amplitude with more than 16 points");
+    da[k].id = a.id;
+    da[k].n = (int)a.t.size();
+    for (size_t q = 0; q < a.t.size(); ++q) {
+      da[k].t[q] = a.t[q];
+      da[k].v[q] = a.v[q];
+    }
+  }
+  dput(c, c->damps, da);
   BcTable t{};
   for (const BcHost &b : c->bcs) {
-    if (t.n >= kMaxBc) throw Error(WF_EUNSUPPORTED, "more than 8 boundary conditions");
     const int k = t.n++;
     t.zone[k] = b.zone;
     t.type[k] = b.type;
+    t.value_type[k] = b.type == 0 ? b.value_type : 0;
     t.T[k] = b.T;
-    if (b.type == 0) {
-      if (b.value_type == 0) {
-        for (int q = 0; q < 3; ++q) t.v[k][q] = b.value[q];
-      } else {
-        // UserAcc amplitude branch (Source/WeldForm.cpp:73-81)
-        bool found = false;
-        for (const AmpHost &a : c->amps)
-          if (a.id == b.amp_id) {
-            const double val = b.amp_factor * a.at(c->time);
-            for (int q = 0; q < 3; ++q) t.v[k][q] = val * b.value[q];
-            found = true;
-          }
-        if (!found) t.zone[k] = -0x7ffffff;  // no matching amplitude: BC inactive
-      }
-    }
+    t.amp_factor[k] = b.amp_factor;
+    t.amp[k] = -1;
+    // UserAcc applies the last amplitude with a matching id (Source/WeldForm.cpp:74-81)
+    for (size_t q = 0; q < c->amps.size(); ++q)
+      if (c->amps[q].id == b.amp_id) t.amp[k] = (int)q;
+    for (int q = 0; q < 3; ++q) t.v[k][q] = b.value[q];
   }
   t.nz = (int)c->rzones.size();
   for (int z = 0; z < t.nz; ++z) t.rzone[z] = c->rzones[z];
-  return t;
+  c->bc = t;
+  c->bc_dirty = false;
+  c->drop_graphs();
 }
 
 void check_flags(Ctx *c) {
@@ -533,36 +556,54 @@ void check_flags(Ctx *c) {
   if (f.nan) throw Error(WF_ENAN, "non-finite value in the particle state");
 }
 
-void one_step(Ctx *c, int scheme) {
+// one iteration of the Leapfrog / KickDrift loop, all on the stream
+void enqueue_step(Ctx *c, int scheme, int cur) {
   Dev &d = c->d;
-  const int isfirst = c->steps == 0;
   Timed whole(c, T_STEP);
-  launch_schedule(d, c->st, isfirst, 0);
+  launch_schedule(d, c->st, 0);
   {
     Timed t(c, T_NBR);
-    launch_neighbours(d, c->st, c->cur);
+    launch_neighbours(d, c->st, cur);
   }
-  c->built = true;
   {
     Timed t(c, T_ACCEL);
-    launch_accel(d, c->st, c->cur);
+    launch_accel(d, c->st, cur);
   }
-  const BcTable bc = make_bc(c);
-  launch_kick(d, c->st, bc, scheme, isfirst, c->dt, c->dt);
+  launch_kick(d, c->st, c->bc, scheme, c->dt, c->dt);
   {
     Timed t(c, T_DENS);
-    launch_dens_drift(d, c->st, c->cur, c->dt, 1);
+    launch_dens_drift(d, c->st, cur, c->dt, 1);
   }
   if (scheme == WF_KICKDRIFT) launch_kick2(d, c->st, c->dt);
-  const int nxt = c->cur ^ 1;
+  const int nxt = cur ^ 1;
   {
     Timed t(c, T_RATES);
-    launch_rates_stress(d, c->st, nxt, c->dt, 1, 1);
+    launch_rates(d, c->st, nxt);
   }
+  launch_stress(d, c->st, nxt, c->dt);
   if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, c->dt);
-  launch_finalize(d, c->st, c->dt, c->steps, bc.nz, 1);
-  HIPCHK(hipGetLastError());
-  c->cur = nxt;
+  launch_finalize(d, c->st, c->dt, c->bc.nz);
+}
+
+void one_step(Ctx *c, int scheme) {
+  refresh_bc(c);
+  if (c->timing || c->cfg.no_graphs) {
+    enqueue_step(c, scheme, c->cur);
+    HIPCHK(hipGetLastError());
+  } else {
+    hipGraphExec_t &ex = c->gexec[scheme][c->cur];
+    if (!ex) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      enqueue_step(c, scheme, c->cur);
+      HIPCHK(hipStreamEndCapture(c->st, &g));
+      HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(g));
+    }
+    HIPCHK(hipGraphLaunch(ex, c->st));
+  }
+  c->built = true;
+  c->cur ^= 1;
   c->time += c->dt;
   c->steps += 1;
 }
@@ -847,6 +888,7 @@ int wf_add_amplitude(void *ctx, int32_t id, int32_t npts, const double *t, const
     a.t.assign(t, t + npts);
     a.v.assign(v, v + npts);
     c->amps.push_back(a);
+    c->bc_dirty = true;
     return WF_OK;
   });
 }
@@ -870,6 +912,7 @@ int wf_add_velocity_bc(void *ctx, int32_t zone, int32_t value_type, const double
       if ((int)c->rzones.size() >= kMaxZones) throw Error(WF_EUNSUPPORTED, "more than 8 velocity-BC zones");
       c->rzones.push_back(zone);
     }
+    c->bc_dirty = true;
     return WF_OK;
   });
 }
@@ -884,6 +927,7 @@ int wf_add_temperature_bc(void *ctx, int32_t zone, double T) {
     b.type = 2;
     b.T = T;
     c->bcs.push_back(b);
+    c->bc_dirty = true;
     return WF_OK;
   });
 }
@@ -995,7 +1039,7 @@ int wf_rebuild_neighbours(void *ctx) {
   return guard(c, [&] {
     as_ctx(ctx);
     if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
-    launch_schedule(c->d, c->st, 0, 1);
+    launch_schedule(c->d, c->st, 1);
     launch_neighbours(c->d, c->st, c->cur);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
@@ -1040,8 +1084,8 @@ int wf_eval(void *ctx, int32_t what, double dt) {
     switch (what) {
       case WF_EVAL_ACCEL: launch_accel(c->d, c->st, c->cur); break;
       case WF_EVAL_DENSITY: launch_dens_drift(c->d, c->st, c->cur, dt, 0); break;
-      case WF_EVAL_RATES: launch_rates_stress(c->d, c->st, c->cur, dt, 1, 0); break;
-      case WF_EVAL_STRESS: launch_rates_stress(c->d, c->st, c->cur, dt, 0, 1); break;
+      case WF_EVAL_RATES: launch_rates(c->d, c->st, c->cur); break;
+      case WF_EVAL_STRESS: launch_stress(c->d, c->st, c->cur, dt); break;
       default: throw Error(WF_EINVAL, "unknown evaluation");
     }
     HIPCHK(hipGetLastError());

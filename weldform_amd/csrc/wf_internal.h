@@ -65,6 +65,8 @@ struct DevFlags {
   int ts_nb_inc;     // reference's ts_nb_inc (search period after yield)
   long long nb_searches;
   long long directed_pairs;
+  long long step;    // completed steps since wf_init (reaction history slot)
+  double time;       // Domain::Time at the start of the next step
 };
 
 // partial-sum slots per block
@@ -79,11 +81,26 @@ enum {
 
 enum { S_KIN = 0, S_INT, S_PLW, S_MAXT, S_MINT, S_MAXPL, S_COUNT };
 
+constexpr int kMaxAmp = 8;
+constexpr int kMaxAmpPts = 16;
+
+// amplitude table (Source/Domain.h:69-89)
+struct DevAmp {
+  int id, n;
+  double t[kMaxAmpPts], v[kMaxAmpPts];
+};
+
+// boundary conditions, constant for a context; amplitude-driven velocities
+// are evaluated on the device at the step's time, so a step has no
+// host-varying argument and can be captured in a graph
 struct BcTable {
   int n;
   int zone[kMaxBc];
   int type[kMaxBc];            // 0 velocity, 2 temperature
-  double v[kMaxBc][3];         // velocity value at the current time
+  int value_type[kMaxBc];      // 0 constant, 1 amplitude
+  int amp[kMaxBc];             // index into the amplitude table, -1 if none
+  double amp_factor[kMaxBc];
+  double v[kMaxBc][3];         // constant value / amplitude direction
   double T[kMaxBc];
   int nz;                      // reaction zones
   int rzone[kMaxZones];
@@ -127,6 +144,7 @@ struct Dev {
   int *nnb, *nbr;
   // materials, flags, reductions
   const DevMat *mats;
+  const DevAmp *amps;
   DevFlags *flags;
   double *partials;            // [nblocks][P_COUNT]
   double *scalars;             // [S_COUNT]
@@ -135,16 +153,16 @@ struct Dev {
 };
 
 // launchers (wf_kernels.hip); all enqueue on `st`
-void launch_schedule(const Dev &d, hipStream_t st, int isfirst, int force);
+void launch_schedule(const Dev &d, hipStream_t st, int force);
 void launch_neighbours(const Dev &d, hipStream_t st, int cur);
 void launch_accel(const Dev &d, hipStream_t st, int cur);
-void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, int isfirst, double dt_prev,
-                 double dt);
+void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt);
 void launch_kick2(const Dev &d, hipStream_t st, double dt);
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update);
-void launch_rates_stress(const Dev &d, hipStream_t st, int buf, double dt, int do_rates, int do_stress);
+void launch_rates(const Dev &d, hipStream_t st, int buf);
+void launch_stress(const Dev &d, hipStream_t st, int buf, double dt);
 void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt);
-void launch_finalize(const Dev &d, hipStream_t st, double dt, long long step, int nz, int full_step);
+void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz);
 void launch_max_pl(const Dev &d, hipStream_t st);
 void launch_shadows(const Dev &d, hipStream_t st);
 

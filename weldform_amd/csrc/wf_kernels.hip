@@ -219,9 +219,10 @@ __device__ inline int cell_coord(double x, double blpf, double cs, int cn) {
 // ---------------------------------------------------------------------------
 // neighbour schedule (Source/SolverLeapfrog.cpp:160-201, 347-356)
 
-WF_KERNEL void k_schedule(Dev d, int isfirst, int force) {
+WF_KERNEL void k_schedule(Dev d, int force) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DevFlags *f = d.flags;
+  const int isfirst = f->step == 0;
   const int mx = f->yielded;
   int search = 0;
   if (force) {
@@ -398,8 +399,8 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
   }
 }
 
-void launch_schedule(const Dev &d, hipStream_t st, int isfirst, int force) {
-  k_schedule<<<1, 64, 0, st>>>(d, isfirst, force);
+void launch_schedule(const Dev &d, hipStream_t st, int force) {
+  k_schedule<<<1, 64, 0, st>>>(d, force);
 }
 
 void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
@@ -484,8 +485,35 @@ void launch_accel(const Dev &d, hipStream_t st, int cur) {
 //       [first step] v -= a*0.5*dt ;  v += a*0.5*(dt_prev + dt)
 //   KickDrift (Source/SolverKickDrift.cpp:208-231): v += a*0.5*dt_prev
 
-WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, int isfirst, double dt_prev, double dt) {
+// amplitude::getValAtTime (Source/Domain.h:75-87)
+__device__ inline double amp_value(const DevAmp &a, double t) {
+  int i = a.n - 1;
+  bool end = false;
+  while (!end) {
+    i--;
+    if (i == 0 || t > a.t[i]) end = true;
+  }
+  return a.v[i] + (a.v[i + 1] - a.v[i]) / (a.t[i + 1] - a.t[i]) * (t - a.t[i]);
+}
+
+WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt) {
   __shared__ double sh[kBlock / 64];
+  __shared__ double bcv[kMaxBc][3];
+  __shared__ int bcon[kMaxBc];
+  const int isfirst = d.flags->step == 0;
+  if (threadIdx.x < bc.n) {
+    // velocity of each BC at the step's time (UserAcc, Source/WeldForm.cpp:67-82)
+    const int b = threadIdx.x;
+    int on = 1;
+    double f = 1.0;
+    if (bc.type[b] == 0 && bc.value_type[b] == 1) {
+      if (bc.amp[b] < 0) on = 0;
+      else f = bc.amp_factor[b] * amp_value(d.amps[bc.amp[b]], d.flags->time);
+    }
+    for (int k = 0; k < 3; ++k) bcv[b][k] = bc.value_type[b] == 1 ? f * bc.v[b][k] : bc.v[b][k];
+    bcon[b] = on;
+  }
+  __syncthreads();
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double4 a = make_double4(0, 0, 0, 0);
   double m = 0.0;
@@ -500,12 +528,12 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, int isfirst, double dt_prev
     if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) d.flags->nan = 1;
     bool fixed = false;
     for (int b = 0; b < bc.n; ++b) {
-      if (bc.zone[b] != id) continue;
+      if (bc.zone[b] != id || !bcon[b]) continue;
       if (bc.type[b] == 0) {
         fixed = true;
         for (int k = 0; k < 3; ++k) {
           av[k] = 0.0;
-          v[k] = bc.v[b][k];
+          v[k] = bcv[b][k];
         }
       } else if (bc.type[b] == 2) {
         d.T[i] = bc.T[b];
@@ -550,9 +578,8 @@ WF_KERNEL void k_kick2(Dev d, double dt) {
   if (d.mixed) d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
 }
 
-void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, int isfirst, double dt_prev,
-                 double dt) {
-  k_kick<<<d.nblocks, kBlock, 0, st>>>(d, bc, scheme, isfirst, dt_prev, dt);
+void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt) {
+  k_kick<<<d.nblocks, kBlock, 0, st>>>(d, bc, scheme, dt_prev, dt);
 }
 
 void launch_kick2(const Dev &d, hipStream_t st, double dt) { k_kick2<<<nblk(d.n), kBlock, 0, st>>>(d, dt); }
@@ -620,12 +647,14 @@ void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_
 }
 
 // ---------------------------------------------------------------------------
-// pass 3: strain-rate / rotation-rate gather fused with the per-particle
-// stress update (CalcRateTensors + RateTensorsReduction,
-// Source/InteractionAlt.cpp:263-479; Particle::CalcStressStrain,
-// Source/Particle.cpp:960-1067), plastic work heat (Source/Particle.cpp:922-939),
-// the temperature update of ThermalCalcs (Source/Thermal.cpp:726-734) and
-// the energy terms (Source/Domain.cpp:1752-1829).
+// pass 3: strain-rate / rotation-rate gather (CalcRateTensors +
+// RateTensorsReduction, Source/InteractionAlt.cpp:263-479) with the kinetic
+// energy pair term (Source/Domain.cpp:1752-1808), then the per-particle
+// stress update (Particle::CalcStressStrain, Source/Particle.cpp:960-1067)
+// with plastic work heat (Source/Particle.cpp:922-939), the temperature
+// update of ThermalCalcs (Source/Thermal.cpp:726-734) and the internal
+// energy term (Source/Domain.cpp:1810-1829).  The gather and the update are
+// separate kernels so the gather keeps a small register footprint.
 
 __device__ inline double jc_yield(const DevMat &M, double strain, double sr, double T) {
   const double T_h = (T - M.jc_Tt) / (M.jc_Tm - M.jc_Tt);
@@ -640,17 +669,16 @@ __device__ inline double hol_yield(const DevMat &M, double strain) {
 }
 
 template <bool MX>
-WF_KERNEL void k_rates_stress(Dev d, int buf, double dt, int do_rates, int do_stress) {
+WF_KERNEL void k_rates(Dev d, int buf) {
   using Sr = Src<MX>;
   using R = typename Sr::R;
   __shared__ double sh[kBlock / 64];
   const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const long long np = d.npad;
-  double e_int = 0.0, e_kin = 0.0, e_plw = 0.0, tmax = -INFINITY, tmin = INFINITY;
+  double e_kin = 0.0;
   if (i < d.n) {
     const int g = d.group[i];
     double SR[6], W[3];  // W = (w01, w02, w12)
-    if (do_rates) {
       R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};
       R ekin = 0;
       if (g >= 0) {
@@ -698,11 +726,23 @@ WF_KERNEL void k_rates_stress(Dev d, int buf, double dt, int do_rates, int do_st
       e_kin = (double)ekin;
       for (int k = 0; k < 6; ++k) d.sr[k * np + i] = SR[k];
       for (int k = 0; k < 3; ++k) d.rr[k * np + i] = W[k];
-    } else {
-      for (int k = 0; k < 6; ++k) SR[k] = d.sr[k * np + i];
-      for (int k = 0; k < 3; ++k) W[k] = d.rr[k * np + i];
-    }
-    if (do_stress && g >= 0) {
+  }
+  const double r1 = block_reduce<0>(e_kin, sh);
+  if (threadIdx.x == 0) part(d, P_KIN, blockIdx.x) = r1;
+}
+
+template <bool MX>
+WF_KERNEL void k_stress(Dev d, int buf, double dt) {
+  __shared__ double sh[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const long long np = d.npad;
+  double e_int = 0.0, e_plw = 0.0, tmax = -INFINITY, tmin = INFINITY;
+  if (i < d.n) {
+    const int g = d.group[i];
+    double SR[6], W[3];
+    for (int k = 0; k < 6; ++k) SR[k] = d.sr[k * np + i];
+    for (int k = 0; k < 3; ++k) W[k] = d.rr[k * np + i];
+    if (g >= 0) {
       const DevMat &M = d.mats[d.mat[i]];
       const double rho = d.rho[buf][i];
       const double P = M.P0 + (M.Cs * M.Cs) * (rho - M.ref_density);
@@ -803,24 +843,29 @@ WF_KERNEL void k_rates_stress(Dev d, int buf, double dt, int do_rates, int do_st
     }
   }
   const double r0 = block_reduce<0>(e_int, sh);
-  const double r1 = block_reduce<0>(e_kin, sh);
   const double r2 = block_reduce<0>(e_plw, sh);
   const double r3 = block_reduce<1>(tmax, sh);
   const double r4 = block_reduce<2>(tmin, sh);
   if (threadIdx.x == 0) {
     part(d, P_INT, blockIdx.x) = r0;
-    part(d, P_KIN, blockIdx.x) = r1;
     part(d, P_PLW, blockIdx.x) = r2;
     part(d, P_MAXT, blockIdx.x) = r3;
     part(d, P_MINT, blockIdx.x) = r4;
   }
 }
 
-void launch_rates_stress(const Dev &d, hipStream_t st, int buf, double dt, int do_rates, int do_stress) {
+void launch_rates(const Dev &d, hipStream_t st, int buf) {
   if (d.mixed)
-    k_rates_stress<true><<<d.nblocks, kBlock, 0, st>>>(d, buf, dt, do_rates, do_stress);
+    k_rates<true><<<d.nblocks, kBlock, 0, st>>>(d, buf);
   else
-    k_rates_stress<false><<<d.nblocks, kBlock, 0, st>>>(d, buf, dt, do_rates, do_stress);
+    k_rates<false><<<d.nblocks, kBlock, 0, st>>>(d, buf);
+}
+
+void launch_stress(const Dev &d, hipStream_t st, int buf, double dt) {
+  if (d.mixed)
+    k_stress<true><<<d.nblocks, kBlock, 0, st>>>(d, buf, dt);
+  else
+    k_stress<false><<<d.nblocks, kBlock, 0, st>>>(d, buf, dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -899,7 +944,7 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
 // end of step: fixed-order reduction of the block partials, scalar updates,
 // reaction history, and the end-of-step half of the neighbour schedule.
 
-__global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, long long step, int nz, int full_step) {
+__global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
   __shared__ double sh[16];
   const int nb = d.nblocks;
   double res[P_COUNT];
@@ -916,7 +961,8 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, long long s
     }
     res[s] = op == 0 ? block_reduce<0>(v, sh) : (op == 1 ? block_reduce<1>(v, sh) : block_reduce<2>(v, sh));
   }
-  if (threadIdx.x != 0 || !full_step) return;
+  if (threadIdx.x != 0) return;
+  const long long step = d.flags->step;
   d.scalars[S_KIN] += res[P_KIN] * dt;
   d.scalars[S_INT] += res[P_INT] * dt;
   if (d.pl_heating) d.scalars[S_PLW] += res[P_PLW] * dt;
@@ -934,10 +980,12 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, long long s
     f->ts_i++;
     if (f->ts_i > f->ts_nb_inc - 1) f->ts_i = 0;
   }
+  f->step = step + 1;
+  f->time += dt;
 }
 
-void launch_finalize(const Dev &d, hipStream_t st, double dt, long long step, int nz, int full_step) {
-  k_finalize<<<1, 1024, 0, st>>>(d, dt, step, nz, full_step);
+void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
+  k_finalize<<<1, 1024, 0, st>>>(d, dt, nz);
 }
 
 // ---------------------------------------------------------------------------
