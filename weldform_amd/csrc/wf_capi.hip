@@ -370,7 +370,6 @@ void allocate(Ctx *c) {
   if (d.mixed) {
     for (int b = 0; b < 2; ++b) {
       d.xhf[b] = c->alloc<float4>(np);
-      d.vrf[b] = c->alloc<float2>(np);
     }
     d.vmf = c->alloc<float4>(np);
     d.srecf = c->alloc<StressRecF>(np);
@@ -945,6 +944,16 @@ int wf_init(void *ctx, double dt) {
     if (c->cfg.thermal_solver)
       for (int64_t i = 0; i < c->n; ++i)
         if (!(c->mats[c->hi[WF_I_MATERIAL][i]].cp_T > 0)) throw Error(WF_EINVAL, "thermal solver needs cp_T > 0");
+    if (c->cfg.precision == WF_MIXED) {
+      // the FP32 gather records carry m/rho in place of h (see Src<true>)
+      std::vector<double> hm(kMaxMat, -1.0);
+      for (int64_t i = 0; i < c->n; ++i) {
+        double &r = hm[c->hi[WF_I_MATERIAL][i]];
+        if (r < 0) r = c->hf[WF_F_H][i];
+        else if (r != c->hf[WF_F_H][i])
+          throw Error(WF_EUNSUPPORTED, "mixed precision needs one smoothing length per material");
+      }
+    }
     HIPCHK(hipSetDevice(c->cfg.device));
     cell_initiate(c);
     make_permutation(c);

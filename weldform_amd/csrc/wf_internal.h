@@ -135,9 +135,8 @@ struct Dev {
   // mixed-precision shadows of the gathered inputs (written by the FP64
   // integrators); positions are stored relative to xc to halve their error
   double xc[3];
-  float4 *xhf[2];              // (x - xc, y - yc, z - zc, h)
+  float4 *xhf[2];              // (x - xc, y - yc, z - zc, mass/rho)
   float4 *vmf;                 // (vx, vy, vz, mass)
-  float2 *vrf[2];              // (mass/rho, rho)
   StressRecF *srecf;
   // neighbours
   int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *scan_tmp;

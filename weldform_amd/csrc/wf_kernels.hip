@@ -164,7 +164,11 @@ __device__ inline R gk_t(R r2, const KH<R> &k) {
   return (R)0;
 }
 
-// what a gather reads of particle j, from the FP64 state or its FP32 shadows
+// What a gather reads of particle j: the FP64 state, or its FP32 shadows.
+// FP64:  x = (x, y, z, h), v = (vx, vy, vz, m), m/rho and rho separate.
+// FP32:  x = (x, y, z, m/rho), v = (vx, vy, vz, m), rho = m / (m/rho); the
+//        smoothing length is uniform within a material (checked at set-up),
+//        so the pair h is the particle's own and needs no gathered load.
 template <bool MX>
 struct Src;
 template <>
@@ -173,8 +177,10 @@ struct Src<false> {
   using R4 = double4;
   __device__ static R4 x(const Dev &d, int b, int j) { return d.xh[b][j]; }
   __device__ static R4 v(const Dev &d, int j) { return d.vm[j]; }
-  __device__ static R vol(const Dev &d, int b, int j) { return d.vol[b][j]; }
-  __device__ static R rho(const Dev &d, int b, int j) { return d.rho[b][j]; }
+  __device__ static R h_own(const Dev &d, int b, int i, const R4 &xi) { return xi.w; }
+  __device__ static KH<R> kh(const KH<R> &ki, const R4 &xi, const R4 &xj) { return pair_h_t<R>(ki, xi.w, xj.w); }
+  __device__ static R vol(const Dev &d, int b, int j, const R4 &xj) { return d.vol[b][j]; }
+  __device__ static R rho(const Dev &d, int b, int j, const R4 &xj, const R4 &vj) { return d.rho[b][j]; }
   __device__ static void srec(const Dev &d, int j, R s[6], R &rho, R &irho2) {
     const StressRec r = d.srec[j];
     for (int k = 0; k < 6; ++k) s[k] = r.s[k];
@@ -188,8 +194,10 @@ struct Src<true> {
   using R4 = float4;
   __device__ static R4 x(const Dev &d, int b, int j) { return d.xhf[b][j]; }
   __device__ static R4 v(const Dev &d, int j) { return d.vmf[j]; }
-  __device__ static R vol(const Dev &d, int b, int j) { return d.vrf[b][j].x; }
-  __device__ static R rho(const Dev &d, int b, int j) { return d.vrf[b][j].y; }
+  __device__ static R h_own(const Dev &d, int b, int i, const R4 &xi) { return (float)d.xh[b][i].w; }
+  __device__ static KH<R> kh(const KH<R> &ki, const R4 &xi, const R4 &xj) { return ki; }
+  __device__ static R vol(const Dev &d, int b, int j, const R4 &xj) { return xj.w; }
+  __device__ static R rho(const Dev &d, int b, int j, const R4 &xj, const R4 &vj) { return vj.w * rcp_t(xj.w); }
   __device__ static void srec(const Dev &d, int j, R s[6], R &rho, R &irho2) {
     const StressRecF r = d.srecf[j];
     for (int k = 0; k < 6; ++k) s[k] = r.s[k];
@@ -434,7 +442,7 @@ WF_KERNEL void k_accel(Dev d, int cur) {
     S::srec(d, i, si, rhoi, irhoi);
     R ti[6];
     for (int k = 0; k < 6; ++k) ti[k] = si[k] * irhoi;
-    const KH<R> ki = kern_h_t<R>(xi.w);
+    const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
     const int cnt = d.nnb[i];
     const int *nb = d.nbr + i;
     for (int k = 0; k < cnt; ++k) {
@@ -443,7 +451,7 @@ WF_KERNEL void k_accel(Dev d, int cur) {
       const auto vj = S::v(d, j);
       R sj[6], rhoj, irhoj;
       S::srec(d, j, sj, rhoj, irhoj);
-      const KH<R> kh = pair_h_t<R>(ki, xi.w, xj.w);
+      const KH<R> kh = S::kh(ki, xi, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R r2 = dx * dx + dy * dy + dz * dz;
       const R GK = gk_t<R>(r2, kh);
@@ -600,16 +608,16 @@ WF_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
   if (d.group[i] >= 0) {
     const auto xi = S::x(d, cur, i);
     const auto vi = S::v(d, i);
-    const KH<R> ki = kern_h_t<R>(xi.w);
+    const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
     const int cnt = d.nnb[i];
     const int *nb = d.nbr + i;
     for (int k = 0; k < cnt; ++k) {
       const int j = nb[(long long)k * d.npad];
       const auto xj = S::x(d, cur, j);
       const auto vj = S::v(d, j);
-      const R volj = S::vol(d, cur, j);
+      const R volj = S::vol(d, cur, j, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-      const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, pair_h_t<R>(ki, xi.w, xj.w));
+      const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, S::kh(ki, xi, xj));
       const R t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
       s += volj * t;
     }
@@ -634,8 +642,7 @@ WF_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
   const double4 xn = make_double4(xd.x + ux, xd.y + uy, xd.z + uz, xd.w);
   d.xh[nxt][i] = xn;
   if (MX) {
-    d.vrf[nxt][i] = make_float2((float)vn, (float)rn);
-    d.xhf[nxt][i] = make_float4((float)(xn.x - d.xc[0]), (float)(xn.y - d.xc[1]), (float)(xn.z - d.xc[2]), (float)xn.w);
+    d.xhf[nxt][i] = make_float4((float)(xn.x - d.xc[0]), (float)(xn.y - d.xc[1]), (float)(xn.z - d.xc[2]), (float)vn);
   }
 }
 
@@ -685,22 +692,22 @@ WF_KERNEL void k_rates(Dev d, int buf) {
         const DevMat &M = d.mats[d.mat[i]];
         const auto xi = Sr::x(d, buf, i);
         const auto vi = Sr::v(d, i);
-        const R rhoi = Sr::rho(d, buf, i);
+        const R rhoi = Sr::rho(d, buf, i, xi, vi);
         const R P0 = (R)M.P0, cs2 = (R)(M.Cs * M.Cs), rho0 = (R)M.ref_density;
         const R irhoi = rcp_t(rhoi);
         const R pri = (P0 + cs2 * (rhoi - rho0)) * irhoi * irhoi;
         const R mi = vi.w;
-        const KH<R> ki = kern_h_t<R>(xi.w);
+        const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi));
         const int cnt = d.nnb[i];
         const int *nb = d.nbr + i;
         for (int k = 0; k < cnt; ++k) {
           const int j = nb[(long long)k * np];
           const auto xj = Sr::x(d, buf, j);
           const auto vj = Sr::v(d, j);
-          const R volj = Sr::vol(d, buf, j);
-          const R rhoj = Sr::rho(d, buf, j);
+          const R volj = Sr::vol(d, buf, j, xj);
+          const R rhoj = Sr::rho(d, buf, j, xj, vj);
           const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-          const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, pair_h_t<R>(ki, xi.w, xj.w));
+          const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, Sr::kh(ki, xi, xj));
           const R vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
           const R c = volj * ((R)-0.5 * GK);
           sr[XX] += c * ((R)2 * vx * dx);
@@ -1013,8 +1020,7 @@ WF_KERNEL void k_shadows(Dev d) {
   if (i >= d.n) return;
   for (int b = 0; b < 2; ++b) {
     const double4 x = d.xh[b][i];
-    d.xhf[b][i] = make_float4((float)(x.x - d.xc[0]), (float)(x.y - d.xc[1]), (float)(x.z - d.xc[2]), (float)x.w);
-    d.vrf[b][i] = make_float2((float)d.vol[b][i], (float)d.rho[b][i]);
+    d.xhf[b][i] = make_float4((float)(x.x - d.xc[0]), (float)(x.y - d.xc[1]), (float)(x.z - d.xc[2]), (float)d.vol[b][i]);
   }
   const double4 v = d.vm[i];
   d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
