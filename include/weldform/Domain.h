@@ -1,0 +1,534 @@
+// weldform/Domain.h -- drop-in SPH::Domain for WeldForm drivers, running the
+// explicit solid step on an MI355X through the C-ABI of weldform_hip.h.
+//
+// Kept from the reference (Source/Domain.h:113-537): class and method names,
+// argument meanings and the public members drivers touch -- Dimension,
+// Nproc, Particles, Kernel_Set, Gradient_Approach_Set, AddCylinderLength,
+// AddBoxLength, AssignZone, DomMax/DomMin, bConds, amps, auto_ts,
+// ts_nb_inc, nonlock_sum, gradKernelCorr, thermal_solver, pl_heating,
+// pl_work_heat_frac, energy sums, getTime(), getStepSize(), WriteCSV and the
+// solver entry points SolveDiffUpdateLeapFrog / SolveDiffUpdateLeapfrog /
+// SolveDiffUpdateKickDrift (tf, dt, dtOut, FileKey, maxidx).
+//
+// Boundary conditions are declarative: the velocity and temperature BCs in
+// bConds (with amps) are applied on the device where the reference calls
+// GeneralAfter = UserAcc (Source/WeldForm.cpp:51-120).  An arbitrary host
+// callback in GeneralBefore / GeneralAfter cannot run inside a device step
+// and is rejected.  Errors are thrown as Fatal*, as in the reference.
+//
+// Link with -lweldform_hip (weldform_amd/lib); compiles with any C++17
+// compiler, no HIP headers needed.
+#pragma once
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../weldform_hip.h"
+#include "Material.h"
+#include "Particle.h"
+#include "matvec.h"
+
+namespace SPH {
+
+enum Kernels_Type { Qubic_Spline = 0, Quintic = 1, Quintic_Spline = 2, Hyperbolic_Spline = 3 };
+enum Gradient_Type { Squared_density = 0, Multiplied_density = 1, Randles_Libersky = 2 };
+enum BC_TYPE { Velocity_BC = 0, Force_BC, Temperature_BC, Convection_BC, Symmetry_BC };
+enum class Precision { FP64 = WF_FP64, Mixed = WF_MIXED };
+
+// amplitude table (Source/Domain.h:69-89)
+struct amplitude {
+  int id = 0;
+  std::vector<double> time, value;
+};
+
+// boundary condition (Source/Domain.h:98-111)
+struct boundaryCondition {
+  int zoneId = 0;
+  int type = Velocity_BC;
+  bool free = true;
+  int valueType = 0;  // 0 constant, 1 amplitude
+  Vec3_t value;
+  Vec3_t value_ang;
+  int ampId = 0;
+  double ampFactor = 1.0;
+  double cv_coeff = 0.0, T_inf = 0.0, T = 0.0;
+};
+
+class Domain;
+inline void General(Domain &) {}
+
+class Domain {
+ public:
+  typedef void (*PtDom)(Domain &dom);
+
+  Domain() {
+    DomMax = Vec3_t(-100000000000.0);
+    DomMin = Vec3_t(100000000000.0);
+  }
+  ~Domain() {
+    close();
+    for (size_t i = 0; i < Particles.Size(); ++i) delete Particles[i];
+  }
+  Domain(const Domain &) = delete;
+  Domain &operator=(const Domain &) = delete;
+
+  // ------------------------------------------------------------ settings
+  int Dimension = 2;                 // reference default; drivers set 3
+  size_t Nproc = 1;                  // accepted, unused (one GPU)
+  Array<Particle *> Particles;
+  Vec3_t DomMax, DomMin;
+  PtDom GeneralBefore = &General;
+  PtDom GeneralAfter = &General;
+  std::vector<boundaryCondition> bConds;
+  std::vector<amplitude> amps;
+  bool auto_ts = true, auto_ts_acc = false, auto_ts_cont = false;
+  int ts_nb_inc = 5;
+  bool nonlock_sum = true;           // sums are fixed-order gathers either way
+  bool gradKernelCorr = false;
+  bool thermal_solver = false;
+  bool pl_heating = false;
+  bool contact = false;
+  double pl_work_heat_frac = 0.9;
+  int solid_part_count = -1;
+  double kin_energy_sum = 0.0, int_energy_sum = 0.0, plastic_work = 0.0;
+  double m_maxT = 0.0, m_minT = 0.0;
+  // device-path extensions
+  Precision precision = Precision::FP64;
+  int device = 0;
+  int max_nb = 128;
+  size_t Scheme = 0;   // accepted, unused: the solver entry point picks the scheme
+  double CFL = 0.3;   // accepted, unused while auto_ts is false
+
+  void Kernel_Set(Kernels_Type const &KT) {
+    KernelType = KT;
+    Cellfac = (KT == Quintic_Spline) ? 3.0 : 2.0;
+  }
+  void Gradient_Approach_Set(Gradient_Type const &GT) { GradientType = GT; }
+  double &getTime() { return Time; }
+  const double &getStepSize() const { return deltat; }
+  double getCellfac() const { return Cellfac; }
+
+  // ------------------------------------------------------------ geometry
+  // AddCylinderLength (Source/Domain.cpp:708-860): cubic lattice, mass
+  // pi R^2 L rho / N over the particles in the domain
+  void AddCylinderLength(int tag, Vec3_t const &V, double Rxy, double Lz, double r, double Density, double h,
+                         bool Fixed, bool ghost = false) {
+    if (Dimension != 3) throw new Fatal("AddCylinderLength: only the 3D generator is available");
+    if (ghost) throw new Fatal("AddCylinderLength: ghost symmetry rows are not supported on the device path");
+    const double Vv[3] = {V(0), V(1), V(2)};
+    const int64_t n = wf_gen_cylinder(Vv, Rxy, Lz, r, nullptr, 0);
+    if (n <= 0) throw new Fatal("AddCylinderLength: no particles generated");
+    std::vector<double> xs(3 * (size_t)n);
+    wf_gen_cylinder(Vv, Rxy, Lz, r, xs.data(), n);
+    for (int64_t i = 0; i < n; ++i)
+      Particles.Push(new Particle(tag, Vec3_t(xs[3 * i], xs[3 * i + 1], xs[3 * i + 2]), Vec3_t(0, 0, 0), 0.0,
+                                  Density, h, Fixed));
+    solid_part_count = (int)Particles.Size();
+    const double Vol = M_PI * Rxy * Rxy * Lz;
+    const double Mass = Vol * Density / Particles.Size();
+    for (size_t i = 0; i < Particles.Size(); ++i) Particles[i]->Mass = Mass;
+    R = r;
+  }
+
+  // AddBoxLength, 3D cubic (type 1) and hexagonal close packing (type 0),
+  // Source/Domain.cpp:326-460; mass = box volume * rho / N
+  void AddBoxLength(int tag, Vec3_t const &V, double Lx, double Ly, double Lz, double r, double Density, double h,
+                    int type, int rotation, bool random, bool Fixed) {
+    if (Dimension != 3) throw new Fatal("AddBoxLength: only the 3D generator is available");
+    if (!(type == 0 || type == 1) || !(rotation == 0 || rotation == 90))
+      throw new Fatal("AddBoxLength: packing type must be 0 or 1 and rotation 0 or 90");
+    if (random) throw new Fatal("AddBoxLength: randomised packing is not supported on the device path");
+    const size_t pre = Particles.Size();
+    size_t k = 0;
+    double zp = V(2);
+    while (zp <= V(2) + Lz - r) {
+      size_t j = 0;
+      double yp = V(1);
+      while (yp <= V(1) + Ly - r) {
+        size_t i = 0;
+        double xp = V(0);
+        while (xp <= V(0) + Lx - r) {
+          double x, y, z;
+          if (type == 0) {
+            const bool odd = (k % 2 != 0) && (j % 2 != 0);
+            x = V(0) + (2 * i + (j % 2) + (k % 2) + (odd ? -1 : 1)) * r;
+            y = V(1) + (std::sqrt(3.0) * (j + (1.0 / 3.0) * (k % 2)) + 1) * r;
+            z = V(2) + ((2 * std::sqrt(6.0) / 3) * k + 1) * r;
+          } else {
+            x = V(0) + (2.0 * i + 1) * r;
+            y = V(1) + (2.0 * j + 1) * r;
+            z = V(2) + (2.0 * k + 1) * r;
+          }
+          Particles.Push(new Particle(tag, Vec3_t(x, y, z), Vec3_t(0, 0, 0), 0.0, Density, h, Fixed));
+          i++;
+          if (type == 0) {
+            const bool odd = (k % 2 != 0) && (j % 2 != 0);
+            xp = V(0) + (2 * i + (j % 2) + (k % 2) + (odd ? -1 : 1)) * r;
+          } else {
+            xp = V(0) + (2 * i + 1) * r;
+          }
+        }
+        j++;
+        yp = (type == 0) ? V(1) + (std::sqrt(3.0) * (j + (1.0 / 3.0) * (k % 2)) + 1) * r : V(1) + (2.0 * j + 1) * r;
+      }
+      k++;
+      zp = (type == 0) ? V(2) + ((2 * std::sqrt(6.0) / 3) * k + 1) * r : V(2) + (2.0 * k + 1) * r;
+    }
+    Vec3_t Max = V;
+    for (size_t i = pre; i < Particles.Size(); ++i)
+      for (int q = 0; q < 3; ++q)
+        if (Particles[i]->x(q) > Max(q)) Max(q) = Particles[i]->x(q);
+    Max += Vec3_t(r);
+    const Vec3_t ext = Max - V;
+    const double Mass = ext(0) * ext(1) * ext(2) * Density / (double)(Particles.Size() - pre);
+    for (size_t i = pre; i < Particles.Size(); ++i) Particles[i]->Mass = Mass;
+    solid_part_count = (int)Particles.Size();
+    R = r;
+  }
+
+  // AssignZone (Source/Domain.cpp:691-706)
+  int AssignZone(Vec3_t &start, Vec3_t &end, int &id) {
+    int cnt = 0;
+    for (size_t a = 0; a < Particles.Size(); ++a) {
+      bool in = true;
+      for (int i = 0; i < 3; ++i)
+        if (Particles[a]->x(i) < start[i] || Particles[a]->x(i) > end[i]) in = false;
+      if (in) {
+        Particles[a]->ID = id;
+        cnt++;
+      }
+    }
+    return cnt;
+  }
+
+  // ------------------------------------------------------------ solvers
+  void SolveDiffUpdateLeapFrog(double tf, double dt, double dtOut, char const *TheFileKey, size_t maxidx) {
+    solve(tf, dt, dtOut, TheFileKey, maxidx, WF_LEAPFROG);
+  }
+  // declared (lower-case f) by Source/Domain.h:198 and called by older drivers
+  void SolveDiffUpdateLeapfrog(double tf, double dt, double dtOut, char const *TheFileKey, size_t maxidx) {
+    solve(tf, dt, dtOut, TheFileKey, maxidx, WF_LEAPFROG);
+  }
+  void SolveDiffUpdateKickDrift(double tf, double dt, double dtOut, char const *TheFileKey, size_t maxidx) {
+    solve(tf, dt, dtOut, TheFileKey, maxidx, WF_KICKDRIFT);
+  }
+
+  // reaction sum(m a) of a velocity-BC zone at the last step (taken before the
+  // BC zeroes a), and its history over the solve
+  Vec3_t ZoneReaction(int zone) const {
+    auto it = reaction_.find(zone);
+    if (it == reaction_.end() || it->second.empty()) return Vec3_t();
+    return it->second.back();
+  }
+  const std::vector<Vec3_t> &ZoneReactionHistory(int zone) { return reaction_[zone]; }
+
+  // XDMF/HDF5 output is not built; the particle table goes to key.csv instead
+  void WriteXDMF(char const *FileKey) { WriteCSV(FileKey); }
+
+  // WriteCSV (Source/Output.cpp:521-554): same columns; contact forces are 0
+  void WriteCSV(char const *FileKey) {
+    std::ostringstream oss;
+    oss << "X, Y, Z, ID, Sigma_eq, Pl_Strain, vx, vy, vz, ax, ay, az, CFx, CFy, CZFz, p, Temp" << std::endl;
+    for (size_t i = 0; i < Particles.Size(); ++i) {
+      Particle *P = Particles[i];
+      for (int j = 0; j < 3; ++j) oss << P->x(j) << ", ";
+      P->CalculateEquivalentStress();
+      oss << P->ID << ", " << P->Sigma_eq << ", " << P->pl_strain << ", " << P->v(0) << ", " << P->v(1) << ", "
+          << P->v(2) << ", " << P->a(0) << ", " << P->a(1) << ", " << P->a(2) << ", " << 0.0 << ", " << 0.0
+          << ", " << 0.0 << ", " << P->Pressure << ", " << P->T << std::endl;
+    }
+    std::ofstream of(std::string(FileKey) + ".csv", std::ios::out);
+    of << oss.str();
+  }
+
+ private:
+  double Time = 0.0, deltat = 0.0, Cellfac = 2.0, R = 0.0;
+  int KernelType = 0, GradientType = 0;
+  void *ctx_ = nullptr;
+  std::map<int, std::vector<Vec3_t>> reaction_;
+  // steps per wf_step call: the device keeps the last 65536 reaction entries
+  static constexpr int64_t kMaxChunk = 1 << 16;
+
+  void close() {
+    if (ctx_) wf_destroy(ctx_);
+    ctx_ = nullptr;
+  }
+  void check(int rc) {
+    if (rc != WF_OK) throw new Fatal(std::string("weldform_hip: ") + wf_last_error(ctx_));
+  }
+
+  void check_supported() {
+    const size_t n = Particles.Size();
+    if (n == 0) throw new Fatal("Particle Count is Null. Please Check Radius and Domain Dimensions.");
+    if (Dimension != 3) throw new Fatal("the device path is 3D only (set Dimension = 3)");
+    if (KernelType != Qubic_Spline) throw new Fatal("the device path implements the cubic spline kernel only");
+    if (GradientType != Squared_density) throw new Fatal("the device path implements GradientType 0 only");
+    if (gradKernelCorr) throw new Fatal("kernel gradient correction is not implemented on the device path");
+    if (auto_ts || auto_ts_acc)
+      throw new Fatal("adaptive time stepping is not implemented on the device path: set auto_ts = false");
+    if (contact) throw new Fatal("rigid contact is not implemented on the device path of this build");
+    if (GeneralBefore != &General || GeneralAfter != &General)
+      throw new Fatal("host callbacks cannot run inside a device step: express BCs through bConds / amps");
+    for (size_t i = 0; i < n; ++i)
+      if (Particles[i]->TI > 0.0) throw new Fatal("tensile instability (TI > 0) is not implemented");
+  }
+
+  // one wf_material per Particle::Material id; particles sharing an id must
+  // share their parameters, since pairs form within one id
+  wf_material material_of(const Particle *P) {
+    wf_material m;
+    std::memset(&m, 0, sizeof m);
+    m.model = P->Material_model;
+    m.G = P->G;
+    m.Cs = P->Cs;
+    m.ref_density = P->RefDensity;
+    m.P0 = P->P0;
+    m.alpha = P->Alpha;
+    m.beta = P->Beta;
+    m.Ep = P->Ep;
+    m.k_T = P->k_T;
+    m.cp_T = P->cp_T;
+    m.th_exp = P->th_exp;
+    if (P->mat) {
+      m.E = P->mat->Elastic().E();
+      m.nu = P->mat->Elastic().nu();
+    }
+    if (P->Material_model == HOLLOMON) {
+      const Hollomon *H = dynamic_cast<const Hollomon *>(P->mat);
+      if (!H) throw new Fatal("Hollomon particle without a Hollomon material");
+      m.hol_K = H->K;
+      m.hol_m = H->m;
+      m.hol_sy0 = H->sy0;
+    } else if (P->Material_model == JOHNSON_COOK) {
+      JohnsonCook *J = dynamic_cast<JohnsonCook *>(P->mat);
+      if (!J) throw new Fatal("Johnson-Cook particle without a JohnsonCook material");
+      m.jc_A = J->A;
+      m.jc_B = J->B;
+      m.jc_n = J->n;
+      m.jc_C = J->C;
+      m.jc_eps0 = J->eps_0;
+      m.jc_m = J->m;
+      m.jc_Tm = J->getT_m();
+      m.jc_Tt = J->getT_t();
+    } else if (P->Material_model != BILINEAR) {
+      throw new Fatal("material model not implemented on the device path");
+    }
+    if (P->Fail != 1) throw new Fatal("only the Fail = 1 (plastic) stress update is implemented");
+    return m;
+  }
+
+  std::vector<int> build_materials(std::vector<wf_material> &table) {
+    const size_t n = Particles.Size();
+    std::map<int, int> idx;
+    table.clear();
+    std::vector<int> mi(n);
+    for (size_t i = 0; i < n; ++i) {
+      const Particle *P = Particles[i];
+      const wf_material m = material_of(P);
+      auto it = idx.find(P->Material);
+      if (it == idx.end()) {
+        if (table.size() == 256) throw new Fatal("more than 256 material ids");
+        it = idx.emplace(P->Material, (int)table.size()).first;
+        table.push_back(m);
+      } else if (std::memcmp(&table[it->second], &m, sizeof m) != 0) {
+        throw new Fatal("particles of one Material id carry different parameters");
+      }
+      mi[i] = it->second;
+    }
+    return mi;
+  }
+
+  void upload() {
+    const size_t n = Particles.Size();
+    std::vector<double> v3(3 * n), v1(n), v9(9 * n);
+    auto put3 = [&](int f, Vec3_t Particle::*m) {
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) v3[3 * i + k] = (Particles[i]->*m)(k);
+      check(wf_set_field(ctx_, f, v3.data()));
+    };
+    auto put1 = [&](int f, double Particle::*m) {
+      for (size_t i = 0; i < n; ++i) v1[i] = Particles[i]->*m;
+      check(wf_set_field(ctx_, f, v1.data()));
+    };
+    auto put9 = [&](int f, Mat3_t Particle::*m) {
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 9; ++k) v9[9 * i + k] = (Particles[i]->*m).m[k];
+      check(wf_set_field(ctx_, f, v9.data()));
+    };
+    put3(WF_F_X, &Particle::x);
+    put3(WF_F_V, &Particle::v);
+    put3(WF_F_A, &Particle::a);
+    put3(WF_F_DISP, &Particle::Displacement);
+    put1(WF_F_DENSITY, &Particle::Density);
+    put1(WF_F_MASS, &Particle::Mass);
+    put1(WF_F_H, &Particle::h);
+    put9(WF_F_SIGMA, &Particle::Sigma);
+    put9(WF_F_SHEAR, &Particle::ShearStress);
+    put9(WF_F_STRAIN, &Particle::Strain);
+    put9(WF_F_STRAIN_PL, &Particle::Strain_pl);
+    put9(WF_F_STRAIN_PL_INCR, &Particle::Strain_pl_incr);
+    put1(WF_F_PL_STRAIN, &Particle::pl_strain);
+    put1(WF_F_DPL_STRAIN, &Particle::delta_pl_strain);
+    put1(WF_F_SIGMAY, &Particle::Sigmay);
+    put1(WF_F_T, &Particle::T);
+    put1(WF_F_DTDT, &Particle::dTdt);
+    put1(WF_F_Q_SOURCE, &Particle::q_source);
+    put1(WF_F_H_CONV, &Particle::h_conv);
+    put1(WF_F_T_INF, &Particle::T_inf);
+    put1(WF_F_Q_PLHEAT, &Particle::q_plheat);
+    put1(WF_F_PS_ENERGY, &Particle::ps_energy);
+    std::vector<int32_t> iv(n);
+    for (size_t i = 0; i < n; ++i) iv[i] = Particles[i]->ID;
+    check(wf_set_int_field(ctx_, WF_I_ID, iv.data()));
+    for (size_t i = 0; i < n; ++i) iv[i] = Particles[i]->IsFree ? 1 : 0;
+    check(wf_set_int_field(ctx_, WF_I_FREE, iv.data()));
+    for (size_t i = 0; i < n; ++i) iv[i] = Particles[i]->Thermal_BC;
+    check(wf_set_int_field(ctx_, WF_I_THERMAL_BC, iv.data()));
+  }
+
+  void download() {
+    const size_t n = Particles.Size();
+    std::vector<double> v3(3 * n), v1(n), v9(9 * n);
+    auto get3 = [&](int f, Vec3_t Particle::*m) {
+      check(wf_get_field(ctx_, f, v3.data()));
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) (Particles[i]->*m)(k) = v3[3 * i + k];
+    };
+    auto get1 = [&](int f, double Particle::*m) {
+      check(wf_get_field(ctx_, f, v1.data()));
+      for (size_t i = 0; i < n; ++i) Particles[i]->*m = v1[i];
+    };
+    auto get9 = [&](int f, Mat3_t Particle::*m) {
+      check(wf_get_field(ctx_, f, v9.data()));
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 9; ++k) (Particles[i]->*m).m[k] = v9[9 * i + k];
+    };
+    get3(WF_F_X, &Particle::x);
+    get3(WF_F_V, &Particle::v);
+    get3(WF_F_A, &Particle::a);
+    get3(WF_F_DISP, &Particle::Displacement);
+    get1(WF_F_DENSITY, &Particle::Density);
+    get1(WF_F_DDENSITY, &Particle::dDensity);
+    get9(WF_F_SIGMA, &Particle::Sigma);
+    get9(WF_F_SHEAR, &Particle::ShearStress);
+    get9(WF_F_STRAIN_RATE, &Particle::StrainRate);
+    get9(WF_F_ROT_RATE, &Particle::RotationRate);
+    get9(WF_F_STRAIN, &Particle::Strain);
+    get9(WF_F_STRAIN_PL, &Particle::Strain_pl);
+    get9(WF_F_STRAIN_PL_INCR, &Particle::Strain_pl_incr);
+    get1(WF_F_PL_STRAIN, &Particle::pl_strain);
+    get1(WF_F_DPL_STRAIN, &Particle::delta_pl_strain);
+    get1(WF_F_SIGMAY, &Particle::Sigmay);
+    get1(WF_F_EFF_SR, &Particle::eff_strain_rate);
+    get1(WF_F_PRESSURE, &Particle::Pressure);
+    get1(WF_F_T, &Particle::T);
+    get1(WF_F_DTDT, &Particle::dTdt);
+    get1(WF_F_Q_PLHEAT, &Particle::q_plheat);
+    get1(WF_F_Q_CONV, &Particle::q_conv);
+    get1(WF_F_PS_ENERGY, &Particle::ps_energy);
+    std::vector<int32_t> nb(n);
+    check(wf_get_int_field(ctx_, WF_I_NB, nb.data()));
+    for (size_t i = 0; i < n; ++i) {
+      Particles[i]->Nb = nb[i];
+      Particles[i]->CalculateEquivalentStress();
+    }
+    wf_scalars s{};
+    check(wf_get_scalars(ctx_, &s));
+    kin_energy_sum = s.kin_energy_sum;
+    int_energy_sum = s.int_energy_sum;
+    plastic_work = s.plastic_work;
+    m_maxT = s.max_T;
+    m_minT = s.min_T;
+  }
+
+  void pull_reactions(int64_t nsteps) {
+    std::vector<double> h(3 * (size_t)nsteps);
+    for (const auto &b : bConds) {
+      if (b.type != Velocity_BC) continue;
+      int64_t got = 0;
+      check(wf_get_reaction_history(ctx_, b.zoneId, h.data(), nsteps, &got));
+      auto &dst = reaction_[b.zoneId];
+      for (int64_t s = 0; s < got; ++s) dst.push_back(Vec3_t(h[3 * s], h[3 * s + 1], h[3 * s + 2]));
+    }
+  }
+
+  // SolveDiffUpdateLeapFrog / KickDrift (Source/SolverLeapfrog.cpp:24-451,
+  // Source/SolverKickDrift.cpp:23-401): the loop runs while Time <= tf and
+  // idx_out <= maxidx and writes output after the step that reaches tout.
+  // Time advances on the host with the same arithmetic, so the device runs
+  // whole stretches of steps between outputs without synchronising.
+  void solve(double tf, double dt, double dtOut, char const *key, size_t maxidx, int scheme) {
+    check_supported();
+    std::vector<wf_material> table;
+    const std::vector<int> mi = build_materials(table);
+    close();
+    const size_t n = Particles.Size();
+    wf_config cfg{};
+    cfg.precision = (int32_t)precision;
+    cfg.ts_nb_inc = ts_nb_inc;
+    cfg.max_nb = max_nb;
+    cfg.thermal_solver = thermal_solver;
+    cfg.pl_heating = pl_heating;
+    cfg.device = device;
+    cfg.cellfac = Cellfac;
+    cfg.pl_work_heat_frac = pl_work_heat_frac;
+    for (int k = 0; k < 3; ++k) {
+      cfg.dom_max[k] = DomMax(k);
+      cfg.dom_min[k] = DomMin(k);
+    }
+    if (wf_create(&cfg, (int64_t)n, &ctx_) != WF_OK) throw new Fatal(std::string("weldform_hip: ") + wf_last_error(nullptr));
+    for (size_t k = 0; k < table.size(); ++k) check(wf_set_material(ctx_, (int32_t)k, &table[k]));
+    check(wf_set_int_field(ctx_, WF_I_MATERIAL, mi.data()));
+    upload();
+    for (const auto &a : amps)
+      check(wf_add_amplitude(ctx_, a.id, (int32_t)a.time.size(), a.time.data(), a.value.data()));
+    for (const auto &b : bConds) {
+      if (b.type == Velocity_BC) {
+        const double v[3] = {b.value(0), b.value(1), b.value(2)};
+        check(wf_add_velocity_bc(ctx_, b.zoneId, b.valueType, v, b.ampId, b.ampFactor));
+      } else if (b.type == Temperature_BC) {
+        check(wf_add_temperature_bc(ctx_, b.zoneId, b.T));
+      }
+    }
+    deltat = dt;
+    check(wf_init(ctx_, dt));
+    reaction_.clear();
+    size_t idx_out = 1;
+    double tout = Time;
+    while (Time <= tf && idx_out <= maxidx) {
+      int64_t k = 0;
+      double t = Time;
+      bool out = false;
+      while (t <= tf && k < kMaxChunk) {
+        t += deltat;
+        ++k;
+        if (t >= tout) {
+          out = true;
+          break;
+        }
+      }
+      check(wf_step(ctx_, scheme, k));
+      check(wf_sync(ctx_));
+      pull_reactions(k);
+      Time = t;
+      if (out) {
+        if (key) {
+          download();
+          char fn[4096];
+          std::snprintf(fn, sizeof fn, "%s_%04zu", key, idx_out);
+          WriteCSV(fn);
+        }
+        idx_out++;
+        tout += dtOut;
+      }
+    }
+    download();
+  }
+};
+
+}  // namespace SPH

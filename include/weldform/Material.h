@@ -1,0 +1,68 @@
+// weldform/Material.h -- material classes with the constructors of the
+// reference's Source/Material.h:64-151,202-220 (Elastic_, Material_,
+// JohnsonCook, Hollomon), holding the parameters the device path needs.
+// Yield and tangent laws themselves run on the device.
+#pragma once
+#include <cmath>
+
+#include "matvec.h"
+
+class Elastic_ {
+ public:
+  Elastic_() {}
+  Elastic_(const double &e, const double &nu) : E_m(e), nu_m(nu) {}
+  const double &E() const { return E_m; }
+  const double &nu() const { return nu_m; }
+
+ private:
+  double E_m = 0.0, nu_m = 0.0;
+};
+
+class Material_ {
+ public:
+  enum Kind { Plain = 0, KindHollomon = 1, KindJohnsonCook = 2 };
+  Material_() {}
+  explicit Material_(const Elastic_ el) : elastic_m(el) {}
+  virtual ~Material_() {}
+  const Elastic_ &Elastic() const { return elastic_m; }
+  virtual Kind kind() const { return Plain; }
+  double &getT_t() { return T_t; }
+  double &getT_m() { return T_m; }
+
+ protected:
+  Elastic_ elastic_m;
+  double T_t = 0.0, T_m = 0.0;
+};
+
+// sy = (A + B eps_p^n) (1 + C ln(eps_dot / eps_0)) (1 - T_h^m)
+// argument order of Source/Material.h:128-136: A, B, n, C, eps_0, m, T_m, T_t
+class JohnsonCook : public Material_ {
+ public:
+  JohnsonCook() {}
+  JohnsonCook(const Elastic_ &el, const double &a, const double &b, const double &n_, const double &c,
+              const double &eps_0_, const double &m_, const double &T_m_, const double &T_t_)
+      : Material_(el), A(a), B(b), C(c), n(n_), m(m_), eps_0(eps_0_) {
+    T_m = T_m_;
+    T_t = T_t_;
+  }
+  Kind kind() const override { return KindJohnsonCook; }
+  double &getRefStrainRate() { return eps_0; }
+  double A = 0, B = 0, C = 0, n = 0, m = 0, eps_0 = 1.0;
+};
+
+// power law with plateau, Source/Material.cpp:48-57
+class Hollomon : public Material_ {
+ public:
+  Hollomon() {}
+  Hollomon(const Elastic_ &el, const double sy0_, const double &k_, const double &m_)
+      : Material_(el), K(k_), m(m_), sy0(sy0_) {
+    eps0 = sy0_ / el.E();
+    eps1 = std::pow(sy0_ / k_, 1. / m_);
+    if (eps0 > eps1)
+      throw new Fatal(
+          "ERROR, Hollomon material bad definition, please correct Yield Stress, Elastic Modulus or Material "
+          "hardening constants.");
+  }
+  Kind kind() const override { return KindHollomon; }
+  double K = 0, m = 1, eps0 = 0, eps1 = 0, sy0 = 0;
+};

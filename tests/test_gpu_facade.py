@@ -1,0 +1,80 @@
+"""The C++ SPH::Domain facade (include/weldform) against the CPU oracle.
+
+tests/cpp/upsetting.cpp builds the 4,524-particle bilinear upsetting case the
+way a reference driver does (AddCylinderLength, per-particle material
+fields, AssignZone, bConds, SolveDiffUpdateLeapFrog / KickDrift), runs it on
+the device and writes the zone-3 reaction history and the final particle
+state.  The same case goes through the oracle here; the bar is the FP64 one
+of the direct C-ABI tests (1e-8 after the run).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle_case import normwise_err, oracle_from_case
+from weldform_amd import build as wfbuild
+from weldform_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _driver():
+    if not os.path.exists(wfbuild.DRIVER):
+        wfbuild.build_driver()
+    return wfbuild.DRIVER
+
+
+def _run_driver(tmp_path, steps, scheme, precision):
+    key = str(tmp_path / "up")
+    print(f"facade driver: {steps} steps {scheme} {precision}", flush=True)
+    r = subprocess.run([_driver(), str(steps), key, scheme, precision], capture_output=True, text=True,
+                       timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    react = np.loadtxt(key + "_reaction.txt").reshape(-1, 3)
+    final = np.loadtxt(key + "_final.txt")
+    return key, react, final
+
+
+def _oracle(steps, scheme):
+    case = cases.config1(steps=steps)
+    d = oracle_from_case(case, threads=8)
+    d.solve_init(case.dt)
+    R = []
+    for _ in range(steps):
+        d.step(scheme)
+        R.append(d.reaction(case.top_zone).copy())
+    return case, d, np.array(R)
+
+
+@pytest.mark.parametrize("scheme,steps", [("leapfrog", 400), ("kickdrift", 200)])
+def test_facade_driver_matches_oracle(tmp_path, scheme, steps):
+    key, react, final = _run_driver(tmp_path, steps, scheme, "fp64")
+    case, d, R = _oracle(steps, scheme)
+    assert react.shape == (steps, 3)
+    assert final.shape == (case.n, 5)
+    e_r = normwise_err(react[:, 2], R[:, 2])
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    e_p = normwise_err(final[:, 4], d.f("PLSTRAIN")) if np.abs(d.f("PLSTRAIN")).max() > 0 else 0.0
+    print(f"facade {scheme}: reaction {e_r:.3e} displacement {e_u:.3e} pl_strain {e_p:.3e}", flush=True)
+    assert e_r < 1e-8 and e_u < 1e-8 and e_p < 1e-8
+    # output files of the solve loop: one after step 1, one at steps/2
+    for idx in (1, 2):
+        fn = f"{key}_{idx:04d}.csv"
+        assert os.path.exists(fn), fn
+        with open(fn) as f:
+            head = f.readline()
+            rows = sum(1 for _ in f)
+        assert head.startswith("X, Y, Z, ID, Sigma_eq, Pl_Strain")
+        assert rows == case.n
+
+
+def test_facade_driver_mixed_precision(tmp_path):
+    steps = 300
+    _, react, final = _run_driver(tmp_path, steps, "leapfrog", "mixed")
+    _, d, R = _oracle(steps, "leapfrog")
+    e_r = normwise_err(react[:, 2], R[:, 2])
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    print(f"facade mixed: reaction {e_r:.3e} displacement {e_u:.3e}", flush=True)
+    assert e_r < 1e-4 and e_u < 1e-4

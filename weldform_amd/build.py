@@ -1,4 +1,5 @@
-"""Build libweldform_hip.so for gfx950 with hipcc (in-tree, no JIT).
+"""Build libweldform_hip.so for gfx950 with hipcc (in-tree, no JIT), and the
+C++ upsetting driver that runs through the SPH::Domain facade.
 
     python -m weldform_amd.build          # or weldform_amd.build.build()
 """
@@ -13,6 +14,9 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libweldform_hip.so")
 SOURCES = ["wf_kernels.hip", "wf_capi.hip"]
 ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")
+ROOT = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "upsetting.cpp")
+DRIVER = os.path.join(LIBDIR, "upsetting")
 
 HIPCC_FLAGS = [
     "--offload-arch=" + ARCH,
@@ -59,5 +63,21 @@ def build(verbose=False):
     return LIB
 
 
+def build_driver(verbose=False):
+    """g++ build of the facade driver: plain C++17, links the C-ABI only."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L" + LIBDIR, "-lweldform_hip", "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 or r.stderr.strip():
+        sys.stderr.write(r.stdout + r.stderr)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed on " + DRIVER_SRC)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_driver(verbose=True))
