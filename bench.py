@@ -30,21 +30,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kind, n, nbar, mixed):
-    """Algorithmic HBM bytes of one launch: every particle's gathered inputs
-    read once (8 B per value in FP64, 4 B in mixed mode), its own state read
-    and written once in FP64, the int32 neighbour row (count + n_bar indices)
-    read once."""
+def algorithmic_bytes(kind, n, nbar, mixed, idx_bytes):
+    """Algorithmic HBM bytes of one launch of a pair kernel: each particle's
+    pair-pass inputs read once and outputs written once (8 B per gathered
+    value in FP64, 4 B in mixed mode; outputs are FP64), its neighbour row
+    read once (idx_bytes per entry: 4 for the global-index rows, 2 for the
+    LDS-position rows of the staged mixed passes) plus the 4 B count, and
+    the per-particle state the kernel fuses in.
+      accel:   x(3) h v(3) m rho sigma(6) = 15 in; a(3) out
+      density: x(3) h v(3) m m/rho = 9 in; drho out; fused drift: rho, disp
+               in, rho, m/rho, x, disp out (+ the FP32 x shadow in mixed mode)
+      rates:   x(3) h v(3) m rho = 9 in; strain rate (6) + rotation (3) out"""
     w = 4.0 if mixed else 8.0
-    lst = 4.0 * nbar + 4.0
-    if kind == "accel":    # gathered x,h,v,m,rho,sigma(6) = 15 ; out a (3 FP64)
+    lst = idx_bytes * nbar + 4.0
+    if kind == "accel":
         return n * (15 * w + 24.0 + lst)
-    if kind == "density":  # gathered x,h,v,m,m/rho = 9 ; own rho, disp in ; rho, m/rho, x, disp out
-        return n * (9 * w + 32.0 + 64.0 + (24.0 if mixed else 0.0) + lst)
-    if kind == "rates":    # gathered x,h,v,m,m/rho,rho = 10 ; stress update in/out (FP64):
-        # S, strain, strain_pl (3x6) + pl, Sy, T in; the same + sigma record,
-        # rates (9), eff. strain rate, pressure out
-        return n * (10 * w + 8.0 * (18 + 3) + 8.0 * (18 + 3 + 8 + 9 + 2) + (32.0 if mixed else 0.0) + lst)
+    if kind == "density":
+        return n * (9 * w + 8.0 + (8.0 + 24.0) + (8.0 + 8.0 + 24.0 + 24.0) + (16.0 if mixed else 0.0) + lst)
+    if kind == "rates":
+        return n * (9 * w + 72.0 + lst)
     raise ValueError(kind)
 
 
@@ -155,7 +159,7 @@ def main():
     times = mx["times"]
     dom = max(("accel", "density", "rates"), key=lambda k: times[k][0])
     dom_ms = times[dom][0]
-    ach = algorithmic_bytes(dom, case.n, nbar, True) / (dom_ms * 1e-3) / 1e9
+    ach = algorithmic_bytes(dom, case.n, nbar, True, 2.0) / (dom_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": mx["ps"],

@@ -124,7 +124,11 @@ typedef struct wf_config {
   int32_t device;           /* HIP device ordinal                              */
   int32_t no_graphs;        /* 1: launch kernels one by one instead of replaying
                                a captured HIP graph per step                  */
-  int32_t reserved;
+  int32_t stage_max;        /* WF_MIXED: neighbours of each 128-particle tile are
+                               staged in LDS when the tile's neighbourhood has
+                               at most this many particles (0: 1024, the most;
+                               -1: never, gather from HBM).  Results do not
+                               depend on it.                                  */
   double cellfac;           /* Domain::Cellfac (2 for the cubic spline)       */
   double pl_work_heat_frac; /* Domain::pl_work_heat_frac (0.9)                */
   double dom_max[3];        /* Domain::DomMax (default -1e11)                  */

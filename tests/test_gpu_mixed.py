@@ -79,3 +79,30 @@ def test_mixed_config2_vs_oracle_golden():
     e_u = normwise_err(g.get("disp")[gz["idx"]], gz["disp"])
     _log("mixed config2 reaction history err", e_r, "final reaction err", e_f, "disp err", e_u)
     assert e_r < TOL_RUN_MIXED and e_f < TOL_RUN_MIXED and e_u < TOL_RUN_MIXED
+
+
+def _mixed_run(case, steps, **kw):
+    g = cases.make_solver(case, precision=capi.MIXED, **kw)
+    g.step(steps)
+    g.sync()
+    return g
+
+
+@pytest.mark.parametrize("stage_max", [0, 520])
+def test_lds_staging_is_transparent(stage_max):
+    """LDS-staged gathers read the same values as HBM gathers in the same
+    order: the run must be bitwise identical with staging off (-1), on for
+    every tile (0) and on for part of the tiles (520 = about half of them
+    fall back to HBM).  v_top = -20 m/s yields early, so the staging lists
+    are rebuilt with the neighbour lists every 5 steps."""
+    c = cases.cylinder_case("cyl_upset_4k_jc_fast", 7.5e-3, 30e-3, 1.0e-3, cases.AL["rho"], cases.AL["E"],
+                            cases.AL["nu"], cases.JC_AL, steps=120, jitter=1e-3, seed=7, v_top=-20.0, T0=300.0)
+    _log(f"staging transparency: stage_max={stage_max} n={c.n} steps={c.steps}")
+    ref = _mixed_run(c, c.steps, stage_max=-1)
+    got = _mixed_run(c, c.steps, stage_max=stage_max)
+    assert ref.scalars()["nb_searches"] > 5
+    assert got.scalars()["max_pl_strain"] > 0
+    for f in ("x", "v", "a", "disp", "density", "sigma", "strain_rate", "rot_rate", "pl_strain"):
+        assert np.array_equal(got.get(f), ref.get(f)), f
+    assert np.array_equal(got.reaction_history(c.top_zone), ref.reaction_history(c.top_zone))
+    assert got.scalars()["kin_energy_sum"] == ref.scalars()["kin_energy_sum"]

@@ -130,10 +130,10 @@ def small_jc(steps=200, dx=1.0e-3, R=7.5e-3, L=30e-3, seed=7):
                          steps=steps, jitter=1e-3, seed=seed, T0=300.0)
 
 
-def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5):
+def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,
-                    dom_max=case.dom_max, dom_min=case.dom_min)
+                    dom_max=case.dom_max, dom_min=case.dom_min, **kw)
     for idx, m in case.materials.items():
         s.set_material(idx, **m)
     s.set("x", case.x)

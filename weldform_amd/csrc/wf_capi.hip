@@ -353,7 +353,7 @@ void allocate(Ctx *c) {
   for (double **p : scal) *p = c->alloc<double>(np);
   int **ints[] = {&d.id, &d.mat, &d.group, &d.thbc, &d.perm, &d.cell, &d.sorted, &d.nnb};
   for (int **p : ints) *p = c->alloc<int>(np);
-  d.nbr = c->alloc<int>((size_t)c->cfg.max_nb * np);
+  d.nbr = c->alloc<int>((size_t)nb_rows(c->cfg.max_nb) * np);
   d.cell_count = c->alloc<int>((size_t)d.ncells + 1);
   d.cell_start = c->alloc<int>((size_t)d.ncells + 1);
   d.cell_cursor = c->alloc<int>((size_t)d.ncells + 1);
@@ -373,6 +373,16 @@ void allocate(Ctx *c) {
     }
     d.vmf = c->alloc<float4>(np);
     d.srecf = c->alloc<StressRecF>(np);
+  }
+  d.ntiles = (int)(np / kBlock);
+  d.stage_max = 0;
+  if (d.mixed && c->cfg.stage_max >= 0) {
+    d.stage_max = c->cfg.stage_max == 0 ? kStageMax : c->cfg.stage_max;
+    d.stage = c->alloc<int>((size_t)d.ntiles * kStageMax);
+    d.stage_cnt = c->alloc<int>((size_t)d.ntiles);
+    d.nbrl = c->alloc<uint16_t>((size_t)nb_rows(c->cfg.max_nb) * np);
+    d.selfl = c->alloc<uint16_t>(np);
+    HIPCHK(hipMemsetAsync(d.stage_cnt, 0xff, sizeof(int) * (size_t)d.ntiles, c->st));
   }
 }
 
@@ -638,6 +648,8 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
     if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED) throw Error(WF_EINVAL, "bad precision");
     if (!(cfg->cellfac > 0)) throw Error(WF_EINVAL, "cellfac must be positive");
     if (cfg->ts_nb_inc < 1) throw Error(WF_EINVAL, "ts_nb_inc must be >= 1");
+    if (cfg->stage_max < -1 || cfg->stage_max > kStageMax)
+      throw Error(WF_EINVAL, "stage_max must be -1, 0 or 1.." + std::to_string(kStageMax));
     Ctx *c = new Ctx();
     c->cfg = *cfg;
     c->n = n;
@@ -1064,12 +1076,12 @@ int wf_get_neighbour_pairs(void *ctx, int32_t *pairs2, int64_t cap, int64_t *cou
     as_ctx(ctx);
     if (!c->inited || !c->built) throw Error(WF_ESTATE, "no neighbour list built yet");
     auto nnb = dget(c, c->d.nnb, (size_t)c->npad);
-    auto nbr = dget(c, c->d.nbr, (size_t)c->cfg.max_nb * c->npad);
+    auto nbr = dget(c, c->d.nbr, (size_t)nb_rows(c->cfg.max_nb) * c->npad);
     std::vector<std::pair<int32_t, int32_t>> p;
     p.reserve((size_t)c->n * 32);
     for (int64_t i = 0; i < c->n; ++i)
       for (int k = 0; k < nnb[i]; ++k) {
-        const int j = nbr[(size_t)k * c->npad + i];
+        const int j = nbr[nb_slot(c->npad, k, i)];
         const int oi = c->perm[i], oj = c->perm[j];
         if (oi < oj) p.push_back({oi, oj});
       }

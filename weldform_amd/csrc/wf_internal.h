@@ -11,8 +11,12 @@
 //   rho[2], vol[2]   double (vol = mass/rho), double-buffered
 //   per-particle own state as component-major arrays (coalesced, never
 //   gathered): S[6], strain[6], epl[6], eplinc[6], sr[6], rr[3], scalars
-//   nbr     int32 [max_nb][npad] neighbour list, column-major (lane i of a
-//           wavefront reads entry k of particle i: one coalesced 256 B load)
+//   nbr     int32 neighbour list in groups of four: entry k of particle i
+//           sits at ((k/4) * npad + i) * 4 + k%4, so lane i of a wavefront
+//           reads entries 4g..4g+3 as one int4 and the wave's load is one
+//           coalesced 1 KiB run
+//   stage   int32 [ntiles][kStageMax], nbrl uint16 [max_nb][npad]: the LDS
+//           staging lists of the mixed-precision pair passes
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,7 +25,22 @@
 
 namespace wf {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 128;
+// LDS staging of the mixed-precision pair passes: a tile is one block of
+// kBlock consecutive particles; its staged set (the tile's particles and all
+// their neighbours, sorted by index) holds at most kStageMax particles.
+constexpr int kStageMax = 1024;
+constexpr int kSortCap = 8192;   // neighbour entries a tile may list in total
+
+// neighbour rows are read in batches of kNbBatch entries, the next batch
+// prefetched while the current one is processed
+constexpr int kNbBatch = 8;
+__host__ __device__ inline long long nb_slot(long long npad, int k, long long i) {
+  return ((long long)(k >> 2) * npad + i) * 4 + (k & 3);
+}
+// entries allocated per particle: the prefetch after the last batch of a
+// full row (max_nb entries) stays inside the allocation
+__host__ __device__ inline int nb_rows(int max_nb) { return (max_nb + kNbBatch - 1) / kNbBatch * kNbBatch + kNbBatch; }
 constexpr int kMaxBc = 8;        // velocity / temperature BC slots per context
 constexpr int kMaxZones = 8;     // velocity-BC zones with a reaction history
 constexpr int kMaxMat = 256;
@@ -141,6 +160,13 @@ struct Dev {
   // neighbours
   int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *scan_tmp;
   int *nnb, *nbr;
+  // LDS staging (mixed precision): per tile the sorted staged particle list
+  // stage[t][0..stage_cnt[t]) (stage_cnt = -1: tile gathers from HBM), each
+  // neighbour entry as a position in that list (nbrl, same layout as nbr)
+  // and each particle's own position (selfl)
+  int stage_max, ntiles;
+  int *stage, *stage_cnt;
+  uint16_t *nbrl, *selfl;
   // materials, flags, reductions
   const DevMat *mats;
   const DevAmp *amps;

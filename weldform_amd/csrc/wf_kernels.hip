@@ -16,6 +16,7 @@
 namespace wf {
 
 #define WF_KERNEL __global__ __launch_bounds__(kBlock)
+#define WF_PAIR_KERNEL WF_KERNEL
 
 static inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -151,17 +152,19 @@ __device__ inline KH<R> pair_h_t(const KH<R> &ki, R hi, R hj) {
   if (hj == hi) return ki;
   return kern_h_t<R>((hi + hj) / (R)2);
 }
+// both branches evaluated and selected (no divergent branch in the pair
+// loops, so loads of later neighbours can be scheduled early); each branch
+// rounds exactly as a branchy version would.  r2 = 0 makes ri infinite and q
+// NaN, which selects nothing until the last select.
 template <typename R>
 __device__ inline R gk_t(R r2, const KH<R> &k) {
-  if (r2 == (R)0) return (R)-3 * k.c5;
   const R ri = rsqrt_t(r2);
   const R q = r2 * ri * k.ih;
-  if (q < (R)1) return k.c5 * ((R)-3 + (R)2.25 * q);
-  if (q < (R)2) {
-    const R t = (R)2 - q;
-    return k.c5 * ((R)-0.75 * t * t) * (k.h * ri);
-  }
-  return (R)0;
+  const R inner = k.c5 * ((R)-3 + (R)2.25 * q);
+  const R t = (R)2 - q;
+  const R outer = k.c5 * ((R)-0.75 * t * t) * (k.h * ri);
+  R g = q < (R)1 ? inner : (q < (R)2 ? outer : (R)0);
+  return r2 == (R)0 ? (R)-3 * k.c5 : g;
 }
 
 // What a gather reads of particle j: the FP64 state, or its FP32 shadows.
@@ -387,7 +390,7 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
             const int j = d.sorted[p];
             if (j == i || d.group[j] != g) continue;
             if (!in_radius(xi, d.xh[cur][j], d.cellfac)) continue;
-            if (cnt < d.max_nb) d.nbr[(long long)cnt * d.npad + i] = j;
+            if (cnt < d.max_nb) d.nbr[nb_slot(d.npad, cnt, i)] = j;
             ++cnt;
           }
         }
@@ -407,6 +410,114 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
   }
 }
 
+// Per tile of kBlock consecutive particles, after every neighbour build:
+//  1. each particle's row is sorted by neighbour index (in LDS), so lanes of
+//     a wavefront visit their k-th neighbours at nearby addresses;
+//  2. (mixed precision, stage = 1) the LDS staging list: the tile's
+//     particles and all their neighbour entries are sorted (bitonic, in LDS)
+//     and made unique, giving the tile's staged set in index order, which a
+//     pass loads in long contiguous runs; each entry is then rewritten as its
+//     position in the set (binary search).  Tiles whose set exceeds
+//     stage_max keep stage_cnt = -1 and gather from HBM.
+// Tiles whose entries exceed kSortCap keep their rows as built and are not
+// staged.  Step 1 does not depend on `stage`, so staged and HBM gathers sum
+// in the same order.
+__device__ inline int lower_pos(const int *a, int n, int v) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+WF_KERNEL void k_stage(Dev d, int stage) {
+  if (!d.flags->do_search) return;
+  __shared__ int buf[kSortCap];
+  __shared__ int uniq[kStageMax];
+  __shared__ int sh[34];
+  const int tid = threadIdx.x;
+  const long long np = d.npad;
+  for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
+    const int i = t * kBlock + tid;
+    const int c = (i < d.n && d.group[i] >= 0) ? d.nnb[i] : 0;
+    int tot;
+    const int off = block_scan_excl(c + (i < d.n ? 1 : 0), sh, &tot);
+    if (tot > kSortCap) {
+      if (tid == 0 && stage) d.stage_cnt[t] = -1;
+      continue;
+    }
+    // 1. sorted rows
+    if (i < d.n) {
+      int *row = buf + off + 1;
+      buf[off] = i;
+      for (int k = 0; k < c; ++k) row[k] = d.nbr[nb_slot(np, k, i)];
+      for (int a = 1; a < c; ++a) {
+        const int v = row[a];
+        int b = a - 1;
+        while (b >= 0 && row[b] > v) {
+          row[b + 1] = row[b];
+          --b;
+        }
+        row[b + 1] = v;
+      }
+      for (int k = 0; k < c; ++k) d.nbr[nb_slot(np, k, i)] = row[k];
+    }
+    if (!stage) {
+      __syncthreads();
+      continue;
+    }
+    // 2. staged set
+    int P = 1;
+    while (P < tot) P <<= 1;
+    for (int p = tot + tid; p < P; p += kBlock) buf[p] = 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int p = tid; p < P; p += kBlock) {
+          const int q = p ^ j;
+          if (q > p) {
+            const int a = buf[p], b = buf[q];
+            if ((a > b) == ((p & k) == 0)) {
+              buf[p] = b;
+              buf[q] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    // unique entries of the first tot (padding sorts last)
+    int carry = 0;
+    for (int base = 0; base < tot; base += kBlock) {
+      const int p = base + tid;
+      const int f = (p < tot && (p == 0 || buf[p] != buf[p - 1])) ? 1 : 0;
+      int sub;
+      const int e = block_scan_excl(f, sh, &sub) + carry;
+      if (f && e < kStageMax) uniq[e] = buf[p];
+      carry += sub;
+    }
+    const int U = carry;
+    if (U > d.stage_max) {
+      if (tid == 0) d.stage_cnt[t] = -1;
+      __syncthreads();
+      continue;
+    }
+    __syncthreads();
+    for (int p = tid; p < U; p += kBlock) d.stage[(long long)t * kStageMax + p] = uniq[p];
+    if (i < d.n) {
+      d.selfl[i] = (uint16_t)lower_pos(uniq, U, i);
+      for (int k = 0; k < c; ++k)
+        d.nbrl[nb_slot(np, k, i)] = (uint16_t)lower_pos(uniq, U, d.nbr[nb_slot(np, k, i)]);
+    }
+    if (tid == 0) d.stage_cnt[t] = U;
+    __syncthreads();
+  }
+}
+
 void launch_schedule(const Dev &d, hipStream_t st, int force) {
   k_schedule<<<1, 64, 0, st>>>(d, force);
 }
@@ -420,17 +531,105 @@ void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
   k_scatter<<<nblk_small(d.n), kBlock, 0, st>>>(d);
   k_cell_sort<<<nblk_small(d.ncells), kBlock, 0, st>>>(d);
   k_build_nbr<<<nblk_small(d.n) * 2, kBlock, 0, st>>>(d, cur);
+  k_stage<<<d.ntiles < 2048 ? d.ntiles : 2048, kBlock, 0, st>>>(d, d.stage_max > 0 ? 1 : 0);
 }
 
 // ---------------------------------------------------------------------------
 // pass 1: acceleration (CalcAccel + AccelReduction, GradientType 0, TI = 0)
 
+// Where a pass reads neighbour j: straight from HBM (Src<MX>, row of global
+// indices in nbr) or from the tile's LDS copy (row of staged positions in
+// nbrl).  Both give the same values, so results do not depend on staging.
 template <bool MX>
-WF_KERNEL void k_accel(Dev d, int cur) {
+struct HbmJ {
   using S = Src<MX>;
   using R = typename S::R;
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= d.n) return;
+  using R4 = typename S::R4;
+  // FP64 gathers: a batch of 4 in flight (8 costs more registers than it
+  // gains); FP32: the whole batch
+  static constexpr int kUnroll = MX ? kNbBatch : 4;
+  const Dev &d;
+  const int buf;
+  const int i;
+  // entries k0..k0+7 (k0 a multiple of 8) of particle i's row
+  __device__ void fetch(int k0, int out[kNbBatch]) const {
+    const int4 *row = reinterpret_cast<const int4 *>(d.nbr) + i;
+    const int4 a = row[(long long)(k0 >> 2) * d.npad];
+    const int4 b = row[(long long)((k0 >> 2) + 1) * d.npad];
+    out[0] = a.x, out[1] = a.y, out[2] = a.z, out[3] = a.w;
+    out[4] = b.x, out[5] = b.y, out[6] = b.z, out[7] = b.w;
+  }
+  __device__ int self() const { return i; }
+  __device__ R4 x(int j) const { return S::x(d, buf, j); }
+  __device__ R4 v(int j) const { return S::v(d, j); }
+  __device__ void srec(int j, R s[6], R &rho, R &irho2) const { S::srec(d, j, s, rho, irho2); }
+  __device__ bool lower(int i, int j) const { return i < j; }
+};
+struct LdsJ {
+  using R = float;
+  using R4 = float4;
+  static constexpr int kUnroll = kNbBatch;
+  const Dev &d;
+  const float4 *lx, *lv;
+  const StressRecF *ls;
+  const int i;
+  const int own;  // i's position in the staged set
+  __device__ void fetch(int k0, int out[kNbBatch]) const {
+    const ushort4 *row = reinterpret_cast<const ushort4 *>(d.nbrl) + i;
+    const ushort4 a = row[(long long)(k0 >> 2) * d.npad];
+    const ushort4 b = row[(long long)((k0 >> 2) + 1) * d.npad];
+    out[0] = a.x, out[1] = a.y, out[2] = a.z, out[3] = a.w;
+    out[4] = b.x, out[5] = b.y, out[6] = b.z, out[7] = b.w;
+  }
+  __device__ int self() const { return own; }
+  __device__ R4 x(int l) const { return lx[l]; }
+  __device__ R4 v(int l) const { return lv[l]; }
+  __device__ void srec(int l, R s[6], R &rho, R &irho2) const {
+    const StressRecF r = ls[l];
+    for (int k = 0; k < 6; ++k) s[k] = r.s[k];
+    rho = r.rho;
+    irho2 = r.irho2;
+  }
+  // staged positions follow particle index order
+  __device__ bool lower(int, int l) const { return own < l; }
+};
+
+// Calls f(j) for each entry of a neighbour row of cnt entries, in row order,
+// then for the pad entries up to the next multiple of kNbBatch with j = the
+// particle itself: a particle paired with itself has x_ij = v_ij = 0, so
+// every pair term is an exact zero and adding it changes no sum.
+template <class G, class F>
+__device__ inline void for_each_nb(const G &g, int cnt, F &&f) {
+  int cur[kNbBatch], nxt[kNbBatch];
+  g.fetch(0, cur);
+  const int self = g.self();
+  for (int base = 0; base < cnt; base += kNbBatch) {
+    g.fetch(base + kNbBatch, nxt);
+#pragma unroll G::kUnroll
+    for (int u = 0; u < kNbBatch; ++u) f(base + u < cnt ? cur[u] : self);
+#pragma unroll
+    for (int u = 0; u < kNbBatch; ++u) cur[u] = nxt[u];
+  }
+}
+
+// copy the tile's staged set into LDS; returns its size or -1 (not staged)
+__device__ inline int stage_in(const Dev &d, int t, const float4 *gx, float4 *lx, float4 *lv, StressRecF *ls) {
+  const int U = d.stage_cnt[t];
+  const int *st = d.stage + (long long)t * kStageMax;
+  for (int p = threadIdx.x; p < U; p += kBlock) {
+    const int j = st[p];
+    lx[p] = gx[j];
+    lv[p] = d.vmf[j];
+    if (ls) ls[p] = d.srecf[j];
+  }
+  __syncthreads();
+  return U;
+}
+
+template <bool MX, class G>
+__device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g) {
+  using S = Src<MX>;
+  using R = typename S::R;
   R ax = 0, ay = 0, az = 0;
   if (d.group[i] >= 0) {
     const DevMat &M = d.mats[d.mat[i]];
@@ -443,26 +642,20 @@ WF_KERNEL void k_accel(Dev d, int cur) {
     R ti[6];
     for (int k = 0; k < 6; ++k) ti[k] = si[k] * irhoi;
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
-    const int cnt = d.nnb[i];
-    const int *nb = d.nbr + i;
-    for (int k = 0; k < cnt; ++k) {
-      const int j = nb[(long long)k * d.npad];
-      const auto xj = S::x(d, cur, j);
-      const auto vj = S::v(d, j);
+    for_each_nb(g, d.nnb[i], [&](int j) {
+      const auto xj = g.x(j);
+      const auto vj = g.v(j);
       R sj[6], rhoj, irhoj;
-      S::srec(d, j, sj, rhoj, irhoj);
+      g.srec(j, sj, rhoj, irhoj);
       const KH<R> kh = S::kh(ki, xi, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R r2 = dx * dx + dy * dy + dz * dz;
       const R GK = gk_t<R>(r2, kh);
-      R pi = 0;
-      if (av) {
-        const R vdot = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-        if (vdot < (R)0) {
-          const R mu = kh.h * vdot * rcp_t(r2 + (R)0.01 * kh.h * kh.h);
-          pi = (alpha * cs * mu + beta * mu * mu) * rcp_t((R)0.5 * (rhoi + rhoj));
-        }
-      }
+      // Monaghan viscosity for approaching pairs (selected, not branched)
+      const R vdot = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+      const R mu = kh.h * vdot * rcp_t(r2 + (R)0.01 * kh.h * kh.h);
+      const R pv = (alpha * cs * mu + beta * mu * mu) * rcp_t((R)0.5 * (rhoi + rhoj));
+      const R pi = (av && vdot < (R)0) ? pv : (R)0;
       const R Mxx = ti[XX] + sj[XX] * irhoj + pi;
       const R Myy = ti[YY] + sj[YY] * irhoj + pi;
       const R Mzz = ti[ZZ] + sj[ZZ] * irhoj + pi;
@@ -474,13 +667,35 @@ WF_KERNEL void k_accel(Dev d, int cur) {
       ax += mj * (gx * Mxx + gy * Mxy + gz * Mxz);
       ay += mj * (gx * Mxy + gy * Myy + gz * Myz);
       az += mj * (gx * Mxz + gy * Myz + gz * Mzz);
-    }
+    });
   }
   d.acc[i] = make_double4((double)ax, (double)ay, (double)az, 0.0);
 }
 
+template <bool MX>
+WF_PAIR_KERNEL void k_accel(Dev d, int cur) {
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= d.n) return;
+  accel_lane<MX>(d, cur, i, HbmJ<MX>{d, cur, i});
+}
+
+WF_PAIR_KERNEL void k_accel_staged(Dev d, int cur) {
+  __shared__ float4 lx[kStageMax], lv[kStageMax];
+  __shared__ StressRecF ls[kStageMax];
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = t * kBlock + threadIdx.x;
+  if (d.stage_cnt[t] < 0) {
+    if (i < d.n) accel_lane<true>(d, cur, i, HbmJ<true>{d, cur, i});
+    return;
+  }
+  stage_in(d, t, d.xhf[cur], lx, lv, ls);
+  if (i < d.n) accel_lane<true>(d, cur, i, LdsJ{d, lx, lv, ls, i, d.selfl[i]});
+}
+
 void launch_accel(const Dev &d, hipStream_t st, int cur) {
-  if (d.mixed)
+  if (d.mixed && d.stage_max > 0)
+    k_accel_staged<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  else if (d.mixed)
     k_accel<true><<<nblk(d.n), kBlock, 0, st>>>(d, cur);
   else
     k_accel<false><<<nblk(d.n), kBlock, 0, st>>>(d, cur);
@@ -598,32 +813,32 @@ void launch_kick2(const Dev &d, hipStream_t st, double dt) { k_kick2<<<nblk(d.n)
 //  Source/SolverLeapfrog.cpp:280-301).  Reads x_n, v_{n+1/2}, m/rho_n and
 // writes rho_{n+1}, m/rho_{n+1}, x_{n+1} into the other buffer.
 
-template <bool MX>
-WF_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
+template <bool MX, class G>
+__device__ inline double dens_lane(const Dev &d, int cur, int i, const G &g) {
   using S = Src<MX>;
   using R = typename S::R;
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= d.n) return;
   R s = 0;
   if (d.group[i] >= 0) {
     const auto xi = S::x(d, cur, i);
     const auto vi = S::v(d, i);
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
-    const int cnt = d.nnb[i];
-    const int *nb = d.nbr + i;
-    for (int k = 0; k < cnt; ++k) {
-      const int j = nb[(long long)k * d.npad];
-      const auto xj = S::x(d, cur, j);
-      const auto vj = S::v(d, j);
-      const R volj = S::vol(d, cur, j, xj);
+    for_each_nb(g, d.nnb[i], [&](int j) {
+      const auto xj = g.x(j);
+      const auto vj = g.v(j);
+      const R volj = MX ? (R)xj.w : S::vol(d, cur, j, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, S::kh(ki, xi, xj));
       const R t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
       s += volj * t;
-    }
+    });
   }
+  return (double)s;
+}
+
+template <bool MX>
+__device__ inline void dens_update(const Dev &d, int cur, int i, double s, double dt, int do_update) {
   const double rhoi = d.rho[cur][i];
-  const double drho = (double)s * rhoi;
+  const double drho = s * rhoi;
   d.drho[i] = drho;
   if (!do_update) return;
   const int nxt = cur ^ 1;
@@ -646,8 +861,31 @@ WF_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
   }
 }
 
+template <bool MX>
+WF_PAIR_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= d.n) return;
+  dens_update<MX>(d, cur, i, dens_lane<MX>(d, cur, i, HbmJ<MX>{d, cur, i}), dt, do_update);
+}
+
+WF_PAIR_KERNEL void k_dens_drift_staged(Dev d, int cur, double dt, int do_update) {
+  __shared__ float4 lx[kStageMax], lv[kStageMax];
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = t * kBlock + threadIdx.x;
+  if (d.stage_cnt[t] < 0) {
+    if (i < d.n) dens_update<true>(d, cur, i, dens_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}), dt, do_update);
+    return;
+  }
+  stage_in(d, t, d.xhf[cur], lx, lv, nullptr);
+  if (i < d.n)
+    dens_update<true>(d, cur, i, dens_lane<true>(d, cur, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]}), dt,
+                      do_update);
+}
+
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update) {
-  if (d.mixed)
+  if (d.mixed && d.stage_max > 0)
+    k_dens_drift_staged<<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
+  else if (d.mixed)
     k_dens_drift<true><<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
   else
     k_dens_drift<false><<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
@@ -675,64 +913,78 @@ __device__ inline double hol_yield(const DevMat &M, double strain) {
   return M.hol_sy0;
 }
 
-template <bool MX>
-WF_KERNEL void k_rates(Dev d, int buf) {
+// one particle's rate-tensor gather; writes StrainRate / RotationRate and
+// returns its kinetic-energy pair sum
+template <bool MX, class G>
+__device__ inline double rates_lane(const Dev &d, int buf, int i, const G &g) {
   using Sr = Src<MX>;
   using R = typename Sr::R;
+  const long long np = d.npad;
+  R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};  // w = (w01, w02, w12)
+  R ekin = 0;
+  if (d.group[i] >= 0) {
+    const DevMat &M = d.mats[d.mat[i]];
+    const auto xi = Sr::x(d, buf, i);
+    const auto vi = Sr::v(d, i);
+    const R rhoi = Sr::rho(d, buf, i, xi, vi);
+    const R P0 = (R)M.P0, cs2 = (R)(M.Cs * M.Cs), rho0 = (R)M.ref_density;
+    const R irhoi = rcp_t(rhoi);
+    const R pri = (P0 + cs2 * (rhoi - rho0)) * irhoi * irhoi;
+    const R mi = vi.w;
+    const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi));
+    for_each_nb(g, d.nnb[i], [&](int j) {
+      const auto xj = g.x(j);
+      const auto vj = g.v(j);
+      const R volj = MX ? (R)xj.w : Sr::vol(d, buf, j, xj);
+      const R rhoj = MX ? vj.w * rcp_t((R)xj.w) : Sr::rho(d, buf, j, xj, vj);
+      const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, Sr::kh(ki, xi, xj));
+      const R vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+      const R c = volj * ((R)-0.5 * GK);
+      sr[XX] += c * ((R)2 * vx * dx);
+      sr[YY] += c * ((R)2 * vy * dy);
+      sr[ZZ] += c * ((R)2 * vz * dz);
+      sr[XY] += c * (vx * dy + vy * dx);
+      sr[XZ] += c * (vx * dz + vz * dx);
+      sr[YZ] += c * (vy * dz + vz * dy);
+      w[0] += c * (vx * dy - vy * dx);
+      w[1] += c * (vx * dz - vz * dx);
+      w[2] += c * (vy * dz - vz * dy);
+      // kinetic-energy pair term (Source/Domain.cpp:1776-1792): the pair
+      // is oriented lower-index first; its sum over particles vanishes
+      const R irj = rcp_t(rhoj);
+      const R Pj = P0 + cs2 * (rhoj - rho0);
+      const bool lo = g.lower(i, j);
+      const R msec = lo ? vj.w : mi;
+      const R term = (R)0.5 * msec * (pri + Pj * irj * irj) * (vx * (GK * dx) + vy * (GK * dy) + vz * (GK * dz));
+      ekin += lo ? term : -term;
+    });
+  }
+  for (int k = 0; k < 6; ++k) d.sr[k * np + i] = (double)sr[k];
+  for (int k = 0; k < 3; ++k) d.rr[k * np + i] = (double)w[k];
+  return (double)ekin;
+}
+
+template <bool MX>
+WF_PAIR_KERNEL void k_rates(Dev d, int buf) {
   __shared__ double sh[kBlock / 64];
   const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  const long long np = d.npad;
+  const double e_kin = i < d.n ? rates_lane<MX>(d, buf, i, HbmJ<MX>{d, buf, i}) : 0.0;
+  const double r1 = block_reduce<0>(e_kin, sh);
+  if (threadIdx.x == 0) part(d, P_KIN, blockIdx.x) = r1;
+}
+
+WF_PAIR_KERNEL void k_rates_staged(Dev d, int buf) {
+  __shared__ float4 lx[kStageMax], lv[kStageMax];
+  __shared__ double sh[kBlock / 64];
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int i = t * kBlock + threadIdx.x;
   double e_kin = 0.0;
-  if (i < d.n) {
-    const int g = d.group[i];
-    double SR[6], W[3];  // W = (w01, w02, w12)
-      R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};
-      R ekin = 0;
-      if (g >= 0) {
-        const DevMat &M = d.mats[d.mat[i]];
-        const auto xi = Sr::x(d, buf, i);
-        const auto vi = Sr::v(d, i);
-        const R rhoi = Sr::rho(d, buf, i, xi, vi);
-        const R P0 = (R)M.P0, cs2 = (R)(M.Cs * M.Cs), rho0 = (R)M.ref_density;
-        const R irhoi = rcp_t(rhoi);
-        const R pri = (P0 + cs2 * (rhoi - rho0)) * irhoi * irhoi;
-        const R mi = vi.w;
-        const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi));
-        const int cnt = d.nnb[i];
-        const int *nb = d.nbr + i;
-        for (int k = 0; k < cnt; ++k) {
-          const int j = nb[(long long)k * np];
-          const auto xj = Sr::x(d, buf, j);
-          const auto vj = Sr::v(d, j);
-          const R volj = Sr::vol(d, buf, j, xj);
-          const R rhoj = Sr::rho(d, buf, j, xj, vj);
-          const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-          const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, Sr::kh(ki, xi, xj));
-          const R vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-          const R c = volj * ((R)-0.5 * GK);
-          sr[XX] += c * ((R)2 * vx * dx);
-          sr[YY] += c * ((R)2 * vy * dy);
-          sr[ZZ] += c * ((R)2 * vz * dz);
-          sr[XY] += c * (vx * dy + vy * dx);
-          sr[XZ] += c * (vx * dz + vz * dx);
-          sr[YZ] += c * (vy * dz + vz * dy);
-          w[0] += c * (vx * dy - vy * dx);
-          w[1] += c * (vx * dz - vz * dx);
-          w[2] += c * (vy * dz - vz * dy);
-          // kinetic-energy pair term (Source/Domain.cpp:1776-1792): the pair
-          // is oriented lower-index first; its sum over particles vanishes
-          const R irj = rcp_t(rhoj);
-          const R Pj = P0 + cs2 * (rhoj - rho0);
-          const R msec = i < j ? vj.w : mi;
-          const R term = (R)0.5 * msec * (pri + Pj * irj * irj) * (vx * (GK * dx) + vy * (GK * dy) + vz * (GK * dz));
-          ekin += i < j ? term : -term;
-        }
-      }
-      for (int k = 0; k < 6; ++k) SR[k] = (double)sr[k];
-      for (int k = 0; k < 3; ++k) W[k] = (double)w[k];
-      e_kin = (double)ekin;
-      for (int k = 0; k < 6; ++k) d.sr[k * np + i] = SR[k];
-      for (int k = 0; k < 3; ++k) d.rr[k * np + i] = W[k];
+  if (d.stage_cnt[t] < 0) {
+    if (i < d.n) e_kin = rates_lane<true>(d, buf, i, HbmJ<true>{d, buf, i});
+  } else {
+    stage_in(d, t, d.xhf[buf], lx, lv, nullptr);
+    if (i < d.n) e_kin = rates_lane<true>(d, buf, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]});
   }
   const double r1 = block_reduce<0>(e_kin, sh);
   if (threadIdx.x == 0) part(d, P_KIN, blockIdx.x) = r1;
@@ -862,7 +1114,9 @@ WF_KERNEL void k_stress(Dev d, int buf, double dt) {
 }
 
 void launch_rates(const Dev &d, hipStream_t st, int buf) {
-  if (d.mixed)
+  if (d.mixed && d.stage_max > 0)
+    k_rates_staged<<<d.nblocks, kBlock, 0, st>>>(d, buf);
+  else if (d.mixed)
     k_rates<true><<<d.nblocks, kBlock, 0, st>>>(d, buf);
   else
     k_rates<false><<<d.nblocks, kBlock, 0, st>>>(d, buf);
@@ -912,9 +1166,8 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
     double temp = 0.0;
     const KernH ki = kern_h(xi.w);
     const int cnt = d.nnb[i];
-    const int *nb = d.nbr + i;
     for (int k = 0; k < cnt; ++k) {
-      const int j = nb[(long long)k * np];
+      const int j = d.nbr[nb_slot(np, k, i)];
       const double4 xj = d.xh[buf][j];
       const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
