@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 1
+#define WF_ABI_VERSION 2
 
 /* error codes */
 #define WF_OK 0
@@ -129,6 +129,13 @@ typedef struct wf_config {
                                at most this many particles (0: 1024, the most;
                                -1: never, gather from HBM).  Results do not
                                depend on it.                                  */
+  int32_t lanes;            /* threads per particle in the pair passes: 1, 2
+                               or 4; each takes every lanes-th batch of 8
+                               neighbours and the partial sums are added in
+                               lane order (0: the default for the precision).
+                               Sums, and so results, depend on it at the
+                               rounding level.                                */
+  int32_t reserved;
   double cellfac;           /* Domain::Cellfac (2 for the cubic spline)       */
   double pl_work_heat_frac; /* Domain::pl_work_heat_frac (0.9)                */
   double dom_max[3];        /* Domain::DomMax (default -1e11)                  */

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTS)
-    assert L.wf_abi_version() == 1
+    assert L.wf_abi_version() == 2
 
 
 def test_struct_layouts_match_ctypes():

@@ -376,13 +376,18 @@ void allocate(Ctx *c) {
   }
   d.ntiles = (int)(np / kBlock);
   d.stage_max = 0;
+  d.lanes_mx = c->cfg.lanes ? c->cfg.lanes : kLanesMixed;
+  d.lanes_f64 = c->cfg.lanes ? c->cfg.lanes : kLanesFP64;
+  if (d.mixed) {
+    // stage_cnt = -1 marks a tile that gathers from HBM
+    d.stage_cnt = c->alloc<int>((size_t)d.ntiles);
+    HIPCHK(hipMemsetAsync(d.stage_cnt, 0xff, sizeof(int) * (size_t)d.ntiles, c->st));
+  }
   if (d.mixed && c->cfg.stage_max >= 0) {
     d.stage_max = c->cfg.stage_max == 0 ? kStageMax : c->cfg.stage_max;
     d.stage = c->alloc<int>((size_t)d.ntiles * kStageMax);
-    d.stage_cnt = c->alloc<int>((size_t)d.ntiles);
     d.nbrl = c->alloc<uint16_t>((size_t)nb_rows(c->cfg.max_nb) * np);
     d.selfl = c->alloc<uint16_t>(np);
-    HIPCHK(hipMemsetAsync(d.stage_cnt, 0xff, sizeof(int) * (size_t)d.ntiles, c->st));
   }
 }
 
@@ -648,6 +653,8 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
     if (cfg->precision != WF_FP64 && cfg->precision != WF_MIXED) throw Error(WF_EINVAL, "bad precision");
     if (!(cfg->cellfac > 0)) throw Error(WF_EINVAL, "cellfac must be positive");
     if (cfg->ts_nb_inc < 1) throw Error(WF_EINVAL, "ts_nb_inc must be >= 1");
+    if (cfg->lanes != 0 && cfg->lanes != 1 && cfg->lanes != 2 && cfg->lanes != 4)
+      throw Error(WF_EINVAL, "lanes must be 0, 1, 2 or 4");
     if (cfg->stage_max < -1 || cfg->stage_max > kStageMax)
       throw Error(WF_EINVAL, "stage_max must be -1, 0 or 1.." + std::to_string(kStageMax));
     Ctx *c = new Ctx();

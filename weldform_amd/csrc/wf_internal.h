@@ -35,12 +35,17 @@ constexpr int kSortCap = 8192;   // neighbour entries a tile may list in total
 // neighbour rows are read in batches of kNbBatch entries, the next batch
 // prefetched while the current one is processed
 constexpr int kNbBatch = 8;
+constexpr int kMaxLanes = 4;     // threads per particle in a pair pass
+constexpr int kLanesMixed = 2;   // defaults (tuned on MI355X)
+constexpr int kLanesFP64 = 2;
 __host__ __device__ inline long long nb_slot(long long npad, int k, long long i) {
   return ((long long)(k >> 2) * npad + i) * 4 + (k & 3);
 }
-// entries allocated per particle: the prefetch after the last batch of a
-// full row (max_nb entries) stays inside the allocation
-__host__ __device__ inline int nb_rows(int max_nb) { return (max_nb + kNbBatch - 1) / kNbBatch * kNbBatch + kNbBatch; }
+// entries allocated per particle: the prefetch after a lane's last batch of
+// a full row (max_nb entries) stays inside the allocation
+__host__ __device__ inline int nb_rows(int max_nb) {
+  return (max_nb + kNbBatch - 1) / kNbBatch * kNbBatch + kNbBatch * kMaxLanes;
+}
 constexpr int kMaxBc = 8;        // velocity / temperature BC slots per context
 constexpr int kMaxZones = 8;     // velocity-BC zones with a reaction history
 constexpr int kMaxMat = 256;
@@ -165,6 +170,7 @@ struct Dev {
   // neighbour entry as a position in that list (nbrl, same layout as nbr)
   // and each particle's own position (selfl)
   int stage_max, ntiles;
+  int lanes_mx, lanes_f64;     // threads per particle in the pair passes
   int *stage, *stage_cnt;
   uint16_t *nbrl, *selfl;
   // materials, flags, reductions

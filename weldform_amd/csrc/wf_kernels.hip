@@ -16,7 +16,6 @@
 namespace wf {
 
 #define WF_KERNEL __global__ __launch_bounds__(kBlock)
-#define WF_PAIR_KERNEL WF_KERNEL
 
 static inline int nblk(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -594,17 +593,22 @@ struct LdsJ {
   __device__ bool lower(int, int l) const { return own < l; }
 };
 
-// Calls f(j) for each entry of a neighbour row of cnt entries, in row order,
-// then for the pad entries up to the next multiple of kNbBatch with j = the
-// particle itself: a particle paired with itself has x_ij = v_ij = 0, so
-// every pair term is an exact zero and adding it changes no sum.
+// Calls f(j) for the entries of a neighbour row of cnt entries that belong
+// to lane `sub` of the particle's `lanes` lanes: batches of kNbBatch entries
+// sub, sub + lanes, sub + 2 lanes, ... in row order, each batch's indices
+// fetched while the previous batch is processed.  Entries past cnt in the
+// last batch call f with j = the particle itself: a particle paired with
+// itself has x_ij = v_ij = 0, so every pair term is an exact zero and adding
+// it changes no sum.
 template <class G, class F>
-__device__ inline void for_each_nb(const G &g, int cnt, F &&f) {
+__device__ inline void for_each_nb(const G &g, int cnt, int sub, int lanes, F &&f) {
   int cur[kNbBatch], nxt[kNbBatch];
-  g.fetch(0, cur);
+  const int step = lanes * kNbBatch;
+  int base = sub * kNbBatch;
+  g.fetch(base, cur);
   const int self = g.self();
-  for (int base = 0; base < cnt; base += kNbBatch) {
-    g.fetch(base + kNbBatch, nxt);
+  for (; base < cnt; base += step) {
+    g.fetch(base + step, nxt);
 #pragma unroll G::kUnroll
     for (int u = 0; u < kNbBatch; ++u) f(base + u < cnt ? cur[u] : self);
 #pragma unroll
@@ -612,22 +616,38 @@ __device__ inline void for_each_nb(const G &g, int cnt, F &&f) {
   }
 }
 
-// copy the tile's staged set into LDS; returns its size or -1 (not staged)
-__device__ inline int stage_in(const Dev &d, int t, const float4 *gx, float4 *lx, float4 *lv, StressRecF *ls) {
-  const int U = d.stage_cnt[t];
+// lanes > 1: lane `sub` > 0 of each particle hands its NV partial sums to
+// lane 0 through LDS (red: (LANES - 1) * NV * kBlock values); lane 0 adds
+// them in lane order.  Every thread of the block must call this.
+template <int LANES, int NV, typename R>
+__device__ inline void combine_lanes(R (&v)[NV], R *red) {
+  if constexpr (LANES > 1) {
+    const int lane = threadIdx.x % kBlock, sub = threadIdx.x / kBlock;
+    if (sub > 0)
+      for (int c = 0; c < NV; ++c) red[((sub - 1) * NV + c) * kBlock + lane] = v[c];
+    __syncthreads();
+    if (sub == 0)
+      for (int q = 1; q < LANES; ++q)
+        for (int c = 0; c < NV; ++c) v[c] += red[((q - 1) * NV + c) * kBlock + lane];
+  }
+}
+
+// copy the tile's staged set into LDS (all threads of the block)
+__device__ inline void stage_in(const Dev &d, int t, int U, const float4 *gx, float4 *lx, float4 *lv,
+                                StressRecF *ls) {
   const int *st = d.stage + (long long)t * kStageMax;
-  for (int p = threadIdx.x; p < U; p += kBlock) {
+  for (int p = threadIdx.x; p < U; p += blockDim.x) {
     const int j = st[p];
     lx[p] = gx[j];
     lv[p] = d.vmf[j];
     if (ls) ls[p] = d.srecf[j];
   }
   __syncthreads();
-  return U;
 }
 
 template <bool MX, class G>
-__device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g) {
+__device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int sub, int lanes,
+                                  typename Src<MX>::R (&acc)[3]) {
   using S = Src<MX>;
   using R = typename S::R;
   R ax = 0, ay = 0, az = 0;
@@ -642,7 +662,7 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g) {
     R ti[6];
     for (int k = 0; k < 6; ++k) ti[k] = si[k] * irhoi;
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
-    for_each_nb(g, d.nnb[i], [&](int j) {
+    for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
       R sj[6], rhoj, irhoj;
@@ -669,36 +689,56 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g) {
       az += mj * (gx * Mxz + gy * Myz + gz * Mzz);
     });
   }
-  d.acc[i] = make_double4((double)ax, (double)ay, (double)az, 0.0);
+  acc[0] = ax;
+  acc[1] = ay;
+  acc[2] = az;
 }
 
-template <bool MX>
-WF_PAIR_KERNEL void k_accel(Dev d, int cur) {
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= d.n) return;
-  accel_lane<MX>(d, cur, i, HbmJ<MX>{d, cur, i});
+// FP64: HBM gathers, LANES threads per particle
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
+  __shared__ double red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  double a[3] = {0, 0, 0};
+  if (i < d.n) accel_lane<false>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES, a);
+  combine_lanes<LANES>(a, red);
+  if (i < d.n && sub == 0) d.acc[i] = make_double4(a[0], a[1], a[2], 0.0);
 }
 
-WF_PAIR_KERNEL void k_accel_staged(Dev d, int cur) {
+// mixed precision: LDS-staged gathers where the tile is staged, else HBM
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
   __shared__ float4 lx[kStageMax], lv[kStageMax];
   __shared__ StressRecF ls[kStageMax];
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int i = t * kBlock + threadIdx.x;
-  if (d.stage_cnt[t] < 0) {
-    if (i < d.n) accel_lane<true>(d, cur, i, HbmJ<true>{d, cur, i});
-    return;
+  __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  const int U = d.stage_cnt[t];
+  float a[3] = {0, 0, 0};
+  if (U < 0) {
+    if (i < d.n) accel_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
+  } else {
+    stage_in(d, t, U, d.xhf[cur], lx, lv, ls);
+    if (i < d.n) accel_lane<true>(d, cur, i, LdsJ{d, lx, lv, ls, i, d.selfl[i]}, sub, LANES, a);
   }
-  stage_in(d, t, d.xhf[cur], lx, lv, ls);
-  if (i < d.n) accel_lane<true>(d, cur, i, LdsJ{d, lx, lv, ls, i, d.selfl[i]});
+  combine_lanes<LANES>(a, red);
+  if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
 }
 
+#define WF_LANES_SWITCH(lanes, KERNEL, ...)                                    \
+  switch (lanes) {                                                            \
+    case 1: KERNEL<1><<<nblk(d.n), kBlock * 1, 0, st>>>(__VA_ARGS__); break;  \
+    case 2: KERNEL<2><<<nblk(d.n), kBlock * 2, 0, st>>>(__VA_ARGS__); break;  \
+    default: KERNEL<4><<<nblk(d.n), kBlock * 4, 0, st>>>(__VA_ARGS__); break; \
+  }
+
 void launch_accel(const Dev &d, hipStream_t st, int cur) {
-  if (d.mixed && d.stage_max > 0)
-    k_accel_staged<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
-  else if (d.mixed)
-    k_accel<true><<<nblk(d.n), kBlock, 0, st>>>(d, cur);
-  else
-    k_accel<false><<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  if (d.mixed) {
+    WF_LANES_SWITCH(d.lanes_mx, k_accel_mx, d, cur)
+  } else {
+    WF_LANES_SWITCH(d.lanes_f64, k_accel_f64, d, cur)
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -814,7 +854,7 @@ void launch_kick2(const Dev &d, hipStream_t st, double dt) { k_kick2<<<nblk(d.n)
 // writes rho_{n+1}, m/rho_{n+1}, x_{n+1} into the other buffer.
 
 template <bool MX, class G>
-__device__ inline double dens_lane(const Dev &d, int cur, int i, const G &g) {
+__device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, const G &g, int sub, int lanes) {
   using S = Src<MX>;
   using R = typename S::R;
   R s = 0;
@@ -822,7 +862,7 @@ __device__ inline double dens_lane(const Dev &d, int cur, int i, const G &g) {
     const auto xi = S::x(d, cur, i);
     const auto vi = S::v(d, i);
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
-    for_each_nb(g, d.nnb[i], [&](int j) {
+    for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
       const R volj = MX ? (R)xj.w : S::vol(d, cur, j, xj);
@@ -832,7 +872,7 @@ __device__ inline double dens_lane(const Dev &d, int cur, int i, const G &g) {
       s += volj * t;
     });
   }
-  return (double)s;
+  return s;
 }
 
 template <bool MX>
@@ -861,34 +901,41 @@ __device__ inline void dens_update(const Dev &d, int cur, int i, double s, doubl
   }
 }
 
-template <bool MX>
-WF_PAIR_KERNEL void k_dens_drift(Dev d, int cur, double dt, int do_update) {
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= d.n) return;
-  dens_update<MX>(d, cur, i, dens_lane<MX>(d, cur, i, HbmJ<MX>{d, cur, i}), dt, do_update);
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_dens_f64(Dev d, int cur, double dt, int do_update) {
+  __shared__ double red[(LANES > 1 ? LANES - 1 : 1) * kBlock];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  double s[1] = {0};
+  if (i < d.n) s[0] = dens_lane<false>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES);
+  combine_lanes<LANES>(s, red);
+  if (i < d.n && sub == 0) dens_update<false>(d, cur, i, s[0], dt, do_update);
 }
 
-WF_PAIR_KERNEL void k_dens_drift_staged(Dev d, int cur, double dt, int do_update) {
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, double dt, int do_update) {
   __shared__ float4 lx[kStageMax], lv[kStageMax];
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int i = t * kBlock + threadIdx.x;
-  if (d.stage_cnt[t] < 0) {
-    if (i < d.n) dens_update<true>(d, cur, i, dens_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}), dt, do_update);
-    return;
+  __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * kBlock];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  const int U = d.stage_cnt[t];
+  float s[1] = {0};
+  if (U < 0) {
+    if (i < d.n) s[0] = dens_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES);
+  } else {
+    stage_in(d, t, U, d.xhf[cur], lx, lv, nullptr);
+    if (i < d.n) s[0] = dens_lane<true>(d, cur, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]}, sub, LANES);
   }
-  stage_in(d, t, d.xhf[cur], lx, lv, nullptr);
-  if (i < d.n)
-    dens_update<true>(d, cur, i, dens_lane<true>(d, cur, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]}), dt,
-                      do_update);
+  combine_lanes<LANES>(s, red);
+  if (i < d.n && sub == 0) dens_update<true>(d, cur, i, (double)s[0], dt, do_update);
 }
 
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update) {
-  if (d.mixed && d.stage_max > 0)
-    k_dens_drift_staged<<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
-  else if (d.mixed)
-    k_dens_drift<true><<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
-  else
-    k_dens_drift<false><<<nblk(d.n), kBlock, 0, st>>>(d, cur, dt, do_update);
+  if (d.mixed) {
+    WF_LANES_SWITCH(d.lanes_mx, k_dens_mx, d, cur, dt, do_update)
+  } else {
+    WF_LANES_SWITCH(d.lanes_f64, k_dens_f64, d, cur, dt, do_update)
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -913,14 +960,14 @@ __device__ inline double hol_yield(const DevMat &M, double strain) {
   return M.hol_sy0;
 }
 
-// one particle's rate-tensor gather; writes StrainRate / RotationRate and
-// returns its kinetic-energy pair sum
+// one particle's rate-tensor gather: v[0..5] strain rate, v[6..8] rotation
+// rate (w01, w02, w12), v[9] the kinetic-energy pair sum
 template <bool MX, class G>
-__device__ inline double rates_lane(const Dev &d, int buf, int i, const G &g) {
+__device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int sub, int lanes,
+                                  typename Src<MX>::R (&v)[10]) {
   using Sr = Src<MX>;
   using R = typename Sr::R;
-  const long long np = d.npad;
-  R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};  // w = (w01, w02, w12)
+  R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};
   R ekin = 0;
   if (d.group[i] >= 0) {
     const DevMat &M = d.mats[d.mat[i]];
@@ -932,7 +979,7 @@ __device__ inline double rates_lane(const Dev &d, int buf, int i, const G &g) {
     const R pri = (P0 + cs2 * (rhoi - rho0)) * irhoi * irhoi;
     const R mi = vi.w;
     const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi));
-    for_each_nb(g, d.nnb[i], [&](int j) {
+    for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
       const R volj = MX ? (R)xj.w : Sr::vol(d, buf, j, xj);
@@ -960,32 +1007,50 @@ __device__ inline double rates_lane(const Dev &d, int buf, int i, const G &g) {
       ekin += lo ? term : -term;
     });
   }
-  for (int k = 0; k < 6; ++k) d.sr[k * np + i] = (double)sr[k];
-  for (int k = 0; k < 3; ++k) d.rr[k * np + i] = (double)w[k];
-  return (double)ekin;
+  for (int k = 0; k < 6; ++k) v[k] = sr[k];
+  for (int k = 0; k < 3; ++k) v[6 + k] = w[k];
+  v[9] = ekin;
 }
 
-template <bool MX>
-WF_PAIR_KERNEL void k_rates(Dev d, int buf) {
-  __shared__ double sh[kBlock / 64];
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  const double e_kin = i < d.n ? rates_lane<MX>(d, buf, i, HbmJ<MX>{d, buf, i}) : 0.0;
+template <typename R>
+__device__ inline double rates_store(const Dev &d, int i, const R (&v)[10]) {
+  const long long np = d.npad;
+  for (int k = 0; k < 6; ++k) d.sr[k * np + i] = (double)v[k];
+  for (int k = 0; k < 3; ++k) d.rr[k * np + i] = (double)v[6 + k];
+  return (double)v[9];
+}
+
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_rates_f64(Dev d, int buf) {
+  __shared__ double red[(LANES > 1 ? LANES - 1 : 1) * 10 * kBlock];
+  __shared__ double sh[kBlock * LANES / 64];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (i < d.n) rates_lane<false>(d, buf, i, HbmJ<false>{d, buf, i}, sub, LANES, v);
+  combine_lanes<LANES>(v, red);
+  const double e_kin = (i < d.n && sub == 0) ? rates_store(d, i, v) : 0.0;
   const double r1 = block_reduce<0>(e_kin, sh);
   if (threadIdx.x == 0) part(d, P_KIN, blockIdx.x) = r1;
 }
 
-WF_PAIR_KERNEL void k_rates_staged(Dev d, int buf) {
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
   __shared__ float4 lx[kStageMax], lv[kStageMax];
-  __shared__ double sh[kBlock / 64];
-  const int t = xcd_tile(blockIdx.x, gridDim.x);
-  const int i = t * kBlock + threadIdx.x;
-  double e_kin = 0.0;
-  if (d.stage_cnt[t] < 0) {
-    if (i < d.n) e_kin = rates_lane<true>(d, buf, i, HbmJ<true>{d, buf, i});
+  __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 10 * kBlock];
+  __shared__ double sh[kBlock * LANES / 64];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  const int U = d.stage_cnt[t];
+  float v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (U < 0) {
+    if (i < d.n) rates_lane<true>(d, buf, i, HbmJ<true>{d, buf, i}, sub, LANES, v);
   } else {
-    stage_in(d, t, d.xhf[buf], lx, lv, nullptr);
-    if (i < d.n) e_kin = rates_lane<true>(d, buf, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]});
+    stage_in(d, t, U, d.xhf[buf], lx, lv, nullptr);
+    if (i < d.n) rates_lane<true>(d, buf, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]}, sub, LANES, v);
   }
+  combine_lanes<LANES>(v, red);
+  const double e_kin = (i < d.n && sub == 0) ? rates_store(d, i, v) : 0.0;
   const double r1 = block_reduce<0>(e_kin, sh);
   if (threadIdx.x == 0) part(d, P_KIN, blockIdx.x) = r1;
 }
@@ -1114,12 +1179,11 @@ WF_KERNEL void k_stress(Dev d, int buf, double dt) {
 }
 
 void launch_rates(const Dev &d, hipStream_t st, int buf) {
-  if (d.mixed && d.stage_max > 0)
-    k_rates_staged<<<d.nblocks, kBlock, 0, st>>>(d, buf);
-  else if (d.mixed)
-    k_rates<true><<<d.nblocks, kBlock, 0, st>>>(d, buf);
-  else
-    k_rates<false><<<d.nblocks, kBlock, 0, st>>>(d, buf);
+  if (d.mixed) {
+    WF_LANES_SWITCH(d.lanes_mx, k_rates_mx, d, buf)
+  } else {
+    WF_LANES_SWITCH(d.lanes_f64, k_rates_f64, d, buf)
+  }
 }
 
 void launch_stress(const Dev &d, hipStream_t st, int buf, double dt) {
