@@ -24,6 +24,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s at 1M (FP32-mix vs FP64 GPU, stated err); adjoint grads/s"
 HBM_PEAK_GBS = 8000.0
+# pair-pass kernels of the mixed-precision solver (two lanes per particle)
+DOM_KERNEL = {"accel": "k_accel_mx<2>", "density": "k_dens_mx<2>", "rates": "k_rates_mx<2>"}
 
 
 def log(*a):
@@ -52,6 +54,21 @@ def algorithmic_bytes(kind, n, nbar, mixed, idx_bytes):
     raise ValueError(kind)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, calibrated on a 1 GiB copy), or
+    None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic_cyl_upset_976k.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
 def timed_run(case, precision, steps, warmup):
     """Graph-replay timed region for the throughput, then the same steps again
     launched one by one with HIP events around each pair kernel (per-kernel
@@ -62,8 +79,10 @@ def timed_run(case, precision, steps, warmup):
     w.step(warmup)
     w.sync()
     w.close()
+    ti = time.perf_counter()
     s = cases.make_solver(case, precision=precision)
     s.sync()
+    init_s = time.perf_counter() - ti
     t0 = time.perf_counter()
     s.step(steps)
     s.sync()
@@ -77,7 +96,7 @@ def timed_run(case, precision, steps, warmup):
     wall_eager = time.perf_counter() - t1
     times = {k: e.kernel_time(k) for k in ("accel", "density", "rates", "neighbours", "step")}
     e.close()
-    return s, wall, wall_eager, times
+    return s, wall, wall_eager, times, init_s
 
 def cpu_baseline(case, steps):
     sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
@@ -130,8 +149,8 @@ def main():
     log(f"bench: {case.name} n={case.n} dt={case.dt:.4e} warmup={args.warmup} steps={args.steps}")
     runs = {}
     for prec, name in ((capi.FP64, "fp64"), (capi.MIXED, "mixed")):
-        s, wall, wall_eager, times = timed_run(case, prec, args.steps, args.warmup)
-        runs[name] = dict(solver=s, wall=wall, wall_eager=wall_eager, times=times,
+        s, wall, wall_eager, times, init_s = timed_run(case, prec, args.steps, args.warmup)
+        runs[name] = dict(solver=s, wall=wall, wall_eager=wall_eager, times=times, init_s=init_s,
                           R=s.reaction_history(case.top_zone)[:, 2], disp=s.get("disp"),
                           ps=case.n * args.steps / wall)
         log(f"bench: {name}: {1e3 * wall / args.steps:.3f} ms/step (graph), "
@@ -193,6 +212,7 @@ def main():
             "timing_note": "value: steps replayed from one captured HIP graph per step; kernel_ms: "
                            "the same steps launched one by one with HIP events on the solver's stream",
             "nb_searches": sc["nb_searches"],
+            "init_seconds": {"mixed": mx["init_s"], "fp64": f64["init_s"]},
         },
         "roofline": {
             "bound": "hbm",
@@ -201,7 +221,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(DOM_KERNEL[dom]),
+            "algorithmic_bytes": algorithmic_bytes(dom, case.n, nbar, True, 2.0),
+            "note": "achieved = algorithmic bytes of one launch / its mean duration (HIP events on the "
+                    "solver stream); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                    "(profiles/traffic_cyl_upset_976k.md)",
         },
     }
     if not args.no_cpu:
