@@ -20,10 +20,10 @@ F64 = ["X", "V", "A", "XPREV", "DISP", "RHO", "DRHO", "MASS", "H", "REFRHO", "P"
        "CS", "P0", "G", "ALPHA", "BETA", "SIGMA", "SHEAR", "SR", "RR", "STRAIN",
        "EPL", "EPLINC", "PLSTRAIN", "DPLSTRAIN", "SIGMAY", "EFFSR", "EP", "ET",
        "T", "DTDT", "KT", "CPT", "THEXP", "QCONV", "QSRC", "QPL", "QCONT",
-       "HCONV", "TINF", "PSENERGY", "NORMAL"]
+       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE"]
 F64_DIM = dict(X=3, V=3, A=3, XPREV=3, DISP=3, SIGMA=9, SHEAR=9, SR=9, RR=9,
-               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3)
-I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB"]
+               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3)
+I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB", "ELEMENT", "MESHIDX", "IDORIG"]
 
 
 class Material(ctypes.Structure):
@@ -68,15 +68,22 @@ def lib():
             getattr(L, name).argtypes = [vp, d]
         for name in ["wfo_step_leapfrog", "wfo_step_kickdrift", "wfo_neighbour_search",
                      "wfo_start_acceleration", "wfo_calc_accel", "wfo_calc_dens",
-                     "wfo_calc_rates", "wfo_general_after"]:
+                     "wfo_calc_rates", "wfo_general_after", "wfo_energies"]:
             getattr(L, name).argtypes = [vp]
         L.wfo_stress_all.argtypes = [vp, d]
         L.wfo_thermal_calcs.argtypes = [vp, d]
+        L.wfo_plastic_work_heat.argtypes = [vp, d]
         L.wfo_npairs.restype = i64
         L.wfo_npairs.argtypes = [vp]
         L.wfo_pairs.restype = ctypes.POINTER(ctypes.c_int)
         L.wfo_pairs.argtypes = [vp]
         L.wfo_reaction.argtypes = [vp, i32, dp]
+        L.wfo_add_plane_mesh.argtypes = [vp, i32, i32, i32, i32, dp, dp, i32, d]
+        L.wfo_contact_force.argtypes = [vp, i32, dp]
+        L.wfo_ncontact_pairs.restype = i64
+        L.wfo_ncontact_pairs.argtypes = [vp]
+        L.wfo_contact_pairs.restype = ctypes.POINTER(ctypes.c_int)
+        L.wfo_contact_pairs.argtypes = [vp]
         L.wfo_kernel.restype = d
         L.wfo_kernel.argtypes = [d, d]
         L.wfo_grad_kernel.restype = d
@@ -198,8 +205,36 @@ class Domain:
     def thermal_calcs(self, dt):
         self._check(lib().wfo_thermal_calcs(self.h, float(dt)))
 
+    def plastic_work_heat(self, dt):
+        self._check(lib().wfo_plastic_work_heat(self.h, float(dt)))
+
+    def energies(self):
+        self._check(lib().wfo_energies(self.h))
+
     def general_after(self):
         self._check(lib().wfo_general_after(self.h))
+
+    def add_plane_mesh(self, first, zone, axis, flip, p1, p2, dens, hfac=1.1):
+        """AxisPlaneMesh + CalcSpheres + AddTrimeshParticles; the mesh's
+        2 dens^2 particles are written from index `first` on."""
+        a = np.ascontiguousarray(p1, dtype=np.float64)
+        b = np.ascontiguousarray(p2, dtype=np.float64)
+        rc = lib().wfo_add_plane_mesh(self.h, int(first), int(zone), int(axis), int(bool(flip)), _dp(a), _dp(b),
+                                      int(dens), float(hfac))
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def contact_force(self, mesh):
+        out = np.zeros(3)
+        self._check(lib().wfo_contact_force(self.h, mesh, _dp(out)))
+        return out
+
+    def contact_pairs(self):
+        P = lib().wfo_ncontact_pairs(self.h)
+        if P == 0:
+            return np.zeros((0, 2), dtype=np.int32)
+        return np.ctypeslib.as_array(lib().wfo_contact_pairs(self.h), shape=(2 * P,)).reshape(P, 2).copy()
 
     def reaction(self, zone):
         out = np.zeros(3)

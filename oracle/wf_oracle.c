@@ -36,10 +36,21 @@ static const int k_fdim[WFO_NFIELDS] = {
     1, 1, 1, 1, 1, 1,     /* EP ET T DTDT KT CPT */
     1, 1, 1, 1, 1,        /* THEXP QCONV QSRC QPL QCONT */
     1, 1, 1,              /* HCONV TINF PSENERGY */
-    3                     /* NORMAL */
+    3,                    /* NORMAL */
+    3                     /* CONTFORCE */
 };
 
 typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T; } bc_t;
+
+/* rigid triangle mesh (Source/Mesh.h, Source/Mesh.cpp) */
+typedef struct {
+  int id, nnodes, nelem, first;   /* first = index of its first rigid particle */
+  double *node, *node_v;          /* 3 per node */
+  int *el;                        /* 3 node indices per element */
+  double *centroid, *normal, *pplane, *radius;
+  int *nfar;
+  double mv[3];                   /* TriMesh::m_v */
+} mesh_t;
 typedef struct { int id, n; double t[WFO_MAX_AMP_PTS], v[WFO_MAX_AMP_PTS]; } amp_t;
 
 struct wfo_domain {
@@ -73,7 +84,16 @@ struct wfo_domain {
   double react[WFO_MAX_ZONES][3];
   int react_pending;
   double *temp_work, *dkin_work;
+  /* contact (Source/Contact.cpp, Source/Contact_Wang.cpp) */
+  int contact, nmesh, id_free_surf;
+  mesh_t mesh[WFO_MAX_MESH];
+  double friction_sta, friction_dyn, DFAC;
+  int *rig, *cont;                /* RIGPairs and ContPairs, (temp1, temp2) */
+  int64_t nrig, caprig, ncont, capcont;
+  double m_contact_force[WFO_MAX_MESH][3], contact_force_sum, contact_reaction_sum, ext_forces_work;
 };
+
+static int contact_nb_update(wfo_domain *d);
 
 static void *xcalloc(size_t n, size_t s) {
   void *p = calloc(n ? n : 1, s);
@@ -115,6 +135,12 @@ void wfo_destroy(wfo_domain *d) {
   free(d->HOC); free(d->LL); free(d->pairs); free(d->ipair); free(d->jpair);
   free(d->Anei); free(d->Aref); free(d->pair_force); free(d->pair_densinc);
   free(d->pair_sr); free(d->pair_rr); free(d->temp_work); free(d->dkin_work);
+  for (int m = 0; m < d->nmesh; m++) {
+    mesh_t *M = &d->mesh[m];
+    free(M->node); free(M->node_v); free(M->el); free(M->centroid); free(M->normal);
+    free(M->pplane); free(M->radius); free(M->nfar);
+  }
+  free(d->rig); free(d->cont);
   free(d);
 }
 
@@ -140,6 +166,10 @@ int wfo_set_param(wfo_domain *d, const char *s, double v) {
   else if (!strcmp(s, "pl_heating")) d->pl_heating = (int)v;
   else if (!strcmp(s, "pl_work_heat_frac")) d->pl_work_heat_frac = v;
   else if (!strcmp(s, "solid_count")) d->solid_count = (int)v;
+  else if (!strcmp(s, "contact")) d->contact = (int)v;
+  else if (!strcmp(s, "friction_sta")) d->friction_sta = v;
+  else if (!strcmp(s, "friction_dyn")) d->friction_dyn = v;
+  else if (!strcmp(s, "DFAC")) d->DFAC = v;
   else if (!strcmp(s, "Time")) d->Time = v;
   else if (!strcmp(s, "deltat")) d->deltat = v;
   else if (!strncmp(s, "DomMax", 6) && s[6] >= '0' && s[6] <= '2') d->DomMax[s[6] - '0'] = v;
@@ -152,6 +182,10 @@ double wfo_get_param(const wfo_domain *d, const char *s) {
   if (!strcmp(s, "Time")) return d->Time;
   if (!strcmp(s, "deltat")) return d->deltat;
   if (!strcmp(s, "kin_energy_sum")) return d->kin_energy_sum;
+  if (!strcmp(s, "contact_force_sum")) return d->contact_force_sum;
+  if (!strcmp(s, "contact_reaction_sum")) return d->contact_reaction_sum;
+  if (!strcmp(s, "ext_forces_work")) return d->ext_forces_work;
+  if (!strcmp(s, "ncont")) return (double)d->ncont;
   if (!strcmp(s, "int_energy_sum")) return d->int_energy_sum;
   if (!strcmp(s, "plastic_work")) return d->plastic_work;
   if (!strcmp(s, "max_pl_strain")) return d->max_pl_strain;
@@ -322,6 +356,23 @@ int wfo_general_after(wfo_domain *d) {
       }
     }
   }
+  /* rigid meshes follow the velocity BC of their zone (Source/WeldForm.cpp:94-119) */
+  for (int b = 0; b < d->nbcs; b++) {
+    const bc_t *bc = &d->bcs[b];
+    if (bc->type != 0) continue;
+    for (int m = 0; m < d->nmesh; m++) {
+      if (d->mesh[m].id != bc->zone) continue;
+      if (bc->value_type == 0) {
+        for (int k = 0; k < 3; k++) d->mesh[m].mv[k] = bc->value[k];
+      } else if (bc->value_type == 1) {
+        for (int j = 0; j < d->namps; j++) {
+          if (d->amps[j].id != bc->amp_id) continue;
+          double val = bc->amp_factor * amp_value(&d->amps[j], d->Time);
+          for (int k = 0; k < 3; k++) d->mesh[m].mv[k] = val * bc->value[k];
+        }
+      }
+    }
+  }
   return 0;
 }
 
@@ -401,20 +452,37 @@ static inline int check_radius(const wfo_domain *d, int p1, int p2) {
   return (rij / h) <= d->Cellfac;
 }
 
-typedef struct { int *p; int64_t n, cap; int64_t other; } pairbuf_t;
-static void pb_push(pairbuf_t *b, int a, int c) {
-  if (b->n == b->cap) {
-    b->cap = b->cap ? 2 * b->cap : 4096;
-    b->p = (int *)realloc(b->p, (size_t)b->cap * 2 * sizeof(int));
-    if (!b->p) { fprintf(stderr, "wf_oracle: out of memory\n"); abort(); }
+typedef struct { int *p; int64_t n, cap; int64_t other; int *r; int64_t nr, capr; } pairbuf_t;
+static void push2(int **p, int64_t *n, int64_t *cap, int a, int c) {
+  if (*n == *cap) {
+    *cap = *cap ? 2 * *cap : 4096;
+    *p = (int *)realloc(*p, (size_t)*cap * 2 * sizeof(int));
+    if (!*p) { fprintf(stderr, "wf_oracle: out of memory\n"); abort(); }
   }
-  b->p[2 * b->n] = a; b->p[2 * b->n + 1] = c; b->n++;
+  (*p)[2 * *n] = a; (*p)[2 * *n + 1] = c; (*n)++;
+}
+static void pb_push(pairbuf_t *b, int a, int c) { push2(&b->p, &b->n, &b->cap, a, c); }
+
+static inline int is_mesh_id(const wfo_domain *d, int id) {
+  for (int m = 0; m < d->nmesh; m++)
+    if (d->mesh[m].id == id) return 1;
+  return 0;
 }
 
 /* AllocateNbPair without contact (Source/Neighbour.cpp:374-417): both free
  * and same Material -> SMPairs.  Free/fixed and cross-material pairs feed
  * paths this oracle does not model; they are only counted. */
 static inline void allocate_pair(const wfo_domain *d, int t1, int t2, pairbuf_t *b) {
+  if (d->contact) {
+    /* contact branch (Source/Neighbour.cpp:375-396): a pair with a
+     * rigid-mesh particle goes to RIGPairs, without a radius test, when the
+     * other particle carries the free-surface ID; it never reaches SMPairs */
+    const int *id = d->fi[WFO_ID];
+    if (is_mesh_id(d, id[t1]) || is_mesh_id(d, id[t2])) {
+      if (id[t1] == d->id_free_surf || id[t2] == d->id_free_surf) push2(&b->r, &b->nr, &b->capr, t1, t2);
+      return;
+    }
+  }
   if (!check_radius(d, t1, t2)) return;
   const int *fr = d->fi[WFO_FREE], *mt = d->fi[WFO_MATERIAL];
   if (fr[t1] || fr[t2]) {
@@ -478,6 +546,8 @@ static void main_neighbour_search(wfo_domain *d) {
     if (bufs[q1].n) memcpy(d->pairs + 2 * off, bufs[q1].p, (size_t)bufs[q1].n * 2 * sizeof(int));
     off += bufs[q1].n;
     free(bufs[q1].p);
+    for (int64_t k = 0; k < bufs[q1].nr; k++) push2(&d->rig, &d->nrig, &d->caprig, bufs[q1].r[2 * k], bufs[q1].r[2 * k + 1]);
+    free(bufs[q1].r);
   }
   free(bufs);
   d->npairs = tot;
@@ -518,6 +588,8 @@ static int calc_pair_pos_list(wfo_domain *d) {
 /* ClearNbData (Source/Neighbour.cpp:479-497) */
 static void clear_nb_data(wfo_domain *d) {
   d->npairs = 0;
+  d->nrig = 0;
+  d->ncont = 0;
   list_generate(d);
   d->nb_cleared = 1;
 }
@@ -526,7 +598,8 @@ int wfo_neighbour_search(wfo_domain *d) {
   if (!d->HOC && cell_initiate(d)) return -1;
   clear_nb_data(d);
   main_neighbour_search(d);
-  return calc_pair_pos_list(d);
+  if (calc_pair_pos_list(d)) return -1;
+  return d->contact ? contact_nb_update(d) : 0;
 }
 int64_t wfo_npairs(const wfo_domain *d) { return d->npairs; }
 const int *wfo_pairs(const wfo_domain *d) { return d->pairs; }
@@ -929,7 +1002,8 @@ static void kin_energy(wfo_domain *d) {
 /* CalcIntEnergyEqn (Source/Domain.cpp:1810-1829, Source/Particle.cpp:941-950) */
 static void int_energy(wfo_domain *d) {
   double inc = 0.;
-  for (int i = 0; i < d->n; i++) {
+  const int max = d->contact ? d->mesh[0].first : d->n;
+  for (int i = 0; i < max; i++) {
     const double *S = d->f[WFO_SHEAR] + 9 * (int64_t)i, *SR = d->f[WFO_SR] + 9 * (int64_t)i;
     double de = d->f[WFO_MASS][i] / d->f[WFO_RHO][i] *
                 (S[0] * SR[0] + 2.0 * S[1] * SR[3] + 2.0 * S[2] * SR[6] + S[4] * SR[4] + 2.0 * S[5] * SR[7] + S[8] * SR[8]);
@@ -937,6 +1011,338 @@ static void int_energy(wfo_domain *d) {
   }
   d->int_energy_sum += inc * d->deltat;
 }
+
+int wfo_plastic_work_heat(wfo_domain *d, double dt) {
+  plastic_work_heat(d, dt);
+  return 0;
+}
+
+int wfo_energies(wfo_domain *d) {
+  kin_energy(d);
+  int_energy(d);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Rigid meshes and contact (Source/Mesh.cpp:70-300, Source/Contact.cpp:10-132,
+ * Source/Contact_Wang.cpp:15-348, Source/Domain.cpp:1415-1491)        */
+
+static void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* TriMesh::CalcNormals, 3D (Source/Mesh.cpp:229-243) */
+static void mesh_normals(mesh_t *M) {
+  for (int e = 0; e < M->nelem; e++) {
+    const double *n0 = M->node + 3 * M->el[3 * e], *n1 = M->node + 3 * M->el[3 * e + 1],
+                 *n2 = M->node + 3 * M->el[3 * e + 2];
+    double u[3], v[3], w[3];
+    for (int k = 0; k < 3; k++) { u[k] = n1[k] - n0[k]; v[k] = n2[k] - n0[k]; }
+    cross3(u, v, w);
+    double nw = norm3(w);
+    for (int k = 0; k < 3; k++) M->normal[3 * e + k] = w[k] / nw;
+  }
+}
+
+/* TriMesh::UpdatePlaneCoeff (Source/Mesh.cpp:222-227) */
+static void mesh_plane_coeff(mesh_t *M) {
+  for (int e = 0; e < M->nelem; e++)
+    M->pplane[e] = dot3(M->node + 3 * M->el[3 * e + M->nfar[e]], M->normal + 3 * e);
+}
+
+/* TriMesh::AxisPlaneMesh, 3D (Source/Mesh.cpp:82-202) followed by
+ * CalcSpheres (:206-220, whose loop leaves nfar = the last node), then
+ * Domain::AddTrimeshParticles (Source/Contact.cpp:10-37) with h =
+ * hfac * radius, hfac passed as float. */
+int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, const double p1[3], const double p2[3],
+                       int dens, double hfac) {
+  if (d->nmesh >= WFO_MAX_MESH) return fail("too many meshes");
+  if (dens < 1) return fail("mesh density must be >= 1");
+  const int nelem = 2 * dens * dens, nnodes = (dens + 1) * (dens + 1);
+  if (first < 0 || first + nelem > d->n) return fail("rigid particles exceed the domain size");
+  mesh_t *M = &d->mesh[d->nmesh];
+  memset(M, 0, sizeof *M);
+  M->id = id;
+  M->first = first;
+  M->nnodes = nnodes;
+  M->nelem = nelem;
+  M->node = (double *)xcalloc((size_t)3 * nnodes, sizeof(double));
+  M->node_v = (double *)xcalloc((size_t)3 * nnodes, sizeof(double));
+  M->el = (int *)xcalloc((size_t)3 * nelem, sizeof(int));
+  M->centroid = (double *)xcalloc((size_t)3 * nelem, sizeof(double));
+  M->normal = (double *)xcalloc((size_t)3 * nelem, sizeof(double));
+  M->pplane = (double *)xcalloc((size_t)nelem, sizeof(double));
+  M->radius = (double *)xcalloc((size_t)nelem, sizeof(double));
+  M->nfar = (int *)xcalloc((size_t)nelem, sizeof(int));
+  double p[3];
+  for (int k = 0; k < 3; k++) p[k] = p2[k] - p1[k];
+  int dir[3];
+  if (axis == 0) { dir[0] = 1; dir[1] = 2; }
+  else if (axis == 1) { dir[0] = 0; dir[1] = 2; }
+  else { dir[0] = 0; dir[1] = 1; }
+  dir[2] = axis;
+  double x3 = p1[dir[2]], x2 = p1[dir[1]];
+  double dl = p[dir[0]] / dens;
+  int nn = 0;
+  for (int j = 0; j < dens + 1; j++) {
+    double x1 = p1[dir[0]];
+    for (int i = 0; i < dens + 1; i++) {
+      double v[3];
+      v[dir[0]] = x1; v[dir[1]] = x2; v[dir[2]] = x3;
+      for (int k = 0; k < 3; k++) M->node[3 * nn + k] = v[k];
+      nn++;
+      x1 += dl;
+    }
+    x2 += dl;
+  }
+  int el = 0;
+  for (int j = 0; j < dens; j++)
+    for (int i = 0; i < dens; i++) {
+      int n[4];
+      n[0] = (dens + 1) * j + i; n[1] = n[0] + 1;
+      n[2] = (dens + 1) * (j + 1) + i; n[3] = n[2] + 1;
+      int con[2][3];
+      if (flip) {
+        con[0][0] = n[0]; con[0][1] = n[1]; con[0][2] = n[2];
+        con[1][0] = n[1]; con[1][1] = n[3]; con[1][2] = n[2];
+      } else {
+        con[0][0] = n[0]; con[0][1] = n[2]; con[0][2] = n[1];
+        con[1][0] = n[1]; con[1][1] = n[2]; con[1][2] = n[3];
+      }
+      for (int e = 0; e < 2; e++) {
+        for (int k = 0; k < 3; k++) M->el[3 * el + k] = con[e][k];
+        const double *a = M->node + 3 * con[e][0], *b = M->node + 3 * con[e][1], *c = M->node + 3 * con[e][2];
+        for (int k = 0; k < 3; k++) M->centroid[3 * el + k] = (a[k] + b[k] + c[k]) / 3.;
+        el++;
+      }
+    }
+  mesh_normals(M);
+  if (flip)
+    for (int k = 0; k < 3 * nelem; k++) M->normal[k] = -M->normal[k];
+  for (int e = 0; e < nelem; e++) {
+    double mx = 0.;
+    for (int q = 0; q < 3; q++) {
+      double rv[3];
+      for (int k = 0; k < 3; k++) rv[k] = M->node[3 * M->el[3 * e + q] + k] - M->centroid[3 * e + k];
+      if (norm3(rv) > mx) mx = norm3(rv);
+      M->nfar[e] = q;
+    }
+    M->radius[e] = mx;
+  }
+  mesh_plane_coeff(M);
+  const double hf = (double)(float)hfac;
+  for (int e = 0; e < nelem; e++) {
+    int r = first + e;
+    for (int k = 0; k < 3; k++) {
+      d->f[WFO_X][3 * r + k] = M->centroid[3 * e + k];
+      d->f[WFO_V][3 * r + k] = 0.0;
+      d->f[WFO_NORMAL][3 * r + k] = M->normal[3 * e + k];
+    }
+    d->f[WFO_H][r] = hf * M->radius[e];
+    d->f[WFO_MASS][r] = 0.0;
+    d->f[WFO_RHO][r] = d->f[WFO_REFRHO][r] = 0.0;
+    d->fi[WFO_ID][r] = id;
+    d->fi[WFO_FREE][r] = 1;
+    d->fi[WFO_ELEMENT][r] = e;
+    d->fi[WFO_MESHIDX][r] = d->nmesh;
+  }
+  d->nmesh++;
+  d->contact = 1;
+  return nelem;
+}
+
+/* Domain::CalculateSurface (Source/Domain.cpp:1415-1491): mass-weighted
+ * sum of x_ij over SMPairs, scaled by the mean particle mass over ALL
+ * particles (rigid ones count with mass 0); |n| >= h/4 and Nb <= 46 marks
+ * a free-surface particle, which takes ID 1. */
+static int calculate_surface(wfo_domain *d, int id) {
+  d->id_free_surf = id;
+  double totmass = 0.;
+  for (int i = 0; i < d->n; i++) totmass += d->f[WFO_MASS][i];
+  totmass /= d->n;
+  const int maxid = d->contact ? d->mesh[0].first : d->n;
+  double *nrm = d->f[WFO_NORMAL];
+  const double *x = d->f[WFO_X], *m = d->f[WFO_MASS];
+  for (int i = 0; i < maxid; i++) {
+    for (int k = 0; k < 3; k++) nrm[3 * i + k] = 0.;
+    d->fi[WFO_ID][i] = d->fi[WFO_IDORIG][i];
+  }
+  for (int64_t p = 0; p < d->npairs; p++) {
+    int a = d->pairs[2 * p], b = d->pairs[2 * p + 1];
+    for (int k = 0; k < 3; k++) {
+      double xij = x[3 * a + k] - x[3 * b + k];
+      nrm[3 * a + k] += m[b] * xij;
+      nrm[3 * b + k] -= m[a] * xij;
+    }
+  }
+  int surf = 0;
+  for (int i = 0; i < maxid; i++) {
+    for (int k = 0; k < 3; k++) nrm[3 * i + k] *= 1. / totmass;
+    if (norm3(nrm + 3 * i) >= 0.25 * d->f[WFO_H][i] && d->fi[WFO_NB][i] <= 46) {
+      d->fi[WFO_ID][i] = id;
+      surf++;
+    }
+  }
+  if (surf == 0) return fail("ERROR: No external particles found. Please check particle masses");
+  return 0;
+}
+
+/* Domain::ContactNbSearch (Source/Contact.cpp:57-132): RIGPairs whose
+ * particles are one rigid and one free-surface particle closer than
+ * h1 + h2 become contact pairs.  They are then ordered by (solid particle,
+ * rigid particle): the reference visits them in RIGPairs order, which
+ * differs only where one particle has two contact points inside
+ * triangles (a point on a shared edge). */
+static int cmp_pair(const void *a, const void *b) {
+  const int *x = (const int *)a, *y = (const int *)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+  return 0;
+}
+static void contact_nb_search(wfo_domain *d) {
+  const int *id = d->fi[WFO_ID];
+  const double *x = d->f[WFO_X], *h = d->f[WFO_H];
+  for (int64_t a = 0; a < d->nrig; a++) {
+    int p1 = d->rig[2 * a], p2 = d->rig[2 * a + 1];
+    if (!(is_mesh_id(d, id[p1]) || is_mesh_id(d, id[p2]))) continue;
+    if (!(id[p1] == d->id_free_surf || id[p2] == d->id_free_surf)) continue;
+    double xij[3] = {x[3 * p1] - x[3 * p2], x[3 * p1 + 1] - x[3 * p2 + 1], x[3 * p1 + 2] - x[3 * p2 + 2]};
+    if (norm3(xij) < h[p1] + h[p2]) {
+      /* store as (SPH particle, rigid particle) */
+      int s = is_mesh_id(d, id[p1]) ? p2 : p1, r = is_mesh_id(d, id[p1]) ? p1 : p2;
+      push2(&d->cont, &d->ncont, &d->capcont, s, r);
+    }
+  }
+  d->nrig = 0;
+  qsort(d->cont, (size_t)d->ncont, 2 * sizeof(int), cmp_pair);
+}
+
+/* ContactNbUpdate (Source/Domain.cpp:2481-2487) */
+static int contact_nb_update(wfo_domain *d) {
+  if (calculate_surface(d, 1)) return -1;
+  contact_nb_search(d);
+  return 0;
+}
+
+/* Domain::CalcContactForcesWang (Source/Contact_Wang.cpp:15-348), 3D,
+ * no contact heat: penalty kij = 2 m / dt^2 on the penetration of the
+ * predicted position into the triangle's plane, damping DFAC, Coulomb
+ * friction on the predicted tangential slip.  contforce is assigned per
+ * pair while a accumulates; the static branch subtracts tgforce / m from
+ * contforce (both as in the reference). */
+static void contact_forces(wfo_domain *d) {
+  const double dt = d->deltat;
+  double *x = d->f[WFO_X], *v = d->f[WFO_V], *a = d->f[WFO_A], *cf = d->f[WFO_CONTFORCE];
+  const double *h = d->f[WFO_H], *ms = d->f[WFO_MASS], *nrmv = d->f[WFO_NORMAL];
+  for (int i = 0; i < d->n; i++)
+    for (int k = 0; k < 3; k++) cf[3 * i + k] = 0.;
+  memset(d->m_contact_force, 0, sizeof d->m_contact_force);
+  d->contact_force_sum = 0.;
+  d->contact_reaction_sum = 0.;
+  double ext_step = 0.;
+  for (int64_t q = 0; q < d->ncont; q++) {
+    const int P1 = d->cont[2 * q], P2 = d->cont[2 * q + 1];
+    const double *n2 = nrmv + 3 * P2;
+    double vr[3];
+    for (int k = 0; k < 3; k++) vr[k] = v[3 * P1 + k] - v[3 * P2 + k];
+    const double delta_ = -dot3(n2, vr);
+    const int m = d->fi[WFO_MESHIDX][P2];
+    const mesh_t *M = &d->mesh[m];
+    const int e = d->fi[WFO_ELEMENT][P2];
+    double x_pred[3];
+    for (int k = 0; k < 3; k++) x_pred[k] = x[3 * P1 + k] + v[3 * P1 + k] * dt + a[3 * P1 + k] * dt * dt / 2.0;
+    const double dist = dot3(n2, x_pred) - M->pplane[e];
+    if (!(dist < h[P1])) continue;
+    double qj[3];
+    for (int k = 0; k < 3; k++) qj[k] = x[3 * P1 + k] - dist * n2[k];
+    int inside = 1;
+    for (int ii = 0; ii < 3 && inside; ii++) {
+      int jj = ii + 1;
+      if (jj > 2) jj = 0;
+      const double *ni = M->node + 3 * M->el[3 * e + ii], *nj = M->node + 3 * M->el[3 * e + jj];
+      double ea[3], eb[3], c[3];
+      for (int k = 0; k < 3; k++) { ea[k] = nj[k] - ni[k]; eb[k] = qj[k] - ni[k]; }
+      cross3(ea, eb, c);
+      if (dot3(c, n2) < 0.0) inside = 0;
+    }
+    if (!inside) continue;
+    const double m1 = ms[P1];
+    const double delta = h[P1] - dist;
+    const double kij = 2.0 * m1 / (dt * dt);
+    const double omega = sqrt(kij / m1);
+    const double psi_cont = 2. * m1 * omega * d->DFAC;
+    for (int k = 0; k < 3; k++) cf[3 * P1 + k] = (kij * delta - psi_cont * delta_) * n2[k];
+    for (int k = 0; k < 3; k++) a[3 * P1 + k] += cf[3 * P1 + k] / m1;
+    if (d->friction_sta > 0.) {
+      double du[3], dtg[3], tgf[3];
+      for (int k = 0; k < 3; k++) du[k] = x_pred[k] - x[3 * P1 + k] - v[3 * P2 + k] * dt;
+      const double dn = dot3(du, n2);
+      for (int k = 0; k < 3; k++) dtg[k] = du[k] - dn * n2[k];
+      for (int k = 0; k < 3; k++) tgf[k] = kij * dtg[k];
+      if (norm3(tgf) < d->friction_sta * norm3(cf + 3 * P1)) {
+        for (int k = 0; k < 3; k++) {
+          cf[3 * P1 + k] -= tgf[k] / m1;
+          a[3 * P1 + k] -= tgf[k] / m1;
+        }
+      } else {
+        const double ncf = norm3(cf + 3 * P1), ntg = norm3(tgf);
+        double tgd[3];
+        for (int k = 0; k < 3; k++) tgd[k] = d->friction_dyn * ncf * tgf[k] / ntg;
+        for (int k = 0; k < 3; k++) {
+          cf[3 * P1 + k] -= tgd[k];
+          a[3 * P1 + k] -= tgd[k] / m1;
+        }
+      }
+    }
+    d->contact_force_sum += norm3(cf + 3 * P1);
+    for (int k = 0; k < 3; k++) d->m_contact_force[m][k] += cf[3 * P1 + k];
+    d->contact_reaction_sum += dot3(a + 3 * P1, n2) * m1;
+    ext_step += dot3(cf + 3 * P1, v + 3 * P2);
+  }
+  d->ext_forces_work += ext_step * dt;
+}
+
+/* TriMesh::Update with m_w = 0 (Source/Mesh.cpp:273-296) and
+ * Domain::UpdateContactParticles (Source/Contact.cpp:41-55) */
+static void mesh_update(wfo_domain *d) {
+  const double dt = d->deltat;
+  for (int m = 0; m < d->nmesh; m++) {
+    mesh_t *M = &d->mesh[m];
+    for (int nd = 0; nd < M->nnodes; nd++)
+      for (int k = 0; k < 3; k++) {
+        M->node_v[3 * nd + k] = M->mv[k] + 0.0;
+        M->node[3 * nd + k] += M->node_v[3 * nd + k] * dt;
+      }
+    for (int e = 0; e < M->nelem; e++)
+      for (int k = 0; k < 3; k++) {
+        double c = 0.;
+        for (int q = 0; q < 3; q++) c += M->node[3 * M->el[3 * e + q] + k];
+        M->centroid[3 * e + k] = c / 3;
+      }
+    mesh_normals(M);
+    mesh_plane_coeff(M);
+    for (int e = 0; e < M->nelem; e++) {
+      int r = M->first + e;
+      for (int k = 0; k < 3; k++) {
+        double s = 0.;
+        for (int q = 0; q < 3; q++) s += M->node_v[3 * M->el[3 * e + q] + k];
+        d->f[WFO_V][3 * r + k] = s / 3;
+        d->f[WFO_A][3 * r + k] = 0.;
+        d->f[WFO_NORMAL][3 * r + k] = M->normal[3 * e + k];
+      }
+    }
+  }
+}
+
+int wfo_contact_force(const wfo_domain *d, int mesh, double out[3]) {
+  if (mesh < 0 || mesh >= d->nmesh) return fail("mesh index out of range");
+  for (int k = 0; k < 3; k++) out[k] = d->m_contact_force[mesh][k];
+  return 0;
+}
+int64_t wfo_ncontact_pairs(const wfo_domain *d) { return d->ncont; }
+const int *wfo_contact_pairs(const wfo_domain *d) { return d->cont; }
 
 /* ------------------------------------------------------------------ */
 /* Solver set-up (Source/SolverLeapfrog.cpp:24-103)                     */
@@ -948,6 +1354,15 @@ int wfo_solve_init(wfo_domain *d, double dt) {
   memset(d->BLPF, 0, sizeof d->BLPF);
   if (cell_initiate(d)) return -1;
   list_generate(d);
+  if (d->contact) {
+    /* Source/SolverLeapfrog.cpp:80-100: ID_orig, one search, Nb counts
+     * and the free surface before the loop */
+    for (int i = 0; i < d->n; i++) d->fi[WFO_IDORIG][i] = d->fi[WFO_ID][i];
+    clear_nb_data(d);
+    main_neighbour_search(d);
+    if (calc_pair_pos_list(d)) return -1;
+    if (calculate_surface(d, 1)) return -1;
+  }
   clear_nb_data(d);
   d->isfirst = 1; d->isyielding = 0; d->ts_i = 0;
   d->initialized = 1;
@@ -967,12 +1382,14 @@ static int schedule_search(wfo_domain *d, double mx) {
     clear_nb_data(d);
     main_neighbour_search(d);
     if (calc_pair_pos_list(d)) return -1;
+    if (d->contact && contact_nb_update(d)) return -1;
     d->isyielding = 1;
   }
   if (mx > MIN_PS_FOR_NBSEARCH || d->isfirst) {
     if (d->ts_i == 0 && d->nb_cleared) {
       main_neighbour_search(d);
       if (calc_pair_pos_list(d)) return -1;
+      if (d->contact && contact_nb_update(d)) return -1;
     }
   }
   return 0;
@@ -1010,6 +1427,7 @@ static void step_tail(wfo_domain *d, double mx) {
   d->steps++;
   d->Time += d->deltat;
   d->isfirst = 0;
+  if (d->contact) mesh_update(d);
   schedule_end(d, mx);
 }
 
@@ -1024,6 +1442,7 @@ int wfo_step_leapfrog(wfo_domain *d) {
   wfo_calc_accel(d);
   d->react_pending = 1;
   wfo_general_after(d);
+  if (d->contact) contact_forces(d);
   double *v = d->f[WFO_V], *a = d->f[WFO_A];
   if (d->isfirst)
     for (int i = 0; i < 3 * d->n; i++) v[i] -= a[i] * 0.5 * d->deltat;
@@ -1051,6 +1470,7 @@ int wfo_step_kickdrift(wfo_domain *d) {
   wfo_calc_accel(d);
   d->react_pending = 1;
   wfo_general_after(d);
+  if (d->contact) contact_forces(d);
   double *v = d->f[WFO_V], *a = d->f[WFO_A];
   double prev = d->deltat;
   for (int i = 0; i < 3 * d->n; i++) v[i] += a[i] * 0.5 * prev * 1.;

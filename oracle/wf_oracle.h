@@ -31,6 +31,7 @@ extern "C" {
 #define WFO_MAX_BCS 32
 #define WFO_MAX_AMPS 16
 #define WFO_MAX_AMP_PTS 64
+#define WFO_MAX_MESH 8
 
 enum { WFO_BILINEAR = 0, WFO_HOLLOMON = 1, WFO_JOHNSON_COOK = 2 };
 
@@ -46,11 +47,12 @@ enum {
   WFO_THEXP, WFO_QCONV, WFO_QSRC, WFO_QPL, WFO_QCONT,      /* 1 */
   WFO_HCONV, WFO_TINF, WFO_PSENERGY,                       /* 1 */
   WFO_NORMAL,                                              /* 3 */
+  WFO_CONTFORCE,                                           /* 3 */
   WFO_NFIELDS
 };
 /* int fields */
 enum { WFO_ID = 0, WFO_MATERIAL, WFO_MATIDX, WFO_MODEL, WFO_FREE, WFO_THBC,
-       WFO_NB, WFO_NIFIELDS };
+       WFO_NB, WFO_ELEMENT, WFO_MESHIDX, WFO_IDORIG, WFO_NIFIELDS };
 
 typedef struct wfo_domain wfo_domain;
 
@@ -101,6 +103,15 @@ int wfo_calc_rates(wfo_domain *d);                 /* CalcRateTensors+Red. */
 int wfo_stress_all(wfo_domain *d, double dt);      /* CalcStressStrain     */
 int wfo_thermal_calcs(wfo_domain *d, double dt);   /* ThermalCalcs         */
 int wfo_general_after(wfo_domain *d);              /* UserAcc BC hook      */
+int wfo_plastic_work_heat(wfo_domain *d, double dt); /* CalcPlasticWorkHeat */
+int wfo_energies(wfo_domain *d);                   /* Kin + Int EnergyEqn  */
+/* rigid plane meshes and contact (Source/Mesh.cpp, Source/Contact*.cpp):
+ * the mesh's particles are written at [first, first + 2 dens^2) */
+int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, const double p1[3],
+                       const double p2[3], int dens, double hfac);
+int wfo_contact_force(const wfo_domain *d, int mesh, double out[3]);
+int64_t wfo_ncontact_pairs(const wfo_domain *d);
+const int *wfo_contact_pairs(const wfo_domain *d);          /* (SPH, rigid) */
 /* reaction observed at the first BC call of the last step:
  * sum of m*a over particles with ID == zone (before BCs zero a) */
 int wfo_reaction(const wfo_domain *d, int zone, double out[3]);

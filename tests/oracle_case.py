@@ -15,6 +15,10 @@ def oracle_from_case(case, threads=1):
     d.f("SIGMAY")[:] = case.sigmay
     if case.T is not None:
         d.f("T")[:] = case.T
+    if case.thbc is not None:
+        d.i("THBC")[:] = case.thbc
+        d.f("HCONV")[:] = case.hconv
+        d.f("TINF")[:] = case.tinf
     d.i("ID")[:] = case.ids
     d.i("MATERIAL")[:] = case.material
     d.i("MATIDX")[:] = case.material

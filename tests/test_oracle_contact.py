@@ -1,0 +1,135 @@
+"""Pins of the oracle's rigid-mesh contact (Source/Mesh.cpp,
+Source/Contact.cpp, Source/Contact_Wang.cpp, Source/Domain.cpp:1415-1491)
+against closed forms: mesh geometry, free-surface detection, the penalty
+force of a particle under a plane, static and dynamic friction, and the
+rigid-body motion of a mesh."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+DX = 1.0e-3
+H = 1.2 * DX
+RHO = 2700.0
+E, NU = 68.9e9, 0.3
+
+
+def _block(nx=6, ny=6, nz=4, plane_gap=0.4 * H, dens=8, flip=False, v0=(0.0, 0.0, 0.0), mu=(0.0, 0.0)):
+    """A cubic-lattice block under a rigid z-plane (normal -z) shifted by a
+    quarter cell so no particle projects onto a triangle edge."""
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    x = (g + 0.5) * DX
+    ns = len(x)
+    nr = 2 * dens * dens
+    d = po.Domain(ns + nr)
+    d.set_param("solid_count", ns)
+    K = E / (3 * (1 - 2 * NU))
+    d.f("X")[:ns] = x
+    d.f("H")[:ns] = H
+    d.f("MASS")[:ns] = RHO * DX ** 3
+    d.f("RHO")[:ns] = d.f("REFRHO")[:ns] = RHO
+    d.f("CS")[:ns] = math.sqrt(K / RHO)
+    d.f("G")[:ns] = E / (2 * (1 + NU))
+    d.f("SIGMAY")[:ns] = 1e12
+    d.f("V")[:ns] = np.asarray(v0)
+    d.set_material(0, model=po.BILINEAR, E=E, nu=NU)
+    ztop = x[:, 2].max()
+    z0 = ztop + plane_gap
+    # element leg = DX: particles sit a quarter cell off every mesh line
+    d.add_plane_mesh(ns, 10, 2, flip, [-0.75 * DX, -0.75 * DX, z0], [(dens - 0.75) * DX, (dens - 0.75) * DX, z0], dens)
+    d.set_param("friction_sta", mu[0])
+    d.set_param("friction_dyn", mu[1])
+    return d, x, ns, z0
+
+
+def test_plane_mesh_geometry():
+    d, x, ns, z0 = _block()
+    dl = DX
+    n = d.f("NORMAL")[ns:]
+    # flipNormals=false: connectivity (n0, n2, n1) gives the -z normal
+    assert np.all(n == np.array([0.0, 0.0, -1.0]))
+    # right isosceles triangles of leg dl: centroid to the far vertex dl sqrt(5)/3
+    r = dl * math.sqrt(5.0) / 3.0
+    assert np.allclose(d.f("H")[ns:], float(np.float32(1.1)) * r, rtol=1e-13)
+    assert np.allclose(d.f("X")[ns:, 2], z0, rtol=1e-15)
+    assert np.all(d.i("ID")[ns:] == 10)
+    # centroids: the square's two triangles either side of its n1-n2
+    # diagonal have centroids at (1/3, 1/3) and (2/3, 2/3) of the cell
+    c = (d.f("X")[ns:, :2] + 0.75 * DX) / dl
+    frac = c - np.floor(c + 1e-9)
+    assert np.allclose(frac[0::2], 1 / 3, atol=1e-9) and np.allclose(frac[1::2], 2 / 3, atol=1e-9)
+
+
+def test_free_surface_detection():
+    d, x, ns, _ = _block()
+    d.solve_init(1e-8)
+    ids = d.i("ID")[:ns]
+    g = np.round(x / DX - 0.5).astype(int)
+    outer = (g == 0).any(1) | (g[:, 0] == 5) | (g[:, 1] == 5) | (g[:, 2] == 3)
+    # every particle on the block's faces is free surface; the 2x2x2 interior core
+    # (inner with respect to all faces by 2 layers) is not
+    assert np.all(ids[outer] == 1)
+    core = ((g[:, 0] >= 2) & (g[:, 0] <= 3) & (g[:, 1] >= 2) & (g[:, 1] <= 3) & (g[:, 2] >= 1) & (g[:, 2] <= 2))
+    assert np.all(ids[core] == 0)
+
+
+def test_penalty_force_closed_form():
+    """A resting block touching the plane: kij = 2m/dt^2, contact force
+    kij (h - dist) n on each top particle, dist = z0 - z_top."""
+    dt = 1e-8
+    d, x, ns, z0 = _block()
+    d.solve_init(dt)
+    d.step()
+    m = RHO * DX ** 3
+    kij = 2.0 * m / (dt * dt)
+    top = np.isclose(x[:, 2], x[:, 2].max())
+    cf = d.f("CONTFORCE")[:ns]
+    want = kij * (H - (z0 - x[top, 2]))
+    assert np.allclose(cf[top, 2], -want, rtol=1e-12) and np.all(cf[top, :2] == 0)
+    assert np.all(cf[~top] == 0)
+    total = d.contact_force(0)
+    assert total[2] == pytest.approx(cf[:, 2].sum(), rel=1e-12)
+    assert d.param("contact_force_sum") == pytest.approx(np.abs(cf[:, 2]).sum(), rel=1e-12)
+
+
+@pytest.mark.parametrize("mu_s,branch", [(0.2, "static"), (1e-9, "dynamic")])
+def test_friction_closed_forms(mu_s, branch):
+    """Tangential slip du = v dt of a block sliding along the plane gives
+    tgforce = kij v dt.  Static: contforce -= tgforce / m (the reference
+    subtracts an acceleration from a force) and a -= tgforce / m.  Dynamic:
+    contforce -= mu_d |F_n| t and a -= mu_d |F_n| t / m."""
+    dt, vx, mu_d = 1e-8, 0.5, 0.3
+    d, x, ns, z0 = _block(v0=(vx, 0.0, 0.0), mu=(mu_s, mu_d))
+    d.solve_init(dt)
+    d.step()
+    m = RHO * DX ** 3
+    kij = 2.0 * m / (dt * dt)
+    top = np.isclose(x[:, 2], x[:, 2].max())
+    Fn = kij * (H - (z0 - x[top, 2]))
+    tg = kij * vx * dt
+    cf = d.f("CONTFORCE")[:ns][top]
+    # du = x_pred - x - v_rigid dt cancels x (|x| / |v dt| ~ 1e6): 1e-9 bar
+    if branch == "static":
+        assert tg < mu_s * Fn.min()
+        assert np.allclose(cf[:, 0], -tg / m, rtol=1e-9)
+    else:
+        assert np.allclose(cf[:, 0], -mu_d * Fn, rtol=1e-12)
+    assert np.allclose(cf[:, 2], -Fn, rtol=1e-12)
+
+
+def test_mesh_motion_and_normal_refresh():
+    """A velocity BC on the mesh zone moves nodes and particles by v dt per
+    step; the flipped normals of flipNormals=true are recomputed from the
+    winding at the first update and so turn over (Source/Mesh.cpp:273-296)."""
+    dt, vz = 1e-8, -2.0
+    d, x, ns, z0 = _block(plane_gap=3 * H, flip=True)
+    d.add_velocity_bc(10, [0.0, 0.0, vz])
+    assert np.all(d.f("NORMAL")[ns:] == np.array([0.0, 0.0, -1.0]))
+    d.solve_init(dt)
+    for _ in range(5):
+        d.step()
+    assert np.allclose(d.f("X")[ns:, 2], z0 + 5 * vz * dt, rtol=0, atol=1e-15)
+    assert np.all(d.f("NORMAL")[ns:] == np.array([0.0, 0.0, 1.0]))
+    assert np.allclose(d.f("V")[ns:], [0.0, 0.0, vz], rtol=1e-15)

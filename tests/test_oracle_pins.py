@@ -336,3 +336,162 @@ def test_threads_do_not_change_results():
         out.append((d.f("X").copy(), d.f("SIGMA").copy(), d.param("int_energy_sum")))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+
+
+# ---------------------------------------------------------------- thermal
+def test_conduction_uniform_and_quadratic_temperature():
+    """CalcTempInc on the lattice interior: a uniform T gives exactly zero; a
+    quadratic T = T0 + c x^2 (Laplacian 2c) gives rho cp dT/dt = 2 k c mu,
+    with the same lattice moment mu as the velocity gradient (all particles
+    share m/rho, so the reference's second-partner volume is uniform)."""
+    k, cp = 50.0, 460.0
+    c = cases.config1(steps=1)
+    d = oracle_from_case(c)
+    d.set_param("thermal_solver", 1)
+    d.f("KT")[:] = k
+    d.f("CPT")[:] = cp
+    d.f("T")[:] = 300.0
+    d.neighbour_search()
+    d.thermal_calcs(0.0)
+    assert np.all(d.f("DTDT") == 0.0)
+    cq = 4.0e6                               # K/m^2
+    d = oracle_from_case(c)
+    d.set_param("thermal_solver", 1)
+    d.f("KT")[:] = k
+    d.f("CPT")[:] = cp
+    d.f("T")[:] = 300.0 + cq * c.x[:, 0] ** 2
+    d.neighbour_search()
+    d.thermal_calcs(0.0)
+    mu = _lattice_moment(c)
+    m = _interior(c, d)
+    want = 2.0 * k * cq * mu / (2700.0 * cp)
+    assert np.abs(d.f("DTDT")[m] - want).max() < 1e-9 * want
+
+
+def _two_particles(thermal=True):
+    """Particles 0 and 1 interact; 2 and 3 are far away and only give the
+    cell grid of CellInitiate its extent."""
+    d = po.Domain(4)
+    d.f("X")[:] = np.array([[0.0, 0.0, 0.0], [0.8e-3, 0.3e-3, -0.1e-3], [20e-3, 20e-3, 20e-3],
+                            [-20e-3, -20e-3, -20e-3]])
+    d.f("H")[:] = 1.2e-3
+    d.f("MASS")[:] = [1.0e-3, 2.0e-3, 1.0e-3, 1.0e-3]
+    d.f("RHO")[:] = d.f("REFRHO")[:] = [1000.0, 4000.0, 1000.0, 1000.0]
+    d.f("SIGMAY")[:] = 1e12
+    d.f("KT")[:] = [50.0, 20.0, 50.0, 50.0]
+    d.f("CPT")[:] = [460.0, 500.0, 460.0, 460.0]
+    d.f("T")[:] = [300.0, 350.0, 300.0, 300.0]
+    d.i("FREE")[:] = 1
+    d.set_material(0, model=po.BILINEAR, E=1e9, nu=0.3)
+    d.set_param("thermal_solver", int(thermal))
+    d.neighbour_search()
+    return d
+
+
+def test_conduction_pair_closed_form_and_volume_quirk():
+    """Two particles: m = V_second 4 k1 k2/(k1+k2) (T_first - T_second) GK
+    goes +m to the first and -m to the second partner, with the SECOND
+    particle's m/rho for both (Source/Thermal.cpp:53-219)."""
+    d = _two_particles()
+    (p1, p2), = d.pairs().tolist()
+    d.thermal_calcs(0.0)
+    x, T, kT = d.f("X"), d.f("T"), d.f("KT")
+    h = 1.2e-3
+    r = float(np.linalg.norm(x[p1] - x[p2]))
+    V2 = d.f("MASS")[p2] / d.f("RHO")[p2]
+    m = V2 * 4.0 * kT[p1] * kT[p2] / (kT[p1] + kT[p2]) * (T[p1] - T[p2]) * po.grad_kernel(r / h, h)
+    rc = d.f("RHO") * d.f("CPT")
+    assert d.f("DTDT")[p1] == pytest.approx(m / rc[p1], rel=1e-13)
+    assert d.f("DTDT")[p2] == pytest.approx(-m / rc[p2], rel=1e-13)
+    assert np.all(d.f("DTDT")[2:] == 0.0)
+    # heat flows from hot to cold
+    hot = int(np.argmax(T[:2]))
+    assert d.f("DTDT")[hot] < 0 < d.f("DTDT")[1 - hot]
+
+
+def test_convection_source_and_thermal_strain_rate():
+    """Isolated particle: q_conv = rho h_c (m/rho)^(2/3) (T_inf - T) / m
+    (Source/Thermal.cpp:339-389), dT/dt = (q_conv + q_src)/(rho cp), and the
+    strain rate gains th_exp dT/dt on its diagonal (ThermalStuct.cpp:7-12)."""
+    d = po.Domain(3)
+    m, rho, hc, Tinf, T0, qs, cp, ath = 2e-4, 7850.0, 35.0, 293.0, 900.0, 5e7, 460.0, 1.2e-5
+    d.f("X")[:] = np.array([[0.0, 0.0, 0.0], [20e-3, 20e-3, 20e-3], [-20e-3, -20e-3, -20e-3]])
+    d.f("H")[:] = 1e-3
+    d.f("MASS")[:] = m
+    d.f("RHO")[:] = d.f("REFRHO")[:] = rho
+    d.f("T")[:] = T0
+    d.f("CPT")[:] = cp
+    d.f("KT")[:] = 40.0
+    d.f("HCONV")[:] = hc
+    d.f("TINF")[:] = Tinf
+    d.f("QSRC")[:] = qs
+    d.f("THEXP")[:] = ath
+    d.i("THBC")[:] = [1, 0, 0]
+    d.f("QSRC")[1:] = 0.0
+    d.set_material(0, model=po.BILINEAR, E=1e9, nu=0.3)
+    d.set_param("thermal_solver", 1)
+    d.neighbour_search()
+    d.thermal_calcs(0.0)
+    qc = rho * hc * (m / rho) ** (2.0 / 3.0) * (Tinf - T0) / m
+    assert d.f("QCONV")[0] == pytest.approx(qc, rel=1e-14)
+    dTdt = (qc + qs) / (rho * cp)
+    assert d.f("DTDT")[0] == pytest.approx(dTdt, rel=1e-14)
+    sr = d.f("SR")[0].reshape(3, 3)
+    assert np.allclose(np.diag(sr), ath * dTdt, rtol=1e-14) and np.all(sr[~np.eye(3, dtype=bool)] == 0)
+    # the next call first advances T with the stored rate
+    d.thermal_calcs(1e-3)
+    assert d.f("T")[0] == pytest.approx(T0 + 1e-3 * dTdt, rel=1e-15)
+
+
+def test_plastic_work_heat_closed_forms():
+    """q_pl = sigma : (d eps_pl / dt) with both off-diagonal halves counted:
+    uniaxial s * e / dt, pure shear tau * gamma / dt.  ps_energy and
+    plastic_work integrate it; a particle whose plastic increment stopped
+    keeps its last q_pl (delta_pl_strain is never reset in the reference)."""
+    dt, s, e, tau, gam = 1e-7, 3e8, 2e-4, 1.5e8, 6e-4
+    d = po.Domain(2)
+    d.f("H")[:] = 1e-3
+    d.f("MASS")[:] = [1e-3, 2e-3]
+    d.f("RHO")[:] = [1000.0, 2000.0]
+    d.set_param("pl_heating", 1)
+    sig = np.zeros((2, 3, 3))
+    inc = np.zeros((2, 3, 3))
+    sig[0, 0, 0] = s
+    inc[0] = np.diag([e, -e / 2, -e / 2])
+    sig[1, 0, 1] = sig[1, 1, 0] = tau
+    inc[1, 0, 1] = inc[1, 1, 0] = gam / 2
+    d.f("SIGMA")[:] = sig
+    d.f("EPLINC")[:] = inc
+    d.f("DPLSTRAIN")[:] = 1e-4
+    d.plastic_work_heat(dt)
+    q = d.f("QPL").copy()
+    assert q[0] == pytest.approx(s * e / dt, rel=1e-14)
+    assert q[1] == pytest.approx(tau * gam / dt, rel=1e-14)
+    assert np.allclose(d.f("PSENERGY"), q * dt, rtol=1e-14)
+    assert d.param("plastic_work") == pytest.approx((1e-6 * q[0] + 1e-6 * q[1]) * dt, rel=1e-14)
+    d.f("DPLSTRAIN")[:] = 0.0
+    d.f("SIGMA")[:] = 0.0
+    d.plastic_work_heat(dt)
+    assert np.array_equal(d.f("QPL"), q)
+    assert np.allclose(d.f("PSENERGY"), 2 * q * dt, rtol=1e-14)
+
+
+def test_energy_sums():
+    """Internal energy: sum of (m/rho) S : eps_dot dt with both off-diagonal
+    halves (Source/Domain.cpp:1810-1829); the kinetic-energy pair terms
+    (Source/Domain.cpp:1752-1808) are added to one partner and subtracted from
+    the other, so their sum is exactly zero."""
+    d = _two_particles(thermal=False)
+    dt = 2e-7
+    d.set_param("deltat", dt)
+    S = np.array([[1.0, 0.5, -0.2], [0.5, -0.7, 0.3], [-0.2, 0.3, -0.3]]) * 1e8
+    R = np.array([[2.0, 1.0, 0.4], [1.0, -1.0, -0.6], [0.4, -0.6, 0.5]]) * 1e2
+    d.f("SHEAR")[:2] = np.stack([S, 2 * S])
+    d.f("SR")[:2] = np.stack([R, -R])
+    d.f("V")[:2] = [[1.0, 0.0, 0.0], [-2.0, 0.5, 0.0]]
+    d.f("P")[:2] = [1e8, 3e8]
+    d.energies()
+    V = d.f("MASS") / d.f("RHO")
+    SR = float(np.sum(S * R))
+    assert d.param("int_energy_sum") == pytest.approx((V[0] * SR - V[1] * 2 * SR) * dt, rel=1e-13)
+    assert d.param("kin_energy_sum") == 0.0

@@ -35,6 +35,9 @@ class Case:
     thermal: bool = False
     pl_heating: bool = False
     top_zone: int = 3
+    thbc: np.ndarray = None     # Thermal_BC per particle (1 = convection)
+    hconv: np.ndarray = None
+    tinf: np.ndarray = None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -130,6 +133,27 @@ def small_jc(steps=200, dx=1.0e-3, R=7.5e-3, L=30e-3, seed=7):
                          steps=steps, jitter=1e-3, seed=seed, T0=300.0)
 
 
+STEEL_JC = dict(model=capi.JOHNSON_COOK, alpha=1.0, beta=0.0, jc_A=792e6, jc_B=510e6, jc_n=0.26,
+                jc_C=0.014, jc_eps0=1.0, jc_m=1.03, jc_Tm=1793.0, jc_Tt=293.0,
+                k_T=50.0, cp_T=460.0, th_exp=1.2e-5)
+
+
+def thermal_small(steps=150, v_top=-20.0, seed=11):
+    """Thermo-mechanical upsetting at config-1 size: steel-like JC billet
+    (rho 7850, E 200 GPa, nu 0.29, k 50 W/mK, cp 460 J/kgK), T0 = 300 K,
+    conduction + plastic heating (0.9), bottom row held at 500 K, top row
+    cooled by convection (h_c = 1e4 W/m^2K, T_inf = 293 K)."""
+    c = cylinder_case("cyl_upset_4k_thermal", 7.5e-3, 30e-3, 1.0e-3, 7850.0, 200e9, 0.29, STEEL_JC,
+                      steps=steps, jitter=1e-3, seed=seed, v_top=v_top, T0=300.0)
+    c.thermal = True
+    c.pl_heating = True
+    c.tbcs = [dict(zone=2, T=500.0)]
+    c.thbc = (c.ids == 3).astype(np.int32)
+    c.hconv = np.where(c.ids == 3, 1.0e4, 0.0)
+    c.tinf = np.full(c.n, 293.0)
+    return c
+
+
 def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,
@@ -143,6 +167,10 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s.set("sigmay", case.sigmay)
     if case.T is not None:
         s.set("T", case.T)
+    if case.thbc is not None:
+        s.set_int("thermal_bc", case.thbc)
+        s.set("h_conv", case.hconv)
+        s.set("T_inf", case.tinf)
     s.set_int("id", case.ids)
     s.set_int("material", case.material)
     for a in case.amps:

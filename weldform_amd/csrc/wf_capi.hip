@@ -373,6 +373,7 @@ void allocate(Ctx *c) {
     }
     d.vmf = c->alloc<float4>(np);
     d.srecf = c->alloc<StressRecF>(np);
+    d.prf = c->alloc<float>(np);
   }
   d.ntiles = (int)(np / kBlock);
   d.stage_max = 0;
@@ -796,7 +797,7 @@ int wf_set_field(void *ctx, int32_t field, const double *h) {
       case WF_F_PS_ENERGY: put_scalar(c, d.psen, h); break;
       default: throw Error(WF_EINVAL, "field not settable");
     }
-    launch_shadows(d, c->st);
+    launch_shadows(d, c->st, c->cur);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     return WF_OK;
@@ -978,7 +979,7 @@ int wf_init(void *ctx, double dt) {
     make_permutation(c);
     allocate(c);
     upload_all(c);
-    launch_shadows(c->d, c->st);
+    launch_shadows(c->d, c->st, c->cur);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     c->dt = dt;

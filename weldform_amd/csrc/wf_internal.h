@@ -60,10 +60,13 @@ struct __align__(64) StressRec {
   double irho2;  // 1 / (rho * rho)
 };
 
+// FP32 record gathered by the mixed-precision acceleration pass: the
+// stress already divided by rho^2 (the factor the pair term applies), formed
+// in FP64 and rounded once
 struct __align__(32) StressRecF {
-  float s[6];
+  float t[6];    // sigma / rho^2
   float rho;
-  float irho2;
+  float pad;
 };
 
 struct DevMat {
@@ -162,6 +165,7 @@ struct Dev {
   float4 *xhf[2];              // (x - xc, y - yc, z - zc, mass/rho)
   float4 *vmf;                 // (vx, vy, vz, mass)
   StressRecF *srecf;
+  float *prf;                  // P / rho^2 (EOS of the current density)
   // neighbours
   int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *scan_tmp;
   int *nnb, *nbr;
@@ -195,6 +199,6 @@ void launch_stress(const Dev &d, hipStream_t st, int buf, double dt);
 void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt);
 void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz);
 void launch_max_pl(const Dev &d, hipStream_t st);
-void launch_shadows(const Dev &d, hipStream_t st);
+void launch_shadows(const Dev &d, hipStream_t st, int cur);
 
 }  // namespace wf

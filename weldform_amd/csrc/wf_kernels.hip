@@ -183,11 +183,11 @@ struct Src<false> {
   __device__ static KH<R> kh(const KH<R> &ki, const R4 &xi, const R4 &xj) { return pair_h_t<R>(ki, xi.w, xj.w); }
   __device__ static R vol(const Dev &d, int b, int j, const R4 &xj) { return d.vol[b][j]; }
   __device__ static R rho(const Dev &d, int b, int j, const R4 &xj, const R4 &vj) { return d.rho[b][j]; }
-  __device__ static void srec(const Dev &d, int j, R s[6], R &rho, R &irho2) {
+  // t = sigma_j / rho_j^2 formed as the reference forms it per pair
+  __device__ static void srec(const Dev &d, int j, R t[6], R &rho) {
     const StressRec r = d.srec[j];
-    for (int k = 0; k < 6; ++k) s[k] = r.s[k];
+    for (int k = 0; k < 6; ++k) t[k] = r.s[k] * r.irho2;
     rho = r.rho;
-    irho2 = r.irho2;
   }
 };
 template <>
@@ -200,11 +200,10 @@ struct Src<true> {
   __device__ static KH<R> kh(const KH<R> &ki, const R4 &xi, const R4 &xj) { return ki; }
   __device__ static R vol(const Dev &d, int b, int j, const R4 &xj) { return xj.w; }
   __device__ static R rho(const Dev &d, int b, int j, const R4 &xj, const R4 &vj) { return vj.w * rcp_t(xj.w); }
-  __device__ static void srec(const Dev &d, int j, R s[6], R &rho, R &irho2) {
+  __device__ static void srec(const Dev &d, int j, R t[6], R &rho) {
     const StressRecF r = d.srecf[j];
-    for (int k = 0; k < 6; ++k) s[k] = r.s[k];
+    for (int k = 0; k < 6; ++k) t[k] = r.t[k];
     rho = r.rho;
-    irho2 = r.irho2;
   }
 };
 
@@ -561,7 +560,16 @@ struct HbmJ {
   __device__ int self() const { return i; }
   __device__ R4 x(int j) const { return S::x(d, buf, j); }
   __device__ R4 v(int j) const { return S::v(d, j); }
-  __device__ void srec(int j, R s[6], R &rho, R &irho2) const { S::srec(d, j, s, rho, irho2); }
+  __device__ void srec(int j, R t[6], R &rho) const { S::srec(d, j, t, rho); }
+  // P_j / rho_j^2 of the kinetic-energy pair term
+  __device__ R pr(int j, R rhoj, R P0, R cs2, R rho0) const {
+    if constexpr (MX) {
+      return d.prf[j];
+    } else {
+      const R irj = rcp_t(rhoj);
+      return (P0 + cs2 * (rhoj - rho0)) * irj * irj;
+    }
+  }
   __device__ bool lower(int i, int j) const { return i < j; }
 };
 struct LdsJ {
@@ -570,7 +578,9 @@ struct LdsJ {
   static constexpr int kUnroll = kNbBatch;
   const Dev &d;
   const float4 *lx, *lv;
-  const StressRecF *ls;
+  const float4 *lt;   // accel: (t0..t3) per staged particle
+  const float2 *lt2;  // accel: (t4, t5)
+  const float *lp;    // rates: P / rho^2
   const int i;
   const int own;  // i's position in the staged set
   __device__ void fetch(int k0, int out[kNbBatch]) const {
@@ -583,12 +593,14 @@ struct LdsJ {
   __device__ int self() const { return own; }
   __device__ R4 x(int l) const { return lx[l]; }
   __device__ R4 v(int l) const { return lv[l]; }
-  __device__ void srec(int l, R s[6], R &rho, R &irho2) const {
-    const StressRecF r = ls[l];
-    for (int k = 0; k < 6; ++k) s[k] = r.s[k];
-    rho = r.rho;
-    irho2 = r.irho2;
+  // accel staging keeps rho in the w slot of lx
+  __device__ void srec(int l, R t[6], R &rho) const {
+    const float4 a = lt[l];
+    const float2 b = lt2[l];
+    t[0] = a.x, t[1] = a.y, t[2] = a.z, t[3] = a.w, t[4] = b.x, t[5] = b.y;
+    rho = lx[l].w;
   }
+  __device__ R pr(int l, R, R, R, R) const { return lp[l]; }
   // staged positions follow particle index order
   __device__ bool lower(int, int l) const { return own < l; }
 };
@@ -632,15 +644,30 @@ __device__ inline void combine_lanes(R (&v)[NV], R *red) {
   }
 }
 
-// copy the tile's staged set into LDS (all threads of the block)
-__device__ inline void stage_in(const Dev &d, int t, int U, const float4 *gx, float4 *lx, float4 *lv,
-                                StressRecF *ls) {
+// copy the tile's staged set into LDS (all threads of the block call these)
+// density / rates: (x, m/rho), (v, m) and, for rates, P / rho^2
+__device__ inline void stage_xv(const Dev &d, int t, int U, const float4 *gx, float4 *lx, float4 *lv, float *lp) {
   const int *st = d.stage + (long long)t * kStageMax;
   for (int p = threadIdx.x; p < U; p += blockDim.x) {
     const int j = st[p];
     lx[p] = gx[j];
     lv[p] = d.vmf[j];
-    if (ls) ls[p] = d.srecf[j];
+    if (lp) lp[p] = d.prf[j];
+  }
+  __syncthreads();
+}
+// acceleration: (x, rho), (v, m), sigma / rho^2 as (t0..t3), (t4, t5)
+__device__ inline void stage_accel(const Dev &d, int t, int U, const float4 *gx, float4 *lx, float4 *lv, float4 *lt,
+                                   float2 *lt2) {
+  const int *st = d.stage + (long long)t * kStageMax;
+  for (int p = threadIdx.x; p < U; p += blockDim.x) {
+    const int j = st[p];
+    const StressRecF r = d.srecf[j];
+    const float4 x = gx[j];
+    lx[p] = make_float4(x.x, x.y, x.z, r.rho);
+    lv[p] = d.vmf[j];
+    lt[p] = make_float4(r.t[0], r.t[1], r.t[2], r.t[3]);
+    lt2[p] = make_float2(r.t[4], r.t[5]);
   }
   __syncthreads();
 }
@@ -657,36 +684,37 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
     const bool av = (M.alpha != 0.0 || M.beta != 0.0);
     const auto xi = S::x(d, cur, i);
     const auto vi = S::v(d, i);
-    R si[6], rhoi, irhoi;
-    S::srec(d, i, si, rhoi, irhoi);
-    R ti[6];
-    for (int k = 0; k < 6; ++k) ti[k] = si[k] * irhoi;
+    R ti[6], rhoi;
+    S::srec(d, i, ti, rhoi);
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
+    const R acs = alpha * cs;
     for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
-      R sj[6], rhoj, irhoj;
-      g.srec(j, sj, rhoj, irhoj);
+      R tj[6], rhoj;
+      g.srec(j, tj, rhoj);
       const KH<R> kh = S::kh(ki, xi, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R r2 = dx * dx + dy * dy + dz * dz;
       const R GK = gk_t<R>(r2, kh);
-      // Monaghan viscosity for approaching pairs (selected, not branched)
+      // Monaghan viscosity for approaching pairs (selected, not branched):
+      // mu = h vdot / D1, Pi = mu (alpha cs + beta mu) / D2 with a single
+      // reciprocal of D1 D2 (D1 = r^2 + 0.01 h^2, D2 = (rho_i + rho_j) / 2)
       const R vdot = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-      const R mu = kh.h * vdot * rcp_t(r2 + (R)0.01 * kh.h * kh.h);
-      const R pv = (alpha * cs * mu + beta * mu * mu) * rcp_t((R)0.5 * (rhoi + rhoj));
+      const R D1 = r2 + (R)0.01 * kh.h * kh.h, D2 = (R)0.5 * (rhoi + rhoj);
+      const R hv = kh.h * vdot * rcp_t(D1 * D2);
+      const R pv = hv * (acs + beta * (hv * D2));
       const R pi = (av && vdot < (R)0) ? pv : (R)0;
-      const R Mxx = ti[XX] + sj[XX] * irhoj + pi;
-      const R Myy = ti[YY] + sj[YY] * irhoj + pi;
-      const R Mzz = ti[ZZ] + sj[ZZ] * irhoj + pi;
-      const R Mxy = ti[XY] + sj[XY] * irhoj;
-      const R Mxz = ti[XZ] + sj[XZ] * irhoj;
-      const R Myz = ti[YZ] + sj[YZ] * irhoj;
-      const R gx = GK * dx, gy = GK * dy, gz = GK * dz;
-      const R mj = vj.w;
-      ax += mj * (gx * Mxx + gy * Mxy + gz * Mxz);
-      ay += mj * (gx * Mxy + gy * Myy + gz * Myz);
-      az += mj * (gx * Mxz + gy * Myz + gz * Mzz);
+      const R Mxx = ti[XX] + tj[XX] + pi;
+      const R Myy = ti[YY] + tj[YY] + pi;
+      const R Mzz = ti[ZZ] + tj[ZZ] + pi;
+      const R Mxy = ti[XY] + tj[XY];
+      const R Mxz = ti[XZ] + tj[XZ];
+      const R Myz = ti[YZ] + tj[YZ];
+      const R w = vj.w * GK;
+      ax += w * (dx * Mxx + dy * Mxy + dz * Mxz);
+      ay += w * (dx * Mxy + dy * Myy + dz * Myz);
+      az += w * (dx * Mxz + dy * Myz + dz * Mzz);
     });
   }
   acc[0] = ax;
@@ -709,8 +737,8 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
 // mixed precision: LDS-staged gathers where the tile is staged, else HBM
 template <int LANES>
 __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
-  __shared__ float4 lx[kStageMax], lv[kStageMax];
-  __shared__ StressRecF ls[kStageMax];
+  __shared__ float4 lx[kStageMax], lv[kStageMax], lt[kStageMax];
+  __shared__ float2 lt2[kStageMax];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
@@ -719,8 +747,8 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
   if (U < 0) {
     if (i < d.n) accel_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
   } else {
-    stage_in(d, t, U, d.xhf[cur], lx, lv, ls);
-    if (i < d.n) accel_lane<true>(d, cur, i, LdsJ{d, lx, lv, ls, i, d.selfl[i]}, sub, LANES, a);
+    stage_accel(d, t, U, d.xhf[cur], lx, lv, lt, lt2);
+    if (i < d.n) accel_lane<true>(d, cur, i, LdsJ{d, lx, lv, lt, lt2, nullptr, i, d.selfl[i]}, sub, LANES, a);
   }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
@@ -868,8 +896,12 @@ __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, co
       const R volj = MX ? (R)xj.w : S::vol(d, cur, j, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, S::kh(ki, xi, xj));
-      const R t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
-      s += volj * t;
+      if constexpr (MX) {
+        s += (volj * GK) * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+      } else {
+        const R t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
+        s += volj * t;
+      }
     });
   }
   return s;
@@ -898,6 +930,8 @@ __device__ inline void dens_update(const Dev &d, int cur, int i, double s, doubl
   d.xh[nxt][i] = xn;
   if (MX) {
     d.xhf[nxt][i] = make_float4((float)(xn.x - d.xc[0]), (float)(xn.y - d.xc[1]), (float)(xn.z - d.xc[2]), (float)vn);
+    const DevMat &M = d.mats[d.mat[i]];
+    d.prf[i] = (float)((M.P0 + M.Cs * M.Cs * (rn - M.ref_density)) / (rn * rn));
   }
 }
 
@@ -923,8 +957,8 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, doubl
   if (U < 0) {
     if (i < d.n) s[0] = dens_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES);
   } else {
-    stage_in(d, t, U, d.xhf[cur], lx, lv, nullptr);
-    if (i < d.n) s[0] = dens_lane<true>(d, cur, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]}, sub, LANES);
+    stage_xv(d, t, U, d.xhf[cur], lx, lv, nullptr);
+    if (i < d.n) s[0] = dens_lane<true>(d, cur, i, LdsJ{d, lx, lv, nullptr, nullptr, nullptr, i, d.selfl[i]}, sub, LANES);
   }
   combine_lanes<LANES>(s, red);
   if (i < d.n && sub == 0) dens_update<true>(d, cur, i, (double)s[0], dt, do_update);
@@ -973,17 +1007,14 @@ __device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int 
     const DevMat &M = d.mats[d.mat[i]];
     const auto xi = Sr::x(d, buf, i);
     const auto vi = Sr::v(d, i);
-    const R rhoi = Sr::rho(d, buf, i, xi, vi);
     const R P0 = (R)M.P0, cs2 = (R)(M.Cs * M.Cs), rho0 = (R)M.ref_density;
-    const R irhoi = rcp_t(rhoi);
-    const R pri = (P0 + cs2 * (rhoi - rho0)) * irhoi * irhoi;
+    const R pri = g.pr(MX ? g.self() : i, MX ? (R)0 : Sr::rho(d, buf, i, xi, vi), P0, cs2, rho0);
     const R mi = vi.w;
     const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi));
     for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
       const R volj = MX ? (R)xj.w : Sr::vol(d, buf, j, xj);
-      const R rhoj = MX ? vj.w * rcp_t((R)xj.w) : Sr::rho(d, buf, j, xj, vj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, Sr::kh(ki, xi, xj));
       const R vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
@@ -999,11 +1030,10 @@ __device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int 
       w[2] += c * (vy * dz - vz * dy);
       // kinetic-energy pair term (Source/Domain.cpp:1776-1792): the pair
       // is oriented lower-index first; its sum over particles vanishes
-      const R irj = rcp_t(rhoj);
-      const R Pj = P0 + cs2 * (rhoj - rho0);
+      const R prj = g.pr(j, MX ? (R)0 : Sr::rho(d, buf, j, xj, vj), P0, cs2, rho0);
       const bool lo = g.lower(i, j);
       const R msec = lo ? vj.w : mi;
-      const R term = (R)0.5 * msec * (pri + Pj * irj * irj) * (vx * (GK * dx) + vy * (GK * dy) + vz * (GK * dz));
+      const R term = (R)0.5 * msec * (pri + prj) * (vx * (GK * dx) + vy * (GK * dy) + vz * (GK * dz));
       ekin += lo ? term : -term;
     });
   }
@@ -1037,6 +1067,7 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_f64(Dev d, int buf) {
 template <int LANES>
 __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
   __shared__ float4 lx[kStageMax], lv[kStageMax];
+  __shared__ float lp[kStageMax];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 10 * kBlock];
   __shared__ double sh[kBlock * LANES / 64];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
@@ -1046,8 +1077,8 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
   if (U < 0) {
     if (i < d.n) rates_lane<true>(d, buf, i, HbmJ<true>{d, buf, i}, sub, LANES, v);
   } else {
-    stage_in(d, t, U, d.xhf[buf], lx, lv, nullptr);
-    if (i < d.n) rates_lane<true>(d, buf, i, LdsJ{d, lx, lv, nullptr, i, d.selfl[i]}, sub, LANES, v);
+    stage_xv(d, t, U, d.xhf[buf], lx, lv, lp);
+    if (i < d.n) rates_lane<true>(d, buf, i, LdsJ{d, lx, lv, nullptr, nullptr, lp, i, d.selfl[i]}, sub, LANES, v);
   }
   combine_lanes<LANES>(v, red);
   const double e_kin = (i < d.n && sub == 0) ? rates_store(d, i, v) : 0.0;
@@ -1135,9 +1166,9 @@ WF_KERNEL void k_stress(Dev d, int buf, double dt) {
       d.srec[i] = rec;
       if (MX) {
         StressRecF rf;
-        for (int k = 0; k < 6; ++k) rf.s[k] = (float)sig[k];
+        for (int k = 0; k < 6; ++k) rf.t[k] = (float)(sig[k] * rec.irho2);
         rf.rho = (float)rho;
-        rf.irho2 = (float)rec.irho2;
+        rf.pad = 0.0f;
         d.srecf[i] = rf;
       }
       const double vol = d.vol[buf][i];
@@ -1268,36 +1299,63 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
 // end of step: fixed-order reduction of the block partials, scalar updates,
 // reaction history, and the end-of-step half of the neighbour schedule.
 
+// All active slots are read in one sweep (every load of a thread in flight
+// together) and reduced together; per slot the order is the one of
+// block_reduce: each thread's blocks in order, the wavefront butterfly, then
+// the wavefronts in order.
 __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
-  __shared__ double sh[16];
+  __shared__ double sh[P_COUNT][16];
   const int nb = d.nblocks;
-  double res[P_COUNT];
+  bool active[P_COUNT];
+  int op[P_COUNT];
+  double v[P_COUNT];
+#pragma unroll
   for (int s = 0; s < P_COUNT; ++s) {
-    res[s] = 0.0;
-    const bool active = s == P_INT || s == P_KIN || (s == P_PLW && d.pl_heating) ||
-                        ((s == P_MAXT || s == P_MINT) && d.thermal) || (s >= P_REACT && s < P_REACT + 3 * nz);
-    if (!active) continue;
-    const int op = (s == P_MAXT) ? 1 : (s == P_MINT ? 2 : 0);
-    double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      const double x = part(d, s, b);
-      v = op == 0 ? v + x : (op == 1 ? fmax(v, x) : fmin(v, x));
-    }
-    res[s] = op == 0 ? block_reduce<0>(v, sh) : (op == 1 ? block_reduce<1>(v, sh) : block_reduce<2>(v, sh));
+    active[s] = s == P_INT || s == P_KIN || (s == P_PLW && d.pl_heating) ||
+                ((s == P_MAXT || s == P_MINT) && d.thermal) || (s >= P_REACT && s < P_REACT + 3 * nz);
+    op[s] = (s == P_MAXT) ? 1 : (s == P_MINT ? 2 : 0);
+    v[s] = op[s] == 0 ? 0.0 : (op[s] == 1 ? -INFINITY : INFINITY);
   }
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+#pragma unroll
+    for (int s = 0; s < P_COUNT; ++s) {
+      if (!active[s]) continue;
+      const double x = part(d, s, b);
+      v[s] = op[s] == 0 ? v[s] + x : (op[s] == 1 ? fmax(v[s], x) : fmin(v[s], x));
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int s = 0; s < P_COUNT; ++s) {
+    if (!active[s]) continue;
+    const double r = op[s] == 0 ? wave_sum(v[s]) : (op[s] == 1 ? wave_max(v[s]) : wave_min(v[s]));
+    if (lane == 0) sh[s][w] = r;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   const long long step = d.flags->step;
-  d.scalars[S_KIN] += res[P_KIN] * dt;
-  d.scalars[S_INT] += res[P_INT] * dt;
-  if (d.pl_heating) d.scalars[S_PLW] += res[P_PLW] * dt;
-  if (d.thermal) {
-    d.scalars[S_MAXT] = fmax(0.0, res[P_MAXT]);
-    d.scalars[S_MINT] = fmin(1000.0, res[P_MINT]);
-  }
-  if (!isfinite(res[P_INT])) d.flags->nan = 1;
   const long long slot = step % d.hist_cap;
-  for (int z = 0; z < nz; ++z)
-    for (int k = 0; k < 3; ++k) d.react_hist[(slot * kMaxZones + z) * 3 + k] = res[P_REACT + 3 * z + k];
+  double e_kin = 0.0, e_int = 0.0, plw = 0.0, tmax = 0.0, tmin = 0.0;
+#pragma unroll
+  for (int s = 0; s < P_COUNT; ++s) {
+    if (!active[s]) continue;
+    double r = sh[s][0];
+    for (int k = 1; k < nw; ++k) r = op[s] == 0 ? r + sh[s][k] : (op[s] == 1 ? fmax(r, sh[s][k]) : fmin(r, sh[s][k]));
+    if (s == P_KIN) e_kin = r;
+    if (s == P_INT) e_int = r;
+    if (s == P_PLW) plw = r;
+    if (s == P_MAXT) tmax = r;
+    if (s == P_MINT) tmin = r;
+    if (s >= P_REACT) d.react_hist[slot * kMaxZones * 3 + (s - P_REACT)] = r;
+  }
+  d.scalars[S_KIN] += e_kin * dt;
+  d.scalars[S_INT] += e_int * dt;
+  if (d.pl_heating) d.scalars[S_PLW] += plw * dt;
+  if (d.thermal) {
+    d.scalars[S_MAXT] = fmax(0.0, tmax);
+    d.scalars[S_MINT] = fmin(1000.0, tmin);
+  }
+  if (!isfinite(e_int)) d.flags->nan = 1;
   DevFlags *f = d.flags;
   if (f->step_yield) {
     if (f->ts_i == f->ts_nb_inc - 1) f->cleared = 1;
@@ -1332,7 +1390,7 @@ __global__ __launch_bounds__(256) void k_max_pl_final(Dev d) {
 }
 
 // rebuild every FP32 shadow from the FP64 state (set-up and host writes)
-WF_KERNEL void k_shadows(Dev d) {
+WF_KERNEL void k_shadows(Dev d, int cur) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= d.n) return;
   for (int b = 0; b < 2; ++b) {
@@ -1343,14 +1401,17 @@ WF_KERNEL void k_shadows(Dev d) {
   d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
   const StressRec r = d.srec[i];
   StressRecF f;
-  for (int k = 0; k < 6; ++k) f.s[k] = (float)r.s[k];
+  for (int k = 0; k < 6; ++k) f.t[k] = (float)(r.s[k] * r.irho2);
   f.rho = (float)r.rho;
-  f.irho2 = (float)r.irho2;
+  f.pad = 0.0f;
   d.srecf[i] = f;
+  const DevMat &M = d.mats[d.mat[i]];
+  const double rho = d.rho[cur][i];
+  d.prf[i] = (float)((M.P0 + M.Cs * M.Cs * (rho - M.ref_density)) / (rho * rho));
 }
 
-void launch_shadows(const Dev &d, hipStream_t st) {
-  if (d.mixed) k_shadows<<<nblk(d.n), kBlock, 0, st>>>(d);
+void launch_shadows(const Dev &d, hipStream_t st, int cur) {
+  if (d.mixed) k_shadows<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
 }
 
 void launch_max_pl(const Dev &d, hipStream_t st) {
