@@ -30,6 +30,7 @@
 
 #include "../weldform_hip.h"
 #include "Material.h"
+#include "Mesh.h"
 #include "Particle.h"
 #include "matvec.h"
 
@@ -93,6 +94,16 @@ class Domain {
   bool thermal_solver = false;
   bool pl_heating = false;
   bool contact = false;
+  // rigid contact (Source/Domain.h:260-330): planes, their first particle
+  // index and zone ids, Wang contact parameters and outputs
+  std::vector<TriMesh *> trimesh;
+  std::vector<int> first_fem_particle_idx, contact_surf_id;
+  int meshcount = 0;
+  double friction_sta = 0.0, friction_dyn = 0.0, DFAC = 0.0, PFAC = 0.6;
+  bool cont_heat_cond = false, cont_heat_fric = false;
+  int contact_alg = 0;  // Wang
+  std::vector<Vec3_t> m_contact_force;
+  double contact_force_sum = 0.0, contact_reaction_sum = 0.0, ext_forces_work = 0.0;
   double pl_work_heat_frac = 0.9;
   int solid_part_count = -1;
   double kin_energy_sum = 0.0, int_energy_sum = 0.0, plastic_work = 0.0;
@@ -218,6 +229,31 @@ class Domain {
     solve(tf, dt, dtOut, TheFileKey, maxidx, WF_KICKDRIFT);
   }
 
+  // Domain::AddTrimeshParticles (Source/Contact.cpp:10-37): one particle per
+  // element at its centroid, h = hfac * element radius, zone id `id`
+  void AddTrimeshParticles(TriMesh *mesh, const float &hfac, const int &id) {
+    if (!mesh || mesh->dens <= 0) throw new Fatal("AddTrimeshParticles: the mesh must come from AxisPlaneMesh");
+    first_fem_particle_idx.push_back((int)Particles.Size());
+    contact_surf_id.push_back(id);
+    trimesh.push_back(mesh);
+    mesh->id = id;
+    mesh_hfac_.push_back(hfac);
+    for (size_t e = 0; e < mesh->element.Size(); ++e) {
+      Particle *P = new Particle(id, mesh->element[e]->centroid, Vec3_t(0, 0, 0), 0.0, 0.0,
+                                 (double)hfac * mesh->element[e]->radius, false);
+      P->normal = mesh->element[e]->normal;
+      P->element = (int)e;
+      P->mesh = meshcount;
+      Particles.Push(P);
+    }
+    m_contact_force.push_back(Vec3_t());
+    meshcount++;
+    contact = true;
+  }
+
+  // contact force on plane m (Domain::m_contact_force[m]) at every step of the solve
+  const std::vector<Vec3_t> &ContactForceHistory(int m) { return cforce_hist_[m]; }
+
   // reaction sum(m a) of a velocity-BC zone at the last step (taken before the
   // BC zeroes a), and its history over the solve
   Vec3_t ZoneReaction(int zone) const {
@@ -230,7 +266,7 @@ class Domain {
   // XDMF/HDF5 output is not built; the particle table goes to key.csv instead
   void WriteXDMF(char const *FileKey) { WriteCSV(FileKey); }
 
-  // WriteCSV (Source/Output.cpp:521-554): same columns; contact forces are 0
+  // WriteCSV (Source/Output.cpp:521-554): same columns
   void WriteCSV(char const *FileKey) {
     std::ostringstream oss;
     oss << "X, Y, Z, ID, Sigma_eq, Pl_Strain, vx, vy, vz, ax, ay, az, CFx, CFy, CZFz, p, Temp" << std::endl;
@@ -239,8 +275,8 @@ class Domain {
       for (int j = 0; j < 3; ++j) oss << P->x(j) << ", ";
       P->CalculateEquivalentStress();
       oss << P->ID << ", " << P->Sigma_eq << ", " << P->pl_strain << ", " << P->v(0) << ", " << P->v(1) << ", "
-          << P->v(2) << ", " << P->a(0) << ", " << P->a(1) << ", " << P->a(2) << ", " << 0.0 << ", " << 0.0
-          << ", " << 0.0 << ", " << P->Pressure << ", " << P->T << std::endl;
+          << P->v(2) << ", " << P->a(0) << ", " << P->a(1) << ", " << P->a(2) << ", " << P->contforce(0) << ", "
+          << P->contforce(1) << ", " << P->contforce(2) << ", " << P->Pressure << ", " << P->T << std::endl;
     }
     std::ofstream of(std::string(FileKey) + ".csv", std::ios::out);
     of << oss.str();
@@ -251,6 +287,11 @@ class Domain {
   int KernelType = 0, GradientType = 0;
   void *ctx_ = nullptr;
   std::map<int, std::vector<Vec3_t>> reaction_;
+  std::map<int, std::vector<Vec3_t>> cforce_hist_;
+  std::vector<float> mesh_hfac_;
+
+  // solid particles come first; rigid-surface particles follow them
+  size_t nsolid() const { return contact && !first_fem_particle_idx.empty() ? (size_t)first_fem_particle_idx[0] : Particles.Size(); }
   // steps per wf_step call: the device keeps the last 65536 reaction entries
   static constexpr int64_t kMaxChunk = 1 << 16;
 
@@ -271,10 +312,17 @@ class Domain {
     if (gradKernelCorr) throw new Fatal("kernel gradient correction is not implemented on the device path");
     if (auto_ts || auto_ts_acc)
       throw new Fatal("adaptive time stepping is not implemented on the device path: set auto_ts = false");
-    if (contact) throw new Fatal("rigid contact is not implemented on the device path of this build");
+    if (contact) {
+      if (trimesh.empty()) throw new Fatal("contact is set but no rigid mesh was added (AddTrimeshParticles)");
+      if (contact_alg != 0) throw new Fatal("only the Wang contact algorithm is implemented on the device path");
+      if (cont_heat_cond || cont_heat_fric) throw new Fatal("contact heat transfer is not implemented on the device path");
+      for (TriMesh *m : trimesh)
+        if (norm(m->m_w) != 0.0) throw new Fatal("rotating contact meshes are not implemented on the device path");
+      if (trimesh.size() > 4) throw new Fatal("at most 4 contact meshes");
+    }
     if (GeneralBefore != &General || GeneralAfter != &General)
       throw new Fatal("host callbacks cannot run inside a device step: express BCs through bConds / amps");
-    for (size_t i = 0; i < n; ++i)
+    for (size_t i = 0; i < nsolid(); ++i)
       if (Particles[i]->TI > 0.0) throw new Fatal("tensile instability (TI > 0) is not implemented");
   }
 
@@ -323,7 +371,7 @@ class Domain {
   }
 
   std::vector<int> build_materials(std::vector<wf_material> &table) {
-    const size_t n = Particles.Size();
+    const size_t n = nsolid();
     std::map<int, int> idx;
     table.clear();
     std::vector<int> mi(n);
@@ -344,7 +392,7 @@ class Domain {
   }
 
   void upload() {
-    const size_t n = Particles.Size();
+    const size_t n = nsolid();
     std::vector<double> v3(3 * n), v1(n), v9(9 * n);
     auto put3 = [&](int f, Vec3_t Particle::*m) {
       for (size_t i = 0; i < n; ++i)
@@ -392,7 +440,7 @@ class Domain {
   }
 
   void download() {
-    const size_t n = Particles.Size();
+    const size_t n = nsolid();
     std::vector<double> v3(3 * n), v1(n), v9(9 * n);
     auto get3 = [&](int f, Vec3_t Particle::*m) {
       check(wf_get_field(ctx_, f, v3.data()));
@@ -431,6 +479,22 @@ class Domain {
     get1(WF_F_Q_PLHEAT, &Particle::q_plheat);
     get1(WF_F_Q_CONV, &Particle::q_conv);
     get1(WF_F_PS_ENERGY, &Particle::ps_energy);
+    if (contact) {
+      get3(WF_F_CONTFORCE, &Particle::contforce);
+      std::vector<int32_t> id(n);
+      check(wf_get_int_field(ctx_, WF_I_ID, id.data()));
+      for (size_t i = 0; i < n; ++i) Particles[i]->ID = id[i];
+      const size_t nr = Particles.Size() - n;
+      std::vector<double> r3(3 * nr);
+      auto rig = [&](int f, Vec3_t Particle::*m) {
+        check(wf_get_rigid_field(ctx_, f, r3.data()));
+        for (size_t r = 0; r < nr; ++r)
+          for (int k = 0; k < 3; ++k) (Particles[n + r]->*m)(k) = r3[3 * r + k];
+      };
+      rig(WF_R_X, &Particle::x);
+      rig(WF_R_V, &Particle::v);
+      rig(WF_R_NORMAL, &Particle::normal);
+    }
     std::vector<int32_t> nb(n);
     check(wf_get_int_field(ctx_, WF_I_NB, nb.data()));
     for (size_t i = 0; i < n; ++i) {
@@ -440,6 +504,9 @@ class Domain {
     wf_scalars s{};
     check(wf_get_scalars(ctx_, &s));
     kin_energy_sum = s.kin_energy_sum;
+    contact_force_sum = s.contact_force_sum;
+    contact_reaction_sum = s.contact_reaction_sum;
+    ext_forces_work = s.ext_forces_work;
     int_energy_sum = s.int_energy_sum;
     plastic_work = s.plastic_work;
     m_maxT = s.max_T;
@@ -448,6 +515,13 @@ class Domain {
 
   void pull_reactions(int64_t nsteps) {
     std::vector<double> h(3 * (size_t)nsteps);
+    for (int m = 0; m < meshcount; ++m) {
+      int64_t got = 0;
+      check(wf_get_contact_force_history(ctx_, m, h.data(), nsteps, &got));
+      auto &dst = cforce_hist_[m];
+      for (int64_t s = 0; s < got; ++s) dst.push_back(Vec3_t(h[3 * s], h[3 * s + 1], h[3 * s + 2]));
+      if (got > 0) m_contact_force[m] = dst.back();
+    }
     for (const auto &b : bConds) {
       if (b.type != Velocity_BC) continue;
       int64_t got = 0;
@@ -467,7 +541,7 @@ class Domain {
     std::vector<wf_material> table;
     const std::vector<int> mi = build_materials(table);
     close();
-    const size_t n = Particles.Size();
+    const size_t n = nsolid();
     wf_config cfg{};
     cfg.precision = (int32_t)precision;
     cfg.ts_nb_inc = ts_nb_inc;
@@ -495,9 +569,17 @@ class Domain {
         check(wf_add_temperature_bc(ctx_, b.zoneId, b.T));
       }
     }
+    for (size_t m = 0; m < trimesh.size(); ++m) {
+      const TriMesh *M = trimesh[m];
+      const double a[3] = {M->p1(0), M->p1(1), M->p1(2)}, b[3] = {M->p2(0), M->p2(1), M->p2(2)};
+      const int rc = wf_add_rigid_plane(ctx_, M->id, M->axis, M->flip ? 1 : 0, a, b, M->dens, (double)mesh_hfac_[m]);
+      if (rc < 0) check(rc);
+    }
+    if (contact) check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));
     deltat = dt;
     check(wf_init(ctx_, dt));
     reaction_.clear();
+    cforce_hist_.clear();
     size_t idx_out = 1;
     double tout = Time;
     while (Time <= tf && idx_out <= maxidx) {

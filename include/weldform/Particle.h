@@ -24,6 +24,8 @@ class Particle {
   // kinematics
   Vec3_t x, v, a, Displacement, VXSPH, normal;
   Vec3_t va, vb;  // Verlet / leapfrog half-step copies of older drivers; unused
+  Vec3_t contforce;          // contact force of the last step (rigid contact)
+  int element = 0, mesh = 0; // rigid-surface particles: element and mesh index
   // state
   double Density, RefDensity, Mass, h;
   double dDensity = 0.0, Pressure = 0.0, P0 = 0.0, Cs = 0.0;

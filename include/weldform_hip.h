@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 2
+#define WF_ABI_VERSION 3
 
 /* error codes */
 #define WF_OK 0
@@ -96,7 +96,15 @@ extern "C" {
 #define WF_F_T_INF 25       /* [1] Particle::T_inf                            */
 #define WF_F_Q_SOURCE 26    /* [1] Particle::q_source                         */
 #define WF_F_PS_ENERGY 27   /* [1] Particle::ps_energy                        */
-#define WF_NFIELDS 28
+#define WF_F_CONTFORCE 28   /* [3] read-only: Particle::contforce of the last
+                               step's contact (Source/Contact_Wang.cpp)      */
+#define WF_NFIELDS 29
+
+/* rigid-surface particle fields (wf_get_rigid_field), 3 doubles each, in
+ * the order the planes were added, element order within a plane */
+#define WF_R_X 0            /* particle position (moves with the plane)      */
+#define WF_R_V 1            /* particle velocity                             */
+#define WF_R_NORMAL 2       /* element normal                                */
 
 /* int fields */
 #define WF_I_ID 0           /* Particle::ID (zone id used by BCs)            */
@@ -170,6 +178,10 @@ typedef struct wf_scalars {
   double plastic_work;      /* Domain::plastic_work                            */
   double max_pl_strain;     /* max over particles of pl_strain                 */
   double max_T, min_T;      /* Domain::m_maxT / m_minT                         */
+  double contact_force_sum; /* Domain::contact_force_sum (last step)          */
+  double contact_reaction_sum; /* Domain::contact_reaction_sum (last step)    */
+  double ext_forces_work;   /* Domain::ext_forces_work                         */
+  int64_t contact_pairs;    /* contact pairs of the last search (ContPairs)   */
 } wf_scalars;
 
 int wf_abi_version(void);
@@ -192,6 +204,24 @@ int wf_add_amplitude(void *ctx, int32_t id, int32_t npts, const double *time, co
 int wf_add_velocity_bc(void *ctx, int32_t zone, int32_t value_type, const double value[3],
                        int32_t amp_id, double amp_factor);
 int wf_add_temperature_bc(void *ctx, int32_t zone, double T);
+
+/* rigid contact surfaces, before wf_init.  wf_add_rigid_plane adds the
+ * plane mesh of TriMesh::AxisPlaneMesh (Source/Mesh.cpp:82-202): normal to
+ * `axis`, from p1 spanning p2, dens x dens squares of side
+ * (p2 - p1)[first in-plane axis] / dens split into two triangles, winding
+ * and normal set by `flip` (the reference's flipNormals); its particles
+ * (Domain::AddTrimeshParticles, Source/Contact.cpp:10-37) carry zone id
+ * `zone` and h = hfac * element radius.  A velocity BC on `zone` moves the
+ * plane.  Returns the number of triangles, or a negative WF_E* code.
+ * wf_set_contact sets Domain::friction_sta, friction_dyn and DFAC of the
+ * Wang penalty contact (Source/Contact_Wang.cpp:15-348). */
+int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, const double p1[3], const double p2[3],
+                       int32_t dens, double hfac);
+int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double damping);
+/* contact force on a plane (Domain::m_contact_force[mesh]) per step; last
+ * min(cap, steps) entries */
+int wf_get_contact_force_history(void *ctx, int32_t mesh, double *out3, int64_t cap, int64_t *count);
+int wf_get_rigid_field(void *ctx, int32_t field, double *host);
 
 /* solve */
 int wf_init(void *ctx, double dt);

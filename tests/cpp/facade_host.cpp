@@ -53,6 +53,20 @@ int main() {
     const int nb = dom.AssignZone(lo, hi, id);
     std::printf("cyl %zu %.17g %d\n", dom.Particles.Size(), dom.Particles[0]->Mass, nb);
   }
+  // rigid plane + AddTrimeshParticles (Source/Mesh.cpp:82-220, Source/Contact.cpp:10-37)
+  {
+    SPH::Domain dom;
+    dom.Dimension = 3;
+    dom.AddBoxLength(0, Vec3_t(0., 0., 0.), 4e-3, 4e-3, 4e-3, 0.5e-3, 2700.0, 1.2e-3, 1, 0, false, false);
+    const size_t ns = dom.Particles.Size();
+    SPH::TriMesh *mesh = new SPH::TriMesh;
+    mesh->AxisPlaneMesh(2, false, Vec3_t(-1e-3, -1e-3, 5e-3), Vec3_t(5e-3, 5e-3, 5e-3), 6);
+    dom.AddTrimeshParticles(mesh, 1.1, 10);
+    const SPH::Particle *R = dom.Particles[ns];
+    std::printf("mesh %zu %zu %.17g %.17g %.17g %d %d\n", dom.Particles.Size() - ns, mesh->element.Size(), R->h,
+                mesh->element[0]->radius, R->normal(2), R->ID, dom.first_fem_particle_idx[0]);
+    delete mesh;
+  }
   expect_fatal("auto_ts", [](SPH::Domain &d) { d.auto_ts = true; });
   expect_fatal("kernel", [](SPH::Domain &d) { d.Kernel_Set(SPH::Quintic_Spline); });
   expect_fatal("gradient", [](SPH::Domain &d) { d.Gradient_Approach_Set(SPH::Randles_Libersky); });

@@ -6,25 +6,31 @@ from weldform_amd import capi
 
 
 def oracle_from_case(case, threads=1):
+    """The case in the oracle: its solid particles first, then one rigid
+    particle per triangle of each rigid plane (the reference's order)."""
     po.set_threads(threads)
-    d = po.Domain(case.n)
-    d.f("X")[:] = case.x
-    d.f("H")[:] = case.h
-    d.f("MASS")[:] = case.mass
-    d.f("RHO")[:] = case.density
-    d.f("SIGMAY")[:] = case.sigmay
+    n = case.n
+    nrig = sum(2 * m["dens"] ** 2 for m in case.meshes)
+    d = po.Domain(n + nrig)
+    s = slice(0, n)
+    d.f("X")[s] = case.x
+    d.f("H")[s] = case.h
+    d.f("MASS")[s] = case.mass
+    d.f("RHO")[s] = case.density
+    d.f("SIGMAY")[s] = case.sigmay
     if case.T is not None:
-        d.f("T")[:] = case.T
+        d.f("T")[s] = case.T
     if case.thbc is not None:
-        d.i("THBC")[:] = case.thbc
-        d.f("HCONV")[:] = case.hconv
-        d.f("TINF")[:] = case.tinf
-    d.i("ID")[:] = case.ids
-    d.i("MATERIAL")[:] = case.material
-    d.i("MATIDX")[:] = case.material
+        d.i("THBC")[s] = case.thbc
+        d.f("HCONV")[s] = case.hconv
+        d.f("TINF")[s] = case.tinf
+    d.i("ID")[s] = case.ids
+    d.i("MATERIAL")[s] = case.material
+    d.i("MATIDX")[s] = case.material
     d.i("FREE")[:] = 1
     for idx, m in case.materials.items():
-        sel = case.material == idx
+        sel = np.zeros(n + nrig, dtype=bool)
+        sel[:n] = case.material == idx
         d.f("REFRHO")[sel] = m["ref_density"]
         d.f("CS")[sel] = m["Cs"]
         d.f("G")[sel] = m["G"]
@@ -53,6 +59,15 @@ def oracle_from_case(case, threads=1):
                           b.get("amp_factor", 1.0))
     for b in case.tbcs:
         d.add_temperature_bc(b["zone"], b["T"])
+    if case.meshes:
+        d.set_param("solid_count", n)
+        first = n
+        for m in case.meshes:
+            first += d.add_plane_mesh(first, m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"],
+                                      m.get("hfac", 1.1))
+        d.set_param("friction_sta", case.contact[0])
+        d.set_param("friction_dyn", case.contact[1])
+        d.set_param("DFAC", case.contact[2])
     return d
 
 

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTS)
-    assert L.wf_abi_version() == 2
+    assert L.wf_abi_version() == 3
 
 
 def test_struct_layouts_match_ctypes():
@@ -151,9 +151,17 @@ def test_cpp_facade_generators_and_errors(tmp_path):
     c = out["cyl"][0].split()
     assert int(c[0]) == 4524 and int(c[2]) == 156
     assert float(c[1]) == np.pi * 7.5e-3 * 7.5e-3 * 30e-3 * 2700.0 / 4524
+    # a 6x6 plane of 72 triangles: one particle each, h = (float)1.1 * radius,
+    # radius = side sqrt(5)/3, normal -z for flipNormals = false
+    mz = out["mesh"][0].split()
+    assert int(mz[0]) == int(mz[1]) == 72
+    side = 1e-3
+    assert float(mz[3]) == pytest.approx(side * 5 ** 0.5 / 3, rel=1e-14)
+    assert float(mz[2]) == float(np.float32(1.1)) * float(mz[3])
+    assert float(mz[4]) == -1.0 and int(mz[5]) == 10 and int(mz[6]) == 64
     fatal = {s.split(" ", 1)[0]: s.split(" ", 1)[1] for s in out["fatal"]}
     expect = dict(auto_ts="auto_ts = false", kernel="cubic spline", gradient="GradientType 0",
-                  callback="host callbacks", tensile="TI > 0", contact="contact", fail="Fail = 1",
+                  callback="host callbacks", tensile="TI > 0", contact="no rigid mesh", fail="Fail = 1",
                   mixed_params="different parameters", hollomon_mat="Hollomon material",
                   hollomon_def="Hollomon material bad definition")
     for k, msg in expect.items():

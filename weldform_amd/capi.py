@@ -27,8 +27,9 @@ FIELDS = {
     "sigmay": (15, 1), "eff_sr": (16, 1), "pressure": (17, 1), "T": (18, 1),
     "dTdt": (19, 1), "ddensity": (20, 1), "q_plheat": (21, 1),
     "strain_pl_incr": (22, 9), "q_conv": (23, 1), "h_conv": (24, 1),
-    "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1),
+    "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1), "contforce": (28, 3),
 }
+RIGID_FIELDS = {"x": 0, "v": 1, "normal": 2}
 INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4}
 EVAL = {"accel": 0, "density": 1, "rates": 2, "stress": 3}
 KERNELS = {"accel": 0, "density": 1, "rates": 2, "neighbours": 3, "step": 4}
@@ -64,7 +65,8 @@ class Scalars(ctypes.Structure):
                 ("npairs", ctypes.c_int64), ("max_nb_count", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)] + [
         (n, ctypes.c_double) for n in
-        ["kin_energy_sum", "int_energy_sum", "plastic_work", "max_pl_strain", "max_T", "min_T"]]
+        ["kin_energy_sum", "int_energy_sum", "plastic_work", "max_pl_strain", "max_T", "min_T",
+         "contact_force_sum", "contact_reaction_sum", "ext_forces_work"]] + [("contact_pairs", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -77,7 +79,8 @@ EXPORTS = [
     "wf_add_temperature_bc", "wf_init", "wf_step", "wf_sync", "wf_get_scalars",
     "wf_get_reaction_history", "wf_rebuild_neighbours", "wf_get_neighbour_pairs",
     "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats",
-    "wf_gen_cylinder",
+    "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
+    "wf_get_rigid_field",
 ]
 
 _lib = None
@@ -121,6 +124,10 @@ def lib():
         "wf_get_kernel_time": ([vp, i32, dp, I64P], ctypes.c_int),
         "wf_get_neighbour_stats": ([vp, I64P, ip], ctypes.c_int),
         "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
+        "wf_add_rigid_plane": ([vp, i32, i32, i32, dp, dp, i32, d], ctypes.c_int),
+        "wf_set_contact": ([vp, d, d, d], ctypes.c_int),
+        "wf_get_contact_force_history": ([vp, i32, dp, i64, I64P], ctypes.c_int),
+        "wf_get_rigid_field": ([vp, i32, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -250,6 +257,29 @@ class Solver:
         cnt = ctypes.c_int64(0)
         self._check(lib().wf_get_reaction_history(self.ctx, zone, _dp(out), cap, ctypes.byref(cnt)))
         return out[:3 * cnt.value].reshape(-1, 3)
+
+    def add_rigid_plane(self, zone, axis, flip, p1, p2, dens, hfac=1.1):
+        a = np.ascontiguousarray(p1, dtype=np.float64)
+        b = np.ascontiguousarray(p2, dtype=np.float64)
+        rc = lib().wf_add_rigid_plane(self.ctx, int(zone), int(axis), int(bool(flip)), _dp(a), _dp(b), int(dens),
+                                      float(hfac))
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def set_contact(self, friction_sta, friction_dyn, damping=0.0):
+        self._check(lib().wf_set_contact(self.ctx, friction_sta, friction_dyn, damping))
+
+    def contact_force_history(self, mesh, cap=1 << 16):
+        out = np.zeros(3 * cap)
+        cnt = ctypes.c_int64(0)
+        self._check(lib().wf_get_contact_force_history(self.ctx, mesh, _dp(out), cap, ctypes.byref(cnt)))
+        return out[:3 * cnt.value].reshape(-1, 3)
+
+    def rigid(self, name, nrig):
+        out = np.zeros((nrig, 3))
+        self._check(lib().wf_get_rigid_field(self.ctx, RIGID_FIELDS[name], _dp(out)))
+        return out
 
     def rebuild_neighbours(self):
         self._check(lib().wf_rebuild_neighbours(self.ctx))

@@ -38,6 +38,8 @@ class Case:
     thbc: np.ndarray = None     # Thermal_BC per particle (1 = convection)
     hconv: np.ndarray = None
     tinf: np.ndarray = None
+    meshes: list = field(default_factory=list)  # dicts: zone, axis, flip, p1, p2, dens, hfac
+    contact: tuple = (0.0, 0.0, 0.0)            # friction_sta, friction_dyn, DFAC
     meta: dict = field(default_factory=dict)
 
     @property
@@ -154,6 +156,34 @@ def thermal_small(steps=150, v_top=-20.0, seed=11):
     return c
 
 
+def contact_small(steps=200, v_punch=-5.0, mu=0.2, dens=22, seed=13, thermal=False):
+    """Config-3 physics at config-1 size: the steel-like JC billet of
+    thermal_small (R 7.5 mm, L 30 mm, dx 1 mm, jittered), compressed between
+    two rigid z-planes of 2 dens^2 triangles: the top one driven at v_punch,
+    the bottom one fixed (flipNormals set so its normal faces the billet
+    after the first mesh update, as the reference's TriMesh::Update does);
+    Coulomb friction mu (static = dynamic), no damping."""
+    c = cylinder_case("cyl_upset_4k_contact", 7.5e-3, 30e-3, 1.0e-3, 7850.0, 200e9, 0.29, STEEL_JC,
+                      steps=steps, jitter=1e-3, seed=seed, v_top=0.0, T0=300.0)
+    R, h = c.meta["R"], c.meta["h"]
+    zlo, zhi = c.x[:, 2].min(), c.x[:, 2].max()
+    W = R + 2.5e-3
+    c.ids[:] = 0
+    c.bcs = [dict(zone=10, value=(0.0, 0.0, v_punch))]
+    c.meshes = [dict(zone=10, axis=2, flip=False, p1=(-W, -W, zhi + 1.02 * h), p2=(W, W, zhi + 1.02 * h), dens=dens,
+                     hfac=1.1),
+                dict(zone=11, axis=2, flip=True, p1=(-W, -W, zlo - 1.02 * h), p2=(W, W, zlo - 1.02 * h), dens=dens,
+                     hfac=1.1)]
+    c.contact = (mu, mu, 0.0)
+    c.top_zone = 10
+    c.dom_max = (-1e11, -1e11, -1e11)
+    c.dom_min = (1e11, 1e11, 1e11)
+    if thermal:
+        c.thermal = True
+        c.pl_heating = True
+    return c
+
+
 def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,
@@ -180,5 +210,9 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
                           b.get("amp_factor", 1.0))
     for b in case.tbcs:
         s.add_temperature_bc(b["zone"], b["T"])
+    for m in case.meshes:
+        s.add_rigid_plane(m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"], m.get("hfac", 1.1))
+    if case.meshes:
+        s.set_contact(*case.contact)
     s.init(case.dt)
     return s

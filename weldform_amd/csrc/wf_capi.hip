@@ -56,8 +56,19 @@ struct AmpHost {
 
 enum { T_ACCEL = 0, T_DENS, T_RATES, T_NBR, T_STEP, T_COUNT };
 
+// rigid plane mesh (TriMesh::AxisPlaneMesh + CalcSpheres, Source/Mesh.cpp:82-220)
+struct MeshHost {
+  int zone = 0;
+  std::vector<double> node;        // 3 per node
+  std::vector<int> el;             // 3 per element
+  std::vector<int> nfar;
+  std::vector<double> centroid, normal, pplane, radius, h;
+};
+
 struct Ctx {
   wf_config cfg{};
+  std::vector<MeshHost> meshes;
+  double fr_sta = 0.0, fr_dyn = 0.0, dfac = 0.0;
   int64_t n = 0;
   long long npad = 0;
   hipStream_t st = nullptr;
@@ -226,6 +237,16 @@ void cell_initiate(Ctx *c) {
   }
   // mixed-precision shadows store positions relative to the body's centre
   for (int k = 0; k < 3; ++k) c->d.xc[k] = 0.5 * (TRPR[k] + BLPF[k]);
+  // rigid-plane particles follow the solid ones in the reference's particle
+  // list and count for the bounding box and h_max
+  for (const MeshHost &M : c->meshes)
+    for (size_t e = 0; e < M.h.size(); ++e) {
+      for (int k = 0; k < 3; ++k) {
+        if (M.centroid[3 * e + k] > TRPR[k]) TRPR[k] = M.centroid[3 * e + k];
+        if (M.centroid[3 * e + k] < BLPF[k]) BLPF[k] = M.centroid[3 * e + k];
+      }
+      if (M.h[e] > hmax) hmax = M.h[e];
+    }
   for (int k = 0; k < 3; ++k) {
     if (c->cfg.dom_max[k] > TRPR[k]) TRPR[k] = c->cfg.dom_max[k];
     if (c->cfg.dom_min[k] < BLPF[k]) BLPF[k] = c->cfg.dom_min[k];
@@ -375,6 +396,38 @@ void allocate(Ctx *c) {
     d.srecf = c->alloc<StressRecF>(np);
     d.prf = c->alloc<float>(np);
   }
+  if (!c->meshes.empty()) {
+    d.contact = 1;
+    d.nmesh = (int)c->meshes.size();
+    int nn = 0, ne = 0;
+    for (int m = 0; m < d.nmesh; ++m) {
+      const MeshHost &M = c->meshes[m];
+      d.mesh_zone[m] = M.zone;
+      d.mesh_node0[m] = nn;
+      d.mesh_nnode[m] = (int)M.node.size() / 3;
+      d.mesh_elem0[m] = ne;
+      d.mesh_nelem[m] = (int)M.h.size();
+      nn += d.mesh_nnode[m];
+      ne += d.mesh_nelem[m];
+    }
+    d.nnode = nn;
+    d.nrig = ne;
+    d.node = c->alloc<double4>(nn);
+    d.elem = c->alloc<int4>(ne);
+    d.enorm = c->alloc<double4>(ne);
+    d.rx = c->alloc<double4>(ne);
+    d.rv = c->alloc<double4>(ne);
+    d.rmesh = c->alloc<int>(ne);
+    d.mv = c->alloc<double4>(kMaxMesh);
+    d.id_orig = c->alloc<int>(np);
+    d.ccnt = c->alloc<int>(np);
+    d.cidx = c->alloc<int>((size_t)kMaxContact * np);
+    d.cforce = c->alloc<double4>(np);
+    d.cf_hist = c->alloc<double>((size_t)d.hist_cap * kMaxMesh * 3);
+    d.fr_sta = c->fr_sta;
+    d.fr_dyn = c->fr_dyn;
+    d.dfac = c->dfac;
+  }
   d.ntiles = (int)(np / kBlock);
   d.stage_max = 0;
   d.lanes_mx = c->cfg.lanes ? c->cfg.lanes : kLanesMixed;
@@ -436,6 +489,40 @@ void upload_all(Ctx *c) {
   dput(c, d.vol[1], vol);
   dput(c, d.pres, pres);
   dput(c, d.srec, rec);
+  if (d.contact) {
+    std::vector<double4> node(d.nnode), en(d.nrig), rx(d.nrig), z(d.nrig);
+    std::vector<int4> el(d.nrig);
+    std::vector<int> rm(d.nrig);
+    for (int m = 0; m < d.nmesh; ++m) {
+      const MeshHost &M = c->meshes[m];
+      const int n0 = d.mesh_node0[m], e0 = d.mesh_elem0[m];
+      for (int q = 0; q < d.mesh_nnode[m]; ++q)
+        node[n0 + q] = make_double4(M.node[3 * q], M.node[3 * q + 1], M.node[3 * q + 2], 0.0);
+      for (int e = 0; e < d.mesh_nelem[m]; ++e) {
+        el[e0 + e] = make_int4(n0 + M.el[3 * e], n0 + M.el[3 * e + 1], n0 + M.el[3 * e + 2], M.nfar[e]);
+        en[e0 + e] = make_double4(M.normal[3 * e], M.normal[3 * e + 1], M.normal[3 * e + 2], M.pplane[e]);
+        rx[e0 + e] = make_double4(M.centroid[3 * e], M.centroid[3 * e + 1], M.centroid[3 * e + 2], M.h[e]);
+        z[e0 + e] = make_double4(0, 0, 0, 0);
+        rm[e0 + e] = m;
+      }
+    }
+    dput(c, d.node, node);
+    dput(c, d.elem, el);
+    dput(c, d.enorm, en);
+    dput(c, d.rx, rx);
+    dput(c, d.rv, z);
+    dput(c, d.rmesh, rm);
+    // ID_orig = ID before the first free-surface search
+    // (Source/SolverLeapfrog.cpp:80-83); the mean mass of CalculateSurface
+    // runs over every particle in the reference's order, rigid ones with 0
+    std::vector<int> io(np, 0);
+    for (int64_t k = 0; k < n; ++k) io[k] = c->hi[WF_I_ID][c->perm[k]];
+    dput(c, d.id_orig, io);
+    double tm = 0.0;
+    for (int64_t o = 0; o < n; ++o) tm += F[WF_F_MASS][o];
+    for (int r = 0; r < d.nrig; ++r) tm += 0.0;
+    d.totmass = tm / (double)(n + d.nrig);
+  }
   // tensors
   if (field(WF_F_SHEAR)) put_sym(c, d.S, field(WF_F_SHEAR));
   if (field(WF_F_STRAIN)) put_sym(c, d.strain, field(WF_F_STRAIN));
@@ -579,17 +666,23 @@ void enqueue_step(Ctx *c, int scheme, int cur) {
   {
     Timed t(c, T_NBR);
     launch_neighbours(d, c->st, cur);
+    if (d.contact) launch_surface(d, c->st, cur, 0);
   }
   {
     Timed t(c, T_ACCEL);
     launch_accel(d, c->st, cur);
   }
+  // contact acts after AccelReduction and the first GeneralAfter
+  // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
+  // so no BC zone applies to them
+  if (d.contact) launch_contact(d, c->st, c->bc, cur, c->dt);
   launch_kick(d, c->st, c->bc, scheme, c->dt, c->dt);
   {
     Timed t(c, T_DENS);
     launch_dens_drift(d, c->st, cur, c->dt, 1);
   }
   if (scheme == WF_KICKDRIFT) launch_kick2(d, c->st, c->dt);
+  if (d.contact) launch_rigid_drift(d, c->st, c->dt);
   const int nxt = cur ^ 1;
   {
     Timed t(c, T_RATES);
@@ -598,6 +691,7 @@ void enqueue_step(Ctx *c, int scheme, int cur) {
   launch_stress(d, c->st, nxt, c->dt);
   if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, c->dt);
   launch_finalize(d, c->st, c->dt, c->bc.nz);
+  if (d.contact) launch_mesh_update(d, c->st, c->dt);
 }
 
 void one_step(Ctx *c, int scheme) {
@@ -861,6 +955,12 @@ int wf_get_field(void *ctx, int32_t field, double *h) {
       case WF_F_T_INF: get_scalar(c, d.tinf, h); break;
       case WF_F_Q_SOURCE: get_scalar(c, d.qsrc, h); break;
       case WF_F_PS_ENERGY: get_scalar(c, d.psen, h); break;
+      case WF_F_CONTFORCE:
+        if (d.contact)
+          get_d4(c, d.cforce, h, 0, 3, 3);
+        else
+          std::memset(h, 0, sizeof(double) * 3 * (size_t)c->n);
+        break;
       default: throw Error(WF_EINVAL, "bad field");
     }
     return WF_OK;
@@ -886,9 +986,10 @@ int wf_get_int_field(void *ctx, int32_t field, int32_t *h) {
   return guard(c, [&] {
     as_ctx(ctx);
     if (field < 0 || field >= WF_NIFIELDS || !h) throw Error(WF_EINVAL, "bad int field");
-    if (field == WF_I_NB) {
+    if (field == WF_I_NB || (field == WF_I_ID && c->inited && c->d.contact)) {
+      // neighbour counts; zone ids once free-surface particles took ID 1
       if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
-      auto t = dget(c, c->d.nnb, (size_t)c->npad);
+      auto t = dget(c, field == WF_I_NB ? c->d.nnb : c->d.id, (size_t)c->npad);
       for (int64_t k = 0; k < c->n; ++k) h[c->perm[k]] = t[k];
       return WF_OK;
     }
@@ -980,6 +1081,13 @@ int wf_init(void *ctx, double dt) {
     allocate(c);
     upload_all(c);
     launch_shadows(c->d, c->st, c->cur);
+    if (c->d.contact) {
+      // free surface before the loop (Source/SolverLeapfrog.cpp:84-100)
+      refresh_bc(c);
+      launch_schedule(c->d, c->st, 1);
+      launch_neighbours(c->d, c->st, 0);
+      launch_surface(c->d, c->st, 0, 1);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     c->dt = dt;
@@ -1037,6 +1145,15 @@ int wf_get_scalars(void *ctx, wf_scalars *out) {
     out->max_pl_strain = s[S_MAXPL];
     out->max_T = s[S_MAXT];
     out->min_T = s[S_MINT];
+    if (c->d.contact) {
+      out->contact_force_sum = s[S_CFSUM];
+      out->contact_reaction_sum = s[S_CREACT];
+      out->ext_forces_work = s[S_EXTW];
+      auto cc = dget(c, c->d.ccnt, (size_t)c->npad);
+      int64_t tot = 0;
+      for (int64_t k = 0; k < c->n; ++k) tot += cc[k];
+      out->contact_pairs = tot;
+    }
     return WF_OK;
   });
 }
@@ -1059,6 +1176,167 @@ int wf_get_reaction_history(void *ctx, int32_t zone, double *out3, int64_t cap, 
       for (int k = 0; k < 3; ++k) out3[3 * s + k] = h[(slot * kMaxZones + z) * 3 + k];
     }
     if (count) *count = m;
+    return WF_OK;
+  });
+}
+
+// TriMesh::AxisPlaneMesh (3D), CalcNormals, CalcSpheres and
+// Domain::AddTrimeshParticles (Source/Mesh.cpp:82-243, Source/Contact.cpp:10-37)
+static void plane_normals(MeshHost &M) {
+#pragma clang fp contract(off)
+  const size_t ne = M.el.size() / 3;
+  for (size_t e = 0; e < ne; ++e) {
+    const double *n0 = &M.node[3 * M.el[3 * e]], *n1 = &M.node[3 * M.el[3 * e + 1]], *n2 = &M.node[3 * M.el[3 * e + 2]];
+    double u[3], v[3];
+    for (int k = 0; k < 3; ++k) {
+      u[k] = n1[k] - n0[k];
+      v[k] = n2[k] - n0[k];
+    }
+    const double w[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const double nw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int k = 0; k < 3; ++k) M.normal[3 * e + k] = w[k] / nw;
+  }
+}
+
+static MeshHost make_plane(int zone, int axis, int flip, const double *p1, const double *p2, int dens, double hfac) {
+#pragma clang fp contract(off)
+  MeshHost M;
+  M.zone = zone;
+  const int ne = 2 * dens * dens;
+  double p[3];
+  for (int k = 0; k < 3; ++k) p[k] = p2[k] - p1[k];
+  int dir[3];
+  if (axis == 0) {
+    dir[0] = 1;
+    dir[1] = 2;
+  } else if (axis == 1) {
+    dir[0] = 0;
+    dir[1] = 2;
+  } else {
+    dir[0] = 0;
+    dir[1] = 1;
+  }
+  dir[2] = axis;
+  const double x3 = p1[dir[2]];
+  double x2 = p1[dir[1]];
+  const double dl = p[dir[0]] / dens;
+  for (int j = 0; j < dens + 1; ++j) {
+    double x1 = p1[dir[0]];
+    for (int i = 0; i < dens + 1; ++i) {
+      double v[3];
+      v[dir[0]] = x1;
+      v[dir[1]] = x2;
+      v[dir[2]] = x3;
+      for (int k = 0; k < 3; ++k) M.node.push_back(v[k]);
+      x1 += dl;
+    }
+    x2 += dl;
+  }
+  M.centroid.resize(3 * (size_t)ne);
+  for (int j = 0; j < dens; ++j)
+    for (int i = 0; i < dens; ++i) {
+      const int n0 = (dens + 1) * j + i, n1 = n0 + 1, n2 = (dens + 1) * (j + 1) + i, n3 = n2 + 1;
+      const int con[2][3] = {{n0, flip ? n1 : n2, flip ? n2 : n1}, {n1, flip ? n3 : n2, flip ? n2 : n3}};
+      for (int e = 0; e < 2; ++e) {
+        const size_t el = M.el.size() / 3;
+        for (int k = 0; k < 3; ++k) M.el.push_back(con[e][k]);
+        const double *a = &M.node[3 * con[e][0]], *b = &M.node[3 * con[e][1]], *c = &M.node[3 * con[e][2]];
+        for (int k = 0; k < 3; ++k) M.centroid[3 * el + k] = (a[k] + b[k] + c[k]) / 3.;
+      }
+    }
+  M.normal.resize(3 * (size_t)ne);
+  plane_normals(M);
+  if (flip)
+    for (double &v : M.normal) v = -v;
+  M.radius.resize(ne);
+  M.nfar.resize(ne);
+  M.pplane.resize(ne);
+  M.h.resize(ne);
+  const double hf = (double)(float)hfac;  // AddTrimeshParticles takes hfac as float
+  for (int e = 0; e < ne; ++e) {
+    double mx = 0.;
+    for (int q = 0; q < 3; ++q) {
+      double rv[3];
+      for (int k = 0; k < 3; ++k) rv[k] = M.node[3 * M.el[3 * e + q] + k] - M.centroid[3 * e + k];
+      const double nr = std::sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+      if (nr > mx) mx = nr;
+      M.nfar[e] = q;  // CalcSpheres leaves the last node (Source/Mesh.cpp:212)
+    }
+    M.radius[e] = mx;
+    const double *nf = &M.node[3 * M.el[3 * e + M.nfar[e]]];
+    M.pplane[e] = nf[0] * M.normal[3 * e] + nf[1] * M.normal[3 * e + 1] + nf[2] * M.normal[3 * e + 2];
+    M.h[e] = hf * M.radius[e];
+  }
+  return M;
+}
+
+int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, const double p1[3], const double p2[3],
+                       int32_t dens, double hfac) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "rigid planes must be added before wf_init");
+    if (!p1 || !p2) throw Error(WF_EINVAL, "null argument");
+    if (axis < 0 || axis > 2) throw Error(WF_EINVAL, "axis must be 0, 1 or 2");
+    if (dens < 1 || dens > 1024) throw Error(WF_EINVAL, "dens must be in 1..1024");
+    if (!(hfac > 0)) throw Error(WF_EINVAL, "hfac must be positive");
+    if ((int)c->meshes.size() >= kMaxMesh) throw Error(WF_EUNSUPPORTED, "at most 4 rigid planes");
+    if (zone == 1) throw Error(WF_EINVAL, "zone 1 is the free-surface id");
+    const int d0 = axis == 0 ? 1 : 0;
+    if (!(std::fabs(p2[d0] - p1[d0]) > 0)) throw Error(WF_EINVAL, "plane has no extent");
+    c->meshes.push_back(make_plane(zone, axis, flip, p1, p2, dens, hfac));
+    return (int)c->meshes.back().h.size();
+  });
+}
+
+int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double damping) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "contact parameters must be set before wf_init");
+    if (!(friction_sta >= 0) || !(friction_dyn >= 0) || !(damping >= 0)) throw Error(WF_EINVAL, "negative contact parameter");
+    c->fr_sta = friction_sta;
+    c->fr_dyn = friction_dyn;
+    c->dfac = damping;
+    return WF_OK;
+  });
+}
+
+int wf_get_contact_force_history(void *ctx, int32_t mesh, double *out3, int64_t cap, int64_t *count) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (mesh < 0 || mesh >= c->d.nmesh) throw Error(WF_EINVAL, "no such rigid plane");
+    const long long hc = c->d.hist_cap;
+    const long long m = std::min<long long>(std::min<long long>(c->steps, hc), cap);
+    auto h = dget(c, c->d.cf_hist, (size_t)hc * kMaxMesh * 3);
+    const long long first = c->steps - m;
+    for (long long s = 0; s < m; ++s) {
+      const long long slot = (first + s) % hc;
+      for (int k = 0; k < 3; ++k) out3[3 * s + k] = h[(slot * kMaxMesh + mesh) * 3 + k];
+    }
+    if (count) *count = m;
+    return WF_OK;
+  });
+}
+
+int wf_get_rigid_field(void *ctx, int32_t field, double *h) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (!h) throw Error(WF_EINVAL, "null argument");
+    const Dev &d = c->d;
+    const double4 *src = field == WF_R_X ? d.rx : field == WF_R_V ? d.rv : field == WF_R_NORMAL ? d.enorm : nullptr;
+    if (!src) throw Error(WF_EINVAL, "bad rigid field");
+    if (d.nrig == 0) return WF_OK;
+    auto t = dget(c, src, (size_t)d.nrig);
+    for (int r = 0; r < d.nrig; ++r) {
+      h[3 * r] = t[r].x;
+      h[3 * r + 1] = t[r].y;
+      h[3 * r + 2] = t[r].z;
+    }
     return WF_OK;
   });
 }

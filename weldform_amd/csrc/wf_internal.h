@@ -48,6 +48,8 @@ __host__ __device__ inline int nb_rows(int max_nb) {
 }
 constexpr int kMaxBc = 8;        // velocity / temperature BC slots per context
 constexpr int kMaxZones = 8;     // velocity-BC zones with a reaction history
+constexpr int kMaxMesh = 4;      // rigid contact planes
+constexpr int kMaxContact = 32;  // contact partners per particle
 constexpr int kMaxMat = 256;
 constexpr int kMaxNb = 128;
 
@@ -103,10 +105,14 @@ enum {
   P_PLW,             // sum vol * q_plheat   (plastic work rate)
   P_MAXT, P_MINT,    // max / min temperature
   P_REACT,           // + 3*zone + comp
-  P_COUNT = P_REACT + 3 * kMaxZones
+  P_CFORCE = P_REACT + 3 * kMaxZones,   // + 3*mesh + comp (contact force per plane)
+  P_CFSUM = P_CFORCE + 3 * kMaxMesh,    // sum |contforce|
+  P_CREACT,                             // sum m a.n of contact particles
+  P_EXTW,                               // sum contforce . v_rigid
+  P_COUNT
 };
 
-enum { S_KIN = 0, S_INT, S_PLW, S_MAXT, S_MINT, S_MAXPL, S_COUNT };
+enum { S_KIN = 0, S_INT, S_PLW, S_MAXT, S_MINT, S_MAXPL, S_CFSUM, S_CREACT, S_EXTW, S_COUNT };
 
 constexpr int kMaxAmp = 8;
 constexpr int kMaxAmpPts = 16;
@@ -177,6 +183,20 @@ struct Dev {
   int lanes_mx, lanes_f64;     // threads per particle in the pair passes
   int *stage, *stage_cnt;
   uint16_t *nbrl, *selfl;
+  // rigid contact planes (all meshes concatenated): nodes, elements
+  // (n0, n1, n2, nfar as global node indices), element normal + pplane,
+  // one rigid particle per element (x, h) and its velocity; per plane the
+  // node / element ranges and its velocity m_v
+  int contact, nmesh, nrig, nnode;
+  int mesh_zone[kMaxMesh], mesh_node0[kMaxMesh], mesh_nnode[kMaxMesh], mesh_elem0[kMaxMesh], mesh_nelem[kMaxMesh];
+  double4 *node, *enorm, *rx, *rv, *mv;
+  int4 *elem;
+  int *rmesh;
+  double totmass;              // mean particle mass over solid and rigid particles
+  double fr_sta, fr_dyn, dfac;
+  int *id_orig, *ccnt, *cidx;  // contact partners [kMaxContact][npad]
+  double4 *cforce;
+  double *cf_hist;             // [hist_cap][kMaxMesh][3]
   // materials, flags, reductions
   const DevMat *mats;
   const DevAmp *amps;
@@ -200,5 +220,9 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt);
 void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz);
 void launch_max_pl(const Dev &d, hipStream_t st);
 void launch_shadows(const Dev &d, hipStream_t st, int cur);
+void launch_surface(const Dev &d, hipStream_t st, int cur, int setup);
+void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt);
+void launch_rigid_drift(const Dev &d, hipStream_t st, double dt);
+void launch_mesh_update(const Dev &d, hipStream_t st, double dt);
 
 }  // namespace wf

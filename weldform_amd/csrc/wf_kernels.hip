@@ -1312,7 +1312,9 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
 #pragma unroll
   for (int s = 0; s < P_COUNT; ++s) {
     active[s] = s == P_INT || s == P_KIN || (s == P_PLW && d.pl_heating) ||
-                ((s == P_MAXT || s == P_MINT) && d.thermal) || (s >= P_REACT && s < P_REACT + 3 * nz);
+                ((s == P_MAXT || s == P_MINT) && d.thermal) || (s >= P_REACT && s < P_REACT + 3 * nz) ||
+                (d.contact && ((s >= P_CFORCE && s < P_CFORCE + 3 * d.nmesh) || s == P_CFSUM || s == P_CREACT ||
+                               s == P_EXTW));
     op[s] = (s == P_MAXT) ? 1 : (s == P_MINT ? 2 : 0);
     v[s] = op[s] == 0 ? 0.0 : (op[s] == 1 ? -INFINITY : INFINITY);
   }
@@ -1335,7 +1337,7 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
   if (threadIdx.x != 0) return;
   const long long step = d.flags->step;
   const long long slot = step % d.hist_cap;
-  double e_kin = 0.0, e_int = 0.0, plw = 0.0, tmax = 0.0, tmin = 0.0;
+  double e_kin = 0.0, e_int = 0.0, plw = 0.0, tmax = 0.0, tmin = 0.0, cfsum = 0.0, creact = 0.0, extw = 0.0;
 #pragma unroll
   for (int s = 0; s < P_COUNT; ++s) {
     if (!active[s]) continue;
@@ -1346,7 +1348,16 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
     if (s == P_PLW) plw = r;
     if (s == P_MAXT) tmax = r;
     if (s == P_MINT) tmin = r;
-    if (s >= P_REACT) d.react_hist[slot * kMaxZones * 3 + (s - P_REACT)] = r;
+    if (s >= P_REACT && s < P_CFORCE) d.react_hist[slot * kMaxZones * 3 + (s - P_REACT)] = r;
+    if (s >= P_CFORCE && s < P_CFSUM) d.cf_hist[slot * kMaxMesh * 3 + (s - P_CFORCE)] = r;
+    if (s == P_CFSUM) cfsum = r;
+    if (s == P_CREACT) creact = r;
+    if (s == P_EXTW) extw = r;
+  }
+  if (d.contact) {
+    d.scalars[S_CFSUM] = cfsum;
+    d.scalars[S_CREACT] = creact;
+    d.scalars[S_EXTW] += extw * dt;
   }
   d.scalars[S_KIN] += e_kin * dt;
   d.scalars[S_INT] += e_int * dt;
@@ -1369,6 +1380,251 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
 void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
   k_finalize<<<1, 1024, 0, st>>>(d, dt, nz);
 }
+
+// ---------------------------------------------------------------------------
+// rigid contact planes: free-surface detection (Domain::CalculateSurface,
+// Source/Domain.cpp:1415-1491), contact pairs (AllocateNbPair's contact
+// branch + ContactNbSearch, Source/Neighbour.cpp:375-396,
+// Source/Contact.cpp:57-132), the Wang penalty force with Coulomb friction
+// (Source/Contact_Wang.cpp:15-348) and the rigid-body motion of the planes
+// (TriMesh::Update, Source/Mesh.cpp:273-296; UpdateContactParticles,
+// Source/Contact.cpp:41-55).  FP64 in both precision modes.
+
+__device__ inline void cross_d(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// After a neighbour build.  A particle was on the free surface (ID 1) when
+// the search ran; the new surface test then uses the new rows.  Contact
+// partners are the rigid particles, in index order, of an adjacent cell
+// closer than h_i + h_r, for particles on the surface before and after.
+// setup = 1: the search before the first step only sets the surface.
+WF_KERNEL void k_surface(Dev d, int cur, int setup) {
+  if (!d.flags->do_search) return;
+  const long long np = d.npad;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+    const int old_surf = d.id[i] == 1;
+    const double4 xi = d.xh[cur][i];
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    const int cnt = d.nnb[i];
+    for (int k = 0; k < cnt; ++k) {
+      const int j = d.nbr[nb_slot(np, k, i)];
+      const double4 xj = d.xh[cur][j];
+      const double mj = d.vm[j].w;
+      nx += mj * (xi.x - xj.x);
+      ny += mj * (xi.y - xj.y);
+      nz += mj * (xi.z - xj.z);
+    }
+    const double f = 1. / d.totmass;
+    nx *= f;
+    ny *= f;
+    nz *= f;
+    const int surf = sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * xi.w && cnt <= 46;
+    d.id[i] = surf ? 1 : d.id_orig[i];
+    int c = 0;
+    if (!setup && old_surf && surf) {
+      const int cx = cell_coord(xi.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
+      const int cy = cell_coord(xi.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
+      const int cz = cell_coord(xi.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
+      for (int r = 0; r < d.nrig; ++r) {
+        const double4 xr = d.rx[r];
+        const int rx = cell_coord(xr.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
+        const int ry = cell_coord(xr.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
+        const int rz = cell_coord(xr.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
+        if (abs(rx - cx) > 1 || abs(ry - cy) > 1 || abs(rz - cz) > 1) continue;
+        const double dx = xi.x - xr.x, dy = xi.y - xr.y, dz = xi.z - xr.z;
+        if (sqrt(dx * dx + dy * dy + dz * dz) < xi.w + xr.w) {
+          if (c < kMaxContact) d.cidx[(long long)c * np + i] = r;
+          ++c;
+        }
+      }
+      if (c > kMaxContact) d.flags->overflow = 1;
+    }
+    d.ccnt[i] = c < kMaxContact ? c : kMaxContact;
+  }
+}
+
+// velocity of each plane and of its particles at the step's time
+// (UserAcc's BC loop and the contact-mesh branch, Source/WeldForm.cpp:67-119)
+WF_KERNEL void k_rigid_bc(Dev d, BcTable bc) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= d.nrig) return;
+  const int m = d.rmesh[r];
+  const int zone = d.mesh_zone[m];
+  bool set = false;
+  double v[3] = {0, 0, 0};
+  for (int b = 0; b < bc.n; ++b) {
+    if (bc.zone[b] != zone || bc.type[b] != 0) continue;
+    if (bc.value_type[b] == 1) {
+      if (bc.amp[b] < 0) continue;
+      const double f = bc.amp_factor[b] * amp_value(d.amps[bc.amp[b]], d.flags->time);
+      for (int k = 0; k < 3; ++k) v[k] = f * bc.v[b][k];
+    } else {
+      for (int k = 0; k < 3; ++k) v[k] = bc.v[b][k];
+    }
+    set = true;
+  }
+  if (!set) return;
+  d.rv[r] = make_double4(v[0], v[1], v[2], 0.0);
+  if (r == d.mesh_elem0[m]) d.mv[m] = make_double4(v[0], v[1], v[2], 0.0);
+}
+
+// Wang contact force of each particle over its partners, in order
+WF_KERNEL void k_contact(Dev d, int cur, double dt) {
+  __shared__ double sh[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double cfm[kMaxMesh][3];
+  for (int m = 0; m < kMaxMesh; ++m) cfm[m][0] = cfm[m][1] = cfm[m][2] = 0.0;
+  double cfsum = 0.0, creact = 0.0, extw = 0.0;
+  if (i < d.n) {
+    double cf[3] = {0, 0, 0};
+    const int cnt = d.ccnt[i];
+    if (cnt > 0) {
+      const double4 x4 = d.xh[cur][i];
+      const double4 v4 = d.vm[i];
+      double4 a4 = d.acc[i];
+      const double x[3] = {x4.x, x4.y, x4.z}, v[3] = {v4.x, v4.y, v4.z};
+      double a[3] = {a4.x, a4.y, a4.z};
+      const double h = x4.w, m1 = v4.w;
+      for (int q = 0; q < cnt; ++q) {
+        const int r = d.cidx[(long long)q * d.npad + i];
+        const int mi = d.rmesh[r];
+        const double4 en = d.enorm[r];
+        const double n2[3] = {en.x, en.y, en.z};
+        const double4 v24 = d.rv[r];
+        const double v2[3] = {v24.x, v24.y, v24.z};
+        const double delta_ = -(n2[0] * (v[0] - v2[0]) + n2[1] * (v[1] - v2[1]) + n2[2] * (v[2] - v2[2]));
+        double xp[3];
+        for (int k = 0; k < 3; ++k) xp[k] = x[k] + v[k] * dt + a[k] * dt * dt / 2.0;
+        const double dist = (n2[0] * xp[0] + n2[1] * xp[1] + n2[2] * xp[2]) - en.w;
+        if (!(dist < h)) continue;
+        double qj[3];
+        for (int k = 0; k < 3; ++k) qj[k] = x[k] - dist * n2[k];
+        const int4 el = d.elem[r];
+        const int nd[3] = {el.x, el.y, el.z};
+        bool inside = true;
+        for (int ii = 0; ii < 3 && inside; ++ii) {
+          const int jj = ii + 1 > 2 ? 0 : ii + 1;
+          const double4 ni = d.node[nd[ii]], nj = d.node[nd[jj]];
+          const double ea[3] = {nj.x - ni.x, nj.y - ni.y, nj.z - ni.z};
+          const double eb[3] = {qj[0] - ni.x, qj[1] - ni.y, qj[2] - ni.z};
+          double c[3];
+          cross_d(ea, eb, c);
+          if (c[0] * n2[0] + c[1] * n2[1] + c[2] * n2[2] < 0.0) inside = false;
+        }
+        if (!inside) continue;
+        const double delta = h - dist;
+        const double kij = 2.0 * m1 / (dt * dt);
+        const double omega = sqrt(kij / m1);
+        const double psi = 2. * m1 * omega * d.dfac;
+        for (int k = 0; k < 3; ++k) cf[k] = (kij * delta - psi * delta_) * n2[k];
+        for (int k = 0; k < 3; ++k) a[k] += cf[k] / m1;
+        if (d.fr_sta > 0.) {
+          double du[3], tg[3];
+          for (int k = 0; k < 3; ++k) du[k] = xp[k] - x[k] - v2[k] * dt;
+          const double dn = du[0] * n2[0] + du[1] * n2[1] + du[2] * n2[2];
+          for (int k = 0; k < 3; ++k) tg[k] = kij * (du[k] - dn * n2[k]);
+          const double ntg = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+          const double ncf = sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
+          if (ntg < d.fr_sta * ncf) {
+            for (int k = 0; k < 3; ++k) {
+              cf[k] -= tg[k] / m1;
+              a[k] -= tg[k] / m1;
+            }
+          } else {
+            for (int k = 0; k < 3; ++k) {
+              const double td = d.fr_dyn * ncf * tg[k] / ntg;
+              cf[k] -= td;
+              a[k] -= td / m1;
+            }
+          }
+        }
+        cfsum += sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
+        for (int m = 0; m < kMaxMesh; ++m)
+          if (m == mi)
+            for (int k = 0; k < 3; ++k) cfm[m][k] += cf[k];
+        creact += (a[0] * n2[0] + a[1] * n2[1] + a[2] * n2[2]) * m1;
+        extw += cf[0] * v2[0] + cf[1] * v2[1] + cf[2] * v2[2];
+      }
+      d.acc[i] = make_double4(a[0], a[1], a[2], a4.w);
+    }
+    d.cforce[i] = make_double4(cf[0], cf[1], cf[2], 0.0);
+  }
+  for (int m = 0; m < d.nmesh; ++m)
+    for (int k = 0; k < 3; ++k) {
+      const double r = block_reduce<0>(cfm[m][k], sh);
+      if (threadIdx.x == 0) part(d, P_CFORCE + 3 * m + k, blockIdx.x) = r;
+    }
+  const double r1 = block_reduce<0>(cfsum, sh);
+  const double r2 = block_reduce<0>(creact, sh);
+  const double r3 = block_reduce<0>(extw, sh);
+  if (threadIdx.x == 0) {
+    part(d, P_CFSUM, blockIdx.x) = r1;
+    part(d, P_CREACT, blockIdx.x) = r2;
+    part(d, P_EXTW, blockIdx.x) = r3;
+  }
+}
+
+// the drift of the rigid particles (the reference's loop covers them)
+WF_KERNEL void k_rigid_drift(Dev d, double dt) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= d.nrig) return;
+  double4 x = d.rx[r];
+  const double4 v = d.rv[r];
+  x.x += v.x * dt;
+  x.y += v.y * dt;
+  x.z += v.z * dt;
+  d.rx[r] = x;
+}
+
+// one block per plane: nodes move with m_v, element normals and plane
+// coefficients are recomputed from the winding, particles take the mean
+// node velocity
+__global__ __launch_bounds__(1024) void k_mesh_update(Dev d, double dt) {
+  const int m = blockIdx.x;
+  const double4 mv = d.mv[m];
+  const int n0 = d.mesh_node0[m], nn = d.mesh_nnode[m];
+  for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+    double4 p = d.node[n0 + k];
+    p.x += (mv.x + 0.0) * dt;
+    p.y += (mv.y + 0.0) * dt;
+    p.z += (mv.z + 0.0) * dt;
+    d.node[n0 + k] = p;
+  }
+  __syncthreads();
+  const int e0 = d.mesh_elem0[m], ne = d.mesh_nelem[m];
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int4 el = d.elem[e0 + e];
+    const double4 a = d.node[el.x], b = d.node[el.y], c = d.node[el.z];
+    const double u[3] = {b.x - a.x, b.y - a.y, b.z - a.z}, v[3] = {c.x - a.x, c.y - a.y, c.z - a.z};
+    double w[3];
+    cross_d(u, v, w);
+    const double nw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    const double n[3] = {w[0] / nw, w[1] / nw, w[2] / nw};
+    const int fa = el.w == 0 ? el.x : (el.w == 1 ? el.y : el.z);
+    const double4 pf = d.node[fa];
+    const double pp = pf.x * n[0] + pf.y * n[1] + pf.z * n[2];
+    d.enorm[e0 + e] = make_double4(n[0], n[1], n[2], pp);
+    const double vm = ((0.0 + (mv.x + 0.0)) + (mv.x + 0.0)) + (mv.x + 0.0);
+    const double vn = ((0.0 + (mv.y + 0.0)) + (mv.y + 0.0)) + (mv.y + 0.0);
+    const double vo = ((0.0 + (mv.z + 0.0)) + (mv.z + 0.0)) + (mv.z + 0.0);
+    d.rv[e0 + e] = make_double4(vm / 3, vn / 3, vo / 3, 0.0);
+  }
+}
+
+void launch_surface(const Dev &d, hipStream_t st, int cur, int setup) {
+  k_surface<<<nblk_small(d.n), kBlock, 0, st>>>(d, cur, setup);
+}
+void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt) {
+  k_rigid_bc<<<(d.nrig + kBlock - 1) / kBlock, kBlock, 0, st>>>(d, bc);
+  k_contact<<<d.nblocks, kBlock, 0, st>>>(d, cur, dt);
+}
+void launch_rigid_drift(const Dev &d, hipStream_t st, double dt) {
+  k_rigid_drift<<<(d.nrig + kBlock - 1) / kBlock, kBlock, 0, st>>>(d, dt);
+}
+void launch_mesh_update(const Dev &d, hipStream_t st, double dt) { k_mesh_update<<<d.nmesh, 1024, 0, st>>>(d, dt); }
 
 // ---------------------------------------------------------------------------
 // utilities
