@@ -98,6 +98,37 @@ def timed_run(case, precision, steps, warmup):
     e.close()
     return s, wall, wall_eager, times, init_s
 
+def contact_run(steps, warmup=10):
+    """Thermo-mechanical contact compression (weldform_amd.cases.config3,
+    384,256 particles, two rigid planes of 2,048 triangles, conduction and
+    plastic heating), mixed precision, graph replay."""
+    from weldform_amd import capi, cases
+
+    case = cases.config3(steps=steps)
+    log(f"bench: {case.name} n={case.n} steps={steps}")
+    w = cases.make_solver(case, precision=capi.MIXED)
+    w.step(warmup)
+    w.sync()
+    w.close()
+    s = cases.make_solver(case, precision=capi.MIXED)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    sc = s.scalars()
+    F = s.contact_force_history(0)
+    res = {"workload": f"{case.name}: R=15mm L=30mm dx=0.3757mm JC steel, thermal + plastic heat, "
+                       f"top plane -5 m/s, bottom plane fixed, mu=0.2, {steps} Leapfrog steps from t=0, mixed",
+           "particles": case.n, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+           "particle_steps_per_s": case.n * steps / wall, "contact_pairs": sc["contact_pairs"],
+           "top_plane_force_z_last": float(F[-1, 2]) if len(F) else 0.0, "max_T": sc["max_T"],
+           "plastic_work": sc["plastic_work"], "max_pl_strain": sc["max_pl_strain"]}
+    log("bench: config3 " + json.dumps(res))
+    s.close()
+    return res
+
+
 def cpu_baseline(case, steps):
     sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
     from oracle_case import oracle_from_case
@@ -123,6 +154,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config3-steps", type=int, default=400,
+                    help="steps of the thermo-mechanical contact case reported beside the headline (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -228,6 +261,8 @@ def main():
                     "(profiles/traffic_cyl_upset_976k.md)",
         },
     }
+    if args.config3_steps > 0:
+        out["config"]["config3_contact_thermal"] = contact_run(args.config3_steps)
     if not args.no_cpu:
         ccase = cases.config2(steps=args.cpu_steps)
         log(f"bench: cpu baseline, {args.cpu_steps} steps")

@@ -19,6 +19,7 @@
 // Link with -lweldform_hip (weldform_amd/lib); compiles with any C++17
 // compiler, no HIP headers needed.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -113,7 +114,12 @@ class Domain {
   int device = 0;
   int max_nb = 128;
   size_t Scheme = 0;   // accepted, unused: the solver entry point picks the scheme
-  double CFL = 0.3;   // accepted, unused while auto_ts is false
+  // adaptive time step (Source/Domain.cpp:109,153,170): velocity criterion
+  // CFL, acceleration criterion sqrt_h_a, and the floor manual_min_ts, an int
+  // in the reference (Source/Domain.h:328), so its 1.e-10 default stores 0
+  double CFL = 0.7;
+  double sqrt_h_a = 0.0025;
+  int manual_min_ts = 0;
 
   void Kernel_Set(Kernels_Type const &KT) {
     KernelType = KT;
@@ -310,8 +316,8 @@ class Domain {
     if (KernelType != Qubic_Spline) throw new Fatal("the device path implements the cubic spline kernel only");
     if (GradientType != Squared_density) throw new Fatal("the device path implements GradientType 0 only");
     if (gradKernelCorr) throw new Fatal("kernel gradient correction is not implemented on the device path");
-    if (auto_ts || auto_ts_acc)
-      throw new Fatal("adaptive time stepping is not implemented on the device path: set auto_ts = false");
+    if (contact && auto_ts_cont)
+      throw new Fatal("the contact time-step criterion (auto_ts_cont) is not implemented on the device path");
     if (contact) {
       if (trimesh.empty()) throw new Fatal("contact is set but no rigid mesh was added (AddTrimeshParticles)");
       if (contact_alg != 0) throw new Fatal("only the Wang contact algorithm is implemented on the device path");
@@ -534,8 +540,13 @@ class Domain {
   // SolveDiffUpdateLeapFrog / KickDrift (Source/SolverLeapfrog.cpp:24-451,
   // Source/SolverKickDrift.cpp:23-401): the loop runs while Time <= tf and
   // idx_out <= maxidx and writes output after the step that reaches tout.
-  // Time advances on the host with the same arithmetic, so the device runs
-  // whole stretches of steps between outputs without synchronising.
+  // With a fixed step, Time advances on the host with the same arithmetic,
+  // so the device runs whole stretches of steps between outputs without
+  // synchronising.  With the adaptive step the device holds Time; a step
+  // never exceeds max(2 x the previous step, manual_min_ts) nor
+  // max(dt, manual_min_ts) (AdaptiveTimeStep blends harmonically towards
+  // the initial dt), so each stretch is as long as cannot reach tout or tf
+  // before its last step, and Time is read back after it.
   void solve(double tf, double dt, double dtOut, char const *key, size_t maxidx, int scheme) {
     check_supported();
     std::vector<wf_material> table;
@@ -551,6 +562,10 @@ class Domain {
     cfg.device = device;
     cfg.cellfac = Cellfac;
     cfg.pl_work_heat_frac = pl_work_heat_frac;
+    cfg.auto_ts = (auto_ts ? 1 : 0) | (auto_ts_acc ? 2 : 0);
+    cfg.cfl = CFL;
+    cfg.sqrt_h_a = sqrt_h_a;
+    cfg.min_ts = (double)manual_min_ts;
     for (int k = 0; k < 3; ++k) {
       cfg.dom_max[k] = DomMax(k);
       cfg.dom_min[k] = DomMin(k);
@@ -582,22 +597,44 @@ class Domain {
     cforce_hist_.clear();
     size_t idx_out = 1;
     double tout = Time;
+    const bool adaptive = auto_ts || auto_ts_acc;
     while (Time <= tf && idx_out <= maxidx) {
       int64_t k = 0;
       double t = Time;
       bool out = false;
-      while (t <= tf && k < kMaxChunk) {
-        t += deltat;
-        ++k;
-        if (t >= tout) {
-          out = true;
-          break;
+      if (!adaptive) {
+        while (t <= tf && k < kMaxChunk) {
+          t += deltat;
+          ++k;
+          if (t >= tout) {
+            out = true;
+            break;
+          }
+        }
+      } else {
+        // largest k whose first k-1 steps surely end before min(tout, tf)
+        const double rem = std::min(tout, tf) - Time, floor_ts = (double)manual_min_ts;
+        double bound = 0.0, step = deltat;
+        k = 1;
+        while (k < kMaxChunk) {
+          step = std::min(std::max(2.0 * step, floor_ts), std::max(dt, floor_ts));
+          if ((bound + step) * (1.0 + 1e-9) >= rem) break;
+          bound += step;
+          ++k;
         }
       }
       check(wf_step(ctx_, scheme, k));
       check(wf_sync(ctx_));
       pull_reactions(k);
-      Time = t;
+      if (adaptive) {
+        wf_scalars s;
+        check(wf_get_scalars(ctx_, &s));
+        Time = s.time;
+        deltat = s.dt;
+        out = Time >= tout;
+      } else {
+        Time = t;
+      }
       if (out) {
         if (key) {
           download();

@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 3
+#define WF_ABI_VERSION 4
 
 /* error codes */
 #define WF_OK 0
@@ -143,11 +143,17 @@ typedef struct wf_config {
                                lane order (0: the default for the precision).
                                Sums, and so results, depend on it at the
                                rounding level.                                */
-  int32_t reserved;
+  int32_t auto_ts;          /* adaptive time step (Domain::AdaptiveTimeStep):
+                               bit 0 velocity criterion (auto_ts), bit 1
+                               acceleration criterion (auto_ts_acc); 0: fixed
+                               dt.  Leapfrog and KickDrift.                    */
   double cellfac;           /* Domain::Cellfac (2 for the cubic spline)       */
   double pl_work_heat_frac; /* Domain::pl_work_heat_frac (0.9)                */
   double dom_max[3];        /* Domain::DomMax (default -1e11)                  */
   double dom_min[3];        /* Domain::DomMin (default +1e11)                  */
+  double cfl;               /* Domain::CFL (0.7) for the velocity criterion    */
+  double sqrt_h_a;          /* Domain::sqrt_h_a (0.0025), acceleration one     */
+  double min_ts;            /* Domain::manual_min_ts (1e-10)                   */
 } wf_config;
 
 typedef struct wf_material {

@@ -91,6 +91,10 @@ struct wfo_domain {
   int *rig, *cont;                /* RIGPairs and ContPairs, (temp1, temp2) */
   int64_t nrig, caprig, ncont, capcont;
   double m_contact_force[WFO_MAX_MESH][3], contact_force_sum, contact_reaction_sum, ext_forces_work;
+  /* adaptive time step (Source/Domain.cpp:206-319): bit 0 velocity
+   * criterion (auto_ts), bit 1 acceleration criterion (auto_ts_acc) */
+  int auto_ts;
+  double CFL, sqrt_h_a, manual_min_ts, deltatint, deltatmin;
 };
 
 static int contact_nb_update(wfo_domain *d);
@@ -117,6 +121,11 @@ wfo_domain *wfo_create(int n) {
   for (int k = 0; k < 3; ++k) { d->DomMax[k] = -100000000000.0; d->DomMin[k] = 100000000000.0; }
   d->ts_nb_inc = 5;
   d->pl_work_heat_frac = 0.9;
+  d->CFL = 0.7;
+  d->sqrt_h_a = 0.0025;
+  /* Domain::manual_min_ts is declared int (Source/Domain.h:328), so the
+   * 1.e-10 of Source/Domain.cpp:170 stores 0: no floor by default */
+  d->manual_min_ts = 0.0;
   d->LL = (int *)xcalloc((size_t)n, sizeof(int));
   d->ipair = (int *)xcalloc((size_t)n, sizeof(int));
   d->jpair = (int *)xcalloc((size_t)n, sizeof(int));
@@ -170,6 +179,10 @@ int wfo_set_param(wfo_domain *d, const char *s, double v) {
   else if (!strcmp(s, "friction_sta")) d->friction_sta = v;
   else if (!strcmp(s, "friction_dyn")) d->friction_dyn = v;
   else if (!strcmp(s, "DFAC")) d->DFAC = v;
+  else if (!strcmp(s, "auto_ts")) d->auto_ts = (int)v;
+  else if (!strcmp(s, "CFL")) d->CFL = v;
+  else if (!strcmp(s, "sqrt_h_a")) d->sqrt_h_a = v;
+  else if (!strcmp(s, "manual_min_ts")) d->manual_min_ts = v;
   else if (!strcmp(s, "Time")) d->Time = v;
   else if (!strcmp(s, "deltat")) d->deltat = v;
   else if (!strncmp(s, "DomMax", 6) && s[6] >= '0' && s[6] <= '2') d->DomMax[s[6] - '0'] = v;
@@ -1346,9 +1359,60 @@ const int *wfo_contact_pairs(const wfo_domain *d) { return d->cont; }
 
 /* ------------------------------------------------------------------ */
 /* Solver set-up (Source/SolverLeapfrog.cpp:24-103)                     */
+/* CheckMinTSVel (Source/Domain.cpp:265-299): smallest neighbour distance of
+ * each particle over its Nishimura lists, dt >= CFL d_min / (Cs + |v|) */
+static void check_min_ts_vel(wfo_domain *d) {
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V], *cs = d->f[WFO_CS];
+  d->deltatmin = d->deltatint;
+  for (int i = 0; i < d->n; i++) {
+    double mn = 1000.;
+    for (int k = 0; k < d->ipair[i] + d->jpair[i]; k++) {
+      int j = k < d->ipair[i] ? d->Anei[(int64_t)i * WFO_MAX_NB + k]
+                              : d->Anei[(int64_t)i * WFO_MAX_NB + WFO_MAX_NB - 1 - (k - d->ipair[i])];
+      double xij[3] = {x[3 * j] - x[3 * i], x[3 * j + 1] - x[3 * i + 1], x[3 * j + 2] - x[3 * i + 2]};
+      double dd = norm3(xij);
+      if (dd < mn) mn = dd;
+    }
+    double test = d->CFL * mn / (cs[i] + norm3(v + 3 * i));
+    if (d->deltatmin > test) d->deltatmin = test;
+  }
+}
+
+/* CheckMinTSAccel (Source/Domain.cpp:304-319): dt >= sqrt_h_a sqrt(h / |a|) */
+static void check_min_ts_accel(wfo_domain *d) {
+  d->deltatmin = d->deltatint;
+  for (int i = 0; i < d->n; i++) {
+    double test = d->sqrt_h_a * sqrt(d->f[WFO_H][i] / norm3(d->f[WFO_A] + 3 * i));
+    if (d->deltatmin > test) d->deltatmin = test;
+  }
+}
+
+/* AdaptiveTimeStep (Source/Domain.cpp:206-239) without contact time steps */
+static void adaptive_time_step(wfo_domain *d) {
+  if (d->deltatint > d->deltatmin) {
+    if (d->deltat < d->deltatmin) d->deltat = 2.0 * d->deltat * d->deltatmin / (d->deltat + d->deltatmin);
+    else d->deltat = d->deltatmin;
+  } else {
+    if (d->deltatint != d->deltat) d->deltat = 2.0 * d->deltat * d->deltatint / (d->deltat + d->deltatint);
+    else d->deltat = d->deltatint;
+  }
+  if (d->deltat < d->manual_min_ts) d->deltat = d->manual_min_ts;
+}
+
+/* the time-step update of both solvers: prev_deltat = deltat, then the
+ * velocity and/or acceleration check (the later one wins) */
+static double update_time_step(wfo_domain *d) {
+  double prev = d->deltat;
+  if (d->auto_ts & 1) check_min_ts_vel(d);
+  if (d->auto_ts & 2) check_min_ts_accel(d);
+  if (d->auto_ts) adaptive_time_step(d);
+  return prev;
+}
+
 int wfo_solve_init(wfo_domain *d, double dt) {
   if (d->n <= 0) return fail("Particle Count is Null");
   d->deltat = dt;
+  d->deltatint = d->deltatmin = dt;
   initial_checks(d);
   memset(d->TRPR, 0, sizeof d->TRPR);
   memset(d->BLPF, 0, sizeof d->BLPF);
@@ -1447,7 +1511,7 @@ int wfo_step_leapfrog(wfo_domain *d) {
   if (d->isfirst)
     for (int i = 0; i < 3 * d->n; i++) v[i] -= a[i] * 0.5 * d->deltat;
   wfo_general_after(d);
-  double prev = d->deltat;
+  double prev = update_time_step(d);
   for (int i = 0; i < 3 * d->n; i++) v[i] += a[i] * 0.5 * (prev + d->deltat);
   wfo_general_after(d);
   wfo_calc_dens(d);
@@ -1472,7 +1536,7 @@ int wfo_step_kickdrift(wfo_domain *d) {
   wfo_general_after(d);
   if (d->contact) contact_forces(d);
   double *v = d->f[WFO_V], *a = d->f[WFO_A];
-  double prev = d->deltat;
+  double prev = update_time_step(d);
   for (int i = 0; i < 3 * d->n; i++) v[i] += a[i] * 0.5 * prev * 1.;
   wfo_general_after(d);
   wfo_calc_dens(d);

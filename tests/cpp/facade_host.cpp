@@ -67,7 +67,10 @@ int main() {
                 mesh->element[0]->radius, R->normal(2), R->ID, dom.first_fem_particle_idx[0]);
     delete mesh;
   }
-  expect_fatal("auto_ts", [](SPH::Domain &d) { d.auto_ts = true; });
+  expect_fatal("auto_ts_cont", [](SPH::Domain &d) {
+    d.contact = true;
+    d.auto_ts_cont = true;
+  });
   expect_fatal("kernel", [](SPH::Domain &d) { d.Kernel_Set(SPH::Quintic_Spline); });
   expect_fatal("gradient", [](SPH::Domain &d) { d.Gradient_Approach_Set(SPH::Randles_Libersky); });
   expect_fatal("callback", [](SPH::Domain &d) { d.GeneralAfter = [](SPH::Domain &) {}; });

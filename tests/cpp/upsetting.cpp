@@ -4,8 +4,11 @@
 // nu 0.3), bilinear hardening Sy = 300 MPa, Et = 1.5 GPa, Monaghan alpha = 1,
 // bottom row fixed (zone 2), top row v_z = -1 m/s (zone 3), dt = 0.4 h / Cs.
 //
-//   upsetting STEPS OUTKEY [leapfrog|kickdrift] [fp64|mixed]
+//   upsetting STEPS OUTKEY [leapfrog|kickdrift] [fp64|mixed] [fixed|auto]
 //
+// "auto" turns on the velocity time-step criterion (Domain::auto_ts) and
+// starts from dt = h / Cs, above its limit 0.7 dx / Cs; the end time stays
+// that of STEPS fixed steps.
 // Writes OUTKEY_reaction.txt (one line "Rx Ry Rz" per step for zone 3),
 // OUTKEY_final.txt (one line "ux uy uz sigma_eq pl_strain" per particle,
 // %.17g) and the solver's CSV outputs OUTKEY_NNNN.csv.
@@ -19,20 +22,21 @@
 
 int main(int argc, char **argv) try {
   if (argc < 3) {
-    std::fprintf(stderr, "usage: %s STEPS OUTKEY [leapfrog|kickdrift] [fp64|mixed]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s STEPS OUTKEY [leapfrog|kickdrift] [fp64|mixed] [fixed|auto]\n", argv[0]);
     return 2;
   }
   const long steps = std::atol(argv[1]);
   const std::string key = argv[2];
   const bool kickdrift = argc > 3 && std::strcmp(argv[3], "kickdrift") == 0;
   const bool mixed = argc > 4 && std::strcmp(argv[4], "mixed") == 0;
+  const bool adaptive = argc > 5 && std::strcmp(argv[5], "auto") == 0;
 
   SPH::Domain dom;
   dom.Dimension = 3;
   dom.Nproc = 1;
   dom.Kernel_Set(SPH::Qubic_Spline);
   dom.Gradient_Approach_Set(SPH::Squared_density);
-  dom.auto_ts = false;
+  dom.auto_ts = adaptive;
   dom.ts_nb_inc = 5;
   dom.precision = mixed ? SPH::Precision::Mixed : SPH::Precision::FP64;
 
@@ -44,6 +48,7 @@ int main(int argc, char **argv) try {
   const double G = E / (2. * (1. + nu));
   const double Cs = std::sqrt(K / rho);
   const double dt = 0.4 * h / Cs;
+  const double dt0 = adaptive ? h / Cs : dt;
 
   // Source/WeldForm.cpp:408-411 leaves DomMax(0) = L, DomMin(0) = -L
   dom.DomMax(0) = L;
@@ -89,9 +94,9 @@ int main(int argc, char **argv) try {
   const double tf = (steps - 0.5) * dt;
   const double dtOut = (steps / 2) * dt;
   if (kickdrift)
-    dom.SolveDiffUpdateKickDrift(tf, dt, dtOut, key.c_str(), 1000);
+    dom.SolveDiffUpdateKickDrift(tf, dt0, dtOut, key.c_str(), 1000);
   else
-    dom.SolveDiffUpdateLeapFrog(tf, dt, dtOut, key.c_str(), 1000);
+    dom.SolveDiffUpdateLeapFrog(tf, dt0, dtOut, key.c_str(), 1000);
 
   const std::vector<Vec3_t> &hist = dom.ZoneReactionHistory(3);
   FILE *f = std::fopen((key + "_reaction.txt").c_str(), "w");
@@ -104,8 +109,8 @@ int main(int argc, char **argv) try {
                  P->Displacement(2), P->Sigma_eq, P->pl_strain);
   }
   std::fclose(f);
-  std::printf("upsetting: %zu particles, %zu steps, time %.9g s\n", dom.Particles.Size(), hist.size(),
-              dom.getTime());
+  std::printf("upsetting: %zu particles, %zu steps, time %.17g s, step %.17g s\n", dom.Particles.Size(),
+              hist.size(), dom.getTime(), dom.getStepSize());
   delete mat;
   return 0;
 } catch (Fatal *e) {

@@ -52,6 +52,9 @@ def oracle_from_case(case, threads=1):
         d.set_param(f"DomMin{k}", case.dom_min[k])
     d.set_param("thermal_solver", int(case.thermal))
     d.set_param("pl_heating", int(case.pl_heating))
+    d.set_param("auto_ts", int(case.auto_ts))
+    d.set_param("CFL", case.cfl)
+    d.set_param("sqrt_h_a", case.sqrt_h_a)
     for a in case.amps:
         d.add_amplitude(a["id"], a["t"], a["v"])
     for b in case.bcs:

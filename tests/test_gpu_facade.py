@@ -26,14 +26,17 @@ def _driver():
     return wfbuild.DRIVER
 
 
-def _run_driver(tmp_path, steps, scheme, precision):
+def _run_driver(tmp_path, steps, scheme, precision, ts="fixed"):
     key = str(tmp_path / "up")
-    print(f"facade driver: {steps} steps {scheme} {precision}", flush=True)
-    r = subprocess.run([_driver(), str(steps), key, scheme, precision], capture_output=True, text=True,
+    print(f"facade driver: {steps} steps {scheme} {precision} {ts}", flush=True)
+    r = subprocess.run([_driver(), str(steps), key, scheme, precision, ts], capture_output=True, text=True,
                        timeout=100)
     assert r.returncode == 0, r.stdout + r.stderr
     react = np.loadtxt(key + "_reaction.txt").reshape(-1, 3)
     final = np.loadtxt(key + "_final.txt")
+    if ts == "auto":
+        words = r.stdout.split()
+        return key, react, final, float(words[words.index("time") + 1])
     return key, react, final
 
 
@@ -78,3 +81,33 @@ def test_facade_driver_mixed_precision(tmp_path):
     e_u = normwise_err(final[:, :3], d.f("DISP"))
     print(f"facade mixed: reaction {e_r:.3e} displacement {e_u:.3e}", flush=True)
     assert e_r < 1e-4 and e_u < 1e-4
+
+
+def test_facade_driver_adaptive_step(tmp_path):
+    """Domain::auto_ts through the facade: the solve loop runs on the device
+    clock (Time <= tf with the adapted steps), outputs follow Time >= tout."""
+    steps = 300
+    key, react, final, t_end = _run_driver(tmp_path, steps, "leapfrog", "fp64", "auto")
+    case = cases.config1(steps=steps)
+    tf = (steps - 0.5) * case.dt
+    case.auto_ts = 1
+    dt0 = case.dt / 0.4   # h / Cs
+    d = oracle_from_case(case, threads=8)
+    d.solve_init(dt0)
+    R = []
+    while d.param("Time") <= tf:
+        d.step("leapfrog")
+        R.append(d.reaction(case.top_zone).copy())
+    R = np.array(R)
+    print(f"facade adaptive: {len(react)} steps (oracle {len(R)}), time {t_end!r} ({d.param('Time')!r})",
+          flush=True)
+    # the step settles at the velocity limit 0.7 dx / Cs = 0.58 h / Cs, above
+    # the fixed 0.4 h / Cs, so fewer steps cover the same time
+    assert len(R) < 0.8 * steps
+    assert react.shape == R.shape
+    assert t_end == pytest.approx(d.param("Time"), rel=1e-12)
+    e_r = normwise_err(react[:, 2], R[:, 2])
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    print(f"facade adaptive: reaction {e_r:.3e} displacement {e_u:.3e}", flush=True)
+    assert e_r < 1e-8 and e_u < 1e-8
+    assert os.path.exists(f"{key}_0001.csv") and os.path.exists(f"{key}_0002.csv")

@@ -495,3 +495,77 @@ def test_energy_sums():
     SR = float(np.sum(S * R))
     assert d.param("int_energy_sum") == pytest.approx((V[0] * SR - V[1] * 2 * SR) * dt, rel=1e-13)
     assert d.param("kin_energy_sum") == 0.0
+
+
+# ---------------------------------------------------------------- time step
+def _rest_block(n=5, dx=1e-3, v=None):
+    g = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    x = (g + 0.5) * dx
+    d = po.Domain(len(x))
+    E, nu, rho = 68.9e9, 0.3, 2700.0
+    K = E / (3 * (1 - 2 * nu))
+    d.f("X")[:] = x
+    d.f("H")[:] = 1.2 * dx
+    d.f("MASS")[:] = rho * dx ** 3
+    d.f("RHO")[:] = d.f("REFRHO")[:] = rho
+    d.f("CS")[:] = math.sqrt(K / rho)
+    d.f("G")[:] = E / (2 * (1 + nu))
+    d.f("SIGMAY")[:] = 1e12
+    if v is not None:
+        d.f("V")[:] = v
+    d.set_material(0, model=po.BILINEAR, E=E, nu=nu)
+    return d, x, math.sqrt(K / rho)
+
+
+def test_adaptive_time_step_velocity_criterion():
+    """CheckMinTSVel + AdaptiveTimeStep (Source/Domain.cpp:206-299): a
+    resting lattice gives dt_min = CFL dx / Cs, taken at once when below
+    the initial dt; a particle moving at |v| lowers it to CFL dx/(Cs+|v|);
+    Time advances by the new dt."""
+    dx, dt0 = 1e-3, 1e-6
+    d, x, cs = _rest_block()
+    d.set_param("auto_ts", 1)
+    d.solve_init(dt0)
+    d.step()
+    want = 0.7 * dx / cs
+    assert d.param("deltat") == pytest.approx(want, rel=1e-13)
+    assert d.param("Time") == pytest.approx(want, rel=1e-13)
+    v = np.zeros((125, 3))
+    v[62] = [30.0, 40.0, 0.0]          # |v| = 50
+    d, x, cs = _rest_block(v=v)
+    d.set_param("auto_ts", 1)
+    d.solve_init(dt0)
+    d.step()
+    assert d.param("deltat") == pytest.approx(0.7 * dx / (cs + 50.0), rel=1e-12)
+
+
+def test_adaptive_time_step_harmonic_recovery():
+    """Below dt_min the step grows by the harmonic mean 2 dt dt_min /
+    (dt + dt_min); with no criterion below the initial dt it returns to it
+    the same way."""
+    dx = 1e-3
+    d, x, cs = _rest_block()
+    dmin = 0.7 * dx / cs
+    d.set_param("auto_ts", 1)
+    d.solve_init(1e-6)
+    d.set_param("deltat", 0.25 * dmin)
+    d.step()
+    assert d.param("deltat") == pytest.approx(2 * 0.25 * dmin * dmin / (1.25 * dmin), rel=1e-13)
+    d, x, cs = _rest_block()
+    dint = 0.5 * dmin
+    d.set_param("auto_ts", 1)
+    d.solve_init(dint)
+    d.set_param("deltat", 0.1 * dint)
+    d.step()
+    assert d.param("deltat") == pytest.approx(2 * 0.1 * dint * dint / (1.1 * dint), rel=1e-13)
+
+
+def test_adaptive_time_step_acceleration_criterion():
+    """CheckMinTSAccel: a resting lattice has a = 0, so sqrt_h_a sqrt(h/|a|)
+    is infinite and dt stays at its initial value; with both criteria the
+    acceleration one, computed last, is the one kept."""
+    d, x, cs = _rest_block()
+    d.set_param("auto_ts", 3)
+    d.solve_init(1e-6)
+    d.step()
+    assert d.param("deltat") == 1e-6

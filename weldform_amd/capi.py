@@ -46,9 +46,10 @@ class Config(ctypes.Structure):
                 ("max_nb", ctypes.c_int32), ("thermal_solver", ctypes.c_int32),
                 ("pl_heating", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("no_graphs", ctypes.c_int32), ("stage_max", ctypes.c_int32), ("lanes", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("cellfac", ctypes.c_double),
+                ("auto_ts", ctypes.c_int32), ("cellfac", ctypes.c_double),
                 ("pl_work_heat_frac", ctypes.c_double), ("dom_max", ctypes.c_double * 3),
-                ("dom_min", ctypes.c_double * 3)]
+                ("dom_min", ctypes.c_double * 3), ("cfl", ctypes.c_double), ("sqrt_h_a", ctypes.c_double),
+                ("min_ts", ctypes.c_double)]
 
 
 class Material(ctypes.Structure):
@@ -168,11 +169,12 @@ class Solver:
     def __init__(self, n, precision=FP64, ts_nb_inc=5, max_nb=128, thermal_solver=False,
                  pl_heating=False, device=0, cellfac=2.0, pl_work_heat_frac=0.9,
                  dom_max=(-1e11, -1e11, -1e11), dom_min=(1e11, 1e11, 1e11), graphs=True, stage_max=0,
-                 lanes=0):
+                 lanes=0, auto_ts=0, cfl=0.7, sqrt_h_a=0.0025, min_ts=0.0):
         cfg = Config(precision=precision, ts_nb_inc=ts_nb_inc, max_nb=max_nb,
                      thermal_solver=int(thermal_solver), pl_heating=int(pl_heating),
                      device=device, no_graphs=int(not graphs), stage_max=stage_max, lanes=lanes, cellfac=cellfac,
-                     pl_work_heat_frac=pl_work_heat_frac)
+                     pl_work_heat_frac=pl_work_heat_frac, auto_ts=auto_ts, cfl=cfl, sqrt_h_a=sqrt_h_a,
+                     min_ts=min_ts)
         cfg.dom_max[:] = list(dom_max)
         cfg.dom_min[:] = list(dom_min)
         self.n = int(n)

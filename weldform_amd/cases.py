@@ -40,6 +40,9 @@ class Case:
     tinf: np.ndarray = None
     meshes: list = field(default_factory=list)  # dicts: zone, axis, flip, p1, p2, dens, hfac
     contact: tuple = (0.0, 0.0, 0.0)            # friction_sta, friction_dyn, DFAC
+    auto_ts: int = 0            # bit 0 velocity, bit 1 acceleration criterion
+    cfl: float = 0.7
+    sqrt_h_a: float = 0.0025
     meta: dict = field(default_factory=dict)
 
     @property
@@ -184,10 +187,38 @@ def contact_small(steps=200, v_punch=-5.0, mu=0.2, dens=22, seed=13, thermal=Fal
     return c
 
 
+def config3(steps=2000, dx=0.3757e-3, R=15e-3, L=30e-3, dens=32, v_punch=-5.0, mu=0.2, seed=2027):
+    """Thermo-mechanical contact compression at ~385k particles: steel-like
+    JC billet R=15 mm, L=30 mm, dx=0.3757 mm (+-1e-3 dx jitter), rho 7850,
+    E 200 GPa, nu 0.29, k 50 W/mK, cp 460 J/kgK, Taylor-Quinney 0.9,
+    T0 = 300 K; conduction + plastic heating every step; rigid z-planes of
+    2 x 32^2 = 2,048 triangles top (driven at -5 m/s) and bottom (fixed),
+    Coulomb friction mu = 0.2; the planes start one h beyond the billet."""
+    c = cylinder_case("cyl_compress_385k_contact_thermal", R, L, dx, 7850.0, 200e9, 0.29, STEEL_JC,
+                      steps=steps, jitter=1e-3, seed=seed, v_top=0.0, T0=300.0)
+    h = c.meta["h"]
+    zlo, zhi = c.x[:, 2].min(), c.x[:, 2].max()
+    W = 1.35 * R
+    c.ids[:] = 0
+    c.bcs = [dict(zone=10, value=(0.0, 0.0, v_punch))]
+    c.meshes = [dict(zone=10, axis=2, flip=False, p1=(-W, -W, zhi + 1.02 * h), p2=(W, W, zhi + 1.02 * h), dens=dens,
+                     hfac=1.1),
+                dict(zone=11, axis=2, flip=True, p1=(-W, -W, zlo - 1.02 * h), p2=(W, W, zlo - 1.02 * h), dens=dens,
+                     hfac=1.1)]
+    c.contact = (mu, mu, 0.0)
+    c.top_zone = 10
+    c.dom_max = (-1e11, -1e11, -1e11)
+    c.dom_min = (1e11, 1e11, 1e11)
+    c.thermal = True
+    c.pl_heating = True
+    return c
+
+
 def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,
-                    dom_max=case.dom_max, dom_min=case.dom_min, **kw)
+                    dom_max=case.dom_max, dom_min=case.dom_min, auto_ts=case.auto_ts, cfl=case.cfl,
+                    sqrt_h_a=case.sqrt_h_a, min_ts=0.0, **kw)
     for idx, m in case.materials.items():
         s.set_material(idx, **m)
     s.set("x", case.x)

@@ -85,7 +85,7 @@ struct Ctx {
   std::vector<void *> allocs;
   DevMat *dmats = nullptr;
   int cur = 0;
-  double time = 0.0, dt = 0.0;
+  double dt = 0.0;  // the initial step (Domain::deltatint); the device flags hold the current one
   long long steps = 0;
   std::vector<int> rzones;
   BcTable bc{};
@@ -432,6 +432,10 @@ void allocate(Ctx *c) {
   d.stage_max = 0;
   d.lanes_mx = c->cfg.lanes ? c->cfg.lanes : kLanesMixed;
   d.lanes_f64 = c->cfg.lanes ? c->cfg.lanes : kLanesFP64;
+  d.auto_ts = c->cfg.auto_ts;
+  d.cfl = c->cfg.cfl;
+  d.sqrt_h_a = c->cfg.sqrt_h_a;
+  d.min_ts = c->cfg.min_ts;
   if (d.mixed) {
     // stage_cnt = -1 marks a tile that gathers from HBM
     d.stage_cnt = c->alloc<int>((size_t)d.ntiles);
@@ -561,6 +565,7 @@ void upload_all(Ctx *c) {
   DevFlags fl{};
   fl.cleared = 1;
   fl.ts_nb_inc = c->cfg.ts_nb_inc;
+  fl.dt = fl.dt_prev = fl.dtint = c->dt;
   const double *pl = field(WF_F_PL_STRAIN);
   if (pl)
     for (int64_t i = 0; i < n; ++i)
@@ -675,23 +680,33 @@ void enqueue_step(Ctx *c, int scheme, int cur) {
   // contact acts after AccelReduction and the first GeneralAfter
   // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
   // so no BC zone applies to them
-  if (d.contact) launch_contact(d, c->st, c->bc, cur, c->dt);
-  launch_kick(d, c->st, c->bc, scheme, c->dt, c->dt);
+  // dt < 0: every kernel reads the step size from the device flags, which
+  // the adaptive step updates between the two halves of the kick
+  // (Source/SolverLeapfrog.cpp:241-262, Source/SolverKickDrift.cpp:208-226)
+  const double dt = d.auto_ts ? -1.0 : c->dt;
+  if (d.contact) launch_contact(d, c->st, c->bc, cur, dt);
+  if (d.auto_ts) {
+    launch_kick(d, c->st, c->bc, scheme, dt, dt, 1);
+    launch_time_step(d, c->st, cur);
+    launch_kick(d, c->st, c->bc, scheme, dt, dt, 2);
+  } else {
+    launch_kick(d, c->st, c->bc, scheme, dt, dt, 0);
+  }
   {
     Timed t(c, T_DENS);
-    launch_dens_drift(d, c->st, cur, c->dt, 1);
+    launch_dens_drift(d, c->st, cur, dt, 1);
   }
-  if (scheme == WF_KICKDRIFT) launch_kick2(d, c->st, c->dt);
-  if (d.contact) launch_rigid_drift(d, c->st, c->dt);
+  if (scheme == WF_KICKDRIFT) launch_kick2(d, c->st, dt);
+  if (d.contact) launch_rigid_drift(d, c->st, dt);
   const int nxt = cur ^ 1;
   {
     Timed t(c, T_RATES);
     launch_rates(d, c->st, nxt);
   }
-  launch_stress(d, c->st, nxt, c->dt);
-  if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, c->dt);
-  launch_finalize(d, c->st, c->dt, c->bc.nz);
-  if (d.contact) launch_mesh_update(d, c->st, c->dt);
+  launch_stress(d, c->st, nxt, dt);
+  if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, dt);
+  launch_finalize(d, c->st, dt, c->bc.nz);
+  if (d.contact) launch_mesh_update(d, c->st, dt);
 }
 
 void one_step(Ctx *c, int scheme) {
@@ -713,7 +728,6 @@ void one_step(Ctx *c, int scheme) {
   }
   c->built = true;
   c->cur ^= 1;
-  c->time += c->dt;
   c->steps += 1;
 }
 
@@ -752,6 +766,9 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
       throw Error(WF_EINVAL, "lanes must be 0, 1, 2 or 4");
     if (cfg->stage_max < -1 || cfg->stage_max > kStageMax)
       throw Error(WF_EINVAL, "stage_max must be -1, 0 or 1.." + std::to_string(kStageMax));
+    if (cfg->auto_ts < 0 || cfg->auto_ts > 3) throw Error(WF_EINVAL, "auto_ts must be 0..3");
+    if (cfg->auto_ts && !(cfg->cfl > 0 && cfg->sqrt_h_a > 0 && cfg->min_ts >= 0))
+      throw Error(WF_EINVAL, "auto_ts needs cfl > 0, sqrt_h_a > 0 and min_ts >= 0");
     Ctx *c = new Ctx();
     c->cfg = *cfg;
     c->n = n;
@@ -1076,6 +1093,7 @@ int wf_init(void *ctx, double dt) {
       }
     }
     HIPCHK(hipSetDevice(c->cfg.device));
+    c->dt = dt;
     cell_initiate(c);
     make_permutation(c);
     allocate(c);
@@ -1090,8 +1108,6 @@ int wf_init(void *ctx, double dt) {
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
-    c->dt = dt;
-    c->time = 0.0;
     c->steps = 0;
     c->cur = 0;
     c->inited = true;
@@ -1133,8 +1149,8 @@ int wf_get_scalars(void *ctx, wf_scalars *out) {
     auto s = dget(c, c->d.scalars, S_COUNT);
     auto f = dget(c, c->d.flags, 1)[0];
     std::memset(out, 0, sizeof *out);
-    out->time = c->time;
-    out->dt = c->dt;
+    out->time = f.time;
+    out->dt = f.dt;
     out->steps = c->steps;
     out->nb_searches = f.nb_searches;
     out->npairs = f.directed_pairs / 2;

@@ -96,6 +96,9 @@ struct DevFlags {
   long long directed_pairs;
   long long step;    // completed steps since wf_init (reaction history slot)
   double time;       // Domain::Time at the start of the next step
+  double dt;         // Domain::deltat (changes only with the adaptive step)
+  double dt_prev;    // prev_deltat of the step
+  double dtint;      // Domain::deltatint (the initial dt)
 };
 
 // partial-sum slots per block
@@ -109,6 +112,7 @@ enum {
   P_CFSUM = P_CFORCE + 3 * kMaxMesh,    // sum |contforce|
   P_CREACT,                             // sum m a.n of contact particles
   P_EXTW,                               // sum contforce . v_rigid
+  P_DTMIN,                              // min time-step criterion
   P_COUNT
 };
 
@@ -181,6 +185,8 @@ struct Dev {
   // and each particle's own position (selfl)
   int stage_max, ntiles;
   int lanes_mx, lanes_f64;     // threads per particle in the pair passes
+  int auto_ts;                 // bit 0 velocity, bit 1 acceleration criterion
+  double cfl, sqrt_h_a, min_ts;
   int *stage, *stage_cnt;
   uint16_t *nbrl, *selfl;
   // rigid contact planes (all meshes concatenated): nodes, elements
@@ -211,7 +217,11 @@ struct Dev {
 void launch_schedule(const Dev &d, hipStream_t st, int force);
 void launch_neighbours(const Dev &d, hipStream_t st, int cur);
 void launch_accel(const Dev &d, hipStream_t st, int cur);
-void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt);
+// dt < 0 in a launcher: the kernel reads the step from the device flags
+// (adaptive time step).  phase 0: the whole kick; 1: BCs and the first-step
+// half kick; 2: the kick with (dt_prev, dt) and the BCs.
+void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt, int phase);
+void launch_time_step(const Dev &d, hipStream_t st, int cur);
 void launch_kick2(const Dev &d, hipStream_t st, double dt);
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update);
 void launch_rates(const Dev &d, hipStream_t st, int buf);

@@ -787,11 +787,15 @@ __device__ inline double amp_value(const DevAmp &a, double t) {
   return a.v[i] + (a.v[i + 1] - a.v[i]) / (a.t[i + 1] - a.t[i]) * (t - a.t[i]);
 }
 
-WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt) {
+WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt, int phase) {
   __shared__ double sh[kBlock / 64];
   __shared__ double bcv[kMaxBc][3];
   __shared__ int bcon[kMaxBc];
   const int isfirst = d.flags->step == 0;
+  if (dt < 0) {
+    dt = d.flags->dt;
+    dt_prev = d.flags->dt_prev;
+  }
   if (threadIdx.x < bc.n) {
     // velocity of each BC at the step's time (UserAcc, Source/WeldForm.cpp:67-82)
     const int b = threadIdx.x;
@@ -832,10 +836,11 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt) 
     }
     if (!fixed) {
       if (scheme == WF_LEAPFROG) {
-        if (isfirst)
+        if (isfirst && phase != 2)
           for (int k = 0; k < 3; ++k) v[k] -= av[k] * 0.5 * dt;
-        for (int k = 0; k < 3; ++k) v[k] += av[k] * 0.5 * (dt_prev + dt);
-      } else {
+        if (phase != 1)
+          for (int k = 0; k < 3; ++k) v[k] += av[k] * 0.5 * (dt_prev + dt);
+      } else if (phase != 1) {
         for (int k = 0; k < 3; ++k) v[k] += av[k] * 0.5 * dt_prev;
       }
     }
@@ -843,7 +848,7 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt) 
     if (d.mixed) d.vmf[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)vm.w);
     d.acc[i] = make_double4(av[0], av[1], av[2], 0.0);
   }
-  for (int z = 0; z < bc.nz; ++z) {
+  for (int z = 0; z < bc.nz && phase != 2; ++z) {
     const bool in = (id == bc.rzone[z]);
     const double fx = block_reduce<0>(in ? m * a.x : 0.0, sh);
     const double fy = block_reduce<0>(in ? m * a.y : 0.0, sh);
@@ -858,6 +863,7 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt) 
 
 // KickDrift's second half kick, v += a*dt/2 (Source/SolverKickDrift.cpp:261-265)
 WF_KERNEL void k_kick2(Dev d, double dt) {
+  if (dt < 0) dt = d.flags->dt;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= d.n) return;
   const double4 a = d.acc[i];
@@ -869,8 +875,65 @@ WF_KERNEL void k_kick2(Dev d, double dt) {
   if (d.mixed) d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
 }
 
-void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt) {
-  k_kick<<<d.nblocks, kBlock, 0, st>>>(d, bc, scheme, dt_prev, dt);
+void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt, int phase) {
+  k_kick<<<d.nblocks, kBlock, 0, st>>>(d, bc, scheme, dt_prev, dt, phase);
+}
+
+// CheckMinTSVel / CheckMinTSAccel (Source/Domain.cpp:265-319): per particle
+// the smallest neighbour distance over CFL (Cs + |v|), or sqrt_h_a
+// sqrt(h / |a|) (the acceleration criterion, evaluated last, wins when both
+// are on); block minima, then AdaptiveTimeStep (:206-239) in one thread.
+WF_KERNEL void k_ts_check(Dev d, int cur) {
+#pragma clang fp contract(off)
+  __shared__ double sh[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double test = INFINITY;
+  if (i < d.n) {
+    const double4 xi = d.xh[cur][i];
+    if (d.auto_ts & 2) {
+      const double4 a = d.acc[i];
+      test = d.sqrt_h_a * sqrt(xi.w / sqrt(a.x * a.x + a.y * a.y + a.z * a.z));
+    } else {
+      double mn = 1000.;
+      const int cnt = d.nnb[i];
+      for (int k = 0; k < cnt; ++k) {
+        const double4 xj = d.xh[cur][d.nbr[nb_slot(d.npad, k, i)]];
+        const double dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const double dd = sqrt(dx * dx + dy * dy + dz * dz);
+        if (dd < mn) mn = dd;
+      }
+      const double4 v = d.vm[i];
+      test = d.cfl * mn / (d.mats[d.mat[i]].Cs + sqrt(v.x * v.x + v.y * v.y + v.z * v.z));
+    }
+  }
+  const double r = block_reduce<2>(test, sh);
+  if (threadIdx.x == 0) part(d, P_DTMIN, blockIdx.x) = r;
+}
+
+__global__ __launch_bounds__(1024) void k_ts_adapt(Dev d) {
+  __shared__ double sh[16];
+  double v = INFINITY;
+  for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) v = fmin(v, part(d, P_DTMIN, b));
+  const double r = block_reduce<2>(v, sh);
+  if (threadIdx.x != 0) return;
+  DevFlags *f = d.flags;
+  const double dint = f->dtint, dmin = fmin(dint, r);
+  double dt = f->dt;
+  f->dt_prev = dt;
+  if (dint > dmin) {
+    if (dt < dmin) dt = 2.0 * dt * dmin / (dt + dmin);
+    else dt = dmin;
+  } else {
+    if (dint != dt) dt = 2.0 * dt * dint / (dt + dint);
+    else dt = dint;
+  }
+  if (dt < d.min_ts) dt = d.min_ts;
+  f->dt = dt;
+}
+
+void launch_time_step(const Dev &d, hipStream_t st, int cur) {
+  k_ts_check<<<d.nblocks, kBlock, 0, st>>>(d, cur);
+  k_ts_adapt<<<1, 1024, 0, st>>>(d);
 }
 
 void launch_kick2(const Dev &d, hipStream_t st, double dt) { k_kick2<<<nblk(d.n), kBlock, 0, st>>>(d, dt); }
@@ -909,6 +972,7 @@ __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, co
 
 template <bool MX>
 __device__ inline void dens_update(const Dev &d, int cur, int i, double s, double dt, int do_update) {
+  if (dt < 0) dt = d.flags->dt;
   const double rhoi = d.rho[cur][i];
   const double drho = s * rhoi;
   d.drho[i] = drho;
@@ -1088,6 +1152,7 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
 
 template <bool MX>
 WF_KERNEL void k_stress(Dev d, int buf, double dt) {
+  if (dt < 0) dt = d.flags->dt;
   __shared__ double sh[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const long long np = d.npad;
@@ -1241,6 +1306,7 @@ __device__ inline bool pair_first(int ci, int cj, int oi, int oj, int ny, int nz
 }
 
 WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
+  if (dt < 0) dt = d.flags->dt;
   __shared__ double sh[kBlock / 64];
   const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const long long np = d.npad;
@@ -1304,6 +1370,7 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
 // block_reduce: each thread's blocks in order, the wavefront butterfly, then
 // the wavefronts in order.
 __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
+  if (dt < 0) dt = d.flags->dt;
   __shared__ double sh[P_COUNT][16];
   const int nb = d.nblocks;
   bool active[P_COUNT];
@@ -1357,7 +1424,7 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
   if (d.contact) {
     d.scalars[S_CFSUM] = cfsum;
     d.scalars[S_CREACT] = creact;
-    d.scalars[S_EXTW] += extw * dt;
+    d.scalars[S_EXTW] += extw;
   }
   d.scalars[S_KIN] += e_kin * dt;
   d.scalars[S_INT] += e_int * dt;
@@ -1473,6 +1540,7 @@ WF_KERNEL void k_rigid_bc(Dev d, BcTable bc) {
 
 // Wang contact force of each particle over its partners, in order
 WF_KERNEL void k_contact(Dev d, int cur, double dt) {
+  if (dt < 0) dt = d.flags->dt;
   __shared__ double sh[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double cfm[kMaxMesh][3];
@@ -1563,12 +1631,15 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
   if (threadIdx.x == 0) {
     part(d, P_CFSUM, blockIdx.x) = r1;
     part(d, P_CREACT, blockIdx.x) = r2;
-    part(d, P_EXTW, blockIdx.x) = r3;
+    // the work takes the step size of the contact evaluation, before any
+    // adaptive update of the step (Source/Contact_Wang.cpp:344)
+    part(d, P_EXTW, blockIdx.x) = r3 * dt;
   }
 }
 
 // the drift of the rigid particles (the reference's loop covers them)
 WF_KERNEL void k_rigid_drift(Dev d, double dt) {
+  if (dt < 0) dt = d.flags->dt;
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= d.nrig) return;
   double4 x = d.rx[r];
@@ -1583,6 +1654,7 @@ WF_KERNEL void k_rigid_drift(Dev d, double dt) {
 // coefficients are recomputed from the winding, particles take the mean
 // node velocity
 __global__ __launch_bounds__(1024) void k_mesh_update(Dev d, double dt) {
+  if (dt < 0) dt = d.flags->dt;
   const int m = blockIdx.x;
   const double4 mv = d.mv[m];
   const int n0 = d.mesh_node0[m], nn = d.mesh_nnode[m];
