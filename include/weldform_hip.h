@@ -32,6 +32,9 @@
  *   wf_eval(WF_EVAL_STRESS)  <- Particle::CalcStressStrain for every particle
  *                               (Source/Particle.cpp:960-1067)
  *   wf_gen_cylinder          <- Domain::AddCylinderLength (Source/Domain.cpp:708-860)
+ *   wf_adj_*                 <- no counterpart in the reference: the reverse-mode
+ *                               adjoint of the FP64 Leapfrog step for parameter
+ *                               calibration (see the block above wf_adj_forward)
  */
 #ifndef WELDFORM_HIP_H
 #define WELDFORM_HIP_H
@@ -220,7 +223,8 @@ int wf_add_temperature_bc(void *ctx, int32_t zone, double T);
  * `zone` and h = hfac * element radius.  A velocity BC on `zone` moves the
  * plane.  Returns the number of triangles, or a negative WF_E* code.
  * wf_set_contact sets Domain::friction_sta, friction_dyn and DFAC of the
- * Wang penalty contact (Source/Contact_Wang.cpp:15-348). */
+ * Wang penalty contact (Source/Contact_Wang.cpp:15-348), before or after
+ * wf_init (the next step uses the new values). */
 int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, const double p1[3], const double p2[3],
                        int32_t dens, double hfac);
 int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double damping);
@@ -252,6 +256,43 @@ int wf_eval(void *ctx, int32_t what, double dt);
 int wf_set_timing(void *ctx, int32_t enable);
 int wf_get_kernel_time(void *ctx, int32_t kernel, double *avg_ms, int64_t *launches);
 int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_count);
+
+/* adjoint (no counterpart in the reference).  A window of cfg->steps FP64
+ * Leapfrog steps from the current state; the loss is
+ *     L = sum_n (F_n - target_n)^2
+ * with F_n the component cfg->loss_component of, per step n, the contact
+ * force on rigid plane cfg->loss_index (WF_ADJ_PLANE, Domain::
+ * m_contact_force) or the reaction of velocity-BC zone cfg->loss_index
+ * (WF_ADJ_ZONE, as wf_get_reaction_history).  wf_adj_forward runs the window,
+ * keeping a snapshot of the device state every cfg->checkpoint_every steps,
+ * and returns L (and F_n in history[steps] when not NULL).  wf_adj_backward
+ * replays each segment from its snapshot with a tape and runs the reverse
+ * step kernels over it, returning dL/d(A, B, n, C, m) of the Johnson-Cook
+ * material cfg->material and dL/d(friction_sta, friction_dyn) in grad[7]
+ * (friction_sta enters only the static/dynamic switch: its derivative is 0).
+ * Branches (yield, contact activity, friction regime, neighbour lists) are
+ * those of the forward.  After it the state is the window's end state;
+ * wf_adj_get_field gives dL/d(initial state) for WF_F_X, WF_F_V,
+ * WF_F_DENSITY, WF_F_SHEAR (9 per particle), WF_F_SIGMAY, WF_F_PL_STRAIN,
+ * and wf_adj_restore puts the state back to the window's start (to run the
+ * window again, e.g. with other material parameters).  Needs precision
+ * WF_FP64, a fixed step (auto_ts 0) and thermal_solver 0. */
+#define WF_ADJ_PLANE 0
+#define WF_ADJ_ZONE 1
+#define WF_ADJ_NPARAM 7
+typedef struct {
+  int64_t steps;              /* window length                                 */
+  int32_t checkpoint_every;   /* snapshot interval k (tape of k steps)         */
+  int32_t loss_kind;          /* WF_ADJ_PLANE or WF_ADJ_ZONE                   */
+  int32_t loss_index;         /* plane (order of wf_add_rigid_plane) or zone id */
+  int32_t loss_component;     /* 0, 1, 2                                       */
+  int32_t material;           /* material index of the Johnson-Cook gradients  */
+  int32_t reserved;
+} wf_adj_config;
+int wf_adj_forward(void *ctx, const wf_adj_config *cfg, const double *target, double *loss, double *history);
+int wf_adj_backward(void *ctx, double *grad);
+int wf_adj_get_field(void *ctx, int32_t field, double *host);
+int wf_adj_restore(void *ctx);
 
 /* host-only geometry helper: AddCylinderLength lattice (3 doubles per
  * particle).  Returns the particle count; writes at most cap particles. */

@@ -1,0 +1,94 @@
+"""Adjoint of the FP64 Leapfrog step (wf_adj_*) against central finite
+differences of the FP64 forward.
+
+The window starts mid-run (the billet already yielding), runs 30 steps with
+a snapshot every 10, and the loss is sum_n (F_n - target_n)^2 over the top
+zone's reaction (no contact) or the punch plane's contact force (contact).
+Bars: the replayed forward reproduces the first forward bitwise (loss and
+end state); every gradient component within 1e-3 relative of the central
+difference with a 1e-4 relative parameter step (components whose finite
+difference is below 1e-6 of the largest are compared in absolute terms)."""
+import numpy as np
+import pytest
+
+from weldform_amd import capi, cases
+
+pytestmark = pytest.mark.gpu
+
+JC_KEYS = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m")
+
+
+def _fd_check(g, case, mat0, target, kind, index, friction=None):
+    loss0, hist0 = g.adj_forward(target, checkpoint_every=10, kind=kind, index=index, component=2)
+    disp_end = g.get("disp").copy()
+    grad = g.adj_backward()
+    # backward leaves the window's end state
+    assert np.array_equal(g.get("disp"), disp_end)
+    # replay reproduces the forward bitwise
+    g.adj_restore()
+    loss_again, hist_again = g.adj_forward(target, checkpoint_every=10, kind=kind, index=index, component=2)
+    assert loss_again == loss0 and np.array_equal(hist_again, hist0)
+
+    def loss_at(key, value):
+        g.adj_restore()
+        m = dict(mat0)
+        if key in JC_KEYS:
+            m[key] = value
+            g.set_material(0, **m)
+        else:
+            g.set_contact(value, value, 0.0)
+        loss, _ = g.adj_forward(target, checkpoint_every=10, kind=kind, index=index, component=2)
+        g.set_material(0, **mat0)
+        if friction is not None:
+            g.set_contact(friction, friction, 0.0)
+        return loss
+
+    checks = [(k, mat0[k], grad[i]) for i, k in enumerate(JC_KEYS)]
+    if friction is not None:
+        checks.append(("mu", friction, grad[5] + grad[6]))
+    fd = {}
+    for key, p0, gk in checks:
+        h = 1e-4 * abs(p0)
+        fd[key] = (loss_at(key, p0 + h) - loss_at(key, p0 - h)) / (2 * h)
+    scale = max(abs(v) for v in fd.values())
+    print(f"adjoint loss {loss0:.6e}", {k: (fd[k], gk) for k, _, gk in checks}, flush=True)
+    assert scale > 0
+    for key, p0, gk in checks:
+        if abs(fd[key]) > 1e-6 * scale:
+            assert abs(gk - fd[key]) <= 1e-3 * abs(fd[key]), (key, gk, fd[key])
+        else:
+            assert abs(gk - fd[key]) <= 1e-6 * scale, (key, gk, fd[key])
+
+
+def test_adjoint_zone_reaction_matches_fd():
+    # JC billet at config-1 size, T0 = 400 K so the thermal-softening
+    # exponent m matters, top row driven at -20 m/s
+    c = cases.cylinder_case("cyl_upset_4k_jc_adjoint", 7.5e-3, 30e-3, 1.0e-3, cases.AL["rho"], cases.AL["E"],
+                            cases.AL["nu"], cases.JC_AL, steps=130, jitter=1e-3, seed=7, v_top=-20.0, T0=400.0)
+    print(f"adjoint zone case: n={c.n}", flush=True)
+    g = cases.make_solver(c, precision=capi.FP64)
+    g.step(100)
+    g.sync()
+    assert g.scalars()["max_pl_strain"] > 0
+    _, hist = g.adj_forward(np.zeros(30), checkpoint_every=10, kind=capi.ADJ_ZONE, index=c.top_zone, component=2)
+    g.adj_restore()
+    _fd_check(g, c, c.materials[0], 0.98 * hist, capi.ADJ_ZONE, c.top_zone)
+
+
+def test_adjoint_plane_force_matches_fd():
+    c = cases.contact_small(steps=130)
+    print(f"adjoint contact case: n={c.n}", flush=True)
+    g = cases.make_solver(c, precision=capi.FP64)
+    g.step(70)
+    g.sync()
+    assert np.abs(g.contact_force_history(0)[-1]).max() > 0
+    _, hist = g.adj_forward(np.zeros(30), checkpoint_every=10, kind=capi.ADJ_PLANE, index=0, component=2)
+    g.adj_restore()
+    _fd_check(g, c, c.materials[0], 0.98 * hist, capi.ADJ_PLANE, 0, friction=c.contact[0])
+
+
+def test_adjoint_rejects_unsupported():
+    c = cases.small_jc(steps=10)
+    g = cases.make_solver(c, precision=capi.MIXED)
+    with pytest.raises(capi.WFError):
+        g.adj_forward(np.zeros(5), kind=capi.ADJ_ZONE, index=c.top_zone)

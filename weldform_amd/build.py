@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libweldform_hip.so")
-SOURCES = ["wf_kernels.hip", "wf_capi.hip"]
+SOURCES = ["wf_kernels.hip", "wf_adjoint.hip", "wf_capi.hip"]
 ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")
 ROOT = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "upsetting.cpp")

@@ -73,6 +73,16 @@ class Scalars(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class AdjConfig(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("checkpoint_every", ctypes.c_int32), ("loss_kind", ctypes.c_int32),
+                ("loss_index", ctypes.c_int32), ("loss_component", ctypes.c_int32), ("material", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+ADJ_PLANE, ADJ_ZONE = 0, 1
+ADJ_PARAMS = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m", "friction_sta", "friction_dyn")
+ADJ_FIELDS = {"x": (0, 3), "v": (1, 3), "density": (4, 1), "shear": (8, 9), "sigmay": (15, 1), "pl_strain": (13, 1)}
+
 EXPORTS = [
     "wf_abi_version", "wf_last_error", "wf_device_count", "wf_create", "wf_destroy",
     "wf_set_material", "wf_set_field", "wf_get_field", "wf_set_int_field",
@@ -81,7 +91,7 @@ EXPORTS = [
     "wf_get_reaction_history", "wf_rebuild_neighbours", "wf_get_neighbour_pairs",
     "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats",
     "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
-    "wf_get_rigid_field",
+    "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
 ]
 
 _lib = None
@@ -129,6 +139,10 @@ def lib():
         "wf_set_contact": ([vp, d, d, d], ctypes.c_int),
         "wf_get_contact_force_history": ([vp, i32, dp, i64, I64P], ctypes.c_int),
         "wf_get_rigid_field": ([vp, i32, dp], ctypes.c_int),
+        "wf_adj_forward": ([vp, ctypes.POINTER(AdjConfig), dp, dp, dp], ctypes.c_int),
+        "wf_adj_backward": ([vp, dp], ctypes.c_int),
+        "wf_adj_get_field": ([vp, i32, dp], ctypes.c_int),
+        "wf_adj_restore": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -282,6 +296,31 @@ class Solver:
         out = np.zeros((nrig, 3))
         self._check(lib().wf_get_rigid_field(self.ctx, RIGID_FIELDS[name], _dp(out)))
         return out
+
+    # adjoint window (FP64 Leapfrog): loss = sum_n (F_n - target_n)^2 over the
+    # plane contact force or zone reaction component F_n of each step
+    def adj_forward(self, target, checkpoint_every=20, kind=ADJ_PLANE, index=0, component=2, material=0):
+        t = np.ascontiguousarray(target, dtype=np.float64)
+        cfg = AdjConfig(steps=len(t), checkpoint_every=int(checkpoint_every), loss_kind=int(kind),
+                        loss_index=int(index), loss_component=int(component), material=int(material))
+        loss = ctypes.c_double(0)
+        hist = np.zeros(len(t))
+        self._check(lib().wf_adj_forward(self.ctx, ctypes.byref(cfg), _dp(t), ctypes.byref(loss), _dp(hist)))
+        return loss.value, hist
+
+    def adj_backward(self):
+        g = np.zeros(len(ADJ_PARAMS))
+        self._check(lib().wf_adj_backward(self.ctx, _dp(g)))
+        return g
+
+    def adj_field(self, name):
+        f, w = ADJ_FIELDS[name]
+        out = np.zeros((self.n, w)) if w > 1 else np.zeros(self.n)
+        self._check(lib().wf_adj_get_field(self.ctx, f, _dp(out)))
+        return out
+
+    def adj_restore(self):
+        self._check(lib().wf_adj_restore(self.ctx))
 
     def rebuild_neighbours(self):
         self._check(lib().wf_rebuild_neighbours(self.ctx))

@@ -82,7 +82,7 @@ struct Ctx {
   bool built = false;
   std::vector<int32_t> perm, inv;  // perm[internal] = original
   Dev d{};
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;  // (pointer, bytes) of the solver state
   DevMat *dmats = nullptr;
   int cur = 0;
   double dt = 0.0;  // the initial step (Domain::deltatint); the device flags hold the current one
@@ -111,7 +111,35 @@ struct Ctx {
     if (count == 0) count = 1;
     HIPCHK(hipMalloc(&p, count * sizeof(T)));
     HIPCHK(hipMemsetAsync(p, 0, count * sizeof(T), st));
-    allocs.push_back(p);
+    allocs.push_back({p, count * sizeof(T)});
+    return (T *)p;
+  }
+  // adjoint storage: checkpoints, tape, adjoint buffers (not part of the
+  // snapshots)
+  struct Adj {
+    wf_adj_config cfg{};
+    bool ready = false;       // a window's forward has run
+    long long step0 = 0;      // steps completed before the window
+    size_t snap_bytes = 0;
+    std::vector<char *> ckpt; // device snapshots of every state allocation
+    std::vector<std::pair<int, long long>> ckh;  // (cur, steps) at each snapshot
+    std::vector<double> hist, target;
+    double loss = 0.0;
+    int seg = 0;              // tape slots (checkpoint interval)
+    std::vector<AdjSlot> slot;  // seg + 1 state slots; intermediates and lists in the first seg
+    int *ver = nullptr;
+    AdjBufs buf{};
+    double *grad = nullptr;
+    bool have_grad = false;
+    std::vector<void *> mem;
+  } adj;
+  template <class T>
+  T *adj_alloc(size_t count) {
+    void *p = nullptr;
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc(&p, count * sizeof(T)));
+    HIPCHK(hipMemsetAsync(p, 0, count * sizeof(T), st));
+    adj.mem.push_back(p);
     return (T *)p;
   }
   void drop_graphs() {
@@ -125,7 +153,8 @@ struct Ctx {
   ~Ctx() {
     drop_graphs();
     for (auto &e : ev_pool) (void)hipEventDestroy(e);
-    for (void *p : allocs) (void)hipFree(p);
+    for (auto &p : allocs) (void)hipFree(p.first);
+    for (void *p : adj.mem) (void)hipFree(p);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -663,20 +692,49 @@ void check_flags(Ctx *c) {
   if (f.nan) throw Error(WF_ENAN, "non-finite value in the particle state");
 }
 
+// adjoint tape of a replayed step: slot `slot` receives the state at the
+// step's start and the step's intermediates (AdjSlot storage)
+struct TapeHook {
+  AdjSlot s;
+  int slot;
+  int force_lists;  // copy the lists even if the step does not search
+  int *ver;
+};
+
+void d2d(Ctx *c, void *dst, const void *src, size_t bytes) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+}
+
+// the state the next step starts from: x, v, rho, S, Sy, pl
+void tape_state(Ctx *c, const AdjSlot &s, int cur) {
+  const Dev &d = c->d;
+  const size_t np = (size_t)c->npad;
+  d2d(c, s.xh, d.xh[cur], np * sizeof(double4));
+  d2d(c, s.vm, d.vm, np * sizeof(double4));
+  d2d(c, s.rho, d.rho[cur], np * sizeof(double));
+  d2d(c, s.S, d.S, 6 * np * sizeof(double));
+  d2d(c, s.sigy, d.sigy, np * sizeof(double));
+  d2d(c, s.pl, d.pl, np * sizeof(double));
+}
+
 // one iteration of the Leapfrog / KickDrift loop, all on the stream
-void enqueue_step(Ctx *c, int scheme, int cur) {
+void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr) {
   Dev &d = c->d;
   Timed whole(c, T_STEP);
+  const size_t np = (size_t)c->npad;
+  if (tp) tape_state(c, tp->s, cur);
   launch_schedule(d, c->st, 0);
   {
     Timed t(c, T_NBR);
     launch_neighbours(d, c->st, cur);
     if (d.contact) launch_surface(d, c->st, cur, 0);
   }
+  if (tp) launch_tape_lists(d, c->st, tp->s, tp->force_lists, tp->ver, tp->slot);
   {
     Timed t(c, T_ACCEL);
     launch_accel(d, c->st, cur);
   }
+  if (tp && d.contact) d2d(c, tp->s.apre, d.acc, np * sizeof(double4));
   // contact acts after AccelReduction and the first GeneralAfter
   // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
   // so no BC zone applies to them
@@ -685,6 +743,11 @@ void enqueue_step(Ctx *c, int scheme, int cur) {
   // (Source/SolverLeapfrog.cpp:241-262, Source/SolverKickDrift.cpp:208-226)
   const double dt = d.auto_ts ? -1.0 : c->dt;
   if (d.contact) launch_contact(d, c->st, c->bc, cur, dt);
+  if (tp && d.contact) {
+    d2d(c, tp->s.node, d.node, (size_t)d.nnode * sizeof(double4));
+    d2d(c, tp->s.enorm, d.enorm, (size_t)d.nrig * sizeof(double4));
+    d2d(c, tp->s.rv, d.rv, (size_t)d.nrig * sizeof(double4));
+  }
   if (d.auto_ts) {
     launch_kick(d, c->st, c->bc, scheme, dt, dt, 1);
     launch_time_step(d, c->st, cur);
@@ -707,6 +770,11 @@ void enqueue_step(Ctx *c, int scheme, int cur) {
   if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, dt);
   launch_finalize(d, c->st, dt, c->bc.nz);
   if (d.contact) launch_mesh_update(d, c->st, dt);
+  if (tp) {
+    d2d(c, tp->s.sr, d.sr, 6 * np * sizeof(double));
+    d2d(c, tp->s.rr, d.rr, 3 * np * sizeof(double));
+    d2d(c, tp->s.drho, d.drho, np * sizeof(double));
+  }
 }
 
 void one_step(Ctx *c, int scheme) {
@@ -729,6 +797,105 @@ void one_step(Ctx *c, int scheme) {
   c->built = true;
   c->cur ^= 1;
   c->steps += 1;
+}
+
+// ---------------------------------------------------------------------------
+// adjoint: snapshots of the whole device state, the tape of a replayed
+// segment and the reverse sweep
+
+void adj_check(Ctx *c) {
+  if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+  if (c->cfg.precision != WF_FP64) throw Error(WF_EUNSUPPORTED, "the adjoint needs precision WF_FP64");
+  if (c->cfg.auto_ts) throw Error(WF_EUNSUPPORTED, "the adjoint needs a fixed time step (auto_ts 0)");
+  if (c->cfg.thermal_solver) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover the thermal solver");
+}
+
+inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+void snapshot(Ctx *c, size_t k) {
+  char *dst = c->adj.ckpt[k];
+  size_t off = 0;
+  for (auto &a : c->allocs) {
+    d2d(c, dst + off, a.first, a.second);
+    off += round256(a.second);
+  }
+  c->adj.ckh[k] = {c->cur, c->steps};
+}
+
+void restore(Ctx *c, size_t k) {
+  const char *src = c->adj.ckpt[k];
+  size_t off = 0;
+  for (auto &a : c->allocs) {
+    d2d(c, a.first, src + off, a.second);
+    off += round256(a.second);
+  }
+  c->cur = c->adj.ckh[k].first;
+  c->steps = c->adj.ckh[k].second;
+}
+
+AdjSlot alloc_slot(Ctx *c, bool inter) {
+  const Dev &d = c->d;
+  const size_t np = (size_t)c->npad;
+  AdjSlot s{};
+  s.xh = c->adj_alloc<double4>(np);
+  s.vm = c->adj_alloc<double4>(np);
+  s.rho = c->adj_alloc<double>(np);
+  s.S = c->adj_alloc<double>(6 * np);
+  s.sigy = c->adj_alloc<double>(np);
+  s.pl = c->adj_alloc<double>(np);
+  if (!inter) return s;
+  s.apre = c->adj_alloc<double4>(d.contact ? np : 1);
+  s.drho = c->adj_alloc<double>(np);
+  s.sr = c->adj_alloc<double>(6 * np);
+  s.rr = c->adj_alloc<double>(3 * np);
+  s.nbr = c->adj_alloc<int>((size_t)nb_rows(c->cfg.max_nb) * np);
+  s.nnb = c->adj_alloc<int>(np);
+  s.id = c->adj_alloc<int>(np);
+  s.cidx = c->adj_alloc<int>(d.contact ? (size_t)kMaxContact * np : 1);
+  s.ccnt = c->adj_alloc<int>(d.contact ? np : 1);
+  s.node = c->adj_alloc<double4>(d.contact ? (size_t)d.nnode : 1);
+  s.enorm = c->adj_alloc<double4>(d.contact ? (size_t)d.nrig : 1);
+  s.rv = c->adj_alloc<double4>(d.contact ? (size_t)d.nrig : 1);
+  return s;
+}
+
+void adj_prepare(Ctx *c, int k, size_t ncp) {
+  auto &A = c->adj;
+  size_t bytes = 0;
+  for (auto &a : c->allocs) bytes += round256(a.second);
+  A.snap_bytes = bytes;
+  while (A.ckpt.size() < ncp) A.ckpt.push_back(c->adj_alloc<char>(bytes));
+  A.ckh.resize(A.ckpt.size());
+  if (A.seg != k) {
+    A.slot.clear();
+    for (int s = 0; s <= k; ++s) A.slot.push_back(alloc_slot(c, s < k));
+    A.ver = c->adj_alloc<int>((size_t)k);
+    A.seg = k;
+  }
+  if (!A.buf.xb) {
+    const size_t np = (size_t)c->npad;
+    AdjBufs &b = A.buf;
+    b.xb = c->adj_alloc<double4>(np);
+    b.vb = c->adj_alloc<double4>(np);
+    b.ab = c->adj_alloc<double4>(np);
+    b.vpb = c->adj_alloc<double4>(np);
+    b.rhob = c->adj_alloc<double>(np);
+    b.Sb = c->adj_alloc<double>(6 * np);
+    b.syb = c->adj_alloc<double>(np);
+    b.plb = c->adj_alloc<double>(np);
+    b.srb = c->adj_alloc<double>(6 * np);
+    b.wb = c->adj_alloc<double>(3 * np);
+    b.db = c->adj_alloc<double>(np);
+    b.thb = c->adj_alloc<double>((size_t)kAdjParams * np);
+    b.part = c->adj_alloc<double>((size_t)kAdjParams * c->d.nblocks);
+    A.grad = c->adj_alloc<double>(kAdjParams);
+  }
+}
+
+int zone_slot(Ctx *c, int zone) {
+  for (size_t z = 0; z < c->rzones.size(); ++z)
+    if (c->rzones[z] == zone) return (int)z;
+  return -1;
 }
 
 }  // namespace
@@ -1309,11 +1476,17 @@ int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double d
   Ctx *c = (Ctx *)ctx;
   return guard(c, [&] {
     as_ctx(ctx);
-    if (c->inited) throw Error(WF_ESTATE, "contact parameters must be set before wf_init");
     if (!(friction_sta >= 0) || !(friction_dyn >= 0) || !(damping >= 0)) throw Error(WF_EINVAL, "negative contact parameter");
     c->fr_sta = friction_sta;
     c->fr_dyn = friction_dyn;
     c->dfac = damping;
+    if (c->inited) {
+      // the kernels take the parameters by value: recapture the step graphs
+      c->d.fr_sta = friction_sta;
+      c->d.fr_dyn = friction_dyn;
+      c->d.dfac = damping;
+      c->drop_graphs();
+    }
     return WF_OK;
   });
 }
@@ -1333,6 +1506,176 @@ int wf_get_contact_force_history(void *ctx, int32_t mesh, double *out3, int64_t 
       for (int k = 0; k < 3; ++k) out3[3 * s + k] = h[(slot * kMaxMesh + mesh) * 3 + k];
     }
     if (count) *count = m;
+    return WF_OK;
+  });
+}
+
+int wf_adj_forward(void *ctx, const wf_adj_config *cfg, const double *target, double *loss, double *history) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    adj_check(c);
+    if (!cfg || !target || !loss) throw Error(WF_EINVAL, "null argument");
+    const Dev &d = c->d;
+    if (cfg->steps < 1 || cfg->steps > d.hist_cap) throw Error(WF_EINVAL, "window steps must be in 1..65536");
+    if (cfg->checkpoint_every < 1) throw Error(WF_EINVAL, "checkpoint_every must be >= 1");
+    if (cfg->loss_component < 0 || cfg->loss_component > 2) throw Error(WF_EINVAL, "loss_component must be 0..2");
+    if (cfg->material < 0 || cfg->material >= kMaxMat) throw Error(WF_EINVAL, "material index out of range");
+    if (cfg->loss_kind == WF_ADJ_PLANE) {
+      if (cfg->loss_index < 0 || cfg->loss_index >= d.nmesh) throw Error(WF_EINVAL, "no such rigid plane");
+    } else if (cfg->loss_kind == WF_ADJ_ZONE) {
+      refresh_bc(c);
+      if (zone_slot(c, cfg->loss_index) < 0) throw Error(WF_EINVAL, "no velocity BC on that zone");
+    } else {
+      throw Error(WF_EINVAL, "unknown loss kind");
+    }
+    HIPCHK(hipSetDevice(c->cfg.device));
+    auto &A = c->adj;
+    const int k = cfg->checkpoint_every;
+    const long long steps = cfg->steps;
+    const long long nseg = (steps + k - 1) / k;
+    adj_prepare(c, k, (size_t)nseg + 1);
+    A.cfg = *cfg;
+    A.ready = false;
+    A.have_grad = false;
+    A.step0 = c->steps;
+    refresh_bc(c);
+    snapshot(c, 0);
+    for (long long s = 0; s < steps; ++s) {
+      one_step(c, WF_LEAPFROG);
+      if ((s + 1) % k == 0 || s + 1 == steps) snapshot(c, (size_t)((s + 1 + k - 1) / k));
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    check_flags(c);
+    A.hist.assign((size_t)steps, 0.0);
+    A.target.assign(target, target + steps);
+    if (cfg->loss_kind == WF_ADJ_PLANE) {
+      auto h = dget(c, d.cf_hist, (size_t)d.hist_cap * kMaxMesh * 3);
+      for (long long n = 0; n < steps; ++n) {
+        const long long slot = (A.step0 + n) % d.hist_cap;
+        A.hist[n] = h[(size_t)slot * kMaxMesh * 3 + cfg->loss_index * 3 + cfg->loss_component];
+      }
+    } else {
+      const int z = zone_slot(c, cfg->loss_index);
+      auto h = dget(c, d.react_hist, (size_t)d.hist_cap * kMaxZones * 3);
+      for (long long n = 0; n < steps; ++n) {
+        const long long slot = (A.step0 + n) % d.hist_cap;
+        A.hist[n] = h[(size_t)slot * kMaxZones * 3 + z * 3 + cfg->loss_component];
+      }
+    }
+    double L = 0.0;
+    for (long long n = 0; n < steps; ++n) L += (A.hist[n] - A.target[n]) * (A.hist[n] - A.target[n]);
+    A.loss = L;
+    *loss = L;
+    if (history) std::memcpy(history, A.hist.data(), (size_t)steps * sizeof(double));
+    A.ready = true;
+    return WF_OK;
+  });
+}
+
+int wf_adj_backward(void *ctx, double *grad) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    adj_check(c);
+    if (!grad) throw Error(WF_EINVAL, "null argument");
+    auto &A = c->adj;
+    if (!A.ready) throw Error(WF_ESTATE, "wf_adj_forward has not run (or the state was restored since)");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    Dev &d = c->d;
+    const int k = A.cfg.checkpoint_every;
+    const long long steps = A.cfg.steps;
+    const long long nseg = (steps + k - 1) / k;
+    adj_prepare(c, k, (size_t)nseg + 1);
+    const size_t np = (size_t)c->npad;
+    AdjBufs &b = A.buf;
+    HIPCHK(hipMemsetAsync(b.xb, 0, np * sizeof(double4), c->st));
+    HIPCHK(hipMemsetAsync(b.vb, 0, np * sizeof(double4), c->st));
+    HIPCHK(hipMemsetAsync(b.rhob, 0, np * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(b.Sb, 0, 6 * np * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(b.syb, 0, np * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(b.plb, 0, np * sizeof(double), c->st));
+    HIPCHK(hipMemsetAsync(b.thb, 0, (size_t)kAdjParams * np * sizeof(double), c->st));
+    refresh_bc(c);
+    const int zslot = A.cfg.loss_kind == WF_ADJ_ZONE ? zone_slot(c, A.cfg.loss_index) : -1;
+    for (long long sg = nseg - 1; sg >= 0; --sg) {
+      restore(c, (size_t)sg);
+      const long long w0 = sg * k;
+      const int len = (int)std::min<long long>(k, steps - w0);
+      HIPCHK(hipMemsetAsync(A.ver, 0, (size_t)k * sizeof(int), c->st));
+      // replay the segment with the tape
+      for (int s = 0; s < len; ++s) {
+        TapeHook hk{A.slot[s], s, s == 0 ? 1 : 0, A.ver};
+        enqueue_step(c, WF_LEAPFROG, c->cur, &hk);
+        HIPCHK(hipGetLastError());
+        c->cur ^= 1;
+        c->steps += 1;
+      }
+      tape_state(c, A.slot[len], c->cur);
+      const auto ver = dget(c, A.ver, (size_t)len);
+      // reverse sweep over the segment
+      for (int s = len - 1; s >= 0; --s) {
+        AdjSlot s0 = A.slot[s];
+        int v = s;
+        while (v > 0 && !ver[v]) --v;
+        s0.nbr = A.slot[v].nbr;
+        s0.nnb = A.slot[v].nnb;
+        s0.id = A.slot[v].id;
+        s0.cidx = A.slot[v].cidx;
+        s0.ccnt = A.slot[v].ccnt;
+        const long long n = w0 + s;
+        AdjSeeds seeds{};
+        const double g = 2.0 * (A.hist[n] - A.target[n]);
+        if (A.cfg.loss_kind == WF_ADJ_PLANE) seeds.cf[A.cfg.loss_index][A.cfg.loss_component] = g;
+        else seeds.react[zslot][A.cfg.loss_component] = g;
+        const double kick = (A.step0 + n == 0) ? 0.5 * c->dt : c->dt;
+        launch_adj_step(d, c->st, s0, A.slot[s + 1], b, c->bc, seeds, c->dt, kick, A.cfg.material);
+      }
+      HIPCHK(hipGetLastError());
+    }
+    launch_adj_reduce(d, c->st, b, A.grad);
+    auto gh = dget(c, A.grad, (size_t)kAdjParams);
+    for (int p = 0; p < kAdjParams; ++p) grad[p] = gh[p];
+    restore(c, (size_t)nseg);
+    HIPCHK(hipStreamSynchronize(c->st));
+    check_flags(c);
+    A.have_grad = true;
+    return WF_OK;
+  });
+}
+
+int wf_adj_get_field(void *ctx, int32_t field, double *h) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!h) throw Error(WF_EINVAL, "null argument");
+    auto &A = c->adj;
+    if (!A.have_grad) throw Error(WF_ESTATE, "wf_adj_backward has not run");
+    switch (field) {
+      case WF_F_X: get_d4(c, A.buf.xb, h, 0, 3, 3); break;
+      case WF_F_V: get_d4(c, A.buf.vb, h, 0, 3, 3); break;
+      case WF_F_DENSITY: get_scalar(c, A.buf.rhob, h); break;
+      case WF_F_SHEAR: get_sym(c, A.buf.Sb, h); break;
+      case WF_F_SIGMAY: get_scalar(c, A.buf.syb, h); break;
+      case WF_F_PL_STRAIN: get_scalar(c, A.buf.plb, h); break;
+      default: throw Error(WF_EINVAL, "no adjoint for that field");
+    }
+    return WF_OK;
+  });
+}
+
+int wf_adj_restore(void *ctx) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    auto &A = c->adj;
+    if (A.ckpt.empty() || !(A.ready || A.have_grad)) throw Error(WF_ESTATE, "no adjoint window to restore");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    restore(c, 0);
+    HIPCHK(hipStreamSynchronize(c->st));
+    A.ready = false;
+    A.have_grad = false;
     return WF_OK;
   });
 }

@@ -213,6 +213,38 @@ struct Dev {
   long long hist_cap;
 };
 
+// ---------------------------------------------------------------------------
+// adjoint of the FP64 Leapfrog step (wf_adjoint.hip).  A segment of steps is
+// replayed from a checkpoint with a tape: slot s holds the state at the
+// start of step s of the segment and that step's intermediates; the list
+// pointers name the neighbour / contact lists in effect for the step.
+constexpr int kAdjParams = 7;  // A, B, n, C, m (Johnson-Cook), friction_sta, friction_dyn
+struct AdjSlot {
+  double4 *xh, *vm;            // (x, h), (v, m) at the step's start
+  double *rho, *S, *sigy, *pl; // rho, S [6][npad], Sy, pl at the step's start
+  double4 *apre;               // acceleration after the pair pass (before contact)
+  double *drho, *sr, *rr;      // density rate, strain rate [6], rotation rate [3]
+  int *nbr, *nnb;              // neighbour list in effect
+  int *cidx, *ccnt, *id;       // contact partners and particle ids in effect
+  double4 *node, *enorm, *rv;  // rigid planes at the step's contact evaluation
+};
+struct AdjBufs {
+  double4 *xb, *vb;            // adjoint of the state x, v
+  double4 *ab, *vpb;           // adjoint of the acceleration and of the kicked velocity
+  double *rhob, *Sb, *syb, *plb;
+  double *srb, *wb, *db;       // adjoint of strain rate [6], rotation rate [3], density-rate sum
+  double *thb;                 // per-particle parameter adjoints [kAdjParams][npad]
+  double *part;                // [kAdjParams][nblocks] block partials
+};
+struct AdjSeeds {
+  double cf[kMaxMesh][3];      // d loss / d plane contact force of the step
+  double react[kMaxZones][3];  // d loss / d zone reaction of the step
+};
+void launch_adj_step(const Dev &d, hipStream_t st, const AdjSlot &s0, const AdjSlot &s1, const AdjBufs &b,
+                     const BcTable &bc, const AdjSeeds &seeds, double dt, double kick, int adj_mat);
+void launch_adj_reduce(const Dev &d, hipStream_t st, const AdjBufs &b, double *out);
+void launch_tape_lists(const Dev &d, hipStream_t st, const AdjSlot &s, int force, int *ver, int slot);
+
 // launchers (wf_kernels.hip); all enqueue on `st`
 void launch_schedule(const Dev &d, hipStream_t st, int force);
 void launch_neighbours(const Dev &d, hipStream_t st, int cur);
