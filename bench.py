@@ -129,6 +129,49 @@ def contact_run(steps, warmup=10):
     return res
 
 
+def adjoint_run(window, pre=100, k=20):
+    """Adjoint calibration window (weldform_amd.cases.config5: the config-3
+    billet and punches at 200,704 particles, FP64, isothermal): after `pre`
+    steps, a `window`-step Leapfrog window with a snapshot every k steps, loss
+    = sum_n (F_z,n - target_n)^2 over the punch plane's contact force; one
+    gradient = wf_adj_forward + wf_adj_backward (dL/d(A, B, n, C, m, mu))."""
+    import numpy as np
+
+    from weldform_amd import capi, cases
+
+    case = cases.config5(steps=window)
+    log(f"bench: {case.name} n={case.n} window={window} checkpoint_every={k}")
+    s = cases.make_solver(case, precision=capi.FP64)
+    s.step(pre)
+    s.sync()
+    _, hist = s.adj_forward(np.zeros(window), checkpoint_every=k, kind=capi.ADJ_PLANE, index=0, component=2)
+    s.adj_restore()
+    target = 0.98 * hist
+    t0 = time.perf_counter()
+    loss, _ = s.adj_forward(target, checkpoint_every=k, kind=capi.ADJ_PLANE, index=0, component=2)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g = s.adj_backward()
+    t_bwd = time.perf_counter() - t0
+    s.adj_restore()
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(window)
+    s.sync()
+    t_plain = time.perf_counter() - t0
+    res = {"workload": f"{case.name}: R=15mm L=30mm dx=0.4675mm JC steel, isothermal, planes -5 m/s / fixed, "
+                       f"mu=0.2; window of {window} FP64 Leapfrog steps after {pre}, snapshot every {k}",
+           "particles": case.n, "window_steps": window, "checkpoint_every": k,
+           "gradients_per_s": 1.0 / (t_fwd + t_bwd),
+           "forward_window_s": t_fwd, "backward_s": t_bwd, "plain_forward_s": t_plain,
+           "backward_over_forward": t_bwd / t_plain,
+           "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
+           "loss": loss, "grad": dict(zip(capi.ADJ_PARAMS, [float(x) for x in g]))}
+    log("bench: adjoint " + json.dumps(res))
+    s.close()
+    return res
+
+
 def cpu_baseline(case, steps):
     sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
     from oracle_case import oracle_from_case
@@ -156,6 +199,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config3-steps", type=int, default=400,
                     help="steps of the thermo-mechanical contact case reported beside the headline (0: skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=400,
+                    help="window of the adjoint calibration leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -263,6 +308,8 @@ def main():
     }
     if args.config3_steps > 0:
         out["config"]["config3_contact_thermal"] = contact_run(args.config3_steps)
+    if args.adjoint_steps > 0:
+        out["config"]["config5_adjoint"] = adjoint_run(args.adjoint_steps)
     if not args.no_cpu:
         ccase = cases.config2(steps=args.cpu_steps)
         log(f"bench: cpu baseline, {args.cpu_steps} steps")

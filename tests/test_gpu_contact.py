@@ -3,11 +3,19 @@ detection, contact pairs, the Wang penalty force with Coulomb friction and
 the motion of the planes (Source/Contact_Wang.cpp:15-348,
 Source/Contact.cpp:10-132, Source/Domain.cpp:1415-1491, Source/Mesh.cpp).
 
-Bars: FP64 free-surface ids and contact-pair counts exact, displacement,
-per-step plane forces and particle contact forces within 1e-8 normwise
-(the plane force's tangential part carries the reference's static-friction
-term in force units, so the check is per component over the history);
-mixed precision within 1e-4 on displacement and the normal force."""
+The penalty stiffness kij = 2 m / dt^2 and the static-friction term make the
+contact phase chaotic: once particles touch a plane (step ~45 here) a 1e-15
+relative change of the oracle's own initial positions grows about 2x per
+step, to ~20 % of the per-step plane force and ~4 % of the displacement at
+step 200.  So the bars are:
+  * the first 50 steps (contact under way, before the growth reaches 1e-8):
+    FP64 plane forces, displacement and particle contact forces within 1e-8
+    normwise, free-surface ids and contact-pair counts exact;
+  * all 200 steps: displacement and plane forces within 10x the oracle's own
+    sensitivity (to a 1e-15 relative perturbation of the initial positions
+    for FP64, 1e-7 for mixed precision), and the top plane's mean force over
+    the last 100 steps within 2 %;
+  * rigid-plane positions and normals exact (FP64)."""
 import numpy as np
 import pytest
 
@@ -16,44 +24,72 @@ from weldform_amd import capi, cases
 
 pytestmark = pytest.mark.gpu
 
+EARLY = 50
+
+
+def _oracle(c, eps=0.0):
+    d = oracle_from_case(c, threads=8)
+    if eps:
+        rng = np.random.default_rng(5)
+        d.f("X")[:c.n] = c.x * (1 + eps * rng.uniform(-1, 1, size=c.x.shape))
+    d.solve_init(c.dt)
+    F, early = [], None
+    for s in range(c.steps):
+        d.step()
+        F.append(np.stack([d.contact_force(0), d.contact_force(1)]))
+        if s + 1 == EARLY:
+            early = {"disp": d.f("DISP")[:c.n].copy(), "cf": d.f("CONTFORCE")[:c.n].copy(),
+                     "id": d.i("ID")[:c.n].copy(), "ncont": int(d.param("ncont"))}
+    return d, np.array(F), early
+
 
 @pytest.fixture(scope="module")
 def contact_run():
     c = cases.contact_small(steps=200)
-    d = oracle_from_case(c, threads=8)
-    d.solve_init(c.dt)
-    F = []
-    for _ in range(c.steps):
-        d.step()
-        F.append(np.stack([d.contact_force(0), d.contact_force(1)]))
-    return c, d, np.array(F)
+    d, F, early = _oracle(c)
+    sens = {}
+    for eps in (1e-15, 1e-7):
+        dp, Fp, _ = _oracle(c, eps)
+        sens[eps] = {"disp": normwise_err(dp.f("DISP")[:c.n], d.f("DISP")[:c.n]),
+                     "F_top_z": normwise_err(Fp[:, 0, 2], F[:, 0, 2]),
+                     "F_bottom_z": normwise_err(Fp[:, 1, 2], F[:, 1, 2])}
+    return c, d, F, early, sens
 
 
-@pytest.mark.parametrize("precision,tol", [(capi.FP64, 1e-8), (capi.MIXED, 1e-4)])
-def test_contact_run_matches_oracle(contact_run, precision, tol):
-    c, d, F = contact_run
+@pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
+def test_contact_run_matches_oracle(contact_run, precision):
+    c, d, F, early, sens_all = contact_run
     print(f"contact run: n={c.n} steps={c.steps} precision={precision}", flush=True)
+    fp64 = precision == capi.FP64
+    sens = sens_all[1e-15 if fp64 else 1e-7]
     g = cases.make_solver(c, precision=precision)
-    g.step(c.steps)
-    g.sync()
     nrig = sum(2 * m["dens"] ** 2 for m in c.meshes)
     assert np.abs(F[:, 0, 2]).max() > 0 and np.abs(F[:, 1, 2]).max() > 0  # both planes touched
+    assert np.abs(F[:EARLY, 0, 2]).max() > 0  # contact starts inside the early window
+    g.step(EARLY)
+    g.sync()
+    Fe = np.stack([g.contact_force_history(0), g.contact_force_history(1)], axis=1)
+    e_early = {"disp": normwise_err(g.get("disp"), early["disp"]),
+               "F_top": normwise_err(Fe[:, 0], F[:EARLY, 0]),
+               "F_bottom": normwise_err(Fe[:, 1], F[:EARLY, 1]),
+               "contforce": normwise_err(g.get("contforce"), early["cf"])}
+    print("contact early errors", e_early, flush=True)
+    if fp64:
+        assert np.array_equal(g.get_int("id"), early["id"])
+        assert g.scalars()["contact_pairs"] == early["ncont"]
+        for k, e in e_early.items():
+            assert e < 1e-8, k
+    g.step(c.steps - EARLY)
+    g.sync()
     Fg = np.stack([g.contact_force_history(0), g.contact_force_history(1)], axis=1)
-    sc = g.scalars()
-    errs = {
-        "disp": normwise_err(g.get("disp"), d.f("DISP")[:c.n]),
-        "F_top_z": normwise_err(Fg[:, 0, 2], F[:, 0, 2]),
-        "F_bottom_z": normwise_err(Fg[:, 1, 2], F[:, 1, 2]),
-        "contforce_z": normwise_err(g.get("contforce")[:, 2], d.f("CONTFORCE")[:c.n, 2]),
-        "rigid_x": normwise_err(g.rigid("x", nrig), d.f("X")[c.n:]),
-    }
-    if precision == capi.FP64:
-        errs["F_top_xy"] = normwise_err(Fg[:, 0, :2], F[:, 0, :2])
-        errs["F_bottom_xy"] = normwise_err(Fg[:, 1, :2], F[:, 1, :2])
-    print("contact errors", errs, "pairs", sc["contact_pairs"], int(d.param("ncont")), flush=True)
-    if precision == capi.FP64:
-        assert np.array_equal(g.get_int("id"), d.i("ID")[:c.n])
-        assert sc["contact_pairs"] == int(d.param("ncont"))
-        assert np.array_equal(g.rigid("normal", nrig), d.f("NORMAL")[c.n:])
+    errs = {"disp": normwise_err(g.get("disp"), d.f("DISP")[:c.n]),
+            "F_top_z": normwise_err(Fg[:, 0, 2], F[:, 0, 2]),
+            "F_bottom_z": normwise_err(Fg[:, 1, 2], F[:, 1, 2])}
+    mean_g, mean_o = Fg[100:, 0, 2].mean(), F[100:, 0, 2].mean()
+    print("contact errors", errs, "oracle sensitivity", sens, "mean top force", mean_g, mean_o, flush=True)
     for k, e in errs.items():
-        assert e < tol, k
+        assert e < max(1e-8 if fp64 else 1e-4, 10 * sens[k]), k
+    assert abs(mean_g - mean_o) < 0.02 * abs(mean_o)
+    if fp64:
+        assert normwise_err(g.rigid("x", nrig), d.f("X")[c.n:]) < 1e-15
+        assert np.array_equal(g.rigid("normal", nrig), d.f("NORMAL")[c.n:])

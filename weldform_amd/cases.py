@@ -214,6 +214,18 @@ def config3(steps=2000, dx=0.3757e-3, R=15e-3, L=30e-3, dens=32, v_punch=-5.0, m
     return c
 
 
+def config5(steps=400, dx=0.4675e-3, R=15e-3, L=30e-3, dens=32, v_punch=-5.0, mu=0.2, seed=2029):
+    """Adjoint calibration case: the config-3 billet and punches at ~200k
+    particles (dx = 0.4675 mm), FP64, isothermal (T = 300 K: the adjoint
+    covers the mechanical step and the contact), Johnson-Cook steel,
+    friction mu = 0.2; the window is `steps` Leapfrog steps."""
+    c = config3(steps=steps, dx=dx, R=R, L=L, dens=dens, v_punch=v_punch, mu=mu, seed=seed)
+    c.name = "cyl_compress_200k_contact_adjoint"
+    c.thermal = False
+    c.pl_heating = False
+    return c
+
+
 def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,
