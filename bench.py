@@ -129,26 +129,30 @@ def contact_run(steps, warmup=10):
     return res
 
 
-def adjoint_run(window, pre=100, k=20):
-    """Adjoint calibration window (weldform_amd.cases.config5: the config-3
-    billet and punches at 200,704 particles, FP64, isothermal): after `pre`
-    steps, a `window`-step Leapfrog window with a snapshot every k steps, loss
-    = sum_n (F_z,n - target_n)^2 over the punch plane's contact force; one
-    gradient = wf_adj_forward + wf_adj_backward (dL/d(A, B, n, C, m, mu))."""
+def adjoint_run(window, pre=100, k=20, contact=True, fd=False):
+    """Adjoint calibration window at 200,704 particles (weldform_amd.cases:
+    config5, the config-3 billet and punches, FP64, isothermal; or
+    config5_zone, the same billet driven through velocity BCs).  After `pre`
+    steps, a `window`-step Leapfrog window with a snapshot every k steps; loss
+    = sum_n (F_z,n - target_n)^2 over the punch plane's contact force (or
+    the top zone's reaction), target = 0.98 x the window's own history; one
+    gradient = wf_adj_forward + wf_adj_backward (dL/d(A, B, n, C, m, mu)).
+    fd: central differences (1e-4 relative) of the five JC parameters."""
     import numpy as np
 
     from weldform_amd import capi, cases
 
-    case = cases.config5(steps=window)
+    case = cases.config5(steps=window) if contact else cases.config5_zone(steps=window)
+    kind, index = (capi.ADJ_PLANE, 0) if contact else (capi.ADJ_ZONE, case.top_zone)
     log(f"bench: {case.name} n={case.n} window={window} checkpoint_every={k}")
     s = cases.make_solver(case, precision=capi.FP64)
     s.step(pre)
     s.sync()
-    _, hist = s.adj_forward(np.zeros(window), checkpoint_every=k, kind=capi.ADJ_PLANE, index=0, component=2)
+    _, hist = s.adj_forward(np.zeros(window), checkpoint_every=k, kind=kind, index=index, component=2)
     s.adj_restore()
     target = 0.98 * hist
     t0 = time.perf_counter()
-    loss, _ = s.adj_forward(target, checkpoint_every=k, kind=capi.ADJ_PLANE, index=0, component=2)
+    loss, _ = s.adj_forward(target, checkpoint_every=k, kind=kind, index=index, component=2)
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
     g = s.adj_backward()
@@ -159,16 +163,89 @@ def adjoint_run(window, pre=100, k=20):
     s.step(window)
     s.sync()
     t_plain = time.perf_counter() - t0
-    res = {"workload": f"{case.name}: R=15mm L=30mm dx=0.4675mm JC steel, isothermal, planes -5 m/s / fixed, "
-                       f"mu=0.2; window of {window} FP64 Leapfrog steps after {pre}, snapshot every {k}",
+    res = {"workload": f"{case.name}: R=15mm L=30mm dx=0.4675mm JC steel, isothermal, "
+                       + ("planes -5 m/s / fixed, mu=0.2, loss on the top plane's contact force" if contact else
+                          "top row -5 m/s, bottom row fixed, loss on the top zone's reaction")
+                       + f"; window of {window} FP64 Leapfrog steps after {pre}, snapshot every {k}",
            "particles": case.n, "window_steps": window, "checkpoint_every": k,
            "gradients_per_s": 1.0 / (t_fwd + t_bwd),
            "forward_window_s": t_fwd, "backward_s": t_bwd, "plain_forward_s": t_plain,
            "backward_over_forward": t_bwd / t_plain,
            "gradient_over_forward": (t_fwd + t_bwd) / t_plain,
            "loss": loss, "grad": dict(zip(capi.ADJ_PARAMS, [float(x) for x in g]))}
+    if fd:
+        mat0 = dict(case.materials[0])
+        rel = {}
+        for p, key in enumerate(("jc_A", "jc_B", "jc_n", "jc_C", "jc_m")):
+            h = 1e-4 * abs(mat0[key])
+            L = []
+            for sgn in (1, -1):
+                s.adj_restore()
+                s.set_material(0, **dict(mat0, **{key: mat0[key] + sgn * h}))
+                L.append(s.adj_forward(target, checkpoint_every=k, kind=kind, index=index, component=2)[0])
+            s.set_material(0, **mat0)
+            fdv = (L[0] - L[1]) / (2 * h)
+            rel[key] = {"fd": fdv, "adjoint": float(g[p]), "rel_err": abs(g[p] - fdv) / abs(fdv) if fdv else None}
+        res["fd_check"] = rel
     log("bench: adjoint " + json.dumps(res))
     s.close()
+    return res
+
+
+def batch_run(steps, n_sample=16):
+    """Batched ensemble (weldform_amd.cases.config4: 256 cylinder specimens,
+    N_i log-uniform on [2k, 40k], JC parameters +-20 %, 2.7M particles) in
+    one context: shape chosen by the autotuner, then `steps` mixed-precision
+    Leapfrog steps replayed from one captured graph per step.  Per-specimen
+    step times: a size-stratified sample of the specimens run alone."""
+    import numpy as np
+
+    from weldform_amd import autotune, capi, cases
+
+    case = cases.config4(steps=steps)
+    log(f"bench: {case.name} n={case.n} steps={steps}")
+    best, table = autotune.autotune(case, precision=capi.MIXED, steps=10, warmup=3, lanes=(1, 2, 4),
+                                    stages=(-1, 1024), log=log)
+    w = cases.make_solver(case, precision=capi.MIXED, **best)
+    w.step(5)
+    w.sync()
+    w.close()
+    s = cases.make_solver(case, precision=capi.MIXED, **best)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    s.close()
+    spec_n = case.meta["spec_n"]
+    order = np.argsort(spec_n)
+    pick = order[np.linspace(0, len(order) - 1, n_sample).round().astype(int)]
+    lone = []
+    for i in pick:
+        sc = cases.specimen(case, int(i))
+        g = cases.make_solver(sc, precision=capi.MIXED, **best)
+        g.step(5)
+        g.sync()
+        t1 = time.perf_counter()
+        g.step(50)
+        g.sync()
+        lone.append((int(sc.n), 1e3 * (time.perf_counter() - t1) / 50))
+        g.close()
+    ln = np.array([x[0] for x in lone], dtype=float)
+    lt = np.array([x[1] for x in lone])
+    # serial estimate: every specimen alone, step time interpolated in its size
+    serial = float(np.interp(spec_n, ln, lt).sum())
+    ms = 1e3 * wall / steps
+    res = {"workload": f"{case.name}: 256 cylinders, N_i log-uniform [2k, 40k], R/L uniform 0.2-0.5, "
+                       f"dx=1mm, JC +-20 %, top row -5 m/s, {steps} mixed Leapfrog steps, one context",
+           "particles": case.n, "specimens": len(spec_n), "steps": steps, "ms_per_step": ms,
+           "particle_steps_per_s": case.n * steps / wall,
+           "autotune": {"best": best, "table": table},
+           "per_specimen_step_ms_batched": {"p50": ms, "p99": ms},
+           "per_specimen_step_ms_alone": {"p50": float(np.percentile(lt, 50)), "p99": float(np.percentile(lt, 99)),
+                                          "sample": lone},
+           "serial_ms_per_step_estimate": serial, "batched_speedup_vs_serial": serial / ms}
+    log("bench: batch " + json.dumps(res))
     return res
 
 
@@ -199,6 +276,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config3-steps", type=int, default=400,
                     help="steps of the thermo-mechanical contact case reported beside the headline (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=500,
+                    help="steps of the batched-ensemble leg (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="window of the adjoint calibration leg (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -308,8 +387,11 @@ def main():
     }
     if args.config3_steps > 0:
         out["config"]["config3_contact_thermal"] = contact_run(args.config3_steps)
+    if args.batch_steps > 0:
+        out["config"]["config4_batch"] = batch_run(args.batch_steps)
     if args.adjoint_steps > 0:
-        out["config"]["config5_adjoint"] = adjoint_run(args.adjoint_steps)
+        out["config"]["config5_adjoint"] = adjoint_run(args.adjoint_steps, contact=True)
+        out["config"]["config5_adjoint_zone"] = adjoint_run(args.adjoint_steps, contact=False, fd=True)
     if not args.no_cpu:
         ccase = cases.config2(steps=args.cpu_steps)
         log(f"bench: cpu baseline, {args.cpu_steps} steps")

@@ -8,11 +8,15 @@ a snapshot every 4).  The contact window is short because the penalty
 contact is chaotic (a perturbation grows ~2x per step once particles touch
 the plane): over 10 steps the finite differences converge (the oracle gives
 the same difference quotient for relative steps 1e-4, 1e-5 and 1e-6 to
-three digits), over 20 or more they do not.
+three digits), over 20 or more they do not.  Contact activation switches
+within a few 1e-4 of a parameter, so the contact case takes a 1e-5 relative
+step (the oracle's quotients at 1e-5 and 1e-6 agree to 1e-5).
 Bars: the replayed forward reproduces the first forward bitwise (loss and
 end state); every gradient component within 1e-3 relative of the central
-difference with a 1e-4 relative parameter step (components whose finite
-difference is below 1e-6 of the largest are compared in absolute terms)."""
+difference with a 1e-4 relative parameter step (1e-5 with contact).
+Parameters differ in scale by ten orders, so components are compared as
+dL/d ln(theta) = theta dL/dtheta; one whose finite difference is below 1e-6
+of the largest is compared in absolute terms."""
 import numpy as np
 import pytest
 
@@ -23,7 +27,7 @@ pytestmark = pytest.mark.gpu
 JC_KEYS = ("jc_A", "jc_B", "jc_n", "jc_C", "jc_m")
 
 
-def _fd_check(g, case, mat0, target, kind, index, friction=None, k=10):
+def _fd_check(g, case, mat0, target, kind, index, friction=None, k=10, rel_step=1e-4):
     loss0, hist0 = g.adj_forward(target, checkpoint_every=k, kind=kind, index=index, component=2)
     disp_end = g.get("disp").copy()
     grad = g.adj_backward()
@@ -53,16 +57,17 @@ def _fd_check(g, case, mat0, target, kind, index, friction=None, k=10):
         checks.append(("mu", friction, grad[5] + grad[6]))
     fd = {}
     for key, p0, gk in checks:
-        h = 1e-4 * abs(p0)
+        h = rel_step * abs(p0)
         fd[key] = (loss_at(key, p0 + h) - loss_at(key, p0 - h)) / (2 * h)
-    scale = max(abs(v) for v in fd.values())
-    print(f"adjoint loss {loss0:.6e}", {k: (fd[k], gk) for k, _, gk in checks}, flush=True)
+    scaled = {key: (p0 * fd[key], p0 * gk) for key, p0, gk in checks}
+    scale = max(abs(v[0]) for v in scaled.values())
+    print(f"adjoint loss {loss0:.6e} theta*dL/dtheta (fd, adjoint)", scaled, flush=True)
     assert scale > 0
-    for key, p0, gk in checks:
-        if abs(fd[key]) > 1e-6 * scale:
-            assert abs(gk - fd[key]) <= 1e-3 * abs(fd[key]), (key, gk, fd[key])
+    for key, (sf, sg) in scaled.items():
+        if abs(sf) > 1e-6 * scale:
+            assert abs(sg - sf) <= 1e-3 * abs(sf), (key, sg, sf)
         else:
-            assert abs(gk - fd[key]) <= 1e-6 * scale, (key, gk, fd[key])
+            assert abs(sg - sf) <= 1e-6 * scale, (key, sg, sf)
 
 
 def test_adjoint_zone_reaction_matches_fd():
@@ -90,7 +95,7 @@ def test_adjoint_plane_force_matches_fd():
     assert g.scalars()["max_pl_strain"] > 0
     _, hist = g.adj_forward(np.zeros(10), checkpoint_every=4, kind=capi.ADJ_PLANE, index=0, component=2)
     g.adj_restore()
-    _fd_check(g, c, c.materials[0], 0.98 * hist, capi.ADJ_PLANE, 0, friction=c.contact[0], k=4)
+    _fd_check(g, c, c.materials[0], 0.98 * hist, capi.ADJ_PLANE, 0, friction=c.contact[0], k=4, rel_step=1e-5)
 
 
 def test_adjoint_rejects_unsupported():
@@ -135,5 +140,5 @@ def test_window_loss_autograd():
     step = 1e-3 * step / np.abs(step).max()
     with torch.no_grad():
         loss2 = WindowLoss.apply(torch.tensor(scale * (1 + step)), w)
-    print("window loss", float(loss), "after a step", float(loss2), flush=True)
+    print("window loss", float(loss.detach()), "after a step", float(loss2), flush=True)
     assert float(loss2) < float(loss)

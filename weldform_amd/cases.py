@@ -226,6 +226,83 @@ def config5(steps=400, dx=0.4675e-3, R=15e-3, L=30e-3, dens=32, v_punch=-5.0, mu
     return c
 
 
+def config4(n_spec=256, steps=500, dx=1.0e-3, n_range=(2000, 40000), v_top=-5.0, seed=2030):
+    """Batched ensemble: n_spec cylinder upsetting specimens advanced as one
+    particle set.  Specimen i: N_i log-uniform on n_range, R/L uniform on
+    [0.2, 0.5], R = (N_i dx^3 (R/L) / pi)^(1/3) on the dx lattice; aluminium
+    elasticity; Johnson-Cook A, B, n, C, m each uniform within +-20 % of the
+    config-2 values (eps_0 = 1/s, T = T_t = 300 K); bottom row fixed
+    (zone 2), top row at v_top (zone 3); Monaghan alpha = 1.  Specimens stand
+    on a square grid with gaps wider than the kernel support, and each has
+    its own material index, so no pair crosses specimens.  The drawn
+    friction coefficients (uniform 0.05-0.3) are recorded in meta; the
+    specimens are driven through velocity BCs, not rigid planes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = n_range
+    N = np.exp(rng.uniform(np.log(lo), np.log(hi), n_spec))
+    ratio = rng.uniform(0.2, 0.5, n_spec)
+    scale = rng.uniform(0.8, 1.2, (n_spec, 5))
+    mu = rng.uniform(0.05, 0.3, n_spec)
+    r = dx / 2.
+    h = 2. * r * 1.2
+    rho, E, nu = AL["rho"], AL["E"], AL["nu"]
+    K, G, Cs = elastic(E, nu, rho)
+    Rs = (N * dx ** 3 * ratio / math.pi) ** (1.0 / 3.0)
+    Ls = Rs / ratio
+    cols = int(math.ceil(math.sqrt(n_spec)))
+    pitch = 2 * Rs.max() + 4 * h
+    xs, hs, ms, ids, mats, sy = [], [], [], [], [], []
+    materials = {}
+    spec_n = []
+    for i in range(n_spec):
+        cx, cy = (i % cols) * pitch, (i // cols) * pitch
+        x = capi.gen_cylinder([cx, cy, 0.0], Rs[i], Ls[i], r)
+        n = len(x)
+        spec_n.append(n)
+        m = dict(JC_AL)
+        for k, key in enumerate(("jc_A", "jc_B", "jc_n", "jc_C", "jc_m")):
+            m[key] = JC_AL[key] * scale[i, k]
+        m.update(E=E, nu=nu, G=G, Cs=Cs, ref_density=rho, P0=0.0)
+        materials[i] = m
+        z = x[:, 2]
+        zid = np.zeros(n, dtype=np.int32)
+        zid[z <= z.min() + 0.5 * r] = 2
+        zid[z >= z.max() - 0.5 * r] = 3
+        xs.append(x)
+        ms.append(np.full(n, math.pi * Rs[i] ** 2 * Ls[i] * rho / n))
+        ids.append(zid)
+        mats.append(np.full(n, i, dtype=np.int32))
+        sy.append(np.full(n, m["jc_A"]))  # JC(0, 0, T_t) = A
+    x = np.concatenate(xs)
+    n = len(x)
+    case = Case(
+        name=f"batch_{n_spec}_cylinders", x=x, h=np.full(n, h), mass=np.concatenate(ms), density=np.full(n, rho),
+        ids=np.concatenate(ids), material=np.concatenate(mats), materials=materials, sigmay=np.concatenate(sy),
+        dt=0.4 * h / Cs, steps=steps, T=np.full(n, 300.0),
+        bcs=[dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, 0.0, v_top))],
+        meta=dict(dx=dx, h=h, R=Rs, L=Ls, N_drawn=N, spec_n=np.array(spec_n), friction=mu, pitch=pitch))
+    return case
+
+
+def specimen(batch, i):
+    """Specimen i of a batch (config4) as a case of its own (material 0)."""
+    sel = batch.material == i
+    n = int(sel.sum())
+    return Case(name=f"{batch.name}_specimen{i}", x=batch.x[sel], h=batch.h[sel], mass=batch.mass[sel],
+                density=batch.density[sel], ids=batch.ids[sel], material=np.zeros(n, dtype=np.int32),
+                materials={0: batch.materials[i]}, sigmay=batch.sigmay[sel], dt=batch.dt, steps=batch.steps,
+                T=None if batch.T is None else batch.T[sel], bcs=batch.bcs, meta=dict(batch.meta, specimen=i))
+
+
+def config5_zone(steps=400, dx=0.4675e-3, R=15e-3, L=30e-3, v_top=-5.0, seed=2029):
+    """The config-5 billet (200,704 particles, JC steel, isothermal) driven
+    through velocity BCs instead of rigid planes: bottom row fixed, top row
+    at v_top; the loss then takes the top zone's reaction."""
+    c = cylinder_case("cyl_upset_200k_jc_adjoint", R, L, dx, 7850.0, 200e9, 0.29, STEEL_JC, steps=steps,
+                      jitter=1e-3, seed=seed, v_top=v_top, T0=300.0)
+    return c
+
+
 def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,

@@ -81,6 +81,7 @@ struct Ctx {
   bool inited = false;
   bool built = false;
   std::vector<int32_t> perm, inv;  // perm[internal] = original
+  std::vector<double> mat_xc;      // per-material FP32 shadow origin [kMaxMat][3]
   Dev d{};
   std::vector<std::pair<void *, size_t>> allocs;  // (pointer, bytes) of the solver state
   DevMat *dmats = nullptr;
@@ -265,7 +266,6 @@ void cell_initiate(Ctx *c) {
     if (h[i] > hmax) hmax = h[i];
   }
   // mixed-precision shadows store positions relative to the body's centre
-  for (int k = 0; k < 3; ++k) c->d.xc[k] = 0.5 * (TRPR[k] + BLPF[k]);
   // rigid-plane particles follow the solid ones in the reference's particle
   // list and count for the bounding box and h_max
   for (const MeshHost &M : c->meshes)
@@ -586,9 +586,27 @@ void upload_all(Ctx *c) {
   dput(c, d.group, grp);
   dput(c, d.thbc, thbc);
   dput(c, d.perm, perm);
+  // per-material origin of the FP32 position shadows
+  {
+    std::vector<double> lo(3 * kMaxMat, 1e300), hi(3 * kMaxMat, -1e300);
+    const double *X = F[WF_F_X].data();
+    for (int64_t o = 0; o < n; ++o) {
+      const int m = c->hi[WF_I_MATERIAL][o];
+      for (int q = 0; q < 3; ++q) {
+        lo[3 * m + q] = std::min(lo[3 * m + q], X[3 * o + q]);
+        hi[3 * m + q] = std::max(hi[3 * m + q], X[3 * o + q]);
+      }
+    }
+    c->mat_xc.assign(3 * kMaxMat, 0.0);
+    for (int m = 0; m < kMaxMat; ++m)
+      if (lo[3 * m] <= hi[3 * m])
+        for (int q = 0; q < 3; ++q) c->mat_xc[3 * m + q] = 0.5 * (lo[3 * m + q] + hi[3 * m + q]);
+  }
   std::vector<DevMat> dm(kMaxMat);
   for (size_t k = 0; k < c->mats.size() && k < (size_t)kMaxMat; ++k)
     if (c->mat_set[k]) dm[k] = to_dev(c->mats[k]);
+  for (int m = 0; m < kMaxMat; ++m)
+    for (int q = 0; q < 3; ++q) dm[m].xc[q] = c->mat_xc[3 * m + q];
   dput(c, c->dmats, dm);
   // device schedule state: the pre-loop ClearNbData leaves the data cleared
   DevFlags fl{};
@@ -982,6 +1000,7 @@ int wf_set_material(void *ctx, int32_t index, const wf_material *m) {
     c->mat_set[index] = 1;
     if (c->inited) {
       std::vector<DevMat> one(1, to_dev(*m));
+      for (int q = 0; q < 3; ++q) one[0].xc[q] = c->mat_xc[3 * index + q];
       HIPCHK(hipMemcpyAsync(c->dmats + index, one.data(), sizeof(DevMat), hipMemcpyHostToDevice, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
     }

@@ -78,6 +78,11 @@ struct DevMat {
   double hol_K, hol_m, hol_sy0, hol_eps0, hol_eps1;
   double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt;
   double k_T, cp_T, th_exp;
+  // origin of the material's FP32 position shadows: the centre of its
+  // particles' initial bounding box (pairs form within one material, so the
+  // pair differences see one origin; a batch of separated specimens keeps
+  // each specimen's shadows as fine as a lone specimen's)
+  double xc[3];
 };
 
 // device-side solver state (one per context)
@@ -170,8 +175,8 @@ struct Dev {
   double *T, *dTdt, *qpl, *psen, *qconv, *hconv, *tinf, *qsrc;
   int *id, *mat, *group, *thbc, *perm;
   // mixed-precision shadows of the gathered inputs (written by the FP64
-  // integrators); positions are stored relative to xc to halve their error
-  double xc[3];
+  // integrators); positions are stored relative to the material's origin
+  // DevMat::xc to keep their rounding at the specimen's scale
   float4 *xhf[2];              // (x - xc, y - yc, z - zc, mass/rho)
   float4 *vmf;                 // (vx, vy, vz, mass)
   StressRecF *srecf;

@@ -832,7 +832,8 @@ __device__ inline void dens_update(const Dev &d, int cur, int i, double s, doubl
   const double4 xn = make_double4(xd.x + ux, xd.y + uy, xd.z + uz, xd.w);
   d.xh[nxt][i] = xn;
   if (MX) {
-    d.xhf[nxt][i] = make_float4((float)(xn.x - d.xc[0]), (float)(xn.y - d.xc[1]), (float)(xn.z - d.xc[2]), (float)vn);
+    const double *o = d.mats[d.mat[i]].xc;
+    d.xhf[nxt][i] = make_float4((float)(xn.x - o[0]), (float)(xn.y - o[1]), (float)(xn.z - o[2]), (float)vn);
     const DevMat &M = d.mats[d.mat[i]];
     d.prf[i] = (float)((M.P0 + M.Cs * M.Cs * (rn - M.ref_density)) / (rn * rn));
   }
@@ -1543,9 +1544,10 @@ __global__ __launch_bounds__(256) void k_max_pl_final(Dev d) {
 WF_KERNEL void k_shadows(Dev d, int cur) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= d.n) return;
+  const double *o = d.mats[d.mat[i]].xc;
   for (int b = 0; b < 2; ++b) {
     const double4 x = d.xh[b][i];
-    d.xhf[b][i] = make_float4((float)(x.x - d.xc[0]), (float)(x.y - d.xc[1]), (float)(x.z - d.xc[2]), (float)d.vol[b][i]);
+    d.xhf[b][i] = make_float4((float)(x.x - o[0]), (float)(x.y - o[1]), (float)(x.z - o[2]), (float)d.vol[b][i]);
   }
   const double4 v = d.vm[i];
   d.vmf[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
