@@ -274,8 +274,10 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
  * those of the forward.  After it the state is the window's end state;
  * wf_adj_get_field gives dL/d(initial state) for WF_F_X, WF_F_V,
  * WF_F_DENSITY, WF_F_SHEAR (9 per particle), WF_F_SIGMAY, WF_F_PL_STRAIN,
- * and wf_adj_restore puts the state back to the window's start (to run the
- * window again, e.g. with other material parameters).  Needs precision
+ * and wf_adj_restore puts the whole device state back to the window's start
+ * (to run the window again, e.g. with other material parameters; material
+ * parameters set since the window started revert too, so set them after
+ * the restore).  Needs precision
  * WF_FP64, a fixed step (auto_ts 0) and thermal_solver 0. */
 #define WF_ADJ_PLANE 0
 #define WF_ADJ_ZONE 1

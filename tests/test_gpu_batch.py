@@ -2,13 +2,18 @@
 with its own material, no pair across specimens) against the CPU oracle and
 against the same specimens run one by one.
 
+The reference's neighbour-rebuild schedule is global: once any particle's
+plastic strain passes 1e-6, every particle's list is rebuilt every
+ts_nb_inc steps (Source/SolverLeapfrog.cpp:160-201).  In a batch the first
+specimen to yield sets the schedule for all, so a specimen in a batch and
+the same specimen alone can use neighbour lists of different ages; the
+oracle, run on the whole batch, follows the batch's schedule.
+
 Bars: FP64 batch vs oracle displacement and plastic strain within 1e-8
-normwise after the run, per specimen; FP64 batch vs lone specimens within
-1e-10 (the batch sums each row in the batch's internal order); mixed batch
-vs lone mixed specimens within 1e-5 (the FP32 shadows of a specimen take
-its own origin in either case) and vs the FP64 oracle within 1e-4; the
-autotuner's candidates give bitwise-equal results for every staging
-capacity."""
+normwise after the run, per specimen; batch vs the specimen alone (FP64 and
+mixed) within 1e-4 (the list ages differ); mixed batch vs the FP64 oracle
+within 1e-4; the autotuner's candidates give bitwise-equal results for every
+staging capacity."""
 import numpy as np
 import pytest
 
@@ -58,8 +63,8 @@ def test_batch_matches_oracle_and_lone_specimens(batch):
         e_ml = normwise_err(res[capi.MIXED][0][sel], lone[capi.MIXED])
         print(f"specimen {i}: n={s.n} vs oracle {e_o:.2e} pl {e_p:.2e} mixed {e_mo:.2e}; "
               f"vs lone fp64 {e_l:.2e} mixed {e_ml:.2e}", flush=True)
-        assert e_o < 1e-8 and e_p < 1e-8 and e_l < 1e-10
-        assert e_mo < 1e-4 and e_ml < 1e-5
+        assert e_o < 1e-8 and e_p < 1e-8 and e_l < 1e-4
+        assert e_mo < 1e-4 and e_ml < 1e-4
 
 
 def test_autotune_shapes_agree(batch):

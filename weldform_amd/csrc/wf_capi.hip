@@ -120,6 +120,7 @@ struct Ctx {
   struct Adj {
     wf_adj_config cfg{};
     bool ready = false;       // a window's forward has run
+    bool window = false;      // snapshot 0 holds a window's start
     long long step0 = 0;      // steps completed before the window
     size_t snap_bytes = 0;
     std::vector<char *> ckpt; // device snapshots of every state allocation
@@ -1589,6 +1590,7 @@ int wf_adj_forward(void *ctx, const wf_adj_config *cfg, const double *target, do
     *loss = L;
     if (history) std::memcpy(history, A.hist.data(), (size_t)steps * sizeof(double));
     A.ready = true;
+    A.window = true;
     return WF_OK;
   });
 }
@@ -1689,7 +1691,7 @@ int wf_adj_restore(void *ctx) {
   return guard(c, [&] {
     as_ctx(ctx);
     auto &A = c->adj;
-    if (A.ckpt.empty() || !(A.ready || A.have_grad)) throw Error(WF_ESTATE, "no adjoint window to restore");
+    if (A.ckpt.empty() || !A.window) throw Error(WF_ESTATE, "no adjoint window to restore");
     HIPCHK(hipSetDevice(c->cfg.device));
     restore(c, 0);
     HIPCHK(hipStreamSynchronize(c->st));
