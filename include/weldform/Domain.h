@@ -328,8 +328,6 @@ class Domain {
     }
     if (GeneralBefore != &General || GeneralAfter != &General)
       throw new Fatal("host callbacks cannot run inside a device step: express BCs through bConds / amps");
-    for (size_t i = 0; i < nsolid(); ++i)
-      if (Particles[i]->TI > 0.0) throw new Fatal("tensile instability (TI > 0) is not implemented");
   }
 
   // one wf_material per Particle::Material id; particles sharing an id must
@@ -348,6 +346,12 @@ class Domain {
     m.k_T = P->k_T;
     m.cp_T = P->cp_T;
     m.th_exp = P->th_exp;
+    if (P->TI > 0.0) {
+      // tensile instability: TI, TIn and TIInitDist travel with the material
+      m.ti = P->TI;
+      m.ti_n = P->TIn;
+      m.ti_dist = P->TIInitDist;
+    }
     if (P->mat) {
       m.E = P->mat->Elastic().E();
       m.nu = P->mat->Elastic().nu();

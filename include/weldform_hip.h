@@ -25,7 +25,9 @@
  *                               (Source/SolverKickDrift.cpp:117-330)
  *   wf_rebuild_neighbours    <- ClearNbData + MainNeighbourSearch + CalcPairPosList
  *                               (Source/Neighbour.cpp:346-359,479-497,617-694)
- *   wf_eval(WF_EVAL_ACCEL)   <- CalcAccel + AccelReduction (Source/InteractionAlt.cpp:12-260)
+ *   wf_eval(WF_EVAL_ACCEL)   <- StartAcceleration's tensile-instability stress
+ *                               (Source/Domain.cpp:1519-1597) + CalcAccel +
+ *                               AccelReduction (Source/InteractionAlt.cpp:12-260)
  *   wf_eval(WF_EVAL_DENSITY) <- CalcDensInc + DensReduction (Source/InteractionAlt.cpp:483-640)
  *   wf_eval(WF_EVAL_RATES)   <- CalcRateTensors + RateTensorsReduction
  *                               (Source/InteractionAlt.cpp:263-479)
@@ -44,7 +46,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 4
+#define WF_ABI_VERSION 5
 
 /* error codes */
 #define WF_OK 0
@@ -172,6 +174,10 @@ typedef struct wf_material {
   double hol_K, hol_m, hol_sy0;                 /* Hollomon                     */
   double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt; /* Johnson-Cook  */
   double k_T, cp_T, th_exp; /* Particle::k_T, cp_T, th_exp                     */
+  double ti, ti_n, ti_dist; /* Particle::TI, TIn (4), TIInitDist: tensile
+                               instability (Source/Domain.cpp:1529-1563,
+                               Source/InteractionAlt.cpp:104-110); ti = 0 off.
+                               Set before wf_init.                            */
 } wf_material;
 
 typedef struct wf_scalars {

@@ -20,9 +20,9 @@ F64 = ["X", "V", "A", "XPREV", "DISP", "RHO", "DRHO", "MASS", "H", "REFRHO", "P"
        "CS", "P0", "G", "ALPHA", "BETA", "SIGMA", "SHEAR", "SR", "RR", "STRAIN",
        "EPL", "EPLINC", "PLSTRAIN", "DPLSTRAIN", "SIGMAY", "EFFSR", "EP", "ET",
        "T", "DTDT", "KT", "CPT", "THEXP", "QCONV", "QSRC", "QPL", "QCONT",
-       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE"]
+       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR"]
 F64_DIM = dict(X=3, V=3, A=3, XPREV=3, DISP=3, SIGMA=9, SHEAR=9, SR=9, RR=9,
-               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3)
+               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9)
 I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB", "ELEMENT", "MESHIDX", "IDORIG"]
 
 
@@ -88,6 +88,8 @@ def lib():
         L.wfo_kernel.argtypes = [d, d]
         L.wfo_grad_kernel.restype = d
         L.wfo_grad_kernel.argtypes = [d, d]
+        L.wfo_sym3_eig.restype = None
+        L.wfo_sym3_eig.argtypes = [dp, dp, dp]
         _lib = L
     return _lib
 
@@ -106,6 +108,15 @@ def kernel(q, h):
 
 def grad_kernel(q, h):
     return lib().wfo_grad_kernel(float(q), float(h))
+
+
+def sym3_eig(A):
+    """(w, V) with A = V diag(w) V^T for a symmetric 3x3 A."""
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(9)
+    w = np.zeros(3)
+    V = np.zeros(9)
+    lib().wfo_sym3_eig(_dp(A), _dp(w), _dp(V))
+    return w, V.reshape(3, 3)
 
 
 def gen_cylinder(V, R, L, r):

@@ -37,7 +37,9 @@ static const int k_fdim[WFO_NFIELDS] = {
     1, 1, 1, 1, 1,        /* THEXP QCONV QSRC QPL QCONT */
     1, 1, 1,              /* HCONV TINF PSENERGY */
     3,                    /* NORMAL */
-    3                     /* CONTFORCE */
+    3,                    /* CONTFORCE */
+    1, 1, 1,              /* TI TIN TIDIST */
+    9                     /* TIR */
 };
 
 typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T; } bc_t;
@@ -115,6 +117,7 @@ wfo_domain *wfo_create(int n) {
   for (int i = 0; i < n; ++i) {
     d->fi[WFO_FREE][i] = 1;
     d->f[WFO_H][i] = 0.0;
+    d->f[WFO_TIN][i] = 4.0;
   }
   /* Domain constructor defaults (Source/Domain.cpp:78-175) */
   d->Cellfac = 2.0;
@@ -618,9 +621,77 @@ int64_t wfo_npairs(const wfo_domain *d) { return d->npairs; }
 const int *wfo_pairs(const wfo_domain *d) { return d->pairs; }
 
 /* ------------------------------------------------------------------ */
-/* StartAcceleration with a = 0, TI = 0 (Source/Domain.cpp:1519-1597)  */
+/* Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (Golub & Van
+ * Loan, Matrix Computations, 4th ed., Alg. 8.5.2-8.5.3): A = V diag(w) V^T
+ * with the eigenvectors in the columns of V.  It stands in for GSL's
+ * gsl_eigen_symmv, which the reference reaches through Rotation / Eig
+ * (Source/Functions.cpp:381-395, External/matvec.h:776); GSL is absent, and
+ * its use here -- the tensile-instability stress, a spectral function of
+ * sigma -- does not depend on the eigenvectors' signs or order. */
+void wfo_sym3_eig(const double *Ain, double *w, double *V) {
+  double A[3][3];
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      A[a][b] = 0.5 * (Ain[3 * a + b] + Ain[3 * b + a]);
+      V[3 * a + b] = a == b ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 64; sweep++) {
+    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+    double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+    if (off == 0.0 || off <= 1e-40 * dia) break;
+    for (int p = 0; p < 2; p++)
+      for (int q = p + 1; q < 3; q++) {
+        if (A[p][q] == 0.0) continue;
+        double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+        double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < 3; k++) { /* A <- A J */
+          double akp = A[k][p], akq = A[k][q];
+          A[k][p] = c * akp - s * akq;
+          A[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < 3; k++) { /* A <- J^T A */
+          double apk = A[p][k], aqk = A[q][k];
+          A[p][k] = c * apk - s * aqk;
+          A[q][k] = s * apk + c * aqk;
+        }
+        A[p][q] = A[q][p] = 0.0;
+        for (int k = 0; k < 3; k++) { /* V <- V J */
+          double vkp = V[3 * k + p], vkq = V[3 * k + q];
+          V[3 * k + p] = c * vkp - s * vkq;
+          V[3 * k + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int k = 0; k < 3; k++) w[k] = A[k][k];
+}
+
+/* The tensile-instability stress of a particle, 3D branch of
+ * StartAcceleration (Source/Domain.cpp:1545-1563): principal stresses
+ * s_k > 0 become -TI s_k / rho^2, the others 0, rotated back with the
+ * eigenvectors (Vec Val, then (Vec Val) Vec^T). */
+static void tensile_instability(wfo_domain *d, int i) {
+  const double *sig = d->f[WFO_SIGMA] + 9 * (int64_t)i;
+  double rho2 = d->f[WFO_RHO][i] * d->f[WFO_RHO][i];
+  double pc = d->f[WFO_TI][i] / (rho2);
+  double w[3], V[9], temp[9];
+  wfo_sym3_eig(sig, w, V);
+  for (int k = 0; k < 3; k++) w[k] = w[k] > 0 ? -pc * w[k] : 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) temp[3 * a + b] = V[3 * a + b] * w[b];
+  double *tir = d->f[WFO_TIR] + 9 * (int64_t)i;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      double s = 0.0;
+      for (int k = 0; k < 3; k++) s += temp[3 * a + k] * V[3 * b + k];
+      tir[3 * a + b] = s;
+    }
+}
+
+/* StartAcceleration with a = 0 (Source/Domain.cpp:1519-1597), 3D */
 int wfo_start_acceleration(wfo_domain *d) {
   for (int i = 0; i < d->n; i++) {
+    if (d->fi[WFO_FREE][i] && d->f[WFO_TI][i] > 0.0) tensile_instability(d, i);
     if (!d->fi[WFO_FREE][i]) {
       d->f[WFO_P][i] = 0.0;
       memset(d->f[WFO_SIGMA] + 9 * (int64_t)i, 0, 9 * sizeof(double));
@@ -650,8 +721,10 @@ static inline void pair_geo(const wfo_domain *d, int64_t p, pairgeo_t *g) {
   g->GK = wfo_grad_kernel(g->rij / g->h, g->h);
 }
 
-/* CalcAccel, Nishimura branch, GradientType 0, TI = 0
- * (Source/InteractionAlt.cpp:12-235) */
+/* CalcAccel, Nishimura branch, GradientType 0
+ * (Source/InteractionAlt.cpp:12-235); the tensile-instability term
+ * TIij = (W(r/h) / W(d0/h))^((n_1+n_2)/2) (TIR_1 + TIR_2), d0 the mean
+ * TIInitDist, when either particle has TI > 0 (:104-110) */
 static void calc_accel_pairs(wfo_domain *d) {
   const double *rho = d->f[WFO_RHO], *sig = d->f[WFO_SIGMA];
 #pragma omp parallel for schedule(static) num_threads(g_threads)
@@ -670,11 +743,20 @@ static void calc_accel_pairs(wfo_domain *d) {
       if (dot3(g.vij, g.xij) < 0) PI_s = (Alpha * Cij * MUij + Beta * MUij * MUij) / (0.5 * (di + dj));
     }
     double si = 1.0 / (di * di), sj = 1.0 / (dj * dj);
+    double tis = 0.0;
+    const double *ti = d->f[WFO_TI], *tir = d->f[WFO_TIR];
+    int ti_on = ti[p1] > 0.0 || ti[p2] > 0.0;
+    if (ti_on) {
+      double K = wfo_kernel(g.rij / g.h, g.h);
+      double K0 = wfo_kernel((d->f[WFO_TIDIST][p1] + d->f[WFO_TIDIST][p2]) / (2.0 * g.h), g.h);
+      tis = pow((K / K0), (d->f[WFO_TIN][p1] + d->f[WFO_TIN][p2]) / 2.0);
+    }
     double M[9];
     for (int a = 0; a < 3; a++)
       for (int c = 0; c < 3; c++) {
         double piac = (a == c) ? 1.0 * PI_s : 0.0 * PI_s;
-        M[3 * a + c] = ((sig[9 * (int64_t)p1 + 3 * a + c] * si + sig[9 * (int64_t)p2 + 3 * a + c] * sj) + piac) + 0.0;
+        double tiac = ti_on ? tis * (tir[9 * (int64_t)p1 + 3 * a + c] + tir[9 * (int64_t)p2 + 3 * a + c]) : 0.0;
+        M[3 * a + c] = ((sig[9 * (int64_t)p1 + 3 * a + c] * si + sig[9 * (int64_t)p2 + 3 * a + c] * sj) + piac) + tiac;
       }
     double gv[3] = {g.GK * g.xij[0], g.GK * g.xij[1], g.GK * g.xij[2]};
     for (int c = 0; c < 3; c++)

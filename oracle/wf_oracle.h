@@ -48,6 +48,8 @@ enum {
   WFO_HCONV, WFO_TINF, WFO_PSENERGY,                       /* 1 */
   WFO_NORMAL,                                              /* 3 */
   WFO_CONTFORCE,                                           /* 3 */
+  WFO_TI, WFO_TIN, WFO_TIDIST,                             /* 1: TI, TIn, TIInitDist */
+  WFO_TIR,                                                 /* 9 */
   WFO_NFIELDS
 };
 /* int fields */
@@ -117,6 +119,8 @@ const int *wfo_contact_pairs(const wfo_domain *d);          /* (SPH, rigid) */
 int wfo_reaction(const wfo_domain *d, int zone, double out[3]);
 /* kernel functions, exported for closed-form pins */
 double wfo_kernel(double q, double h);
+/* symmetric 3x3 eigen-decomposition (cyclic Jacobi): A = V diag(w) V^T */
+void wfo_sym3_eig(const double *A, double *w, double *V);
 double wfo_grad_kernel(double q, double h);
 
 #ifdef __cplusplus

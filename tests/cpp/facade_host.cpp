@@ -74,7 +74,6 @@ int main() {
   expect_fatal("kernel", [](SPH::Domain &d) { d.Kernel_Set(SPH::Quintic_Spline); });
   expect_fatal("gradient", [](SPH::Domain &d) { d.Gradient_Approach_Set(SPH::Randles_Libersky); });
   expect_fatal("callback", [](SPH::Domain &d) { d.GeneralAfter = [](SPH::Domain &) {}; });
-  expect_fatal("tensile", [](SPH::Domain &d) { d.Particles[0]->TI = 0.3; });
   expect_fatal("contact", [](SPH::Domain &d) { d.contact = true; });
   expect_fatal("fail", [](SPH::Domain &d) { d.Particles[3]->Fail = 0; });
   expect_fatal("mixed_params", [](SPH::Domain &d) { d.Particles[5]->Alpha = 0.5; });

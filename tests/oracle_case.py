@@ -38,6 +38,9 @@ def oracle_from_case(case, threads=1):
         d.f("ALPHA")[sel] = m.get("alpha", 0.0)
         d.f("BETA")[sel] = m.get("beta", 0.0)
         d.f("EP")[sel] = m.get("Ep", 0.0)
+        d.f("TI")[sel] = m.get("ti", 0.0)
+        d.f("TIN")[sel] = m.get("ti_n", 4.0)
+        d.f("TIDIST")[sel] = m.get("ti_dist", 0.0)
         d.f("KT")[sel] = m.get("k_T", 0.0)
         d.f("CPT")[sel] = m.get("cp_T", 0.0)
         d.f("THEXP")[sel] = m.get("th_exp", 0.0)

@@ -569,3 +569,95 @@ def test_adaptive_time_step_acceleration_criterion():
     d.solve_init(1e-6)
     d.step()
     assert d.param("deltat") == 1e-6
+
+
+# ---------------------------------------------------------------- tensile instability
+def test_sym3_eig_against_lapack():
+    """The Jacobi eigen-decomposition reproduces A and agrees with LAPACK's
+    (numpy.linalg.eigh) eigenvalues, including repeated ones."""
+    rng = np.random.default_rng(8)
+    mats = [rng.normal(size=(3, 3)) * 1e8 for _ in range(40)]
+    mats += [np.diag([2.0, 2.0, -1.0]) * 1e8, np.eye(3) * 5e7, np.zeros((3, 3))]
+    for M in mats:
+        A = 0.5 * (M + M.T)
+        w, V = po.sym3_eig(A)
+        scale = max(np.abs(A).max(), 1.0)
+        assert np.abs(V @ np.diag(w) @ V.T - A).max() < 1e-14 * scale
+        assert np.abs(V.T @ V - np.eye(3)).max() < 1e-14
+        assert np.allclose(np.sort(w), np.linalg.eigh(A)[0], rtol=0, atol=1e-14 * scale)
+
+
+def _ti_particles(sig, TI=0.3, rho=2700.0):
+    n = len(sig)
+    d = po.Domain(n)
+    d.f("RHO")[:] = rho
+    d.f("SIGMA")[:] = np.asarray(sig).reshape(n, 3, 3)
+    d.f("TI")[:] = TI
+    d.start_acceleration()
+    return d.f("TIR").reshape(n, 3, 3).copy()
+
+
+def test_tensile_instability_stress_is_the_positive_spectral_part():
+    """TIR = -TI/rho^2 sigma_+, sigma_+ the positive part of sigma: closed
+    form on diagonal stresses, rotation covariance, and LAPACK's
+    eigenvectors on random stresses (StartAcceleration, Source/Domain.cpp:
+    1545-1563)."""
+    TI, rho = 0.3, 2700.0
+    c = TI / rho ** 2
+    D = np.diag([3e8, -1e8, 5e7])
+    t = _ti_particles([D])[0]
+    assert np.allclose(t, np.diag([-c * 3e8, 0.0, -c * 5e7]), rtol=1e-15, atol=0)
+    # rotation covariance
+    th = 0.7
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1.0]])
+    R = R @ np.array([[1, 0, 0], [0, math.cos(0.4), -math.sin(0.4)], [0, math.sin(0.4), math.cos(0.4)]])
+    t = _ti_particles([R @ D @ R.T])[0]
+    assert np.abs(t - R @ np.diag([-c * 3e8, 0.0, -c * 5e7]) @ R.T).max() < 1e-14 * c * 3e8
+    # random stresses: the positive part from LAPACK's decomposition
+    rng = np.random.default_rng(9)
+    S = rng.normal(size=(30, 3, 3)) * 2e8
+    S = 0.5 * (S + np.transpose(S, (0, 2, 1)))
+    t = _ti_particles(S)
+    for k in range(30):
+        w, V = np.linalg.eigh(S[k])
+        want = V @ np.diag(np.where(w > 0, -c * w, 0.0)) @ V.T
+        assert np.abs(t[k] - want).max() < 1e-13 * c * np.abs(w).max()
+    # compressive stress: no correction
+    assert np.all(_ti_particles([-np.eye(3) * 1e8])[0] == 0.0)
+
+
+def test_tensile_instability_pair_term():
+    """Two particles: switching TI on adds m_j GK (W(r)/W(d0))^n (TIR_i + TIR_j)
+    x_ij to a_i (CalcAccel, Source/InteractionAlt.cpp:104-126)."""
+    h, dx, rho, m = H, 1e-3, 2700.0, 2.7e-6
+    x = np.array([[0.0, 0.0, 0.0], [0.8e-3, 0.3e-3, -0.2e-3], [5e-3, 5e-3, 5e-3], [-5e-3, -5e-3, -5e-3]])
+
+    def run(TI):
+        d = po.Domain(4)
+        d.f("X")[:] = x
+        d.f("H")[:] = h
+        d.f("MASS")[:] = m
+        d.f("RHO")[:] = d.f("REFRHO")[:] = rho
+        d.f("CS")[:] = 5000.0
+        d.f("TI")[:] = TI
+        d.f("TIDIST")[:] = dx
+        sig = np.zeros((4, 3, 3))
+        sig[0] = np.diag([2e8, -1e8, 1e8])
+        sig[1] = np.array([[1e8, 5e7, 0], [5e7, 3e8, 0], [0, 0, -2e8]])
+        d.f("SIGMA")[:] = sig
+        d.set_material(0, model=po.BILINEAR, E=68.9e9, nu=0.3)
+        d.neighbour_search()
+        d.start_acceleration()
+        d.calc_accel()
+        return d.f("A").copy(), d.f("TIR").reshape(4, 3, 3).copy()
+
+    a0, _ = run(0.0)
+    a1, tir = run(0.3)
+    xij = x[0] - x[1]
+    r = np.linalg.norm(xij)
+    GK = po.grad_kernel(r / h, h)
+    f = (po.kernel(r / h, h) / po.kernel(dx / h, h)) ** 4.0
+    want = m * GK * (f * (tir[0] + tir[1])) @ xij
+    assert np.abs(tir[0]).max() > 0 and np.abs(tir[1]).max() > 0
+    assert np.allclose(a1[0] - a0[0], want, rtol=1e-10, atol=0)
+    assert np.allclose(a1[1] - a0[1], -want, rtol=1e-10, atol=0)

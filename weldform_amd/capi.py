@@ -57,7 +57,7 @@ class Material(ctypes.Structure):
         (n, ctypes.c_double) for n in
         ["E", "nu", "G", "Cs", "ref_density", "P0", "alpha", "beta", "Ep",
          "hol_K", "hol_m", "hol_sy0", "jc_A", "jc_B", "jc_n", "jc_C", "jc_eps0",
-         "jc_m", "jc_Tm", "jc_Tt", "k_T", "cp_T", "th_exp"]]
+         "jc_m", "jc_Tm", "jc_Tt", "k_T", "cp_T", "th_exp", "ti", "ti_n", "ti_dist"]]
 
 
 class Scalars(ctypes.Structure):

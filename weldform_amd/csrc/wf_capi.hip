@@ -369,6 +369,9 @@ DevMat to_dev(const wf_material &m) {
   d.k_T = m.k_T;
   d.cp_T = m.cp_T;
   d.th_exp = m.th_exp;
+  d.ti = m.ti;
+  d.ti_n = m.ti_n;
+  d.ti_dist = m.ti_dist;
   return d;
 }
 
@@ -463,6 +466,13 @@ void allocate(Ctx *c) {
   d.lanes_mx = c->cfg.lanes ? c->cfg.lanes : kLanesMixed;
   d.lanes_f64 = c->cfg.lanes ? c->cfg.lanes : kLanesFP64;
   d.auto_ts = c->cfg.auto_ts;
+  d.ti_on = 0;
+  for (size_t k = 0; k < c->mats.size(); ++k)
+    if (c->mat_set[k] && c->mats[k].ti > 0.0) d.ti_on = 1;
+  if (d.ti_on) {
+    d.tir = c->alloc<double>(6 * np);
+    if (d.mixed) d.tirf = c->alloc<float>(6 * np);
+  }
   d.cfl = c->cfg.cfl;
   d.sqrt_h_a = c->cfg.sqrt_h_a;
   d.min_ts = c->cfg.min_ts;
@@ -749,6 +759,8 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr) {
     if (d.contact) launch_surface(d, c->st, cur, 0);
   }
   if (tp) launch_tape_lists(d, c->st, tp->s, tp->force_lists, tp->ver, tp->slot);
+  // StartAcceleration's tensile-instability stress (Source/Domain.cpp:1529-1563)
+  if (d.ti_on) launch_tir(d, c->st, cur);
   {
     Timed t(c, T_ACCEL);
     launch_accel(d, c->st, cur);
@@ -827,6 +839,7 @@ void adj_check(Ctx *c) {
   if (c->cfg.precision != WF_FP64) throw Error(WF_EUNSUPPORTED, "the adjoint needs precision WF_FP64");
   if (c->cfg.auto_ts) throw Error(WF_EUNSUPPORTED, "the adjoint needs a fixed time step (auto_ts 0)");
   if (c->cfg.thermal_solver) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover the thermal solver");
+  if (c->d.ti_on) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover the tensile-instability term");
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -997,6 +1010,11 @@ int wf_set_material(void *ctx, int32_t index, const wf_material *m) {
                     "Hollomon material bad definition, please correct Yield Stress, Elastic Modulus or Material "
                     "hardening constants.");
     }
+    if (!(m->ti >= 0.0)) throw Error(WF_EINVAL, "ti must be >= 0");
+    if (m->ti > 0.0 && !(m->ti_dist >= 0.0 && std::isfinite(m->ti_n)))
+      throw Error(WF_EINVAL, "tensile instability needs ti_dist >= 0 and a finite ti_n");
+    if (c->inited && m->ti > 0.0 && !c->d.tir)
+      throw Error(WF_ESTATE, "tensile instability must be switched on before wf_init");
     c->mats[index] = *m;
     c->mat_set[index] = 1;
     if (c->inited) {
@@ -1769,7 +1787,10 @@ int wf_eval(void *ctx, int32_t what, double dt) {
     if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
     if (!c->built && what != WF_EVAL_STRESS) throw Error(WF_ESTATE, "call wf_rebuild_neighbours first");
     switch (what) {
-      case WF_EVAL_ACCEL: launch_accel(c->d, c->st, c->cur); break;
+      case WF_EVAL_ACCEL:
+        if (c->d.ti_on) launch_tir(c->d, c->st, c->cur);
+        launch_accel(c->d, c->st, c->cur);
+        break;
       case WF_EVAL_DENSITY: launch_dens_drift(c->d, c->st, c->cur, dt, 0); break;
       case WF_EVAL_RATES: launch_rates(c->d, c->st, c->cur); break;
       case WF_EVAL_STRESS: launch_stress(c->d, c->st, c->cur, dt); break;

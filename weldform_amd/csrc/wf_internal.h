@@ -78,6 +78,7 @@ struct DevMat {
   double hol_K, hol_m, hol_sy0, hol_eps0, hol_eps1;
   double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt;
   double k_T, cp_T, th_exp;
+  double ti, ti_n, ti_dist;    // tensile instability: TI, TIn, TIInitDist
   // origin of the material's FP32 position shadows: the centre of its
   // particles' initial bounding box (pairs form within one material, so the
   // pair differences see one origin; a batch of separated specimens keeps
@@ -191,6 +192,9 @@ struct Dev {
   int stage_max, ntiles;
   int lanes_mx, lanes_f64;     // threads per particle in the pair passes
   int auto_ts;                 // bit 0 velocity, bit 1 acceleration criterion
+  int ti_on;                   // some material has TI > 0
+  double *tir;                 // tensile-instability stress [6][npad] (FP64)
+  float *tirf;                 // its FP32 copy (mixed precision)
   double cfl, sqrt_h_a, min_ts;
   int *stage, *stage_cnt;
   uint16_t *nbrl, *selfl;
@@ -254,6 +258,7 @@ void launch_tape_lists(const Dev &d, hipStream_t st, const AdjSlot &s, int force
 void launch_schedule(const Dev &d, hipStream_t st, int force);
 void launch_neighbours(const Dev &d, hipStream_t st, int cur);
 void launch_accel(const Dev &d, hipStream_t st, int cur);
+void launch_tir(const Dev &d, hipStream_t st, int cur);
 // dt < 0 in a launcher: the kernel reads the step from the device flags
 // (adaptive time step).  phase 0: the whole kick; 1: BCs and the first-step
 // half kick; 2: the kick with (dt_prev, dt) and the BCs.

@@ -261,11 +261,15 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
 //  1. each particle's row is sorted by neighbour index (in LDS), so lanes of
 //     a wavefront visit their k-th neighbours at nearby addresses;
 //  2. (mixed precision, stage = 1) the LDS staging list: the tile's
-//     particles and all their neighbour entries are sorted (bitonic, in LDS)
-//     and made unique, giving the tile's staged set in index order, which a
-//     pass loads in long contiguous runs; each entry is then rewritten as its
-//     position in the set (binary search).  Tiles whose set exceeds
-//     stage_max keep stage_cnt = -1 and gather from HBM.
+//     particles and all their neighbour entries go into an LDS hash set
+//     (open addressing, kHashSlots slots), whose distinct values are then
+//     sorted (bitonic, in LDS), giving the tile's staged set in index order,
+//     which a pass loads in long contiguous runs; each entry is then
+//     rewritten as its position in the set (binary search).  The set holds
+//     ~6 % of the entries (each neighbour appears in many rows), so hashing
+//     first sorts ~730 values instead of ~7000.  The hash table's layout
+//     depends on insertion order, the sorted set does not.  Tiles whose set
+//     exceeds stage_max keep stage_cnt = -1 and gather from HBM.
 // Tiles whose entries exceed kSortCap keep their rows as built and are not
 // staged.  Step 1 does not depend on `stage`, so staged and HBM gathers sum
 // in the same order.
@@ -281,10 +285,15 @@ __device__ inline int lower_pos(const int *a, int n, int v) {
   return lo;
 }
 
+constexpr int kHashSlots = 2 * kStageMax;  // load <= 1/2 for sets of <= kStageMax values
+__device__ inline unsigned stage_hash(int v) { return ((unsigned)v * 2654435761u) >> 21; }  // 11 bits
+
 WF_KERNEL void k_stage(Dev d, int stage) {
   if (!d.flags->do_search) return;
   __shared__ int buf[kSortCap];
   __shared__ int uniq[kStageMax];
+  __shared__ int ht[kHashSlots];
+  __shared__ int nset, full;
   __shared__ int sh[34];
   const int tid = threadIdx.x;
   const long long np = d.npad;
@@ -317,43 +326,66 @@ WF_KERNEL void k_stage(Dev d, int stage) {
       __syncthreads();
       continue;
     }
-    // 2. staged set
+    // 2. staged set: distinct values through the LDS hash set.  A value is
+    // counted by the thread whose CAS claims its slot; probing stops once
+    // the set has more than stage_max values (the tile then gathers from
+    // HBM), so every probe sequence ends within kHashSlots slots.
+    for (int p = tid; p < kHashSlots; p += kBlock) ht[p] = -1;
+    if (tid == 0) {
+      nset = 0;
+      full = 0;
+    }
+    __syncthreads();
+    for (int p = tid; p < tot; p += kBlock) {
+      const int v = buf[p];
+      unsigned s = stage_hash(v) & (kHashSlots - 1);
+      for (int probe = 0; probe < kHashSlots; ++probe) {
+        if (full) break;
+        const int old = atomicCAS(&ht[s], -1, v);
+        if (old == -1) {
+          if (atomicAdd(&nset, 1) >= d.stage_max) full = 1;
+          break;
+        }
+        if (old == v) break;
+        s = (s + 1) & (kHashSlots - 1);
+      }
+    }
+    __syncthreads();
+    if (full || nset > d.stage_max) {
+      if (tid == 0) d.stage_cnt[t] = -1;
+      __syncthreads();
+      continue;
+    }
+    // compact the table (slot order), pad to a power of two, sort
+    int carry = 0;
+    for (int base = 0; base < kHashSlots; base += kBlock) {
+      const int v = ht[base + tid];
+      const int f = v >= 0 ? 1 : 0;
+      int sub;
+      const int e = block_scan_excl(f, sh, &sub) + carry;
+      if (f) uniq[e] = v;
+      carry += sub;
+    }
+    const int U = carry;
     int P = 1;
-    while (P < tot) P <<= 1;
-    for (int p = tot + tid; p < P; p += kBlock) buf[p] = 0x7fffffff;
+    while (P < U) P <<= 1;
+    for (int p = U + tid; p < P; p += kBlock) uniq[p] = 0x7fffffff;
     __syncthreads();
     for (int k = 2; k <= P; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
         for (int p = tid; p < P; p += kBlock) {
           const int q = p ^ j;
           if (q > p) {
-            const int a = buf[p], b = buf[q];
+            const int a = uniq[p], b = uniq[q];
             if ((a > b) == ((p & k) == 0)) {
-              buf[p] = b;
-              buf[q] = a;
+              uniq[p] = b;
+              uniq[q] = a;
             }
           }
         }
         __syncthreads();
       }
     }
-    // unique entries of the first tot (padding sorts last)
-    int carry = 0;
-    for (int base = 0; base < tot; base += kBlock) {
-      const int p = base + tid;
-      const int f = (p < tot && (p == 0 || buf[p] != buf[p - 1])) ? 1 : 0;
-      int sub;
-      const int e = block_scan_excl(f, sh, &sub) + carry;
-      if (f && e < kStageMax) uniq[e] = buf[p];
-      carry += sub;
-    }
-    const int U = carry;
-    if (U > d.stage_max) {
-      if (tid == 0) d.stage_cnt[t] = -1;
-      __syncthreads();
-      continue;
-    }
-    __syncthreads();
     for (int p = tid; p < U; p += kBlock) d.stage[(long long)t * kStageMax + p] = uniq[p];
     if (i < d.n) {
       d.selfl[i] = (uint16_t)lower_pos(uniq, U, i);
@@ -410,6 +442,13 @@ struct HbmJ {
   __device__ R4 x(int j) const { return S::x(d, buf, j); }
   __device__ R4 v(int j) const { return S::v(d, j); }
   __device__ void srec(int j, R t[6], R &rho) const { S::srec(d, j, t, rho); }
+  // tensile-instability stress of particle j
+  __device__ void tir(int j, R t[6]) const {
+    for (int k = 0; k < 6; ++k) {
+      if constexpr (MX) t[k] = d.tirf[(long long)k * d.npad + j];
+      else t[k] = d.tir[(long long)k * d.npad + j];
+    }
+  }
   // P_j / rho_j^2 of the kinetic-energy pair term
   __device__ R pr(int j, R rhoj, R P0, R cs2, R rho0) const {
     if constexpr (MX) {
@@ -521,7 +560,17 @@ __device__ inline void stage_accel(const Dev &d, int t, int U, const float4 *gx,
   __syncthreads();
 }
 
-template <bool MX, class G>
+// cubic-spline kernel shape w(q) = W(q h) pi h^3 (Source/Functions.cpp:25-37)
+template <typename R>
+__device__ inline R w_shape(R q) {
+  const R t = (R)2 - q;
+  return q < (R)1 ? (R)1 - (R)1.5 * q * q + (R)0.75 * q * q * q : (q < (R)2 ? (R)0.25 * t * t * t : (R)0);
+}
+
+// TI: with the tensile-instability term (W(r)/W(d0))^n (TIR_i + TIR_j) of
+// CalcAccel (Source/InteractionAlt.cpp:104-110); pairs form within one
+// material, so n and d0 are the material's
+template <bool MX, bool TI, class G>
 __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int sub, int lanes,
                                   typename Src<MX>::R (&acc)[3]) {
   using S = Src<MX>;
@@ -537,6 +586,9 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
     S::srec(d, i, ti, rhoi);
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
     const R acs = alpha * cs;
+    R tri[6] = {0, 0, 0, 0, 0, 0};
+    const R tin = (R)M.ti_n, tid = (R)M.ti_dist;
+    if constexpr (TI) g.tir(i, tri);
     for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
@@ -557,13 +609,27 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
       const R Mxx = ti[XX] + tj[XX] + pi;
       const R Myy = ti[YY] + tj[YY] + pi;
       const R Mzz = ti[ZZ] + tj[ZZ] + pi;
-      const R Mxy = ti[XY] + tj[XY];
-      const R Mxz = ti[XZ] + tj[XZ];
-      const R Myz = ti[YZ] + tj[YZ];
+      R Mxy = ti[XY] + tj[XY];
+      R Mxz = ti[XZ] + tj[XZ];
+      R Myz = ti[YZ] + tj[YZ];
+      R Mxx2 = Mxx, Myy2 = Myy, Mzz2 = Mzz;
+      if constexpr (TI) {
+        // a padded self entry has r2 = 0: q = 0 keeps the factor finite
+        const R q = r2 > (R)0 ? r2 * rsqrt_t(r2) * kh.ih : (R)0;
+        const R s = pow(w_shape<R>(q) / w_shape<R>(tid * kh.ih), tin);
+        R trj[6];
+        g.tir(j, trj);
+        Mxx2 += s * (tri[XX] + trj[XX]);
+        Myy2 += s * (tri[YY] + trj[YY]);
+        Mzz2 += s * (tri[ZZ] + trj[ZZ]);
+        Mxy += s * (tri[XY] + trj[XY]);
+        Mxz += s * (tri[XZ] + trj[XZ]);
+        Myz += s * (tri[YZ] + trj[YZ]);
+      }
       const R w = vj.w * GK;
-      ax += w * (dx * Mxx + dy * Mxy + dz * Mxz);
-      ay += w * (dx * Mxy + dy * Myy + dz * Myz);
-      az += w * (dx * Mxz + dy * Myz + dz * Mzz);
+      ax += w * (dx * Mxx2 + dy * Mxy + dz * Mxz);
+      ay += w * (dx * Mxy + dy * Myy2 + dz * Myz);
+      az += w * (dx * Mxz + dy * Myz + dz * Mzz2);
     });
   }
   acc[0] = ax;
@@ -578,7 +644,10 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
   double a[3] = {0, 0, 0};
-  if (i < d.n) accel_lane<false>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES, a);
+  if (i < d.n) {
+    if (d.ti_on) accel_lane<false, true>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES, a);
+    else accel_lane<false, false>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES, a);
+  }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4(a[0], a[1], a[2], 0.0);
 }
@@ -591,13 +660,18 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
-  const int U = d.stage_cnt[t];
+  // with tensile instability the pass gathers from HBM (the staged records
+  // do not carry the TI stress)
+  const int U = d.ti_on ? -1 : d.stage_cnt[t];
   float a[3] = {0, 0, 0};
   if (U < 0) {
-    if (i < d.n) accel_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
+    if (i < d.n) {
+      if (d.ti_on) accel_lane<true, true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
+      else accel_lane<true, false>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
+    }
   } else {
     stage_accel(d, t, U, d.xhf[cur], lx, lv, lt, lt2);
-    if (i < d.n) accel_lane<true>(d, cur, i, LdsJ{d, lx, lv, lt, lt2, nullptr, i, d.selfl[i]}, sub, LANES, a);
+    if (i < d.n) accel_lane<true, false>(d, cur, i, LdsJ{d, lx, lv, lt, lt2, nullptr, i, d.selfl[i]}, sub, LANES, a);
   }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
@@ -609,6 +683,66 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
     case 2: KERNEL<2><<<nblk(d.n), kBlock * 2, 0, st>>>(__VA_ARGS__); break;  \
     default: KERNEL<4><<<nblk(d.n), kBlock * 4, 0, st>>>(__VA_ARGS__); break; \
   }
+
+// StartAcceleration's tensile-instability stress, 3D branch
+// (Source/Domain.cpp:1529-1563): principal stresses s_k > 0 become
+// -TI s_k / rho^2, the others 0, rotated back with the eigenvectors
+// ((Vec Val) Vec^T).  The eigen-decomposition is cyclic Jacobi (Golub & Van
+// Loan, Alg. 8.5.3) in FP64; the result is a spectral function of sigma, so
+// it does not depend on the eigenvectors' signs or order.
+WF_KERNEL void k_tir(Dev d, int cur) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= d.n) return;
+  const long long np = d.npad;
+  double out[6] = {0, 0, 0, 0, 0, 0};
+  if (d.group[i] >= 0 && d.mats[d.mat[i]].ti > 0.0) {
+    const StressRec r = d.srec[i];
+    double A[3][3] = {{r.s[XX], r.s[XY], r.s[XZ]}, {r.s[XY], r.s[YY], r.s[YZ]}, {r.s[XZ], r.s[YZ], r.s[ZZ]}};
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 64; ++sweep) {
+      const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+      const double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+      if (off == 0.0 || off <= 1e-40 * dia) break;
+      for (int p = 0; p < 2; ++p)
+        for (int q = p + 1; q < 3; ++q) {
+          if (A[p][q] == 0.0) continue;
+          const double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+          const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+          const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+          for (int k = 0; k < 3; ++k) {
+            const double akp = A[k][p], akq = A[k][q];
+            A[k][p] = c * akp - s * akq;
+            A[k][q] = s * akp + c * akq;
+          }
+          for (int k = 0; k < 3; ++k) {
+            const double apk = A[p][k], aqk = A[q][k];
+            A[p][k] = c * apk - s * aqk;
+            A[q][k] = s * apk + c * aqk;
+          }
+          A[p][q] = A[q][p] = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            const double vkp = V[k][p], vkq = V[k][q];
+            V[k][p] = c * vkp - s * vkq;
+            V[k][q] = s * vkp + c * vkq;
+          }
+        }
+    }
+    const double pc = d.mats[d.mat[i]].ti / (r.rho * r.rho);
+    double w[3];
+    for (int k = 0; k < 3; ++k) w[k] = A[k][k] > 0 ? -pc * A[k][k] : 0.0;
+    const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+    for (int m = 0; m < 6; ++m) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += (V[ia[m]][k] * w[k]) * V[ib[m]][k];
+      out[m] = s;
+    }
+  }
+  for (int m = 0; m < 6; ++m) d.tir[(long long)m * np + i] = out[m];
+  if (d.mixed)
+    for (int m = 0; m < 6; ++m) d.tirf[(long long)m * np + i] = (float)out[m];
+}
+
+void launch_tir(const Dev &d, hipStream_t st, int cur) { k_tir<<<nblk(d.n), kBlock, 0, st>>>(d, cur); }
 
 void launch_accel(const Dev &d, hipStream_t st, int cur) {
   if (d.mixed) {
