@@ -102,6 +102,7 @@ class Domain {
   int meshcount = 0;
   double friction_sta = 0.0, friction_dyn = 0.0, DFAC = 0.0, PFAC = 0.6;
   bool cont_heat_cond = false, cont_heat_fric = false;
+  double contact_hc = 0.0;  // contact heat conductance (accepted; contact heat is rejected at solve)
   int contact_alg = 0;  // Wang
   std::vector<Vec3_t> m_contact_force;
   double contact_force_sum = 0.0, contact_reaction_sum = 0.0, ext_forces_work = 0.0;

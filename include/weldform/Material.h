@@ -26,6 +26,9 @@ class Material_ {
   virtual ~Material_() {}
   const Elastic_ &Elastic() const { return elastic_m; }
   virtual Kind kind() const { return Plain; }
+  // yield stress at (strain, strain rate, temperature); the base class
+  // returns 0 (Source/Material.h:98-101)
+  virtual double CalcYieldStress(const double &, const double &, const double &) { return 0.0; }
   double &getT_t() { return T_t; }
   double &getT_m() { return T_m; }
 
@@ -47,6 +50,13 @@ class JohnsonCook : public Material_ {
   }
   Kind kind() const override { return KindJohnsonCook; }
   double &getRefStrainRate() { return eps_0; }
+  // Source/Material.cpp:6-25
+  double CalcYieldStress(const double &strain, const double &strain_rate, const double &temp) override {
+    const double T_h = (temp - T_t) / (T_m - T_t);
+    double f = 1.0;
+    if (strain_rate > eps_0) f = (1.0 + C * std::log(strain_rate / eps_0));
+    return (A + B * std::pow(strain, n)) * f * (1.0 - std::pow(T_h, m));
+  }
   double A = 0, B = 0, C = 0, n = 0, m = 0, eps_0 = 1.0;
 };
 
@@ -64,5 +74,13 @@ class Hollomon : public Material_ {
           "hardening constants.");
   }
   Kind kind() const override { return KindHollomon; }
+  // Source/Material.cpp:59-64.  Hollomon overrides only this one-argument
+  // form, so the three-argument call of the JSON driver reaches the base
+  // class and gives 0 (Source/Material.h:204-224), as in the reference.
+  using Material_::CalcYieldStress;
+  double CalcYieldStress(const double &strain) {
+    if (strain + eps0 > eps1) return K * std::pow(strain + eps0, m);
+    return sy0;
+  }
   double K = 0, m = 1, eps0 = 0, eps1 = 0, sy0 = 0;
 };

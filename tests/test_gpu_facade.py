@@ -111,3 +111,23 @@ def test_facade_driver_adaptive_step(tmp_path):
     print(f"facade adaptive: reaction {e_r:.3e} displacement {e_u:.3e}", flush=True)
     assert e_r < 1e-8 and e_u < 1e-8
     assert os.path.exists(f"{key}_0001.csv") and os.path.exists(f"{key}_0002.csv")
+
+
+def test_json_scene_driver_matches_oracle(tmp_path):
+    """The WeldForm command-line driver on tests/json/upsetting_bilinear.json
+    (config 1 in the reference's scene format: 200 Leapfrog steps) against
+    the oracle's config-1 run."""
+    scene = os.path.join(os.path.dirname(__file__), "json", "upsetting_bilinear.json")
+    if not os.path.exists(wfbuild.JSON_DRIVER):
+        wfbuild.build_json_driver()
+    dump = str(tmp_path / "final.txt")
+    print("json driver: upsetting_bilinear.json", flush=True)
+    r = subprocess.run([wfbuild.JSON_DRIVER, scene, "--out", str(tmp_path / "run"), "--dump", dump],
+                       capture_output=True, text=True, timeout=100, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    final = np.loadtxt(dump)
+    case, d, R = _oracle(200, "leapfrog")
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    print(f"json driver: displacement {e_u:.3e}", flush=True)
+    assert final.shape == (case.n, 4) and e_u < 1e-8
+    assert os.path.exists(str(tmp_path / "run_0001.csv")) and os.path.exists(str(tmp_path / "run_final.csv"))

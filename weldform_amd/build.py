@@ -17,6 +17,8 @@ ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")
 ROOT = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "upsetting.cpp")
 DRIVER = os.path.join(LIBDIR, "upsetting")
+JSON_DRIVER_SRC = os.path.join(HERE, "driver", "weldform.cpp")
+JSON_DRIVER = os.path.join(LIBDIR, "WeldForm")
 
 HIPCC_FLAGS = [
     "--offload-arch=" + ARCH,
@@ -63,19 +65,30 @@ def build(verbose=False):
     return LIB
 
 
-def build_driver(verbose=False):
-    """g++ build of the facade driver: plain C++17, links the C-ABI only."""
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC,
-           "-L" + LIBDIR, "-lweldform_hip", "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"]
+def _gxx(src, out, verbose):
+    """g++ build of a facade program: plain C++17, links the C-ABI only."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), src,
+           "-L" + LIBDIR, "-lweldform_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0 or r.stderr.strip():
         sys.stderr.write(r.stdout + r.stderr)
         if r.returncode != 0:
-            raise RuntimeError("g++ failed on " + DRIVER_SRC)
-    os.replace(DRIVER + ".tmp", DRIVER)
-    return DRIVER
+            raise RuntimeError("g++ failed on " + src)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_driver(verbose=False):
+    """The C++ upsetting driver of the tests (tests/cpp/upsetting.cpp)."""
+    return _gxx(DRIVER_SRC, DRIVER, verbose)
+
+
+def build_json_driver(verbose=False):
+    """The WeldForm command-line driver: JSON scene files of the reference's
+    format on the device path (weldform_amd/driver/weldform.cpp)."""
+    return _gxx(JSON_DRIVER_SRC, JSON_DRIVER, verbose)
 
 
 if __name__ == "__main__":
