@@ -141,4 +141,4 @@ def test_window_loss_autograd():
     with torch.no_grad():
         loss2 = WindowLoss.apply(torch.tensor(scale * (1 + step)), w)
     print("window loss", float(loss.detach()), "after a step", float(loss2), flush=True)
-    assert float(loss2) < float(loss)
+    assert float(loss2) < float(loss.detach())

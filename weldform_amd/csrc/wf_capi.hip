@@ -418,6 +418,7 @@ void allocate(Ctx *c) {
   d.amps = c->damps;
   d.flags = c->alloc<DevFlags>(1);
   d.partials = c->alloc<double>((size_t)d.nblocks * P_COUNT);
+  d.fin = c->alloc<double>(P_COUNT);
   d.scalars = c->alloc<double>(S_COUNT);
   d.hist_cap = 1 << 16;
   d.react_hist = c->alloc<double>((size_t)d.hist_cap * kMaxZones * 3);

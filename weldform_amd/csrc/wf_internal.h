@@ -216,7 +216,8 @@ struct Dev {
   const DevMat *mats;
   const DevAmp *amps;
   DevFlags *flags;
-  double *partials;            // [nblocks][P_COUNT]
+  double *partials;            // [P_COUNT][nblocks]
+  double *fin;                 // [P_COUNT] the reduced slots of the step
   double *scalars;             // [S_COUNT]
   double *react_hist;          // [hist_cap][kMaxZones][3]
   long long hist_cap;

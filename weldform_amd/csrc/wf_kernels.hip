@@ -1326,52 +1326,41 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
 // ---------------------------------------------------------------------------
 // end of step: fixed-order reduction of the block partials, scalar updates,
 // reaction history, and the end-of-step half of the neighbour schedule.
+// One block per active slot sums that slot's partials (each thread its
+// blocks in order, the wavefront butterfly, then the wavefronts in order),
+// so the slots reduce in parallel across CUs; one thread then applies them.
 
-// All active slots are read in one sweep (every load of a thread in flight
-// together) and reduced together; per slot the order is the one of
-// block_reduce: each thread's blocks in order, the wavefront butterfly, then
-// the wavefronts in order.
-__global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
-  if (dt < 0) dt = d.flags->dt;
-  __shared__ double sh[P_COUNT][16];
-  const int nb = d.nblocks;
-  bool active[P_COUNT];
-  int op[P_COUNT];
-  double v[P_COUNT];
-#pragma unroll
-  for (int s = 0; s < P_COUNT; ++s) {
-    active[s] = s == P_INT || s == P_KIN || (s == P_PLW && d.pl_heating) ||
-                ((s == P_MAXT || s == P_MINT) && d.thermal) || (s >= P_REACT && s < P_REACT + 3 * nz) ||
-                (d.contact && ((s >= P_CFORCE && s < P_CFORCE + 3 * d.nmesh) || s == P_CFSUM || s == P_CREACT ||
-                               s == P_EXTW));
-    op[s] = (s == P_MAXT) ? 1 : (s == P_MINT ? 2 : 0);
-    v[s] = op[s] == 0 ? 0.0 : (op[s] == 1 ? -INFINITY : INFINITY);
+struct FinList {
+  int n;
+  int slot[P_COUNT];
+  int op[P_COUNT];  // 0 sum, 1 max, 2 min
+};
+
+__global__ __launch_bounds__(256) void k_fin_sum(Dev d, FinList L) {
+  __shared__ double sh[4];
+  const int s = L.slot[blockIdx.x], op = L.op[blockIdx.x];
+  double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
+  for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) {
+    const double x = part(d, s, b);
+    v = op == 0 ? v + x : (op == 1 ? fmax(v, x) : fmin(v, x));
   }
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-#pragma unroll
-    for (int s = 0; s < P_COUNT; ++s) {
-      if (!active[s]) continue;
-      const double x = part(d, s, b);
-      v[s] = op[s] == 0 ? v[s] + x : (op[s] == 1 ? fmax(v[s], x) : fmin(v[s], x));
-    }
-  }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-#pragma unroll
-  for (int s = 0; s < P_COUNT; ++s) {
-    if (!active[s]) continue;
-    const double r = op[s] == 0 ? wave_sum(v[s]) : (op[s] == 1 ? wave_max(v[s]) : wave_min(v[s]));
-    if (lane == 0) sh[s][w] = r;
-  }
-  __syncthreads();
+  double r;
+  if (op == 0) r = block_reduce<0>(v, sh);
+  else if (op == 1) r = block_reduce<1>(v, sh);
+  else r = block_reduce<2>(v, sh);
+  if (threadIdx.x == 0) d.fin[s] = r;
+}
+
+__global__ __launch_bounds__(64) void k_fin_apply(Dev d, double dt, FinList L) {
   if (threadIdx.x != 0) return;
-  const long long step = d.flags->step;
+  if (dt < 0) dt = d.flags->dt;
+  DevFlags *f = d.flags;
+  const long long step = f->step;
   const long long slot = step % d.hist_cap;
   double e_kin = 0.0, e_int = 0.0, plw = 0.0, tmax = 0.0, tmin = 0.0, cfsum = 0.0, creact = 0.0, extw = 0.0;
-#pragma unroll
-  for (int s = 0; s < P_COUNT; ++s) {
-    if (!active[s]) continue;
-    double r = sh[s][0];
-    for (int k = 1; k < nw; ++k) r = op[s] == 0 ? r + sh[s][k] : (op[s] == 1 ? fmax(r, sh[s][k]) : fmin(r, sh[s][k]));
+  for (int k = 0; k < L.n; ++k) {
+    const int s = L.slot[k];
+    const double r = d.fin[s];
     if (s == P_KIN) e_kin = r;
     if (s == P_INT) e_int = r;
     if (s == P_PLW) plw = r;
@@ -1395,8 +1384,7 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
     d.scalars[S_MAXT] = fmax(0.0, tmax);
     d.scalars[S_MINT] = fmin(1000.0, tmin);
   }
-  if (!isfinite(e_int)) d.flags->nan = 1;
-  DevFlags *f = d.flags;
+  if (!isfinite(e_int)) f->nan = 1;
   if (f->step_yield) {
     if (f->ts_i == f->ts_nb_inc - 1) f->cleared = 1;
     f->ts_i++;
@@ -1407,7 +1395,28 @@ __global__ __launch_bounds__(1024) void k_finalize(Dev d, double dt, int nz) {
 }
 
 void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
-  k_finalize<<<1, 1024, 0, st>>>(d, dt, nz);
+  FinList L{};
+  auto add = [&](int s, int op) {
+    L.slot[L.n] = s;
+    L.op[L.n] = op;
+    ++L.n;
+  };
+  add(P_INT, 0);
+  add(P_KIN, 0);
+  if (d.pl_heating) add(P_PLW, 0);
+  if (d.thermal) {
+    add(P_MAXT, 1);
+    add(P_MINT, 2);
+  }
+  for (int s = P_REACT; s < P_REACT + 3 * nz; ++s) add(s, 0);
+  if (d.contact) {
+    for (int s = P_CFORCE; s < P_CFORCE + 3 * d.nmesh; ++s) add(s, 0);
+    add(P_CFSUM, 0);
+    add(P_CREACT, 0);
+    add(P_EXTW, 0);
+  }
+  k_fin_sum<<<L.n, 256, 0, st>>>(d, L);
+  k_fin_apply<<<1, 64, 0, st>>>(d, dt, L);
 }
 
 // ---------------------------------------------------------------------------
