@@ -205,7 +205,7 @@ def batch_run(steps, n_sample=16):
     case = cases.config4(steps=steps)
     log(f"bench: {case.name} n={case.n} steps={steps}")
     best, table = autotune.autotune(case, precision=capi.MIXED, steps=10, warmup=3, lanes=(1, 2, 4),
-                                    stages=(-1, 1024), log=log)
+                                    stages=(-1, 0), log=log)
     w = cases.make_solver(case, precision=capi.MIXED, **best)
     w.step(5)
     w.sync()

@@ -69,11 +69,11 @@ def test_batch_matches_oracle_and_lone_specimens(batch):
 
 def test_autotune_shapes_agree(batch):
     c, _ = batch
-    best, table = autotune.autotune(c, precision=capi.MIXED, steps=5, warmup=2, lanes=(2,), stages=(-1, 512, 1024))
+    best, table = autotune.autotune(c, precision=capi.MIXED, steps=5, warmup=2, lanes=(2,), stages=(-1, 512, 0))
     print("autotune", best, table, flush=True)
     assert best in [{"lanes": r["lanes"], "stage_max": r["stage_max"]} for r in table]
     out = []
-    for st in (-1, 512, 1024):
+    for st in (-1, 512, 0):
         g = cases.make_solver(c, precision=capi.MIXED, lanes=2, stage_max=st)
         g.step(10)
         g.sync()

@@ -4,8 +4,9 @@ The pair passes have two shape knobs that leave the physics alone: the
 threads per particle (`lanes`: 1, 2 or 4; partial sums are combined in lane
 order, so a choice fixes the rounding of the sums but no choice is more
 exact than another) and, in mixed precision, the LDS staging capacity per
-128-particle tile (`stage_max`: -1 gathers from HBM, otherwise the largest
-staged set; results are bitwise the same for every value).  `autotune`
+128-particle tile (`stage_max`: -1 gathers from HBM, 0 the largest capacity,
+otherwise the largest staged set; results are bitwise the same for every
+value).  `autotune`
 times a short run of the case for each candidate shape, after a warm-up that
 covers the first neighbour search, and returns the fastest.
 """
@@ -14,7 +15,7 @@ import time
 from . import capi, cases
 
 LANES = (1, 2, 4)
-STAGES = (-1, 512, 1024)
+STAGES = (-1, 512, 0)
 
 
 def autotune(case, precision=capi.MIXED, steps=20, warmup=5, lanes=LANES, stages=STAGES, log=None):

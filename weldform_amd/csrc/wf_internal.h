@@ -29,7 +29,11 @@ constexpr int kBlock = 128;
 // LDS staging of the mixed-precision pair passes: a tile is one block of
 // kBlock consecutive particles; its staged set (the tile's particles and all
 // their neighbours, sorted by index) holds at most kStageMax particles.
-constexpr int kStageMax = 1024;
+// 896 keeps 99.7 % of the tiles of a cubic lattice with 2h = 2.4 dx staged
+// (sets of 622-932, median 733) while three acceleration blocks, four rate
+// blocks and five density blocks fit a CU's 160 KB of LDS.
+constexpr int kStageMax = 896;
+constexpr int kStagePow2 = 1024;  // the power of two >= kStageMax the set sort pads to
 constexpr int kSortCap = 8192;   // neighbour entries a tile may list in total
 
 // neighbour rows are read in batches of kNbBatch entries, the next batch

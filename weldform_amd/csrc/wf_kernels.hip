@@ -285,13 +285,15 @@ __device__ inline int lower_pos(const int *a, int n, int v) {
   return lo;
 }
 
-constexpr int kHashSlots = 2 * kStageMax;  // load <= 1/2 for sets of <= kStageMax values
+constexpr int kHashSlots = 2048;  // >= 2 kStageMax: load <= 1/2 for sets of <= kStageMax values
+static_assert(kHashSlots >= 2 * kStageMax && (kHashSlots & (kHashSlots - 1)) == 0, "hash table size");
+static_assert(kStagePow2 >= kStageMax && (kStagePow2 & (kStagePow2 - 1)) == 0, "sort padding");
 __device__ inline unsigned stage_hash(int v) { return ((unsigned)v * 2654435761u) >> 21; }  // 11 bits
 
 WF_KERNEL void k_stage(Dev d, int stage) {
   if (!d.flags->do_search) return;
   __shared__ int buf[kSortCap];
-  __shared__ int uniq[kStageMax];
+  __shared__ int uniq[kStagePow2];
   __shared__ int ht[kHashSlots];
   __shared__ int nset, full;
   __shared__ int sh[34];
