@@ -249,6 +249,50 @@ def batch_run(steps, n_sample=16):
     return res
 
 
+def output_run(s, n, reps=3):
+    """The reference's output files of the 976k state (Domain::WriteXDMF and
+    Domain::WriteCSV): the tables packed on the device and copied to the
+    host in one transfer each (best of `reps`), then written to a temporary
+    directory (XDMF/HDF5 when the HDF5 library is present, CSV with 16
+    formatting threads)."""
+    import shutil
+    import tempfile
+
+    from weldform_amd import capi, output
+
+    s.sync()
+    res = {"particles": n, "xdmf_table_bytes": (capi.XDMF_NF + capi.XDMF_NI) * 4 * n,
+           "csv_table_bytes": capi.CSV_NCOL * 8 * n}
+    best = {}
+    for name, fn in (("pack_xdmf", s.pack_xdmf), ("pack_csv", s.pack_csv)):
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            tab = fn()
+            dt = time.perf_counter() - t0
+            best[name] = min(best.get(name, dt), dt)
+        if name == "pack_xdmf":
+            f, iv = tab
+        else:
+            rows = tab
+    res["pack_xdmf_ms"] = 1e3 * best["pack_xdmf"]
+    res["pack_csv_ms"] = 1e3 * best["pack_csv"]
+    res["pack_xdmf_GBps_host_inclusive"] = res["xdmf_table_bytes"] / best["pack_xdmf"] / 1e9
+    d = tempfile.mkdtemp(prefix="wf_out_")
+    try:
+        if output.has_hdf5():
+            t0 = time.perf_counter()
+            output.write_xdmf_table(os.path.join(d, "bench_0001"), n, f, iv)
+            res["write_xdmf_ms"] = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        output.write_csv_rows(os.path.join(d, "bench_0001"), rows, threads=16)
+        res["write_csv_ms"] = 1e3 * (time.perf_counter() - t0)
+        res["file_bytes"] = {fn: os.path.getsize(os.path.join(d, fn)) for fn in sorted(os.listdir(d))}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    log("bench: output " + json.dumps(res))
+    return res
+
+
 def cpu_baseline(case, steps):
     sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
     from oracle_case import oracle_from_case
@@ -274,6 +318,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-output", action="store_true", help="skip the output-file leg")
     ap.add_argument("--config3-steps", type=int, default=400,
                     help="steps of the thermo-mechanical contact case reported beside the headline (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=500,
@@ -385,6 +430,8 @@ def main():
                     "(profiles/traffic_cyl_upset_976k.md)",
         },
     }
+    if not args.no_output:
+        out["config"]["output_976k"] = output_run(mx["solver"], case.n)
     if args.config3_steps > 0:
         out["config"]["config3_contact_thermal"] = contact_run(args.config3_steps)
     if args.batch_steps > 0:

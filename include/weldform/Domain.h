@@ -16,8 +16,8 @@
 // callback in GeneralBefore / GeneralAfter cannot run inside a device step
 // and is rejected.  Errors are thrown as Fatal*, as in the reference.
 //
-// Link with -lweldform_hip (weldform_amd/lib); compiles with any C++17
-// compiler, no HIP headers needed.
+// Link with -lweldform_hip -lweldform_io (weldform_amd/lib); compiles with
+// any C++17 compiler, no HIP headers needed.
 #pragma once
 #include <algorithm>
 #include <cmath>
@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../weldform_hip.h"
+#include "../weldform_io.h"
 #include "Material.h"
 #include "Mesh.h"
 #include "Particle.h"
@@ -270,23 +271,54 @@ class Domain {
   }
   const std::vector<Vec3_t> &ZoneReactionHistory(int zone) { return reaction_[zone]; }
 
-  // XDMF/HDF5 output is not built; the particle table goes to key.csv instead
-  void WriteXDMF(char const *FileKey) { WriteCSV(FileKey); }
+  // Domain::OutputName (Source/Domain.h:449): names of the three UserOutput
+  // datasets (their values are the default UserOutput's 0)
+  std::string OutputName[3] = {"Property1", "Property2", "Property3"};
 
-  // WriteCSV (Source/Output.cpp:521-554): same columns
+  // WriteXDMF (Source/Output.cpp:75-518): FileKey.hdf5 + FileKey.xmf of every
+  // particle (solid, then rigid-surface), from the Particle objects
+  void WriteXDMF(char const *FileKey) {
+    const size_t N = Particles.Size();
+    std::vector<float> f((size_t)WF_XDMF_NF * N);
+    std::vector<int32_t> iv((size_t)WF_XDMF_NI * N);
+    host_xdmf(0, N, N, f.data(), iv.data());
+    write_xdmf(FileKey, N, f, iv);
+  }
+
+  // WriteCSV (Source/Output.cpp:521-554), from the Particle objects
   void WriteCSV(char const *FileKey) {
-    std::ostringstream oss;
-    oss << "X, Y, Z, ID, Sigma_eq, Pl_Strain, vx, vy, vz, ax, ay, az, CFx, CFy, CZFz, p, Temp" << std::endl;
-    for (size_t i = 0; i < Particles.Size(); ++i) {
-      Particle *P = Particles[i];
-      for (int j = 0; j < 3; ++j) oss << P->x(j) << ", ";
-      P->CalculateEquivalentStress();
-      oss << P->ID << ", " << P->Sigma_eq << ", " << P->pl_strain << ", " << P->v(0) << ", " << P->v(1) << ", "
-          << P->v(2) << ", " << P->a(0) << ", " << P->a(1) << ", " << P->a(2) << ", " << P->contforce(0) << ", "
-          << P->contforce(1) << ", " << P->contforce(2) << ", " << P->Pressure << ", " << P->T << std::endl;
+    std::vector<double> rows((size_t)WF_CSV_NCOL * Particles.Size());
+    host_csv(0, Particles.Size(), rows.data());
+    write_csv(FileKey, Particles.Size(), rows);
+  }
+
+  // the same two files of the device state during a solve: the solid
+  // particles' tables are packed on the device (wf_pack_xdmf / wf_pack_csv),
+  // the rigid-surface particles' rows come from their Particle objects
+  // after their positions and velocities are pulled
+  void WriteDeviceOutput(char const *FileKey, bool xdmf, bool csv) {
+    if (!ctx_) throw new Fatal("WriteDeviceOutput: no device state (call inside or after a solve)");
+    const size_t n = nsolid(), N = Particles.Size();
+    pull_rigid();
+    if (xdmf) {
+      std::vector<float> fd((size_t)WF_XDMF_NF * n), f((size_t)WF_XDMF_NF * N);
+      std::vector<int32_t> ivd((size_t)WF_XDMF_NI * n), iv((size_t)WF_XDMF_NI * N);
+      check(wf_pack_xdmf(ctx_, fd.data(), ivd.data()));
+      for (int d = 0; d < WF_XDMF_NF;) {
+        const int w = xdmf_width(d);
+        std::memcpy(&f[(size_t)d * N], &fd[(size_t)d * n], sizeof(float) * w * n);
+        d += w;
+      }
+      for (int d = 0; d < WF_XDMF_NI; ++d) std::memcpy(&iv[(size_t)d * N], &ivd[(size_t)d * n], sizeof(int32_t) * n);
+      host_xdmf(n, N, N, f.data(), iv.data());
+      write_xdmf(FileKey, N, f, iv);
     }
-    std::ofstream of(std::string(FileKey) + ".csv", std::ios::out);
-    of << oss.str();
+    if (csv) {
+      std::vector<double> rows((size_t)WF_CSV_NCOL * N);
+      check(wf_pack_csv(ctx_, rows.data()));
+      host_csv(n, N, rows.data() + (size_t)WF_CSV_NCOL * n);
+      write_csv(FileKey, N, rows);
+    }
   }
 
  private:
@@ -495,21 +527,14 @@ class Domain {
       std::vector<int32_t> id(n);
       check(wf_get_int_field(ctx_, WF_I_ID, id.data()));
       for (size_t i = 0; i < n; ++i) Particles[i]->ID = id[i];
-      const size_t nr = Particles.Size() - n;
-      std::vector<double> r3(3 * nr);
-      auto rig = [&](int f, Vec3_t Particle::*m) {
-        check(wf_get_rigid_field(ctx_, f, r3.data()));
-        for (size_t r = 0; r < nr; ++r)
-          for (int k = 0; k < 3; ++k) (Particles[n + r]->*m)(k) = r3[3 * r + k];
-      };
-      rig(WF_R_X, &Particle::x);
-      rig(WF_R_V, &Particle::v);
-      rig(WF_R_NORMAL, &Particle::normal);
+      pull_rigid();
     }
-    std::vector<int32_t> nb(n);
+    std::vector<int32_t> nb(n), cnb(n);
     check(wf_get_int_field(ctx_, WF_I_NB, nb.data()));
+    check(wf_get_int_field(ctx_, WF_I_CONTNB, cnb.data()));
     for (size_t i = 0; i < n; ++i) {
       Particles[i]->Nb = nb[i];
+      Particles[i]->ContNb = cnb[i];
       Particles[i]->CalculateEquivalentStress();
     }
     wf_scalars s{};
@@ -522,6 +547,95 @@ class Domain {
     plastic_work = s.plastic_work;
     m_maxT = s.max_T;
     m_minT = s.min_T;
+  }
+
+  // width of the float dataset starting at offset d of the XDMF table
+  static int xdmf_width(int d) {
+    switch (d) {
+      case WF_XDMF_POSITION: case WF_XDMF_VELOCITY: case WF_XDMF_ACCELERATION:
+      case WF_XDMF_DISPLACEMENT: case WF_XDMF_CONTACT_FORCE: return 3;
+      case WF_XDMF_SIGMA: case WF_XDMF_SHEARS: case WF_XDMF_STRAIN: case WF_XDMF_STRAIN_RATE:
+      case WF_XDMF_STRAIN_PL: case WF_XDMF_GRADCORRMAT: return 6;
+      default: return 1;
+    }
+  }
+
+  // rows [i0, i1) of the XDMF table of N particles from the Particle objects
+  // (Source/Output.cpp:113-215)
+  void host_xdmf(size_t i0, size_t i1, size_t N, float *f, int32_t *iv) {
+    auto at = [&](int d, size_t i) { return f + (size_t)d * N + (size_t)xdmf_width(d) * i; };
+    auto t6 = [](float *o, const Mat3_t &m) {
+      o[0] = (float)m(0, 0), o[1] = (float)m(1, 1), o[2] = (float)m(2, 2);
+      o[3] = (float)m(0, 1), o[4] = (float)m(1, 2), o[5] = (float)m(0, 2);
+    };
+    for (size_t i = i0; i < i1; ++i) {
+      Particle *P = Particles[i];
+      P->CalculateEquivalentStress();
+      for (int k = 0; k < 3; ++k) {
+        at(WF_XDMF_POSITION, i)[k] = (float)P->x(k);
+        at(WF_XDMF_VELOCITY, i)[k] = (float)P->v(k);
+        at(WF_XDMF_ACCELERATION, i)[k] = (float)P->a(k);
+        at(WF_XDMF_DISPLACEMENT, i)[k] = (float)P->Displacement(k);
+        at(WF_XDMF_CONTACT_FORCE, i)[k] = (float)P->contforce(k);
+      }
+      *at(WF_XDMF_PRESSURE, i) = (float)P->Pressure;
+      *at(WF_XDMF_DENSITY, i) = (float)P->Density;
+      *at(WF_XDMF_PROPERTY1, i) = *at(WF_XDMF_PROPERTY2, i) = *at(WF_XDMF_PROPERTY3, i) = 0.f;
+      *at(WF_XDMF_MASS, i) = (float)P->Mass;
+      *at(WF_XDMF_H, i) = (float)P->h;
+      t6(at(WF_XDMF_SIGMA, i), P->Sigma);
+      t6(at(WF_XDMF_SHEARS, i), P->Sigma);  // the reference writes Sigma under ShearS
+      t6(at(WF_XDMF_STRAIN, i), P->Strain);
+      t6(at(WF_XDMF_STRAIN_RATE, i), P->StrainRate);
+      t6(at(WF_XDMF_STRAIN_PL, i), P->Strain_pl);
+      t6(at(WF_XDMF_GRADCORRMAT, i), P->gradCorrM);
+      *at(WF_XDMF_SIGMA_EQ, i) = (float)P->Sigma_eq;
+      *at(WF_XDMF_TEMPERATURE, i) = (float)P->T;
+      *at(WF_XDMF_PL_STRAIN, i) = (float)P->pl_strain;
+      *at(WF_XDMF_Q_FRICTION, i) = (float)P->friction_hfl;
+      *at(WF_XDMF_C_SHEARABS, i) = (float)P->cshearabs;
+      *at(WF_XDMF_PS_EN, i) = (float)P->ps_energy;
+      iv[(size_t)WF_XDMF_TAG * N + i] = P->ID;
+      iv[(size_t)WF_XDMF_NEIGHBORS * N + i] = P->Nb;
+      iv[(size_t)WF_XDMF_CONTNEIB * N + i] = P->ContNb;
+    }
+  }
+
+  // rows [i0, i1) of the CSV table from the Particle objects
+  void host_csv(size_t i0, size_t i1, double *rows) {
+    for (size_t i = i0; i < i1; ++i) {
+      Particle *P = Particles[i];
+      P->CalculateEquivalentStress();
+      double *r = rows + (size_t)WF_CSV_NCOL * (i - i0);
+      const double v[WF_CSV_NCOL] = {P->x(0), P->x(1), P->x(2), (double)P->ID, P->Sigma_eq, P->pl_strain,
+                                     P->v(0), P->v(1), P->v(2), P->a(0), P->a(1), P->a(2),
+                                     P->contforce(0), P->contforce(1), P->contforce(2), P->Pressure, P->T};
+      std::memcpy(r, v, sizeof v);
+    }
+  }
+
+  void write_xdmf(char const *key, size_t N, const std::vector<float> &f, const std::vector<int32_t> &iv) {
+    const char *names[3] = {OutputName[0].c_str(), OutputName[1].c_str(), OutputName[2].c_str()};
+    if (wfio_write_xdmf(key, (int64_t)N, f.data(), iv.data(), names) != 0) throw new Fatal(wfio_last_error());
+  }
+  void write_csv(char const *key, size_t N, const std::vector<double> &rows) {
+    const int threads = Nproc > 0 ? (int)Nproc : 1;
+    if (wfio_write_csv(key, (int64_t)N, rows.data(), threads) != 0) throw new Fatal(wfio_last_error());
+  }
+
+  // positions and velocities of the rigid-surface particles
+  void pull_rigid() {
+    const size_t n = nsolid(), nr = Particles.Size() - n;
+    if (!contact || nr == 0) return;
+    std::vector<double> r3(3 * nr);
+    auto rig = [&](int f, Vec3_t Particle::*m) {
+      check(wf_get_rigid_field(ctx_, f, r3.data()));
+      for (size_t r = 0; r < nr; ++r)
+        for (int k = 0; k < 3; ++k) (Particles[n + r]->*m)(k) = r3[3 * r + k];
+    };
+    rig(WF_R_X, &Particle::x);
+    rig(WF_R_V, &Particle::v);
+    rig(WF_R_NORMAL, &Particle::normal);
   }
 
   void pull_reactions(int64_t nsteps) {
@@ -598,6 +712,10 @@ class Domain {
     if (contact) check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));
     deltat = dt;
     check(wf_init(ctx_, dt));
+    if (key && wfio_has_hdf5()) {
+      // the initial model (Source/SolverLeapfrog.cpp:57-63)
+      WriteDeviceOutput((std::string(key) + "_Initial").c_str(), true, false);
+    }
     reaction_.clear();
     cforce_hist_.clear();
     size_t idx_out = 1;
@@ -642,10 +760,10 @@ class Domain {
       }
       if (out) {
         if (key) {
-          download();
+          // SolverLeapfrog.cpp:364-373: FileKey_%04d.{hdf5,xmf,csv}
           char fn[4096];
           std::snprintf(fn, sizeof fn, "%s_%04zu", key, idx_out);
-          WriteCSV(fn);
+          WriteDeviceOutput(fn, wfio_has_hdf5() != 0, true);
         }
         idx_out++;
         tout += dtOut;

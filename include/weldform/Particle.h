@@ -43,6 +43,9 @@ class Particle {
   double q_source = 0.0, q_conv = 0.0, T_inf = 0.0, h_conv = 0.0, q_plheat = 0.0, ps_energy = 0.0;
   int Thermal_BC = TH_BC_NONE;
   int Nb = 0;
+  int ContNb = 0;             // contact pairs of the last contact search
+  Mat3_t gradCorrM;           // kernel-gradient correction (not formed here: 0)
+  double friction_hfl = 0.0, cshearabs = 0.0;  // friction heating outputs (0 here)
   bool print_history = false;
   bool Shepard = false;  // density filter flag; unused on the device path
 

@@ -34,6 +34,9 @@
  *   wf_eval(WF_EVAL_STRESS)  <- Particle::CalcStressStrain for every particle
  *                               (Source/Particle.cpp:960-1067)
  *   wf_gen_cylinder          <- Domain::AddCylinderLength (Source/Domain.cpp:708-860)
+ *   wf_pack_xdmf / wf_pack_csv <- the particle tables of Domain::WriteXDMF /
+ *                               Domain::WriteCSV (Source/Output.cpp:75-554); the
+ *                               files themselves: weldform_io.h
  *   wf_adj_*                 <- no counterpart in the reference: the reverse-mode
  *                               adjoint of the FP64 Leapfrog step for parameter
  *                               calibration (see the block above wf_adj_forward)
@@ -46,7 +49,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 5
+#define WF_ABI_VERSION 6
 
 /* error codes */
 #define WF_OK 0
@@ -119,7 +122,9 @@ extern "C" {
 #define WF_I_FREE 2         /* Particle::IsFree (only free-free pairs)        */
 #define WF_I_THERMAL_BC 3   /* Particle::Thermal_BC (1 = convection)          */
 #define WF_I_NB 4           /* read-only: Particle::Nb (neighbour count)      */
-#define WF_NIFIELDS 5
+#define WF_I_CONTNB 5       /* read-only: Particle::ContNb (contact pairs of
+                               the last contact search; 0 without contact)  */
+#define WF_NIFIELDS 6
 
 /* single-pass evaluations (wf_eval) */
 #define WF_EVAL_ACCEL 0     /* a   <- sum over pairs, from x, v, Density, Sigma */
@@ -139,7 +144,7 @@ typedef struct wf_config {
                                a captured HIP graph per step                  */
   int32_t stage_max;        /* WF_MIXED: neighbours of each 128-particle tile are
                                staged in LDS when the tile's neighbourhood has
-                               at most this many particles (0: 1024, the most;
+                               at most this many particles (0: 896, the most;
                                -1: never, gather from HBM).  Results do not
                                depend on it.                                  */
   int32_t lanes;            /* threads per particle in the pair passes: 1, 2
@@ -262,6 +267,59 @@ int wf_eval(void *ctx, int32_t what, double dt);
 int wf_set_timing(void *ctx, int32_t enable);
 int wf_get_kernel_time(void *ctx, int32_t kernel, double *avg_ms, int64_t *launches);
 int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_count);
+
+/* output tables, packed on the device in the caller's particle order and
+ * copied to the host in one transfer (the first call allocates a device
+ * staging buffer of WF_XDMF_NF * 4 + WF_XDMF_NI * 4 bytes per particle,
+ * kept until wf_destroy).
+ *
+ * wf_pack_xdmf <- the dataset packing of Domain::WriteXDMF
+ * (Source/Output.cpp:83-215): f receives WF_XDMF_NF * n floats, dataset
+ * after dataset, dataset D of width w at f[D * n], particle i's values at
+ * f[D * n + w * i ...]; iv receives WF_XDMF_NI * n ints the same way.
+ * Tensors of width 6 are (00, 11, 22, 01, 12, 02).  As in the reference,
+ * ShearS holds Sigma (Source/Output.cpp:250-251), Property1..3 hold the
+ * default UserOutput's 0, Sigma_eq is CalculateEquivalentStress of
+ * ShearStress.  gradcorrmat, q_friction and c_shearabs are 0: the
+ * kernel-gradient correction and the contact's friction heating that would
+ * set them are not run on the device path. */
+#define WF_XDMF_POSITION 0        /* width 3 */
+#define WF_XDMF_VELOCITY 3        /* width 3 */
+#define WF_XDMF_ACCELERATION 6    /* width 3 */
+#define WF_XDMF_PRESSURE 9
+#define WF_XDMF_DENSITY 10
+#define WF_XDMF_PROPERTY1 11
+#define WF_XDMF_PROPERTY2 12
+#define WF_XDMF_PROPERTY3 13
+#define WF_XDMF_MASS 14
+#define WF_XDMF_H 15
+#define WF_XDMF_SIGMA 16          /* width 6 */
+#define WF_XDMF_SHEARS 22         /* width 6 */
+#define WF_XDMF_STRAIN 28         /* width 6 */
+#define WF_XDMF_STRAIN_RATE 34    /* width 6 */
+#define WF_XDMF_STRAIN_PL 40      /* width 6 */
+#define WF_XDMF_GRADCORRMAT 46    /* width 6 */
+#define WF_XDMF_SIGMA_EQ 52
+#define WF_XDMF_TEMPERATURE 53
+#define WF_XDMF_PL_STRAIN 54
+#define WF_XDMF_Q_FRICTION 55
+#define WF_XDMF_C_SHEARABS 56
+#define WF_XDMF_PS_EN 57
+#define WF_XDMF_DISPLACEMENT 58   /* width 3 */
+#define WF_XDMF_CONTACT_FORCE 61  /* width 3 */
+#define WF_XDMF_NF 64
+#define WF_XDMF_TAG 0             /* int datasets */
+#define WF_XDMF_NEIGHBORS 1
+#define WF_XDMF_CONTNEIB 2
+#define WF_XDMF_NI 3
+int wf_pack_xdmf(void *ctx, float *f, int32_t *iv);
+
+/* wf_pack_csv <- the rows of Domain::WriteCSV (Source/Output.cpp:521-554):
+ * rows receives WF_CSV_NCOL doubles per particle, particle after particle:
+ * X, Y, Z, ID, Sigma_eq, Pl_Strain, vx, vy, vz, ax, ay, az, CFx, CFy, CFz,
+ * p, Temp. */
+#define WF_CSV_NCOL 17
+int wf_pack_csv(void *ctx, double *rows);
 
 /* adjoint (no counterpart in the reference).  A window of cfg->steps FP64
  * Leapfrog steps from the current state; the loss is

@@ -107,7 +107,7 @@ def _facade_host_output(tmp_path):
         wfbuild.build()
     r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror",
                         "-I" + os.path.join(wfbuild.ROOT, "include"), src, "-L" + wfbuild.LIBDIR,
-                        "-lweldform_hip", "-Wl,-rpath," + wfbuild.LIBDIR, "-o", exe],
+                        "-lweldform_hip", "-lweldform_io", "-Wl,-rpath," + wfbuild.LIBDIR, "-o", exe],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
@@ -193,7 +193,7 @@ int main() {
     cpp.write_text(code)
     exe = str(tmp_path / "hcp")
     rr = subprocess.run(["g++", "-std=c++17", "-I" + os.path.join(wfbuild.ROOT, "include"), str(cpp),
-                         "-L" + wfbuild.LIBDIR, "-lweldform_hip", "-Wl,-rpath," + wfbuild.LIBDIR, "-o", exe],
+                         "-L" + wfbuild.LIBDIR, "-lweldform_hip", "-lweldform_io", "-Wl,-rpath," + wfbuild.LIBDIR, "-o", exe],
                         capture_output=True, text=True)
     assert rr.returncode == 0, rr.stderr
     x = np.loadtxt(subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines())

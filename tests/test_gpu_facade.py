@@ -13,9 +13,10 @@ import subprocess
 import numpy as np
 import pytest
 
+import h5read
 from oracle_case import normwise_err, oracle_from_case
 from weldform_amd import build as wfbuild
-from weldform_amd import cases
+from weldform_amd import cases, output
 
 pytestmark = pytest.mark.gpu
 
@@ -130,4 +131,12 @@ def test_json_scene_driver_matches_oracle(tmp_path):
     e_u = normwise_err(final[:, :3], d.f("DISP"))
     print(f"json driver: displacement {e_u:.3e}", flush=True)
     assert final.shape == (case.n, 4) and e_u < 1e-8
-    assert os.path.exists(str(tmp_path / "run_0001.csv")) and os.path.exists(str(tmp_path / "run_final.csv"))
+    assert os.path.exists(str(tmp_path / "run_0001.csv"))
+    if output.has_hdf5():
+        # the solve's XDMF files (initial model, each output) and the final
+        # state from the Particle objects, maz.hdf5 (Source/WeldForm.cpp:941)
+        for k in ("run_Initial", "run_0001", "maz"):
+            assert os.path.exists(str(tmp_path / (k + ".xmf"))), k
+        ds = h5read.read(str(tmp_path / "maz.hdf5"))
+        assert ds["NP"][1][0] == case.n
+        assert np.array_equal(ds["Displacement"][1].reshape(-1, 3), final[:, :3].astype(np.float32))

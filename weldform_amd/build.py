@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libweldform_hip.so")
-SOURCES = ["wf_kernels.hip", "wf_adjoint.hip", "wf_capi.hip"]
+SOURCES = ["wf_kernels.hip", "wf_adjoint.hip", "wf_output.hip", "wf_capi.hip"]
 ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")
 ROOT = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "upsetting.cpp")
@@ -65,10 +65,46 @@ def build(verbose=False):
     return LIB
 
 
+IO_SRC = os.path.join(HERE, "io", "wf_io.cpp")
+IO_LIB = os.path.join(LIBDIR, "libweldform_io.so")
+# the HDF5 library (C + high-level API) the XDMF writer links when present
+HDF5_PREFIXES = [p for p in (os.environ.get("WF_HDF5_PREFIX"), "/opt/conda", "/usr") if p]
+
+
+def hdf5_prefix():
+    for p in HDF5_PREFIXES:
+        inc = os.path.join(p, "include", "hdf5.h")
+        for lib in ("lib", "lib64", "lib/x86_64-linux-gnu"):
+            if os.path.exists(inc) and os.path.exists(os.path.join(p, lib, "libhdf5_hl.so")):
+                return p, os.path.join(p, lib)
+    return None, None
+
+
+def build_io(verbose=False):
+    """libweldform_io.so: the XDMF/HDF5 and CSV writers (host C++); built
+    without the XDMF writer when no HDF5 library is found."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-shared", "-fPIC", "-pthread", IO_SRC]
+    prefix, libdir = hdf5_prefix()
+    if prefix:
+        cmd += ["-DWFIO_HDF5", "-I" + os.path.join(prefix, "include"), "-L" + libdir, "-lhdf5_hl", "-lhdf5",
+                "-Wl,-rpath," + libdir]
+    cmd += ["-o", IO_LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0 or r.stderr.strip():
+        sys.stderr.write(r.stdout + r.stderr)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed on " + IO_SRC)
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    return IO_LIB
+
+
 def _gxx(src, out, verbose):
     """g++ build of a facade program: plain C++17, links the C-ABI only."""
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"), src,
-           "-L" + LIBDIR, "-lweldform_hip", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
+           "-L" + LIBDIR, "-lweldform_hip", "-lweldform_io", "-Wl,-rpath,$ORIGIN", "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -93,4 +129,6 @@ def build_json_driver(verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_io(verbose=True))
     print(build_driver(verbose=True))
+    print(build_json_driver(verbose=True))

@@ -30,7 +30,7 @@ FIELDS = {
     "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1), "contforce": (28, 3),
 }
 RIGID_FIELDS = {"x": 0, "v": 1, "normal": 2}
-INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4}
+INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4, "contnb": 5}
 EVAL = {"accel": 0, "density": 1, "rates": 2, "stress": 3}
 KERNELS = {"accel": 0, "density": 1, "rates": 2, "neighbours": 3, "step": 4}
 
@@ -92,7 +92,12 @@ EXPORTS = [
     "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats",
     "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
+    "wf_pack_xdmf", "wf_pack_csv",
 ]
+
+# output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
+# CSV columns
+XDMF_NF, XDMF_NI, CSV_NCOL = 64, 3, 17
 
 _lib = None
 
@@ -143,6 +148,8 @@ def lib():
         "wf_adj_backward": ([vp, dp], ctypes.c_int),
         "wf_adj_get_field": ([vp, i32, dp], ctypes.c_int),
         "wf_adj_restore": ([vp], ctypes.c_int),
+        "wf_pack_xdmf": ([vp, ctypes.POINTER(ctypes.c_float), ip], ctypes.c_int),
+        "wf_pack_csv": ([vp, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -321,6 +328,21 @@ class Solver:
 
     def adj_restore(self):
         self._check(lib().wf_adj_restore(self.ctx))
+
+    # output tables ----------------------------------------------------------
+    def pack_xdmf(self):
+        """Domain::WriteXDMF's datasets packed on the device: (f, iv), float32
+        [XDMF_NF * n] and int32 [XDMF_NI * n], dataset after dataset."""
+        f = np.empty(XDMF_NF * self.n, dtype=np.float32)
+        iv = np.empty(XDMF_NI * self.n, dtype=np.int32)
+        self._check(lib().wf_pack_xdmf(self.ctx, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ip(iv)))
+        return f, iv
+
+    def pack_csv(self):
+        """Domain::WriteCSV's rows packed on the device, shape (n, CSV_NCOL)."""
+        rows = np.empty(CSV_NCOL * self.n)
+        self._check(lib().wf_pack_csv(self.ctx, _dp(rows)))
+        return rows.reshape(self.n, CSV_NCOL)
 
     def rebuild_neighbours(self):
         self._check(lib().wf_rebuild_neighbours(self.ctx))

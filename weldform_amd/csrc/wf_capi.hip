@@ -135,6 +135,9 @@ struct Ctx {
     bool have_grad = false;
     std::vector<void *> mem;
   } adj;
+  // output staging (wf_pack_*): allocated on first use, not snapshotted
+  void *outbuf = nullptr;
+  size_t out_cap = 0;
   template <class T>
   T *adj_alloc(size_t count) {
     void *p = nullptr;
@@ -157,6 +160,7 @@ struct Ctx {
     for (auto &e : ev_pool) (void)hipEventDestroy(e);
     for (auto &p : allocs) (void)hipFree(p.first);
     for (void *p : adj.mem) (void)hipFree(p);
+    if (outbuf) (void)hipFree(outbuf);
     if (st) (void)hipStreamDestroy(st);
   }
 };
@@ -1190,11 +1194,65 @@ int wf_get_field(void *ctx, int32_t field, double *h) {
   });
 }
 
+// ---------------------------------------------------------------------------
+// output tables: Domain::WriteXDMF's datasets and Domain::WriteCSV's rows
+// (Source/Output.cpp:75-554), packed by wf_output.hip
+
+static void *out_staging(Ctx *c, size_t bytes) {
+  if (c->out_cap < bytes) {
+    if (c->outbuf) {
+      HIPCHK(hipStreamSynchronize(c->st));
+      HIPCHK(hipFree(c->outbuf));
+      c->outbuf = nullptr;
+      c->out_cap = 0;
+    }
+    HIPCHK(hipMalloc(&c->outbuf, bytes));
+    c->out_cap = bytes;
+  }
+  return c->outbuf;
+}
+
+int wf_pack_xdmf(void *ctx, float *f, int32_t *iv) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!f || !iv) throw Error(WF_EINVAL, "null output buffer");
+    if (!c->inited) throw Error(WF_ESTATE, "wf_pack_xdmf before wf_init");
+    const size_t nf = (size_t)WF_XDMF_NF * (size_t)c->n, ni = (size_t)WF_XDMF_NI * (size_t)c->n;
+    char *buf = (char *)out_staging(c, (nf + ni) * 4);
+    float *df = (float *)buf;
+    int *di = (int *)(buf + nf * 4);
+    launch_pack_xdmf(c->d, c->st, c->cur, df, di);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(f, df, nf * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(iv, di, ni * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return WF_OK;
+  });
+}
+
+int wf_pack_csv(void *ctx, double *rows) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!rows) throw Error(WF_EINVAL, "null output buffer");
+    if (!c->inited) throw Error(WF_ESTATE, "wf_pack_csv before wf_init");
+    const size_t nr = (size_t)WF_CSV_NCOL * (size_t)c->n;
+    double *dr = (double *)out_staging(c, nr * 8);
+    launch_pack_csv(c->d, c->st, c->cur, dr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(rows, dr, nr * 8, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return WF_OK;
+  });
+}
+
 int wf_set_int_field(void *ctx, int32_t field, const int32_t *h) {
   Ctx *c = (Ctx *)ctx;
   return guard(c, [&] {
     as_ctx(ctx);
-    if (field < 0 || field >= WF_NIFIELDS || field == WF_I_NB || !h) throw Error(WF_EINVAL, "bad int field");
+    if (field < 0 || field >= WF_NIFIELDS || field == WF_I_NB || field == WF_I_CONTNB || !h)
+      throw Error(WF_EINVAL, "bad int field");
     if (c->inited) throw Error(WF_ESTATE, "int fields are fixed after wf_init");
     if (field == WF_I_MATERIAL)
       for (int64_t i = 0; i < c->n; ++i)
@@ -1209,10 +1267,16 @@ int wf_get_int_field(void *ctx, int32_t field, int32_t *h) {
   return guard(c, [&] {
     as_ctx(ctx);
     if (field < 0 || field >= WF_NIFIELDS || !h) throw Error(WF_EINVAL, "bad int field");
-    if (field == WF_I_NB || (field == WF_I_ID && c->inited && c->d.contact)) {
-      // neighbour counts; zone ids once free-surface particles took ID 1
+    if (field == WF_I_CONTNB && c->inited && !c->d.contact) {
+      std::memset(h, 0, (size_t)c->n * sizeof(int32_t));
+      return WF_OK;
+    }
+    if (field == WF_I_NB || field == WF_I_CONTNB || (field == WF_I_ID && c->inited && c->d.contact)) {
+      // neighbour and contact-pair counts; zone ids once free-surface
+      // particles took ID 1
       if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
-      auto t = dget(c, field == WF_I_NB ? c->d.nnb : c->d.id, (size_t)c->npad);
+      const int *src = field == WF_I_NB ? c->d.nnb : (field == WF_I_CONTNB ? c->d.ccnt : c->d.id);
+      auto t = dget(c, src, (size_t)c->npad);
       for (int64_t k = 0; k < c->n; ++k) h[c->perm[k]] = t[k];
       return WF_OK;
     }

@@ -281,5 +281,8 @@ void launch_surface(const Dev &d, hipStream_t st, int cur, int setup);
 void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt);
 void launch_rigid_drift(const Dev &d, hipStream_t st, double dt);
 void launch_mesh_update(const Dev &d, hipStream_t st, double dt);
+// output tables (wf_output.hip)
+void launch_pack_xdmf(const Dev &d, hipStream_t st, int cur, float *f, int *iv);
+void launch_pack_csv(const Dev &d, hipStream_t st, int cur, double *rows);
 
 }  // namespace wf

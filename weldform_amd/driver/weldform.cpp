@@ -387,7 +387,9 @@ int main(int argc, char **argv) try {
   }
   if (kickdrift) dom.SolveDiffUpdateKickDrift(sim_time, timestep, output_time, key.c_str(), 1000000);
   else dom.SolveDiffUpdateLeapFrog(sim_time, timestep, output_time, key.c_str(), 1000000);
-  dom.WriteXDMF((key + "_final").c_str());
+  // the state after the solve goes to maz.hdf5 / maz.xmf (Source/WeldForm.cpp:941);
+  // without the HDF5 library the solve's CSV files are the output
+  if (wfio_has_hdf5()) dom.WriteXDMF("maz");
   if (!dump.empty()) {
     FILE *f = std::fopen(dump.c_str(), "w");
     if (!f) throw new Fatal("cannot write " + dump);
