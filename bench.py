@@ -129,21 +129,18 @@ def contact_run(steps, warmup=10):
     return res
 
 
-def adjoint_run(window, pre=100, k=20, contact=True, fd=False):
-    """Adjoint calibration window at 200,704 particles (weldform_amd.cases:
-    config5, the config-3 billet and punches, FP64, isothermal; or
-    config5_zone, the same billet driven through velocity BCs).  After `pre`
-    steps, a `window`-step Leapfrog window with a snapshot every k steps; loss
-    = sum_n (F_z,n - target_n)^2 over the punch plane's contact force (or
-    the top zone's reaction), target = 0.98 x the window's own history; one
-    gradient = wf_adj_forward + wf_adj_backward (dL/d(A, B, n, C, m, mu)).
-    fd: central differences (1e-4 relative) of the five JC parameters."""
+def adjoint_run(case, kind, index, window, pre=100, k=20, fd=False, rel_step=1e-6, desc=""):
+    """Adjoint calibration window: after `pre` steps, a `window`-step FP64
+    Leapfrog window with a snapshot every k steps; loss = sum_n (F_z,n -
+    target_n)^2 over a rigid plane's contact force (kind ADJ_PLANE) or a
+    velocity-BC zone's reaction (ADJ_ZONE), target = 0.98 x the window's own
+    history; one gradient = wf_adj_forward + wf_adj_backward
+    (dL/d(A, B, n, C, m, friction)).  fd: central differences of the five JC
+    parameters with a relative step rel_step, compared as theta dL/dtheta."""
     import numpy as np
 
     from weldform_amd import capi, cases
 
-    case = cases.config5(steps=window) if contact else cases.config5_zone(steps=window)
-    kind, index = (capi.ADJ_PLANE, 0) if contact else (capi.ADJ_ZONE, case.top_zone)
     log(f"bench: {case.name} n={case.n} window={window} checkpoint_every={k}")
     s = cases.make_solver(case, precision=capi.FP64)
     s.step(pre)
@@ -163,10 +160,7 @@ def adjoint_run(window, pre=100, k=20, contact=True, fd=False):
     s.step(window)
     s.sync()
     t_plain = time.perf_counter() - t0
-    res = {"workload": f"{case.name}: R=15mm L=30mm dx=0.4675mm JC steel, isothermal, "
-                       + ("planes -5 m/s / fixed, mu=0.2, loss on the top plane's contact force" if contact else
-                          "top row -5 m/s, bottom row fixed, loss on the top zone's reaction")
-                       + f"; window of {window} FP64 Leapfrog steps after {pre}, snapshot every {k}",
+    res = {"workload": f"{case.name}: {desc}; window of {window} FP64 Leapfrog steps after {pre}, snapshot every {k}",
            "particles": case.n, "window_steps": window, "checkpoint_every": k,
            "gradients_per_s": 1.0 / (t_fwd + t_bwd),
            "forward_window_s": t_fwd, "backward_s": t_bwd, "plain_forward_s": t_plain,
@@ -177,7 +171,9 @@ def adjoint_run(window, pre=100, k=20, contact=True, fd=False):
         mat0 = dict(case.materials[0])
         rel = {}
         for p, key in enumerate(("jc_A", "jc_B", "jc_n", "jc_C", "jc_m")):
-            h = 1e-4 * abs(mat0[key])
+            if mat0[key] == 0:
+                continue
+            h = rel_step * abs(mat0[key])
             L = []
             for sgn in (1, -1):
                 s.adj_restore()
@@ -185,11 +181,41 @@ def adjoint_run(window, pre=100, k=20, contact=True, fd=False):
                 L.append(s.adj_forward(target, checkpoint_every=k, kind=kind, index=index, component=2)[0])
             s.set_material(0, **mat0)
             fdv = (L[0] - L[1]) / (2 * h)
-            rel[key] = {"fd": fdv, "adjoint": float(g[p]), "rel_err": abs(g[p] - fdv) / abs(fdv) if fdv else None}
-        res["fd_check"] = rel
+            rel[key] = {"theta_fd": fdv * mat0[key], "theta_adjoint": float(g[p]) * mat0[key],
+                        "rel_err": abs(g[p] - fdv) / abs(fdv) if fdv else None}
+        res["fd_check"] = {"rel_step": rel_step, "params": rel,
+                           "max_rel_err": max(v["rel_err"] for v in rel.values() if v["rel_err"] is not None)}
     log("bench: adjoint " + json.dumps(res))
     s.close()
     return res
+
+
+def adjoint_legs(window):
+    """Timing at 200k (config 5 with contact, and the same billet under
+    velocity BCs), plus finite-difference checks where the gradient is well
+    posed: the 200k zone window with linear hardening (JC n = 1) and the
+    4.5k aluminium billet over a 100-step window."""
+    from weldform_amd import capi, cases
+
+    out = {}
+    c5 = cases.config5(steps=window)
+    out["config5_adjoint"] = adjoint_run(c5, capi.ADJ_PLANE, 0, window, desc=(
+        "R=15mm L=30mm dx=0.4675mm JC steel, isothermal, planes -5 m/s / fixed, mu=0.2, loss on the top "
+        "plane's contact force"))
+    cz = cases.config5_zone(steps=window)
+    out["config5_adjoint_zone"] = adjoint_run(cz, capi.ADJ_ZONE, cz.top_zone, window, desc=(
+        "the same billet, top row -5 m/s, bottom row fixed, loss on the top zone's reaction"))
+    cl = cases.config5_zone(steps=window)
+    cl.name = "cyl_upset_200k_jc_n1_adjoint"
+    cl.materials = {0: dict(cl.materials[0], jc_n=1.0)}
+    out["config5_adjoint_zone_n1_fd"] = adjoint_run(cl, capi.ADJ_ZONE, cl.top_zone, window, fd=True, desc=(
+        "config5_zone with linear hardening (JC n = 1), finite-difference check over the whole window"))
+    ca = cases.cylinder_case("cyl_upset_4k_jc_adjoint", 7.5e-3, 30e-3, 1.0e-3, cases.AL["rho"], cases.AL["E"],
+                             cases.AL["nu"], cases.JC_AL, steps=200, jitter=1e-3, seed=7, v_top=-20.0, T0=400.0)
+    out["adjoint_4k_fd"] = adjoint_run(ca, capi.ADJ_ZONE, ca.top_zone, 100, fd=True, desc=(
+        "4,524-particle aluminium billet, JC (n = 0.32), T0 = 400 K, top row -20 m/s, loss on the top zone's "
+        "reaction"))
+    return out
 
 
 def batch_run(steps, n_sample=16):
@@ -437,8 +463,7 @@ def main():
     if args.batch_steps > 0:
         out["config"]["config4_batch"] = batch_run(args.batch_steps)
     if args.adjoint_steps > 0:
-        out["config"]["config5_adjoint"] = adjoint_run(args.adjoint_steps, contact=True)
-        out["config"]["config5_adjoint_zone"] = adjoint_run(args.adjoint_steps, contact=False, fd=True)
+        out["config"].update(adjoint_legs(args.adjoint_steps))
     if not args.no_cpu:
         ccase = cases.config2(steps=args.cpu_steps)
         log(f"bench: cpu baseline, {args.cpu_steps} steps")

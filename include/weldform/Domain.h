@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iostream>
 #include <map>
 #include <sstream>
 #include <string>
@@ -91,8 +92,8 @@ class Domain {
   std::vector<amplitude> amps;
   bool auto_ts = true, auto_ts_acc = false, auto_ts_cont = false;
   int ts_nb_inc = 5;
-  bool nonlock_sum = true;           // sums are fixed-order gathers either way
-  bool gradKernelCorr = false;
+  bool nonlock_sum = true;           // false: the Locking sums (float m/rho weights in the rates)
+  bool gradKernelCorr = false;       // Bonet kernel-gradient correction
   bool thermal_solver = false;
   bool pl_heating = false;
   bool contact = false;
@@ -348,7 +349,6 @@ class Domain {
     if (Dimension != 3) throw new Fatal("the device path is 3D only (set Dimension = 3)");
     if (KernelType != Qubic_Spline) throw new Fatal("the device path implements the cubic spline kernel only");
     if (GradientType != Squared_density) throw new Fatal("the device path implements GradientType 0 only");
-    if (gradKernelCorr) throw new Fatal("kernel gradient correction is not implemented on the device path");
     if (contact && auto_ts_cont)
       throw new Fatal("the contact time-step criterion (auto_ts_cont) is not implemented on the device path");
     if (contact) {
@@ -522,6 +522,7 @@ class Domain {
     get1(WF_F_Q_PLHEAT, &Particle::q_plheat);
     get1(WF_F_Q_CONV, &Particle::q_conv);
     get1(WF_F_PS_ENERGY, &Particle::ps_energy);
+    if (gradKernelCorr) get9(WF_F_GRADCORR, &Particle::gradCorrM);
     if (contact) {
       get3(WF_F_CONTFORCE, &Particle::contforce);
       std::vector<int32_t> id(n);
@@ -685,6 +686,15 @@ class Domain {
     cfg.cfl = CFL;
     cfg.sqrt_h_a = sqrt_h_a;
     cfg.min_ts = (double)manual_min_ts;
+    if (nonlock_sum && gradKernelCorr) {
+      // Source/SolverLeapfrog.cpp:114-117
+      std::cout << "WARNING: Nishimura summation is not working with Gradient Kernel Correction. Summation "
+                   "changed to Locking."
+                << std::endl;
+      nonlock_sum = false;
+    }
+    cfg.grad_corr = gradKernelCorr ? 1 : 0;
+    cfg.locking = nonlock_sum ? 0 : 1;
     for (int k = 0; k < 3; ++k) {
       cfg.dom_max[k] = DomMax(k);
       cfg.dom_min[k] = DomMin(k);

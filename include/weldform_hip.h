@@ -106,7 +106,9 @@ extern "C" {
 #define WF_F_PS_ENERGY 27   /* [1] Particle::ps_energy                        */
 #define WF_F_CONTFORCE 28   /* [3] read-only: Particle::contforce of the last
                                step's contact (Source/Contact_Wang.cpp)      */
-#define WF_NFIELDS 29
+#define WF_F_GRADCORR 29    /* [9] read-only: Particle::gradCorrM (identity
+                               unless wf_config.grad_corr)                  */
+#define WF_NFIELDS 30
 
 /* rigid-surface particle fields (wf_get_rigid_field), 3 doubles each, in
  * the order the planes were added, element order within a plane */
@@ -164,6 +166,15 @@ typedef struct wf_config {
   double cfl;               /* Domain::CFL (0.7) for the velocity criterion    */
   double sqrt_h_a;          /* Domain::sqrt_h_a (0.0025), acceleration one     */
   double min_ts;            /* Domain::manual_min_ts (1e-10)                   */
+  int32_t grad_corr;        /* Domain::gradKernelCorr: Bonet kernel-gradient
+                               correction (CalcGradCorrMatrix, Source/Domain.cpp:
+                               1832-1940) in the three pair passes; all
+                               particles must be free                         */
+  int32_t locking;          /* !Domain::nonlock_sum: the Locking sums, whose
+                               rate tensors weigh pairs by m_j/rho_j rounded
+                               to float (Source/InteractionAlt.cpp:412,425).
+                               The reference switches to them when
+                               gradKernelCorr is set; so does the facade.     */
 } wf_config;
 
 typedef struct wf_material {
@@ -280,9 +291,9 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
  * Tensors of width 6 are (00, 11, 22, 01, 12, 02).  As in the reference,
  * ShearS holds Sigma (Source/Output.cpp:250-251), Property1..3 hold the
  * default UserOutput's 0, Sigma_eq is CalculateEquivalentStress of
- * ShearStress.  gradcorrmat, q_friction and c_shearabs are 0: the
- * kernel-gradient correction and the contact's friction heating that would
- * set them are not run on the device path. */
+ * ShearStress.  gradcorrmat is gradCorrM with wf_config.grad_corr, else 0;
+ * q_friction and c_shearabs are 0: the contact's friction heating that
+ * would set them is not run on the device path. */
 #define WF_XDMF_POSITION 0        /* width 3 */
 #define WF_XDMF_VELOCITY 3        /* width 3 */
 #define WF_XDMF_ACCELERATION 6    /* width 3 */

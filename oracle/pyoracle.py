@@ -20,9 +20,9 @@ F64 = ["X", "V", "A", "XPREV", "DISP", "RHO", "DRHO", "MASS", "H", "REFRHO", "P"
        "CS", "P0", "G", "ALPHA", "BETA", "SIGMA", "SHEAR", "SR", "RR", "STRAIN",
        "EPL", "EPLINC", "PLSTRAIN", "DPLSTRAIN", "SIGMAY", "EFFSR", "EP", "ET",
        "T", "DTDT", "KT", "CPT", "THEXP", "QCONV", "QSRC", "QPL", "QCONT",
-       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR"]
+       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR", "GCM"]
 F64_DIM = dict(X=3, V=3, A=3, XPREV=3, DISP=3, SIGMA=9, SHEAR=9, SR=9, RR=9,
-               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9)
+               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9, GCM=9)
 I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB", "ELEMENT", "MESHIDX", "IDORIG"]
 
 
@@ -68,7 +68,7 @@ def lib():
             getattr(L, name).argtypes = [vp, d]
         for name in ["wfo_step_leapfrog", "wfo_step_kickdrift", "wfo_neighbour_search",
                      "wfo_start_acceleration", "wfo_calc_accel", "wfo_calc_dens",
-                     "wfo_calc_rates", "wfo_general_after", "wfo_energies"]:
+                     "wfo_calc_rates", "wfo_general_after", "wfo_energies", "wfo_calc_grad_corr"]:
             getattr(L, name).argtypes = [vp]
         L.wfo_stress_all.argtypes = [vp, d]
         L.wfo_thermal_calcs.argtypes = [vp, d]
@@ -203,6 +203,9 @@ class Domain:
 
     def calc_accel(self):
         self._check(lib().wfo_calc_accel(self.h))
+
+    def calc_grad_corr(self):
+        self._check(lib().wfo_calc_grad_corr(self.h))
 
     def calc_dens(self):
         self._check(lib().wfo_calc_dens(self.h))

@@ -39,7 +39,8 @@ static const int k_fdim[WFO_NFIELDS] = {
     3,                    /* NORMAL */
     3,                    /* CONTFORCE */
     1, 1, 1,              /* TI TIN TIDIST */
-    9                     /* TIR */
+    9,                    /* TIR */
+    9                     /* GCM */
 };
 
 typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T; } bc_t;
@@ -97,6 +98,11 @@ struct wfo_domain {
    * criterion (auto_ts), bit 1 acceleration criterion (auto_ts_acc) */
   int auto_ts;
   double CFL, sqrt_h_a, manual_min_ts, deltatint, deltatmin;
+  /* Domain::nonlock_sum (1: Nishimura pair buffers and reductions, 0:
+   * Locking, each pair adds into both particles in pair order) and
+   * Domain::gradKernelCorr (Bonet kernel-gradient correction) */
+  int nonlock_sum, grad_corr;
+  double *pair_force2;            /* Locking: the second particle's pair term */
 };
 
 static int contact_nb_update(wfo_domain *d);
@@ -129,6 +135,7 @@ wfo_domain *wfo_create(int n) {
   /* Domain::manual_min_ts is declared int (Source/Domain.h:328), so the
    * 1.e-10 of Source/Domain.cpp:170 stores 0: no floor by default */
   d->manual_min_ts = 0.0;
+  d->nonlock_sum = 1;
   d->LL = (int *)xcalloc((size_t)n, sizeof(int));
   d->ipair = (int *)xcalloc((size_t)n, sizeof(int));
   d->jpair = (int *)xcalloc((size_t)n, sizeof(int));
@@ -146,7 +153,7 @@ void wfo_destroy(wfo_domain *d) {
   for (int k = 0; k < WFO_NIFIELDS; ++k) free(d->fi[k]);
   free(d->HOC); free(d->LL); free(d->pairs); free(d->ipair); free(d->jpair);
   free(d->Anei); free(d->Aref); free(d->pair_force); free(d->pair_densinc);
-  free(d->pair_sr); free(d->pair_rr); free(d->temp_work); free(d->dkin_work);
+  free(d->pair_sr); free(d->pair_rr); free(d->temp_work); free(d->dkin_work); free(d->pair_force2);
   for (int m = 0; m < d->nmesh; m++) {
     mesh_t *M = &d->mesh[m];
     free(M->node); free(M->node_v); free(M->el); free(M->centroid); free(M->normal);
@@ -186,6 +193,8 @@ int wfo_set_param(wfo_domain *d, const char *s, double v) {
   else if (!strcmp(s, "CFL")) d->CFL = v;
   else if (!strcmp(s, "sqrt_h_a")) d->sqrt_h_a = v;
   else if (!strcmp(s, "manual_min_ts")) d->manual_min_ts = v;
+  else if (!strcmp(s, "nonlock_sum")) d->nonlock_sum = (int)v;
+  else if (!strcmp(s, "grad_corr")) d->grad_corr = (int)v;
   else if (!strcmp(s, "Time")) d->Time = v;
   else if (!strcmp(s, "deltat")) d->deltat = v;
   else if (!strncmp(s, "DomMax", 6) && s[6] >= '0' && s[6] <= '2') d->DomMax[s[6] - '0'] = v;
@@ -211,6 +220,8 @@ double wfo_get_param(const wfo_domain *d, const char *s) {
   if (!strcmp(s, "m_maxT")) return d->m_maxT;
   if (!strcmp(s, "m_minT")) return d->m_minT;
   if (!strcmp(s, "n_other_pairs")) return (double)d->n_other_pairs;
+  if (!strcmp(s, "nonlock_sum")) return (double)d->nonlock_sum;
+  if (!strcmp(s, "grad_corr")) return (double)d->grad_corr;
   if (!strncmp(s, "CellNo", 6)) return d->CellNo[s[6] - '0'];
   if (!strncmp(s, "CellSize", 8)) return d->CellSize[s[8] - '0'];
   if (!strncmp(s, "BLPF", 4)) return d->BLPF[s[4] - '0'];
@@ -781,14 +792,170 @@ static void accel_reduction(wfo_domain *d) {
   }
 }
 
+/* g = (GK L) x_ij, Mult(GKc, xij, vc) of the corrected passes
+ * (Source/InteractionAlt.cpp:118-133): L the particle's gradCorrM */
+static inline void corr_grad(const wfo_domain *d, int i, double GK, const double *xij, double *g) {
+  const double *L = d->f[WFO_GCM] + 9 * (int64_t)i;
+  for (int a = 0; a < 3; a++) g[a] = (GK * L[3 * a]) * xij[0] + (GK * L[3 * a + 1]) * xij[1] + (GK * L[3 * a + 2]) * xij[2];
+}
+
+/* pair geometry of the Locking passes: P1 = first, P2 = second as stored */
+static inline void pair_geo_locking(const wfo_domain *d, int64_t p, pairgeo_t *g) {
+  g->p1 = d->pairs[2 * p];
+  g->p2 = d->pairs[2 * p + 1];
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V];
+  g->h = (d->f[WFO_H][g->p1] + d->f[WFO_H][g->p2]) / 2;
+  for (int k = 0; k < 3; k++) {
+    g->xij[k] = x[3 * g->p1 + k] - x[3 * g->p2 + k];
+    g->vij[k] = v[3 * g->p1 + k] - v[3 * g->p2 + k];
+  }
+  g->rij = norm3(g->xij);
+  g->GK = wfo_grad_kernel(g->rij / g->h, g->h);
+}
+
+/* CalcAccel, Locking branch (nonlock_sum false), GradientType 0
+ * (Source/InteractionAlt.cpp:12-235): each pair in pair order adds
+ * mj (g1 . M) to particle 1 and subtracts mi (g2 . M) from particle 2, with
+ * g = GK x_ij, or g_k = (GK L_k) x_ij under the kernel-gradient
+ * correction; a starts from StartAcceleration's 0 */
+static void calc_accel_locking(wfo_domain *d) {
+  const double *rho = d->f[WFO_RHO], *sig = d->f[WFO_SIGMA], *m = d->f[WFO_MASS];
+  double *acc = d->f[WFO_A];
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo_locking(d, p, &g);
+    int p1 = g.p1, p2 = g.p2;
+    double Alpha = (d->f[WFO_ALPHA][p1] + d->f[WFO_ALPHA][p2]) / 2.0;
+    double Beta = (d->f[WFO_BETA][p1] + d->f[WFO_BETA][p2]) / 2.0;
+    double di = rho[p1], dj = rho[p2];
+    double PI_s = 0.0;
+    if (Alpha != 0.0 || Beta != 0.0) {
+      double MUij = g.h * dot3(g.vij, g.xij) / (g.rij * g.rij + 0.01 * g.h * g.h);
+      double Cij = 0.5 * (d->f[WFO_CS][p1] + d->f[WFO_CS][p2]);
+      if (dot3(g.vij, g.xij) < 0) PI_s = (Alpha * Cij * MUij + Beta * MUij * MUij) / (0.5 * (di + dj));
+    }
+    double si = 1.0 / (di * di), sj = 1.0 / (dj * dj);
+    double tis = 0.0;
+    const double *ti = d->f[WFO_TI], *tir = d->f[WFO_TIR];
+    int ti_on = ti[p1] > 0.0 || ti[p2] > 0.0;
+    if (ti_on) {
+      double K = wfo_kernel(g.rij / g.h, g.h);
+      double K0 = wfo_kernel((d->f[WFO_TIDIST][p1] + d->f[WFO_TIDIST][p2]) / (2.0 * g.h), g.h);
+      tis = pow((K / K0), (d->f[WFO_TIN][p1] + d->f[WFO_TIN][p2]) / 2.0);
+    }
+    double M[9];
+    for (int a = 0; a < 3; a++)
+      for (int c = 0; c < 3; c++) {
+        double piac = (a == c) ? 1.0 * PI_s : 0.0 * PI_s;
+        double tiac = ti_on ? tis * (tir[9 * (int64_t)p1 + 3 * a + c] + tir[9 * (int64_t)p2 + 3 * a + c]) : 0.0;
+        M[3 * a + c] = ((sig[9 * (int64_t)p1 + 3 * a + c] * si + sig[9 * (int64_t)p2 + 3 * a + c] * sj) + piac) + tiac;
+      }
+    double g1[3], g2[3];
+    if (d->grad_corr) {
+      corr_grad(d, p1, g.GK, g.xij, g1);
+      corr_grad(d, p2, g.GK, g.xij, g2);
+    } else {
+      for (int k = 0; k < 3; k++) g1[k] = g2[k] = g.GK * g.xij[k];
+    }
+    for (int c = 0; c < 3; c++) {
+      double t1 = g1[0] * M[c] + g1[1] * M[3 + c] + g1[2] * M[6 + c];
+      double t2 = g2[0] * M[c] + g2[1] * M[3 + c] + g2[2] * M[6 + c];
+      acc[3 * p1 + c] += m[p2] * t1;
+      acc[3 * p2 + c] -= m[p1] * t2;
+    }
+  }
+}
+
 int wfo_calc_accel(wfo_domain *d) {
+  if (!d->nonlock_sum) {
+    for (int i = 0; i < 3 * d->n; i++) d->f[WFO_A][i] = 0.0;
+    calc_accel_locking(d);
+    return 0;
+  }
   calc_accel_pairs(d);
   accel_reduction(d);
   return 0;
 }
 
+/* Domain::CalcGradCorrMatrix (Source/Domain.cpp:1832-1940), 3D: for every
+ * pair in pair order, m = (GK x_ij) (x) x_ij; particle 1 subtracts
+ * (mj/dj) m, particle 2 (mi/di) m; then each free particle's sum is
+ * inverted (InvLog, External/matvec.h:673-693: |det| < 1e-10 fails and the
+ * identity is taken); fixed particles get the identity */
+int wfo_calc_grad_corr(wfo_domain *d) {
+  double *T = (double *)xcalloc((size_t)d->n * 9, sizeof(double));
+  const double *rho = d->f[WFO_RHO], *m = d->f[WFO_MASS];
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo_locking(d, p, &g);
+    double gx[3] = {g.GK * g.xij[0], g.GK * g.xij[1], g.GK * g.xij[2]};
+    double w1 = m[g.p2] / rho[g.p2], w2 = m[g.p1] / rho[g.p1];
+    for (int a = 0; a < 3; a++)
+      for (int c = 0; c < 3; c++) {
+        double mm = gx[a] * g.xij[c];
+        T[9 * (int64_t)g.p1 + 3 * a + c] = T[9 * (int64_t)g.p1 + 3 * a + c] - w1 * mm;
+        T[9 * (int64_t)g.p2 + 3 * a + c] = T[9 * (int64_t)g.p2 + 3 * a + c] - w2 * mm;
+      }
+  }
+  double *L = d->f[WFO_GCM];
+  for (int i = 0; i < d->solid_count; i++) {
+    double *Li = L + 9 * (int64_t)i;
+    const double *M = T + 9 * (int64_t)i;
+    int ok = d->fi[WFO_FREE][i];
+    if (ok) {
+      double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                   M[2] * (M[3] * M[7] - M[4] * M[6]);
+      if (fabs(det) < 1.0e-10) {
+        ok = 0;
+      } else {
+        Li[0] = (M[4] * M[8] - M[5] * M[7]) / det;
+        Li[1] = (M[2] * M[7] - M[1] * M[8]) / det;
+        Li[2] = (M[1] * M[5] - M[2] * M[4]) / det;
+        Li[3] = (M[5] * M[6] - M[3] * M[8]) / det;
+        Li[4] = (M[0] * M[8] - M[2] * M[6]) / det;
+        Li[5] = (M[2] * M[3] - M[0] * M[5]) / det;
+        Li[6] = (M[3] * M[7] - M[4] * M[6]) / det;
+        Li[7] = (M[1] * M[6] - M[0] * M[7]) / det;
+        Li[8] = (M[0] * M[4] - M[1] * M[3]) / det;
+      }
+    }
+    if (!ok)
+      for (int k = 0; k < 9; k++) Li[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  }
+  free(T);
+  return 0;
+}
+
+/* CalcDensInc, Locking branch (Source/InteractionAlt.cpp:483-600): pair
+ * order, particle 1 adds mj (di/dj) (v_ij . g1), particle 2 mi (dj/di)
+ * (v_ij . g2) */
+static void calc_dens_locking(wfo_domain *d) {
+  const double *m = d->f[WFO_MASS], *rho = d->f[WFO_RHO];
+  double *drho = d->f[WFO_DRHO];
+  for (int i = 0; i < d->n; i++) drho[i] = 0.0;
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo_locking(d, p, &g);
+    double g1[3], g2[3];
+    if (d->grad_corr) {
+      corr_grad(d, g.p1, g.GK, g.xij, g1);
+      corr_grad(d, g.p2, g.GK, g.xij, g2);
+    } else {
+      for (int k = 0; k < 3; k++) g1[k] = g2[k] = g.GK * g.xij[k];
+    }
+    double t1 = dot3(g.vij, g1), t2 = dot3(g.vij, g2);
+    double di = rho[g.p1], dj = rho[g.p2];
+    drho[g.p1] += m[g.p2] * (di / dj) * t1;
+    drho[g.p2] += m[g.p1] * (dj / di) * t2;
+  }
+}
+
 /* CalcDensInc + DensReduction (Source/InteractionAlt.cpp:483-640) */
 int wfo_calc_dens(wfo_domain *d) {
+  if (!d->nonlock_sum) {
+    calc_dens_locking(d);
+    return 0;
+  }
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int64_t p = 0; p < d->npairs; p++) {
     pairgeo_t g;
@@ -813,8 +980,73 @@ int wfo_calc_dens(wfo_domain *d) {
   return 0;
 }
 
+/* CalcRateTensors, Locking branch (Source/InteractionAlt.cpp:263-440):
+ * pair order; the weight is mj/dj rounded to float (`float mj_dj`,
+ * :412, :425); without the correction the pair's StrainRate / RotationRate
+ * (-0.5 GK times the v (x) x forms), with it -0.5 (G +- G^T) for
+ * G = v_ij (x) g_k, g_k = (GK L_k) x_ij */
+static void calc_rates_locking(wfo_domain *d) {
+  const double *m = d->f[WFO_MASS], *rho = d->f[WFO_RHO];
+  for (int i = 0; i < d->n; i++) {
+    memset(d->f[WFO_SR] + 9 * (int64_t)i, 0, 9 * sizeof(double));
+    memset(d->f[WFO_RR] + 9 * (int64_t)i, 0, 9 * sizeof(double));
+  }
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo_locking(d, p, &g);
+    const double *va = g.vij, *xi = g.xij;
+    double SR[2][9], RR[2][9];
+    if (!d->grad_corr) {
+      double S[9], R[9];
+      S[0] = 2.0 * va[0] * xi[0];
+      S[1] = va[0] * xi[1] + va[1] * xi[0];
+      S[2] = va[0] * xi[2] + va[2] * xi[0];
+      S[3] = S[1];
+      S[4] = 2.0 * va[1] * xi[1];
+      S[5] = va[1] * xi[2] + va[2] * xi[1];
+      S[6] = S[2];
+      S[7] = S[5];
+      S[8] = 2.0 * va[2] * xi[2];
+      R[0] = R[4] = R[8] = 0.0;
+      R[1] = va[0] * xi[1] - va[1] * xi[0];
+      R[2] = va[0] * xi[2] - va[2] * xi[0];
+      R[5] = va[1] * xi[2] - va[2] * xi[1];
+      R[3] = -R[1];
+      R[6] = -R[2];
+      R[7] = -R[5];
+      double s = -0.5 * g.GK;
+      for (int k = 0; k < 9; k++) SR[0][k] = SR[1][k] = s * S[k], RR[0][k] = RR[1][k] = s * R[k];
+    } else {
+      for (int q = 0; q < 2; q++) {
+        double gk[3], G[9];
+        corr_grad(d, q ? g.p2 : g.p1, g.GK, xi, gk);
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) G[3 * a + c] = va[a] * gk[c];
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) {
+            SR[q][3 * a + c] = -0.5 * (G[3 * a + c] + G[3 * c + a]);
+            RR[q][3 * a + c] = -0.5 * (G[3 * a + c] - G[3 * c + a]);
+          }
+      }
+    }
+    float w1 = (float)(m[g.p2] / rho[g.p2]), w2 = (float)(m[g.p1] / rho[g.p1]);
+    double *S1 = d->f[WFO_SR] + 9 * (int64_t)g.p1, *R1 = d->f[WFO_RR] + 9 * (int64_t)g.p1;
+    double *S2 = d->f[WFO_SR] + 9 * (int64_t)g.p2, *R2 = d->f[WFO_RR] + 9 * (int64_t)g.p2;
+    for (int k = 0; k < 9; k++) {
+      S1[k] = S1[k] + w1 * SR[0][k];
+      R1[k] = R1[k] + w1 * RR[0][k];
+      S2[k] = S2[k] + w2 * SR[1][k];
+      R2[k] = R2[k] + w2 * RR[1][k];
+    }
+  }
+}
+
 /* CalcRateTensors + RateTensorsReduction (Source/InteractionAlt.cpp:263-479) */
 int wfo_calc_rates(wfo_domain *d) {
+  if (!d->nonlock_sum) {
+    calc_rates_locking(d);
+    return 0;
+  }
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int64_t p = 0; p < d->npairs; p++) {
     pairgeo_t g;
@@ -1535,9 +1767,12 @@ static int schedule_search(wfo_domain *d, double mx) {
     if (d->ts_i == 0 && d->nb_cleared) {
       main_neighbour_search(d);
       if (calc_pair_pos_list(d)) return -1;
+      if (d->grad_corr) wfo_calc_grad_corr(d);
       if (d->contact && contact_nb_update(d)) return -1;
     }
   }
+  /* SolverLeapfrog.cpp:207-212: the correction again on the first step */
+  if (d->isfirst && d->grad_corr) wfo_calc_grad_corr(d);
   return 0;
 }
 

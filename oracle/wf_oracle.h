@@ -12,7 +12,8 @@
  *   Source/Neighbour.cpp:10-694, Source/InteractionAlt.cpp:12-640,
  *   Source/Functions.cpp:25-124,206-256, Source/Particle.cpp:28-157,922-1067,
  *   Source/Material.cpp:6-72, Source/Domain.cpp:78-175,674-860,1519-1597,
- *   1752-1829, Source/Thermal.cpp:53-219,339-389,721-748,
+ *   1752-1829,1832-1940 (kernel-gradient correction; with the Locking
+ *   branches of Source/InteractionAlt.cpp), Source/Thermal.cpp:53-219,339-389,721-748,
  *   Source/ThermalStuct.cpp:7-39, Source/WeldForm.cpp:51-120,338-369,884-921.
  * Each function in wf_oracle.c cites the lines it follows.
  *
@@ -50,6 +51,7 @@ enum {
   WFO_CONTFORCE,                                           /* 3 */
   WFO_TI, WFO_TIN, WFO_TIDIST,                             /* 1: TI, TIn, TIInitDist */
   WFO_TIR,                                                 /* 9 */
+  WFO_GCM,                                                 /* 9: gradCorrM */
   WFO_NFIELDS
 };
 /* int fields */
@@ -99,6 +101,7 @@ int wfo_neighbour_search(wfo_domain *d);          /* Clear + Main + tables */
 int64_t wfo_npairs(const wfo_domain *d);
 const int *wfo_pairs(const wfo_domain *d);         /* (temp1,temp2) pairs  */
 int wfo_start_acceleration(wfo_domain *d);
+int wfo_calc_grad_corr(wfo_domain *d);   /* CalcGradCorrMatrix: gradCorrM of every particle */
 int wfo_calc_accel(wfo_domain *d);                 /* CalcAccel+Reduction  */
 int wfo_calc_dens(wfo_domain *d);                  /* CalcDensInc+Reduction*/
 int wfo_calc_rates(wfo_domain *d);                 /* CalcRateTensors+Red. */

@@ -85,6 +85,31 @@ def test_adjoint_zone_reaction_matches_fd():
     _fd_check(g, c, c.materials[0], 0.98 * hist, capi.ADJ_ZONE, c.top_zone)
 
 
+def _al_billet(name, steps, **mat):
+    return cases.cylinder_case(name, 7.5e-3, 30e-3, 1.0e-3, cases.AL["rho"], cases.AL["E"], cases.AL["nu"],
+                               dict(cases.JC_AL, **mat), steps=steps, jitter=1e-3, seed=7, v_top=-20.0, T0=400.0)
+
+
+@pytest.mark.parametrize("window,n_exp", [(100, 0.32), (400, 1.0)])
+def test_adjoint_zone_long_windows(window, n_exp):
+    """Five and twenty segments of 20 steps.  With the aluminium's n = 0.32
+    the window stays at 100 steps: JC hardening B n pl^(n-1) is unbounded as
+    pl -> 0, and the reference's explicit return map (scale-back with the
+    previous step's Sy) then has a tangent eigenvalue -H/(3G) beyond -1 for
+    freshly yielded particles, so over a few hundred steps the exact
+    derivative grows without bound while finite differences saturate.  With
+    linear hardening (n = 1) the derivative stays well posed over 400
+    steps."""
+    c = _al_billet(f"cyl_upset_4k_jc_n{n_exp:g}", 100 + window, jc_n=n_exp)
+    print(f"adjoint long window: n={c.n} window={window} jc_n={n_exp}", flush=True)
+    g = cases.make_solver(c, precision=capi.FP64)
+    g.step(100)
+    g.sync()
+    _, hist = g.adj_forward(np.zeros(window), checkpoint_every=20, kind=capi.ADJ_ZONE, index=c.top_zone, component=2)
+    g.adj_restore()
+    _fd_check(g, c, c.materials[0], 0.98 * hist, capi.ADJ_ZONE, c.top_zone, k=20, rel_step=1e-6)
+
+
 def test_adjoint_plane_force_matches_fd():
     c = cases.contact_small(steps=310)
     print(f"adjoint contact case: n={c.n}", flush=True)

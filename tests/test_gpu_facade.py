@@ -7,6 +7,7 @@ the device and writes the zone-3 reaction history and the final particle
 state.  The same case goes through the oracle here; the bar is the FP64 one
 of the direct C-ABI tests (1e-8 after the run).
 """
+import json
 import os
 import subprocess
 
@@ -41,9 +42,13 @@ def _run_driver(tmp_path, steps, scheme, precision, ts="fixed"):
     return key, react, final
 
 
-def _oracle(steps, scheme):
+def _oracle(steps, scheme, grad_corr=False):
     case = cases.config1(steps=steps)
     d = oracle_from_case(case, threads=8)
+    if grad_corr:
+        # the reference's solver switches to the Locking sums with the correction
+        d.set_param("nonlock_sum", 0)
+        d.set_param("grad_corr", 1)
     d.solve_init(case.dt)
     R = []
     for _ in range(steps):
@@ -114,20 +119,30 @@ def test_facade_driver_adaptive_step(tmp_path):
     assert os.path.exists(f"{key}_0001.csv") and os.path.exists(f"{key}_0002.csv")
 
 
-def test_json_scene_driver_matches_oracle(tmp_path):
+@pytest.mark.parametrize("grad_corr", [False, True])
+def test_json_scene_driver_matches_oracle(tmp_path, grad_corr):
     """The WeldForm command-line driver on tests/json/upsetting_bilinear.json
     (config 1 in the reference's scene format: 200 Leapfrog steps) against
-    the oracle's config-1 run."""
+    the oracle's config-1 run; also with "kernelGradCorr": true, as the
+    reference's Compression.json sets it."""
     scene = os.path.join(os.path.dirname(__file__), "json", "upsetting_bilinear.json")
+    if grad_corr:
+        sc = json.load(open(scene))
+        sc["Configuration"]["kernelGradCorr"] = True
+        scene = str(tmp_path / "upsetting_gradcorr.json")
+        with open(scene, "w") as f:
+            json.dump(sc, f)
     if not os.path.exists(wfbuild.JSON_DRIVER):
         wfbuild.build_json_driver()
     dump = str(tmp_path / "final.txt")
-    print("json driver: upsetting_bilinear.json", flush=True)
+    print(f"json driver: upsetting_bilinear.json kernelGradCorr={grad_corr}", flush=True)
     r = subprocess.run([wfbuild.JSON_DRIVER, scene, "--out", str(tmp_path / "run"), "--dump", dump],
                        capture_output=True, text=True, timeout=100, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
+    if grad_corr:
+        assert "Summation changed to Locking" in r.stdout
     final = np.loadtxt(dump)
-    case, d, R = _oracle(200, "leapfrog")
+    case, d, R = _oracle(200, "leapfrog", grad_corr=grad_corr)
     e_u = normwise_err(final[:, :3], d.f("DISP"))
     print(f"json driver: displacement {e_u:.3e}", flush=True)
     assert final.shape == (case.n, 4) and e_u < 1e-8

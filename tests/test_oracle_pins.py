@@ -661,3 +661,87 @@ def test_tensile_instability_pair_term():
     assert np.abs(tir[0]).max() > 0 and np.abs(tir[1]).max() > 0
     assert np.allclose(a1[0] - a0[0], want, rtol=1e-10, atol=0)
     assert np.allclose(a1[1] - a0[1], -want, rtol=1e-10, atol=0)
+
+
+# ------------------------------------- Locking sums, kernel-gradient correction
+def _jittered(grad_corr=False, nonlock=True, A=None):
+    c = cases.small_jc(steps=1)
+    d = oracle_from_case(c)
+    d.set_param("nonlock_sum", 1 if nonlock else 0)
+    d.set_param("grad_corr", 1 if grad_corr else 0)
+    if A is not None:
+        d.f("V")[:] = c.x @ np.asarray(A).T
+    d.f("SIGMA")[:] = np.einsum("n,ab->nab", 1.0 + c.x[:, 2] * 30.0, np.array(
+        [[1.0, 0.2, -0.3], [0.2, -2.0, 0.5], [-0.3, 0.5, 0.7]]) * 1e8)
+    d.neighbour_search()
+    d.start_acceleration()
+    if grad_corr:
+        d.calc_grad_corr()
+    return c, d
+
+
+def test_locking_sums_equal_nishimura_up_to_rounding():
+    """The Locking passes add each pair into both particles in pair order;
+    the Nishimura passes gather.  Same sums: acceleration and density rate
+    to rounding, rates to the float rounding of the Locking weight m_j/rho_j
+    (`float mj_dj`, Source/InteractionAlt.cpp:412)."""
+    A = np.array([[1.0, 2.0, -0.5], [0.3, -1.5, 0.7], [-0.2, 0.4, 0.8]]) * 100.0
+    out = {}
+    for nonlock in (True, False):
+        c, d = _jittered(nonlock=nonlock, A=A)
+        d.calc_accel()
+        d.calc_dens()
+        d.calc_rates()
+        out[nonlock] = {k: d.f(k).copy() for k in ("A", "DRHO", "SR", "RR")}
+    for k, tol in (("A", 1e-12), ("DRHO", 1e-12), ("SR", 1e-6), ("RR", 1e-6)):
+        a, b = out[True][k], out[False][k]
+        err = np.abs(a - b).max() / np.abs(a).max()
+        assert err < tol, (k, err)
+    assert not np.array_equal(out[True]["SR"], out[False]["SR"])  # the float weights do change them
+
+
+def test_grad_corr_matrix_pins():
+    """gradCorrM = (-sum_j V_j GK x_ij (x) x_ij)^-1: on the lattice interior
+    the sum is mu I (the lattice moment), so L = I / mu; everywhere L is
+    symmetric and inverts the sum formed here from the brute-force pair set."""
+    c = cases.config1(steps=1)
+    d = oracle_from_case(c)
+    d.set_param("grad_corr", 1)
+    d.neighbour_search()
+    d.calc_grad_corr()
+    L = d.f("GCM").reshape(-1, 3, 3)
+    mu = _lattice_moment(c)
+    m = _interior(c, d)
+    assert np.abs(L[m] - np.eye(3) / mu).max() < 1e-12
+    assert np.abs(L - L.transpose(0, 2, 1)).max() < 1e-9 * np.abs(L).max()
+    P = brute_pairs(c.x, c.h)
+    xij = c.x[P[:, 0]] - c.x[P[:, 1]]
+    h = 0.5 * (c.h[P[:, 0]] + c.h[P[:, 1]])
+    r = np.linalg.norm(xij, axis=1)
+    GK = np.array([po.grad_kernel(q, hh) for q, hh in zip(r / h, h)])
+    V = c.mass / c.density
+    dyad = GK[:, None, None] * xij[:, :, None] * xij[:, None, :]
+    M = np.zeros((c.n, 3, 3))
+    np.add.at(M, P[:, 0], -V[P[:, 1], None, None] * dyad)
+    np.add.at(M, P[:, 1], -V[P[:, 0], None, None] * dyad)
+    assert np.abs(np.einsum("nab,nbc->nac", L, M) - np.eye(3)).max() < 1e-10
+
+
+def test_grad_corr_reproduces_linear_fields_everywhere():
+    """With the correction (Locking, as the reference forces it), a linear
+    velocity field v = A x gives every particle -- surface included -- the
+    exact strain rate sym(A) and rotation rate skew(A) (to the float weight
+    rounding) and the exact density rate -rho tr(A); without it the surface
+    particles are far off."""
+    A = np.array([[1.0, 2.0, -0.5], [0.3, -1.5, 0.7], [-0.2, 0.4, 0.8]]) * 100.0
+    c, d = _jittered(grad_corr=True, nonlock=False, A=A)
+    d.calc_dens()
+    d.calc_rates()
+    scale = np.abs(A).max()
+    assert np.abs(d.f("SR") - 0.5 * (A + A.T)).max() < 1e-6 * scale
+    assert np.abs(d.f("RR") - 0.5 * (A - A.T)).max() < 1e-6 * scale
+    rho = d.f("RHO")
+    assert np.abs(d.f("DRHO") + rho * np.trace(A)).max() < 1e-10 * rho.max() * scale
+    c, d = _jittered(grad_corr=False, nonlock=False, A=A)
+    d.calc_rates()
+    assert np.abs(d.f("SR") - 0.5 * (A + A.T)).max() > 0.1 * scale

@@ -27,7 +27,7 @@ FIELDS = {
     "sigmay": (15, 1), "eff_sr": (16, 1), "pressure": (17, 1), "T": (18, 1),
     "dTdt": (19, 1), "ddensity": (20, 1), "q_plheat": (21, 1),
     "strain_pl_incr": (22, 9), "q_conv": (23, 1), "h_conv": (24, 1),
-    "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1), "contforce": (28, 3),
+    "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1), "contforce": (28, 3), "gradcorr": (29, 9),
 }
 RIGID_FIELDS = {"x": 0, "v": 1, "normal": 2}
 INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4, "contnb": 5}
@@ -49,7 +49,7 @@ class Config(ctypes.Structure):
                 ("auto_ts", ctypes.c_int32), ("cellfac", ctypes.c_double),
                 ("pl_work_heat_frac", ctypes.c_double), ("dom_max", ctypes.c_double * 3),
                 ("dom_min", ctypes.c_double * 3), ("cfl", ctypes.c_double), ("sqrt_h_a", ctypes.c_double),
-                ("min_ts", ctypes.c_double)]
+                ("min_ts", ctypes.c_double), ("grad_corr", ctypes.c_int32), ("locking", ctypes.c_int32)]
 
 
 class Material(ctypes.Structure):
@@ -190,12 +190,12 @@ class Solver:
     def __init__(self, n, precision=FP64, ts_nb_inc=5, max_nb=128, thermal_solver=False,
                  pl_heating=False, device=0, cellfac=2.0, pl_work_heat_frac=0.9,
                  dom_max=(-1e11, -1e11, -1e11), dom_min=(1e11, 1e11, 1e11), graphs=True, stage_max=0,
-                 lanes=0, auto_ts=0, cfl=0.7, sqrt_h_a=0.0025, min_ts=0.0):
+                 lanes=0, auto_ts=0, cfl=0.7, sqrt_h_a=0.0025, min_ts=0.0, grad_corr=False, locking=False):
         cfg = Config(precision=precision, ts_nb_inc=ts_nb_inc, max_nb=max_nb,
                      thermal_solver=int(thermal_solver), pl_heating=int(pl_heating),
                      device=device, no_graphs=int(not graphs), stage_max=stage_max, lanes=lanes, cellfac=cellfac,
                      pl_work_heat_frac=pl_work_heat_frac, auto_ts=auto_ts, cfl=cfl, sqrt_h_a=sqrt_h_a,
-                     min_ts=min_ts)
+                     min_ts=min_ts, grad_corr=int(grad_corr), locking=int(locking))
         cfg.dom_max[:] = list(dom_max)
         cfg.dom_min[:] = list(dom_min)
         self.n = int(n)

@@ -32,8 +32,8 @@ struct Error : std::runtime_error {
       throw Error(WF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));               \
   } while (0)
 
-const int kFieldDim[WF_NFIELDS] = {3, 3, 3, 3, 1, 1, 1, 9, 9, 9, 9, 9, 9, 1,
-                                   1, 1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1};
+const int kFieldDim[WF_NFIELDS] = {3, 3, 3, 3, 1, 1, 1, 9, 9, 9, 9, 9, 9, 1, 1,
+                                   1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1, 3, 9};
 
 struct BcHost {
   int zone, type, value_type, amp_id;
@@ -481,6 +481,15 @@ void allocate(Ctx *c) {
   d.cfl = c->cfg.cfl;
   d.sqrt_h_a = c->cfg.sqrt_h_a;
   d.min_ts = c->cfg.min_ts;
+  d.locking = c->cfg.locking ? 1 : 0;
+  d.gc_on = c->cfg.grad_corr ? 1 : 0;
+  if (d.gc_on) {
+    // gradCorrM starts as the identity; the first step (or a rebuild) forms it
+    d.gcm = c->alloc<double>(6 * np);
+    std::vector<double> eye(6 * np, 0.0);
+    for (size_t k = 0; k < 3 * np; ++k) eye[k] = 1.0;
+    dput(c, d.gcm, eye);
+  }
   if (d.mixed) {
     // stage_cnt = -1 marks a tile that gathers from HBM
     d.stage_cnt = c->alloc<int>((size_t)d.ntiles);
@@ -761,6 +770,7 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr) {
   {
     Timed t(c, T_NBR);
     launch_neighbours(d, c->st, cur);
+    if (d.gc_on) launch_gradcorr(d, c->st, cur);
     if (d.contact) launch_surface(d, c->st, cur, 0);
   }
   if (tp) launch_tape_lists(d, c->st, tp->s, tp->force_lists, tp->ver, tp->slot);
@@ -845,6 +855,8 @@ void adj_check(Ctx *c) {
   if (c->cfg.auto_ts) throw Error(WF_EUNSUPPORTED, "the adjoint needs a fixed time step (auto_ts 0)");
   if (c->cfg.thermal_solver) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover the thermal solver");
   if (c->d.ti_on) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover the tensile-instability term");
+  if (c->d.gc_on || c->d.locking)
+    throw Error(WF_EUNSUPPORTED, "the adjoint covers the Nishimura sums without kernel-gradient correction");
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1037,6 +1049,7 @@ int wf_set_field(void *ctx, int32_t field, const double *h) {
   return guard(c, [&] {
     as_ctx(ctx);
     if (field < 0 || field >= WF_NFIELDS || !h) throw Error(WF_EINVAL, "bad field");
+    if (field == WF_F_CONTFORCE || field == WF_F_GRADCORR) throw Error(WF_EINVAL, "read-only field");
     const size_t cnt = (size_t)kFieldDim[field] * c->n;
     c->hf[field].assign(h, h + cnt);
     if (!c->inited) return WF_OK;
@@ -1187,6 +1200,14 @@ int wf_get_field(void *ctx, int32_t field, double *h) {
           get_d4(c, d.cforce, h, 0, 3, 3);
         else
           std::memset(h, 0, sizeof(double) * 3 * (size_t)c->n);
+        break;
+      case WF_F_GRADCORR:
+        if (d.gc_on) {
+          get_sym(c, d.gcm, h);
+        } else {
+          for (int64_t k = 0; k < c->n; ++k)
+            for (int q = 0; q < 9; ++q) h[9 * (size_t)k + q] = (q % 4 == 0) ? 1.0 : 0.0;
+        }
         break;
       default: throw Error(WF_EINVAL, "bad field");
     }
@@ -1352,6 +1373,11 @@ int wf_init(void *ctx, double dt) {
     if (c->cfg.thermal_solver)
       for (int64_t i = 0; i < c->n; ++i)
         if (!(c->mats[c->hi[WF_I_MATERIAL][i]].cp_T > 0)) throw Error(WF_EINVAL, "thermal solver needs cp_T > 0");
+    if (c->cfg.grad_corr)
+      for (int64_t i = 0; i < c->n; ++i)
+        if (!c->hi[WF_I_FREE][i])
+          throw Error(WF_EUNSUPPORTED,
+                      "the kernel-gradient correction with fixed particles (FSMPairs) is not implemented");
     if (c->cfg.precision == WF_MIXED) {
       // the FP32 gather records carry m/rho in place of h (see Src<true>)
       std::vector<double> hm(kMaxMat, -1.0);
@@ -1811,6 +1837,7 @@ int wf_rebuild_neighbours(void *ctx) {
     if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
     launch_schedule(c->d, c->st, 1);
     launch_neighbours(c->d, c->st, c->cur);
+    if (c->d.gc_on) launch_gradcorr(c->d, c->st, c->cur);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     c->built = true;

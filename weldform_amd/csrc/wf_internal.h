@@ -101,6 +101,7 @@ struct DevFlags {
   int overflow;      // a neighbour row was truncated
   int nan;           // non-finite acceleration seen
   int max_count;     // largest neighbour count of the last build
+  int gc_due;        // the kernel-gradient correction is formed in this step
   int ts_nb_inc;     // reference's ts_nb_inc (search period after yield)
   long long nb_searches;
   long long directed_pairs;
@@ -197,6 +198,9 @@ struct Dev {
   int lanes_mx, lanes_f64;     // threads per particle in the pair passes
   int auto_ts;                 // bit 0 velocity, bit 1 acceleration criterion
   int ti_on;                   // some material has TI > 0
+  int gc_on;                   // kernel-gradient correction (Domain::gradKernelCorr)
+  int locking;                 // Locking sums: float m/rho weights in the rate gather
+  double *gcm;                 // gradCorrM, symmetric [6][npad] (FP64)
   double *tir;                 // tensile-instability stress [6][npad] (FP64)
   float *tirf;                 // its FP32 copy (mixed precision)
   double cfl, sqrt_h_a, min_ts;
@@ -262,6 +266,7 @@ void launch_tape_lists(const Dev &d, hipStream_t st, const AdjSlot &s, int force
 // launchers (wf_kernels.hip); all enqueue on `st`
 void launch_schedule(const Dev &d, hipStream_t st, int force);
 void launch_neighbours(const Dev &d, hipStream_t st, int cur);
+void launch_gradcorr(const Dev &d, hipStream_t st, int cur);
 void launch_accel(const Dev &d, hipStream_t st, int cur);
 void launch_tir(const Dev &d, hipStream_t st, int cur);
 // dt < 0 in a launcher: the kernel reads the step from the device flags

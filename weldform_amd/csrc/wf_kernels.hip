@@ -82,21 +82,24 @@ WF_KERNEL void k_schedule(Dev d, int force) {
   DevFlags *f = d.flags;
   const int isfirst = f->step == 0;
   const int mx = f->yielded;
-  int search = 0;
+  int search = 0, periodic = 0;
   if (force) {
-    search = 1;
+    search = periodic = 1;
   } else {
     if (mx && !f->isyielding) {
       search = 1;
       f->isyielding = 1;
       f->cleared = 0;
     } else if ((mx || isfirst) && f->ts_i == 0 && f->cleared) {
-      search = 1;
+      search = periodic = 1;
       f->cleared = 0;
     }
   }
   f->step_yield = mx;
   f->do_search = search;
+  // CalcGradCorrMatrix follows the periodic search and runs on the first
+  // step, not after the first-yield search (Source/SolverLeapfrog.cpp:160-212)
+  f->gc_due = d.gc_on && (periodic || isfirst);
   if (search) {
     f->nb_searches += 1;
     f->max_count = 0;
@@ -399,6 +402,54 @@ WF_KERNEL void k_stage(Dev d, int stage) {
   }
 }
 
+// Domain::CalcGradCorrMatrix (Source/Domain.cpp:1832-1940), 3D:
+// L_i = (-sum_j V_j GK_ij x_ij (x) x_ij)^-1 over i's neighbour row (FP64
+// gather, V_j = m_j / rho_j, x_ij = x_i - x_j), symmetric; the identity
+// where |det| < 1e-10 (InvLog, External/matvec.h:673-693) and for fixed
+// particles.  Runs when the schedule set gc_due.
+WF_KERNEL void k_gradcorr(Dev d, int cur) {
+  if (!d.flags->gc_due) return;
+  const long long np = d.npad;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+    double L[6] = {1.0, 1.0, 1.0, 0.0, 0.0, 0.0};
+    if (d.group[i] >= 0) {
+      const double4 xi = d.xh[cur][i];
+      const KH<double> ki = kern_h_t<double>(xi.w);
+      double m[6] = {0, 0, 0, 0, 0, 0};
+      const int cnt = d.nnb[i];
+      for (int k = 0; k < cnt; ++k) {
+        const int j = d.nbr[nb_slot(np, k, i)];
+        const double4 xj = d.xh[cur][j];
+        const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const double GK = gk_t<double>(dx * dx + dy * dy + dz * dz, pair_h_t<double>(ki, xi.w, xj.w));
+        const double w = d.vol[cur][j] * GK;
+        m[XX] -= w * dx * dx;
+        m[YY] -= w * dy * dy;
+        m[ZZ] -= w * dz * dz;
+        m[XY] -= w * dx * dy;
+        m[XZ] -= w * dx * dz;
+        m[YZ] -= w * dy * dz;
+      }
+      const double c00 = m[YY] * m[ZZ] - m[YZ] * m[YZ], c01 = m[XZ] * m[YZ] - m[XY] * m[ZZ];
+      const double c02 = m[XY] * m[YZ] - m[XZ] * m[YY];
+      const double det = m[XX] * c00 + m[XY] * c01 + m[XZ] * c02;
+      if (fabs(det) >= 1.0e-10) {
+        L[XX] = c00 / det;
+        L[XY] = c01 / det;
+        L[XZ] = c02 / det;
+        L[YY] = (m[XX] * m[ZZ] - m[XZ] * m[XZ]) / det;
+        L[YZ] = (m[XZ] * m[XY] - m[XX] * m[YZ]) / det;
+        L[ZZ] = (m[XX] * m[YY] - m[XY] * m[XY]) / det;
+      }
+    }
+    for (int k = 0; k < 6; ++k) d.gcm[k * np + i] = L[k];
+  }
+}
+
+void launch_gradcorr(const Dev &d, hipStream_t st, int cur) {
+  k_gradcorr<<<nblk_small(d.n), kBlock, 0, st>>>(d, cur);
+}
+
 void launch_schedule(const Dev &d, hipStream_t st, int force) {
   k_schedule<<<1, 64, 0, st>>>(d, force);
 }
@@ -572,13 +623,39 @@ __device__ inline R w_shape(R q) {
 // TI: with the tensile-instability term (W(r)/W(d0))^n (TIR_i + TIR_j) of
 // CalcAccel (Source/InteractionAlt.cpp:104-110); pairs form within one
 // material, so n and d0 are the material's
-template <bool MX, bool TI, class G>
+// particle i's gradCorrM (symmetric, 6 components): the identity when the
+// correction is off
+template <bool GC, typename R>
+__device__ inline void own_corr(const Dev &d, int i, R (&L)[6]) {
+  L[XX] = L[YY] = L[ZZ] = (R)1;
+  L[XY] = L[XZ] = L[YZ] = (R)0;
+  if constexpr (GC)
+    for (int k = 0; k < 6; ++k) L[k] = (R)d.gcm[(long long)k * d.npad + i];
+}
+// e = L x_ij: the direction the kernel gradient GK e takes under the
+// correction, (GK L) x_ij of Source/InteractionAlt.cpp:118-133
+template <bool GC, typename R>
+__device__ inline void corr_dir(const R (&L)[6], R dx, R dy, R dz, R &ex, R &ey, R &ez) {
+  if constexpr (GC) {
+    ex = L[XX] * dx + L[XY] * dy + L[XZ] * dz;
+    ey = L[XY] * dx + L[YY] * dy + L[YZ] * dz;
+    ez = L[XZ] * dx + L[YZ] * dy + L[ZZ] * dz;
+  } else {
+    ex = dx, ey = dy, ez = dz;
+  }
+}
+
+// GC: with the kernel-gradient correction (particle i's own gradCorrM
+// applied to x_ij in the pair term)
+template <bool MX, bool TI, bool GC, class G>
 __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int sub, int lanes,
                                   typename Src<MX>::R (&acc)[3]) {
   using S = Src<MX>;
   using R = typename S::R;
   R ax = 0, ay = 0, az = 0;
   if (d.group[i] >= 0) {
+    R Li[6];
+    own_corr<GC>(d, i, Li);
     const DevMat &M = d.mats[d.mat[i]];
     const R alpha = (R)M.alpha, beta = (R)M.beta, cs = (R)M.Cs;
     const bool av = (M.alpha != 0.0 || M.beta != 0.0);
@@ -629,9 +706,11 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
         Myz += s * (tri[YZ] + trj[YZ]);
       }
       const R w = vj.w * GK;
-      ax += w * (dx * Mxx2 + dy * Mxy + dz * Mxz);
-      ay += w * (dx * Mxy + dy * Myy2 + dz * Myz);
-      az += w * (dx * Mxz + dy * Myz + dz * Mzz2);
+      R ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
+      ax += w * (ex * Mxx2 + ey * Mxy + ez * Mxz);
+      ay += w * (ex * Mxy + ey * Myy2 + ez * Myz);
+      az += w * (ex * Mxz + ey * Myz + ez * Mzz2);
     });
   }
   acc[0] = ax;
@@ -647,8 +726,14 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
   double a[3] = {0, 0, 0};
   if (i < d.n) {
-    if (d.ti_on) accel_lane<false, true>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES, a);
-    else accel_lane<false, false>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES, a);
+    const HbmJ<false> g{d, cur, i};
+    if (d.gc_on) {
+      if (d.ti_on) accel_lane<false, true, true>(d, cur, i, g, sub, LANES, a);
+      else accel_lane<false, false, true>(d, cur, i, g, sub, LANES, a);
+    } else {
+      if (d.ti_on) accel_lane<false, true, false>(d, cur, i, g, sub, LANES, a);
+      else accel_lane<false, false, false>(d, cur, i, g, sub, LANES, a);
+    }
   }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4(a[0], a[1], a[2], 0.0);
@@ -668,12 +753,22 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
   float a[3] = {0, 0, 0};
   if (U < 0) {
     if (i < d.n) {
-      if (d.ti_on) accel_lane<true, true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
-      else accel_lane<true, false>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES, a);
+      const HbmJ<true> g{d, cur, i};
+      if (d.gc_on) {
+        if (d.ti_on) accel_lane<true, true, true>(d, cur, i, g, sub, LANES, a);
+        else accel_lane<true, false, true>(d, cur, i, g, sub, LANES, a);
+      } else {
+        if (d.ti_on) accel_lane<true, true, false>(d, cur, i, g, sub, LANES, a);
+        else accel_lane<true, false, false>(d, cur, i, g, sub, LANES, a);
+      }
     }
   } else {
     stage_accel(d, t, U, d.xhf[cur], lx, lv, lt, lt2);
-    if (i < d.n) accel_lane<true, false>(d, cur, i, LdsJ{d, lx, lv, lt, lt2, nullptr, i, d.selfl[i]}, sub, LANES, a);
+    if (i < d.n) {
+      const LdsJ g{d, lx, lv, lt, lt2, nullptr, i, d.selfl[i]};
+      if (d.gc_on) accel_lane<true, false, true>(d, cur, i, g, sub, LANES, a);
+      else accel_lane<true, false, false>(d, cur, i, g, sub, LANES, a);
+    }
   }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
@@ -919,12 +1014,14 @@ void launch_kick2(const Dev &d, hipStream_t st, double dt) { k_kick2<<<nblk(d.n)
 //  Source/SolverLeapfrog.cpp:280-301).  Reads x_n, v_{n+1/2}, m/rho_n and
 // writes rho_{n+1}, m/rho_{n+1}, x_{n+1} into the other buffer.
 
-template <bool MX, class G>
+template <bool MX, bool GC, class G>
 __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, const G &g, int sub, int lanes) {
   using S = Src<MX>;
   using R = typename S::R;
   R s = 0;
   if (d.group[i] >= 0) {
+    R Li[6];
+    own_corr<GC>(d, i, Li);
     const auto xi = S::x(d, cur, i);
     const auto vi = S::v(d, i);
     const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
@@ -934,10 +1031,12 @@ __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, co
       const R volj = MX ? (R)xj.w : S::vol(d, cur, j, xj);
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, S::kh(ki, xi, xj));
+      R ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
       if constexpr (MX) {
-        s += (volj * GK) * ((vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz);
+        s += (volj * GK) * ((vi.x - vj.x) * ex + (vi.y - vj.y) * ey + (vi.z - vj.z) * ez);
       } else {
-        const R t = (vi.x - vj.x) * (GK * dx) + (vi.y - vj.y) * (GK * dy) + (vi.z - vj.z) * (GK * dz);
+        const R t = (vi.x - vj.x) * (GK * ex) + (vi.y - vj.y) * (GK * ey) + (vi.z - vj.z) * (GK * ez);
         s += volj * t;
       }
     });
@@ -981,7 +1080,10 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_f64(Dev d, int cur, doub
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
   double s[1] = {0};
-  if (i < d.n) s[0] = dens_lane<false>(d, cur, i, HbmJ<false>{d, cur, i}, sub, LANES);
+  if (i < d.n) {
+    const HbmJ<false> g{d, cur, i};
+    s[0] = d.gc_on ? dens_lane<false, true>(d, cur, i, g, sub, LANES) : dens_lane<false, false>(d, cur, i, g, sub, LANES);
+  }
   combine_lanes<LANES>(s, red);
   if (i < d.n && sub == 0) dens_update<false>(d, cur, i, s[0], dt, do_update);
 }
@@ -995,10 +1097,16 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, doubl
   const int U = d.stage_cnt[t];
   float s[1] = {0};
   if (U < 0) {
-    if (i < d.n) s[0] = dens_lane<true>(d, cur, i, HbmJ<true>{d, cur, i}, sub, LANES);
+    if (i < d.n) {
+      const HbmJ<true> g{d, cur, i};
+      s[0] = d.gc_on ? dens_lane<true, true>(d, cur, i, g, sub, LANES) : dens_lane<true, false>(d, cur, i, g, sub, LANES);
+    }
   } else {
     stage_xv(d, t, U, d.xhf[cur], lx, lv, nullptr);
-    if (i < d.n) s[0] = dens_lane<true>(d, cur, i, LdsJ{d, lx, lv, nullptr, nullptr, nullptr, i, d.selfl[i]}, sub, LANES);
+    if (i < d.n) {
+      const LdsJ g{d, lx, lv, nullptr, nullptr, nullptr, i, d.selfl[i]};
+      s[0] = d.gc_on ? dens_lane<true, true>(d, cur, i, g, sub, LANES) : dens_lane<true, false>(d, cur, i, g, sub, LANES);
+    }
   }
   combine_lanes<LANES>(s, red);
   if (i < d.n && sub == 0) dens_update<true>(d, cur, i, (double)s[0], dt, do_update);
@@ -1024,7 +1132,9 @@ void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_
 
 // one particle's rate-tensor gather: v[0..5] strain rate, v[6..8] rotation
 // rate (w01, w02, w12), v[9] the kinetic-energy pair sum
-template <bool MX, class G>
+// LK: the Locking sums' weight, m_j / rho_j rounded to float (the FP32
+// path's weight already is)
+template <bool MX, bool GC, bool LK, class G>
 __device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int sub, int lanes,
                                   typename Src<MX>::R (&v)[10]) {
   using Sr = Src<MX>;
@@ -1032,6 +1142,8 @@ __device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int 
   R sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0};
   R ekin = 0;
   if (d.group[i] >= 0) {
+    R Li[6];
+    own_corr<GC>(d, i, Li);
     const DevMat &M = d.mats[d.mat[i]];
     const auto xi = Sr::x(d, buf, i);
     const auto vi = Sr::v(d, i);
@@ -1042,20 +1154,23 @@ __device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int 
     for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
-      const R volj = MX ? (R)xj.w : Sr::vol(d, buf, j, xj);
+      R volj = MX ? (R)xj.w : Sr::vol(d, buf, j, xj);
+      if constexpr (LK && !MX) volj = (R)(float)volj;
       const R dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const R GK = gk_t<R>(dx * dx + dy * dy + dz * dz, Sr::kh(ki, xi, xj));
       const R vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
       const R c = volj * ((R)-0.5 * GK);
-      sr[XX] += c * ((R)2 * vx * dx);
-      sr[YY] += c * ((R)2 * vy * dy);
-      sr[ZZ] += c * ((R)2 * vz * dz);
-      sr[XY] += c * (vx * dy + vy * dx);
-      sr[XZ] += c * (vx * dz + vz * dx);
-      sr[YZ] += c * (vy * dz + vz * dy);
-      w[0] += c * (vx * dy - vy * dx);
-      w[1] += c * (vx * dz - vz * dx);
-      w[2] += c * (vy * dz - vz * dy);
+      R ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
+      sr[XX] += c * ((R)2 * vx * ex);
+      sr[YY] += c * ((R)2 * vy * ey);
+      sr[ZZ] += c * ((R)2 * vz * ez);
+      sr[XY] += c * (vx * ey + vy * ex);
+      sr[XZ] += c * (vx * ez + vz * ex);
+      sr[YZ] += c * (vy * ez + vz * ey);
+      w[0] += c * (vx * ey - vy * ex);
+      w[1] += c * (vx * ez - vz * ex);
+      w[2] += c * (vy * ez - vz * ey);
       // kinetic-energy pair term (Source/Domain.cpp:1776-1792): the pair
       // is oriented lower-index first; its sum over particles vanishes
       const R prj = g.pr(j, MX ? (R)0 : Sr::rho(d, buf, j, xj, vj), P0, cs2, rho0);
@@ -1085,7 +1200,16 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_f64(Dev d, int buf) {
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (i < d.n) rates_lane<false>(d, buf, i, HbmJ<false>{d, buf, i}, sub, LANES, v);
+  if (i < d.n) {
+    const HbmJ<false> g{d, buf, i};
+    if (d.gc_on) {
+      if (d.locking) rates_lane<false, true, true>(d, buf, i, g, sub, LANES, v);
+      else rates_lane<false, true, false>(d, buf, i, g, sub, LANES, v);
+    } else {
+      if (d.locking) rates_lane<false, false, true>(d, buf, i, g, sub, LANES, v);
+      else rates_lane<false, false, false>(d, buf, i, g, sub, LANES, v);
+    }
+  }
   combine_lanes<LANES>(v, red);
   const double e_kin = (i < d.n && sub == 0) ? rates_store(d, i, v) : 0.0;
   const double r1 = block_reduce<0>(e_kin, sh);
@@ -1103,10 +1227,18 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
   const int U = d.stage_cnt[t];
   float v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (U < 0) {
-    if (i < d.n) rates_lane<true>(d, buf, i, HbmJ<true>{d, buf, i}, sub, LANES, v);
+    if (i < d.n) {
+      const HbmJ<true> g{d, buf, i};
+      if (d.gc_on) rates_lane<true, true, false>(d, buf, i, g, sub, LANES, v);
+      else rates_lane<true, false, false>(d, buf, i, g, sub, LANES, v);
+    }
   } else {
     stage_xv(d, t, U, d.xhf[buf], lx, lv, lp);
-    if (i < d.n) rates_lane<true>(d, buf, i, LdsJ{d, lx, lv, nullptr, nullptr, lp, i, d.selfl[i]}, sub, LANES, v);
+    if (i < d.n) {
+      const LdsJ g{d, lx, lv, nullptr, nullptr, lp, i, d.selfl[i]};
+      if (d.gc_on) rates_lane<true, true, false>(d, buf, i, g, sub, LANES, v);
+      else rates_lane<true, false, false>(d, buf, i, g, sub, LANES, v);
+    }
   }
   combine_lanes<LANES>(v, red);
   const double e_kin = (i < d.n && sub == 0) ? rates_store(d, i, v) : 0.0;

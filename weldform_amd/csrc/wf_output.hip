@@ -68,9 +68,11 @@ WF_KERNEL void k_pack_xdmf(Dev d, int cur, float *f, int *iv) {
   put_t6(col(WF_XDMF_STRAIN, 6), d.strain, np, k);
   put_t6(col(WF_XDMF_STRAIN_RATE, 6), d.sr, np, k);
   put_t6(col(WF_XDMF_STRAIN_PL, 6), d.epl, np, k);
-  {
-    // gradCorrM is only formed by the kernel-gradient correction, which this
-    // step does not run; the reference leaves it unset then
+  if (d.gc_on) {
+    put_t6(col(WF_XDMF_GRADCORRMAT, 6), d.gcm, np, k);
+  } else {
+    // gradCorrM is only formed by the kernel-gradient correction; the
+    // reference leaves it unset without it
     float *g = col(WF_XDMF_GRADCORRMAT, 6);
     for (int c = 0; c < 6; ++c) g[c] = 0.f;
   }
