@@ -241,7 +241,7 @@ class Domain {
   // Domain::AddTrimeshParticles (Source/Contact.cpp:10-37): one particle per
   // element at its centroid, h = hfac * element radius, zone id `id`
   void AddTrimeshParticles(TriMesh *mesh, const float &hfac, const int &id) {
-    if (!mesh || mesh->dens <= 0) throw new Fatal("AddTrimeshParticles: the mesh must come from AxisPlaneMesh");
+    if (!mesh || mesh->element.Size() == 0) throw new Fatal("AddTrimeshParticles: empty mesh");
     first_fem_particle_idx.push_back((int)Particles.Size());
     contact_surf_id.push_back(id);
     trimesh.push_back(mesh);
@@ -355,8 +355,15 @@ class Domain {
       if (trimesh.empty()) throw new Fatal("contact is set but no rigid mesh was added (AddTrimeshParticles)");
       if (contact_alg != 0) throw new Fatal("only the Wang contact algorithm is implemented on the device path");
       if (cont_heat_cond || cont_heat_fric) throw new Fatal("contact heat transfer is not implemented on the device path");
-      for (TriMesh *m : trimesh)
-        if (norm(m->m_w) != 0.0) throw new Fatal("rotating contact meshes are not implemented on the device path");
+      // a mesh's m_w comes from the constant velocity BC of its zone (UserAcc
+      // sets it every call); one set directly on a mesh without such a BC is
+      // not carried to the device
+      for (TriMesh *m : trimesh) {
+        if (norm(m->m_w) == 0.0) continue;
+        bool bc = false;
+        for (const auto &b : bConds) bc = bc || (b.type == Velocity_BC && b.valueType == 0 && b.zoneId == m->id);
+        if (!bc) throw new Fatal("a rotating contact mesh needs a constant velocity BC on its zone (valueAng)");
+      }
       if (trimesh.size() > 4) throw new Fatal("at most 4 contact meshes");
     }
     if (GeneralBefore != &General || GeneralAfter != &General)
@@ -709,14 +716,34 @@ class Domain {
       if (b.type == Velocity_BC) {
         const double v[3] = {b.value(0), b.value(1), b.value(2)};
         check(wf_add_velocity_bc(ctx_, b.zoneId, b.valueType, v, b.ampId, b.ampFactor));
+        if (norm(b.value_ang) != 0.0) {
+          const double w[3] = {b.value_ang(0), b.value_ang(1), b.value_ang(2)};
+          check(wf_set_bc_angular_velocity(ctx_, b.zoneId, w));
+        }
       } else if (b.type == Temperature_BC) {
         check(wf_add_temperature_bc(ctx_, b.zoneId, b.T));
       }
     }
     for (size_t m = 0; m < trimesh.size(); ++m) {
       const TriMesh *M = trimesh[m];
-      const double a[3] = {M->p1(0), M->p1(1), M->p1(2)}, b[3] = {M->p2(0), M->p2(1), M->p2(2)};
-      const int rc = wf_add_rigid_plane(ctx_, M->id, M->axis, M->flip ? 1 : 0, a, b, M->dens, (double)mesh_hfac_[m]);
+      int rc;
+      if (M->dens > 0) {
+        const double a[3] = {M->p1(0), M->p1(1), M->p1(2)}, b[3] = {M->p2(0), M->p2(1), M->p2(2)};
+        rc = wf_add_rigid_plane(ctx_, M->id, M->axis, M->flip ? 1 : 0, a, b, M->dens, (double)mesh_hfac_[m]);
+      } else {
+        // a mesh read from a file: its nodes, triangles and current normals
+        std::vector<double> nd(3 * M->node.Size()), nr(3 * M->element.Size());
+        std::vector<int32_t> el(3 * M->element.Size());
+        for (size_t n = 0; n < M->node.Size(); ++n)
+          for (int k = 0; k < 3; ++k) nd[3 * n + k] = (*M->node[n])(k);
+        for (size_t e = 0; e < M->element.Size(); ++e)
+          for (int k = 0; k < 3; ++k) {
+            el[3 * e + k] = M->element[e]->node[k];
+            nr[3 * e + k] = M->element[e]->normal(k);
+          }
+        rc = wf_add_rigid_mesh(ctx_, M->id, (int64_t)M->node.Size(), nd.data(), (int64_t)M->element.Size(), el.data(),
+                               nr.data(), (double)mesh_hfac_[m]);
+      }
       if (rc < 0) check(rc);
     }
     if (contact) check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));

@@ -1,12 +1,15 @@
 // weldform/Mesh.h -- rigid triangle meshes for contact, with the interface
 // of the reference's TriMesh (Source/Mesh.h:30-90): AxisPlaneMesh builds a
-// plane of 2 dens^2 triangles (Source/Mesh.cpp:82-202), CalcSpheres the
-// element radii and plane coefficients (:206-227).  The device receives the
-// plane's parameters and rebuilds the same mesh; this host copy supplies the
-// particle positions of Domain::AddTrimeshParticles.
+// plane of 2 dens^2 triangles (Source/Mesh.cpp:82-202), TriMesh(reader,
+// flip) a mesh read from a Nastran file (:17-64), CalcSpheres the element
+// radii and plane coefficients (:206-227).  The device receives a plane's
+// parameters, or a file mesh's nodes and triangles, and rebuilds the same
+// mesh; this host copy supplies the particle positions of
+// Domain::AddTrimeshParticles.
 #pragma once
 #include <cmath>
 
+#include "NastranReader.h"
 #include "matvec.h"
 
 namespace SPH {
@@ -37,10 +40,47 @@ class TriMesh {
   Vec3_t m_v, m_w;
   int id = 0;
   int dimension = 3;
-  // the plane's parameters, passed to the device
+  // the plane's parameters, passed to the device (dens = 0: a mesh read
+  // from a file, passed as nodes and triangles)
   int axis = 2, dens = 0;
   bool flip = false;
   Vec3_t p1, p2;
+
+  // TriMesh(NastranReader &, flipnormals), 3D: nodes as read, triangles
+  // with their first two nodes swapped when flipnormals is set; centroids,
+  // and normals from the file's winding -- the swap shows only once normals
+  // are recomputed (a translation, Scale, or the per-step TriMesh::Update)
+  TriMesh(const NastranReader &nr, bool flipnormals) {
+    if (nr.dim != 3) throw new Fatal("TriMesh: only 3D meshes are supported");
+    for (int n = 0; n < nr.node_count; ++n) {
+      node.Push(new Vec3_t(nr.node[3 * n], nr.node[3 * n + 1], nr.node[3 * n + 2]));
+      node_v.Push(new Vec3_t(0., 0., 0.));
+    }
+    for (int e = 0; e < nr.elem_count; ++e) {
+      const int a = nr.elcon[3 * e], b = nr.elcon[3 * e + 1], c = nr.elcon[3 * e + 2];
+      Element *el = flipnormals ? new Element(b, a, c) : new Element(a, b, c);
+      for (int k = 0; k < 3; ++k) el->centroid(k) = ((*node[a])(k) + (*node[b])(k) + (*node[c])(k)) / 3.;
+      const Vec3_t w = cross(*node[b] - *node[a], *node[c] - *node[a]);
+      el->normal = w / norm(w);
+      element.Push(el);
+    }
+  }
+
+  // TriMesh::CalcCentroids (Source/Mesh.cpp:71-79)
+  void CalcCentroids() {
+    for (size_t e = 0; e < element.Size(); ++e) {
+      Vec3_t c;
+      for (int i = 0; i < 3; ++i) c = c + *node[element[e]->node[i]];
+      element[e]->centroid = c / 3.;
+    }
+  }
+  // TriMesh::Scale (Source/Mesh.cpp:310-323): nodes scaled, centroids and
+  // normals recomputed
+  void Scale(const double &f) {
+    for (size_t n = 0; n < node.Size(); ++n) *node[n] = f * *node[n];
+    CalcCentroids();
+    CalcNormals();
+  }
 
   void SetVel(const Vec3_t &v) { m_v = v; }
   void SetRotAxisVel(const Vec3_t &w) { m_w = w; }

@@ -235,6 +235,12 @@ int wf_add_amplitude(void *ctx, int32_t id, int32_t npts, const double *time, co
 int wf_add_velocity_bc(void *ctx, int32_t zone, int32_t value_type, const double value[3],
                        int32_t amp_id, double amp_factor);
 int wf_add_temperature_bc(void *ctx, int32_t zone, double T);
+/* boundaryCondition::value_ang (Source/Domain.h:98-111) of the velocity BCs
+ * already added on `zone`: a constant BC on a rigid mesh's zone sets the
+ * mesh's angular velocity m_w (SetRotAxisVel, Source/WeldForm.cpp:97-101),
+ * and TriMesh::Update moves its nodes with m_v + m_w x node (rotation about
+ * the origin).  Amplitude BCs leave m_w as it was. */
+int wf_set_bc_angular_velocity(void *ctx, int32_t zone, const double w[3]);
 
 /* rigid contact surfaces, before wf_init.  wf_add_rigid_plane adds the
  * plane mesh of TriMesh::AxisPlaneMesh (Source/Mesh.cpp:82-202): normal to
@@ -249,6 +255,18 @@ int wf_add_temperature_bc(void *ctx, int32_t zone, double T);
  * wf_init (the next step uses the new values). */
 int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, const double p1[3], const double p2[3],
                        int32_t dens, double hfac);
+/* wf_add_rigid_mesh adds a rigid surface of nelem triangles over nnode
+ * nodes (3 doubles each; elems: 3 0-based node indices per triangle, the
+ * winding the one the mesh update recomputes normals from), as
+ * TriMesh(NastranReader &, flip) builds it from a mesh file
+ * (Source/Mesh.cpp:17-64) after the caller has applied flipNormals (first
+ * two nodes swapped), scaleFactor and translation: centroids, the initial
+ * element normals (3 per triangle; NULL: from the winding -- the reference
+ * takes them from the file's unflipped winding until they are recomputed),
+ * CalcSpheres, and particles with h = hfac * element radius.  Planes and
+ * meshes share the limit of 4 surfaces.  Returns the number of triangles. */
+int wf_add_rigid_mesh(void *ctx, int32_t zone, int64_t nnode, const double *nodes, int64_t nelem,
+                      const int32_t *elems, const double *normals, double hfac);
 int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double damping);
 /* contact force on a plane (Domain::m_contact_force[mesh]) per step; last
  * min(cap, steps) entries */

@@ -60,6 +60,7 @@ def lib():
         L.wfo_last_error.restype = ctypes.c_char_p
         L.wfo_add_velocity_bc.argtypes = [vp, i32, i32, dp, i32, d]
         L.wfo_add_temperature_bc.argtypes = [vp, i32, d]
+        L.wfo_set_bc_angular.argtypes = [vp, i32, dp]
         L.wfo_add_amplitude.argtypes = [vp, i32, i32, dp, dp]
         L.wfo_assign_zone.argtypes = [vp, dp, dp, i32]
         L.wfo_gen_cylinder.restype = i64
@@ -79,6 +80,7 @@ def lib():
         L.wfo_pairs.argtypes = [vp]
         L.wfo_reaction.argtypes = [vp, i32, dp]
         L.wfo_add_plane_mesh.argtypes = [vp, i32, i32, i32, i32, dp, dp, i32, d]
+        L.wfo_add_mesh.argtypes = [vp, i32, i32, i32, dp, i32, ctypes.POINTER(ctypes.c_int), dp, d]
         L.wfo_contact_force.argtypes = [vp, i32, dp]
         L.wfo_ncontact_pairs.restype = i64
         L.wfo_ncontact_pairs.argtypes = [vp]
@@ -169,6 +171,10 @@ class Domain:
         v = np.ascontiguousarray(value, dtype=np.float64)
         self._check(lib().wfo_add_velocity_bc(self.h, zone, value_type, _dp(v), amp_id, amp_factor))
 
+    def set_bc_angular(self, zone, w):
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        self._check(lib().wfo_set_bc_angular(self.h, int(zone), _dp(a)))
+
     def add_temperature_bc(self, zone, T):
         self._check(lib().wfo_add_temperature_bc(self.h, zone, float(T)))
 
@@ -235,6 +241,20 @@ class Domain:
         b = np.ascontiguousarray(p2, dtype=np.float64)
         rc = lib().wfo_add_plane_mesh(self.h, int(first), int(zone), int(axis), int(bool(flip)), _dp(a), _dp(b),
                                       int(dens), float(hfac))
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def add_mesh(self, first, zone, nodes, elems, hfac=1.1, normals=None):
+        """A triangle mesh from explicit nodes (nn, 3), 0-based elements
+        (ne, 3) and optional initial normals (ne, 3) + CalcSpheres +
+        AddTrimeshParticles from index `first`."""
+        nd = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
+        el = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 3)
+        nr = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+        rc = lib().wfo_add_mesh(self.h, int(first), int(zone), len(nd), _dp(nd), len(el),
+                                el.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None if nr is None else _dp(nr),
+                                float(hfac))
         if rc < 0:
             self._check(rc)
         return rc

@@ -43,7 +43,7 @@ static const int k_fdim[WFO_NFIELDS] = {
     9                     /* GCM */
 };
 
-typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T; } bc_t;
+typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T, value_ang[3]; } bc_t;
 
 /* rigid triangle mesh (Source/Mesh.h, Source/Mesh.cpp) */
 typedef struct {
@@ -53,6 +53,7 @@ typedef struct {
   double *centroid, *normal, *pplane, *radius;
   int *nfar;
   double mv[3];                   /* TriMesh::m_v */
+  double mw[3];                   /* TriMesh::m_w (rotation about the origin) */
 } mesh_t;
 typedef struct { int id, n; double t[WFO_MAX_AMP_PTS], v[WFO_MAX_AMP_PTS]; } amp_t;
 
@@ -329,6 +330,17 @@ int wfo_add_velocity_bc(wfo_domain *d, int zone, int value_type, const double va
   for (int k = 0; k < 3; k++) b->value[k] = value[k];
   return 0;
 }
+/* boundaryCondition::value_ang of the velocity BCs of a zone: the rotation
+ * a constant BC gives its rigid mesh (Source/WeldForm.cpp:97-101) */
+int wfo_set_bc_angular(wfo_domain *d, int zone, const double w[3]) {
+  int found = 0;
+  for (int b = 0; b < d->nbcs; b++)
+    if (d->bcs[b].type == 0 && d->bcs[b].zone == zone) {
+      for (int k = 0; k < 3; k++) d->bcs[b].value_ang[k] = w[k];
+      found = 1;
+    }
+  return found ? 0 : fail("no velocity BC on that zone");
+}
 int wfo_add_temperature_bc(wfo_domain *d, int zone, double T) {
   if (d->nbcs >= WFO_MAX_BCS) return fail("too many BCs");
   bc_t *b = &d->bcs[d->nbcs++];
@@ -390,7 +402,10 @@ int wfo_general_after(wfo_domain *d) {
     for (int m = 0; m < d->nmesh; m++) {
       if (d->mesh[m].id != bc->zone) continue;
       if (bc->value_type == 0) {
-        for (int k = 0; k < 3; k++) d->mesh[m].mv[k] = bc->value[k];
+        for (int k = 0; k < 3; k++) {
+          d->mesh[m].mv[k] = bc->value[k];
+          d->mesh[m].mw[k] = bc->value_ang[k];   /* SetRotAxisVel, constant BCs only */
+        }
       } else if (bc->value_type == 1) {
         for (int j = 0; j < d->namps; j++) {
           if (d->amps[j].id != bc->amp_id) continue;
@@ -1379,16 +1394,9 @@ static void mesh_plane_coeff(mesh_t *M) {
     M->pplane[e] = dot3(M->node + 3 * M->el[3 * e + M->nfar[e]], M->normal + 3 * e);
 }
 
-/* TriMesh::AxisPlaneMesh, 3D (Source/Mesh.cpp:82-202) followed by
- * CalcSpheres (:206-220, whose loop leaves nfar = the last node), then
- * Domain::AddTrimeshParticles (Source/Contact.cpp:10-37) with h =
- * hfac * radius, hfac passed as float. */
-int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, const double p1[3], const double p2[3],
-                       int dens, double hfac) {
-  if (d->nmesh >= WFO_MAX_MESH) return fail("too many meshes");
-  if (dens < 1) return fail("mesh density must be >= 1");
-  const int nelem = 2 * dens * dens, nnodes = (dens + 1) * (dens + 1);
-  if (first < 0 || first + nelem > d->n) return fail("rigid particles exceed the domain size");
+static int mesh_finish(wfo_domain *d, mesh_t *M, int first, int id, double hfac);
+
+static mesh_t *mesh_alloc(wfo_domain *d, int id, int first, int nnodes, int nelem) {
   mesh_t *M = &d->mesh[d->nmesh];
   memset(M, 0, sizeof *M);
   M->id = id;
@@ -1403,6 +1411,47 @@ int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, con
   M->pplane = (double *)xcalloc((size_t)nelem, sizeof(double));
   M->radius = (double *)xcalloc((size_t)nelem, sizeof(double));
   M->nfar = (int *)xcalloc((size_t)nelem, sizeof(int));
+  return M;
+}
+
+/* TriMesh(NastranReader &, flipnormals) (Source/Mesh.cpp:17-64), 3D, from
+ * the nodes and triangles after the driver's flip / scale / translation:
+ * centroid (n0 + n1 + n2) / 3, normal = (n1 - n0) x (n2 - n0) normalised
+ * unless given; then CalcSpheres and AddTrimeshParticles */
+int wfo_add_mesh(wfo_domain *d, int first, int id, int nnodes, const double *nodes, int nelem, const int *el,
+                 const double *normals, double hfac) {
+  if (d->nmesh >= WFO_MAX_MESH) return fail("too many meshes");
+  if (nnodes < 3 || nelem < 1) return fail("empty mesh");
+  if (first < 0 || first + nelem > d->n) return fail("rigid particles exceed the domain size");
+  for (int k = 0; k < 3 * nelem; k++)
+    if (el[k] < 0 || el[k] >= nnodes) return fail("element node out of range");
+  mesh_t *M = mesh_alloc(d, id, first, nnodes, nelem);
+  memcpy(M->node, nodes, sizeof(double) * 3 * (size_t)nnodes);
+  memcpy(M->el, el, sizeof(int) * 3 * (size_t)nelem);
+  for (int e = 0; e < nelem; e++) {
+    const double *a = M->node + 3 * el[3 * e], *b = M->node + 3 * el[3 * e + 1], *c = M->node + 3 * el[3 * e + 2];
+    for (int k = 0; k < 3; k++) M->centroid[3 * e + k] = (a[k] + b[k] + c[k]) / 3.;
+  }
+  /* the initial normals: given (the reference's file meshes start from the
+   * unflipped winding) or from the winding */
+  if (normals)
+    memcpy(M->normal, normals, sizeof(double) * 3 * (size_t)nelem);
+  else
+    mesh_normals(M);
+  return mesh_finish(d, M, first, id, hfac);
+}
+
+/* TriMesh::AxisPlaneMesh, 3D (Source/Mesh.cpp:82-202) followed by
+ * CalcSpheres (:206-220, whose loop leaves nfar = the last node), then
+ * Domain::AddTrimeshParticles (Source/Contact.cpp:10-37) with h =
+ * hfac * radius, hfac passed as float. */
+int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, const double p1[3], const double p2[3],
+                       int dens, double hfac) {
+  if (d->nmesh >= WFO_MAX_MESH) return fail("too many meshes");
+  if (dens < 1) return fail("mesh density must be >= 1");
+  const int nelem = 2 * dens * dens, nnodes = (dens + 1) * (dens + 1);
+  if (first < 0 || first + nelem > d->n) return fail("rigid particles exceed the domain size");
+  mesh_t *M = mesh_alloc(d, id, first, nnodes, nelem);
   double p[3];
   for (int k = 0; k < 3; k++) p[k] = p2[k] - p1[k];
   int dir[3];
@@ -1448,6 +1497,14 @@ int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, con
   mesh_normals(M);
   if (flip)
     for (int k = 0; k < 3 * nelem; k++) M->normal[k] = -M->normal[k];
+  return mesh_finish(d, M, first, id, hfac);
+}
+
+/* CalcSpheres (Source/Mesh.cpp:206-220: nfar = the last node) and
+ * Domain::AddTrimeshParticles (Source/Contact.cpp:10-37) for a mesh whose
+ * nodes, elements, centroids and normals are set */
+static int mesh_finish(wfo_domain *d, mesh_t *M, int first, int id, double hfac) {
+  const int nelem = M->nelem;
   for (int e = 0; e < nelem; e++) {
     double mx = 0.;
     for (int q = 0; q < 3; q++) {
@@ -1637,11 +1694,16 @@ static void mesh_update(wfo_domain *d) {
   const double dt = d->deltat;
   for (int m = 0; m < d->nmesh; m++) {
     mesh_t *M = &d->mesh[m];
-    for (int nd = 0; nd < M->nnodes; nd++)
+    /* TriMesh::Update (Source/Mesh.cpp:273-296): node_v = m_v + m_w x node,
+     * taken at the node's position before it moves */
+    for (int nd = 0; nd < M->nnodes; nd++) {
+      double vr[3];
+      cross3(M->mw, M->node + 3 * nd, vr);
       for (int k = 0; k < 3; k++) {
-        M->node_v[3 * nd + k] = M->mv[k] + 0.0;
+        M->node_v[3 * nd + k] = M->mv[k] + vr[k];
         M->node[3 * nd + k] += M->node_v[3 * nd + k] * dt;
       }
+    }
     for (int e = 0; e < M->nelem; e++)
       for (int k = 0; k < 3; k++) {
         double c = 0.;

@@ -81,6 +81,7 @@ const char *wfo_last_error(void);
 /* boundary conditions (Source/Domain.h:69-111, Source/WeldForm.cpp:51-92) */
 int wfo_add_velocity_bc(wfo_domain *d, int zone, int value_type,
                         const double value[3], int amp_id, double amp_factor);
+int wfo_set_bc_angular(wfo_domain *d, int zone, const double w[3]);
 int wfo_add_temperature_bc(wfo_domain *d, int zone, double T);
 int wfo_add_amplitude(wfo_domain *d, int id, int npts, const double *t,
                       const double *v);
@@ -112,6 +113,8 @@ int wfo_plastic_work_heat(wfo_domain *d, double dt); /* CalcPlasticWorkHeat */
 int wfo_energies(wfo_domain *d);                   /* Kin + Int EnergyEqn  */
 /* rigid plane meshes and contact (Source/Mesh.cpp, Source/Contact*.cpp):
  * the mesh's particles are written at [first, first + 2 dens^2) */
+int wfo_add_mesh(wfo_domain *d, int first, int id, int nnodes, const double *nodes, int nelem, const int *el,
+                 const double *normals, double hfac);    /* general triangle mesh (TriMesh from a mesh file) */
 int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, const double p1[3],
                        const double p2[3], int dens, double hfac);
 int wfo_contact_force(const wfo_domain *d, int mesh, double out[3]);

@@ -10,7 +10,7 @@ def oracle_from_case(case, threads=1):
     particle per triangle of each rigid plane (the reference's order)."""
     po.set_threads(threads)
     n = case.n
-    nrig = sum(2 * m["dens"] ** 2 for m in case.meshes)
+    nrig = sum(len(m["elems"]) if "elems" in m else 2 * m["dens"] ** 2 for m in case.meshes)
     d = po.Domain(n + nrig)
     s = slice(0, n)
     d.f("X")[s] = case.x
@@ -63,14 +63,19 @@ def oracle_from_case(case, threads=1):
     for b in case.bcs:
         d.add_velocity_bc(b["zone"], b["value"], b.get("value_type", 0), b.get("amp_id", 0),
                           b.get("amp_factor", 1.0))
+        if "w" in b:
+            d.set_bc_angular(b["zone"], b["w"])
     for b in case.tbcs:
         d.add_temperature_bc(b["zone"], b["T"])
     if case.meshes:
         d.set_param("solid_count", n)
         first = n
         for m in case.meshes:
-            first += d.add_plane_mesh(first, m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"],
-                                      m.get("hfac", 1.1))
+            if "elems" in m:
+                first += d.add_mesh(first, m["zone"], m["nodes"], m["elems"], m.get("hfac", 1.1), m.get("normals"))
+            else:
+                first += d.add_plane_mesh(first, m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"],
+                                          m.get("hfac", 1.1))
         d.set_param("friction_sta", case.contact[0])
         d.set_param("friction_dyn", case.contact[1])
         d.set_param("DFAC", case.contact[2])

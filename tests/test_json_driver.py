@@ -79,7 +79,8 @@ def test_material_initial_yield_quirks(tmp_path):
     (lambda s: s["Materials"][0].update(type="GMT"), "GMT"),
     (lambda s: s["Materials"][0].update(damageModel="Rankine"), "damage"),
     (lambda s: s["Configuration"].update(smoothlenUpdate=True), "smoothlenUpdate"),
-    (lambda s: s.update(RigidBodies=[{"type": "File", "zoneId": 10}]), "Plane only"),
+    (lambda s: s.update(RigidBodies=[{"type": "Line", "zoneId": 10}]), "Plane and File only"),
+    (lambda s: s.update(RigidBodies=[{"type": "File", "zoneId": 10, "fileName": "missing.nas"}]), "cannot open"),
 ])
 def test_scene_features_off_the_path_stop(tmp_path, edit, message):
     s = json.load(open(SCENE))

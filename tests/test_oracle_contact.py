@@ -133,3 +133,30 @@ def test_mesh_motion_and_normal_refresh():
     assert np.allclose(d.f("X")[ns:, 2], z0 + 5 * vz * dt, rtol=0, atol=1e-15)
     assert np.all(d.f("NORMAL")[ns:] == np.array([0.0, 0.0, 1.0]))
     assert np.allclose(d.f("V")[ns:], [0.0, 0.0, vz], rtol=1e-15)
+
+
+def test_mesh_rotation_explicit_euler_closed_form():
+    """A constant velocity BC with valueAng w = (wx, 0, 0) on the mesh zone:
+    TriMesh::Update moves each node by (w x node) dt, an explicit Euler step
+    of a rotation, so after k steps every element normal is the initial one
+    rotated about x by k atan(wx dt) (the scheme scales the rotated part by
+    sqrt(1 + (wx dt)^2) per step, which normalisation removes), and the
+    particles take the mean node velocity, w x centroid of the nodes' last
+    positions before the update (Source/Mesh.cpp:273-296,
+    Source/Contact.cpp:41-55)."""
+    dt, wx, k = 1e-6, 2000.0, 7
+    d, x, ns, z0 = _block(plane_gap=3 * H)
+    d.add_velocity_bc(10, [0.0, 0.0, 0.0])
+    d.set_bc_angular(10, [wx, 0.0, 0.0])
+    d.solve_init(dt)
+    for _ in range(k):
+        d.step()
+    th = k * math.atan(wx * dt)
+    n = d.f("NORMAL")[ns:]
+    want = np.array([0.0, math.sin(th), -math.cos(th)])   # R_x(th) (0, 0, -1)
+    assert np.abs(n - want).max() < 1e-12
+    # rotating about x leaves the x velocity 0 and gives y, z velocities of
+    # w x c for the pre-update centroid c: v_y = -wx c_z, v_z = wx c_y
+    v = d.f("V")[ns:]
+    assert np.all(v[:, 0] == 0.0)
+    assert np.abs(v[:, 1]).max() > 0 and np.abs(v[:, 2]).max() > 0

@@ -92,7 +92,7 @@ EXPORTS = [
     "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats",
     "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
-    "wf_pack_xdmf", "wf_pack_csv",
+    "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
 ]
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
@@ -150,6 +150,8 @@ def lib():
         "wf_adj_restore": ([vp], ctypes.c_int),
         "wf_pack_xdmf": ([vp, ctypes.POINTER(ctypes.c_float), ip], ctypes.c_int),
         "wf_pack_csv": ([vp, dp], ctypes.c_int),
+        "wf_add_rigid_mesh": ([vp, i32, i64, dp, i64, ip, dp, d], ctypes.c_int),
+        "wf_set_bc_angular_velocity": ([vp, i32, dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -257,6 +259,10 @@ class Solver:
         v = np.ascontiguousarray(value, dtype=np.float64)
         self._check(lib().wf_add_velocity_bc(self.ctx, zone, value_type, _dp(v), amp_id, amp_factor))
 
+    def set_bc_angular_velocity(self, zone, w):
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        self._check(lib().wf_set_bc_angular_velocity(self.ctx, int(zone), _dp(a)))
+
     def add_temperature_bc(self, zone, T):
         self._check(lib().wf_add_temperature_bc(self.ctx, zone, float(T)))
 
@@ -286,6 +292,18 @@ class Solver:
         b = np.ascontiguousarray(p2, dtype=np.float64)
         rc = lib().wf_add_rigid_plane(self.ctx, int(zone), int(axis), int(bool(flip)), _dp(a), _dp(b), int(dens),
                                       float(hfac))
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def add_rigid_mesh(self, zone, nodes, elems, hfac=1.1, normals=None):
+        """A rigid triangle mesh: nodes (nn, 3), elems (ne, 3) 0-based, and
+        optionally the initial element normals (ne, 3)."""
+        nd = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
+        el = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 3)
+        nr = None if normals is None else np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+        rc = lib().wf_add_rigid_mesh(self.ctx, int(zone), len(nd), _dp(nd), len(el), _ip(el),
+                                     None if nr is None else _dp(nr), float(hfac))
         if rc < 0:
             self._check(rc)
         return rc

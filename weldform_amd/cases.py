@@ -38,7 +38,8 @@ class Case:
     thbc: np.ndarray = None     # Thermal_BC per particle (1 = convection)
     hconv: np.ndarray = None
     tinf: np.ndarray = None
-    meshes: list = field(default_factory=list)  # dicts: zone, axis, flip, p1, p2, dens, hfac
+    # dicts: zone, axis, flip, p1, p2, dens, hfac (planes) or zone, nodes, elems, hfac (meshes)
+    meshes: list = field(default_factory=list)
     contact: tuple = (0.0, 0.0, 0.0)            # friction_sta, friction_dyn, DFAC
     auto_ts: int = 0            # bit 0 velocity, bit 1 acceleration criterion
     cfl: float = 0.7
@@ -328,10 +329,15 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     for b in case.bcs:
         s.add_velocity_bc(b["zone"], b["value"], b.get("value_type", 0), b.get("amp_id", 0),
                           b.get("amp_factor", 1.0))
+        if "w" in b:
+            s.set_bc_angular_velocity(b["zone"], b["w"])
     for b in case.tbcs:
         s.add_temperature_bc(b["zone"], b["T"])
     for m in case.meshes:
-        s.add_rigid_plane(m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"], m.get("hfac", 1.1))
+        if "elems" in m:
+            s.add_rigid_mesh(m["zone"], m["nodes"], m["elems"], m.get("hfac", 1.1), m.get("normals"))
+        else:
+            s.add_rigid_plane(m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"], m.get("hfac", 1.1))
     if case.meshes:
         s.set_contact(*case.contact)
     s.init(case.dt)

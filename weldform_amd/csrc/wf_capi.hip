@@ -37,7 +37,7 @@ const int kFieldDim[WF_NFIELDS] = {3, 3, 3, 3, 1, 1, 1, 9, 9, 9, 9, 9, 9, 1, 1,
 
 struct BcHost {
   int zone, type, value_type, amp_id;
-  double value[3], amp_factor, T;
+  double value[3], amp_factor, T, w[3];
 };
 struct AmpHost {
   int id;
@@ -457,6 +457,8 @@ void allocate(Ctx *c) {
     d.rv = c->alloc<double4>(ne);
     d.rmesh = c->alloc<int>(ne);
     d.mv = c->alloc<double4>(kMaxMesh);
+    d.mw = c->alloc<double4>(kMaxMesh);
+    d.nodev = c->alloc<double4>(nn);
     d.id_orig = c->alloc<int>(np);
     d.ccnt = c->alloc<int>(np);
     d.cidx = c->alloc<int>((size_t)kMaxContact * np);
@@ -718,7 +720,10 @@ void refresh_bc(Ctx *c) {
     // UserAcc applies the last amplitude with a matching id (Source/WeldForm.cpp:74-81)
     for (size_t q = 0; q < c->amps.size(); ++q)
       if (c->amps[q].id == b.amp_id) t.amp[k] = (int)q;
-    for (int q = 0; q < 3; ++q) t.v[k][q] = b.value[q];
+    for (int q = 0; q < 3; ++q) {
+      t.v[k][q] = b.value[q];
+      t.w[k][q] = b.w[q];
+    }
   }
   t.nz = (int)c->rzones.size();
   for (int z = 0; z < t.nz; ++z) t.rzone[z] = c->rzones[z];
@@ -1345,6 +1350,24 @@ int wf_add_velocity_bc(void *ctx, int32_t zone, int32_t value_type, const double
   });
 }
 
+int wf_set_bc_angular_velocity(void *ctx, int32_t zone, const double w[3]) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!w || !std::isfinite(w[0]) || !std::isfinite(w[1]) || !std::isfinite(w[2]))
+      throw Error(WF_EINVAL, "bad angular velocity");
+    bool found = false;
+    for (BcHost &b : c->bcs)
+      if (b.type == 0 && b.zone == zone) {
+        for (int k = 0; k < 3; ++k) b.w[k] = w[k];
+        found = true;
+      }
+    if (!found) throw Error(WF_EINVAL, "no velocity BC on that zone");
+    c->bc_dirty = true;
+    return WF_OK;
+  });
+}
+
 int wf_add_temperature_bc(void *ctx, int32_t zone, double T) {
   Ctx *c = (Ctx *)ctx;
   return guard(c, [&] {
@@ -1598,6 +1621,79 @@ int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, cons
     if (!(std::fabs(p2[d0] - p1[d0]) > 0)) throw Error(WF_EINVAL, "plane has no extent");
     c->meshes.push_back(make_plane(zone, axis, flip, p1, p2, dens, hfac));
     return (int)c->meshes.back().h.size();
+  });
+}
+
+// TriMesh from explicit nodes and triangles (TriMesh(NastranReader &, bool),
+// Source/Mesh.cpp:17-64, after the caller's flip / scale / translation):
+// centroids (n0 + n1 + n2) / 3, normals from the winding (CalcNormals), then
+// CalcSpheres (radius, nfar = the last node, plane coefficients) and
+// AddTrimeshParticles' h = (float)hfac * radius
+static MeshHost make_mesh(int zone, int64_t nnode, const double *nodes, int64_t nelem, const int32_t *el,
+                          const double *normals, double hfac) {
+#pragma clang fp contract(off)
+  MeshHost M;
+  M.zone = zone;
+  M.node.assign(nodes, nodes + 3 * nnode);
+  M.el.assign(el, el + 3 * nelem);
+  M.centroid.resize(3 * (size_t)nelem);
+  for (int64_t e = 0; e < nelem; ++e) {
+    const double *a = &M.node[3 * (size_t)el[3 * e]], *b = &M.node[3 * (size_t)el[3 * e + 1]],
+                 *c = &M.node[3 * (size_t)el[3 * e + 2]];
+    for (int k = 0; k < 3; ++k) M.centroid[3 * e + k] = (a[k] + b[k] + c[k]) / 3.;
+  }
+  M.normal.resize(3 * (size_t)nelem);
+  if (normals)
+    M.normal.assign(normals, normals + 3 * nelem);
+  else
+    plane_normals(M);
+  M.radius.resize(nelem);
+  M.nfar.resize(nelem);
+  M.pplane.resize(nelem);
+  M.h.resize(nelem);
+  const double hf = (double)(float)hfac;
+  for (int64_t e = 0; e < nelem; ++e) {
+    double mx = 0.;
+    for (int q = 0; q < 3; ++q) {
+      double rv[3];
+      for (int k = 0; k < 3; ++k) rv[k] = M.node[3 * (size_t)M.el[3 * e + q] + k] - M.centroid[3 * e + k];
+      const double nr = std::sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+      if (nr > mx) mx = nr;
+      M.nfar[e] = q;
+    }
+    M.radius[e] = mx;
+    const double *nf = &M.node[3 * (size_t)M.el[3 * e + M.nfar[e]]];
+    M.pplane[e] = nf[0] * M.normal[3 * e] + nf[1] * M.normal[3 * e + 1] + nf[2] * M.normal[3 * e + 2];
+    M.h[e] = hf * M.radius[e];
+  }
+  return M;
+}
+
+int wf_add_rigid_mesh(void *ctx, int32_t zone, int64_t nnode, const double *nodes, int64_t nelem,
+                      const int32_t *elems, const double *normals, double hfac) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "rigid meshes must be added before wf_init");
+    if (!nodes || !elems) throw Error(WF_EINVAL, "null argument");
+    if (nnode < 3 || nelem < 1 || nnode > (1 << 24) || nelem > (1 << 24))
+      throw Error(WF_EINVAL, "mesh needs 3..2^24 nodes and 1..2^24 triangles");
+    if (!(hfac > 0)) throw Error(WF_EINVAL, "hfac must be positive");
+    if ((int)c->meshes.size() >= kMaxMesh) throw Error(WF_EUNSUPPORTED, "at most 4 rigid meshes");
+    if (zone == 1) throw Error(WF_EINVAL, "zone 1 is the free-surface id");
+    for (int64_t k = 0; k < 3 * nelem; ++k)
+      if (elems[k] < 0 || elems[k] >= nnode) throw Error(WF_EINVAL, "triangle references a node out of range");
+    for (int64_t k = 0; k < 3 * nnode; ++k)
+      if (!std::isfinite(nodes[k])) throw Error(WF_EINVAL, "non-finite node coordinate");
+    if (normals)
+      for (int64_t k = 0; k < 3 * nelem; ++k)
+        if (!std::isfinite(normals[k])) throw Error(WF_EINVAL, "non-finite element normal");
+    MeshHost M = make_mesh(zone, nnode, nodes, nelem, elems, normals, hfac);
+    for (int64_t e = 0; e < nelem; ++e)
+      if (!(M.radius[e] > 0) || !std::isfinite(M.normal[3 * e]))
+        throw Error(WF_EINVAL, "degenerate triangle " + std::to_string(e));
+    c->meshes.push_back(std::move(M));
+    return (int)nelem;
   });
 }
 

@@ -149,6 +149,7 @@ struct BcTable {
   int amp[kMaxBc];             // index into the amplitude table, -1 if none
   double amp_factor[kMaxBc];
   double v[kMaxBc][3];         // constant value / amplitude direction
+  double w[kMaxBc][3];         // angular velocity a constant BC gives its rigid mesh
   double T[kMaxBc];
   int nz;                      // reaction zones
   int rzone[kMaxZones];
@@ -213,6 +214,7 @@ struct Dev {
   int contact, nmesh, nrig, nnode;
   int mesh_zone[kMaxMesh], mesh_node0[kMaxMesh], mesh_nnode[kMaxMesh], mesh_elem0[kMaxMesh], mesh_nelem[kMaxMesh];
   double4 *node, *enorm, *rx, *rv, *mv;
+  double4 *mw, *nodev;         // mesh angular velocity [kMaxMesh]; node velocities of the last update
   int4 *elem;
   int *rmesh;
   double totmass;              // mean particle mass over solid and rigid particles

@@ -1620,8 +1620,8 @@ WF_KERNEL void k_rigid_bc(Dev d, BcTable bc) {
   if (r >= d.nrig) return;
   const int m = d.rmesh[r];
   const int zone = d.mesh_zone[m];
-  bool set = false;
-  double v[3] = {0, 0, 0};
+  bool set = false, wset = false;
+  double v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
   for (int b = 0; b < bc.n; ++b) {
     if (bc.zone[b] != zone || bc.type[b] != 0) continue;
     if (bc.value_type[b] == 1) {
@@ -1630,12 +1630,18 @@ WF_KERNEL void k_rigid_bc(Dev d, BcTable bc) {
       for (int k = 0; k < 3; ++k) v[k] = f * bc.v[b][k];
     } else {
       for (int k = 0; k < 3; ++k) v[k] = bc.v[b][k];
+      // SetRotAxisVel: constant BCs only (Source/WeldForm.cpp:97-101)
+      for (int k = 0; k < 3; ++k) w[k] = bc.w[b][k];
+      wset = true;
     }
     set = true;
   }
   if (!set) return;
   d.rv[r] = make_double4(v[0], v[1], v[2], 0.0);
-  if (r == d.mesh_elem0[m]) d.mv[m] = make_double4(v[0], v[1], v[2], 0.0);
+  if (r == d.mesh_elem0[m]) {
+    d.mv[m] = make_double4(v[0], v[1], v[2], 0.0);
+    if (wset) d.mw[m] = make_double4(w[0], w[1], w[2], 0.0);
+  }
 }
 
 // Wang contact force of each particle over its partners, in order
@@ -1750,20 +1756,24 @@ WF_KERNEL void k_rigid_drift(Dev d, double dt) {
   d.rx[r] = x;
 }
 
-// one block per plane: nodes move with m_v, element normals and plane
-// coefficients are recomputed from the winding, particles take the mean
-// node velocity
+// one block per mesh (TriMesh::Update, Source/Mesh.cpp:273-296): node
+// velocity m_v + m_w x node at the node's position before it moves, nodes
+// move with it, element normals and plane coefficients are recomputed from
+// the winding, particles take the mean node velocity
 __global__ __launch_bounds__(1024) void k_mesh_update(Dev d, double dt) {
   if (dt < 0) dt = d.flags->dt;
   const int m = blockIdx.x;
-  const double4 mv = d.mv[m];
+  const double4 mv = d.mv[m], mw = d.mw[m];
   const int n0 = d.mesh_node0[m], nn = d.mesh_nnode[m];
   for (int k = threadIdx.x; k < nn; k += blockDim.x) {
     double4 p = d.node[n0 + k];
-    p.x += (mv.x + 0.0) * dt;
-    p.y += (mv.y + 0.0) * dt;
-    p.z += (mv.z + 0.0) * dt;
+    const double wp[3] = {mw.y * p.z - mw.z * p.y, mw.z * p.x - mw.x * p.z, mw.x * p.y - mw.y * p.x};
+    const double4 nv = make_double4(mv.x + wp[0], mv.y + wp[1], mv.z + wp[2], 0.0);
+    p.x += nv.x * dt;
+    p.y += nv.y * dt;
+    p.z += nv.z * dt;
     d.node[n0 + k] = p;
+    d.nodev[n0 + k] = nv;
   }
   __syncthreads();
   const int e0 = d.mesh_elem0[m], ne = d.mesh_nelem[m];
@@ -1779,9 +1789,10 @@ __global__ __launch_bounds__(1024) void k_mesh_update(Dev d, double dt) {
     const double4 pf = d.node[fa];
     const double pp = pf.x * n[0] + pf.y * n[1] + pf.z * n[2];
     d.enorm[e0 + e] = make_double4(n[0], n[1], n[2], pp);
-    const double vm = ((0.0 + (mv.x + 0.0)) + (mv.x + 0.0)) + (mv.x + 0.0);
-    const double vn = ((0.0 + (mv.y + 0.0)) + (mv.y + 0.0)) + (mv.y + 0.0);
-    const double vo = ((0.0 + (mv.z + 0.0)) + (mv.z + 0.0)) + (mv.z + 0.0);
+    const double4 va = d.nodev[el.x], vb = d.nodev[el.y], vc = d.nodev[el.z];
+    const double vm = ((0.0 + va.x) + vb.x) + vc.x;
+    const double vn = ((0.0 + va.y) + vb.y) + vc.y;
+    const double vo = ((0.0 + va.z) + vb.z) + vc.z;
     d.rv[e0 + e] = make_double4(vm / 3, vn / 3, vo / 3, 0.0);
   }
 }

@@ -216,27 +216,42 @@ int main(int argc, char **argv) try {
     readVector(body["dim"], dim);
     bool flipnormals = false;
     readValue(body["flipNormals"], flipnormals);
-    if (rigbody_type != "Plane")
-      throw new Fatal("rigid body type " + rigbody_type + " is not implemented on the device path (Plane only)");
-    if (dim(0) != 0. && dim(1) != 0. && dim(2) != 0.)
+    if (rigbody_type != "Plane" && rigbody_type != "File")
+      throw new Fatal("rigid body type " + rigbody_type + " is not implemented on the device path (Plane and File only)");
+    if (rigbody_type == "Plane" && dim(0) != 0. && dim(1) != 0. && dim(2) != 0.)
       throw new Fatal("ERROR: Contact Plane Surface should have one null dimension");
-    int dens = 10;
-    readValue(body["partSide"], dens);
+    dom.contact = true;
+    TriMesh *M;
+    if (rigbody_type == "Plane") {
+      int dens = 10;
+      readValue(body["partSide"], dens);
+      M = new TriMesh;
+      M->AxisPlaneMesh(2, flipnormals, bs, Vec3_t(bs(0) + dim(0), bs(1) + dim(1), bs(2)), dens);
+    } else {
+      std::string filename;
+      readValue(body["fileName"], filename);
+      std::printf("Reading Mesh input file %s\n", filename.c_str());
+      NastranReader reader(filename.c_str());
+      M = new TriMesh(reader, flipnormals);
+    }
+    mesh.push_back(M);
+    // scale and translation as the reference applies them (its TriMesh
+    // helpers recompute centroids and normals from the winding); a plane
+    // changed this way goes to the device as a general mesh
     double scalefactor = 1.0;
     readValue(body["scaleFactor"], scalefactor);
-    if (scalefactor != 1.0) throw new Fatal("scaled rigid meshes are not implemented on the device path");
-    readVector(body["translation"], translation);
-    // the mesh is rebuilt on the device from its corner points, so a
-    // translation moves the corners; the reference then recomputes the
-    // normals from the winding without the flip, so the two may not meet
-    if (norm(translation) > 1.0e-6) {
-      if (flipnormals) throw new Fatal("a translated rigid plane with flipNormals is not supported");
-      bs = bs + translation;
+    if (scalefactor != 1.0) {
+      M->Scale(scalefactor);
+      M->dens = 0;
     }
-    dom.contact = true;
-    TriMesh *M = new TriMesh;
-    M->AxisPlaneMesh(2, flipnormals, bs, Vec3_t(bs(0) + dim(0), bs(1) + dim(1), bs(2)), dens);
-    mesh.push_back(M);
+    readVector(body["translation"], translation);
+    if (norm(translation) > 1.0e-6) {
+      for (size_t n = 0; n < M->node.Size(); ++n) *M->node[n] = *M->node[n] + translation;
+      M->CalcCentroids();
+      M->CalcNormals();
+      M->dens = 0;
+    }
+    M->CalcSpheres();
     int zid = 0;
     readValue(body["zoneId"], zid);
     dom.AddTrimeshParticles(M, 1.1, zid);
