@@ -104,7 +104,10 @@ class Domain {
   int meshcount = 0;
   double friction_sta = 0.0, friction_dyn = 0.0, DFAC = 0.0, PFAC = 0.6;
   bool cont_heat_cond = false, cont_heat_fric = false;
-  double contact_hc = 0.0;  // contact heat conductance (accepted; contact heat is rejected at solve)
+  double contact_hc = 0.0;  // contact heat conductance
+  // min time step of the contact forces (auto_ts_cont) and the contact heat
+  // totals (Source/Domain.h:327, 346-347; Source/Thermal.cpp:194-210)
+  double min_force_ts = 0.0, contact_friction_work = 0.0, accum_cont_heat_cond = 0.0;
   int contact_alg = 0;  // Wang
   std::vector<Vec3_t> m_contact_force;
   double contact_force_sum = 0.0, contact_reaction_sum = 0.0, ext_forces_work = 0.0;
@@ -349,12 +352,9 @@ class Domain {
     if (Dimension != 3) throw new Fatal("the device path is 3D only (set Dimension = 3)");
     if (KernelType != Qubic_Spline) throw new Fatal("the device path implements the cubic spline kernel only");
     if (GradientType != Squared_density) throw new Fatal("the device path implements GradientType 0 only");
-    if (contact && auto_ts_cont)
-      throw new Fatal("the contact time-step criterion (auto_ts_cont) is not implemented on the device path");
     if (contact) {
       if (trimesh.empty()) throw new Fatal("contact is set but no rigid mesh was added (AddTrimeshParticles)");
       if (contact_alg != 0) throw new Fatal("only the Wang contact algorithm is implemented on the device path");
-      if (cont_heat_cond || cont_heat_fric) throw new Fatal("contact heat transfer is not implemented on the device path");
       // a mesh's m_w comes from the constant velocity BC of its zone (UserAcc
       // sets it every call); one set directly on a mesh without such a BC is
       // not carried to the device
@@ -551,6 +551,8 @@ class Domain {
     contact_force_sum = s.contact_force_sum;
     contact_reaction_sum = s.contact_reaction_sum;
     ext_forces_work = s.ext_forces_work;
+    contact_friction_work = s.contact_friction_work;
+    accum_cont_heat_cond = s.accum_cont_heat_cond;
     int_energy_sum = s.int_energy_sum;
     plastic_work = s.plastic_work;
     m_maxT = s.max_T;
@@ -689,7 +691,7 @@ class Domain {
     cfg.device = device;
     cfg.cellfac = Cellfac;
     cfg.pl_work_heat_frac = pl_work_heat_frac;
-    cfg.auto_ts = (auto_ts ? 1 : 0) | (auto_ts_acc ? 2 : 0);
+    cfg.auto_ts = (auto_ts ? 1 : 0) | (auto_ts_acc ? 2 : 0) | (auto_ts_cont ? 4 : 0);
     cfg.cfl = CFL;
     cfg.sqrt_h_a = sqrt_h_a;
     cfg.min_ts = (double)manual_min_ts;
@@ -746,7 +748,16 @@ class Domain {
       }
       if (rc < 0) check(rc);
     }
-    if (contact) check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));
+    if (contact) {
+      check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));
+      if (cont_heat_cond || cont_heat_fric) check(wf_set_contact_heat(ctx_, cont_heat_cond, cont_heat_fric, contact_hc));
+      // the rigid particles' temperatures as the initial conditions and
+      // temperature BCs of their zones left them
+      const size_t ns = nsolid();
+      std::vector<double> rt(Particles.Size() - ns);
+      for (size_t r = 0; r < rt.size(); ++r) rt[r] = Particles[ns + r]->T;
+      check(wf_set_rigid_temperature(ctx_, (int64_t)rt.size(), rt.data()));
+    }
     deltat = dt;
     check(wf_init(ctx_, dt));
     if (key && wfio_has_hdf5()) {

@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 6
+#define WF_ABI_VERSION 7
 
 /* error codes */
 #define WF_OK 0
@@ -157,7 +157,12 @@ typedef struct wf_config {
                                rounding level.                                */
   int32_t auto_ts;          /* adaptive time step (Domain::AdaptiveTimeStep):
                                bit 0 velocity criterion (auto_ts), bit 1
-                               acceleration criterion (auto_ts_acc); 0: fixed
+                               acceleration criterion (auto_ts_acc), bit 2
+                               the contact criterion (auto_ts_cont: the step
+                               drops to Domain::min_force_ts, the least
+                               2 m |v| / |f_n| of the contact pairs, Source/
+                               Domain.cpp:222-229, Source/Contact_Wang.cpp:
+                               226-233; it acts with bit 0 or 1); 0: fixed
                                dt.  Leapfrog and KickDrift.                    */
   double cellfac;           /* Domain::Cellfac (2 for the cubic spline)       */
   double pl_work_heat_frac; /* Domain::pl_work_heat_frac (0.9)                */
@@ -213,6 +218,8 @@ typedef struct wf_scalars {
   double contact_reaction_sum; /* Domain::contact_reaction_sum (last step)    */
   double ext_forces_work;   /* Domain::ext_forces_work                         */
   int64_t contact_pairs;    /* contact pairs of the last search (ContPairs)   */
+  double contact_friction_work; /* Domain::contact_friction_work (friction heat) */
+  double accum_cont_heat_cond;  /* Domain::accum_cont_heat_cond (conducted heat) */
 } wf_scalars;
 
 int wf_abi_version(void);
@@ -268,6 +275,21 @@ int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, cons
 int wf_add_rigid_mesh(void *ctx, int32_t zone, int64_t nnode, const double *nodes, int64_t nelem,
                       const int32_t *elems, const double *normals, double hfac);
 int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double damping);
+/* wf_set_contact_heat <- Domain::cont_heat_cond / contact_hc / cont_heat_fric
+ * (Source/Domain.h:339-347, set from a scene's Contact "heatConductance",
+ * "heatCondCoeff" and "heatFriction", Source/WeldForm.cpp:711-723): every
+ * contact pair sets q_cont_conv = rho h_c dS2 (T_rigid - T) / m with
+ * dS2 = (m / rho)^(2/3) (Source/Contact_Wang.cpp:320-324), and in the
+ * sliding branch q_fric_work = |f_t . v_r| rho / m, friction_hfl and
+ * cshearabs (:291-300); the thermal step adds both to dT/dt
+ * (Source/Thermal.cpp:165-189).  Before wf_init.
+ * wf_set_rigid_temperature <- the T of the rigid particles (set by an
+ * initial condition or temperature BC on the mesh's zone, Source/WeldForm.cpp:
+ * 80-86, 850-866; the thermal step leaves them unchanged): count = the rigid
+ * particles of the meshes added, in mesh order; 0 unless set.  Before
+ * wf_init. */
+int wf_set_contact_heat(void *ctx, int32_t conduction, int32_t friction, double hc);
+int wf_set_rigid_temperature(void *ctx, int64_t count, const double *T);
 /* contact force on a plane (Domain::m_contact_force[mesh]) per step; last
  * min(cap, steps) entries */
 int wf_get_contact_force_history(void *ctx, int32_t mesh, double *out3, int64_t cap, int64_t *count);
@@ -310,8 +332,8 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
  * ShearS holds Sigma (Source/Output.cpp:250-251), Property1..3 hold the
  * default UserOutput's 0, Sigma_eq is CalculateEquivalentStress of
  * ShearStress.  gradcorrmat is gradCorrM with wf_config.grad_corr, else 0;
- * q_friction and c_shearabs are 0: the contact's friction heating that
- * would set them is not run on the device path. */
+ * q_friction (friction_hfl) and c_shearabs are those of the last contact
+ * evaluation, 0 without wf_set_contact_heat's friction heating. */
 #define WF_XDMF_POSITION 0        /* width 3 */
 #define WF_XDMF_VELOCITY 3        /* width 3 */
 #define WF_XDMF_ACCELERATION 6    /* width 3 */

@@ -20,7 +20,8 @@ F64 = ["X", "V", "A", "XPREV", "DISP", "RHO", "DRHO", "MASS", "H", "REFRHO", "P"
        "CS", "P0", "G", "ALPHA", "BETA", "SIGMA", "SHEAR", "SR", "RR", "STRAIN",
        "EPL", "EPLINC", "PLSTRAIN", "DPLSTRAIN", "SIGMAY", "EFFSR", "EP", "ET",
        "T", "DTDT", "KT", "CPT", "THEXP", "QCONV", "QSRC", "QPL", "QCONT",
-       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR", "GCM"]
+       "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR", "GCM",
+       "QFRIC", "CSHEAR", "FHFL"]
 F64_DIM = dict(X=3, V=3, A=3, XPREV=3, DISP=3, SIGMA=9, SHEAR=9, SR=9, RR=9,
                STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9, GCM=9)
 I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB", "ELEMENT", "MESHIDX", "IDORIG"]

@@ -40,7 +40,8 @@ static const int k_fdim[WFO_NFIELDS] = {
     3,                    /* CONTFORCE */
     1, 1, 1,              /* TI TIN TIDIST */
     9,                    /* TIR */
-    9                     /* GCM */
+    9,                    /* GCM */
+    1, 1, 1               /* QFRIC CSHEAR FHFL */
 };
 
 typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T, value_ang[3]; } bc_t;
@@ -92,11 +93,19 @@ struct wfo_domain {
   int contact, nmesh, id_free_surf;
   mesh_t mesh[WFO_MAX_MESH];
   double friction_sta, friction_dyn, DFAC;
+  /* contact heat (Source/Domain.h:339-347): conduction through the contact
+   * with conductance contact_hc, friction work as heat; their totals */
+  int cont_heat_cond, cont_heat_fric;
+  double contact_hc, contact_friction_work, accum_cont_heat_cond;
+  /* the contact time-step criterion (auto_ts bit 2, Domain::auto_ts_cont)
+   * and its running minimum Domain::min_force_ts */
+  double min_force_ts;
   int *rig, *cont;                /* RIGPairs and ContPairs, (temp1, temp2) */
   int64_t nrig, caprig, ncont, capcont;
   double m_contact_force[WFO_MAX_MESH][3], contact_force_sum, contact_reaction_sum, ext_forces_work;
   /* adaptive time step (Source/Domain.cpp:206-319): bit 0 velocity
-   * criterion (auto_ts), bit 1 acceleration criterion (auto_ts_acc) */
+   * criterion (auto_ts), bit 1 acceleration criterion (auto_ts_acc), bit 2
+   * the contact criterion (auto_ts_cont), which acts only with bit 0 or 1 */
   int auto_ts;
   double CFL, sqrt_h_a, manual_min_ts, deltatint, deltatmin;
   /* Domain::nonlock_sum (1: Nishimura pair buffers and reductions, 0:
@@ -190,6 +199,10 @@ int wfo_set_param(wfo_domain *d, const char *s, double v) {
   else if (!strcmp(s, "friction_sta")) d->friction_sta = v;
   else if (!strcmp(s, "friction_dyn")) d->friction_dyn = v;
   else if (!strcmp(s, "DFAC")) d->DFAC = v;
+  else if (!strcmp(s, "cont_heat_cond")) d->cont_heat_cond = (int)v;
+  else if (!strcmp(s, "cont_heat_fric")) d->cont_heat_fric = (int)v;
+  else if (!strcmp(s, "contact_hc")) d->contact_hc = v;
+  else if (!strcmp(s, "min_force_ts")) d->min_force_ts = v;
   else if (!strcmp(s, "auto_ts")) d->auto_ts = (int)v;
   else if (!strcmp(s, "CFL")) d->CFL = v;
   else if (!strcmp(s, "sqrt_h_a")) d->sqrt_h_a = v;
@@ -211,6 +224,9 @@ double wfo_get_param(const wfo_domain *d, const char *s) {
   if (!strcmp(s, "contact_force_sum")) return d->contact_force_sum;
   if (!strcmp(s, "contact_reaction_sum")) return d->contact_reaction_sum;
   if (!strcmp(s, "ext_forces_work")) return d->ext_forces_work;
+  if (!strcmp(s, "min_force_ts")) return d->min_force_ts;
+  if (!strcmp(s, "contact_friction_work")) return d->contact_friction_work;
+  if (!strcmp(s, "accum_cont_heat_cond")) return d->accum_cont_heat_cond;
   if (!strcmp(s, "ncont")) return (double)d->ncont;
   if (!strcmp(s, "int_energy_sum")) return d->int_energy_sum;
   if (!strcmp(s, "plastic_work")) return d->plastic_work;
@@ -1290,8 +1306,20 @@ static void temp_inc(wfo_domain *d) {
     double f = 1. / (dd * d->f[WFO_CPT][i]);
     d->f[WFO_DTDT][i] = f * (d->temp_work[i] + d->f[WFO_QCONV][i] + d->f[WFO_QSRC][i] +
                              d->f[WFO_QPL][i] * d->pl_work_heat_frac + d->f[WFO_QCONT][i]);
+    /* Source/Thermal.cpp:175-189 */
+    if (d->contact && d->cont_heat_fric) d->f[WFO_DTDT][i] += f * d->f[WFO_QFRIC][i];
   }
+  /* the contact heat totals (Source/Thermal.cpp:194-210) */
+  double fr_temp = 0., cond_temp = 0.;
+  if (d->contact)
+    for (int i = 0; i < d->solid_count; i++) {
+      const double dd = d->f[WFO_RHO][i];
+      fr_temp += d->f[WFO_QFRIC][i] * d->f[WFO_MASS][i] / dd;
+      if (d->cont_heat_cond) cond_temp += d->f[WFO_QCONT][i] * d->f[WFO_MASS][i] / dd;
+    }
   for (int i = d->solid_count; i < d->n; i++) d->f[WFO_DTDT][i] = 0.;
+  d->contact_friction_work += fr_temp * d->deltat;
+  d->accum_cont_heat_cond += cond_temp * d->deltat;
 }
 
 /* ThermalCalcs (Source/Thermal.cpp:721-748) with CalcThermalExpStrainRate
@@ -1620,8 +1648,17 @@ static void contact_forces(wfo_domain *d) {
   const double dt = d->deltat;
   double *x = d->f[WFO_X], *v = d->f[WFO_V], *a = d->f[WFO_A], *cf = d->f[WFO_CONTFORCE];
   const double *h = d->f[WFO_H], *ms = d->f[WFO_MASS], *nrmv = d->f[WFO_NORMAL];
-  for (int i = 0; i < d->n; i++)
+  double *qfric = d->f[WFO_QFRIC], *cshear = d->f[WFO_CSHEAR], *fhfl = d->f[WFO_FHFL], *qcont = d->f[WFO_QCONT];
+  const double *T = d->f[WFO_T], *rho = d->f[WFO_RHO];
+  /* the reset of every particle (Source/Contact_Wang.cpp:29-41) */
+  for (int i = 0; i < d->n; i++) {
     for (int k = 0; k < 3; k++) cf[3 * i + k] = 0.;
+    qfric[i] = cshear[i] = fhfl[i] = qcont[i] = 0.;
+  }
+  /* min_force_ts_ (Source/Contact_Wang.cpp:17): the running minimum of
+   * the pairs' dt_fext, which Domain::min_force_ts follows whenever it
+   * drops */
+  double min_force_ts_ = 1000.;
   memset(d->m_contact_force, 0, sizeof d->m_contact_force);
   d->contact_force_sum = 0.;
   d->contact_reaction_sum = 0.;
@@ -1658,7 +1695,17 @@ static void contact_forces(wfo_domain *d) {
     const double omega = sqrt(kij / m1);
     const double psi_cont = 2. * m1 * omega * d->DFAC;
     for (int k = 0; k < 3; k++) cf[3 * P1 + k] = (kij * delta - psi_cont * delta_) * n2[k];
+    /* dt_fext with contact_force_factor 1 (Source/Contact_Wang.cpp:226-233),
+     * on the normal force before the friction */
+    const double dt_fext = 1. * (m1 * 2. * norm3(v + 3 * P1) / norm3(cf + 3 * P1));
+    if (dt_fext < min_force_ts_) {
+      min_force_ts_ = dt_fext;
+      d->min_force_ts = min_force_ts_;
+    }
     for (int k = 0; k < 3; k++) a[3 * P1 + k] += cf[3 * P1 + k] / m1;
+    /* Fraser 3-119 (Source/Contact_Wang.cpp:245-247): the contact area */
+    const double dens = rho[P1];
+    const double dS2 = pow(m1 / dens, 2.0 / 3.0);
     if (d->friction_sta > 0.) {
       double du[3], dtg[3], tgf[3];
       for (int k = 0; k < 3; k++) du[k] = x_pred[k] - x[3 * P1 + k] - v[3 * P2 + k] * dt;
@@ -1678,12 +1725,22 @@ static void contact_forces(wfo_domain *d) {
           cf[3 * P1 + k] -= tgd[k];
           a[3 * P1 + k] -= tgd[k] / m1;
         }
+        /* friction work as heat in the sliding branch
+         * (Source/Contact_Wang.cpp:291-300) */
+        if (d->cont_heat_fric) {
+          const double abs_fv = fabs(dot3(tgd, vr));
+          qfric[P1] = abs_fv * dens / m1;
+          cshear[P1] = norm3(tgd) / dS2;
+          fhfl[P1] = abs_fv / dS2;
+        }
       }
     }
     d->contact_force_sum += norm3(cf + 3 * P1);
     for (int k = 0; k < 3; k++) d->m_contact_force[m][k] += cf[3 * P1 + k];
     d->contact_reaction_sum += dot3(a + 3 * P1, n2) * m1;
     ext_step += dot3(cf + 3 * P1, v + 3 * P2);
+    /* contact conductance, Fraser 3.121 (Source/Contact_Wang.cpp:320-324) */
+    if (d->cont_heat_cond) qcont[P1] = dens * d->contact_hc * dS2 * (T[P2] - T[P1]) / m1;
   }
   d->ext_forces_work += ext_step * dt;
 }
@@ -1763,7 +1820,8 @@ static void check_min_ts_accel(wfo_domain *d) {
   }
 }
 
-/* AdaptiveTimeStep (Source/Domain.cpp:206-239) without contact time steps */
+/* AdaptiveTimeStep (Source/Domain.cpp:206-239); the contact criterion
+ * (auto_ts bit 2) takes min_force_ts when it is the smaller */
 static void adaptive_time_step(wfo_domain *d) {
   if (d->deltatint > d->deltatmin) {
     if (d->deltat < d->deltatmin) d->deltat = 2.0 * d->deltat * d->deltatmin / (d->deltat + d->deltatmin);
@@ -1772,6 +1830,7 @@ static void adaptive_time_step(wfo_domain *d) {
     if (d->deltatint != d->deltat) d->deltat = 2.0 * d->deltat * d->deltatint / (d->deltat + d->deltatint);
     else d->deltat = d->deltatint;
   }
+  if (d->contact && (d->auto_ts & 4) && d->min_force_ts < d->deltat) d->deltat = d->min_force_ts;
   if (d->deltat < d->manual_min_ts) d->deltat = d->manual_min_ts;
 }
 
@@ -1781,7 +1840,7 @@ static double update_time_step(wfo_domain *d) {
   double prev = d->deltat;
   if (d->auto_ts & 1) check_min_ts_vel(d);
   if (d->auto_ts & 2) check_min_ts_accel(d);
-  if (d->auto_ts) adaptive_time_step(d);
+  if (d->auto_ts & 3) adaptive_time_step(d);
   return prev;
 }
 
@@ -1798,6 +1857,7 @@ int wfo_solve_init(wfo_domain *d, double dt) {
     /* Source/SolverLeapfrog.cpp:80-100: ID_orig, one search, Nb counts
      * and the free surface before the loop */
     for (int i = 0; i < d->n; i++) d->fi[WFO_IDORIG][i] = d->fi[WFO_ID][i];
+    d->min_force_ts = dt;   /* Source/SolverLeapfrog.cpp:95 */
     clear_nb_data(d);
     main_neighbour_search(d);
     if (calc_pair_pos_list(d)) return -1;

@@ -52,6 +52,7 @@ enum {
   WFO_TI, WFO_TIN, WFO_TIDIST,                             /* 1: TI, TIn, TIInitDist */
   WFO_TIR,                                                 /* 9 */
   WFO_GCM,                                                 /* 9: gradCorrM */
+  WFO_QFRIC, WFO_CSHEAR, WFO_FHFL,                         /* 1: q_fric_work, cshearabs, friction_hfl */
   WFO_NFIELDS
 };
 /* int fields */

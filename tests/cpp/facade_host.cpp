@@ -67,9 +67,12 @@ int main() {
                 mesh->element[0]->radius, R->normal(2), R->ID, dom.first_fem_particle_idx[0]);
     delete mesh;
   }
+  // the contact time-step criterion and contact heat are accepted; contact
+  // without a rigid surface is not
   expect_fatal("auto_ts_cont", [](SPH::Domain &d) {
     d.contact = true;
     d.auto_ts_cont = true;
+    d.cont_heat_cond = d.cont_heat_fric = true;
   });
   expect_fatal("kernel", [](SPH::Domain &d) { d.Kernel_Set(SPH::Quintic_Spline); });
   expect_fatal("gradient", [](SPH::Domain &d) { d.Gradient_Approach_Set(SPH::Randles_Libersky); });

@@ -20,6 +20,8 @@ def oracle_from_case(case, threads=1):
     d.f("SIGMAY")[s] = case.sigmay
     if case.T is not None:
         d.f("T")[s] = case.T
+    if case.v is not None:
+        d.f("V")[s] = case.v
     if case.thbc is not None:
         d.i("THBC")[s] = case.thbc
         d.f("HCONV")[s] = case.hconv
@@ -79,6 +81,11 @@ def oracle_from_case(case, threads=1):
         d.set_param("friction_sta", case.contact[0])
         d.set_param("friction_dyn", case.contact[1])
         d.set_param("DFAC", case.contact[2])
+        d.set_param("cont_heat_cond", int(bool(case.contact_heat[0])))
+        d.set_param("cont_heat_fric", int(bool(case.contact_heat[1])))
+        d.set_param("contact_hc", float(case.contact_heat[2]))
+        if case.rigid_T is not None:
+            d.f("T")[n:] = case.rigid_T
     return d
 
 

@@ -161,7 +161,7 @@ def test_cpp_facade_generators_and_errors(tmp_path):
     assert float(mz[2]) == float(np.float32(1.1)) * float(mz[3])
     assert float(mz[4]) == -1.0 and int(mz[5]) == 10 and int(mz[6]) == 64
     fatal = {s.split(" ", 1)[0]: s.split(" ", 1)[1] for s in out["fatal"]}
-    expect = dict(auto_ts_cont="auto_ts_cont", kernel="cubic spline", gradient="GradientType 0",
+    expect = dict(auto_ts_cont="no rigid mesh", kernel="cubic spline", gradient="GradientType 0",
                   callback="host callbacks", contact="no rigid mesh", fail="Fail = 1",
                   mixed_params="different parameters", hollomon_mat="Hollomon material",
                   hollomon_def="Hollomon material bad definition")

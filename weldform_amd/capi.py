@@ -67,7 +67,8 @@ class Scalars(ctypes.Structure):
                 ("reserved", ctypes.c_int32)] + [
         (n, ctypes.c_double) for n in
         ["kin_energy_sum", "int_energy_sum", "plastic_work", "max_pl_strain", "max_T", "min_T",
-         "contact_force_sum", "contact_reaction_sum", "ext_forces_work"]] + [("contact_pairs", ctypes.c_int64)]
+         "contact_force_sum", "contact_reaction_sum", "ext_forces_work"]] + [("contact_pairs", ctypes.c_int64)] + [
+        (n, ctypes.c_double) for n in ["contact_friction_work", "accum_cont_heat_cond"]]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -93,11 +94,13 @@ EXPORTS = [
     "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
     "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
+    "wf_set_contact_heat", "wf_set_rigid_temperature",
 ]
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
 # CSV columns
 XDMF_NF, XDMF_NI, CSV_NCOL = 64, 3, 17
+XDMF_Q_FRICTION, XDMF_C_SHEARABS = 55, 56
 
 _lib = None
 
@@ -142,6 +145,8 @@ def lib():
         "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
         "wf_add_rigid_plane": ([vp, i32, i32, i32, dp, dp, i32, d], ctypes.c_int),
         "wf_set_contact": ([vp, d, d, d], ctypes.c_int),
+        "wf_set_contact_heat": ([vp, ctypes.c_int32, ctypes.c_int32, d], ctypes.c_int),
+        "wf_set_rigid_temperature": ([vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "wf_get_contact_force_history": ([vp, i32, dp, i64, I64P], ctypes.c_int),
         "wf_get_rigid_field": ([vp, i32, dp], ctypes.c_int),
         "wf_adj_forward": ([vp, ctypes.POINTER(AdjConfig), dp, dp, dp], ctypes.c_int),
@@ -310,6 +315,16 @@ class Solver:
 
     def set_contact(self, friction_sta, friction_dyn, damping=0.0):
         self._check(lib().wf_set_contact(self.ctx, friction_sta, friction_dyn, damping))
+
+    def set_contact_heat(self, conduction=False, friction=False, hc=0.0):
+        """Contact heat conduction with conductance hc and friction
+        heating (Domain::cont_heat_cond / contact_hc / cont_heat_fric)."""
+        self._check(lib().wf_set_contact_heat(self.ctx, int(bool(conduction)), int(bool(friction)), float(hc)))
+
+    def set_rigid_temperature(self, T):
+        """Temperatures of the rigid particles, one each, in mesh order."""
+        t = np.ascontiguousarray(T, dtype=np.float64).ravel()
+        self._check(lib().wf_set_rigid_temperature(self.ctx, len(t), _dp(t)))
 
     def contact_force_history(self, mesh, cap=1 << 16):
         out = np.zeros(3 * cap)

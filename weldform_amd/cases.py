@@ -27,6 +27,7 @@ class Case:
     dt: float
     steps: int
     T: np.ndarray = None
+    v: np.ndarray = None        # initial particle velocities (default at rest)
     bcs: list = field(default_factory=list)     # dicts: zone, value, value_type, amp_id, amp_factor
     tbcs: list = field(default_factory=list)    # dicts: zone, T
     amps: list = field(default_factory=list)    # dicts: id, t, v
@@ -41,7 +42,9 @@ class Case:
     # dicts: zone, axis, flip, p1, p2, dens, hfac (planes) or zone, nodes, elems, hfac (meshes)
     meshes: list = field(default_factory=list)
     contact: tuple = (0.0, 0.0, 0.0)            # friction_sta, friction_dyn, DFAC
-    auto_ts: int = 0            # bit 0 velocity, bit 1 acceleration criterion
+    contact_heat: tuple = (False, False, 0.0)   # conduction, friction heating, conductance h_c
+    rigid_T: np.ndarray = None  # temperature of each rigid particle (mesh order)
+    auto_ts: int = 0            # bit 0 velocity, bit 1 acceleration, bit 2 contact criterion
     cfl: float = 0.7
     sqrt_h_a: float = 0.0025
     meta: dict = field(default_factory=dict)
@@ -318,6 +321,8 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
     s.set("sigmay", case.sigmay)
     if case.T is not None:
         s.set("T", case.T)
+    if case.v is not None:
+        s.set("v", case.v)
     if case.thbc is not None:
         s.set_int("thermal_bc", case.thbc)
         s.set("h_conv", case.hconv)
@@ -340,5 +345,9 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
             s.add_rigid_plane(m["zone"], m["axis"], m["flip"], m["p1"], m["p2"], m["dens"], m.get("hfac", 1.1))
     if case.meshes:
         s.set_contact(*case.contact)
+        if any(case.contact_heat[:2]):
+            s.set_contact_heat(*case.contact_heat)
+        if case.rigid_T is not None:
+            s.set_rigid_temperature(case.rigid_T)
     s.init(case.dt)
     return s

@@ -69,6 +69,9 @@ struct Ctx {
   wf_config cfg{};
   std::vector<MeshHost> meshes;
   double fr_sta = 0.0, fr_dyn = 0.0, dfac = 0.0;
+  int heat_cond = 0, heat_fric = 0;
+  double heat_hc = 0.0;
+  std::vector<double> rigid_T;     // one per rigid particle (mesh order), 0 unless set
   int64_t n = 0;
   long long npad = 0;
   hipStream_t st = nullptr;
@@ -467,12 +470,24 @@ void allocate(Ctx *c) {
     d.fr_sta = c->fr_sta;
     d.fr_dyn = c->fr_dyn;
     d.dfac = c->dfac;
+    d.rT = c->alloc<double>(ne);
+    d.heat_cond = c->heat_cond;
+    d.heat_fric = c->heat_fric;
+    d.hc = c->heat_hc;
+    if (d.heat_cond || d.heat_fric) {
+      d.qfric = c->alloc<double>(np);
+      d.fhfl = c->alloc<double>(np);
+      d.cshear = c->alloc<double>(np);
+      d.qcont = c->alloc<double>(np);
+    }
   }
   d.ntiles = (int)(np / kBlock);
   d.stage_max = 0;
   d.lanes_mx = c->cfg.lanes ? c->cfg.lanes : kLanesMixed;
   d.lanes_f64 = c->cfg.lanes ? c->cfg.lanes : kLanesFP64;
-  d.auto_ts = c->cfg.auto_ts;
+  // the contact criterion (bit 2) acts only inside AdaptiveTimeStep, which
+  // runs with the velocity or acceleration criterion
+  d.auto_ts = (c->cfg.auto_ts & 3) ? c->cfg.auto_ts : 0;
   d.ti_on = 0;
   for (size_t k = 0; k < c->mats.size(); ++k)
     if (c->mat_set[k] && c->mats[k].ti > 0.0) d.ti_on = 1;
@@ -572,6 +587,10 @@ void upload_all(Ctx *c) {
     dput(c, d.rx, rx);
     dput(c, d.rv, z);
     dput(c, d.rmesh, rm);
+    if (!c->rigid_T.empty()) {
+      if ((int)c->rigid_T.size() != d.nrig) throw Error(WF_EINVAL, "rigid temperatures: one per rigid particle");
+      dput(c, d.rT, c->rigid_T);
+    }
     // ID_orig = ID before the first free-surface search
     // (Source/SolverLeapfrog.cpp:80-83); the mean mass of CalculateSurface
     // runs over every particle in the reference's order, rigid ones with 0
@@ -640,6 +659,7 @@ void upload_all(Ctx *c) {
   fl.cleared = 1;
   fl.ts_nb_inc = c->cfg.ts_nb_inc;
   fl.dt = fl.dt_prev = fl.dtint = c->dt;
+  fl.min_force_ts = c->dt;   // Source/SolverLeapfrog.cpp:95
   const double *pl = field(WF_F_PL_STRAIN);
   if (pl)
     for (int64_t i = 0; i < n; ++i)
@@ -987,7 +1007,7 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
       throw Error(WF_EINVAL, "lanes must be 0, 1, 2 or 4");
     if (cfg->stage_max < -1 || cfg->stage_max > kStageMax)
       throw Error(WF_EINVAL, "stage_max must be -1, 0 or 1.." + std::to_string(kStageMax));
-    if (cfg->auto_ts < 0 || cfg->auto_ts > 3) throw Error(WF_EINVAL, "auto_ts must be 0..3");
+    if (cfg->auto_ts < 0 || cfg->auto_ts > 7) throw Error(WF_EINVAL, "auto_ts must be 0..7");
     if (cfg->auto_ts && !(cfg->cfl > 0 && cfg->sqrt_h_a > 0 && cfg->min_ts >= 0))
       throw Error(WF_EINVAL, "auto_ts needs cfl > 0, sqrt_h_a > 0 and min_ts >= 0");
     Ctx *c = new Ctx();
@@ -1484,6 +1504,8 @@ int wf_get_scalars(void *ctx, wf_scalars *out) {
       out->contact_force_sum = s[S_CFSUM];
       out->contact_reaction_sum = s[S_CREACT];
       out->ext_forces_work = s[S_EXTW];
+      out->contact_friction_work = s[S_FRW];
+      out->accum_cont_heat_cond = s[S_CHC];
       auto cc = dget(c, c->d.ccnt, (size_t)c->npad);
       int64_t tot = 0;
       for (int64_t k = 0; k < c->n; ++k) tot += cc[k];
@@ -1712,6 +1734,33 @@ int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double d
       c->d.dfac = damping;
       c->drop_graphs();
     }
+    return WF_OK;
+  });
+}
+
+int wf_set_contact_heat(void *ctx, int32_t conduction, int32_t friction, double hc) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_contact_heat after wf_init");
+    if (conduction && !(hc >= 0)) throw Error(WF_EINVAL, "negative contact conductance");
+    c->heat_cond = conduction ? 1 : 0;
+    c->heat_fric = friction ? 1 : 0;
+    c->heat_hc = conduction ? hc : 0.0;
+    return WF_OK;
+  });
+}
+
+int wf_set_rigid_temperature(void *ctx, int64_t count, const double *T) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_rigid_temperature after wf_init");
+    if (count < 0 || (count > 0 && !T)) throw Error(WF_EINVAL, "null argument");
+    int64_t nrig = 0;
+    for (const MeshHost &M : c->meshes) nrig += (int64_t)M.h.size();
+    if (count != nrig) throw Error(WF_EINVAL, "rigid temperatures: one per rigid particle of the meshes added");
+    c->rigid_T.assign(T, T + count);
     return WF_OK;
   });
 }

@@ -110,6 +110,7 @@ struct DevFlags {
   double dt;         // Domain::deltat (changes only with the adaptive step)
   double dt_prev;    // prev_deltat of the step
   double dtint;      // Domain::deltatint (the initial dt)
+  double min_force_ts;  // Domain::min_force_ts (the contact step criterion)
 };
 
 // partial-sum slots per block
@@ -124,10 +125,13 @@ enum {
   P_CREACT,                             // sum m a.n of contact particles
   P_EXTW,                               // sum contforce . v_rigid
   P_DTMIN,                              // min time-step criterion
+  P_FTS,                                // min dt_fext of the contact pairs
+  P_FRW,                                // sum q_fric_work m / rho
+  P_CHC,                                // sum q_cont_conv m / rho
   P_COUNT
 };
 
-enum { S_KIN = 0, S_INT, S_PLW, S_MAXT, S_MINT, S_MAXPL, S_CFSUM, S_CREACT, S_EXTW, S_COUNT };
+enum { S_KIN = 0, S_INT, S_PLW, S_MAXT, S_MINT, S_MAXPL, S_CFSUM, S_CREACT, S_EXTW, S_FRW, S_CHC, S_COUNT };
 
 constexpr int kMaxAmp = 8;
 constexpr int kMaxAmpPts = 16;
@@ -219,6 +223,14 @@ struct Dev {
   int *rmesh;
   double totmass;              // mean particle mass over solid and rigid particles
   double fr_sta, fr_dyn, dfac;
+  // contact heat (Source/Contact_Wang.cpp:245-324): conduction with
+  // conductance hc from the rigid particles' temperatures rT, friction work
+  // as heat; per particle q_fric_work, friction_hfl, cshearabs and
+  // q_cont_conv (allocated when either is on)
+  int heat_cond, heat_fric;
+  double hc;
+  double *rT;
+  double *qfric, *fhfl, *cshear, *qcont;
   int *id_orig, *ccnt, *cidx;  // contact partners [kMaxContact][npad]
   double4 *cforce;
   double *cf_hist;             // [hist_cap][kMaxMesh][3]

@@ -985,8 +985,16 @@ __global__ __launch_bounds__(1024) void k_ts_adapt(Dev d) {
   double v = INFINITY;
   for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) v = fmin(v, part(d, P_DTMIN, b));
   const double r = block_reduce<2>(v, sh);
+  // the contact criterion: the least dt_fext of this step's pairs
+  const bool cont_ts = d.contact && (d.auto_ts & 4);
+  double fts = INFINITY;
+  if (cont_ts) {
+    for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) fts = fmin(fts, part(d, P_FTS, b));
+    fts = block_reduce<2>(fts, sh);
+  }
   if (threadIdx.x != 0) return;
   DevFlags *f = d.flags;
+  if (fts < INFINITY) f->min_force_ts = fts;
   const double dint = f->dtint, dmin = fmin(dint, r);
   double dt = f->dt;
   f->dt_prev = dt;
@@ -997,6 +1005,7 @@ __global__ __launch_bounds__(1024) void k_ts_adapt(Dev d) {
     if (dint != dt) dt = 2.0 * dt * dint / (dt + dint);
     else dt = dint;
   }
+  if (cont_ts && f->min_force_ts < dt) dt = f->min_force_ts;
   if (dt < d.min_ts) dt = d.min_ts;
   f->dt = dt;
 }
@@ -1406,7 +1415,7 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
   __shared__ double sh[kBlock / 64];
   const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const long long np = d.npad;
-  double e_int = 0.0;
+  double e_int = 0.0, e_frw = 0.0, e_chc = 0.0;
   if (i < d.n && d.group[i] >= 0) {
     const DevMat &M = d.mats[d.mat[i]];
     const double4 xi = d.xh[buf][i];
@@ -1435,8 +1444,15 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
       temp += vsec * kk * (Ti - d.T[j]) * GK;
     }
     const double f = 1. / (rho * M.cp_T);
-    const double dTdt = f * (temp + d.qconv[i] + d.qsrc[i] + d.qpl[i] * d.pl_frac + 0.0);
+    const double qc = d.heat_cond ? d.qcont[i] : 0.0;
+    double dTdt = f * (temp + d.qconv[i] + d.qsrc[i] + d.qpl[i] * d.pl_frac + qc);
+    if (d.heat_fric) dTdt += f * d.qfric[i];
     d.dTdt[i] = dTdt;
+    if (d.qfric) {
+      // the contact heat totals (Source/Thermal.cpp:194-210)
+      e_frw = d.qfric[i] * m / rho;
+      e_chc = qc * m / rho;
+    }
     double SR[6];
     for (int k = 0; k < 6; ++k) SR[k] = d.sr[k * np + i];
     const double ex = M.th_exp * dTdt;
@@ -1451,6 +1467,14 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
   }
   const double r0 = block_reduce<0>(e_int, sh);
   if (threadIdx.x == 0) part(d, P_INT, blockIdx.x) = r0;
+  if (d.qfric) {
+    const double r1 = block_reduce<0>(e_frw, sh);
+    const double r2 = block_reduce<0>(e_chc, sh);
+    if (threadIdx.x == 0) {
+      part(d, P_FRW, blockIdx.x) = r1;
+      part(d, P_CHC, blockIdx.x) = r2;
+    }
+  }
 }
 
 void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
@@ -1505,6 +1529,8 @@ __global__ __launch_bounds__(64) void k_fin_apply(Dev d, double dt, FinList L) {
     if (s == P_CFSUM) cfsum = r;
     if (s == P_CREACT) creact = r;
     if (s == P_EXTW) extw = r;
+    if (s == P_FRW) d.scalars[S_FRW] += r * dt;
+    if (s == P_CHC) d.scalars[S_CHC] += r * dt;
   }
   if (d.contact) {
     d.scalars[S_CFSUM] = cfsum;
@@ -1548,6 +1574,10 @@ void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
     add(P_CFSUM, 0);
     add(P_CREACT, 0);
     add(P_EXTW, 0);
+    if (d.thermal && d.qfric) {
+      add(P_FRW, 0);
+      add(P_CHC, 0);
+    }
   }
   k_fin_sum<<<L.n, 256, 0, st>>>(d, L);
   k_fin_apply<<<1, 64, 0, st>>>(d, dt, L);
@@ -1651,9 +1681,10 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double cfm[kMaxMesh][3];
   for (int m = 0; m < kMaxMesh; ++m) cfm[m][0] = cfm[m][1] = cfm[m][2] = 0.0;
-  double cfsum = 0.0, creact = 0.0, extw = 0.0;
+  double cfsum = 0.0, creact = 0.0, extw = 0.0, fts = INFINITY;
   if (i < d.n) {
     double cf[3] = {0, 0, 0};
+    double qf = 0.0, fh = 0.0, cs = 0.0, qc = 0.0;
     const int cnt = d.ccnt[i];
     if (cnt > 0) {
       const double4 x4 = d.xh[cur][i];
@@ -1694,7 +1725,18 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
         const double omega = sqrt(kij / m1);
         const double psi = 2. * m1 * omega * d.dfac;
         for (int k = 0; k < 3; ++k) cf[k] = (kij * delta - psi * delta_) * n2[k];
+        {
+          // dt_fext on the normal force (Source/Contact_Wang.cpp:226-233);
+          // Domain::min_force_ts takes the least below 1000
+          const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+          const double nc = sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
+          const double dt_fext = 1. * (m1 * 2. * nv / nc);
+          if (dt_fext < 1000. && dt_fext < fts) fts = dt_fext;
+        }
         for (int k = 0; k < 3; ++k) a[k] += cf[k] / m1;
+        // Fraser 3-119: the contact area of the particle
+        const double dens = d.rho[cur][i];
+        const double dS2 = d.qfric ? pow(m1 / dens, 2.0 / 3.0) : 0.0;
         if (d.fr_sta > 0.) {
           double du[3], tg[3];
           for (int k = 0; k < 3; ++k) du[k] = xp[k] - x[k] - v2[k] * dt;
@@ -1708,10 +1750,18 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
               a[k] -= tg[k] / m1;
             }
           } else {
+            double td[3];
             for (int k = 0; k < 3; ++k) {
-              const double td = d.fr_dyn * ncf * tg[k] / ntg;
-              cf[k] -= td;
-              a[k] -= td / m1;
+              td[k] = d.fr_dyn * ncf * tg[k] / ntg;
+              cf[k] -= td[k];
+              a[k] -= td[k] / m1;
+            }
+            if (d.heat_fric) {
+              // friction work as heat (Source/Contact_Wang.cpp:291-300)
+              const double afv = fabs(td[0] * (v[0] - v2[0]) + td[1] * (v[1] - v2[1]) + td[2] * (v[2] - v2[2]));
+              qf = afv * dens / m1;
+              cs = sqrt(td[0] * td[0] + td[1] * td[1] + td[2] * td[2]) / dS2;
+              fh = afv / dS2;
             }
           }
         }
@@ -1721,10 +1771,18 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
             for (int k = 0; k < 3; ++k) cfm[m][k] += cf[k];
         creact += (a[0] * n2[0] + a[1] * n2[1] + a[2] * n2[2]) * m1;
         extw += cf[0] * v2[0] + cf[1] * v2[1] + cf[2] * v2[2];
+        // contact conductance, Fraser 3.121 (Source/Contact_Wang.cpp:320-324)
+        if (d.heat_cond) qc = dens * d.hc * dS2 * (d.rT[r] - d.T[i]) / m1;
       }
       d.acc[i] = make_double4(a[0], a[1], a[2], a4.w);
     }
     d.cforce[i] = make_double4(cf[0], cf[1], cf[2], 0.0);
+    if (d.qfric) {
+      d.qfric[i] = qf;
+      d.fhfl[i] = fh;
+      d.cshear[i] = cs;
+      d.qcont[i] = qc;
+    }
   }
   for (int m = 0; m < d.nmesh; ++m)
     for (int k = 0; k < 3; ++k) {
@@ -1734,7 +1792,9 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
   const double r1 = block_reduce<0>(cfsum, sh);
   const double r2 = block_reduce<0>(creact, sh);
   const double r3 = block_reduce<0>(extw, sh);
+  const double r4 = block_reduce<2>(fts, sh);
   if (threadIdx.x == 0) {
+    part(d, P_FTS, blockIdx.x) = r4;
     part(d, P_CFSUM, blockIdx.x) = r1;
     part(d, P_CREACT, blockIdx.x) = r2;
     // the work takes the step size of the contact evaluation, before any

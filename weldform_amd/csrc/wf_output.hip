@@ -79,11 +79,11 @@ WF_KERNEL void k_pack_xdmf(Dev d, int cur, float *f, int *iv) {
   *col(WF_XDMF_SIGMA_EQ, 1) = (float)sigma_eq(d.S, np, k);
   *col(WF_XDMF_TEMPERATURE, 1) = (float)d.T[k];
   *col(WF_XDMF_PL_STRAIN, 1) = (float)d.pl[k];
-  // friction heat flux and contact shear stress are only set by the
-  // contact's friction-heating branch (Source/Contact_Wang.cpp:289-298),
-  // which the device path does not run; contact resets them to 0 (:34-39)
-  *col(WF_XDMF_Q_FRICTION, 1) = 0.f;
-  *col(WF_XDMF_C_SHEARABS, 1) = 0.f;
+  // friction heat flux (friction_hfl) and contact shear stress are set by
+  // the contact's friction-heating branch (Source/Contact_Wang.cpp:289-298)
+  // and reset to 0 by every contact evaluation (:34-39)
+  *col(WF_XDMF_Q_FRICTION, 1) = d.fhfl ? (float)d.fhfl[k] : 0.f;
+  *col(WF_XDMF_C_SHEARABS, 1) = d.cshear ? (float)d.cshear[k] : 0.f;
   *col(WF_XDMF_PS_EN, 1) = (float)d.psen[k];
   p = col(WF_XDMF_DISPLACEMENT, 3);
   p[0] = (float)u.x, p[1] = (float)u.y, p[2] = (float)u.z;

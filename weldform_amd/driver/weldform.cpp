@@ -417,6 +417,9 @@ int main(int argc, char **argv) try {
   }
   std::printf("done: time %.9g, step %.6g, kinetic %.9g, internal %.9g\n", dom.getTime(), dom.getStepSize(),
               dom.kin_energy_sum, dom.int_energy_sum);
+  // the totals the reference prints with its outputs (Source/SolverLeapfrog.cpp:404-407)
+  if (dom.cont_heat_cond) std::printf("Total contact heat flux%.9g\n", dom.accum_cont_heat_cond);
+  if (dom.cont_heat_fric) std::printf("Contact friction work %.9g\n", dom.contact_friction_work);
   for (TriMesh *M : mesh) delete M;
   delete mat;
   return 0;
