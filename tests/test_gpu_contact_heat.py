@@ -4,12 +4,14 @@ CPU oracle (Source/Contact_Wang.cpp:226-233, 245-324; Source/Thermal.cpp:
 
   * Contact heat: the thermal contact case (a punch at 400 K on a 300 K
     billet, h_c 5e4 W/m^2K, friction heating, static 0.3 / dynamic 0.2)
-    over 120 steps: the billet's heating T - 300, displacement and contact
-    forces, the XDMF q_friction / c_shearabs columns, and the two heat
-    totals.  Bars as the other contact runs: the penalty contact is chaotic,
-    so max(1e-8, 10x the oracle's own sensitivity to a 1e-15 relative change
-    of the initial positions) in FP64; max(1e-4, 10x its sensitivity to
-    1e-7) for the mixed path.
+    over the first 55 steps (contact starts at step ~45): the billet's
+    heating T - 300, displacement and contact forces, the XDMF q_friction /
+    c_shearabs columns, and the two heat totals.  Bars as the other contact
+    runs: the penalty contact is chaotic, so max(1e-8, 10x the oracle's own
+    sensitivity to a 1e-15 relative change of the initial positions) in
+    FP64 (that sensitivity is 1.5e-9 - 4e-9 at step 55 and grows past 1e-5
+    by step 70; measured: heating and the columns bit-equal, the rest
+    <= 1e-12); the mixed path within 1e-4 (measured <= 1.1e-5).
   * auto_ts_cont: the velocity criterion with the contact criterion, the
     billet moving up into the punch, up to the step where the contact
     criterion first sets the step (45 steps): step size, time and
@@ -53,11 +55,11 @@ def _col(f, n, off, w=1):
 
 @pytest.mark.parametrize("precision", [capi.FP64, capi.MIXED])
 def test_contact_heat_matches_oracle(precision):
-    c = _heat_case(120)
+    c = _heat_case(55)
     fp64 = precision == capi.FP64
     print(f"contact heat: n={c.n} steps={c.steps} precision={precision}", flush=True)
     d = _oracle(c)
-    dp = _oracle(c, 1e-15 if fp64 else 1e-7)
+    dp = _oracle(c, 1e-15)
     n = c.n
 
     def fields(get):
@@ -89,7 +91,7 @@ def test_contact_heat_matches_oracle(precision):
     for k, e in errs.items():
         # float32 columns carry their own rounding
         fl = max(floor, 1e-6) if k in ("q_friction", "c_shearabs") else floor
-        assert e < max(fl, 10 * sens[k]), k
+        assert e < (max(fl, 10 * sens[k]) if fp64 else fl), k
     g.close()
 
 

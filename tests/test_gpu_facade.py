@@ -155,3 +155,32 @@ def test_json_scene_driver_matches_oracle(tmp_path, grad_corr):
         ds = h5read.read(str(tmp_path / "maz.hdf5"))
         assert ds["NP"][1][0] == case.n
         assert np.array_equal(ds["Displacement"][1].reshape(-1, 3), final[:, :3].astype(np.float32))
+
+
+def test_json_contact_heat_scene_runs(tmp_path):
+    """The driver on config 1's scene with a punch plane at 450 K (an
+    initial condition on its zone), contact conductance and friction
+    heating: the facade hands the rigid temperatures and heat settings to
+    the device, so heat flows into the 300 K billet."""
+    s = json.load(open(os.path.join(os.path.dirname(__file__), "json", "upsetting_bilinear.json")))
+    s["Configuration"]["solver"] = "Mech-Thermal-LeapFrog"
+    s["Materials"][0].update(thermalCond=237.0, thermalHeatCap=897.0)
+    s["RigidBodies"] = [{"type": "Plane", "zoneId": 10, "start": [-0.01, -0.01, 0.0296], "dim": [0.02, 0.02, 0.0],
+                         "partSide": 20}]
+    s["Contact"] = [{"fricCoeffStatic": 0.2, "fricCoeffDynamic": 0.2, "penaltyFactor": 0.6,
+                     "heatConductance": True, "heatCondCoeff": 5.0e4, "heatFriction": True}]
+    s["BoundaryConditions"].append({"zoneId": 10, "valueType": 0, "value": [0.0, 0.0, -1.0]})
+    s["InitialConditions"] = [{"Temp": 300.0}, {"Temp": 450.0, "zoneId": 10}]
+    scene = str(tmp_path / "contact_heat.json")
+    with open(scene, "w") as f:
+        json.dump(s, f)
+    if not os.path.exists(wfbuild.JSON_DRIVER):
+        wfbuild.build_json_driver()
+    print("json driver: contact heat scene", flush=True)
+    r = subprocess.run([wfbuild.JSON_DRIVER, scene], capture_output=True, text=True, timeout=100, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "rigid T 450 .. 450" in r.stdout
+    flux = float(r.stdout.split("Total contact heat flux")[1].split()[0])
+    fric = float(r.stdout.split("Contact friction work")[1].split()[0])
+    print(f"json driver: contact heat {flux:.6g} J, friction heat {fric:.6g} J", flush=True)
+    assert flux > 0 and np.isfinite(flux) and fric >= 0 and np.isfinite(fric)

@@ -98,3 +98,14 @@ def test_contact_scene_adds_rigid_particles(tmp_path):
     rc, out, err = _dry(tmp_path, s)
     assert rc == 0, err
     assert f"{4524 + 2 * 20 * 20} particles (4524 solid)" in out and "contact 1" in out and "meshes 1" in out
+    assert "contact heat conduction 0 (hc 0), friction 0" in out
+    # contact heat (Source/WeldForm.cpp:711-723) and the punch's temperature
+    # from an initial condition on its zone (:850-866)
+    s["Configuration"]["solver"] = "Mech-Thermal-LeapFrog"
+    s["Contact"][0].update(heatConductance=True, heatCondCoeff=5.0e4, heatFriction=True)
+    s["InitialConditions"] = [{"Temp": 300.0}, {"Temp": 450.0, "zoneId": 10}]
+    s["Configuration"]["autoTS"] = [True, False, True]
+    rc, out, err = _dry(tmp_path, s)
+    assert rc == 0, err
+    assert "contact heat conduction 1 (hc 50000), friction 1, rigid T 450 .. 450" in out
+    assert "autoTS [1 0 1]" in out

@@ -395,6 +395,13 @@ int main(int argc, char **argv) try {
               (int)dom.thermal_solver);
   std::printf("Sigmay0 %.17g, Ep %.17g, BCs %zu, amplitudes %zu, meshes %zu\n", dom.Particles[0]->Sigmay,
               dom.Particles[0]->Ep, dom.bConds.size(), dom.amps.size(), mesh.size());
+  if (dom.contact) {
+    // contact heat and the rigid particles' temperatures (first and last)
+    const size_t ns = (size_t)dom.solid_part_count, nall = dom.Particles.Size();
+    std::printf("contact heat conduction %d (hc %.17g), friction %d, rigid T %.17g .. %.17g\n", (int)dom.cont_heat_cond,
+                dom.contact_hc, (int)dom.cont_heat_fric, nall > ns ? dom.Particles[ns]->T : 0.0,
+                nall > ns ? dom.Particles[nall - 1]->T : 0.0);
+  }
   if (dry) {
     for (TriMesh *M : mesh) delete M;
     delete mat;
