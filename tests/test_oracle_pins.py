@@ -306,6 +306,125 @@ def test_free_particles_drift_and_bc():
     assert d.f("V")[2] == pytest.approx([0.0, 0.0, -0.6], rel=1e-12)
 
 
+def _pressure_block(Q):
+    """A 14 mm aluminium cylinder (dx = 1 mm, no BC zones) at rest whose
+    stress is a fixed pressure field p(x) = p0 + g.x + x.Q x / 2: per-particle
+    P0 = p(x), Cs = 0 and G = 0, so the stress update returns sigma = -p I
+    every step whatever the density and strain rate.  On the lattice interior
+    the acceleration is then a(x) = -mu grad p(x) / rho (the symmetric pair
+    sum of a quadratic field leaves the gradient term only), with the lattice
+    moment mu of the full neighbourhood.  A step of 1e-8 s keeps the lattice
+    distortion of two steps (|grad v| dt ~ 1e-8) below the tolerances.
+    Returns the case, the oracle, a(x) of the closed form, mu and the masks
+    deep[k] of particles whose neighbourhoods are interior to depth k: a
+    quantity of step s that reads neighbours' values of step s - 1 follows
+    the closed form one depth further in."""
+    c = cases.cylinder_case("pin_pressure", 14e-3, 24e-3, 1.0e-3, cases.AL["rho"], cases.AL["E"], cases.AL["nu"],
+                            dict(model=po.BILINEAR, alpha=0.0, beta=0.0, Ep=0.0, sy0=1e12), steps=2)
+    c.bcs = []
+    c.dt = 1e-8
+    d = oracle_from_case(c)
+    g = np.array([3.0, -1.0, 2.0]) * 1e10
+    ctr = c.x.mean(axis=0)
+    xr = c.x - ctr
+    p = 1e8 + xr @ g + 0.5 * np.einsum("ni,ij,nj->n", xr, Q, xr)
+    n = c.n
+    d.f("P0")[:n] = p
+    d.f("CS")[:n] = 0.0
+    d.f("G")[:n] = 0.0
+    d.f("SIGMA")[:n] = -p[:, None, None] * np.eye(3)[None]
+    mu = _lattice_moment(c)
+    a_cf = -mu * (g + xr @ Q) / c.density[0]
+    d.neighbour_search()
+    nb = d.i("NB")[:n]
+    interior = nb == nb.max()
+    P = d.pairs()
+    def deeper(m):
+        bad = np.zeros(n, dtype=bool)
+        np.logical_or.at(bad, P[:, 0], ~m[P[:, 1]])
+        np.logical_or.at(bad, P[:, 1], ~m[P[:, 0]])
+        return m & ~bad
+    deep = [interior]
+    for _ in range(3):
+        deep.append(deeper(deep[-1]))
+    assert deep[3].sum() > 100
+    return c, d, a_cf, mu, deep
+
+
+def test_leapfrog_step_composition_closed_form():
+    """SolveDiffUpdateLeapFrog from rest (Source/SolverLeapfrog.cpp:249-301):
+    step 1 kicks v -= a dt/2 then v += a (dt_prev + dt)/2, so v_1 = a dt/2;
+    step 2 gives v_2 = v_1 + a dt = 3/2 a dt.  The density rate of step 1
+    sees v_{1/2} = v_1 at x_0 and the rates see v_1 at x_1, so on the
+    interior drho_1 = -rho mu tr(grad v_1) and SR_1 = mu sym(grad v_1);
+    step 2's rates see v_2 = 3 v_1."""
+    Q = np.array([[2.0, 0.5, -0.3], [0.5, -1.0, 0.8], [-0.3, 0.8, 1.5]]) * 1e12
+    c, d, a_cf, mu, deep = _pressure_block(Q)
+    dt, n, rho = c.dt, c.n, c.density[0]
+    d.solve_init(dt)
+    d.step()
+    m0, m1, m2, m3 = deep
+    scale = np.abs(a_cf[m0]).max()
+    assert np.abs(d.f("A")[:n][m0] - a_cf[m0]).max() < 1e-9 * scale
+    assert np.abs(d.f("V")[:n][m0] - 0.5 * dt * a_cf[m0]).max() < 1e-9 * scale * dt
+    grad_v1 = -mu * Q * (0.5 * dt) / rho
+    sr_scale = np.abs(grad_v1).max() * mu
+    assert np.abs(d.f("SR")[:n][m1] - mu * grad_v1).max() < 1e-6 * sr_scale
+    want = -rho * mu * np.trace(grad_v1)
+    assert np.abs(d.f("DRHO")[:n][m1] - want).max() < 1e-6 * abs(want)
+    d.step()
+    assert np.abs(d.f("V")[:n][m2] - 1.5 * dt * a_cf[m2]).max() < 1e-6 * scale * dt
+    assert np.abs(d.f("SR")[:n][m3] - 3.0 * mu * grad_v1).max() < 1e-5 * sr_scale
+
+
+def test_kickdrift_step_composition_closed_form():
+    """SolveDiffUpdateKickDrift from rest (Source/SolverKickDrift.cpp:224-279):
+    v += a dt_prev/2, density with that v at x_n, drift, v += a dt/2, rates
+    with v_{n+1} at x_{n+1}: after step 1 v_1 = a dt, drho_1 sees a dt/2 and
+    SR_1 sees a dt; after step 2 v_2 = 2 a dt."""
+    Q = np.array([[2.0, 0.5, -0.3], [0.5, -1.0, 0.8], [-0.3, 0.8, 1.5]]) * 1e12
+    c, d, a_cf, mu, deep = _pressure_block(Q)
+    dt, n, rho = c.dt, c.n, c.density[0]
+    d.solve_init(dt)
+    d.step("kickdrift")
+    m0, m1, m2, m3 = deep
+    scale = np.abs(a_cf[m0]).max()
+    assert np.abs(d.f("V")[:n][m0] - dt * a_cf[m0]).max() < 1e-9 * scale * dt
+    grad_half = -mu * Q * (0.5 * dt) / rho
+    want = -rho * mu * np.trace(grad_half)
+    assert np.abs(d.f("DRHO")[:n][m1] - want).max() < 1e-6 * abs(want)
+    sr_scale = np.abs(grad_half).max() * mu
+    assert np.abs(d.f("SR")[:n][m1] - 2.0 * mu * grad_half).max() < 1e-6 * sr_scale
+    d.step("kickdrift")
+    assert np.abs(d.f("V")[:n][m2] - 2.0 * dt * a_cf[m2]).max() < 1e-6 * scale * dt
+
+
+def test_reaction_is_read_before_the_bc_zeroes_acceleration():
+    """The reaction observer sums m a over each velocity-BC zone after
+    AccelReduction and before the BC zeroes a (Source/SolverLeapfrog.cpp:
+    225-228, Source/WeldForm.cpp:51-92).  The pair forces are antisymmetric,
+    so the zones' reactions plus m a of every other particle sum to zero."""
+    c = cases.small_jc(steps=3)
+    c.bcs[1]["value"] = (0.0, 0.0, -20.0)
+    d = oracle_from_case(c)
+    rng = np.random.default_rng(11)
+    d.f("V")[:] = rng.normal(0, 1.0, size=(c.n, 3))
+    s = rng.normal(0, 1e8, size=(c.n, 3, 3))
+    d.f("SIGMA")[:] = 0.5 * (s + np.transpose(s, (0, 2, 1)))
+    d.solve_init(c.dt)
+    for _ in range(3):
+        d.step()
+        ids = d.i("ID")
+        free = ~np.isin(ids, [b["zone"] for b in c.bcs])
+        ma = d.f("A")[free] * c.mass[free, None]
+        R = sum(d.reaction(b["zone"]) for b in c.bcs)
+        total = np.abs(d.f("A") * c.mass[:, None]).sum() + np.abs(R).sum()
+        assert np.abs(R).max() > 1e-3 * total / c.n
+        assert np.abs(R + ma.sum(axis=0)).max() < 1e-11 * total
+        # the zone particles' acceleration was zeroed after the observation
+        assert np.all(d.f("A")[~free] == 0.0)
+
+
 def test_neighbour_schedule():
     """Searches run on step 1, on the first step whose start-of-step max
     plastic strain exceeds 1e-6, and every ts_nb_inc = 5 steps after."""
