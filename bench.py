@@ -81,6 +81,7 @@ def timed_run(case, precision, steps, warmup):
     w.close()
     ti = time.perf_counter()
     s = cases.make_solver(case, precision=precision)
+    s.prepare()
     s.sync()
     init_s = time.perf_counter() - ti
     t0 = time.perf_counter()
@@ -101,7 +102,13 @@ def timed_run(case, precision, steps, warmup):
 def contact_run(steps, warmup=10):
     """Thermo-mechanical contact compression (weldform_amd.cases.config3,
     384,256 particles, two rigid planes of 2,048 triangles, conduction and
-    plastic heating), mixed precision, graph replay."""
+    plastic heating), mixed precision, graph replay.  After the timed run,
+    properties the physics fixes: the heat balance sum m c_p (T_N + dt
+    dT/dt_N - T_0) = 0.9 x plastic work + contact heat (ThermalCalcs
+    integrates T with the previous step's dT/dt), the signs of the planes'
+    contact forces, a finite state."""
+    import numpy as np
+
     from weldform_amd import capi, cases
 
     case = cases.config3(steps=steps)
@@ -111,19 +118,32 @@ def contact_run(steps, warmup=10):
     w.sync()
     w.close()
     s = cases.make_solver(case, precision=capi.MIXED)
+    s.prepare()
+    T0 = s.get("T")
     s.sync()
     t0 = time.perf_counter()
     s.step(steps)
     s.sync()
     wall = time.perf_counter() - t0
     sc = s.scalars()
-    F = s.contact_force_history(0)
+    F = np.stack([s.contact_force_history(0), s.contact_force_history(1)], axis=1)
+    T, dTdt = s.get("T"), s.get("dTdt")
+    cp = case.materials[0]["cp_T"]
+    gain = float(np.sum(case.mass * cp * (T + case.dt * dTdt - T0)))
+    heat = 0.9 * sc["plastic_work"] + sc["accum_cont_heat_cond"] + sc["contact_friction_work"]
+    finite = bool(np.isfinite(T).all() and np.isfinite(s.get("disp")).all())
     res = {"workload": f"{case.name}: R=15mm L=30mm dx=0.3757mm JC steel, thermal + plastic heat, "
                        f"top plane -5 m/s, bottom plane fixed, mu=0.2, {steps} Leapfrog steps from t=0, mixed",
            "particles": case.n, "steps": steps, "ms_per_step": 1e3 * wall / steps,
            "particle_steps_per_s": case.n * steps / wall, "contact_pairs": sc["contact_pairs"],
-           "top_plane_force_z_last": float(F[-1, 2]) if len(F) else 0.0, "max_T": sc["max_T"],
-           "plastic_work": sc["plastic_work"], "max_pl_strain": sc["max_pl_strain"]}
+           "top_plane_force_z_mean_last_200": float(F[-200:, 0, 2].mean()),
+           "bottom_plane_force_z_mean_last_200": float(F[-200:, 1, 2].mean()), "max_T": sc["max_T"],
+           "plastic_work": sc["plastic_work"], "max_pl_strain": sc["max_pl_strain"],
+           "nb_searches": sc["nb_searches"],
+           "checks": {"heat_gain_J": gain, "heat_sources_J": heat,
+                      "heat_balance_rel_err": abs(gain - heat) / heat if heat > 0 else None,
+                      "plane_force_signs_ok": bool(F[-200:, 0, 2].mean() < 0 < F[-200:, 1, 2].mean()),
+                      "finite_state": finite}}
     log("bench: config3 " + json.dumps(res))
     s.close()
     return res
@@ -237,6 +257,7 @@ def batch_run(steps, n_sample=16):
     w.sync()
     w.close()
     s = cases.make_solver(case, precision=capi.MIXED, **best)
+    s.prepare()
     s.sync()
     t0 = time.perf_counter()
     s.step(steps)
@@ -319,22 +340,60 @@ def output_run(s, n, reps=3):
     return res
 
 
-def cpu_baseline(case, steps):
-    sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
-    from oracle_case import oracle_from_case
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
-    threads = min(16, os.cpu_count() or 1)
-    d = oracle_from_case(case, threads=threads)
-    d.solve_init(case.dt)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+def cpu_baseline(steps2, steps1=300, reps=3):
+    """The oracle (C + OpenMP restatement of SolveDiffUpdateLeapFrog in the
+    reference's Nproc = 1 pair order) on the host cores: config 2 (the
+    headline workload) and config 1.  Step 1 (with the initial neighbour
+    search) runs untimed; then `reps` runs of the step loop are timed with
+    the steady clock and the median is reported, as the reference's step
+    loop would be timed (set-up excluded)."""
+    sys.path[:0] = [os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")]
+    import statistics
+
+    from oracle_case import oracle_from_case
+    from weldform_amd import cases
+
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = int(env) if env and env.isdigit() else len(os.sched_getaffinity(0))
+    out = {}
+    for name, case, k in (("config2", cases.config2(steps=steps2), steps2),
+                          ("config1", cases.config1(steps=steps1), steps1)):
+        d = oracle_from_case(case, threads=threads)
+        d.solve_init(case.dt)
         d.step()
-    wall = time.perf_counter() - t0
-    return {"value": case.n * steps / wall, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle (C+OpenMP restatement of SolveDiffUpdateLeapFrog, Nishimura order) "
-                      f"on the same {case.n}-particle case, first {steps} steps incl. the initial "
-                      f"neighbour search, {wall:.1f} s wall"}
+        walls = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                d.step()
+            walls.append(time.perf_counter() - t0)
+        wall = statistics.median(walls)
+        pairs = len(d.pairs())
+        out[name] = {"particles": case.n, "steps": k, "median_wall_s": wall, "walls_s": walls,
+                     "particle_steps_per_s": case.n * k / wall, "pair_evaluations_per_s": pairs * k / wall,
+                     "ms_per_step": 1e3 * wall / k, "pairs_unordered": pairs}
+        log(f"bench: cpu baseline {name} " + json.dumps(out[name]))
+        del d
+    c2 = out["config2"]
+    return {"value": c2["particle_steps_per_s"], "unit": "particle-steps/s", "cores": threads,
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"oracle (C+OpenMP restatement of SolveDiffUpdateLeapFrog, Nishimura order, "
+                      f"gcc -O3) on the {c2['particles']}-particle headline case: step 1 (initial search) "
+                      f"untimed, then the median of {reps} timed runs of {steps2} steps "
+                      f"({c2['median_wall_s']:.2f} s); config 1 (4,524 particles) median of {reps} x "
+                      f"{steps1} steps beside it",
+            "configs": out}
 
 
 def main():
@@ -342,10 +401,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="timed steps per run of the CPU baseline on the headline case (median of 3 runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-output", action="store_true", help="skip the output-file leg")
-    ap.add_argument("--config3-steps", type=int, default=400,
+    ap.add_argument("--config3-steps", type=int, default=2000,
                     help="steps of the thermo-mechanical contact case reported beside the headline (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=500,
                     help="steps of the batched-ensemble leg (0: skip)")
@@ -465,9 +525,8 @@ def main():
     if args.adjoint_steps > 0:
         out["config"].update(adjoint_legs(args.adjoint_steps))
     if not args.no_cpu:
-        ccase = cases.config2(steps=args.cpu_steps)
-        log(f"bench: cpu baseline, {args.cpu_steps} steps")
-        out["cpu_baseline"] = cpu_baseline(ccase, args.cpu_steps)
+        log(f"bench: cpu baseline, {args.cpu_steps} steps x 3")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_steps)
     print(json.dumps(out), flush=True)
 
 

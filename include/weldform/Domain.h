@@ -10,11 +10,16 @@
 // solver entry points SolveDiffUpdateLeapFrog / SolveDiffUpdateLeapfrog /
 // SolveDiffUpdateKickDrift (tf, dt, dtOut, FileKey, maxidx).
 //
-// Boundary conditions are declarative: the velocity and temperature BCs in
-// bConds (with amps) are applied on the device where the reference calls
-// GeneralAfter = UserAcc (Source/WeldForm.cpp:51-120).  An arbitrary host
-// callback in GeneralBefore / GeneralAfter cannot run inside a device step
-// and is rejected.  Errors are thrown as Fatal*, as in the reference.
+// Boundary conditions: the velocity and temperature BCs in bConds (with
+// amps) are applied on the device where the reference calls GeneralAfter =
+// UserAcc (Source/WeldForm.cpp:51-120), inside captured device steps.  A
+// driver's own host function in GeneralBefore / GeneralAfter (e.g.
+// Compression.cpp:39-76,133) runs on a slow path: each step is cut at the
+// reference's four call sites (wf_step_part), and around each call the
+// fields a hook reads (x, v, a, Displacement, Density, T, ID, and the
+// Domain's time and sums) are copied into the Particle objects and the ones
+// it writes (v, a, and T with the thermal solver) copied back.  Errors are
+// thrown as Fatal*, as in the reference.
 //
 // Link with -lweldform_hip -lweldform_io (weldform_amd/lib); compiles with
 // any C++17 compiler, no HIP headers needed.
@@ -115,6 +120,10 @@ class Domain {
   int solid_part_count = -1;
   double kin_energy_sum = 0.0, int_energy_sum = 0.0, plastic_work = 0.0;
   double m_maxT = 0.0, m_minT = 0.0;
+  // Domain::max_disp (Source/Domain.h:470): the Leapfrog and KickDrift loops
+  // compute a local of the same name (Source/SolverLeapfrog.cpp:140-148), so
+  // the member a hook can read stays (0, 0, 0) on these paths
+  Vec3_t max_disp = Vec3_t(0., 0., 0.);
   // device-path extensions
   Precision precision = Precision::FP64;
   int device = 0;
@@ -366,8 +375,70 @@ class Domain {
       }
       if (trimesh.size() > 4) throw new Fatal("at most 4 contact meshes");
     }
-    if (GeneralBefore != &General || GeneralAfter != &General)
-      throw new Fatal("host callbacks cannot run inside a device step: express BCs through bConds / amps");
+  }
+
+  bool host_hooks() const { return GeneralBefore != &General || GeneralAfter != &General; }
+
+  // the slow path around a host hook: what the hook may read, from the
+  // device into the Particle objects ...
+  void hook_pull() {
+    const size_t n = nsolid();
+    std::vector<double> v3(3 * n), v1(n);
+    auto get3 = [&](int f, Vec3_t Particle::*m) {
+      check(wf_get_field(ctx_, f, v3.data()));
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) (Particles[i]->*m)(k) = v3[3 * i + k];
+    };
+    auto get1 = [&](int f, double Particle::*m) {
+      check(wf_get_field(ctx_, f, v1.data()));
+      for (size_t i = 0; i < n; ++i) Particles[i]->*m = v1[i];
+    };
+    get3(WF_F_X, &Particle::x);
+    get3(WF_F_V, &Particle::v);
+    get3(WF_F_A, &Particle::a);
+    get3(WF_F_DISP, &Particle::Displacement);
+    get1(WF_F_DENSITY, &Particle::Density);
+    get1(WF_F_T, &Particle::T);
+    std::vector<int32_t> id(n);
+    check(wf_get_int_field(ctx_, WF_I_ID, id.data()));
+    for (size_t i = 0; i < n; ++i) Particles[i]->ID = id[i];
+    pull_scalars();
+  }
+  // ... and what it may write, back to the device
+  void hook_push() {
+    const size_t n = nsolid();
+    std::vector<double> v3(3 * n), v1(n);
+    auto put3 = [&](int f, Vec3_t Particle::*m) {
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) v3[3 * i + k] = (Particles[i]->*m)(k);
+      check(wf_set_field(ctx_, f, v3.data()));
+    };
+    put3(WF_F_V, &Particle::v);
+    put3(WF_F_A, &Particle::a);
+    if (thermal_solver) {
+      for (size_t i = 0; i < n; ++i) v1[i] = Particles[i]->T;
+      check(wf_set_field(ctx_, WF_F_T, v1.data()));
+    }
+  }
+  void run_hook(PtDom f) {
+    if (f == &General) return;
+    hook_pull();
+    f(*this);
+    hook_push();
+  }
+  // one step with the host hooks at the reference's call sites
+  // (Source/SolverLeapfrog.cpp:215,228,254,273; Source/SolverKickDrift.cpp:
+  // 196,208,231,267)
+  void step_with_hooks(int scheme) {
+    check(wf_step_part(ctx_, scheme, WF_PART_SEARCH));
+    run_hook(GeneralBefore);
+    check(wf_step_part(ctx_, scheme, WF_PART_ACCEL));
+    run_hook(GeneralAfter);
+    check(wf_step_part(ctx_, scheme, WF_PART_KICK1));
+    run_hook(GeneralAfter);
+    check(wf_step_part(ctx_, scheme, WF_PART_KICK2));
+    run_hook(GeneralAfter);
+    check(wf_step_part(ctx_, scheme, WF_PART_REST));
   }
 
   // one wf_material per Particle::Material id; particles sharing an id must
@@ -545,6 +616,10 @@ class Domain {
       Particles[i]->ContNb = cnb[i];
       Particles[i]->CalculateEquivalentStress();
     }
+    pull_scalars();
+  }
+
+  void pull_scalars() {
     wf_scalars s{};
     check(wf_get_scalars(ctx_, &s));
     kin_energy_sum = s.kin_energy_sum;
@@ -553,6 +628,7 @@ class Domain {
     ext_forces_work = s.ext_forces_work;
     contact_friction_work = s.contact_friction_work;
     accum_cont_heat_cond = s.accum_cont_heat_cond;
+    min_force_ts = s.min_force_ts;
     int_energy_sum = s.int_energy_sum;
     plastic_work = s.plastic_work;
     m_maxT = s.max_T;
@@ -773,8 +849,10 @@ class Domain {
       int64_t k = 0;
       double t = Time;
       bool out = false;
+      // host hooks: one step per stretch, cut at the hooks' call sites
+      const int64_t chunk = host_hooks() ? 1 : kMaxChunk;
       if (!adaptive) {
-        while (t <= tf && k < kMaxChunk) {
+        while (t <= tf && k < chunk) {
           t += deltat;
           ++k;
           if (t >= tout) {
@@ -787,14 +865,17 @@ class Domain {
         const double rem = std::min(tout, tf) - Time, floor_ts = (double)manual_min_ts;
         double bound = 0.0, step = deltat;
         k = 1;
-        while (k < kMaxChunk) {
+        while (k < chunk) {
           step = std::min(std::max(2.0 * step, floor_ts), std::max(dt, floor_ts));
           if ((bound + step) * (1.0 + 1e-9) >= rem) break;
           bound += step;
           ++k;
         }
       }
-      check(wf_step(ctx_, scheme, k));
+      if (host_hooks())
+        step_with_hooks(scheme);
+      else
+        check(wf_step(ctx_, scheme, k));
       check(wf_sync(ctx_));
       pull_reactions(k);
       if (adaptive) {

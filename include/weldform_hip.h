@@ -49,7 +49,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 7
+#define WF_ABI_VERSION 8
 
 /* error codes */
 #define WF_OK 0
@@ -220,6 +220,7 @@ typedef struct wf_scalars {
   int64_t contact_pairs;    /* contact pairs of the last search (ContPairs)   */
   double contact_friction_work; /* Domain::contact_friction_work (friction heat) */
   double accum_cont_heat_cond;  /* Domain::accum_cont_heat_cond (conducted heat) */
+  double min_force_ts;      /* Domain::min_force_ts (contact step criterion, auto_ts bit 2) */
 } wf_scalars;
 
 int wf_abi_version(void);
@@ -298,7 +299,36 @@ int wf_get_rigid_field(void *ctx, int32_t field, double *host);
 /* solve */
 int wf_init(void *ctx, double dt);
 int wf_step(void *ctx, int32_t scheme, int64_t nsteps); /* enqueues; see wf_sync */
+/* set-up of the step loop of `scheme` (WF_LEAPFROG / WF_KICKDRIFT) without
+ * running a step: captures and instantiates the step's HIP graphs (one per
+ * position-buffer parity), which wf_step otherwise does on its first two
+ * calls.  No reference counterpart (the reference has no set-up of its
+ * loop beyond SolveDiffUpdateLeapFrog's own first lines,
+ * Source/SolverLeapfrog.cpp:24-123); optional. */
+int wf_prepare(void *ctx, int32_t scheme);
 int wf_sync(void *ctx);                                 /* waits, reports device errors */
+/* one part of a step, for a caller that runs host hooks between the parts:
+ * the reference's GeneralBefore / GeneralAfter (Source/Domain.h:398-399),
+ * called at Source/SolverLeapfrog.cpp:215,228,254,273 and
+ * Source/SolverKickDrift.cpp:196,208,231,267.  Parts 0..4 in order per step:
+ *   WF_PART_SEARCH  neighbour schedule and search            -> GeneralBefore
+ *   WF_PART_ACCEL   CalcAccel + AccelReduction, then the reaction observer
+ *                   (before any hook changes a)              -> GeneralAfter
+ *   WF_PART_KICK1   contact; Leapfrog: the first-step half kick;
+ *                   KickDrift: the time step, v += a dt_prev/2 -> GeneralAfter
+ *   WF_PART_KICK2   Leapfrog: the time step, v += a (dt_prev + dt)/2;
+ *                   KickDrift: density, drift, v += a dt/2   -> GeneralAfter
+ *   WF_PART_REST    the rest of the step
+ * Each call waits for its work, so fields a hook changes with wf_set_field
+ * (v, a, T, ...) enter the parts after it.  The velocity and temperature BCs
+ * of wf_add_*_bc still apply in the kick parts, as UserAcc would.  wf_step
+ * is refused while a step is part-way. */
+#define WF_PART_SEARCH 0
+#define WF_PART_ACCEL 1
+#define WF_PART_KICK1 2
+#define WF_PART_KICK2 3
+#define WF_PART_REST 4
+int wf_step_part(void *ctx, int32_t scheme, int32_t part);
 int wf_get_scalars(void *ctx, wf_scalars *out);
 /* reaction sum(m*a) of every particle of a velocity-BC zone, taken where
  * the reference first calls GeneralAfter in a step (after AccelReduction,

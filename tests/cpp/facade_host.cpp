@@ -76,6 +76,8 @@ int main() {
   });
   expect_fatal("kernel", [](SPH::Domain &d) { d.Kernel_Set(SPH::Quintic_Spline); });
   expect_fatal("gradient", [](SPH::Domain &d) { d.Gradient_Approach_Set(SPH::Randles_Libersky); });
+  // host hooks are accepted (the slow path): without a device the solve
+  // fails where every solve does, at wf_create
   expect_fatal("callback", [](SPH::Domain &d) { d.GeneralAfter = [](SPH::Domain &) {}; });
   expect_fatal("contact", [](SPH::Domain &d) { d.contact = true; });
   expect_fatal("fail", [](SPH::Domain &d) { d.Particles[3]->Fail = 0; });

@@ -79,6 +79,44 @@ def test_facade_driver_matches_oracle(tmp_path, scheme, steps):
         assert rows == case.n
 
 
+@pytest.mark.parametrize("scheme,steps", [("leapfrog", 300), ("kickdrift", 150)])
+def test_host_hook_driver_matches_oracle(tmp_path, scheme, steps):
+    """A driver whose BCs are a host GeneralAfter (Compression.cpp's form,
+    tests/cpp/compression_hooks.cpp: a ramped top-row velocity, a fixed
+    bottom row, its own reaction observer) runs on the facade's slow path,
+    the step cut at the reference's GeneralBefore / GeneralAfter call sites.
+    The oracle runs the same BCs as UserAcc zones with the equivalent
+    amplitude; FP64 bar 1e-8 after the run, as the other facade tests."""
+    if not os.path.exists(wfbuild.HOOK_DRIVER):
+        wfbuild.build_hook_driver()
+    key = str(tmp_path / "hooks")
+    print(f"hook driver: {steps} steps {scheme}", flush=True)
+    r = subprocess.run([wfbuild.HOOK_DRIVER, str(steps), key, scheme], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stdout + r.stderr
+    words = r.stdout.split()
+    assert int(words[words.index("GeneralBefore") - 1]) == steps
+    assert int(words[words.index("GeneralAfter") - 1]) == 3 * steps
+    react = np.loadtxt(key + "_reaction.txt").reshape(-1, 3)
+    final = np.loadtxt(key + "_final.txt")
+    case = cases.config1(steps=steps)
+    tau = 50 * case.dt
+    case.amps = [dict(id=1, t=[0.0, tau, 1.0], v=[0.0, 1.0, 1.0])]
+    case.bcs = [dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, 0.0, -1.0), value_type=1, amp_id=1)]
+    d = oracle_from_case(case, threads=8)
+    d.solve_init(case.dt)
+    R = []
+    for _ in range(steps):
+        d.step(scheme)
+        R.append(d.reaction(case.top_zone).copy())
+    R = np.array(R)
+    assert react.shape == R.shape
+    e_r = normwise_err(react[:, 2], R[:, 2])
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    e_p = normwise_err(final[:, 4], d.f("PLSTRAIN")) if np.abs(d.f("PLSTRAIN")).max() > 0 else 0.0
+    print(f"hook driver {scheme}: reaction {e_r:.3e} displacement {e_u:.3e} pl_strain {e_p:.3e}", flush=True)
+    assert e_r < 1e-8 and e_u < 1e-8 and e_p < 1e-8
+
+
 def test_facade_driver_mixed_precision(tmp_path):
     steps = 300
     _, react, final = _run_driver(tmp_path, steps, "leapfrog", "mixed")

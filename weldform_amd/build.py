@@ -17,6 +17,8 @@ ARCH = os.environ.get("WF_OFFLOAD_ARCH", "gfx950")
 ROOT = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "upsetting.cpp")
 DRIVER = os.path.join(LIBDIR, "upsetting")
+HOOK_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "compression_hooks.cpp")
+HOOK_DRIVER = os.path.join(LIBDIR, "compression_hooks")
 JSON_DRIVER_SRC = os.path.join(HERE, "driver", "weldform.cpp")
 JSON_DRIVER = os.path.join(LIBDIR, "WeldForm")
 
@@ -121,6 +123,12 @@ def build_driver(verbose=False):
     return _gxx(DRIVER_SRC, DRIVER, verbose)
 
 
+def build_hook_driver(verbose=False):
+    """The test driver whose boundary conditions are a host GeneralAfter
+    hook (tests/cpp/compression_hooks.cpp)."""
+    return _gxx(HOOK_DRIVER_SRC, HOOK_DRIVER, verbose)
+
+
 def build_json_driver(verbose=False):
     """The WeldForm command-line driver: JSON scene files of the reference's
     format on the device path (weldform_amd/driver/weldform.cpp)."""
@@ -131,4 +139,5 @@ if __name__ == "__main__":
     print(build(verbose=True))
     print(build_io(verbose=True))
     print(build_driver(verbose=True))
+    print(build_hook_driver(verbose=True))
     print(build_json_driver(verbose=True))

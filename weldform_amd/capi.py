@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libweldform_hip.so")
 OK, EINVAL, EHIP, ESTATE, ENBOVERFLOW, ENAN, EUNSUPPORTED, ENOMEM = 0, -1, -2, -3, -4, -5, -6, -7
 FP64, MIXED = 0, 1
 LEAPFROG, KICKDRIFT = 0, 1
+PART_SEARCH, PART_ACCEL, PART_KICK1, PART_KICK2, PART_REST = range(5)
 BILINEAR, HOLLOMON, JOHNSON_COOK = 0, 1, 2
 
 FIELDS = {
@@ -68,7 +69,7 @@ class Scalars(ctypes.Structure):
         (n, ctypes.c_double) for n in
         ["kin_energy_sum", "int_energy_sum", "plastic_work", "max_pl_strain", "max_T", "min_T",
          "contact_force_sum", "contact_reaction_sum", "ext_forces_work"]] + [("contact_pairs", ctypes.c_int64)] + [
-        (n, ctypes.c_double) for n in ["contact_friction_work", "accum_cont_heat_cond"]]
+        (n, ctypes.c_double) for n in ["contact_friction_work", "accum_cont_heat_cond", "min_force_ts"]]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -94,7 +95,7 @@ EXPORTS = [
     "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
     "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
-    "wf_set_contact_heat", "wf_set_rigid_temperature",
+    "wf_set_contact_heat", "wf_set_rigid_temperature", "wf_prepare", "wf_step_part",
 ]
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
@@ -133,6 +134,8 @@ def lib():
         "wf_add_temperature_bc": ([vp, i32, d], ctypes.c_int),
         "wf_init": ([vp, d], ctypes.c_int),
         "wf_step": ([vp, i32, i64], ctypes.c_int),
+        "wf_prepare": ([vp, i32], ctypes.c_int),
+        "wf_step_part": ([vp, i32, i32], ctypes.c_int),
         "wf_sync": ([vp], ctypes.c_int),
         "wf_get_scalars": ([vp, ctypes.POINTER(Scalars)], ctypes.c_int),
         "wf_get_reaction_history": ([vp, i32, dp, i64, I64P], ctypes.c_int),
@@ -277,6 +280,14 @@ class Solver:
 
     def step(self, nsteps=1, scheme=LEAPFROG):
         self._check(lib().wf_step(self.ctx, scheme, int(nsteps)))
+
+    def step_part(self, part, scheme=LEAPFROG):
+        """One part of a step (PART_SEARCH .. PART_REST), for host hooks between parts."""
+        self._check(lib().wf_step_part(self.ctx, scheme, int(part)))
+
+    def prepare(self, scheme=LEAPFROG):
+        """Capture the step's HIP graphs now rather than in the first steps."""
+        self._check(lib().wf_prepare(self.ctx, scheme))
 
     def sync(self):
         self._check(lib().wf_sync(self.ctx))

@@ -91,6 +91,7 @@ struct Ctx {
   int cur = 0;
   double dt = 0.0;  // the initial step (Domain::deltatint); the device flags hold the current one
   long long steps = 0;
+  int next_part = 0;  // the part wf_step_part expects next (0: a step boundary)
   std::vector<int> rzones;
   BcTable bc{};
   bool bc_dirty = true;
@@ -418,7 +419,8 @@ void allocate(Ctx *c) {
   d.cell_count = c->alloc<int>((size_t)d.ncells + 1);
   d.cell_start = c->alloc<int>((size_t)d.ncells + 1);
   d.cell_cursor = c->alloc<int>((size_t)d.ncells + 1);
-  d.scan_tmp = c->alloc<int>((size_t)(d.ncells + 1023) / 1024 + 1);
+  d.sxyz = c->alloc<float4>(np);
+  d.sinfo = c->alloc<int2>(np);
   c->dmats = c->alloc<DevMat>(kMaxMat);
   d.mats = c->dmats;
   c->damps = c->alloc<DevAmp>(kMaxAmp);
@@ -462,6 +464,11 @@ void allocate(Ctx *c) {
     d.mv = c->alloc<double4>(kMaxMesh);
     d.mw = c->alloc<double4>(kMaxMesh);
     d.nodev = c->alloc<double4>(nn);
+    d.rcell = c->alloc<int>(ne);
+    d.rsorted = c->alloc<int>(ne);
+    d.rcount = c->alloc<int>((size_t)d.ncells + 1);
+    d.rstart = c->alloc<int>((size_t)d.ncells + 1);
+    d.rcursor = c->alloc<int>((size_t)d.ncells + 1);
     d.id_orig = c->alloc<int>(np);
     d.ccnt = c->alloc<int>(np);
     d.cidx = c->alloc<int>((size_t)kMaxContact * np);
@@ -785,67 +792,129 @@ void tape_state(Ctx *c, const AdjSlot &s, int cur) {
   d2d(c, s.pl, d.pl, np * sizeof(double));
 }
 
-// one iteration of the Leapfrog / KickDrift loop, all on the stream
-void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr) {
+// One iteration of the Leapfrog / KickDrift loop, all on the stream.  The
+// iteration is cut into parts where the reference calls its host hooks
+// (Source/SolverLeapfrog.cpp:215-273, Source/SolverKickDrift.cpp:195-264):
+//   P_SEARCH  neighbour schedule and search            -> GeneralBefore
+//   P_ACCEL   CalcAccel + AccelReduction               -> GeneralAfter
+//   P_KICK1   contact; Leapfrog: BCs and the first-step half kick;
+//             KickDrift: the time step and v += a dt_prev / 2  -> GeneralAfter
+//   P_KICK2   Leapfrog: the time step and v += a (dt_prev + dt) / 2;
+//             KickDrift: density, drift, v += a dt / 2 -> GeneralAfter
+//   P_REST    the rest of the step (Leapfrog: density and drift first)
+// part < 0 enqueues the whole iteration (the captured graph); part >= 0
+// enqueues one part for a caller that runs host hooks between the parts,
+// with the reaction observed at the end of P_ACCEL, before any hook.
+enum { P_SEARCH = 0, P_ACCEL, P_KICK1, P_KICK2, P_REST, P_PARTS };
+
+void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int part = -1) {
   Dev &d = c->d;
   Timed whole(c, T_STEP);
   const size_t np = (size_t)c->npad;
-  if (tp) tape_state(c, tp->s, cur);
-  launch_schedule(d, c->st, 0);
-  {
-    Timed t(c, T_NBR);
-    launch_neighbours(d, c->st, cur);
-    if (d.gc_on) launch_gradcorr(d, c->st, cur);
-    if (d.contact) launch_surface(d, c->st, cur, 0);
-  }
-  if (tp) launch_tape_lists(d, c->st, tp->s, tp->force_lists, tp->ver, tp->slot);
-  // StartAcceleration's tensile-instability stress (Source/Domain.cpp:1529-1563)
-  if (d.ti_on) launch_tir(d, c->st, cur);
-  {
-    Timed t(c, T_ACCEL);
-    launch_accel(d, c->st, cur);
-  }
-  if (tp && d.contact) d2d(c, tp->s.apre, d.acc, np * sizeof(double4));
-  // contact acts after AccelReduction and the first GeneralAfter
-  // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
-  // so no BC zone applies to them
+  const bool split = part >= 0;
+  auto in = [&](int p) { return part < 0 || part == p; };
   // dt < 0: every kernel reads the step size from the device flags, which
   // the adaptive step updates between the two halves of the kick
   // (Source/SolverLeapfrog.cpp:241-262, Source/SolverKickDrift.cpp:208-226)
   const double dt = d.auto_ts ? -1.0 : c->dt;
-  if (d.contact) launch_contact(d, c->st, c->bc, cur, dt);
-  if (tp && d.contact) {
-    d2d(c, tp->s.node, d.node, (size_t)d.nnode * sizeof(double4));
-    d2d(c, tp->s.enorm, d.enorm, (size_t)d.nrig * sizeof(double4));
-    d2d(c, tp->s.rv, d.rv, (size_t)d.nrig * sizeof(double4));
-  }
-  if (d.auto_ts) {
-    launch_kick(d, c->st, c->bc, scheme, dt, dt, 1);
-    launch_time_step(d, c->st, cur);
-    launch_kick(d, c->st, c->bc, scheme, dt, dt, 2);
-  } else {
-    launch_kick(d, c->st, c->bc, scheme, dt, dt, 0);
-  }
-  {
-    Timed t(c, T_DENS);
-    launch_dens_drift(d, c->st, cur, dt, 1);
-  }
-  if (scheme == WF_KICKDRIFT) launch_kick2(d, c->st, dt);
-  if (d.contact) launch_rigid_drift(d, c->st, dt);
   const int nxt = cur ^ 1;
-  {
-    Timed t(c, T_RATES);
-    launch_rates(d, c->st, nxt);
+  if (in(P_SEARCH)) {
+    if (tp) tape_state(c, tp->s, cur);
+    launch_schedule(d, c->st, 0);
+    {
+      Timed t(c, T_NBR);
+      launch_neighbours(d, c->st, cur);
+      if (d.gc_on) launch_gradcorr(d, c->st, cur);
+      if (d.contact) launch_surface(d, c->st, cur, 0);
+    }
+    if (tp) launch_tape_lists(d, c->st, tp->s, tp->force_lists, tp->ver, tp->slot);
   }
-  launch_stress(d, c->st, nxt, dt);
-  if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, dt);
-  launch_finalize(d, c->st, dt, c->bc.nz);
-  if (d.contact) launch_mesh_update(d, c->st, dt);
-  if (tp) {
-    d2d(c, tp->s.sr, d.sr, 6 * np * sizeof(double));
-    d2d(c, tp->s.rr, d.rr, 3 * np * sizeof(double));
-    d2d(c, tp->s.drho, d.drho, np * sizeof(double));
+  if (in(P_ACCEL)) {
+    // StartAcceleration's tensile-instability stress (Source/Domain.cpp:1529-1563)
+    if (d.ti_on) launch_tir(d, c->st, cur);
+    {
+      Timed t(c, T_ACCEL);
+      launch_accel(d, c->st, cur);
+    }
+    if (tp && d.contact) d2d(c, tp->s.apre, d.acc, np * sizeof(double4));
+    if (split) launch_kick(d, c->st, c->bc, scheme, dt, dt, K_OBSERVE);
   }
+  if (in(P_KICK1)) {
+    // contact acts after AccelReduction and the first GeneralAfter
+    // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
+    // so no BC zone applies to them
+    if (d.contact) launch_contact(d, c->st, c->bc, cur, dt);
+    if (tp && d.contact) {
+      d2d(c, tp->s.node, d.node, (size_t)d.nnode * sizeof(double4));
+      d2d(c, tp->s.enorm, d.enorm, (size_t)d.nrig * sizeof(double4));
+      d2d(c, tp->s.rv, d.rv, (size_t)d.nrig * sizeof(double4));
+    }
+    const int obs = split ? 0 : 1;
+    if (scheme == WF_LEAPFROG) {
+      if (!split && !d.auto_ts) {
+        launch_kick(d, c->st, c->bc, scheme, dt, dt, K_WHOLE);
+      } else {
+        launch_kick(d, c->st, c->bc, scheme, dt, dt, obs ? K_FIRST : K_FIRST_NOOBS);
+      }
+    } else {
+      if (d.auto_ts) {
+        launch_kick(d, c->st, c->bc, scheme, dt, dt, obs ? K_FIRST : K_FIRST_NOOBS);
+        launch_time_step(d, c->st, cur);
+        launch_kick(d, c->st, c->bc, scheme, dt, dt, K_SECOND);
+      } else {
+        launch_kick(d, c->st, c->bc, scheme, dt, dt, obs ? K_WHOLE : K_WHOLE_NOOBS);
+      }
+    }
+  }
+  if (in(P_KICK2)) {
+    if (scheme == WF_LEAPFROG) {
+      if (d.auto_ts) launch_time_step(d, c->st, cur);
+      if (split || d.auto_ts) launch_kick(d, c->st, c->bc, scheme, dt, dt, K_SECOND);
+    } else {
+      {
+        Timed t(c, T_DENS);
+        launch_dens_drift(d, c->st, cur, dt, 1);
+      }
+      launch_kick2(d, c->st, dt);
+      if (d.contact) launch_rigid_drift(d, c->st, dt);
+    }
+  }
+  if (in(P_REST)) {
+    if (scheme == WF_LEAPFROG) {
+      {
+        Timed t(c, T_DENS);
+        launch_dens_drift(d, c->st, cur, dt, 1);
+      }
+      if (d.contact) launch_rigid_drift(d, c->st, dt);
+    }
+    {
+      Timed t(c, T_RATES);
+      launch_rates(d, c->st, nxt);
+    }
+    launch_stress(d, c->st, nxt, dt);
+    if (c->cfg.thermal_solver) launch_thermal(d, c->st, nxt, dt);
+    launch_finalize(d, c->st, dt, c->bc.nz);
+    if (d.contact) launch_mesh_update(d, c->st, dt);
+    if (tp) {
+      d2d(c, tp->s.sr, d.sr, 6 * np * sizeof(double));
+      d2d(c, tp->s.rr, d.rr, 3 * np * sizeof(double));
+      d2d(c, tp->s.drho, d.drho, np * sizeof(double));
+    }
+  }
+}
+
+// the captured step of (scheme, buffer parity cur), instantiated on first use
+hipGraphExec_t step_graph(Ctx *c, int scheme, int cur) {
+  hipGraphExec_t &ex = c->gexec[scheme][cur];
+  if (!ex) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c, scheme, cur);
+    HIPCHK(hipStreamEndCapture(c->st, &g));
+    HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(g));
+  }
+  return ex;
 }
 
 void one_step(Ctx *c, int scheme) {
@@ -854,16 +923,7 @@ void one_step(Ctx *c, int scheme) {
     enqueue_step(c, scheme, c->cur);
     HIPCHK(hipGetLastError());
   } else {
-    hipGraphExec_t &ex = c->gexec[scheme][c->cur];
-    if (!ex) {
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-      enqueue_step(c, scheme, c->cur);
-      HIPCHK(hipStreamEndCapture(c->st, &g));
-      HIPCHK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
-      HIPCHK(hipGraphDestroy(g));
-    }
-    HIPCHK(hipGraphLaunch(ex, c->st));
+    HIPCHK(hipGraphLaunch(step_graph(c, scheme, c->cur), c->st));
   }
   c->built = true;
   c->cur ^= 1;
@@ -1431,6 +1491,11 @@ int wf_init(void *ctx, double dt) {
           throw Error(WF_EUNSUPPORTED, "mixed precision needs one smoothing length per material");
       }
     }
+    if ((c->heat_cond || c->heat_fric) && c->meshes.empty())
+      throw Error(WF_EINVAL, "contact heat (wf_set_contact_heat) needs a rigid mesh (wf_add_rigid_plane / _mesh)");
+    if (c->heat_cond && c->rigid_T.empty())
+      throw Error(WF_EINVAL,
+                  "contact conduction needs the rigid particles' temperatures (wf_set_rigid_temperature)");
     HIPCHK(hipSetDevice(c->cfg.device));
     c->dt = dt;
     cell_initiate(c);
@@ -1460,7 +1525,47 @@ int wf_step(void *ctx, int32_t scheme, int64_t nsteps) {
     as_ctx(ctx);
     if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
     if (scheme != WF_LEAPFROG && scheme != WF_KICKDRIFT) throw Error(WF_EINVAL, "unknown scheme");
+    if (c->next_part != 0) throw Error(WF_ESTATE, "wf_step inside a step run by wf_step_part");
     for (int64_t s = 0; s < nsteps; ++s) one_step(c, scheme);
+    return WF_OK;
+  });
+}
+
+int wf_step_part(void *ctx, int32_t scheme, int32_t part) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (scheme != WF_LEAPFROG && scheme != WF_KICKDRIFT) throw Error(WF_EINVAL, "unknown scheme");
+    if (part < 0 || part >= P_PARTS) throw Error(WF_EINVAL, "part must be 0..4");
+    if (part != c->next_part)
+      throw Error(WF_ESTATE, "wf_step_part: expected part " + std::to_string(c->next_part) + ", got " +
+                                 std::to_string(part));
+    if (part == 0) refresh_bc(c);
+    enqueue_step(c, scheme, c->cur, nullptr, part);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    harvest_timing(c);
+    c->next_part = (part + 1) % P_PARTS;
+    if (c->next_part == 0) {
+      c->built = true;
+      c->cur ^= 1;
+      c->steps += 1;
+      check_flags(c);
+    }
+    return WF_OK;
+  });
+}
+
+int wf_prepare(void *ctx, int32_t scheme) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (scheme != WF_LEAPFROG && scheme != WF_KICKDRIFT) throw Error(WF_EINVAL, "unknown scheme");
+    refresh_bc(c);
+    if (!c->cfg.no_graphs)
+      for (int cur = 0; cur < 2; ++cur) step_graph(c, scheme, cur);
     return WF_OK;
   });
 }
@@ -1506,6 +1611,7 @@ int wf_get_scalars(void *ctx, wf_scalars *out) {
       out->ext_forces_work = s[S_EXTW];
       out->contact_friction_work = s[S_FRW];
       out->accum_cont_heat_cond = s[S_CHC];
+      out->min_force_ts = f.min_force_ts;
       auto cc = dget(c, c->d.ccnt, (size_t)c->npad);
       int64_t tot = 0;
       for (int64_t k = 0; k < c->n; ++k) tot += cc[k];

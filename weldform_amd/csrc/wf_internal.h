@@ -34,7 +34,6 @@ constexpr int kBlock = 128;
 // blocks and five density blocks fit a CU's 160 KB of LDS.
 constexpr int kStageMax = 896;
 constexpr int kStagePow2 = 1024;  // the power of two >= kStageMax the set sort pads to
-constexpr int kSortCap = 8192;   // neighbour entries a tile may list in total
 
 // neighbour rows are read in batches of kNbBatch entries, the next batch
 // prefetched while the current one is processed
@@ -193,7 +192,15 @@ struct Dev {
   StressRecF *srecf;
   float *prf;                  // P / rho^2 (EOS of the current density)
   // neighbours
-  int *cell, *cell_count, *cell_start, *cell_cursor, *sorted, *scan_tmp;
+  int *cell, *cell_count, *cell_start, *cell_cursor, *sorted;
+  // cell-ordered candidate records of the last counting sort (position p of
+  // `sorted`): offsets of the particle from its cell's lower corner and h in
+  // FP32, and (particle index, group)
+  float4 *sxyz;
+  int2 *sinfo;
+  // rigid (contact) particles binned in the same cells: counts, starts,
+  // cursors [ncells + 1], cell of each rigid particle and the cell order
+  int *rcell, *rcount, *rstart, *rcursor, *rsorted;
   int *nnb, *nbr;
   // LDS staging (mixed precision): per tile the sorted staged particle list
   // stage[t][0..stage_cnt[t]) (stage_cnt = -1: tile gathers from HBM), each
@@ -286,6 +293,12 @@ void launch_tir(const Dev &d, hipStream_t st, int cur);
 // dt < 0 in a launcher: the kernel reads the step from the device flags
 // (adaptive time step).  phase 0: the whole kick; 1: BCs and the first-step
 // half kick; 2: the kick with (dt_prev, dt) and the BCs.
+// kick phases: K_WHOLE the whole kick; K_FIRST BCs and the first-step half
+// kick; K_SECOND the kick with (dt_prev, dt) and the BCs; K_OBSERVE only the
+// reaction observer.  K_WHOLE and K_FIRST observe the reaction too (before
+// the BCs zero a); the _NOOBS forms do not (it was observed before a host
+// hook ran).
+enum { K_WHOLE = 0, K_FIRST, K_SECOND, K_OBSERVE, K_WHOLE_NOOBS, K_FIRST_NOOBS };
 void launch_kick(const Dev &d, hipStream_t st, const BcTable &bc, int scheme, double dt_prev, double dt, int phase);
 void launch_time_step(const Dev &d, hipStream_t st, int cur);
 void launch_kick2(const Dev &d, hipStream_t st, double dt);

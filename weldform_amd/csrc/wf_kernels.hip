@@ -108,21 +108,34 @@ WF_KERNEL void k_schedule(Dev d, int force) {
 }
 
 // ---------------------------------------------------------------------------
-// neighbour build: counting sort of particles by reference cell, then one
-// lane per particle scans the 27 surrounding cells with the exact predicate.
+// neighbour build: counting sort of the particles (and of the rigid contact
+// particles) by reference cell, then one lane per particle scans the 27
+// surrounding cells with the exact predicate.  A search step runs
+// k_cell_hist, k_scan, k_scatter, k_cell_sort, k_build_nbr and, with LDS
+// staging, k_stage; on other steps each of them returns at once.
 
-// cell_count and cell_cursor are zero on entry (allocation, then cleared by
-// k_cell_sort once a search has consumed them)
+__device__ inline int cell_of(const Dev &d, const double4 x) {
+  const int cx = cell_coord(x.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
+  const int cy = cell_coord(x.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
+  const int cz = cell_coord(x.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
+  return (cx * d.cellno[1] + cy) * d.cellno[2] + cz;
+}
+
+// threads 0..n-1: the SPH particles; n..n+nrig-1: the rigid particles.
+// The counts and cursors are zero on entry (allocation, then cleared by
+// k_cell_sort once a search has consumed them).
 WF_KERNEL void k_cell_hist(Dev d, int cur) {
   if (!d.flags->do_search) return;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
-    const double4 x = d.xh[cur][i];
-    const int cx = cell_coord(x.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
-    const int cy = cell_coord(x.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
-    const int cz = cell_coord(x.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
-    const int c = (cx * d.cellno[1] + cy) * d.cellno[2] + cz;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < d.n) {
+    const int c = cell_of(d, d.xh[cur][i]);
     d.cell[i] = c;
     atomicAdd(&d.cell_count[c], 1);
+  } else if (i - d.n < d.nrig) {
+    const int r = i - d.n;
+    const int c = cell_of(d, d.rx[r]);
+    d.rcell[r] = c;
+    atomicAdd(&d.rcount[c], 1);
   }
 }
 
@@ -136,7 +149,8 @@ __device__ inline int wave_scan_incl(int v) {
   return v;
 }
 
-// block-wide exclusive scan of 1024 values; returns the block total in *tot
+// block-wide exclusive scan (blockDim.x <= 1024); returns the block total
+// in *tot
 __device__ inline int block_scan_excl(int v, int *sh, int *tot) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int inc = wave_scan_incl(v);
@@ -155,54 +169,74 @@ __device__ inline int block_scan_excl(int v, int *sh, int *tot) {
   return r;
 }
 
-__global__ __launch_bounds__(1024) void k_scan1(Dev d) {
-  if (!d.flags->do_search) return;
-  __shared__ int sh[33];
-  const int i = blockIdx.x * 1024 + threadIdx.x;
-  const int v = i < d.ncells ? d.cell_count[i] : 0;
-  int tot;
-  const int e = block_scan_excl(v, sh, &tot);
-  if (i < d.ncells) d.cell_start[i] = e;
-  if (threadIdx.x == 0) d.scan_tmp[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(1024) void k_scan2(Dev d, int nb) {
-  if (!d.flags->do_search) return;
-  __shared__ int sh[33];
+// exclusive scan of cnt[0..ncells) into start[0..ncells] by one block of
+// 1024 threads: rounds of K consecutive cells per thread, the next round's
+// counts loaded while the current round is scanned
+__device__ inline void scan_cells(const int *cnt, int *start, int ncells, int *sh) {
+  constexpr int K = 16;
+  constexpr int R = K * 1024;
+  int v[K], nv[K];
+  const int t0 = (int)threadIdx.x * K;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = t0 + k < ncells ? cnt[t0 + k] : 0;
   int carry = 0;
-  for (int base = 0; base < nb; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int v = i < nb ? d.scan_tmp[i] : 0;
+  for (int base = 0; base < ncells; base += R) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int c = base + R + t0 + k;
+      nv[k] = c < ncells ? cnt[c] : 0;
+    }
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) s += v[k];
     int tot;
-    const int e = block_scan_excl(v, sh, &tot);
-    if (i < nb) d.scan_tmp[i] = e + carry;
+    int off = block_scan_excl(s, sh, &tot) + carry;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int c = base + t0 + k;
+      if (c < ncells) start[c] = off;
+      off += v[k];
+      v[k] = nv[k];
+    }
     carry += tot;
   }
-  if (threadIdx.x == 0) d.cell_start[d.ncells] = carry;
+  if (threadIdx.x == 0) start[ncells] = carry;
 }
 
-__global__ __launch_bounds__(1024) void k_scan3(Dev d) {
+__global__ __launch_bounds__(1024) void k_scan(Dev d) {
   if (!d.flags->do_search) return;
-  const int i = blockIdx.x * 1024 + threadIdx.x;
-  if (i < d.ncells) d.cell_start[i] += d.scan_tmp[blockIdx.x];
+  __shared__ int sh[34];
+  scan_cells(d.cell_count, d.cell_start, d.ncells, sh);
+  if (d.nrig) scan_cells(d.rcount, d.rstart, d.ncells, sh);
 }
 
 WF_KERNEL void k_scatter(Dev d) {
   if (!d.flags->do_search) return;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < d.n) {
     const int c = d.cell[i];
-    const int pos = d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1);
-    d.sorted[pos] = i;
+    d.sorted[d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1)] = i;
+  } else if (i - d.n < d.nrig) {
+    const int r = i - d.n, c = d.rcell[r];
+    d.rsorted[d.rstart[c] + atomicAdd(&d.rcursor[c], 1)] = r;
   }
 }
 
-// cells hold ~10-30 particles: an insertion sort makes the within-cell
-// order (and with it every neighbour row) independent of atomic order
-WF_KERNEL void k_cell_sort(Dev d) {
+// per cell: clear the counters for the next search, sort the cell's
+// particles by index (cells hold ~10-30 particles; the order, and with it
+// every neighbour row, becomes independent of the atomic order) and write
+// their candidate records (FP32 offsets from the cell's lower corner, h,
+// index, group) in cell order
+WF_KERNEL void k_cell_sort(Dev d, int cur) {
   if (!d.flags->do_search) return;
-  for (int c = blockIdx.x * kBlock + threadIdx.x; c < d.ncells; c += gridDim.x * kBlock) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= d.ncells) return;
   d.cell_count[c] = 0;
   d.cell_cursor[c] = 0;
+  if (d.nrig) {
+    d.rcount[c] = 0;
+    d.rcursor[c] = 0;
+  }
   const int s = d.cell_start[c], e = d.cell_start[c + 1];
   for (int a = s + 1; a < e; ++a) {
     const int v = d.sorted[a];
@@ -213,147 +247,142 @@ WF_KERNEL void k_cell_sort(Dev d) {
     }
     d.sorted[b + 1] = v;
   }
+  const int nz = d.cellno[2], ny = d.cellno[1];
+  const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
+  const double ox = d.blpf[0] + cx * d.cellsize[0], oy = d.blpf[1] + cy * d.cellsize[1],
+               oz = d.blpf[2] + cz * d.cellsize[2];
+  for (int p = s; p < e; ++p) {
+    const int j = d.sorted[p];
+    const double4 x = d.xh[cur][j];
+    d.sxyz[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
+    d.sinfo[p] = make_int2(j, d.group[j]);
   }
 }
 
+// One lane per particle over the 27 cells around its own, in cell order
+// (each cell's particles by index), so a row lists its neighbours in a fixed
+// order.  The radius test runs in FP32 on offsets from the cell corners
+// (|error| < 1e-6 relative at the radius) and is trusted outside a band of
+// 1e-4 around it; inside the band, and for a particle lying far outside the
+// grid (clamped into a boundary cell, where its FP32 offset loses digits),
+// the exact FP64 predicate CheckRadius decides.
 WF_KERNEL void k_build_nbr(Dev d, int cur) {
   if (!d.flags->do_search) return;
   __shared__ double sh[kBlock / 64];
-  double tot = 0.0, mx = 0.0;
-  for (int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
-    int cnt = 0;
+  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  int cnt = 0;
+  if (i < d.n && d.group[i] >= 0) {
     const int g = d.group[i];
-    if (g >= 0) {
-      const double4 xi = d.xh[cur][i];
-      const int c = d.cell[i];
-      const int nz = d.cellno[2], ny = d.cellno[1], nx = d.cellno[0];
-      const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
-      for (int ax = cx - 1; ax <= cx + 1; ++ax) {
-        if (ax < 0 || ax >= nx) continue;
-        for (int ay = cy - 1; ay <= cy + 1; ++ay) {
-          if (ay < 0 || ay >= ny) continue;
-          // cells (ax, ay, cz-1..cz+1) are consecutive in the cell order
-          const int c0 = (ax * ny + ay) * nz;
-          const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < nz ? cz + 1 : nz - 1;
-          const int s0 = d.cell_start[c0 + zlo], e0 = d.cell_start[c0 + zhi + 1];
+    const double4 xi = d.xh[cur][i];
+    const int c = d.cell[i];
+    const int nz = d.cellno[2], ny = d.cellno[1], nx = d.cellno[0];
+    const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
+    const float csx = (float)d.cellsize[0], csy = (float)d.cellsize[1], csz = (float)d.cellsize[2];
+    const float oxi = (float)(xi.x - (d.blpf[0] + cx * d.cellsize[0]));
+    const float oyi = (float)(xi.y - (d.blpf[1] + cy * d.cellsize[1]));
+    const float ozi = (float)(xi.z - (d.blpf[2] + cz * d.cellsize[2]));
+    const float hi = (float)xi.w, hcf = 0.5f * (float)d.cellfac;
+    const bool exact_all = fabsf(oxi) > 64.f * csx || fabsf(oyi) > 64.f * csy || fabsf(ozi) > 64.f * csz;
+    const long long np = d.npad;
+    for (int ax = cx - 1; ax <= cx + 1; ++ax) {
+      if (ax < 0 || ax >= nx) continue;
+      const float bx = (float)(cx - ax) * csx + oxi;
+      for (int ay = cy - 1; ay <= cy + 1; ++ay) {
+        if (ay < 0 || ay >= ny) continue;
+        const float by = (float)(cy - ay) * csy + oyi;
+        for (int az = cz - 1; az <= cz + 1; ++az) {
+          if (az < 0 || az >= nz) continue;
+          const float bz = (float)(cz - az) * csz + ozi;
+          const int cc = (ax * ny + ay) * nz + az;
+          const int s0 = d.cell_start[cc], e0 = d.cell_start[cc + 1];
           for (int p = s0; p < e0; ++p) {
-            const int j = d.sorted[p];
-            if (j == i || d.group[j] != g) continue;
-            if (!in_radius(xi, d.xh[cur][j], d.cellfac)) continue;
-            if (cnt < d.max_nb) d.nbr[nb_slot(d.npad, cnt, i)] = j;
-            ++cnt;
+            const int2 inf = d.sinfo[p];
+            if (inf.x == i || inf.y != g) continue;
+            const float4 q = d.sxyz[p];
+            const float dx = bx - q.x, dy = by - q.y, dz = bz - q.z;
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float t = hcf * (hi + q.w);
+            const float t2 = t * t;
+            bool in = r2 < t2;
+            if (exact_all || fabsf(r2 - t2) <= 1e-4f * t2) in = in_radius(xi, d.xh[cur][inf.x], d.cellfac);
+            if (in) {
+              if (cnt < d.max_nb) d.nbr[nb_slot(np, cnt, i)] = inf.x;
+              ++cnt;
+            }
           }
         }
       }
     }
+  }
+  if (i < d.n) {
     d.nnb[i] = cnt < d.max_nb ? cnt : d.max_nb;
     if (cnt > d.max_nb) d.flags->overflow = 1;
-    tot += cnt;
-    mx = cnt > mx ? cnt : mx;
   }
   // deterministic totals via block reduction + integer atomics
-  const double bt = block_reduce<0>(tot, sh);
-  const double bm = block_reduce<1>(mx, sh);
+  const double bt = block_reduce<0>((double)cnt, sh);
+  const double bm = block_reduce<1>((double)cnt, sh);
   if (threadIdx.x == 0) {
     atomicAdd((unsigned long long *)&d.flags->directed_pairs, (unsigned long long)bt);
     atomicMax(&d.flags->max_count, (int)bm);
   }
 }
 
-// Per tile of kBlock consecutive particles, after every neighbour build:
-//  1. each particle's row is sorted by neighbour index (in LDS), so lanes of
-//     a wavefront visit their k-th neighbours at nearby addresses;
-//  2. (mixed precision, stage = 1) the LDS staging list: the tile's
-//     particles and all their neighbour entries go into an LDS hash set
-//     (open addressing, kHashSlots slots), whose distinct values are then
-//     sorted (bitonic, in LDS), giving the tile's staged set in index order,
-//     which a pass loads in long contiguous runs; each entry is then
-//     rewritten as its position in the set (binary search).  The set holds
-//     ~6 % of the entries (each neighbour appears in many rows), so hashing
-//     first sorts ~730 values instead of ~7000.  The hash table's layout
-//     depends on insertion order, the sorted set does not.  Tiles whose set
-//     exceeds stage_max keep stage_cnt = -1 and gather from HBM.
-// Tiles whose entries exceed kSortCap keep their rows as built and are not
-// staged.  Step 1 does not depend on `stage`, so staged and HBM gathers sum
-// in the same order.
-__device__ inline int lower_pos(const int *a, int n, int v) {
-  int lo = 0, hi = n;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (a[mid] < v)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
+// LDS staging lists of the mixed-precision pair passes, per tile of kBlock
+// consecutive particles after every neighbour build: the tile's particles
+// and all their neighbour entries go into an LDS hash set (open
+// addressing); its distinct values (the staged set, ~6 % of the ~7000
+// entries: each neighbour appears in many rows) are sorted (bitonic, in
+// LDS) so a pass loads the set in long contiguous runs; each value's hash
+// slot then records its position in the set, and every row entry is
+// rewritten as that position.  The hash table's layout depends on insertion
+// order, the sorted set does not.  Tiles whose set exceeds stage_max keep
+// stage_cnt = -1 and gather from HBM.  Rows keep their order, so staged and
+// HBM gathers sum in the same order.
+constexpr int kStageThreads = 256;
 constexpr int kHashSlots = 2048;  // >= 2 kStageMax: load <= 1/2 for sets of <= kStageMax values
 static_assert(kHashSlots >= 2 * kStageMax && (kHashSlots & (kHashSlots - 1)) == 0, "hash table size");
 static_assert(kStagePow2 >= kStageMax && (kStagePow2 & (kStagePow2 - 1)) == 0, "sort padding");
+static_assert(kStageThreads == 2 * kBlock, "two threads per particle of a tile");
 __device__ inline unsigned stage_hash(int v) { return ((unsigned)v * 2654435761u) >> 21; }  // 11 bits
 
-WF_KERNEL void k_stage(Dev d, int stage) {
+__global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
   if (!d.flags->do_search) return;
-  __shared__ int buf[kSortCap];
   __shared__ int uniq[kStagePow2];
   __shared__ int ht[kHashSlots];
+  __shared__ unsigned short hp[kHashSlots];
   __shared__ int nset, full;
   __shared__ int sh[34];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, li = tid % kBlock, half = tid / kBlock;
   const long long np = d.npad;
+  constexpr unsigned mask = kHashSlots - 1;
   for (int t = blockIdx.x; t < d.ntiles; t += gridDim.x) {
-    const int i = t * kBlock + tid;
+    const int i = t * kBlock + li;
     const int c = (i < d.n && d.group[i] >= 0) ? d.nnb[i] : 0;
-    int tot;
-    const int off = block_scan_excl(c + (i < d.n ? 1 : 0), sh, &tot);
-    if (tot > kSortCap) {
-      if (tid == 0 && stage) d.stage_cnt[t] = -1;
-      continue;
-    }
-    // 1. sorted rows
-    if (i < d.n) {
-      int *row = buf + off + 1;
-      buf[off] = i;
-      for (int k = 0; k < c; ++k) row[k] = d.nbr[nb_slot(np, k, i)];
-      for (int a = 1; a < c; ++a) {
-        const int v = row[a];
-        int b = a - 1;
-        while (b >= 0 && row[b] > v) {
-          row[b + 1] = row[b];
-          --b;
-        }
-        row[b + 1] = v;
-      }
-      for (int k = 0; k < c; ++k) d.nbr[nb_slot(np, k, i)] = row[k];
-    }
-    if (!stage) {
-      __syncthreads();
-      continue;
-    }
-    // 2. staged set: distinct values through the LDS hash set.  A value is
-    // counted by the thread whose CAS claims its slot; probing stops once
-    // the set has more than stage_max values (the tile then gathers from
-    // HBM), so every probe sequence ends within kHashSlots slots.
-    for (int p = tid; p < kHashSlots; p += kBlock) ht[p] = -1;
+    for (int p = tid; p < kHashSlots; p += kStageThreads) ht[p] = -1;
     if (tid == 0) {
       nset = 0;
       full = 0;
     }
     __syncthreads();
-    for (int p = tid; p < tot; p += kBlock) {
-      const int v = buf[p];
-      unsigned s = stage_hash(v) & (kHashSlots - 1);
+    // a value is counted by the thread whose CAS claims its slot; probing
+    // stops once the set has more than stage_max values (the tile then
+    // gathers from HBM), and every probe sequence ends within kHashSlots
+    auto insert = [&](int v) {
+      unsigned s = stage_hash(v) & mask;
       for (int probe = 0; probe < kHashSlots; ++probe) {
-        if (full) break;
+        if (*(volatile int *)&full) break;
         const int old = atomicCAS(&ht[s], -1, v);
         if (old == -1) {
           if (atomicAdd(&nset, 1) >= d.stage_max) full = 1;
           break;
         }
         if (old == v) break;
-        s = (s + 1) & (kHashSlots - 1);
+        s = (s + 1) & mask;
       }
+    };
+    if (i < d.n) {
+      if (half == 0) insert(i);
+      for (int k = half; k < c; k += 2) insert(d.nbr[nb_slot(np, k, i)]);
     }
     __syncthreads();
     if (full || nset > d.stage_max) {
@@ -363,7 +392,7 @@ WF_KERNEL void k_stage(Dev d, int stage) {
     }
     // compact the table (slot order), pad to a power of two, sort
     int carry = 0;
-    for (int base = 0; base < kHashSlots; base += kBlock) {
+    for (int base = 0; base < kHashSlots; base += kStageThreads) {
       const int v = ht[base + tid];
       const int f = v >= 0 ? 1 : 0;
       int sub;
@@ -374,11 +403,11 @@ WF_KERNEL void k_stage(Dev d, int stage) {
     const int U = carry;
     int P = 1;
     while (P < U) P <<= 1;
-    for (int p = U + tid; p < P; p += kBlock) uniq[p] = 0x7fffffff;
+    for (int p = U + tid; p < P; p += kStageThreads) uniq[p] = 0x7fffffff;
     __syncthreads();
     for (int k = 2; k <= P; k <<= 1) {
       for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int p = tid; p < P; p += kBlock) {
+        for (int p = tid; p < P; p += kStageThreads) {
           const int q = p ^ j;
           if (q > p) {
             const int a = uniq[p], b = uniq[q];
@@ -391,11 +420,23 @@ WF_KERNEL void k_stage(Dev d, int stage) {
         __syncthreads();
       }
     }
-    for (int p = tid; p < U; p += kBlock) d.stage[(long long)t * kStageMax + p] = uniq[p];
+    // each value's slot records its position in the sorted set
+    for (int p = tid; p < U; p += kStageThreads) {
+      const int v = uniq[p];
+      unsigned s = stage_hash(v) & mask;
+      while (ht[s] != v) s = (s + 1) & mask;
+      hp[s] = (unsigned short)p;
+      d.stage[(long long)t * kStageMax + p] = v;
+    }
+    __syncthreads();
+    auto pos = [&](int v) -> unsigned short {
+      unsigned s = stage_hash(v) & mask;
+      while (ht[s] != v) s = (s + 1) & mask;
+      return hp[s];
+    };
     if (i < d.n) {
-      d.selfl[i] = (uint16_t)lower_pos(uniq, U, i);
-      for (int k = 0; k < c; ++k)
-        d.nbrl[nb_slot(np, k, i)] = (uint16_t)lower_pos(uniq, U, d.nbr[nb_slot(np, k, i)]);
+      if (half == 0) d.selfl[i] = pos(i);
+      for (int k = half; k < c; k += 2) d.nbrl[nb_slot(np, k, i)] = pos(d.nbr[nb_slot(np, k, i)]);
     }
     if (tid == 0) d.stage_cnt[t] = U;
     __syncthreads();
@@ -455,15 +496,12 @@ void launch_schedule(const Dev &d, hipStream_t st, int force) {
 }
 
 void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
-  k_cell_hist<<<nblk_small(d.n), kBlock, 0, st>>>(d, cur);
-  const int ns = (d.ncells + 1023) / 1024;
-  k_scan1<<<ns, 1024, 0, st>>>(d);
-  k_scan2<<<1, 1024, 0, st>>>(d, ns);
-  k_scan3<<<ns, 1024, 0, st>>>(d);
-  k_scatter<<<nblk_small(d.n), kBlock, 0, st>>>(d);
-  k_cell_sort<<<nblk_small(d.ncells), kBlock, 0, st>>>(d);
-  k_build_nbr<<<nblk_small(d.n) * 2, kBlock, 0, st>>>(d, cur);
-  k_stage<<<d.ntiles < 2048 ? d.ntiles : 2048, kBlock, 0, st>>>(d, d.stage_max > 0 ? 1 : 0);
+  k_cell_hist<<<nblk((long long)d.n + d.nrig), kBlock, 0, st>>>(d, cur);
+  k_scan<<<1, 1024, 0, st>>>(d);
+  k_scatter<<<nblk((long long)d.n + d.nrig), kBlock, 0, st>>>(d);
+  k_cell_sort<<<nblk(d.ncells), kBlock, 0, st>>>(d, cur);
+  k_build_nbr<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
+  if (d.stage_max > 0) k_stage<<<d.ntiles < 2048 ? d.ntiles : 2048, kStageThreads, 0, st>>>(d);
 }
 
 // ---------------------------------------------------------------------------
@@ -857,10 +895,12 @@ void launch_accel(const Dev &d, hipStream_t st, int cur) {
 //   KickDrift (Source/SolverKickDrift.cpp:208-231): v += a*0.5*dt_prev
 
 
-WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt, int phase) {
+WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt, int kphase) {
   __shared__ double sh[kBlock / 64];
   __shared__ double bcv[kMaxBc][3];
   __shared__ int bcon[kMaxBc];
+  const bool observe = kphase == K_WHOLE || kphase == K_FIRST || kphase == K_OBSERVE;
+  const int phase = kphase == K_WHOLE_NOOBS ? K_WHOLE : (kphase == K_FIRST_NOOBS ? K_FIRST : kphase);
   const int isfirst = d.flags->step == 0;
   if (dt < 0) {
     dt = d.flags->dt;
@@ -888,6 +928,9 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt, 
     const double4 vm = d.vm[i];
     m = vm.w;
     id = d.id[i];
+  }
+  if (i < d.n && phase != K_OBSERVE) {
+    const double4 vm = d.vm[i];
     double v[3] = {vm.x, vm.y, vm.z};
     double av[3] = {a.x, a.y, a.z};
     if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) d.flags->nan = 1;
@@ -906,11 +949,11 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt, 
     }
     if (!fixed) {
       if (scheme == WF_LEAPFROG) {
-        if (isfirst && phase != 2)
+        if (isfirst && phase != K_SECOND)
           for (int k = 0; k < 3; ++k) v[k] -= av[k] * 0.5 * dt;
-        if (phase != 1)
+        if (phase != K_FIRST)
           for (int k = 0; k < 3; ++k) v[k] += av[k] * 0.5 * (dt_prev + dt);
-      } else if (phase != 1) {
+      } else if (phase != K_FIRST) {
         for (int k = 0; k < 3; ++k) v[k] += av[k] * 0.5 * dt_prev;
       }
     }
@@ -918,7 +961,7 @@ WF_KERNEL void k_kick(Dev d, BcTable bc, int scheme, double dt_prev, double dt, 
     if (d.mixed) d.vmf[i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)vm.w);
     d.acc[i] = make_double4(av[0], av[1], av[2], 0.0);
   }
-  for (int z = 0; z < bc.nz && phase != 2; ++z) {
+  for (int z = 0; z < bc.nz && observe; ++z) {
     const bool in = (id == bc.rzone[z]);
     const double fx = block_reduce<0>(in ? m * a.x : 0.0, sh);
     const double fy = block_reduce<0>(in ? m * a.y : 0.0, sh);
@@ -1601,7 +1644,9 @@ void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
 WF_KERNEL void k_surface(Dev d, int cur, int setup) {
   if (!d.flags->do_search) return;
   const long long np = d.npad;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < d.n; i += gridDim.x * kBlock) {
+  {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= d.n) return;
     const int old_surf = d.id[i] == 1;
     const double4 xi = d.xh[cur][i];
     double nx = 0.0, ny = 0.0, nz = 0.0;
@@ -1622,19 +1667,34 @@ WF_KERNEL void k_surface(Dev d, int cur, int setup) {
     d.id[i] = surf ? 1 : d.id_orig[i];
     int c = 0;
     if (!setup && old_surf && surf) {
-      const int cx = cell_coord(xi.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
-      const int cy = cell_coord(xi.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
-      const int cz = cell_coord(xi.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
-      for (int r = 0; r < d.nrig; ++r) {
-        const double4 xr = d.rx[r];
-        const int rx = cell_coord(xr.x, d.blpf[0], d.cellsize[0], d.cellno[0]);
-        const int ry = cell_coord(xr.y, d.blpf[1], d.cellsize[1], d.cellno[1]);
-        const int rz = cell_coord(xr.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
-        if (abs(rx - cx) > 1 || abs(ry - cy) > 1 || abs(rz - cz) > 1) continue;
-        const double dx = xi.x - xr.x, dy = xi.y - xr.y, dz = xi.z - xr.z;
-        if (sqrt(dx * dx + dy * dy + dz * dz) < xi.w + xr.w) {
-          if (c < kMaxContact) d.cidx[(long long)c * np + i] = r;
-          ++c;
+      // ContactNbSearch: the rigid particles of the 27 cells around i's own
+      // (binned by k_cell_hist / k_scatter at this search), |x_ij| < h_i +
+      // h_r; partners are kept in rigid-particle index order
+      const int ci = d.cell[i];
+      const int gz = d.cellno[2], gy = d.cellno[1], gx = d.cellno[0];
+      const int cz = ci % gz, cy = (ci / gz) % gy, cx = ci / (gz * gy);
+      for (int ax = cx - 1; ax <= cx + 1; ++ax) {
+        if (ax < 0 || ax >= gx) continue;
+        for (int ay = cy - 1; ay <= cy + 1; ++ay) {
+          if (ay < 0 || ay >= gy) continue;
+          const int c0 = (ax * gy + ay) * gz;
+          const int zlo = cz > 0 ? cz - 1 : 0, zhi = cz + 1 < gz ? cz + 1 : gz - 1;
+          for (int p = d.rstart[c0 + zlo]; p < d.rstart[c0 + zhi + 1]; ++p) {
+            const int r = d.rsorted[p];
+            const double4 xr = d.rx[r];
+            const double dx = xi.x - xr.x, dy = xi.y - xr.y, dz = xi.z - xr.z;
+            if (sqrt(dx * dx + dy * dy + dz * dz) < xi.w + xr.w) {
+              if (c < kMaxContact) {
+                int q = c;
+                while (q > 0 && d.cidx[(long long)(q - 1) * np + i] > r) {
+                  d.cidx[(long long)q * np + i] = d.cidx[(long long)(q - 1) * np + i];
+                  --q;
+                }
+                d.cidx[(long long)q * np + i] = r;
+              }
+              ++c;
+            }
+          }
         }
       }
       if (c > kMaxContact) d.flags->overflow = 1;
@@ -1734,9 +1794,10 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
           if (dt_fext < 1000. && dt_fext < fts) fts = dt_fext;
         }
         for (int k = 0; k < 3; ++k) a[k] += cf[k] / m1;
-        // Fraser 3-119: the contact area of the particle
+        // Fraser 3-119: the contact area of the particle, formed whenever a
+        // contact heat branch (the only readers, which divide by it) is on
         const double dens = d.rho[cur][i];
-        const double dS2 = d.qfric ? pow(m1 / dens, 2.0 / 3.0) : 0.0;
+        const double dS2 = (d.heat_cond || d.heat_fric) ? pow(m1 / dens, 2.0 / 3.0) : 0.0;
         if (d.fr_sta > 0.) {
           double du[3], tg[3];
           for (int k = 0; k < 3; ++k) du[k] = xp[k] - x[k] - v2[k] * dt;
@@ -1858,7 +1919,7 @@ __global__ __launch_bounds__(1024) void k_mesh_update(Dev d, double dt) {
 }
 
 void launch_surface(const Dev &d, hipStream_t st, int cur, int setup) {
-  k_surface<<<nblk_small(d.n), kBlock, 0, st>>>(d, cur, setup);
+  k_surface<<<nblk(d.n), kBlock, 0, st>>>(d, cur, setup);
 }
 void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt) {
   k_rigid_bc<<<(d.nrig + kBlock - 1) / kBlock, kBlock, 0, st>>>(d, bc);
