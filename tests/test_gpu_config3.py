@@ -4,13 +4,15 @@ one run (Source/SolverLeapfrog.cpp:215-330, Source/Contact_Wang.cpp:15-348,
 Source/Thermal.cpp:53-219,721-748, Source/ThermalStuct.cpp:14-39).
 
   * Against the oracle (the steel billet of contact_small with the thermal
-    solver and plastic heating on, mu = 0.2, 200 steps): the first 50 steps
-    (contact under way from step ~45) in FP64 within 1e-8 normwise for the
-    temperature rise, displacement, plastic strain and both plane forces;
-    after 200 steps every field within 10x the oracle's own sensitivity to
-    a 1e-15 relative change of its initial positions (1e-7 for the mixed
-    path, floored at 1e-4), because the penalty contact is chaotic once
-    particles touch (see test_gpu_contact.py).
+    solver and plastic heating on, mu = 0.2, the punch at -20 m/s so the
+    billet touches the punch at step 13 and yields under it from step 19,
+    200 steps): after 22 steps (contact, plastic strain, plastic heat and
+    conduction all active) and after 200 steps the temperature rise, displacement, plastic strain and
+    both plane forces within 1e-8 normwise in FP64 (1e-4 mixed), or within
+    10x the oracle's own sensitivity to a 1e-15 relative change of its
+    initial positions (1e-7 for the mixed path) where that is larger,
+    because the penalty contact is chaotic once particles touch (see
+    test_gpu_contact.py).
   * Energy over long runs, where the contact phase cannot be compared
     step by step: the heat the billet gains in a step, sum m c_p (T' - T),
     equals the previous step's increments of 0.9 x the plastic work plus
@@ -29,8 +31,9 @@ from weldform_amd import capi, cases
 
 pytestmark = pytest.mark.gpu
 
-EARLY = 50
+EARLY = 22
 HC = 5.0e4
+V_PUNCH = -20.0
 
 
 def _oracle(c, eps=0.0):
@@ -58,7 +61,7 @@ def _oracle(c, eps=0.0):
 
 @pytest.fixture(scope="module")
 def config3_small():
-    c = cases.contact_small(steps=200, thermal=True)
+    c = cases.contact_small(steps=200, thermal=True, v_punch=V_PUNCH)
     c.name = "cyl_compress_4k_contact_thermal"
     d, ref, early = _oracle(c)
     assert c.pl_heating and c.thermal and c.contact[0] == 0.2
@@ -67,10 +70,8 @@ def config3_small():
     assert np.abs(ref["F"][:, 0, 2]).max() > 0 and np.abs(ref["F"][:, 1, 2]).max() > 0
     sens = {}
     for eps in (1e-15, 1e-7):
-        _, per, _ = _oracle(c, eps)
-        sens[eps] = {k: normwise_err(per[k], ref[k]) for k in ("heat", "disp", "pl")}
-        for m, name in ((0, "F_top"), (1, "F_bottom")):
-            sens[eps][name] = normwise_err(per["F"][:, m, 2], ref["F"][:, m, 2])
+        _, per, per_early = _oracle(c, eps)
+        sens[eps] = (_errs(per, ref), _errs(per_early, early))
     return c, ref, early, sens
 
 
@@ -91,26 +92,26 @@ def test_config3_contact_conduction_plastic_heat_matches_oracle(config3_small, p
     c, ref, early, sens_all = config3_small
     fp64 = precision == capi.FP64
     print(f"config3 small: n={c.n} steps={c.steps} precision={precision}", flush=True)
-    sens = sens_all[1e-15 if fp64 else 1e-7]
+    sens, sens_early = sens_all[1e-15 if fp64 else 1e-7]
+    floor = 1e-8 if fp64 else 1e-4
     g = cases.make_solver(c, precision=precision)
     g.step(EARLY)
     g.sync()
     e_early = _errs(_gpu_fields(g), early)
-    print("config3 early errors", e_early, flush=True)
-    if fp64:
-        for k, e in e_early.items():
-            assert e < 1e-8, k
+    print("config3 early errors", e_early, "oracle sensitivity", sens_early, flush=True)
+    for k, e in e_early.items():
+        assert e < max(floor, 10 * sens_early[k]), k
     g.step(c.steps - EARLY)
     g.sync()
     errs = _errs(_gpu_fields(g), ref)
     print("config3 errors", errs, "oracle sensitivity", sens, flush=True)
     for k, e in errs.items():
-        assert e < max(1e-8 if fp64 else 1e-4, 10 * sens[k]), k
+        assert e < max(floor, 10 * sens[k]), k
     g.close()
 
 
 def _heat_case(steps):
-    c = cases.contact_small(steps=steps, thermal=True)
+    c = cases.contact_small(steps=steps, thermal=True, v_punch=V_PUNCH)
     c.contact_heat = (True, True, HC)
     nr = 2 * 22 ** 2
     c.rigid_T = np.concatenate([np.full(nr, 400.0), np.full(nr, 300.0)])
