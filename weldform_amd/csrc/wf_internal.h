@@ -44,10 +44,14 @@ constexpr int kLanesFP64 = 2;
 __host__ __device__ inline long long nb_slot(long long npad, int k, long long i) {
   return ((long long)(k >> 2) * npad + i) * 4 + (k & 3);
 }
-// entries allocated per particle: the prefetch after a lane's last batch of
-// a full row (max_nb entries) stays inside the allocation
+// spare entries per particle past a full row (max_nb entries) that the
+// pair passes' row prefetch may touch: a batch of kNbBatch per lane, or
+// three groups of four per lane
+constexpr int kRowSpare = 16 * kMaxLanes;
+static_assert(kRowSpare >= kNbBatch * kMaxLanes, "row spare covers the batch prefetch");
+// entries allocated per particle
 __host__ __device__ inline int nb_rows(int max_nb) {
-  return (max_nb + kNbBatch - 1) / kNbBatch * kNbBatch + kNbBatch * kMaxLanes;
+  return (max_nb + kNbBatch - 1) / kNbBatch * kNbBatch + kRowSpare;
 }
 constexpr int kMaxBc = 8;        // velocity / temperature BC slots per context
 constexpr int kMaxZones = 8;     // velocity-BC zones with a reaction history

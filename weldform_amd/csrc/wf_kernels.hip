@@ -9,6 +9,7 @@
 //
 // Global reductions (reaction, energies, plastic work) are two-stage:
 // fixed-order block partials, then one single-block finalize kernel.
+#include <float.h>
 #include <math.h>
 
 #include "wf_device.h"
@@ -315,8 +316,12 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
     }
   }
   if (i < d.n) {
-    d.nnb[i] = cnt < d.max_nb ? cnt : d.max_nb;
+    const int kept = cnt < d.max_nb ? cnt : d.max_nb;
+    d.nnb[i] = kept;
     if (cnt > d.max_nb) d.flags->overflow = 1;
+    // the pair passes read rows in groups of four: the rest of the last
+    // group holds the particle itself (a pair term of exact zeros)
+    for (int k = kept; k < ((kept + 3) & ~3); ++k) d.nbr[nb_slot(d.npad, k, i)] = i;
   }
   // deterministic totals via block reduction + integer atomics
   const double bt = block_reduce<0>((double)cnt, sh);
@@ -435,7 +440,11 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
       return hp[s];
     };
     if (i < d.n) {
-      if (half == 0) d.selfl[i] = pos(i);
+      if (half == 0) {
+        const unsigned short own = pos(i);
+        d.selfl[i] = own;
+        for (int k = c; k < ((c + 3) & ~3); ++k) d.nbrl[nb_slot(np, k, i)] = own;
+      }
       for (int k = half; k < c; k += 2) d.nbrl[nb_slot(np, k, i)] = pos(d.nbr[nb_slot(np, k, i)]);
     }
     if (tid == 0) d.stage_cnt[t] = U;
@@ -551,39 +560,6 @@ struct HbmJ {
   }
   __device__ bool lower(int i, int j) const { return i < j; }
 };
-struct LdsJ {
-  using R = float;
-  using R4 = float4;
-  static constexpr int kUnroll = kNbBatch;
-  const Dev &d;
-  const float4 *lx, *lv;
-  const float4 *lt;   // accel: (t0..t3) per staged particle
-  const float2 *lt2;  // accel: (t4, t5)
-  const float *lp;    // rates: P / rho^2
-  const int i;
-  const int own;  // i's position in the staged set
-  __device__ void fetch(int k0, int out[kNbBatch]) const {
-    const ushort4 *row = reinterpret_cast<const ushort4 *>(d.nbrl) + i;
-    const ushort4 a = row[(long long)(k0 >> 2) * d.npad];
-    const ushort4 b = row[(long long)((k0 >> 2) + 1) * d.npad];
-    out[0] = a.x, out[1] = a.y, out[2] = a.z, out[3] = a.w;
-    out[4] = b.x, out[5] = b.y, out[6] = b.z, out[7] = b.w;
-  }
-  __device__ int self() const { return own; }
-  __device__ R4 x(int l) const { return lx[l]; }
-  __device__ R4 v(int l) const { return lv[l]; }
-  // accel staging keeps rho in the w slot of lx
-  __device__ void srec(int l, R t[6], R &rho) const {
-    const float4 a = lt[l];
-    const float2 b = lt2[l];
-    t[0] = a.x, t[1] = a.y, t[2] = a.z, t[3] = a.w, t[4] = b.x, t[5] = b.y;
-    rho = lx[l].w;
-  }
-  __device__ R pr(int l, R, R, R, R) const { return lp[l]; }
-  // staged positions follow particle index order
-  __device__ bool lower(int, int l) const { return own < l; }
-};
-
 // Calls f(j) for the entries of a neighbour row of cnt entries that belong
 // to lane `sub` of the particle's `lanes` lanes: batches of kNbBatch entries
 // sub, sub + lanes, sub + 2 lanes, ... in row order, each batch's indices
@@ -756,6 +732,175 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
   acc[2] = az;
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-precision pair passes (no tensile instability).  A lane walks its
+// particle's row in groups of four entries (rows are padded to a multiple
+// of four with the particle itself, whose pair terms are exact zeros):
+// lane `sub` of LANES takes groups sub, sub + LANES, ..., the row words two
+// groups ahead in flight.  All four neighbours' records of a group are
+// loaded before any is used and the pair arithmetic has no branch, so a
+// group's loads are in flight together.  The records come from the tile's
+// LDS copy (row entries are positions in the staged set, nbrl) or, for a
+// tile whose set did not fit, straight from HBM (row entries are particle
+// indices, nbr); both give the same values in the same order.
+
+struct LdsRows {
+  using G = ushort4;
+  const ushort4 *row;
+  long long np;
+  int own;  // the particle's position in the staged set
+  __device__ G group(int g) const { return row[(long long)g * np]; }
+  __device__ G self() const {
+    const unsigned short o = (unsigned short)own;
+    return make_ushort4(o, o, o, o);
+  }
+};
+struct HbmRows {
+  using G = int4;
+  const int4 *row;
+  long long np;
+  int own;  // the particle's index
+  __device__ G group(int g) const { return row[(long long)g * np]; }
+  __device__ G self() const { return make_int4(own, own, own, own); }
+};
+template <class G>
+__device__ inline int entry(const G &c, int u) {
+  return u == 0 ? (int)c.x : (u == 1 ? (int)c.y : (u == 2 ? (int)c.z : (int)c.w));
+}
+
+// Two groups per iteration, the next iteration's two groups of row words
+// loaded at its top so they are in flight while these are processed.  The
+// iteration is one branch-free block (a branch between the groups lets the
+// compiler sink the prefetch down to its use): a second group past the
+// row's end is replaced by `self`, a group of four self entries, whose
+// pair terms are zeros.  The groups beyond a row's end that the prefetch
+// touches (at most 4 LANES - 1) lie inside the allocation (nb_rows keeps
+// kRowSpare spare entries) and are never used.
+template <int LANES, class Rows, class F>
+__device__ inline void walk_groups(const Rows &rows, int cnt, int sub, const typename Rows::G &self,
+                                   F &&group_fn) {
+  static_assert(4 * 4 * LANES <= kRowSpare, "prefetch stays in the row allocation");
+  const int ng = (cnt + 3) >> 2;
+  typename Rows::G a = rows.group(sub);
+  typename Rows::G b = rows.group(sub + LANES);
+  for (int g = sub; g < ng; g += 2 * LANES) {
+    const typename Rows::G a2 = rows.group(g + 2 * LANES);
+    const typename Rows::G b2 = rows.group(g + 3 * LANES);
+    // compiler-only barrier: the prefetch stays issued here (it may not be
+    // re-formed as a load next to its use); emits no instruction
+    asm volatile("" ::: "memory");
+    group_fn(a);
+    // the second group's loads are not hoisted over the first group's
+    // arithmetic (that would hold both groups' records in registers)
+    __builtin_amdgcn_sched_barrier(0);
+    group_fn(g + LANES < ng ? b : self);
+    // the register moves of the hand-over stay at the end of the iteration
+    // (moved earlier they would wait for the prefetch)
+    __builtin_amdgcn_sched_barrier(0);
+    a = a2;
+    b = b2;
+  }
+}
+
+// particle i's kernel constants for the FP32 gradient factor
+struct GkF {
+  float ih, gin_a, gin_b, gout;  // 1/h, 2.25 c5, -3 c5, -0.75 c5 h
+};
+__device__ inline GkF gk_consts(float h) {
+  const KH<float> k = kern_h_t<float>(h);
+  return GkF{k.ih, 2.25f * k.c5, -3.0f * k.c5, -0.75f * k.c5 * h};
+}
+// GK of the cubic spline (Source/Functions.cpp:74-87) for r2 = |x_ij|^2:
+// c5 (2.25 q - 3) for q < 1, -0.75 c5 h (2 - q)^2 / r for 1 <= q < 2, 0
+// beyond, selected without a branch.  r2 = 0 (a padded self entry, or two
+// coincident particles) gives q = 0 and GK = -3 c5, the reference's q = 0
+// value, with every operand finite.
+__device__ inline float gk_f(float r2, const GkF &k) {
+  const float ri = __builtin_amdgcn_rsqf(fmaxf(r2, FLT_MIN));
+  const float q = (r2 * ri) * k.ih;
+  const float gin = fmaf(q, k.gin_a, k.gin_b);
+  const float t = fmaxf(2.0f - q, 0.0f);
+  const float gout = (t * t) * (ri * k.gout);
+  return q < 1.0f ? gin : gout;
+}
+
+// acceleration records of neighbour j: X = (x, y, z, rho), V = (v, m),
+// T = sigma / rho^2 as (xx, yy, zz, xy), T2 = (xz, yz)
+struct AccelLds {
+  const float4 *lx, *lv, *lt;
+  const float2 *lt2;
+  __device__ void load(int l, float4 &X, float4 &V, float4 &T, float2 &T2) const {
+    X = lx[l];
+    V = lv[l];
+    T = lt[l];
+    T2 = lt2[l];
+  }
+};
+struct AccelHbm {
+  const float4 *gx, *gv;
+  const float4 *gs;  // StressRecF as two float4: (t0..t3), (t4, t5, rho, pad)
+  __device__ void load(int j, float4 &X, float4 &V, float4 &T, float2 &T2) const {
+    const float4 x = gx[j], s1 = gs[2 * j + 1];
+    V = gv[j];
+    T = gs[2 * j];
+    X = make_float4(x.x, x.y, x.z, s1.z);
+    T2 = make_float2(s1.x, s1.y);
+  }
+};
+
+// CalcAccel + AccelReduction in FP32 (Source/InteractionAlt.cpp:12-260,
+// GradientType 0): a_i = sum_j m_j GK (sigma_i/rho_i^2 + sigma_j/rho_j^2 +
+// Pi_ij) e_ij with e = x_ij (or gradCorrM_i x_ij).  The particle's own term
+// sigma_i/rho_i^2 (sum_j m_j GK e_ij) is applied once after the row.
+template <int LANES, bool GC, class Rows, class Rec>
+__device__ inline void accel_mx_lane(const Dev &d, int cur, int i, int sub, const Rows &rows, const Rec &rec,
+                                     float (&acc)[3]) {
+  float ax = 0, ay = 0, az = 0, sx = 0, sy = 0, sz = 0;
+  float Li[6];
+  own_corr<GC>(d, i, Li);
+  const DevMat &M = d.mats[d.mat[i]];
+  const float4 xi = d.xhf[cur][i], vi = d.vmf[i];
+  const StressRecF ri = d.srecf[i];
+  const float h = (float)d.xh[cur][i].w;
+  const GkF k = gk_consts(h);
+  const float h2c = 0.01f * h * h, hh = 2.0f * h;
+  const float acs = (float)(M.alpha * M.Cs), bh = 0.5f * (float)M.beta;
+  walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
+    float4 X[4], V[4], T[4];
+    float2 T2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rec.load(entry(c, u), X[u], V[u], T[u], T2[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dx = xi.x - X[u].x, dy = xi.y - X[u].y, dz = xi.z - X[u].z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float GK = gk_f(r2, k);
+      // Monaghan viscosity for approaching pairs: with D = rho_i + rho_j,
+      // mu = h vdot / (r^2 + 0.01 h^2), Pi = mu (alpha c + beta mu) / (D/2)
+      const float vdot = (vi.x - V[u].x) * dx + (vi.y - V[u].y) * dy + (vi.z - V[u].z) * dz;
+      const float D = ri.rho + X[u].w;
+      const float hv = (hh * vdot) * __builtin_amdgcn_rcpf((r2 + h2c) * D);
+      const float pv = hv * fmaf(bh, hv * D, acs);
+      const float pi = vdot < 0.0f ? pv : 0.0f;
+      const float w = V[u].w * GK;
+      float ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
+      const float ux = fmaf(T[u].x, ex, fmaf(T[u].w, ey, fmaf(T2[u].x, ez, pi * ex)));
+      const float uy = fmaf(T[u].w, ex, fmaf(T[u].y, ey, fmaf(T2[u].y, ez, pi * ey)));
+      const float uz = fmaf(T2[u].x, ex, fmaf(T2[u].y, ey, fmaf(T[u].z, ez, pi * ez)));
+      ax = fmaf(w, ux, ax);
+      ay = fmaf(w, uy, ay);
+      az = fmaf(w, uz, az);
+      sx = fmaf(w, ex, sx);
+      sy = fmaf(w, ey, sy);
+      sz = fmaf(w, ez, sz);
+    }
+  });
+  acc[0] = ax + (ri.t[XX] * sx + ri.t[XY] * sy + ri.t[XZ] * sz);
+  acc[1] = ay + (ri.t[XY] * sx + ri.t[YY] * sy + ri.t[YZ] * sz);
+  acc[2] = az + (ri.t[XZ] * sx + ri.t[YZ] * sy + ri.t[ZZ] * sz);
+}
+
 // FP64: HBM gathers, LANES threads per particle
 template <int LANES>
 __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
@@ -778,35 +923,49 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
 }
 
 // mixed precision: LDS-staged gathers where the tile is staged, else HBM
-template <int LANES>
-__global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
+// the staged records fill ~51 KB of LDS, so three blocks share a CU: with
+// two or more lanes per particle the kernel is held to the registers of
+// three waves per SIMD
+template <int LANES, bool GC>
+__global__ __launch_bounds__(kBlock *LANES) __attribute__((amdgpu_waves_per_eu(LANES >= 2 ? 3 : 1))) void k_accel_mx(
+    Dev d, int cur) {
   __shared__ float4 lx[kStageMax], lv[kStageMax], lt[kStageMax];
   __shared__ float2 lt2[kStageMax];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
-  // with tensile instability the pass gathers from HBM (the staged records
-  // do not carry the TI stress)
-  const int U = d.ti_on ? -1 : d.stage_cnt[t];
+  const int U = d.stage_cnt[t];
   float a[3] = {0, 0, 0};
   if (U < 0) {
-    if (i < d.n) {
-      const HbmJ<true> g{d, cur, i};
-      if (d.gc_on) {
-        if (d.ti_on) accel_lane<true, true, true>(d, cur, i, g, sub, LANES, a);
-        else accel_lane<true, false, true>(d, cur, i, g, sub, LANES, a);
-      } else {
-        if (d.ti_on) accel_lane<true, true, false>(d, cur, i, g, sub, LANES, a);
-        else accel_lane<true, false, false>(d, cur, i, g, sub, LANES, a);
-      }
+    if (i < d.n && d.group[i] >= 0) {
+      const HbmRows rows{reinterpret_cast<const int4 *>(d.nbr) + i, d.npad, i};
+      const AccelHbm rec{d.xhf[cur], d.vmf, reinterpret_cast<const float4 *>(d.srecf)};
+      accel_mx_lane<LANES, GC>(d, cur, i, sub, rows, rec, a);
     }
   } else {
     stage_accel(d, t, U, d.xhf[cur], lx, lv, lt, lt2);
-    if (i < d.n) {
-      const LdsJ g{d, lx, lv, lt, lt2, nullptr, i, d.selfl[i]};
-      if (d.gc_on) accel_lane<true, false, true>(d, cur, i, g, sub, LANES, a);
-      else accel_lane<true, false, false>(d, cur, i, g, sub, LANES, a);
+    if (i < d.n && d.group[i] >= 0) {
+      const LdsRows rows{reinterpret_cast<const ushort4 *>(d.nbrl) + i, d.npad, (int)d.selfl[i]};
+      const AccelLds rec{lx, lv, lt, lt2};
+      accel_mx_lane<LANES, GC>(d, cur, i, sub, rows, rec, a);
     }
+  }
+  combine_lanes<LANES>(a, red);
+  if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
+}
+
+// mixed precision with tensile instability: HBM gathers (the staged
+// records do not carry the TI stress)
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_accel_mx_ti(Dev d, int cur) {
+  __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
+  const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
+  const int i = t * kBlock + (int)(threadIdx.x % kBlock);
+  float a[3] = {0, 0, 0};
+  if (i < d.n) {
+    const HbmJ<true> g{d, cur, i};
+    if (d.gc_on) accel_lane<true, true, true>(d, cur, i, g, sub, LANES, a);
+    else accel_lane<true, true, false>(d, cur, i, g, sub, LANES, a);
   }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
@@ -817,6 +976,21 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx(Dev d, int cur) {
     case 1: KERNEL<1><<<nblk(d.n), kBlock * 1, 0, st>>>(__VA_ARGS__); break;  \
     case 2: KERNEL<2><<<nblk(d.n), kBlock * 2, 0, st>>>(__VA_ARGS__); break;  \
     default: KERNEL<4><<<nblk(d.n), kBlock * 4, 0, st>>>(__VA_ARGS__); break; \
+  }
+// the mixed-precision passes are also instantiated per gradient correction
+#define WF_LANES_GC_SWITCH(lanes, gc, KERNEL, ...)                   \
+  if (gc) {                                                         \
+    switch (lanes) {                                                \
+      case 1: KERNEL<1, true><<<nblk(d.n), kBlock * 1, 0, st>>>(__VA_ARGS__); break;  \
+      case 2: KERNEL<2, true><<<nblk(d.n), kBlock * 2, 0, st>>>(__VA_ARGS__); break;  \
+      default: KERNEL<4, true><<<nblk(d.n), kBlock * 4, 0, st>>>(__VA_ARGS__); break; \
+    }                                                               \
+  } else {                                                          \
+    switch (lanes) {                                                \
+      case 1: KERNEL<1, false><<<nblk(d.n), kBlock * 1, 0, st>>>(__VA_ARGS__); break;  \
+      case 2: KERNEL<2, false><<<nblk(d.n), kBlock * 2, 0, st>>>(__VA_ARGS__); break;  \
+      default: KERNEL<4, false><<<nblk(d.n), kBlock * 4, 0, st>>>(__VA_ARGS__); break; \
+    }                                                               \
   }
 
 // StartAcceleration's tensile-instability stress, 3D branch
@@ -880,8 +1054,10 @@ WF_KERNEL void k_tir(Dev d, int cur) {
 void launch_tir(const Dev &d, hipStream_t st, int cur) { k_tir<<<nblk(d.n), kBlock, 0, st>>>(d, cur); }
 
 void launch_accel(const Dev &d, hipStream_t st, int cur) {
-  if (d.mixed) {
-    WF_LANES_SWITCH(d.lanes_mx, k_accel_mx, d, cur)
+  if (d.mixed && d.ti_on) {
+    WF_LANES_SWITCH(d.lanes_mx, k_accel_mx_ti, d, cur)
+  } else if (d.mixed) {
+    WF_LANES_GC_SWITCH(d.lanes_mx, d.gc_on, k_accel_mx, d, cur)
   } else {
     WF_LANES_SWITCH(d.lanes_f64, k_accel_f64, d, cur)
   }
@@ -1140,7 +1316,54 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_f64(Dev d, int cur, doub
   if (i < d.n && sub == 0) dens_update<false>(d, cur, i, s[0], dt, do_update);
 }
 
-template <int LANES>
+// records of the density and rate passes: X = (x, y, z, m/rho), V = (v, m)
+// and, for the rates, P / rho^2
+struct XvLds {
+  const float4 *lx, *lv;
+  const float *lp;
+  __device__ void load(int l, float4 &X, float4 &V) const {
+    X = lx[l];
+    V = lv[l];
+  }
+  __device__ float pr(int l) const { return lp[l]; }
+};
+struct XvHbm {
+  const float4 *gx, *gv;
+  const float *gp;
+  __device__ void load(int j, float4 &X, float4 &V) const {
+    X = gx[j];
+    V = gv[j];
+  }
+  __device__ float pr(int j) const { return gp[j]; }
+};
+
+// CalcDensInc + DensReduction in FP32 (Source/InteractionAlt.cpp:483-640):
+// sum_j (m_j / rho_j) v_ij . (GK e_ij); the caller multiplies by rho_i
+template <int LANES, bool GC, class Rows, class Rec>
+__device__ inline float dens_mx_lane(const Dev &d, int cur, int i, int sub, const Rows &rows, const Rec &rec) {
+  float s = 0;
+  float Li[6];
+  own_corr<GC>(d, i, Li);
+  const float4 xi = d.xhf[cur][i], vi = d.vmf[i];
+  const GkF k = gk_consts((float)d.xh[cur][i].w);
+  walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
+    float4 X[4], V[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rec.load(entry(c, u), X[u], V[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dx = xi.x - X[u].x, dy = xi.y - X[u].y, dz = xi.z - X[u].z;
+      const float GK = gk_f(dx * dx + dy * dy + dz * dz, k);
+      float ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
+      const float t = (vi.x - V[u].x) * ex + (vi.y - V[u].y) * ey + (vi.z - V[u].z) * ez;
+      s = fmaf(X[u].w * GK, t, s);
+    }
+  });
+  return s;
+}
+
+template <int LANES, bool GC>
 __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, double dt, int do_update) {
   __shared__ float4 lx[kStageMax], lv[kStageMax];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * kBlock];
@@ -1149,15 +1372,15 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, doubl
   const int U = d.stage_cnt[t];
   float s[1] = {0};
   if (U < 0) {
-    if (i < d.n) {
-      const HbmJ<true> g{d, cur, i};
-      s[0] = d.gc_on ? dens_lane<true, true>(d, cur, i, g, sub, LANES) : dens_lane<true, false>(d, cur, i, g, sub, LANES);
+    if (i < d.n && d.group[i] >= 0) {
+      const HbmRows rows{reinterpret_cast<const int4 *>(d.nbr) + i, d.npad, i};
+      s[0] = dens_mx_lane<LANES, GC>(d, cur, i, sub, rows, XvHbm{d.xhf[cur], d.vmf, nullptr});
     }
   } else {
     stage_xv(d, t, U, d.xhf[cur], lx, lv, nullptr);
-    if (i < d.n) {
-      const LdsJ g{d, lx, lv, nullptr, nullptr, nullptr, i, d.selfl[i]};
-      s[0] = d.gc_on ? dens_lane<true, true>(d, cur, i, g, sub, LANES) : dens_lane<true, false>(d, cur, i, g, sub, LANES);
+    if (i < d.n && d.group[i] >= 0) {
+      const LdsRows rows{reinterpret_cast<const ushort4 *>(d.nbrl) + i, d.npad, (int)d.selfl[i]};
+      s[0] = dens_mx_lane<LANES, GC>(d, cur, i, sub, rows, XvLds{lx, lv, nullptr});
     }
   }
   combine_lanes<LANES>(s, red);
@@ -1166,7 +1389,7 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, doubl
 
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update) {
   if (d.mixed) {
-    WF_LANES_SWITCH(d.lanes_mx, k_dens_mx, d, cur, dt, do_update)
+    WF_LANES_GC_SWITCH(d.lanes_mx, d.gc_on, k_dens_mx, d, cur, dt, do_update)
   } else {
     WF_LANES_SWITCH(d.lanes_f64, k_dens_f64, d, cur, dt, do_update)
   }
@@ -1268,7 +1491,72 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_f64(Dev d, int buf) {
   if (threadIdx.x == 0) part(d, P_KIN, blockIdx.x) = r1;
 }
 
-template <int LANES>
+// CalcRateTensors + RateTensorsReduction in FP32
+// (Source/InteractionAlt.cpp:263-479) with the kinetic-energy pair term
+// (Source/Domain.cpp:1776-1792).  With c_j = (m_j / rho_j) GK and
+// (a, b, g) = c_j e, the sums kept per lane are sum v_x a, v_y b, v_z g
+// (diagonal) and the six cross products sum v_x b, v_y a, v_x g, v_z a,
+// v_y g, v_z b; strain rate and rotation are their halved sums and
+// differences (eps_xy = -(P + Q)/2, w_01 = -(P - Q)/2).  The kinetic term
+// carries the lower-index particle's partner mass with a plus sign and the
+// higher's with a minus sign; its factor 1/2 is applied at the end.
+// v[0..5] strain rate, v[6..8] rotation (w01, w02, w12), v[9] the kinetic sum.
+template <int LANES, bool GC, bool STAGED, class Rows, class Rec>
+__device__ inline void rates_mx_lane(const Dev &d, int buf, int i, int own, int sub, const Rows &rows,
+                                     const Rec &rec, float (&v)[10]) {
+  float dxx = 0, dyy = 0, dzz = 0, pxy = 0, qxy = 0, pxz = 0, qxz = 0, pyz = 0, qyz = 0, ek = 0;
+  float Li[6];
+  own_corr<GC>(d, i, Li);
+  const float4 xi = d.xhf[buf][i], vi = d.vmf[i];
+  const float pri = d.prf[i], mneg = -vi.w;
+  const GkF k = gk_consts((float)d.xh[buf][i].w);
+  walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
+    float4 X[4], V[4];
+    float P[4];
+    int l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      l[u] = entry(c, u);
+      rec.load(l[u], X[u], V[u]);
+      P[u] = rec.pr(l[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float dx = xi.x - X[u].x, dy = xi.y - X[u].y, dz = xi.z - X[u].z;
+      const float GK = gk_f(dx * dx + dy * dy + dz * dz, k);
+      const float vx = vi.x - V[u].x, vy = vi.y - V[u].y, vz = vi.z - V[u].z;
+      const float cj = X[u].w * GK;
+      float ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
+      const float a = cj * ex, b = cj * ey, g = cj * ez;
+      dxx = fmaf(vx, a, dxx);
+      dyy = fmaf(vy, b, dyy);
+      dzz = fmaf(vz, g, dzz);
+      pxy = fmaf(vx, b, pxy);
+      qxy = fmaf(vy, a, qxy);
+      pxz = fmaf(vx, g, pxz);
+      qxz = fmaf(vz, a, qxz);
+      pyz = fmaf(vy, g, pyz);
+      qyz = fmaf(vz, b, qyz);
+      // staged positions follow particle index order
+      const bool lo = STAGED ? own < l[u] : i < l[u];
+      const float ms = lo ? V[u].w : mneg;
+      ek = fmaf(ms * (pri + P[u]), GK * (vx * dx + vy * dy + vz * dz), ek);
+    }
+  });
+  v[XX] = -dxx;
+  v[YY] = -dyy;
+  v[ZZ] = -dzz;
+  v[XY] = -0.5f * (pxy + qxy);
+  v[XZ] = -0.5f * (pxz + qxz);
+  v[YZ] = -0.5f * (pyz + qyz);
+  v[6] = -0.5f * (pxy - qxy);
+  v[7] = -0.5f * (pxz - qxz);
+  v[8] = -0.5f * (pyz - qyz);
+  v[9] = 0.5f * ek;
+}
+
+template <int LANES, bool GC>
 __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
   __shared__ float4 lx[kStageMax], lv[kStageMax];
   __shared__ float lp[kStageMax];
@@ -1279,17 +1567,15 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
   const int U = d.stage_cnt[t];
   float v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (U < 0) {
-    if (i < d.n) {
-      const HbmJ<true> g{d, buf, i};
-      if (d.gc_on) rates_lane<true, true, false>(d, buf, i, g, sub, LANES, v);
-      else rates_lane<true, false, false>(d, buf, i, g, sub, LANES, v);
+    if (i < d.n && d.group[i] >= 0) {
+      const HbmRows rows{reinterpret_cast<const int4 *>(d.nbr) + i, d.npad, i};
+      rates_mx_lane<LANES, GC, false>(d, buf, i, i, sub, rows, XvHbm{d.xhf[buf], d.vmf, d.prf}, v);
     }
   } else {
     stage_xv(d, t, U, d.xhf[buf], lx, lv, lp);
-    if (i < d.n) {
-      const LdsJ g{d, lx, lv, nullptr, nullptr, lp, i, d.selfl[i]};
-      if (d.gc_on) rates_lane<true, true, false>(d, buf, i, g, sub, LANES, v);
-      else rates_lane<true, false, false>(d, buf, i, g, sub, LANES, v);
+    if (i < d.n && d.group[i] >= 0) {
+      const LdsRows rows{reinterpret_cast<const ushort4 *>(d.nbrl) + i, d.npad, (int)d.selfl[i]};
+      rates_mx_lane<LANES, GC, true>(d, buf, i, d.selfl[i], sub, rows, XvLds{lx, lv, lp}, v);
     }
   }
   combine_lanes<LANES>(v, red);
@@ -1424,7 +1710,7 @@ WF_KERNEL void k_stress(Dev d, int buf, double dt) {
 
 void launch_rates(const Dev &d, hipStream_t st, int buf) {
   if (d.mixed) {
-    WF_LANES_SWITCH(d.lanes_mx, k_rates_mx, d, buf)
+    WF_LANES_GC_SWITCH(d.lanes_mx, d.gc_on, k_rates_mx, d, buf)
   } else {
     WF_LANES_SWITCH(d.lanes_f64, k_rates_f64, d, buf)
   }
