@@ -496,8 +496,11 @@ void allocate(Ctx *c) {
   // runs with the velocity or acceleration criterion
   d.auto_ts = (c->cfg.auto_ts & 3) ? c->cfg.auto_ts : 0;
   d.ti_on = 0;
-  for (size_t k = 0; k < c->mats.size(); ++k)
+  d.beta_on = 0;
+  for (size_t k = 0; k < c->mats.size(); ++k) {
     if (c->mat_set[k] && c->mats[k].ti > 0.0) d.ti_on = 1;
+    if (c->mat_set[k] && c->mats[k].beta != 0.0) d.beta_on = 1;
+  }
   if (d.ti_on) {
     d.tir = c->alloc<double>(6 * np);
     if (d.mixed) d.tirf = c->alloc<float>(6 * np);
@@ -1124,6 +1127,14 @@ int wf_set_material(void *ctx, int32_t index, const wf_material *m) {
       for (int q = 0; q < 3; ++q) one[0].xc[q] = c->mat_xc[3 * index + q];
       HIPCHK(hipMemcpyAsync(c->dmats + index, one.data(), sizeof(DevMat), hipMemcpyHostToDevice, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
+      // the acceleration kernel is chosen by whether a material has beta != 0
+      int beta_on = 0;
+      for (size_t k = 0; k < c->mats.size(); ++k)
+        if (c->mat_set[k] && c->mats[k].beta != 0.0) beta_on = 1;
+      if (beta_on != c->d.beta_on) {
+        c->d.beta_on = beta_on;
+        c->drop_graphs();
+      }
     }
     return WF_OK;
   });

@@ -214,6 +214,7 @@ struct Dev {
   int lanes_mx, lanes_f64;     // threads per particle in the pair passes
   int auto_ts;                 // bit 0 velocity, bit 1 acceleration criterion
   int ti_on;                   // some material has TI > 0
+  int beta_on;                 // some material has a quadratic viscosity term (beta != 0)
   int gc_on;                   // kernel-gradient correction (Domain::gradKernelCorr)
   int locking;                 // Locking sums: float m/rho weights in the rate gather
   double *gcm;                 // gradCorrM, symmetric [6][npad] (FP64)

@@ -599,30 +599,72 @@ __device__ inline void combine_lanes(R (&v)[NV], R *red) {
   }
 }
 
-// copy the tile's staged set into LDS (all threads of the block call these)
+// copy the tile's staged set into LDS (all NT threads of the block call
+// these).  A thread's share (entries p = tid + r NT) is loaded in three
+// rounds, each with all of the share's loads in flight: the set's particle
+// indices, then the records (from particle 0 for entries past the set's
+// end, so no load is conditional), then the LDS stores of the set's
+// entries.  Entries past the set's end are stored to the spare slot
+// kStageMax (the LDS arrays hold kStageMax + 1 records), so no store is
+// conditional either and nothing moves a load down to its store.
 // density / rates: (x, m/rho), (v, m) and, for rates, P / rho^2
+template <int NT>
 __device__ inline void stage_xv(const Dev &d, int t, int U, const float4 *gx, float4 *lx, float4 *lv, float *lp) {
+  constexpr int R = (kStageMax + NT - 1) / NT;
   const int *st = d.stage + (long long)t * kStageMax;
-  for (int p = threadIdx.x; p < U; p += blockDim.x) {
-    const int j = st[p];
-    lx[p] = gx[j];
-    lv[p] = d.vmf[j];
-    if (lp) lp[p] = d.prf[j];
+  int j[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int p = (int)threadIdx.x + r * NT;
+    j[r] = p < U ? st[p] : 0;
+  }
+  float4 x[R], v[R];
+  float pr[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    x[r] = gx[j[r]];
+    v[r] = d.vmf[j[r]];
+    if (lp) pr[r] = d.prf[j[r]];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int p = (int)threadIdx.x + r * NT;
+    const int q = p < U ? p : kStageMax;
+    lx[q] = x[r];
+    lv[q] = v[r];
+    if (lp) lp[q] = pr[r];
   }
   __syncthreads();
 }
 // acceleration: (x, rho), (v, m), sigma / rho^2 as (t0..t3), (t4, t5)
+template <int NT>
 __device__ inline void stage_accel(const Dev &d, int t, int U, const float4 *gx, float4 *lx, float4 *lv, float4 *lt,
                                    float2 *lt2) {
+  constexpr int R = (kStageMax + NT - 1) / NT;
   const int *st = d.stage + (long long)t * kStageMax;
-  for (int p = threadIdx.x; p < U; p += blockDim.x) {
-    const int j = st[p];
-    const StressRecF r = d.srecf[j];
-    const float4 x = gx[j];
-    lx[p] = make_float4(x.x, x.y, x.z, r.rho);
-    lv[p] = d.vmf[j];
-    lt[p] = make_float4(r.t[0], r.t[1], r.t[2], r.t[3]);
-    lt2[p] = make_float2(r.t[4], r.t[5]);
+  const float4 *gs = reinterpret_cast<const float4 *>(d.srecf);  // (t0..t3), (t4, t5, rho, pad)
+  int j[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int p = (int)threadIdx.x + r * NT;
+    j[r] = p < U ? st[p] : 0;
+  }
+  float4 x[R], v[R], s0[R], s1[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    x[r] = gx[j[r]];
+    v[r] = d.vmf[j[r]];
+    s0[r] = gs[2 * j[r]];
+    s1[r] = gs[2 * j[r] + 1];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int p = (int)threadIdx.x + r * NT;
+    const int q = p < U ? p : kStageMax;
+    lx[q] = make_float4(x[r].x, x[r].y, x[r].z, s1[r].z);
+    lv[q] = v[r];
+    lt[q] = s0[r];
+    lt2[q] = make_float2(s1[r].x, s1[r].y);
   }
   __syncthreads();
 }
@@ -768,24 +810,28 @@ __device__ inline int entry(const G &c, int u) {
   return u == 0 ? (int)c.x : (u == 1 ? (int)c.y : (u == 2 ? (int)c.z : (int)c.w));
 }
 
-// Two groups per iteration, the next iteration's two groups of row words
-// loaded at its top so they are in flight while these are processed.  The
-// iteration is one branch-free block (a branch between the groups lets the
-// compiler sink the prefetch down to its use): a second group past the
-// row's end is replaced by `self`, a group of four self entries, whose
-// pair terms are zeros.  The groups beyond a row's end that the prefetch
-// touches (at most 4 LANES - 1) lie inside the allocation (nb_rows keeps
-// kRowSpare spare entries) and are never used.
+// Lane `sub` of LANES takes the group pairs (2k, 2k + 1) with k = sub,
+// sub + LANES, ...: a row of ng groups costs its lanes ng groups in all
+// (the second group of the last pair is a self group when ng is odd).  An
+// iteration processes one pair, the next pair's row words loaded at its top
+// so they are in flight while this pair is processed.  The iteration is one
+// branch-free block (a branch between the groups lets the compiler sink the
+// prefetch down to its use): a second group past the row's end is replaced
+// by `self`, a group of four self entries, whose pair terms are zeros.  The
+// groups beyond a row's end that the prefetch touches (at most 2 LANES + 1)
+// lie inside the allocation (nb_rows keeps kRowSpare spare entries) and are
+// never used.
 template <int LANES, class Rows, class F>
 __device__ inline void walk_groups(const Rows &rows, int cnt, int sub, const typename Rows::G &self,
                                    F &&group_fn) {
-  static_assert(4 * 4 * LANES <= kRowSpare, "prefetch stays in the row allocation");
+  static_assert(4 * (2 * LANES + 2) <= kRowSpare, "prefetch stays in the row allocation");
   const int ng = (cnt + 3) >> 2;
-  typename Rows::G a = rows.group(sub);
-  typename Rows::G b = rows.group(sub + LANES);
-  for (int g = sub; g < ng; g += 2 * LANES) {
+  int g = 2 * sub;
+  typename Rows::G a = rows.group(g);
+  typename Rows::G b = rows.group(g + 1);
+  for (; g < ng; g += 2 * LANES) {
     const typename Rows::G a2 = rows.group(g + 2 * LANES);
-    const typename Rows::G b2 = rows.group(g + 3 * LANES);
+    const typename Rows::G b2 = rows.group(g + 2 * LANES + 1);
     // compiler-only barrier: the prefetch stays issued here (it may not be
     // re-formed as a load next to its use); emits no instruction
     asm volatile("" ::: "memory");
@@ -793,7 +839,7 @@ __device__ inline void walk_groups(const Rows &rows, int cnt, int sub, const typ
     // the second group's loads are not hoisted over the first group's
     // arithmetic (that would hold both groups' records in registers)
     __builtin_amdgcn_sched_barrier(0);
-    group_fn(g + LANES < ng ? b : self);
+    group_fn(g + 1 < ng ? b : self);
     // the register moves of the hand-over stay at the end of the iteration
     // (moved earlier they would wait for the prefetch)
     __builtin_amdgcn_sched_barrier(0);
@@ -852,7 +898,9 @@ struct AccelHbm {
 // GradientType 0): a_i = sum_j m_j GK (sigma_i/rho_i^2 + sigma_j/rho_j^2 +
 // Pi_ij) e_ij with e = x_ij (or gradCorrM_i x_ij).  The particle's own term
 // sigma_i/rho_i^2 (sum_j m_j GK e_ij) is applied once after the row.
-template <int LANES, bool GC, class Rows, class Rec>
+// BETA: some material has a quadratic viscosity term (beta != 0); without
+// it Pi = 2 alpha c h vdot / ((r^2 + 0.01 h^2) D)
+template <int LANES, bool GC, bool BETA, class Rows, class Rec>
 __device__ inline void accel_mx_lane(const Dev &d, int cur, int i, int sub, const Rows &rows, const Rec &rec,
                                      float (&acc)[3]) {
   float ax = 0, ay = 0, az = 0, sx = 0, sy = 0, sz = 0;
@@ -864,7 +912,7 @@ __device__ inline void accel_mx_lane(const Dev &d, int cur, int i, int sub, cons
   const float h = (float)d.xh[cur][i].w;
   const GkF k = gk_consts(h);
   const float h2c = 0.01f * h * h, hh = 2.0f * h;
-  const float acs = (float)(M.alpha * M.Cs), bh = 0.5f * (float)M.beta;
+  const float acs = (float)(M.alpha * M.Cs), bh = 0.5f * (float)M.beta, hha = hh * acs;
   walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
     float4 X[4], V[4], T[4];
     float2 T2[4];
@@ -879,8 +927,14 @@ __device__ inline void accel_mx_lane(const Dev &d, int cur, int i, int sub, cons
       // mu = h vdot / (r^2 + 0.01 h^2), Pi = mu (alpha c + beta mu) / (D/2)
       const float vdot = (vi.x - V[u].x) * dx + (vi.y - V[u].y) * dy + (vi.z - V[u].z) * dz;
       const float D = ri.rho + X[u].w;
-      const float hv = (hh * vdot) * __builtin_amdgcn_rcpf((r2 + h2c) * D);
-      const float pv = hv * fmaf(bh, hv * D, acs);
+      const float rd = __builtin_amdgcn_rcpf((r2 + h2c) * D);
+      float pv;
+      if constexpr (BETA) {
+        const float hv = (hh * vdot) * rd;
+        pv = hv * fmaf(bh, hv * D, acs);
+      } else {
+        pv = (hha * vdot) * rd;
+      }
       const float pi = vdot < 0.0f ? pv : 0.0f;
       const float w = V[u].w * GK;
       float ex, ey, ez;
@@ -923,14 +977,10 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
 }
 
 // mixed precision: LDS-staged gathers where the tile is staged, else HBM
-// the staged records fill ~51 KB of LDS, so three blocks share a CU: with
-// two or more lanes per particle the kernel is held to the registers of
-// three waves per SIMD
-template <int LANES, bool GC>
-__global__ __launch_bounds__(kBlock *LANES) __attribute__((amdgpu_waves_per_eu(LANES >= 2 ? 3 : 1))) void k_accel_mx(
-    Dev d, int cur) {
-  __shared__ float4 lx[kStageMax], lv[kStageMax], lt[kStageMax];
-  __shared__ float2 lt2[kStageMax];
+template <int LANES, bool GC, bool BETA>
+__device__ inline void k_accel_mx_body(const Dev &d, int cur) {
+  __shared__ float4 lx[kStageMax + 1], lv[kStageMax + 1], lt[kStageMax + 1];
+  __shared__ float2 lt2[kStageMax + 1];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 3 * kBlock];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
@@ -940,18 +990,33 @@ __global__ __launch_bounds__(kBlock *LANES) __attribute__((amdgpu_waves_per_eu(L
     if (i < d.n && d.group[i] >= 0) {
       const HbmRows rows{reinterpret_cast<const int4 *>(d.nbr) + i, d.npad, i};
       const AccelHbm rec{d.xhf[cur], d.vmf, reinterpret_cast<const float4 *>(d.srecf)};
-      accel_mx_lane<LANES, GC>(d, cur, i, sub, rows, rec, a);
+      accel_mx_lane<LANES, GC, BETA>(d, cur, i, sub, rows, rec, a);
     }
   } else {
-    stage_accel(d, t, U, d.xhf[cur], lx, lv, lt, lt2);
+    stage_accel<kBlock * LANES>(d, t, U, d.xhf[cur], lx, lv, lt, lt2);
     if (i < d.n && d.group[i] >= 0) {
       const LdsRows rows{reinterpret_cast<const ushort4 *>(d.nbrl) + i, d.npad, (int)d.selfl[i]};
       const AccelLds rec{lx, lv, lt, lt2};
-      accel_mx_lane<LANES, GC>(d, cur, i, sub, rows, rec, a);
+      accel_mx_lane<LANES, GC, BETA>(d, cur, i, sub, rows, rec, a);
     }
   }
   combine_lanes<LANES>(a, red);
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
+}
+
+// the kernels with and without the quadratic viscosity term.  The staged
+// records fill ~51 KB of LDS, so three blocks share a CU: with two or more
+// lanes per particle the kernels are held to the registers of three waves
+// per SIMD
+template <int LANES, bool GC>
+__global__ __launch_bounds__(kBlock *LANES) __attribute__((amdgpu_waves_per_eu(LANES >= 2 ? 3 : 1))) void k_accel_mx_b(
+    Dev d, int cur) {
+  k_accel_mx_body<LANES, GC, true>(d, cur);
+}
+template <int LANES, bool GC>
+__global__ __launch_bounds__(kBlock *LANES) __attribute__((amdgpu_waves_per_eu(LANES >= 2 ? 3 : 1))) void k_accel_mx_nb(
+    Dev d, int cur) {
+  k_accel_mx_body<LANES, GC, false>(d, cur);
 }
 
 // mixed precision with tensile instability: HBM gathers (the staged
@@ -1056,8 +1121,10 @@ void launch_tir(const Dev &d, hipStream_t st, int cur) { k_tir<<<nblk(d.n), kBlo
 void launch_accel(const Dev &d, hipStream_t st, int cur) {
   if (d.mixed && d.ti_on) {
     WF_LANES_SWITCH(d.lanes_mx, k_accel_mx_ti, d, cur)
+  } else if (d.mixed && d.beta_on) {
+    WF_LANES_GC_SWITCH(d.lanes_mx, d.gc_on, k_accel_mx_b, d, cur)
   } else if (d.mixed) {
-    WF_LANES_GC_SWITCH(d.lanes_mx, d.gc_on, k_accel_mx, d, cur)
+    WF_LANES_GC_SWITCH(d.lanes_mx, d.gc_on, k_accel_mx_nb, d, cur)
   } else {
     WF_LANES_SWITCH(d.lanes_f64, k_accel_f64, d, cur)
   }
@@ -1272,22 +1339,32 @@ __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, co
   return s;
 }
 
+// the particle's own FP64 state the update reads
+struct DensOwn {
+  double rho;
+  double4 x, v, u;
+};
+__device__ inline DensOwn dens_own(const Dev &d, int cur, int i) {
+  return DensOwn{d.rho[cur][i], d.xh[cur][i], d.vm[i], d.disp[i]};
+}
+
 template <bool MX>
-__device__ inline void dens_update(const Dev &d, int cur, int i, double s, double dt, int do_update) {
+__device__ inline void dens_update(const Dev &d, int cur, int i, double s, double dt, int do_update,
+                                   const DensOwn &o) {
   if (dt < 0) dt = d.flags->dt;
-  const double rhoi = d.rho[cur][i];
+  const double rhoi = o.rho;
   const double drho = s * rhoi;
   d.drho[i] = drho;
   if (!do_update) return;
   const int nxt = cur ^ 1;
-  const double4 xd = d.xh[cur][i];
-  const double4 vd = d.vm[i];
+  const double4 xd = o.x;
+  const double4 vd = o.v;
   const double rn = rhoi + dt * drho;
   const double vn = vd.w / rn;
   d.rho[nxt][i] = rn;
   d.vol[nxt][i] = vn;
   const double ux = vd.x * dt, uy = vd.y * dt, uz = vd.z * dt;
-  double4 u = d.disp[i];
+  double4 u = o.u;
   u.x += ux;
   u.y += uy;
   u.z += uz;
@@ -1313,7 +1390,7 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_f64(Dev d, int cur, doub
     s[0] = d.gc_on ? dens_lane<false, true>(d, cur, i, g, sub, LANES) : dens_lane<false, false>(d, cur, i, g, sub, LANES);
   }
   combine_lanes<LANES>(s, red);
-  if (i < d.n && sub == 0) dens_update<false>(d, cur, i, s[0], dt, do_update);
+  if (i < d.n && sub == 0) dens_update<false>(d, cur, i, s[0], dt, do_update, dens_own(d, cur, i));
 }
 
 // records of the density and rate passes: X = (x, y, z, m/rho), V = (v, m)
@@ -1365,11 +1442,14 @@ __device__ inline float dens_mx_lane(const Dev &d, int cur, int i, int sub, cons
 
 template <int LANES, bool GC>
 __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, double dt, int do_update) {
-  __shared__ float4 lx[kStageMax], lv[kStageMax];
+  __shared__ float4 lx[kStageMax + 1], lv[kStageMax + 1];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * kBlock];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
   const int i = t * kBlock + (int)(threadIdx.x % kBlock);
   const int U = d.stage_cnt[t];
+  // the update's FP64 inputs are loaded now and arrive during the row walk
+  DensOwn own{};
+  if (i < d.n && sub == 0) own = dens_own(d, cur, i);
   float s[1] = {0};
   if (U < 0) {
     if (i < d.n && d.group[i] >= 0) {
@@ -1377,14 +1457,14 @@ __global__ __launch_bounds__(kBlock *LANES) void k_dens_mx(Dev d, int cur, doubl
       s[0] = dens_mx_lane<LANES, GC>(d, cur, i, sub, rows, XvHbm{d.xhf[cur], d.vmf, nullptr});
     }
   } else {
-    stage_xv(d, t, U, d.xhf[cur], lx, lv, nullptr);
+    stage_xv<kBlock * LANES>(d, t, U, d.xhf[cur], lx, lv, nullptr);
     if (i < d.n && d.group[i] >= 0) {
       const LdsRows rows{reinterpret_cast<const ushort4 *>(d.nbrl) + i, d.npad, (int)d.selfl[i]};
       s[0] = dens_mx_lane<LANES, GC>(d, cur, i, sub, rows, XvLds{lx, lv, nullptr});
     }
   }
   combine_lanes<LANES>(s, red);
-  if (i < d.n && sub == 0) dens_update<true>(d, cur, i, (double)s[0], dt, do_update);
+  if (i < d.n && sub == 0) dens_update<true>(d, cur, i, (double)s[0], dt, do_update, own);
 }
 
 void launch_dens_drift(const Dev &d, hipStream_t st, int cur, double dt, int do_update) {
@@ -1558,8 +1638,8 @@ __device__ inline void rates_mx_lane(const Dev &d, int buf, int i, int own, int 
 
 template <int LANES, bool GC>
 __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
-  __shared__ float4 lx[kStageMax], lv[kStageMax];
-  __shared__ float lp[kStageMax];
+  __shared__ float4 lx[kStageMax + 1], lv[kStageMax + 1];
+  __shared__ float lp[kStageMax + 1];
   __shared__ float red[(LANES > 1 ? LANES - 1 : 1) * 10 * kBlock];
   __shared__ double sh[kBlock * LANES / 64];
   const int t = xcd_tile(blockIdx.x, gridDim.x), sub = threadIdx.x / kBlock;
@@ -1572,7 +1652,7 @@ __global__ __launch_bounds__(kBlock *LANES) void k_rates_mx(Dev d, int buf) {
       rates_mx_lane<LANES, GC, false>(d, buf, i, i, sub, rows, XvHbm{d.xhf[buf], d.vmf, d.prf}, v);
     }
   } else {
-    stage_xv(d, t, U, d.xhf[buf], lx, lv, lp);
+    stage_xv<kBlock * LANES>(d, t, U, d.xhf[buf], lx, lv, lp);
     if (i < d.n && d.group[i] >= 0) {
       const LdsRows rows{reinterpret_cast<const ushort4 *>(d.nbrl) + i, d.npad, (int)d.selfl[i]};
       rates_mx_lane<LANES, GC, true>(d, buf, i, d.selfl[i], sub, rows, XvLds{lx, lv, lp}, v);
