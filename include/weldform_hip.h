@@ -395,6 +395,18 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
 #define WF_XDMF_NI 3
 int wf_pack_xdmf(void *ctx, float *f, int32_t *iv);
 
+/* wf_pack_xdmf_noalloc: no counterpart in the reference.  wf_pack_xdmf for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same tables,
+ * bit for bit, staged in a static device array of 64 MiB per GPU instead
+ * of a staging buffer allocated on the first call.  The array is shared by
+ * every call on that GPU and kept until the process ends, so calls that
+ * overlap on one GPU (from several threads or contexts) race.  Contexts of
+ * more than wf_pack_xdmf_noalloc_max() particles (250,406: 268 bytes each)
+ * get WF_EINVAL.  No hipMalloc / hipFree is called. */
+int wf_pack_xdmf_noalloc(void *ctx, float *f, int32_t *iv);
+int64_t wf_pack_xdmf_noalloc_max(void);
+
 /* wf_pack_csv <- the rows of Domain::WriteCSV (Source/Output.cpp:521-554):
  * rows receives WF_CSV_NCOL doubles per particle, particle after particle:
  * X, Y, Z, ID, Sigma_eq, Pl_Strain, vx, vy, vz, ax, ay, az, CFx, CFy, CFz,

@@ -96,6 +96,7 @@ EXPORTS = [
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
     "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
     "wf_set_contact_heat", "wf_set_rigid_temperature", "wf_prepare", "wf_step_part",
+    "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max",
 ]
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
@@ -157,6 +158,8 @@ def lib():
         "wf_adj_get_field": ([vp, i32, dp], ctypes.c_int),
         "wf_adj_restore": ([vp], ctypes.c_int),
         "wf_pack_xdmf": ([vp, ctypes.POINTER(ctypes.c_float), ip], ctypes.c_int),
+        "wf_pack_xdmf_noalloc": ([vp, ctypes.POINTER(ctypes.c_float), ip], ctypes.c_int),
+        "wf_pack_xdmf_noalloc_max": ([], ctypes.c_int64),
         "wf_pack_csv": ([vp, dp], ctypes.c_int),
         "wf_add_rigid_mesh": ([vp, i32, i64, dp, i64, ip, dp, d], ctypes.c_int),
         "wf_set_bc_angular_velocity": ([vp, i32, dp], ctypes.c_int),
@@ -380,6 +383,14 @@ class Solver:
         f = np.empty(XDMF_NF * self.n, dtype=np.float32)
         iv = np.empty(XDMF_NI * self.n, dtype=np.int32)
         self._check(lib().wf_pack_xdmf(self.ctx, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ip(iv)))
+        return f, iv
+
+    def pack_xdmf_noalloc(self):
+        """pack_xdmf staged in the library's static device scratch (no device
+        allocation; contexts of at most pack_xdmf_noalloc_max() particles)."""
+        f = np.empty(XDMF_NF * self.n, dtype=np.float32)
+        iv = np.empty(XDMF_NI * self.n, dtype=np.int32)
+        self._check(lib().wf_pack_xdmf_noalloc(self.ctx, f.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), _ip(iv)))
         return f, iv
 
     def pack_csv(self):

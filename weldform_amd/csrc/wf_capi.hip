@@ -1348,6 +1348,36 @@ int wf_pack_xdmf(void *ctx, float *f, int32_t *iv) {
   });
 }
 
+int wf_pack_xdmf_noalloc(void *ctx, float *f, int32_t *iv) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!f || !iv) throw Error(WF_EINVAL, "null output buffer");
+    if (!c->inited) throw Error(WF_ESTATE, "wf_pack_xdmf_noalloc before wf_init");
+    const size_t nf = (size_t)WF_XDMF_NF * (size_t)c->n, ni = (size_t)WF_XDMF_NI * (size_t)c->n;
+    if ((nf + ni) * 4 > kOutScratchBytes)
+      throw Error(WF_EINVAL, "wf_pack_xdmf_noalloc: " + std::to_string(c->n) + " particles need " +
+                                 std::to_string((nf + ni) * 4) + " bytes of staging, more than the static scratch of " +
+                                 std::to_string(kOutScratchBytes) + " bytes (at most " +
+                                 std::to_string(wf_pack_xdmf_noalloc_max()) + " particles); use wf_pack_xdmf");
+    HIPCHK(hipSetDevice(c->cfg.device));
+    void *scr = nullptr;
+    HIPCHK(out_scratch(c->cfg.device, &scr));
+    float *df = (float *)scr;
+    int *di = (int *)((char *)scr + nf * 4);
+    launch_pack_xdmf(c->d, c->st, c->cur, df, di);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(f, df, nf * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipMemcpyAsync(iv, di, ni * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return WF_OK;
+  });
+}
+
+int64_t wf_pack_xdmf_noalloc_max(void) {
+  return (int64_t)(kOutScratchBytes / (4 * (size_t)(WF_XDMF_NF + WF_XDMF_NI)));
+}
+
 int wf_pack_csv(void *ctx, double *rows) {
   Ctx *c = (Ctx *)ctx;
   return guard(c, [&] {

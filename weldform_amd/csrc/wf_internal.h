@@ -321,5 +321,10 @@ void launch_mesh_update(const Dev &d, hipStream_t st, double dt);
 // output tables (wf_output.hip)
 void launch_pack_xdmf(const Dev &d, hipStream_t st, int cur, float *f, int *iv);
 void launch_pack_csv(const Dev &d, hipStream_t st, int cur, double *rows);
+// the static device scratch of wf_pack_xdmf_noalloc: its device address on
+// `device` (looked up on the first call per device)
+constexpr size_t kOutScratchBytes = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+hipError_t out_scratch(int device, void **ptr);
 
 }  // namespace wf

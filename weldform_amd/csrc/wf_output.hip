@@ -8,6 +8,8 @@
 // A thread packs one particle: it reads each own-state array once
 // (coalesced, internal order) and scatters to perm[k], the particle's
 // position in the caller's order.
+#include <mutex>
+
 #include "wf_device.h"
 
 namespace wf {
@@ -123,6 +125,38 @@ WF_KERNEL void k_pack_csv(Dev d, int cur, double *rows) {
 
 void launch_pack_xdmf(const Dev &d, hipStream_t st, int cur, float *f, int *iv) {
   k_pack_xdmf<<<nblk(d.n), kBlock, 0, st>>>(d, cur, f, iv);
+}
+
+// ---------------------------------------------------------------------------
+// Static device scratch of the allocation-free output entry point
+// (wf_pack_xdmf_noalloc): kOutScratchBytes per GPU, held from the library's
+// first use of a device until the process ends.  Host code never passes the
+// array's name to a kernel or a copy: on the host it is only a stand-in for
+// the symbol.  out_scratch() looks up the array's device address once per
+// device (with that device current) and returns it.
+static __device__ __align__(256) unsigned char scratch[kOutScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t out_scratch(int device, void **ptr) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int prev = 0;
+    hipError_t e = hipGetDevice(&prev);
+    if (e != hipSuccess) return e;
+    if (prev != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev != device) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *ptr = g_scratch_ptr[device];
+  return hipSuccess;
 }
 
 void launch_pack_csv(const Dev &d, hipStream_t st, int cur, double *rows) {
