@@ -171,44 +171,63 @@ __device__ inline int block_scan_excl(int v, int *sh, int *tot) {
 }
 
 // exclusive scan of cnt[0..ncells) into start[0..ncells] by one block of
-// 1024 threads: rounds of K consecutive cells per thread, the next round's
-// counts loaded while the current round is scanned
-__device__ inline void scan_cells(const int *cnt, int *start, int ncells, int *sh) {
-  constexpr int K = 16;
-  constexpr int R = K * 1024;
-  int v[K], nv[K];
-  const int t0 = (int)threadIdx.x * K;
+// 1024 threads, in rounds of K * 1024 cells: a round is loaded coalesced
+// (the next round's counts prefetched into registers) into LDS, each thread
+// scans K consecutive cells there (the buffer padded by one word per 16 so
+// the per-thread runs fall in distinct banks), the threads' sums are
+// block-scanned, and the offsets leave LDS coalesced.
+constexpr int kScanK = 16;
+constexpr int kScanRound = kScanK * 1024;
+__device__ inline int scan_pad(int p) { return p + (p >> 4); }
+__device__ inline void scan_cells(const int *cnt, int *start, int ncells, int *sh, int *buf) {
+  const int tid = threadIdx.x;
+  int nv[kScanK];
 #pragma unroll
-  for (int k = 0; k < K; ++k) v[k] = t0 + k < ncells ? cnt[t0 + k] : 0;
+  for (int k = 0; k < kScanK; ++k) {
+    const int c = k * 1024 + tid;
+    nv[k] = c < ncells ? cnt[c] : 0;
+  }
   int carry = 0;
-  for (int base = 0; base < ncells; base += R) {
+  for (int base = 0; base < ncells; base += kScanRound) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int c = base + R + t0 + k;
+    for (int k = 0; k < kScanK; ++k) buf[scan_pad(k * 1024 + tid)] = nv[k];
+#pragma unroll
+    for (int k = 0; k < kScanK; ++k) {
+      const int c = base + kScanRound + k * 1024 + tid;
       nv[k] = c < ncells ? cnt[c] : 0;
     }
-    int s = 0;
+    __syncthreads();
+    int v[kScanK], sum = 0;
 #pragma unroll
-    for (int k = 0; k < K; ++k) s += v[k];
+    for (int k = 0; k < kScanK; ++k) {
+      v[k] = buf[scan_pad(tid * kScanK + k)];
+      sum += v[k];
+    }
     int tot;
-    int off = block_scan_excl(s, sh, &tot) + carry;
+    int off = block_scan_excl(sum, sh, &tot) + carry;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int c = base + t0 + k;
-      if (c < ncells) start[c] = off;
+    for (int k = 0; k < kScanK; ++k) {
+      buf[scan_pad(tid * kScanK + k)] = off;
       off += v[k];
-      v[k] = nv[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanK; ++k) {
+      const int c = base + k * 1024 + tid;
+      if (c < ncells) start[c] = buf[scan_pad(k * 1024 + tid)];
     }
     carry += tot;
+    __syncthreads();
   }
-  if (threadIdx.x == 0) start[ncells] = carry;
+  if (tid == 0) start[ncells] = carry;
 }
 
 __global__ __launch_bounds__(1024) void k_scan(Dev d) {
   if (!d.flags->do_search) return;
   __shared__ int sh[34];
-  scan_cells(d.cell_count, d.cell_start, d.ncells, sh);
-  if (d.nrig) scan_cells(d.rcount, d.rstart, d.ncells, sh);
+  __shared__ int buf[kScanRound + kScanRound / 16];
+  scan_cells(d.cell_count, d.cell_start, d.ncells, sh, buf);
+  if (d.nrig) scan_cells(d.rcount, d.rstart, d.ncells, sh, buf);
 }
 
 WF_KERNEL void k_scatter(Dev d) {
@@ -260,17 +279,27 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
   }
 }
 
-// One lane per particle over the 27 cells around its own, in cell order
-// (each cell's particles by index), so a row lists its neighbours in a fixed
-// order.  The radius test runs in FP32 on offsets from the cell corners
-// (|error| < 1e-6 relative at the radius) and is trusted outside a band of
-// 1e-4 around it; inside the band, and for a particle lying far outside the
-// grid (clamped into a boundary cell, where its FP32 offset loses digits),
-// the exact FP64 predicate CheckRadius decides.
+// One lane per particle (lanes in cell order) over the 27 cells around the
+// particle's own, in cell order (each cell's particles by index), so a row
+// lists its neighbours in a fixed order.  The 27 cells are 9 columns of up to three consecutive cell ids,
+// whose candidates are one contiguous range of the cell-ordered records; a
+// range is walked four candidates at a time, their loads issued together.
+// The radius test runs in FP32 on offsets from the cell corners (|error| <
+// 1e-6 relative at the radius) and is trusted outside a band of 1e-4
+// around it; inside the band, and for a particle lying far outside the grid
+// (clamped into a boundary cell, where its FP32 offset loses digits), the
+// exact FP64 predicate CheckRadius decides.
 WF_KERNEL void k_build_nbr(Dev d, int cur) {
   if (!d.flags->do_search) return;
   __shared__ double sh[kBlock / 64];
-  const int i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  // lanes take the particles in cell order: a cell's particles walk the same
+  // candidate ranges in step, so their loads are shared and their loops
+  // agree
+  const int ps = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int i = ps < d.n ? d.sorted[ps] : ps;
+  const long long np = d.npad;
+  int4 *row = reinterpret_cast<int4 *>(d.nbr) + i;
+  int4 pend = make_int4(i, i, i, i);  // the row's group being filled
   int cnt = 0;
   if (i < d.n && d.group[i] >= 0) {
     const int g = d.group[i];
@@ -284,32 +313,52 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
     const float ozi = (float)(xi.z - (d.blpf[2] + cz * d.cellsize[2]));
     const float hi = (float)xi.w, hcf = 0.5f * (float)d.cellfac;
     const bool exact_all = fabsf(oxi) > 64.f * csx || fabsf(oyi) > 64.f * csy || fabsf(ozi) > 64.f * csz;
-    const long long np = d.npad;
-    for (int ax = cx - 1; ax <= cx + 1; ++ax) {
-      if (ax < 0 || ax >= nx) continue;
-      const float bx = (float)(cx - ax) * csx + oxi;
-      for (int ay = cy - 1; ay <= cy + 1; ++ay) {
-        if (ay < 0 || ay >= ny) continue;
-        const float by = (float)(cy - ay) * csy + oyi;
-        for (int az = cz - 1; az <= cz + 1; ++az) {
-          if (az < 0 || az >= nz) continue;
+    const int az0 = cz > 0 ? cz - 1 : 0, az1 = cz + 1 < nz ? cz + 1 : nz - 1, m = az1 - az0;
+    for (int col = 0; col < 9; ++col) {
+      const int ax = cx + col / 3 - 1, ay = cy + col % 3 - 1;
+      if (ax < 0 || ax >= nx || ay < 0 || ay >= ny) continue;
+      const float bx = (float)(cx - ax) * csx + oxi, by = (float)(cy - ay) * csy + oyi;
+      const int cc0 = (ax * ny + ay) * nz + az0;
+      // candidates of cells az0, az0 + 1, az0 + 2 start at s0, s1, s2
+      const int s0 = d.cell_start[cc0], s1 = d.cell_start[cc0 + 1];
+      const int s2 = m >= 1 ? d.cell_start[cc0 + 2] : s1;
+      const int se = d.cell_start[cc0 + m + 1];
+      for (int pb = s0; pb < se; pb += 4) {
+        int2 f[4];
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int p = pb + u < se ? pb + u : se - 1;
+          f[u] = d.sinfo[p];
+          q[u] = d.sxyz[p];
+        }
+        // compiler-only barrier: the four candidates' loads stay issued
+        // together (not sunk into the accept branches); emits no instruction
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int p = pb + u;
+          const bool valid = p < se && f[u].x != i && f[u].y == g;
+          const int az = az0 + (p >= s1 ? 1 : 0) + (p >= s2 ? 1 : 0);
           const float bz = (float)(cz - az) * csz + ozi;
-          const int cc = (ax * ny + ay) * nz + az;
-          const int s0 = d.cell_start[cc], e0 = d.cell_start[cc + 1];
-          for (int p = s0; p < e0; ++p) {
-            const int2 inf = d.sinfo[p];
-            if (inf.x == i || inf.y != g) continue;
-            const float4 q = d.sxyz[p];
-            const float dx = bx - q.x, dy = by - q.y, dz = bz - q.z;
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float t = hcf * (hi + q.w);
-            const float t2 = t * t;
-            bool in = r2 < t2;
-            if (exact_all || fabsf(r2 - t2) <= 1e-4f * t2) in = in_radius(xi, d.xh[cur][inf.x], d.cellfac);
-            if (in) {
-              if (cnt < d.max_nb) d.nbr[nb_slot(np, cnt, i)] = inf.x;
-              ++cnt;
+          const float dx = bx - q[u].x, dy = by - q[u].y, dz = bz - q[u].z;
+          const float r2 = dx * dx + dy * dy + dz * dz;
+          const float t = hcf * (hi + q[u].w);
+          const float t2 = t * t;
+          bool in = valid && r2 < t2;
+          if (valid && (exact_all || fabsf(r2 - t2) <= 1e-4f * t2)) in = in_radius(xi, d.xh[cur][f[u].x], d.cellfac);
+          if (in) {
+            // entries are gathered four at a time and stored as one group
+            // (past max_nb they are only counted)
+            if (cnt < d.max_nb) {
+              const int k = cnt & 3;
+              pend.x = k == 0 ? f[u].x : pend.x;
+              pend.y = k == 1 ? f[u].x : pend.y;
+              pend.z = k == 2 ? f[u].x : pend.z;
+              pend.w = k == 3 ? f[u].x : pend.w;
+              if (k == 3) row[(long long)(cnt >> 2) * np] = pend;
             }
+            ++cnt;
           }
         }
       }
@@ -321,7 +370,13 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
     if (cnt > d.max_nb) d.flags->overflow = 1;
     // the pair passes read rows in groups of four: the rest of the last
     // group holds the particle itself (a pair term of exact zeros)
-    for (int k = kept; k < ((kept + 3) & ~3); ++k) d.nbr[nb_slot(d.npad, k, i)] = i;
+    if (kept & 3) {
+      const int k = kept & 3;
+      pend.y = k <= 1 ? i : pend.y;
+      pend.z = k <= 2 ? i : pend.z;
+      pend.w = i;
+      row[(long long)(kept >> 2) * np] = pend;
+    }
   }
   // deterministic totals via block reduction + integer atomics
   const double bt = block_reduce<0>((double)cnt, sh);
@@ -385,9 +440,26 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
         s = (s + 1) & mask;
       }
     };
+    // a row's groups of four entries (the last padded with i itself), the
+    // two halves taking alternate groups, four groups' loads in flight; the
+    // groups read past a row's end lie inside the row allocation
+    const int4 *row = reinterpret_cast<const int4 *>(d.nbr) + i;
+    const int ng = (c + 3) >> 2;
     if (i < d.n) {
       if (half == 0) insert(i);
-      for (int k = half; k < c; k += 2) insert(d.nbr[nb_slot(np, k, i)]);
+      for (int g0 = half; g0 < ng; g0 += 8) {
+        int4 v[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = row[(long long)(g0 + 2 * b) * np];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (g0 + 2 * b >= ng) break;
+          insert(v[b].x);
+          insert(v[b].y);
+          insert(v[b].z);
+          insert(v[b].w);
+        }
+      }
     }
     __syncthreads();
     if (full || nset > d.stage_max) {
@@ -440,12 +512,19 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
       return hp[s];
     };
     if (i < d.n) {
-      if (half == 0) {
-        const unsigned short own = pos(i);
-        d.selfl[i] = own;
-        for (int k = c; k < ((c + 3) & ~3); ++k) d.nbrl[nb_slot(np, k, i)] = own;
+      if (half == 0) d.selfl[i] = pos(i);
+      // the padding entries hold i, so they become the particle's own position
+      ushort4 *rowl = reinterpret_cast<ushort4 *>(d.nbrl) + i;
+      for (int g0 = half; g0 < ng; g0 += 8) {
+        int4 v[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[b] = row[(long long)(g0 + 2 * b) * np];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (g0 + 2 * b >= ng) break;
+          rowl[(long long)(g0 + 2 * b) * np] = make_ushort4(pos(v[b].x), pos(v[b].y), pos(v[b].z), pos(v[b].w));
+        }
       }
-      for (int k = half; k < c; k += 2) d.nbrl[nb_slot(np, k, i)] = pos(d.nbr[nb_slot(np, k, i)]);
     }
     if (tid == 0) d.stage_cnt[t] = U;
     __syncthreads();
