@@ -16,7 +16,9 @@
 // friction regime, viscosity switch) is the forward's.
 //
 // Pair passes are gathers over each particle's own neighbour row, as in the
-// forward: the pair term of (i, j) depends on i's and j's inputs, and its
+// forward (kAdjLanes threads per particle taking alternate entries, partial
+// sums added in lane order): the pair term of (i, j) depends on i's and j's
+// inputs, and its
 // adjoint with respect to i's inputs needs only the pair's combined seed
 // (m_j abar_i - m_i abar_j for the acceleration, vol_j sbar_i + vol_i sbar_j
 // for the density and rate sums), so every particle accumulates its own
@@ -24,6 +26,8 @@
 #include "wf_device.h"
 
 namespace wf {
+
+constexpr int kAdjLanes = 2;  // threads per particle in the reverse gathers
 
 // d GK / d r2 of gk_t (the pair term depends on x_ij through r2 = x_ij.x_ij)
 __device__ inline double gk_dr2(double r2, const KH<double> &k) {
@@ -185,8 +189,16 @@ WF_KERNEL void k_adj_stress(Dev d, AdjSlot s0, AdjBufs b, double dt, int adj_mat
     b.syb[i] = sybar_old;
     b.plb[i] = plbar;
   }
-  for (int k = 0; k < 6; ++k) b.srb[k * np + i] = SRb[k];
-  for (int k = 0; k < 3; ++k) b.wb[k * np + i] = Wb[k];
+  b.rsb[3 * i] = make_double4(SRb[XX], SRb[YY], SRb[ZZ], SRb[XY]);
+  b.rsb[3 * i + 1] = make_double4(SRb[XZ], SRb[YZ], Wb[0], Wb[1]);
+  b.rsb[3 * i + 2] = make_double4(Wb[2], 0.0, 0.0, 0.0);
+}
+
+// the packed rate-adjoint record of particle j: strain rate [6], rotation [3]
+__device__ inline void rate_seeds(const AdjBufs &b, int j, double sb[6], double wb[3]) {
+  const double4 a = b.rsb[3 * j], c = b.rsb[3 * j + 1], e = b.rsb[3 * j + 2];
+  sb[XX] = a.x, sb[YY] = a.y, sb[ZZ] = a.z, sb[XY] = a.w, sb[XZ] = c.x, sb[YZ] = c.y;
+  wb[0] = c.z, wb[1] = c.w, wb[2] = e.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -194,24 +206,30 @@ WF_KERNEL void k_adj_stress(Dev d, AdjSlot s0, AdjBufs b, double dt, int adj_mat
 // x', v', rho', then the drift x' = x + v' dt and the density update
 // rho' = rho (1 + dt D) backwards
 
-WF_KERNEL void k_adj_rates(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, double dt) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= d.n) return;
+// LANES threads per particle take alternate row entries; their partial
+// sums are added in lane order
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_adj_rates(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, double dt) {
+  __shared__ double red[(LANES > 1 ? LANES - 1 : 1) * 7 * kBlock];
+  const int sub = threadIdx.x / kBlock;
+  const int i = blockIdx.x * kBlock + (int)(threadIdx.x % kBlock);
   const long long np = d.npad;
-  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, volb = 0.0;
-  const double4 xi = s1.xh[i], vi = s1.vm[i];
-  const double rhoi1 = s1.rho[i];
-  if (d.group[i] >= 0) {
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // xb[3], vb[3], volb
+  const bool on = i < d.n;
+  const double4 xi = on ? s1.xh[i] : make_double4(0, 0, 0, 1), vi = on ? s1.vm[i] : make_double4(0, 0, 0, 0);
+  const double rhoi1 = on ? s1.rho[i] : 1.0;
+  if (on && d.group[i] >= 0) {
     const double voli = vi.w / rhoi1;
     double sbi[6], wbi[3];
-    for (int k = 0; k < 6; ++k) sbi[k] = b.srb[k * np + i];
-    for (int k = 0; k < 3; ++k) wbi[k] = b.wb[k * np + i];
+    rate_seeds(b, i, sbi, wbi);
     const KH<double> ki = kern_h_t<double>(xi.w);
     const int cnt = s0.nnb[i];
-    for (int q = 0; q < cnt; ++q) {
+    for (int q = sub; q < cnt; q += LANES) {
       const int j = s0.nbr[nb_slot(np, q, i)];
       const double4 xj = s1.xh[j], vj = s1.vm[j];
       const double volj = vj.w / s1.rho[j];
+      double sbj[6], wbj[3];
+      rate_seeds(b, j, sbj, wbj);
       const KH<double> kh = pair_h_t<double>(ki, xi.w, xj.w);
       const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
       const double r2 = dx * dx + dy * dy + dz * dz;
@@ -222,16 +240,14 @@ WF_KERNEL void k_adj_rates(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, double dt) 
       const double w[3] = {vx * dy - vy * dx, vx * dz - vz * dx, vy * dz - vz * dy};
       double eb[6], wbv[3], cpb = 0.0;
       for (int k = 0; k < 6; ++k) {
-        const double sbj = b.srb[k * np + j];
-        eb[k] = volj * sbi[k] + voli * sbj;
+        eb[k] = volj * sbi[k] + voli * sbj[k];
         cpb += eb[k] * e[k];
-        volb += sbj * cp * e[k];
+        acc[6] += sbj[k] * cp * e[k];
       }
       for (int k = 0; k < 3; ++k) {
-        const double wbj = b.wb[k * np + j];
-        wbv[k] = volj * wbi[k] + voli * wbj;
+        wbv[k] = volj * wbi[k] + voli * wbj[k];
         cpb += wbv[k] * w[k];
-        volb += wbj * cp * w[k];
+        acc[6] += wbj[k] * cp * w[k];
       }
       for (int k = 0; k < 6; ++k) eb[k] *= cp;
       for (int k = 0; k < 3; ++k) wbv[k] *= cp;
@@ -245,10 +261,13 @@ WF_KERNEL void k_adj_rates(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, double dt) 
       bdx += 2 * r2b * dx;
       bdy += 2 * r2b * dy;
       bdz += 2 * r2b * dz;
-      vb[0] += bvx, vb[1] += bvy, vb[2] += bvz;
-      xb[0] += bdx, xb[1] += bdy, xb[2] += bdz;
+      acc[3] += bvx, acc[4] += bvy, acc[5] += bvz;
+      acc[0] += bdx, acc[1] += bdy, acc[2] += bdz;
     }
   }
+  combine_lanes<LANES>(acc, red);
+  if (!on || sub != 0) return;
+  const double xb[3] = {acc[0], acc[1], acc[2]}, vb[3] = {acc[3], acc[4], acc[5]}, volb = acc[6];
   // x' and v' adjoints: incoming state adjoint plus the gather
   const double4 xin = b.xb[i], vin = b.vb[i];
   const double xpb[3] = {xin.x + xb[0], xin.y + xb[1], xin.z + xb[2]};
@@ -269,19 +288,23 @@ WF_KERNEL void k_adj_rates(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, double dt) 
 // cpp:241-273) with the velocity BCs (Source/WeldForm.cpp:51-92) and the
 // zone-reaction seeds of the observer
 
-WF_KERNEL void k_adj_dens(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, BcTable bc, AdjSeeds seeds, double kick) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= d.n) return;
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_adj_dens(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, BcTable bc,
+                                                            AdjSeeds seeds, double kick) {
+  __shared__ double red[(LANES > 1 ? LANES - 1 : 1) * 7 * kBlock];
+  const int sub = threadIdx.x / kBlock;
+  const int i = blockIdx.x * kBlock + (int)(threadIdx.x % kBlock);
   const long long np = d.npad;
-  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, volb = 0.0;
-  const double4 xi = s0.xh[i], vi = s1.vm[i];
-  const double rhoi = s0.rho[i];
-  if (d.group[i] >= 0) {
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};  // xb[3], vb[3], volb
+  const bool on = i < d.n;
+  const double4 xi = on ? s0.xh[i] : make_double4(0, 0, 0, 1), vi = on ? s1.vm[i] : make_double4(0, 0, 0, 0);
+  const double rhoi = on ? s0.rho[i] : 1.0;
+  if (on && d.group[i] >= 0) {
     const double voli = vi.w / rhoi;
     const double dbi = b.db[i];
     const KH<double> ki = kern_h_t<double>(xi.w);
     const int cnt = s0.nnb[i];
-    for (int q = 0; q < cnt; ++q) {
+    for (int q = sub; q < cnt; q += LANES) {
       const int j = s0.nbr[nb_slot(np, q, i)];
       const double4 xj = s0.xh[j], vj = s1.vm[j];
       const double volj = vj.w / s0.rho[j];
@@ -293,14 +316,17 @@ WF_KERNEL void k_adj_dens(Dev d, AdjSlot s0, AdjSlot s1, AdjBufs b, BcTable bc, 
       const double vd = vx * dx + vy * dy + vz * dz;
       const double dbj = b.db[j];
       const double wb = volj * dbi + voli * dbj;
-      volb += dbj * GK * vd;
-      vb[0] += wb * GK * dx, vb[1] += wb * GK * dy, vb[2] += wb * GK * dz;
+      acc[6] += dbj * GK * vd;
+      acc[3] += wb * GK * dx, acc[4] += wb * GK * dy, acc[5] += wb * GK * dz;
       const double r2b = wb * vd * gk_dr2(r2, kh);
-      xb[0] += wb * GK * vx + 2 * r2b * dx;
-      xb[1] += wb * GK * vy + 2 * r2b * dy;
-      xb[2] += wb * GK * vz + 2 * r2b * dz;
+      acc[0] += wb * GK * vx + 2 * r2b * dx;
+      acc[1] += wb * GK * vy + 2 * r2b * dy;
+      acc[2] += wb * GK * vz + 2 * r2b * dz;
     }
   }
+  combine_lanes<LANES>(acc, red);
+  if (!on || sub != 0) return;
+  const double xb[3] = {acc[0], acc[1], acc[2]}, vb[3] = {acc[3], acc[4], acc[5]}, volb = acc[6];
   double4 x0 = b.xb[i];
   b.xb[i] = make_double4(x0.x + xb[0], x0.y + xb[1], x0.z + xb[2], 0.0);
   b.rhob[i] += volb * (-vi.w / (rhoi * rhoi));
@@ -474,80 +500,100 @@ WF_KERNEL void k_adj_contact(Dev d, AdjSlot s0, AdjBufs b, AdjSeeds seeds, doubl
 // acceleration gather (CalcAccel, Source/InteractionAlt.cpp:12-260) at x, v,
 // rho, sigma = S - P(rho) I backwards
 
-__device__ inline void sigma_t(const Dev &d, const AdjSlot &s, int j, double t[6], double &rho, double &irho2) {
+// the step's sigma / rho^2 (sigma = S - P(rho) I) packed per particle for
+// the acceleration gather (two records, see AdjBufs::tsig)
+WF_KERNEL void k_adj_tsig(Dev d, AdjSlot s0, AdjBufs b) {
+  const int j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= d.n) return;
   const DevMat &M = d.mats[d.mat[j]];
-  rho = s.rho[j];
+  const double rho = s0.rho[j];
   const double P = M.P0 + (M.Cs * M.Cs) * (rho - M.ref_density);
-  irho2 = 1.0 / (rho * rho);
+  const double irho2 = 1.0 / (rho * rho);
   const long long np = d.npad;
+  double t[6];
   for (int k = 0; k < 6; ++k) {
-    const double sg = k < 3 ? -P + s.S[k * np + j] : s.S[k * np + j];
+    const double sg = k < 3 ? -P + s0.S[k * np + j] : s0.S[k * np + j];
     t[k] = sg * irho2;
   }
+  b.tsig[2 * j] = make_double4(t[XX], t[YY], t[ZZ], t[XY]);
+  b.tsig[2 * j + 1] = make_double4(t[XZ], t[YZ], rho, irho2);
+}
+__device__ inline void sigma_t(const AdjBufs &b, int j, double t[6], double &rho, double &irho2) {
+  const double4 a = b.tsig[2 * j], c = b.tsig[2 * j + 1];
+  t[XX] = a.x, t[YY] = a.y, t[ZZ] = a.z, t[XY] = a.w, t[XZ] = c.x, t[YZ] = c.y;
+  rho = c.z;
+  irho2 = c.w;
 }
 
-WF_KERNEL void k_adj_accel(Dev d, AdjSlot s0, AdjBufs b) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= d.n) return;
-  if (d.group[i] < 0) return;
+template <int LANES>
+__global__ __launch_bounds__(kBlock *LANES) void k_adj_accel(Dev d, AdjSlot s0, AdjBufs b) {
+  __shared__ double red[(LANES > 1 ? LANES - 1 : 1) * 13 * kBlock];
+  const int sub = threadIdx.x / kBlock;
+  const int i = blockIdx.x * kBlock + (int)(threadIdx.x % kBlock);
+  const bool on = i < d.n && d.group[i] >= 0;
   const long long np = d.npad;
-  const DevMat &M = d.mats[d.mat[i]];
-  const double alpha = M.alpha, beta = M.beta, acs = M.alpha * M.Cs;
+  const DevMat &M = d.mats[on ? d.mat[i] : 0];
+  const double beta = M.beta, acs = M.alpha * M.Cs;
   const bool av = (M.alpha != 0.0 || M.beta != 0.0);
-  const double4 xi = s0.xh[i], vi = s0.vm[i], abi = b.ab[i];
-  double ti[6], rhoi, irho2i;
-  sigma_t(d, s0, i, ti, rhoi, irho2i);
-  (void)alpha;
-  const KH<double> ki = kern_h_t<double>(xi.w);
-  double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, tb[6] = {0, 0, 0, 0, 0, 0}, rhob = 0.0;
-  const int cnt = s0.nnb[i];
-  for (int q = 0; q < cnt; ++q) {
-    const int j = s0.nbr[nb_slot(np, q, i)];
-    const double4 xj = s0.xh[j], vj = s0.vm[j], abj = b.ab[j];
-    double tj[6], rhoj, irho2j;
-    sigma_t(d, s0, j, tj, rhoj, irho2j);
-    const KH<double> kh = pair_h_t<double>(ki, xi.w, xj.w);
-    const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const double GK = gk_t<double>(r2, kh);
-    const double vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-    const double vdot = vx * dx + vy * dy + vz * dz;
-    const double D1 = r2 + 0.01 * kh.h * kh.h, D2 = 0.5 * (rhoi + rhoj);
-    const double hv = kh.h * vdot / (D1 * D2);
-    const double pv = hv * (acs + beta * (hv * D2));
-    const bool act = av && vdot < 0.0;
-    const double pi = act ? pv : 0.0;
-    const double Mxx = ti[XX] + tj[XX] + pi, Myy = ti[YY] + tj[YY] + pi, Mzz = ti[ZZ] + tj[ZZ] + pi;
-    const double Mxy = ti[XY] + tj[XY], Mxz = ti[XZ] + tj[XZ], Myz = ti[YZ] + tj[YZ];
-    // combined seed of the pair term P_ij = GK M x_ij (a_i += m_j P, a_j -= m_i P)
-    const double cx = vj.w * abi.x - vi.w * abj.x, cy = vj.w * abi.y - vi.w * abj.y,
-                 cz = vj.w * abi.z - vi.w * abj.z;
-    const double mdx = Mxx * dx + Mxy * dy + Mxz * dz, mdy = Mxy * dx + Myy * dy + Myz * dz,
-                 mdz = Mxz * dx + Myz * dy + Mzz * dz;
-    const double GKb = cx * mdx + cy * mdy + cz * mdz;
-    const double mb[6] = {GK * cx * dx, GK * cy * dy, GK * cz * dz, GK * (cx * dy + cy * dx), GK * (cx * dz + cz * dx),
-                          GK * (cy * dz + cz * dy)};
-    for (int k = 0; k < 6; ++k) tb[k] += mb[k];
-    double bdx = GK * (Mxx * cx + Mxy * cy + Mxz * cz);
-    double bdy = GK * (Mxy * cx + Myy * cy + Myz * cz);
-    double bdz = GK * (Mxz * cx + Myz * cy + Mzz * cz);
-    double r2b = GKb * gk_dr2(r2, kh);
-    if (act) {
-      const double pib = mb[0] + mb[1] + mb[2];
-      const double hvb = pib * (acs + 2.0 * beta * hv * D2);
-      double D2b = pib * beta * hv * hv;
-      const double vdb = hvb * kh.h / (D1 * D2);
-      const double D1b = -hvb * hv / D1;
-      D2b += -hvb * hv / D2;
-      r2b += D1b;
-      rhob += 0.5 * D2b;
-      vb[0] += vdb * dx, vb[1] += vdb * dy, vb[2] += vdb * dz;
-      bdx += vdb * vx, bdy += vdb * vy, bdz += vdb * vz;
+  double acc[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // xb[3], vb[3], tb[6], rhob
+  double ti[6] = {0, 0, 0, 0, 0, 0}, rhoi = 1.0, irho2i = 1.0;
+  if (on) {
+    const double4 xi = s0.xh[i], vi = s0.vm[i], abi = b.ab[i];
+    sigma_t(b, i, ti, rhoi, irho2i);
+    const KH<double> ki = kern_h_t<double>(xi.w);
+    const int cnt = s0.nnb[i];
+    for (int q = sub; q < cnt; q += LANES) {
+      const int j = s0.nbr[nb_slot(np, q, i)];
+      const double4 xj = s0.xh[j], vj = s0.vm[j], abj = b.ab[j];
+      double tj[6], rhoj, irho2j;
+      sigma_t(b, j, tj, rhoj, irho2j);
+      const KH<double> kh = pair_h_t<double>(ki, xi.w, xj.w);
+      const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double GK = gk_t<double>(r2, kh);
+      const double vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+      const double vdot = vx * dx + vy * dy + vz * dz;
+      const double D1 = r2 + 0.01 * kh.h * kh.h, D2 = 0.5 * (rhoi + rhoj);
+      const double hv = kh.h * vdot / (D1 * D2);
+      const double pv = hv * (acs + beta * (hv * D2));
+      const bool act = av && vdot < 0.0;
+      const double pi = act ? pv : 0.0;
+      const double Mxx = ti[XX] + tj[XX] + pi, Myy = ti[YY] + tj[YY] + pi, Mzz = ti[ZZ] + tj[ZZ] + pi;
+      const double Mxy = ti[XY] + tj[XY], Mxz = ti[XZ] + tj[XZ], Myz = ti[YZ] + tj[YZ];
+      // combined seed of the pair term P_ij = GK M x_ij (a_i += m_j P, a_j -= m_i P)
+      const double cx = vj.w * abi.x - vi.w * abj.x, cy = vj.w * abi.y - vi.w * abj.y,
+                   cz = vj.w * abi.z - vi.w * abj.z;
+      const double mdx = Mxx * dx + Mxy * dy + Mxz * dz, mdy = Mxy * dx + Myy * dy + Myz * dz,
+                   mdz = Mxz * dx + Myz * dy + Mzz * dz;
+      const double GKb = cx * mdx + cy * mdy + cz * mdz;
+      const double mb[6] = {GK * cx * dx, GK * cy * dy, GK * cz * dz, GK * (cx * dy + cy * dx), GK * (cx * dz + cz * dx),
+                            GK * (cy * dz + cz * dy)};
+      for (int k = 0; k < 6; ++k) acc[6 + k] += mb[k];
+      double bdx = GK * (Mxx * cx + Mxy * cy + Mxz * cz);
+      double bdy = GK * (Mxy * cx + Myy * cy + Myz * cz);
+      double bdz = GK * (Mxz * cx + Myz * cy + Mzz * cz);
+      double r2b = GKb * gk_dr2(r2, kh);
+      if (act) {
+        const double pib = mb[0] + mb[1] + mb[2];
+        const double hvb = pib * (acs + 2.0 * beta * hv * D2);
+        double D2b = pib * beta * hv * hv;
+        const double vdb = hvb * kh.h / (D1 * D2);
+        const double D1b = -hvb * hv / D1;
+        D2b += -hvb * hv / D2;
+        r2b += D1b;
+        acc[12] += 0.5 * D2b;
+        acc[3] += vdb * dx, acc[4] += vdb * dy, acc[5] += vdb * dz;
+        bdx += vdb * vx, bdy += vdb * vy, bdz += vdb * vz;
+      }
+      acc[0] += bdx + 2 * r2b * dx;
+      acc[1] += bdy + 2 * r2b * dy;
+      acc[2] += bdz + 2 * r2b * dz;
     }
-    xb[0] += bdx + 2 * r2b * dx;
-    xb[1] += bdy + 2 * r2b * dy;
-    xb[2] += bdz + 2 * r2b * dz;
   }
+  combine_lanes<LANES>(acc, red);
+  if (!on || sub != 0) return;
+  const double *tb = acc + 6;
+  double rhob = acc[12];
   // t = sigma / rho^2, sigma = S - P(rho) I
   double sgb[6], tsig = 0.0;
   for (int k = 0; k < 6; ++k) {
@@ -559,18 +605,20 @@ WF_KERNEL void k_adj_accel(Dev d, AdjSlot s0, AdjBufs b) {
   for (int k = 0; k < 6; ++k) b.Sb[k * np + i] += sgb[k];
   b.rhob[i] += rhob;
   const double4 xo = b.xb[i], vo = b.vb[i];
-  b.xb[i] = make_double4(xo.x + xb[0], xo.y + xb[1], xo.z + xb[2], 0.0);
-  b.vb[i] = make_double4(vo.x + vb[0], vo.y + vb[1], vo.z + vb[2], 0.0);
+  b.xb[i] = make_double4(xo.x + acc[0], xo.y + acc[1], xo.z + acc[2], 0.0);
+  b.vb[i] = make_double4(vo.x + acc[3], vo.y + acc[4], vo.z + acc[5], 0.0);
 }
 
 void launch_adj_step(const Dev &d, hipStream_t st, const AdjSlot &s0, const AdjSlot &s1, const AdjBufs &b,
                      const BcTable &bc, const AdjSeeds &seeds, double dt, double kick, int adj_mat) {
   const int nb = nblk(d.n);
+  constexpr int L = kAdjLanes;
+  k_adj_tsig<<<nb, kBlock, 0, st>>>(d, s0, b);
   k_adj_stress<<<nb, kBlock, 0, st>>>(d, s0, b, dt, adj_mat);
-  k_adj_rates<<<nb, kBlock, 0, st>>>(d, s0, s1, b, dt);
-  k_adj_dens<<<nb, kBlock, 0, st>>>(d, s0, s1, b, bc, seeds, kick);
+  k_adj_rates<L><<<nb, kBlock * L, 0, st>>>(d, s0, s1, b, dt);
+  k_adj_dens<L><<<nb, kBlock * L, 0, st>>>(d, s0, s1, b, bc, seeds, kick);
   if (d.contact) k_adj_contact<<<nb, kBlock, 0, st>>>(d, s0, b, seeds, dt);
-  k_adj_accel<<<nb, kBlock, 0, st>>>(d, s0, b);
+  k_adj_accel<L><<<nb, kBlock * L, 0, st>>>(d, s0, b);
 }
 
 // parameter adjoints: fixed-order block partials, then one block

@@ -1020,8 +1020,8 @@ void adj_prepare(Ctx *c, int k, size_t ncp) {
     b.Sb = c->adj_alloc<double>(6 * np);
     b.syb = c->adj_alloc<double>(np);
     b.plb = c->adj_alloc<double>(np);
-    b.srb = c->adj_alloc<double>(6 * np);
-    b.wb = c->adj_alloc<double>(3 * np);
+    b.rsb = c->adj_alloc<double4>(3 * np);
+    b.tsig = c->adj_alloc<double4>(2 * np);
     b.db = c->adj_alloc<double>(np);
     b.thb = c->adj_alloc<double>((size_t)kAdjParams * np);
     b.part = c->adj_alloc<double>((size_t)kAdjParams * c->d.nblocks);

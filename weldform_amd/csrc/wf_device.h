@@ -68,6 +68,22 @@ __device__ inline double block_reduce(double v, double *sh) {
   return r;
 }
 
+// lanes > 1: lane `sub` > 0 of each particle hands its NV partial sums to
+// lane 0 through LDS (red: (LANES - 1) * NV * kBlock values); lane 0 adds
+// them in lane order.  Every thread of the block must call this.
+template <int LANES, int NV, typename R>
+__device__ inline void combine_lanes(R (&v)[NV], R *red) {
+  if constexpr (LANES > 1) {
+    const int lane = threadIdx.x % kBlock, sub = threadIdx.x / kBlock;
+    if (sub > 0)
+      for (int c = 0; c < NV; ++c) red[((sub - 1) * NV + c) * kBlock + lane] = v[c];
+    __syncthreads();
+    if (sub == 0)
+      for (int q = 1; q < LANES; ++q)
+        for (int c = 0; c < NV; ++c) v[c] += red[((q - 1) * NV + c) * kBlock + lane];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // kernel functions (Source/Functions.cpp:74-87) and the neighbour predicate
 

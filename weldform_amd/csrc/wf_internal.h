@@ -276,7 +276,11 @@ struct AdjBufs {
   double4 *xb, *vb;            // adjoint of the state x, v
   double4 *ab, *vpb;           // adjoint of the acceleration and of the kicked velocity
   double *rhob, *Sb, *syb, *plb;
-  double *srb, *wb, *db;       // adjoint of strain rate [6], rotation rate [3], density-rate sum
+  double4 *rsb;                // adjoints of the strain rate and rotation rate, 3 records per
+                               // particle: (xx, yy, zz, xy), (xz, yz, w01, w02), (w12, 0, 0, 0)
+  double *db;                  // adjoint of the density-rate sum
+  double4 *tsig;               // the step's sigma / rho^2, 2 records per particle:
+                               // (xx, yy, zz, xy), (xz, yz, rho, 1 / rho^2)
   double *thb;                 // per-particle parameter adjoints [kAdjParams][npad]
   double *part;                // [kAdjParams][nblocks] block partials
 };

@@ -662,22 +662,6 @@ __device__ inline void for_each_nb(const G &g, int cnt, int sub, int lanes, F &&
   }
 }
 
-// lanes > 1: lane `sub` > 0 of each particle hands its NV partial sums to
-// lane 0 through LDS (red: (LANES - 1) * NV * kBlock values); lane 0 adds
-// them in lane order.  Every thread of the block must call this.
-template <int LANES, int NV, typename R>
-__device__ inline void combine_lanes(R (&v)[NV], R *red) {
-  if constexpr (LANES > 1) {
-    const int lane = threadIdx.x % kBlock, sub = threadIdx.x / kBlock;
-    if (sub > 0)
-      for (int c = 0; c < NV; ++c) red[((sub - 1) * NV + c) * kBlock + lane] = v[c];
-    __syncthreads();
-    if (sub == 0)
-      for (int q = 1; q < LANES; ++q)
-        for (int c = 0; c < NV; ++c) v[c] += red[((q - 1) * NV + c) * kBlock + lane];
-  }
-}
-
 // copy the tile's staged set into LDS (all NT threads of the block call
 // these).  A thread's share (entries p = tid + r NT) is loaded in three
 // rounds, each with all of the share's loads in flight: the set's particle
