@@ -27,7 +27,7 @@
 
 namespace wf {
 
-constexpr int kAdjLanes = 2;  // threads per particle in the reverse gathers
+constexpr int kAdjLanes = 4;  // threads per particle in the reverse gathers
 
 // d GK / d r2 of gk_t (the pair term depends on x_ij through r2 = x_ij.x_ij)
 __device__ inline double gk_dr2(double r2, const KH<double> &k) {
