@@ -348,6 +348,13 @@ int wf_eval(void *ctx, int32_t what, double dt);
 int wf_set_timing(void *ctx, int32_t enable);
 int wf_get_kernel_time(void *ctx, int32_t kernel, double *avg_ms, int64_t *launches);
 int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_count);
+/* cells per axis of the reference grid (CellInitiate, Source/Neighbour.cpp:
+ * 10-139: sized by h_max over all particles, rigid ones included; the
+ * contact partners and the thermal pair orientation use it) and of the grid
+ * the SPH neighbour search runs on (cells at least Cellfac x the largest SPH
+ * h wide, finer than the reference's when rigid particles are larger; the
+ * same pairs are found).  Diagnostic; no counterpart in the reference. */
+int wf_get_cell_grids(void *ctx, int32_t reference[3], int32_t search[3]);
 
 /* output tables, packed on the device in the caller's particle order and
  * copied to the host in one transfer (the first call allocates a device

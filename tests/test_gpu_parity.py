@@ -59,6 +59,34 @@ def test_neighbour_sets_jittered():
     assert np.array_equal(s.neighbour_pairs().astype(np.int64), unordered(d.pairs()))
 
 
+def test_neighbour_sets_with_coarse_rigid_planes():
+    """Rigid planes of 2 x 8^2 triangles (rigid h ~2.3x the billet's): the
+    reference grid's cells are sized by the rigid h, the SPH search runs on
+    a finer grid, and the pairs are still the reference's, bit for bit; the
+    first 20 contact-free steps then match the oracle in FP64."""
+    c = cases.contact_small(steps=20, dens=8)
+    _log("case", c.name, "n", c.n)
+    s = cases.make_solver(c)
+    s.rebuild_neighbours()
+    ref, grid = s.cell_grids()
+    _log("reference grid", ref, "search grid", grid)
+    assert np.prod(grid) > 2 * np.prod(ref)
+    d = oracle_from_case(c)
+    d.neighbour_search()
+    assert np.array_equal(s.neighbour_pairs().astype(np.int64), unordered(d.pairs()))
+    s.close()
+    g = cases.make_solver(c)
+    g.step(20)
+    g.sync()
+    d = oracle_from_case(c)
+    d.solve_init(c.dt)
+    for _ in range(20):
+        d.step()
+    assert normwise_err(g.get("disp"), d.f("DISP")[:c.n]) < 1e-8
+    assert normwise_err(g.get("sigma"), d.f("SIGMA")[:c.n]) < 1e-8
+    g.close()
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_single_passes_fp64(c1, seed):
     v, rho, sig = _random_state(c1, seed)

@@ -96,7 +96,7 @@ EXPORTS = [
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
     "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
     "wf_set_contact_heat", "wf_set_rigid_temperature", "wf_prepare", "wf_step_part",
-    "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max",
+    "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max", "wf_get_cell_grids",
 ]
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
@@ -146,6 +146,7 @@ def lib():
         "wf_set_timing": ([vp, i32], ctypes.c_int),
         "wf_get_kernel_time": ([vp, i32, dp, I64P], ctypes.c_int),
         "wf_get_neighbour_stats": ([vp, I64P, ip], ctypes.c_int),
+        "wf_get_cell_grids": ([vp, ip, ip], ctypes.c_int),
         "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
         "wf_add_rigid_plane": ([vp, i32, i32, i32, dp, dp, i32, d], ctypes.c_int),
         "wf_set_contact": ([vp, d, d, d], ctypes.c_int),
@@ -420,6 +421,13 @@ class Solver:
         cnt = ctypes.c_int64(0)
         self._check(lib().wf_get_kernel_time(self.ctx, KERNELS[kernel], ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+    def cell_grids(self):
+        """(reference grid cells per axis, SPH search grid cells per axis)."""
+        r = np.zeros(3, dtype=np.int32)
+        g = np.zeros(3, dtype=np.int32)
+        self._check(lib().wf_get_cell_grids(self.ctx, _ip(r), _ip(g)))
+        return r, g
 
     def neighbour_stats(self):
         p = ctypes.c_int64(0)

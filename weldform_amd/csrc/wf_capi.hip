@@ -305,6 +305,37 @@ void cell_initiate(Ctx *c) {
   if (ncells > (1LL << 30)) throw Error(WF_EINVAL, "CellInitiate: too many cells");
   for (int k = 0; k < 3; ++k) c->d.cellsize[k] = (TRPR[k] - BLPF[k]) / c->d.cellno[k];
   c->d.ncells = (int)ncells;
+  // The SPH neighbour search's own grid.  The reference's cells are sized by
+  // h_max over all particles, rigid-surface ones included; with large rigid
+  // particles a cell then holds hundreds of SPH particles.  SPH pairs only
+  // form within R = Cellfac x h_max over the SPH particles.  When every
+  // reference cell is at least R wide, the reference finds every SPH pair
+  // within R (clamped particles included: a pair within R lies in adjacent
+  // clamped cells of any grid of cells at least R wide), so a finer grid of
+  // cells at least R wide over the same box finds the same pairs.  It is
+  // used when its cells are at most 2/3 of the reference's along some axis.
+  double hs = 0.0;
+  for (int64_t i = 0; i < c->n; ++i) hs = std::max(hs, h[i]);
+  const double R = c->cfg.cellfac * hs;
+  bool finer = R > 0.0;
+  long long sn = 1;
+  int scn[3];
+  for (int k = 0; k < 3; ++k) {
+    if (c->d.cellsize[k] < R) finer = false;
+    scn[k] = std::max(1, (int)std::floor((TRPR[k] - BLPF[k]) / R));
+    sn *= scn[k];
+  }
+  bool gain = false;
+  for (int k = 0; k < 3; ++k) gain = gain || (TRPR[k] - BLPF[k]) / scn[k] <= c->d.cellsize[k] * (2.0 / 3.0);
+  if (!(finer && gain && sn <= (1LL << 28))) {
+    for (int k = 0; k < 3; ++k) scn[k] = c->d.cellno[k];
+    sn = ncells;
+  }
+  for (int k = 0; k < 3; ++k) {
+    c->d.scellno[k] = scn[k];
+    c->d.scellsize[k] = (TRPR[k] - BLPF[k]) / scn[k];
+  }
+  c->d.sncells = (int)sn;
 }
 
 uint64_t spread3(uint64_t v) {
@@ -416,9 +447,10 @@ void allocate(Ctx *c) {
   int **ints[] = {&d.id, &d.mat, &d.group, &d.thbc, &d.perm, &d.cell, &d.sorted, &d.nnb};
   for (int **p : ints) *p = c->alloc<int>(np);
   d.nbr = c->alloc<int>((size_t)nb_rows(c->cfg.max_nb) * np);
-  d.cell_count = c->alloc<int>((size_t)d.ncells + 1);
-  d.cell_start = c->alloc<int>((size_t)d.ncells + 1);
-  d.cell_cursor = c->alloc<int>((size_t)d.ncells + 1);
+  d.cell_count = c->alloc<int>((size_t)d.sncells + 1);
+  d.cell_start = c->alloc<int>((size_t)d.sncells + 1);
+  d.cell_cursor = c->alloc<int>((size_t)d.sncells + 1);
+  d.scell = c->alloc<int>(np);
   d.sxyz = c->alloc<float4>(np);
   d.sinfo = c->alloc<int2>(np);
   c->dmats = c->alloc<DevMat>(kMaxMat);
@@ -2220,6 +2252,19 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
     auto f = dget(c, c->d.flags, 1)[0];
     if (directed_pairs) *directed_pairs = f.directed_pairs;
     if (max_count) *max_count = f.max_count;
+    return WF_OK;
+  });
+}
+
+int wf_get_cell_grids(void *ctx, int32_t reference[3], int32_t search[3]) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    for (int k = 0; k < 3; ++k) {
+      if (reference) reference[k] = c->d.cellno[k];
+      if (search) search[k] = c->d.scellno[k];
+    }
     return WF_OK;
   });
 }

@@ -195,8 +195,16 @@ struct Dev {
   float4 *vmf;                 // (vx, vy, vz, mass)
   StressRecF *srecf;
   float *prf;                  // P / rho^2 (EOS of the current density)
-  // neighbours
+  // neighbours.  cell: each particle's cell in the reference grid (blpf,
+  // cellsize, cellno: CellInitiate's), which the rigid binning, the contact
+  // partners and the thermal pair orientation use; scell: its cell in the
+  // search grid (same box, scellsize >= Cellfac x the largest SPH h), which
+  // the SPH neighbour build uses and the counting sort below is over
   int *cell, *cell_count, *cell_start, *cell_cursor, *sorted;
+  int *scell;
+  int scellno[3];
+  double scellsize[3];
+  int sncells;
   // cell-ordered candidate records of the last counting sort (position p of
   // `sorted`): offsets of the particle from its cell's lower corner and h in
   // FP32, and (particle index, group)

@@ -121,6 +121,13 @@ __device__ inline int cell_of(const Dev &d, const double4 x) {
   const int cz = cell_coord(x.z, d.blpf[2], d.cellsize[2], d.cellno[2]);
   return (cx * d.cellno[1] + cy) * d.cellno[2] + cz;
 }
+// the particle's cell in the SPH search grid
+__device__ inline int scell_of(const Dev &d, const double4 x) {
+  const int cx = cell_coord(x.x, d.blpf[0], d.scellsize[0], d.scellno[0]);
+  const int cy = cell_coord(x.y, d.blpf[1], d.scellsize[1], d.scellno[1]);
+  const int cz = cell_coord(x.z, d.blpf[2], d.scellsize[2], d.scellno[2]);
+  return (cx * d.scellno[1] + cy) * d.scellno[2] + cz;
+}
 
 // threads 0..n-1: the SPH particles; n..n+nrig-1: the rigid particles.
 // The counts and cursors are zero on entry (allocation, then cleared by
@@ -129,8 +136,10 @@ WF_KERNEL void k_cell_hist(Dev d, int cur) {
   if (!d.flags->do_search) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < d.n) {
-    const int c = cell_of(d, d.xh[cur][i]);
-    d.cell[i] = c;
+    const double4 x = d.xh[cur][i];
+    d.cell[i] = cell_of(d, x);
+    const int c = scell_of(d, x);
+    d.scell[i] = c;
     atomicAdd(&d.cell_count[c], 1);
   } else if (i - d.n < d.nrig) {
     const int r = i - d.n;
@@ -226,7 +235,7 @@ __global__ __launch_bounds__(1024) void k_scan(Dev d) {
   if (!d.flags->do_search) return;
   __shared__ int sh[34];
   __shared__ int buf[kScanRound + kScanRound / 16];
-  scan_cells(d.cell_count, d.cell_start, d.ncells, sh, buf);
+  scan_cells(d.cell_count, d.cell_start, d.sncells, sh, buf);
   if (d.nrig) scan_cells(d.rcount, d.rstart, d.ncells, sh, buf);
 }
 
@@ -234,7 +243,7 @@ WF_KERNEL void k_scatter(Dev d) {
   if (!d.flags->do_search) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < d.n) {
-    const int c = d.cell[i];
+    const int c = d.scell[i];
     d.sorted[d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1)] = i;
   } else if (i - d.n < d.nrig) {
     const int r = i - d.n, c = d.rcell[r];
@@ -250,13 +259,13 @@ WF_KERNEL void k_scatter(Dev d) {
 WF_KERNEL void k_cell_sort(Dev d, int cur) {
   if (!d.flags->do_search) return;
   const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (c >= d.ncells) return;
-  d.cell_count[c] = 0;
-  d.cell_cursor[c] = 0;
-  if (d.nrig) {
+  if (d.nrig && c < d.ncells) {
     d.rcount[c] = 0;
     d.rcursor[c] = 0;
   }
+  if (c >= d.sncells) return;
+  d.cell_count[c] = 0;
+  d.cell_cursor[c] = 0;
   const int s = d.cell_start[c], e = d.cell_start[c + 1];
   for (int a = s + 1; a < e; ++a) {
     const int v = d.sorted[a];
@@ -267,10 +276,10 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
     }
     d.sorted[b + 1] = v;
   }
-  const int nz = d.cellno[2], ny = d.cellno[1];
+  const int nz = d.scellno[2], ny = d.scellno[1];
   const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
-  const double ox = d.blpf[0] + cx * d.cellsize[0], oy = d.blpf[1] + cy * d.cellsize[1],
-               oz = d.blpf[2] + cz * d.cellsize[2];
+  const double ox = d.blpf[0] + cx * d.scellsize[0], oy = d.blpf[1] + cy * d.scellsize[1],
+               oz = d.blpf[2] + cz * d.scellsize[2];
   for (int p = s; p < e; ++p) {
     const int j = d.sorted[p];
     const double4 x = d.xh[cur][j];
@@ -281,8 +290,9 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
 
 // One lane per particle (lanes in cell order) over the 27 cells around the
 // particle's own, in cell order (each cell's particles by index), so a row
-// lists its neighbours in a fixed order.  The 27 cells are 9 columns of up to three consecutive cell ids,
-// whose candidates are one contiguous range of the cell-ordered records; a
+// lists its neighbours in a fixed order (cells of the SPH search grid).  The
+// 27 cells are 9 columns of up to three consecutive cell ids, whose
+// candidates are one contiguous range of the cell-ordered records; a
 // range is walked four candidates at a time, their loads issued together.
 // The radius test runs in FP32 on offsets from the cell corners (|error| <
 // 1e-6 relative at the radius) and is trusted outside a band of 1e-4
@@ -304,13 +314,13 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
   if (i < d.n && d.group[i] >= 0) {
     const int g = d.group[i];
     const double4 xi = d.xh[cur][i];
-    const int c = d.cell[i];
-    const int nz = d.cellno[2], ny = d.cellno[1], nx = d.cellno[0];
+    const int c = d.scell[i];
+    const int nz = d.scellno[2], ny = d.scellno[1], nx = d.scellno[0];
     const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
-    const float csx = (float)d.cellsize[0], csy = (float)d.cellsize[1], csz = (float)d.cellsize[2];
-    const float oxi = (float)(xi.x - (d.blpf[0] + cx * d.cellsize[0]));
-    const float oyi = (float)(xi.y - (d.blpf[1] + cy * d.cellsize[1]));
-    const float ozi = (float)(xi.z - (d.blpf[2] + cz * d.cellsize[2]));
+    const float csx = (float)d.scellsize[0], csy = (float)d.scellsize[1], csz = (float)d.scellsize[2];
+    const float oxi = (float)(xi.x - (d.blpf[0] + cx * d.scellsize[0]));
+    const float oyi = (float)(xi.y - (d.blpf[1] + cy * d.scellsize[1]));
+    const float ozi = (float)(xi.z - (d.blpf[2] + cz * d.scellsize[2]));
     const float hi = (float)xi.w, hcf = 0.5f * (float)d.cellfac;
     const bool exact_all = fabsf(oxi) > 64.f * csx || fabsf(oyi) > 64.f * csy || fabsf(ozi) > 64.f * csz;
     const int az0 = cz > 0 ? cz - 1 : 0, az1 = cz + 1 < nz ? cz + 1 : nz - 1, m = az1 - az0;
@@ -587,7 +597,7 @@ void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
   k_cell_hist<<<nblk((long long)d.n + d.nrig), kBlock, 0, st>>>(d, cur);
   k_scan<<<1, 1024, 0, st>>>(d);
   k_scatter<<<nblk((long long)d.n + d.nrig), kBlock, 0, st>>>(d);
-  k_cell_sort<<<nblk(d.ncells), kBlock, 0, st>>>(d, cur);
+  k_cell_sort<<<nblk(d.sncells > d.ncells ? d.sncells : d.ncells), kBlock, 0, st>>>(d, cur);
   k_build_nbr<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
   if (d.stage_max > 0) k_stage<<<d.ntiles < 2048 ? d.ntiles : 2048, kStageThreads, 0, st>>>(d);
 }
