@@ -69,6 +69,20 @@ def profiled_traffic(kernel):
     return None
 
 
+def profiled_valu(kernel):
+    """(VALU issue share, VALU wave-instructions per launch) of `kernel` from
+    the committed rocprofv3 counter summary, or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "pair_pass_counters_cyl_upset_976k.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 4 and parts[0] == "valu_issue" and parts[1] == kernel:
+                return float(parts[2]), float(parts[3])
+    return None
+
+
 def timed_run(case, precision, steps, warmup):
     """Graph-replay timed region for the throughput, then the same steps again
     launched one by one with HIP events around each pair kernel (per-kernel
@@ -511,9 +525,14 @@ def main():
             "frac": ach / HBM_PEAK_GBS,
             "traffic": profiled_traffic(DOM_KERNEL[dom]),
             "algorithmic_bytes": algorithmic_bytes(dom, case.n, nbar, True, 2.0),
+            # the pair passes are VALU-bound: share of the SIMDs' issue cycles
+            # their VALU instructions take, from the rocprofv3 counter summary
+            # (profiles/pair_pass_counters_cyl_upset_976k.md)
+            "valu_issue_frac": {k: (profiled_valu(v) or (None,))[0] for k, v in DOM_KERNEL.items()},
             "note": "achieved = algorithmic bytes of one launch / its mean duration (HIP events on the "
                     "solver stream); traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE "
-                    "(profiles/traffic_cyl_upset_976k.md)",
+                    "(profiles/traffic_cyl_upset_976k.md); valu_issue_frac = VALU issue cycles / SIMD cycles "
+                    "at 2.4 GHz (profiles/pair_pass_counters_cyl_upset_976k.md)",
         },
     }
     if not args.no_output:
