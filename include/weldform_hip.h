@@ -276,6 +276,13 @@ int wf_add_rigid_plane(void *ctx, int32_t zone, int32_t axis, int32_t flip, cons
 int wf_add_rigid_mesh(void *ctx, int32_t zone, int64_t nnode, const double *nodes, int64_t nelem,
                       const int32_t *elems, const double *normals, double hfac);
 int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double damping);
+/* Coulomb friction of material `index`'s particles against the rigid
+ * surfaces, in place of wf_set_contact's for them (no counterpart in the
+ * reference, whose friction_sta/dyn belong to the Domain: a batch of
+ * specimens in one context gives each specimen its own, as its own Domain
+ * would).  May be called before or after wf_init; the adjoint refuses
+ * contexts that use it. */
+int wf_set_material_friction(void *ctx, int32_t index, double friction_sta, double friction_dyn);
 /* wf_set_contact_heat <- Domain::cont_heat_cond / contact_hc / cont_heat_fric
  * (Source/Domain.h:339-347, set from a scene's Contact "heatConductance",
  * "heatCondCoeff" and "heatFriction", Source/WeldForm.cpp:711-723): every
@@ -334,6 +341,19 @@ int wf_get_scalars(void *ctx, wf_scalars *out);
  * the reference first calls GeneralAfter in a step (after AccelReduction,
  * before the BC zeroes a).  Copies the last min(cap, steps) entries. */
 int wf_get_reaction_history(void *ctx, int32_t zone, double *out3, int64_t cap, int64_t *count);
+/* Per-group sums (no counterpart in the reference; for a batch of specimens
+ * in one context, one group per material index): call before wf_init.
+ * Every step then keeps, per group, the reaction of velocity-BC zone `zone`
+ * (sum of m a over the group's particles carrying that id, observed where
+ * wf_get_reaction_history's is; zone -1: none) and, with rigid meshes, the
+ * contact force of each mesh on the group (the m_contact_force split by
+ * group), for the last `cap` steps.  Each sum is taken in a fixed order.
+ * wf_get_group_history: what 0 the reaction (mesh ignored), 1 the contact
+ * force of `mesh`; out3 receives the last min(steps, cap) triples, oldest
+ * first. */
+int wf_set_group_sums(void *ctx, int32_t zone, int64_t cap);
+int wf_get_group_history(void *ctx, int32_t what, int32_t group, int32_t mesh, double *out3, int64_t cap,
+                         int64_t *count);
 
 /* neighbours */
 int wf_rebuild_neighbours(void *ctx);

@@ -57,6 +57,7 @@ def lib():
         L.wfo_get_param.restype = d
         L.wfo_get_param.argtypes = [vp, ctypes.c_char_p]
         L.wfo_set_material.argtypes = [vp, i32, ctypes.POINTER(Material)]
+        L.wfo_set_material_friction.argtypes = [vp, i32, d, d]
         L.wfo_set_threads.argtypes = [i32]
         L.wfo_last_error.restype = ctypes.c_char_p
         L.wfo_add_velocity_bc.argtypes = [vp, i32, i32, dp, i32, d]
@@ -83,6 +84,7 @@ def lib():
         L.wfo_add_plane_mesh.argtypes = [vp, i32, i32, i32, i32, dp, dp, i32, d]
         L.wfo_add_mesh.argtypes = [vp, i32, i32, i32, dp, i32, ctypes.POINTER(ctypes.c_int), dp, d]
         L.wfo_contact_force.argtypes = [vp, i32, dp]
+        L.wfo_group_contact_force.argtypes = [vp, i32, i32, dp]
         L.wfo_ncontact_pairs.restype = i64
         L.wfo_ncontact_pairs.argtypes = [vp]
         L.wfo_contact_pairs.restype = ctypes.POINTER(ctypes.c_int)
@@ -167,6 +169,9 @@ class Domain:
     def set_material(self, idx, **kw):
         m = Material(**kw)
         self._check(lib().wfo_set_material(self.h, int(idx), ctypes.byref(m)))
+
+    def set_material_friction(self, idx, sta, dyn):
+        self._check(lib().wfo_set_material_friction(self.h, int(idx), float(sta), float(dyn)))
 
     def add_velocity_bc(self, zone, value, value_type=0, amp_id=0, amp_factor=1.0):
         v = np.ascontiguousarray(value, dtype=np.float64)
@@ -263,6 +268,11 @@ class Domain:
     def contact_force(self, mesh):
         out = np.zeros(3)
         self._check(lib().wfo_contact_force(self.h, mesh, _dp(out)))
+        return out
+
+    def group_contact_force(self, group, mesh):
+        out = np.zeros(3)
+        self._check(lib().wfo_group_contact_force(self.h, group, mesh, _dp(out)))
         return out
 
     def contact_pairs(self):

@@ -63,6 +63,9 @@ struct wfo_domain {
   double *f[WFO_NFIELDS];
   int *fi[WFO_NIFIELDS];
   wfo_material mats[256];
+  /* per-material Coulomb friction (static, dynamic); a negative static
+   * value means the domain's friction_sta / friction_dyn */
+  double mat_fr[256][2];
   /* domain flags / parameters (Source/Domain.cpp:78-175) */
   double Cellfac, DomMax[3], DomMin[3], Time, deltat, pl_work_heat_frac;
   int ts_nb_inc, thermal_solver, pl_heating;
@@ -103,6 +106,9 @@ struct wfo_domain {
   int *rig, *cont;                /* RIGPairs and ContPairs, (temp1, temp2) */
   int64_t nrig, caprig, ncont, capcont;
   double m_contact_force[WFO_MAX_MESH][3], contact_force_sum, contact_reaction_sum, ext_forces_work;
+  /* m_contact_force split by the particle's material index (one group per
+   * specimen of a batch) */
+  double g_contact_force[256][WFO_MAX_MESH][3];
   /* adaptive time step (Source/Domain.cpp:206-319): bit 0 velocity
    * criterion (auto_ts), bit 1 acceleration criterion (auto_ts_acc), bit 2
    * the contact criterion (auto_ts_cont), which acts only with bit 0 or 1 */
@@ -135,6 +141,7 @@ wfo_domain *wfo_create(int n) {
     d->f[WFO_H][i] = 0.0;
     d->f[WFO_TIN][i] = 4.0;
   }
+  for (int k = 0; k < 256; ++k) d->mat_fr[k][0] = d->mat_fr[k][1] = -1.;
   /* Domain constructor defaults (Source/Domain.cpp:78-175) */
   d->Cellfac = 2.0;
   for (int k = 0; k < 3; ++k) { d->DomMax[k] = -100000000000.0; d->DomMin[k] = 100000000000.0; }
@@ -185,6 +192,16 @@ int wfo_set_material(wfo_domain *d, int idx, const wfo_material *m) {
     double eps0 = m->sy0 / m->E, eps1 = pow(m->sy0 / m->K, 1. / m->m);
     if (eps0 > eps1) return fail("Hollomon material bad definition");
   }
+  return 0;
+}
+
+/* Friction per material: the batched ensembles give every specimen its own
+ * Coulomb coefficients; the reference has one pair per domain
+ * (Domain::friction_sta / friction_dyn, Source/Contact_Wang.cpp:250-280). */
+int wfo_set_material_friction(wfo_domain *d, int idx, double sta, double dyn) {
+  if (idx < 0 || idx >= 256) return fail("material index out of range");
+  d->mat_fr[idx][0] = sta;
+  d->mat_fr[idx][1] = dyn;
   return 0;
 }
 
@@ -1660,6 +1677,7 @@ static void contact_forces(wfo_domain *d) {
    * drops */
   double min_force_ts_ = 1000.;
   memset(d->m_contact_force, 0, sizeof d->m_contact_force);
+  memset(d->g_contact_force, 0, sizeof d->g_contact_force);
   d->contact_force_sum = 0.;
   d->contact_reaction_sum = 0.;
   double ext_step = 0.;
@@ -1706,13 +1724,16 @@ static void contact_forces(wfo_domain *d) {
     /* Fraser 3-119 (Source/Contact_Wang.cpp:245-247): the contact area */
     const double dens = rho[P1];
     const double dS2 = pow(m1 / dens, 2.0 / 3.0);
-    if (d->friction_sta > 0.) {
+    const double *mfr = d->mat_fr[d->fi[WFO_MATIDX][P1]];
+    const double fr_sta = mfr[0] >= 0. ? mfr[0] : d->friction_sta;
+    const double fr_dyn = mfr[0] >= 0. ? mfr[1] : d->friction_dyn;
+    if (fr_sta > 0.) {
       double du[3], dtg[3], tgf[3];
       for (int k = 0; k < 3; k++) du[k] = x_pred[k] - x[3 * P1 + k] - v[3 * P2 + k] * dt;
       const double dn = dot3(du, n2);
       for (int k = 0; k < 3; k++) dtg[k] = du[k] - dn * n2[k];
       for (int k = 0; k < 3; k++) tgf[k] = kij * dtg[k];
-      if (norm3(tgf) < d->friction_sta * norm3(cf + 3 * P1)) {
+      if (norm3(tgf) < fr_sta * norm3(cf + 3 * P1)) {
         for (int k = 0; k < 3; k++) {
           cf[3 * P1 + k] -= tgf[k] / m1;
           a[3 * P1 + k] -= tgf[k] / m1;
@@ -1720,7 +1741,7 @@ static void contact_forces(wfo_domain *d) {
       } else {
         const double ncf = norm3(cf + 3 * P1), ntg = norm3(tgf);
         double tgd[3];
-        for (int k = 0; k < 3; k++) tgd[k] = d->friction_dyn * ncf * tgf[k] / ntg;
+        for (int k = 0; k < 3; k++) tgd[k] = fr_dyn * ncf * tgf[k] / ntg;
         for (int k = 0; k < 3; k++) {
           cf[3 * P1 + k] -= tgd[k];
           a[3 * P1 + k] -= tgd[k] / m1;
@@ -1737,6 +1758,7 @@ static void contact_forces(wfo_domain *d) {
     }
     d->contact_force_sum += norm3(cf + 3 * P1);
     for (int k = 0; k < 3; k++) d->m_contact_force[m][k] += cf[3 * P1 + k];
+    for (int k = 0; k < 3; k++) d->g_contact_force[d->fi[WFO_MATIDX][P1]][m][k] += cf[3 * P1 + k];
     d->contact_reaction_sum += dot3(a + 3 * P1, n2) * m1;
     ext_step += dot3(cf + 3 * P1, v + 3 * P2);
     /* contact conductance, Fraser 3.121 (Source/Contact_Wang.cpp:320-324) */
@@ -1785,6 +1807,12 @@ static void mesh_update(wfo_domain *d) {
 int wfo_contact_force(const wfo_domain *d, int mesh, double out[3]) {
   if (mesh < 0 || mesh >= d->nmesh) return fail("mesh index out of range");
   for (int k = 0; k < 3; k++) out[k] = d->m_contact_force[mesh][k];
+  return 0;
+}
+int wfo_group_contact_force(const wfo_domain *d, int group, int mesh, double out[3]) {
+  if (mesh < 0 || mesh >= d->nmesh) return fail("mesh index out of range");
+  if (group < 0 || group >= 256) return fail("group index out of range");
+  for (int k = 0; k < 3; k++) out[k] = d->g_contact_force[group][mesh][k];
   return 0;
 }
 int64_t wfo_ncontact_pairs(const wfo_domain *d) { return d->ncont; }

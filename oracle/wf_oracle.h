@@ -76,6 +76,7 @@ int *wfo_i32(wfo_domain *d, int field);
 int wfo_set_param(wfo_domain *d, const char *name, double v);
 double wfo_get_param(const wfo_domain *d, const char *name);
 int wfo_set_material(wfo_domain *d, int idx, const wfo_material *m);
+int wfo_set_material_friction(wfo_domain *d, int idx, double sta, double dyn);
 void wfo_set_threads(int n);
 const char *wfo_last_error(void);
 
@@ -119,6 +120,7 @@ int wfo_add_mesh(wfo_domain *d, int first, int id, int nnodes, const double *nod
 int wfo_add_plane_mesh(wfo_domain *d, int first, int id, int axis, int flip, const double p1[3],
                        const double p2[3], int dens, double hfac);
 int wfo_contact_force(const wfo_domain *d, int mesh, double out[3]);
+int wfo_group_contact_force(const wfo_domain *d, int group, int mesh, double out[3]);
 int64_t wfo_ncontact_pairs(const wfo_domain *d);
 const int *wfo_contact_pairs(const wfo_domain *d);          /* (SPH, rigid) */
 /* reaction observed at the first BC call of the last step:

@@ -80,6 +80,8 @@ def oracle_from_case(case, threads=1):
                                           m.get("hfac", 1.1))
         d.set_param("friction_sta", case.contact[0])
         d.set_param("friction_dyn", case.contact[1])
+        for idx, (fs, fd) in case.mat_friction.items():
+            d.set_material_friction(idx, fs, fd)
         d.set_param("DFAC", case.contact[2])
         d.set_param("cont_heat_cond", int(bool(case.contact_heat[0])))
         d.set_param("cont_heat_fric", int(bool(case.contact_heat[1])))

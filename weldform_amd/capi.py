@@ -97,6 +97,7 @@ EXPORTS = [
     "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
     "wf_set_contact_heat", "wf_set_rigid_temperature", "wf_prepare", "wf_step_part",
     "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max", "wf_get_cell_grids",
+    "wf_set_material_friction", "wf_set_group_sums", "wf_get_group_history",
 ]
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
@@ -147,6 +148,9 @@ def lib():
         "wf_get_kernel_time": ([vp, i32, dp, I64P], ctypes.c_int),
         "wf_get_neighbour_stats": ([vp, I64P, ip], ctypes.c_int),
         "wf_get_cell_grids": ([vp, ip, ip], ctypes.c_int),
+        "wf_set_material_friction": ([vp, i32, d, d], ctypes.c_int),
+        "wf_set_group_sums": ([vp, i32, i64], ctypes.c_int),
+        "wf_get_group_history": ([vp, i32, i32, i32, dp, i64, I64P], ctypes.c_int),
         "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
         "wf_add_rigid_plane": ([vp, i32, i32, i32, dp, dp, i32, d], ctypes.c_int),
         "wf_set_contact": ([vp, d, d, d], ctypes.c_int),
@@ -421,6 +425,20 @@ class Solver:
         cnt = ctypes.c_int64(0)
         self._check(lib().wf_get_kernel_time(self.ctx, KERNELS[kernel], ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+    def set_material_friction(self, index, friction_sta, friction_dyn):
+        self._check(lib().wf_set_material_friction(self.ctx, index, friction_sta, friction_dyn))
+
+    def set_group_sums(self, zone=-1, cap=4096):
+        self._check(lib().wf_set_group_sums(self.ctx, zone, cap))
+
+    def group_history(self, what, group, mesh=0, cap=1 << 16):
+        """Per-group sums, oldest first: what 0 the zone reaction, 1 the
+        contact force of rigid mesh `mesh` on the group; shape (steps, 3)."""
+        out = np.zeros(3 * cap)
+        cnt = ctypes.c_int64(0)
+        self._check(lib().wf_get_group_history(self.ctx, what, group, mesh, _dp(out), cap, ctypes.byref(cnt)))
+        return out[:3 * cnt.value].reshape(-1, 3)
 
     def cell_grids(self):
         """(reference grid cells per axis, SPH search grid cells per axis)."""

@@ -47,6 +47,8 @@ class Case:
     auto_ts: int = 0            # bit 0 velocity, bit 1 acceleration, bit 2 contact criterion
     cfl: float = 0.7
     sqrt_h_a: float = 0.0025
+    mat_friction: dict = field(default_factory=dict)  # material idx -> (friction_sta, friction_dyn)
+    group_sums: tuple = None    # (zone, cap) of wf_set_group_sums, or None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -230,17 +232,46 @@ def config5(steps=400, dx=0.4675e-3, R=15e-3, L=30e-3, dens=32, v_punch=-5.0, mu
     return c
 
 
-def config4(n_spec=256, steps=500, dx=1.0e-3, n_range=(2000, 40000), v_top=-5.0, seed=2030):
+def patch_mesh(cx, cy, z, half, dens, down):
+    """A square rigid patch at height z, side 2 half, dens x dens squares of
+    two triangles; normals (from the winding) -z when `down`, else +z."""
+    t = np.linspace(-half, half, dens + 1)
+    X, Y = np.meshgrid(cx + t, cy + t, indexing="ij")
+    nodes = np.stack([X.ravel(), Y.ravel(), np.full(X.size, z)], axis=1)
+    el = []
+    for a in range(dens):
+        for b in range(dens):
+            n0, n1, n2, n3 = a * (dens + 1) + b, (a + 1) * (dens + 1) + b, (a + 1) * (dens + 1) + b + 1, a * (dens + 1) + b + 1
+            if down:
+                el += [(n0, n3, n1), (n1, n3, n2)]
+            else:
+                el += [(n0, n1, n3), (n1, n2, n3)]
+    return nodes, np.array(el, dtype=np.int32)
+
+
+def merge_meshes(parts):
+    nodes, elems, off = [], [], 0
+    for nd, el in parts:
+        nodes.append(nd)
+        elems.append(el + off)
+        off += len(nd)
+    return np.concatenate(nodes), np.concatenate(elems)
+
+
+def config4(n_spec=256, steps=500, dx=1.0e-3, n_range=(2000, 40000), v_top=-5.0, seed=2030, patch_dens=4):
     """Batched ensemble: n_spec cylinder upsetting specimens advanced as one
     particle set.  Specimen i: N_i log-uniform on n_range, R/L uniform on
     [0.2, 0.5], R = (N_i dx^3 (R/L) / pi)^(1/3) on the dx lattice; aluminium
     elasticity; Johnson-Cook A, B, n, C, m each uniform within +-20 % of the
-    config-2 values (eps_0 = 1/s, T = T_t = 300 K); bottom row fixed
-    (zone 2), top row at v_top (zone 3); Monaghan alpha = 1.  Specimens stand
-    on a square grid with gaps wider than the kernel support, and each has
-    its own material index, so no pair crosses specimens.  The drawn
-    friction coefficients (uniform 0.05-0.3) are recorded in meta; the
-    specimens are driven through velocity BCs, not rigid planes."""
+    config-2 values (eps_0 = 1/s, T = T_t = 300 K); Monaghan alpha = 1.
+    Each specimen is compressed between its own rigid patches (squares of
+    side >= 2 (R_i + 3 dx), patch_dens^2 x 2 triangles, one h beyond its
+    ends): the top patches form rigid mesh 0 (zone 10, driven at v_top), the
+    bottom ones mesh 1 (zone 11, fixed); Wang contact with the specimen's own
+    Coulomb friction mu_i, uniform on [0.05, 0.3] (wf_set_material_friction).
+    Specimens stand on a square grid with gaps wider than the kernel
+    support, and each has its own material index, so no pair crosses
+    specimens; per-specimen contact forces come from the group sums."""
     rng = np.random.Generator(np.random.PCG64(seed))
     lo, hi = n_range
     N = np.exp(rng.uniform(np.log(lo), np.log(hi), n_spec))
@@ -254,10 +285,10 @@ def config4(n_spec=256, steps=500, dx=1.0e-3, n_range=(2000, 40000), v_top=-5.0,
     Rs = (N * dx ** 3 * ratio / math.pi) ** (1.0 / 3.0)
     Ls = Rs / ratio
     cols = int(math.ceil(math.sqrt(n_spec)))
-    pitch = 2 * Rs.max() + 4 * h
+    pitch = 2 * Rs.max() + 4 * h + 5 * dx
     xs, hs, ms, ids, mats, sy = [], [], [], [], [], []
-    materials = {}
-    spec_n = []
+    materials, friction = {}, {}
+    spec_n, tops, bots = [], [], []
     for i in range(n_spec):
         cx, cy = (i % cols) * pitch, (i // cols) * pitch
         x = capi.gen_cylinder([cx, cy, 0.0], Rs[i], Ls[i], r)
@@ -268,34 +299,59 @@ def config4(n_spec=256, steps=500, dx=1.0e-3, n_range=(2000, 40000), v_top=-5.0,
             m[key] = JC_AL[key] * scale[i, k]
         m.update(E=E, nu=nu, G=G, Cs=Cs, ref_density=rho, P0=0.0)
         materials[i] = m
+        friction[i] = (float(mu[i]), float(mu[i]))
         z = x[:, 2]
-        zid = np.zeros(n, dtype=np.int32)
-        zid[z <= z.min() + 0.5 * r] = 2
-        zid[z >= z.max() - 0.5 * r] = 3
+        # element legs a whole number of dx and the patch centre off the
+        # axis by (dx/4, dx/8): the lattice (x, y) sit on multiples of dx/2
+        # from the axis, so no particle lies on an element edge or diagonal,
+        # where the inside test would be decided by rounding
+        leg = dx * math.ceil((Rs[i] + 3 * dx) / (0.5 * patch_dens * dx))
+        half = 0.5 * patch_dens * leg
+        px, py = cx + 0.25 * dx, cy + 0.125 * dx
+        tops.append(patch_mesh(px, py, z.max() + 1.02 * h, half, patch_dens, down=True))
+        bots.append(patch_mesh(px, py, z.min() - 1.02 * h, half, patch_dens, down=False))
         xs.append(x)
         ms.append(np.full(n, math.pi * Rs[i] ** 2 * Ls[i] * rho / n))
-        ids.append(zid)
+        ids.append(np.zeros(n, dtype=np.int32))
         mats.append(np.full(n, i, dtype=np.int32))
         sy.append(np.full(n, m["jc_A"]))  # JC(0, 0, T_t) = A
     x = np.concatenate(xs)
     n = len(x)
+    tn, te = merge_meshes(tops)
+    bn, be = merge_meshes(bots)
     case = Case(
         name=f"batch_{n_spec}_cylinders", x=x, h=np.full(n, h), mass=np.concatenate(ms), density=np.full(n, rho),
         ids=np.concatenate(ids), material=np.concatenate(mats), materials=materials, sigmay=np.concatenate(sy),
         dt=0.4 * h / Cs, steps=steps, T=np.full(n, 300.0),
-        bcs=[dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, 0.0, v_top))],
-        meta=dict(dx=dx, h=h, R=Rs, L=Ls, N_drawn=N, spec_n=np.array(spec_n), friction=mu, pitch=pitch))
+        bcs=[dict(zone=10, value=(0.0, 0.0, v_top)), dict(zone=11, value=(0.0, 0.0, 0.0))],
+        meshes=[dict(zone=10, nodes=tn, elems=te, hfac=1.1), dict(zone=11, nodes=bn, elems=be, hfac=1.1)],
+        contact=(0.2, 0.2, 0.0), mat_friction=friction, group_sums=(-1, max(steps, 1)),
+        top_zone=10,
+        meta=dict(dx=dx, h=h, R=Rs, L=Ls, N_drawn=N, spec_n=np.array(spec_n), friction=mu, pitch=pitch,
+                  patch_dens=patch_dens))
     return case
 
 
 def specimen(batch, i):
-    """Specimen i of a batch (config4) as a case of its own (material 0)."""
+    """Specimen i of a batch (config4) as a case of its own (material 0, its
+    own top and bottom patches as the two rigid meshes, its friction)."""
     sel = batch.material == i
     n = int(sel.sum())
+    meshes = []
+    if batch.meshes:
+        dens = batch.meta["patch_dens"]
+        npn, npe = (dens + 1) ** 2, 2 * dens * dens
+        for m in batch.meshes:
+            nodes = m["nodes"][i * npn:(i + 1) * npn]
+            elems = m["elems"][i * npe:(i + 1) * npe] - i * npn
+            meshes.append(dict(m, nodes=nodes, elems=elems))
     return Case(name=f"{batch.name}_specimen{i}", x=batch.x[sel], h=batch.h[sel], mass=batch.mass[sel],
                 density=batch.density[sel], ids=batch.ids[sel], material=np.zeros(n, dtype=np.int32),
                 materials={0: batch.materials[i]}, sigmay=batch.sigmay[sel], dt=batch.dt, steps=batch.steps,
-                T=None if batch.T is None else batch.T[sel], bcs=batch.bcs, meta=dict(batch.meta, specimen=i))
+                T=None if batch.T is None else batch.T[sel], bcs=batch.bcs, meshes=meshes, contact=batch.contact,
+                mat_friction={0: batch.mat_friction[i]} if i in batch.mat_friction else {},
+                group_sums=None if batch.group_sums is None else (batch.group_sums[0], batch.group_sums[1]),
+                top_zone=batch.top_zone, meta=dict(batch.meta, specimen=i))
 
 
 def config5_zone(steps=400, dx=0.4675e-3, R=15e-3, L=30e-3, v_top=-5.0, seed=2029):
@@ -349,5 +405,9 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
             s.set_contact_heat(*case.contact_heat)
         if case.rigid_T is not None:
             s.set_rigid_temperature(case.rigid_T)
+    for idx, (fs, fd) in case.mat_friction.items():
+        s.set_material_friction(idx, fs, fd)
+    if case.group_sums is not None:
+        s.set_group_sums(*case.group_sums)
     s.init(case.dt)
     return s

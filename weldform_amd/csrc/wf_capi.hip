@@ -7,6 +7,7 @@
 // wf_step: the neighbour-search schedule lives on the device.
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -85,6 +86,9 @@ struct Ctx {
   bool built = false;
   std::vector<int32_t> perm, inv;  // perm[internal] = original
   std::vector<double> mat_xc;      // per-material FP32 shadow origin [kMaxMat][3]
+  std::vector<double> mat_fr;      // per-material friction (static, dynamic) [kMaxMat][2]; < 0: unset
+  int gsum_zone = -2;              // wf_set_group_sums: -2 off, -1 contact forces only, else the zone
+  long long gsum_cap = 0;
   Dev d{};
   std::vector<std::pair<void *, size_t>> allocs;  // (pointer, bytes) of the solver state
   DevMat *dmats = nullptr;
@@ -560,6 +564,30 @@ void allocate(Ctx *c) {
     d.nbrl = c->alloc<uint16_t>((size_t)nb_rows(c->cfg.max_nb) * np);
     d.selfl = c->alloc<uint16_t>(np);
   }
+  // per-group sums: each material index's particles in internal order
+  d.ngroups = 0;
+  if (c->gsum_zone != -2) {
+    const std::vector<int32_t> &mat = c->hi[WF_I_MATERIAL];
+    int ng = 0;
+    for (int64_t k = 0; k < c->n; ++k) ng = std::max(ng, mat[c->perm[k]] + 1);
+    std::vector<int> off(ng + 1, 0), idx(c->n);
+    for (int64_t k = 0; k < c->n; ++k) off[mat[c->perm[k]] + 1]++;
+    for (int g = 0; g < ng; ++g) off[g + 1] += off[g];
+    std::vector<int> cur(off.begin(), off.end() - 1);
+    for (int64_t k = 0; k < c->n; ++k) idx[cur[mat[c->perm[k]]]++] = (int)k;
+    d.ngroups = ng;
+    d.gzone = c->gsum_zone;
+    d.ghist_cap = c->gsum_cap;
+    d.goff = c->alloc<int>(off.size());
+    d.gidx = c->alloc<int>(idx.size());
+    dput(c, d.goff, off);
+    dput(c, d.gidx, idx);
+    d.gr_hist = c->alloc<double>((size_t)d.ghist_cap * ng * 3);
+    if (d.contact) {
+      d.gcf = c->alloc<double4>((size_t)kMaxMesh * np);
+      d.gc_hist = c->alloc<double>((size_t)d.ghist_cap * ng * kMaxMesh * 3);
+    }
+  }
 }
 
 void upload_all(Ctx *c) {
@@ -693,8 +721,11 @@ void upload_all(Ctx *c) {
   std::vector<DevMat> dm(kMaxMat);
   for (size_t k = 0; k < c->mats.size() && k < (size_t)kMaxMat; ++k)
     if (c->mat_set[k]) dm[k] = to_dev(c->mats[k]);
-  for (int m = 0; m < kMaxMat; ++m)
+  for (int m = 0; m < kMaxMat; ++m) {
     for (int q = 0; q < 3; ++q) dm[m].xc[q] = c->mat_xc[3 * m + q];
+    dm[m].fr_sta = c->mat_fr[2 * m];
+    dm[m].fr_dyn = c->mat_fr[2 * m + 1];
+  }
   dput(c, c->dmats, dm);
   // device schedule state: the pre-loop ClearNbData leaves the data cleared
   DevFlags fl{};
@@ -871,6 +902,7 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int
       Timed t(c, T_ACCEL);
       launch_accel(d, c->st, cur);
     }
+    if (d.ngroups > 0 && d.gzone >= 0) launch_group_sums(d, c->st, 0);
     if (tp && d.contact) d2d(c, tp->s.apre, d.acc, np * sizeof(double4));
     if (split) launch_kick(d, c->st, c->bc, scheme, dt, dt, K_OBSERVE);
   }
@@ -879,6 +911,7 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int
     // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
     // so no BC zone applies to them
     if (d.contact) launch_contact(d, c->st, c->bc, cur, dt);
+    if (d.contact && d.ngroups > 0) launch_group_sums(d, c->st, 1);
     if (tp && d.contact) {
       d2d(c, tp->s.node, d.node, (size_t)d.nnode * sizeof(double4));
       d2d(c, tp->s.enorm, d.enorm, (size_t)d.nrig * sizeof(double4));
@@ -977,6 +1010,9 @@ void adj_check(Ctx *c) {
   if (c->d.ti_on) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover the tensile-instability term");
   if (c->d.gc_on || c->d.locking)
     throw Error(WF_EUNSUPPORTED, "the adjoint covers the Nishimura sums without kernel-gradient correction");
+  for (int m = 0; m < kMaxMat; ++m)
+    if (c->mat_fr[2 * m] >= 0.0)
+      throw Error(WF_EUNSUPPORTED, "the adjoint covers the context's friction, not per-material friction");
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1111,6 +1147,7 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
     c->npad = (n + kBlock - 1) / kBlock * kBlock;
     c->mats.resize(kMaxMat);
     c->mat_set.assign(kMaxMat, 0);
+    c->mat_fr.assign(2 * kMaxMat, -1.0);
     for (int f = 0; f < WF_NIFIELDS; ++f) c->hi[f].assign(n, 0);
     c->hi[WF_I_FREE].assign(n, 1);
     try {
@@ -1157,6 +1194,8 @@ int wf_set_material(void *ctx, int32_t index, const wf_material *m) {
     if (c->inited) {
       std::vector<DevMat> one(1, to_dev(*m));
       for (int q = 0; q < 3; ++q) one[0].xc[q] = c->mat_xc[3 * index + q];
+      one[0].fr_sta = c->mat_fr[2 * index];
+      one[0].fr_dyn = c->mat_fr[2 * index + 1];
       HIPCHK(hipMemcpyAsync(c->dmats + index, one.data(), sizeof(DevMat), hipMemcpyHostToDevice, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
       // the acceleration kernel is chosen by whether a material has beta != 0
@@ -2252,6 +2291,67 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
     auto f = dget(c, c->d.flags, 1)[0];
     if (directed_pairs) *directed_pairs = f.directed_pairs;
     if (max_count) *max_count = f.max_count;
+    return WF_OK;
+  });
+}
+
+int wf_set_material_friction(void *ctx, int32_t index, double friction_sta, double friction_dyn) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (index < 0 || index >= kMaxMat) throw Error(WF_EINVAL, "material index out of range");
+    if (!(friction_sta >= 0) || !(friction_dyn >= 0)) throw Error(WF_EINVAL, "negative friction coefficient");
+    c->mat_fr[2 * index] = friction_sta;
+    c->mat_fr[2 * index + 1] = friction_dyn;
+    if (c->inited) {
+      // the friction sits in the device material, read by the contact kernel
+      const size_t off = (size_t)index * sizeof(DevMat) + offsetof(DevMat, fr_sta);
+      const double v[2] = {friction_sta, friction_dyn};
+      static_assert(offsetof(DevMat, fr_dyn) == offsetof(DevMat, fr_sta) + sizeof(double), "friction pair");
+      HIPCHK(hipMemcpyAsync((char *)c->dmats + off, v, sizeof v, hipMemcpyHostToDevice, c->st));
+      HIPCHK(hipStreamSynchronize(c->st));
+    }
+    return WF_OK;
+  });
+}
+
+int wf_set_group_sums(void *ctx, int32_t zone, int64_t cap) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_group_sums must precede wf_init");
+    if (zone < -1) throw Error(WF_EINVAL, "zone must be >= 0, or -1 for the contact forces only");
+    if (cap < 1 || cap > (1 << 20)) throw Error(WF_EINVAL, "cap must be 1..2^20 steps");
+    c->gsum_zone = zone;
+    c->gsum_cap = cap;
+    return WF_OK;
+  });
+}
+
+int wf_get_group_history(void *ctx, int32_t what, int32_t group, int32_t mesh, double *out3, int64_t cap,
+                         int64_t *count) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    const Dev &d = c->d;
+    if (d.ngroups == 0) throw Error(WF_ESTATE, "group sums are off (wf_set_group_sums)");
+    if (group < 0 || group >= d.ngroups) throw Error(WF_EINVAL, "group out of range");
+    if (what == 0 && d.gzone < 0) throw Error(WF_EINVAL, "no zone was given for the group reactions");
+    if (what == 1 && (!d.contact || mesh < 0 || mesh >= d.nmesh)) throw Error(WF_EINVAL, "no such rigid mesh");
+    if (what != 0 && what != 1) throw Error(WF_EINVAL, "what must be 0 (reaction) or 1 (contact force)");
+    if (!out3 && cap > 0) throw Error(WF_EINVAL, "null output buffer");
+    const long long hc = d.ghist_cap;
+    const long long m = std::min<long long>(std::min<long long>(c->steps, hc), cap);
+    const size_t per = what == 0 ? 3 : (size_t)kMaxMesh * 3;
+    auto h = dget(c, what == 0 ? d.gr_hist : d.gc_hist, (size_t)hc * d.ngroups * per);
+    const long long first = c->steps - m;
+    for (long long s = 0; s < m; ++s) {
+      const long long slot = (first + s) % hc;
+      const size_t base = ((size_t)slot * d.ngroups + group) * per + (what == 0 ? 0 : (size_t)mesh * 3);
+      for (int k = 0; k < 3; ++k) out3[3 * s + k] = h[base + k];
+    }
+    if (count) *count = m;
     return WF_OK;
   });
 }

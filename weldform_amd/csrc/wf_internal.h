@@ -86,6 +86,9 @@ struct DevMat {
   double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt;
   double k_T, cp_T, th_exp;
   double ti, ti_n, ti_dist;    // tensile instability: TI, TIn, TIInitDist
+  // Coulomb friction of the material's particles against the rigid
+  // surfaces (wf_set_material_friction); negative: the context's
+  double fr_sta, fr_dyn;
   // origin of the material's FP32 position shadows: the centre of its
   // particles' initial bounding box (pairs form within one material, so the
   // pair differences see one origin; a batch of separated specimens keeps
@@ -254,6 +257,16 @@ struct Dev {
   int *id_orig, *ccnt, *cidx;  // contact partners [kMaxContact][npad]
   double4 *cforce;
   double *cf_hist;             // [hist_cap][kMaxMesh][3]
+  // sums per particle group (material index, wf_set_group_sums): each
+  // group's particles in internal order (goff, gidx), per step the reaction
+  // of velocity-BC zone gzone (sum m a at the observer) and the contact force
+  // of each rigid mesh on the group, kept for the last ghist_cap steps
+  int ngroups, gzone;
+  long long ghist_cap;
+  int *goff, *gidx;
+  double4 *gcf;                // [kMaxMesh][npad] a particle's contact force per mesh (where ccnt > 0)
+  double *gr_hist;             // [ghist_cap][ngroups][3]
+  double *gc_hist;             // [ghist_cap][ngroups][kMaxMesh][3]
   // materials, flags, reductions
   const DevMat *mats;
   const DevAmp *amps;
@@ -328,6 +341,9 @@ void launch_max_pl(const Dev &d, hipStream_t st);
 void launch_shadows(const Dev &d, hipStream_t st, int cur);
 void launch_surface(const Dev &d, hipStream_t st, int cur, int setup);
 void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt);
+// per-group sums of the step: what 0 the zone reaction (after the
+// acceleration pass), 1 the contact forces per mesh (after contact)
+void launch_group_sums(const Dev &d, hipStream_t st, int what);
 void launch_rigid_drift(const Dev &d, hipStream_t st, double dt);
 void launch_mesh_update(const Dev &d, hipStream_t st, double dt);
 // output tables (wf_output.hip)

@@ -2184,6 +2184,8 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
   if (i < d.n) {
     double cf[3] = {0, 0, 0};
     double qf = 0.0, fh = 0.0, cs = 0.0, qc = 0.0;
+    double cfi[kMaxMesh][3];  // this particle's contact force per mesh
+    for (int m = 0; m < kMaxMesh; ++m) cfi[m][0] = cfi[m][1] = cfi[m][2] = 0.0;
     const int cnt = d.ccnt[i];
     if (cnt > 0) {
       const double4 x4 = d.xh[cur][i];
@@ -2192,6 +2194,9 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
       const double x[3] = {x4.x, x4.y, x4.z}, v[3] = {v4.x, v4.y, v4.z};
       double a[3] = {a4.x, a4.y, a4.z};
       const double h = x4.w, m1 = v4.w;
+      const DevMat &Mi = d.mats[d.mat[i]];
+      const double fr_sta = Mi.fr_sta >= 0.0 ? Mi.fr_sta : d.fr_sta;
+      const double fr_dyn = Mi.fr_sta >= 0.0 ? Mi.fr_dyn : d.fr_dyn;
       for (int q = 0; q < cnt; ++q) {
         const int r = d.cidx[(long long)q * d.npad + i];
         const int mi = d.rmesh[r];
@@ -2237,14 +2242,14 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
         // contact heat branch (the only readers, which divide by it) is on
         const double dens = d.rho[cur][i];
         const double dS2 = (d.heat_cond || d.heat_fric) ? pow(m1 / dens, 2.0 / 3.0) : 0.0;
-        if (d.fr_sta > 0.) {
+        if (fr_sta > 0.) {
           double du[3], tg[3];
           for (int k = 0; k < 3; ++k) du[k] = xp[k] - x[k] - v2[k] * dt;
           const double dn = du[0] * n2[0] + du[1] * n2[1] + du[2] * n2[2];
           for (int k = 0; k < 3; ++k) tg[k] = kij * (du[k] - dn * n2[k]);
           const double ntg = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
           const double ncf = sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
-          if (ntg < d.fr_sta * ncf) {
+          if (ntg < fr_sta * ncf) {
             for (int k = 0; k < 3; ++k) {
               cf[k] -= tg[k] / m1;
               a[k] -= tg[k] / m1;
@@ -2252,7 +2257,7 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
           } else {
             double td[3];
             for (int k = 0; k < 3; ++k) {
-              td[k] = d.fr_dyn * ncf * tg[k] / ntg;
+              td[k] = fr_dyn * ncf * tg[k] / ntg;
               cf[k] -= td[k];
               a[k] -= td[k] / m1;
             }
@@ -2268,7 +2273,10 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
         cfsum += sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
         for (int m = 0; m < kMaxMesh; ++m)
           if (m == mi)
-            for (int k = 0; k < 3; ++k) cfm[m][k] += cf[k];
+            for (int k = 0; k < 3; ++k) {
+              cfm[m][k] += cf[k];
+              cfi[m][k] += cf[k];
+            }
         creact += (a[0] * n2[0] + a[1] * n2[1] + a[2] * n2[2]) * m1;
         extw += cf[0] * v2[0] + cf[1] * v2[1] + cf[2] * v2[2];
         // contact conductance, Fraser 3.121 (Source/Contact_Wang.cpp:320-324)
@@ -2277,6 +2285,8 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
       d.acc[i] = make_double4(a[0], a[1], a[2], a4.w);
     }
     d.cforce[i] = make_double4(cf[0], cf[1], cf[2], 0.0);
+    if (d.gcf && cnt > 0)
+      for (int m = 0; m < d.nmesh; ++m) d.gcf[(long long)m * d.npad + i] = make_double4(cfi[m][0], cfi[m][1], cfi[m][2], 0.0);
     if (d.qfric) {
       d.qfric[i] = qf;
       d.fhfl[i] = fh;
@@ -2301,6 +2311,53 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
     // adaptive update of the step (Source/Contact_Wang.cpp:344)
     part(d, P_EXTW, blockIdx.x) = r3 * dt;
   }
+}
+
+// Per-group sums (wf_set_group_sums): one block per group walks the group's
+// particles in internal order (thread t takes every kGroupThreads-th) and
+// reduces in a fixed order, so the sums are deterministic.  what 0: the
+// reaction of zone gzone, sum of m a over the group's particles carrying
+// that id, read where the zone reactions are (after the acceleration pass,
+// before contact and the BCs); what 1: the contact force each rigid mesh
+// exerts on the group (after contact).
+constexpr int kGroupThreads = 256;
+__global__ __launch_bounds__(kGroupThreads) void k_group_sums(Dev d, int what) {
+  __shared__ double sh[kGroupThreads / 64];
+  const int g = blockIdx.x;
+  const long long slot = (d.flags->step % d.ghist_cap) * d.ngroups + g;
+  const int b = d.goff[g], e = d.goff[g + 1];
+  if (what == 0) {
+    double s[3] = {0, 0, 0};
+    for (int p = b + threadIdx.x; p < e; p += kGroupThreads) {
+      const int i = d.gidx[p];
+      if (d.id[i] != d.gzone) continue;
+      const double4 a = d.acc[i];
+      const double m = d.vm[i].w;
+      s[0] += m * a.x, s[1] += m * a.y, s[2] += m * a.z;
+    }
+    for (int k = 0; k < 3; ++k) {
+      const double r = block_reduce<0>(s[k], sh);
+      if (threadIdx.x == 0) d.gr_hist[slot * 3 + k] = r;
+    }
+  } else {
+    for (int m = 0; m < d.nmesh; ++m) {
+      double s[3] = {0, 0, 0};
+      for (int p = b + threadIdx.x; p < e; p += kGroupThreads) {
+        const int i = d.gidx[p];
+        if (d.ccnt[i] <= 0) continue;
+        const double4 f = d.gcf[(long long)m * d.npad + i];
+        s[0] += f.x, s[1] += f.y, s[2] += f.z;
+      }
+      for (int k = 0; k < 3; ++k) {
+        const double r = block_reduce<0>(s[k], sh);
+        if (threadIdx.x == 0) d.gc_hist[(slot * kMaxMesh + m) * 3 + k] = r;
+      }
+    }
+  }
+}
+
+void launch_group_sums(const Dev &d, hipStream_t st, int what) {
+  if (d.ngroups > 0) k_group_sums<<<d.ngroups, kGroupThreads, 0, st>>>(d, what);
 }
 
 // the drift of the rigid particles (the reference's loop covers them)
