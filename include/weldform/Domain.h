@@ -47,6 +47,10 @@ namespace SPH {
 enum Kernels_Type { Qubic_Spline = 0, Quintic = 1, Quintic_Spline = 2, Hyperbolic_Spline = 3 };
 enum Gradient_Type { Squared_density = 0, Multiplied_density = 1, Randles_Libersky = 2 };
 enum BC_TYPE { Velocity_BC = 0, Force_BC, Temperature_BC, Convection_BC, Symmetry_BC };
+// Source/Domain.h:61-64
+enum Function_Type { Constant = 0, Linear = 1, Multilinear = 2 };
+enum Friction_Type { Fr_Sta = 0, Fr_Dyn, Fr_StaDyn, Fr_Bound };
+enum Contact_Alg { Fraser = 0, Wang, Seo, Zhan, LSDyna };
 enum class Precision { FP64 = WF_FP64, Mixed = WF_MIXED };
 
 // amplitude table (Source/Domain.h:69-89)
@@ -107,13 +111,20 @@ class Domain {
   std::vector<TriMesh *> trimesh;
   std::vector<int> first_fem_particle_idx, contact_surf_id;
   int meshcount = 0;
-  double friction_sta = 0.0, friction_dyn = 0.0, DFAC = 0.0, PFAC = 0.6;
+  // PFAC: the reference's constructor leaves it unset; 0.5 is its JSON
+  // driver's default (Source/WeldForm.cpp:685)
+  double friction_sta = 0.0, friction_dyn = 0.0, DFAC = 0.0, PFAC = 0.5;
+  // friction_function Linear: friction_m T + friction_b (Wang's and
+  // LS-Dyna's laws); fric_type Fr_Bound (Fraser's law)
+  Function_Type friction_function = Constant;
+  double friction_m = 0.0, friction_b = 0.0;
+  Friction_Type fric_type = Fr_Dyn;
   bool cont_heat_cond = false, cont_heat_fric = false;
   double contact_hc = 0.0;  // contact heat conductance
   // min time step of the contact forces (auto_ts_cont) and the contact heat
   // totals (Source/Domain.h:327, 346-347; Source/Thermal.cpp:194-210)
   double min_force_ts = 0.0, contact_friction_work = 0.0, accum_cont_heat_cond = 0.0;
-  int contact_alg = 0;  // Wang
+  Contact_Alg contact_alg = Wang;  // Source/Domain.cpp:169
   std::vector<Vec3_t> m_contact_force;
   double contact_force_sum = 0.0, contact_reaction_sum = 0.0, ext_forces_work = 0.0;
   double pl_work_heat_frac = 0.9;
@@ -363,7 +374,10 @@ class Domain {
     if (GradientType != Squared_density) throw new Fatal("the device path implements GradientType 0 only");
     if (contact) {
       if (trimesh.empty()) throw new Fatal("contact is set but no rigid mesh was added (AddTrimeshParticles)");
-      if (contact_alg != 0) throw new Fatal("only the Wang contact algorithm is implemented on the device path");
+      // SolveDiffUpdateLeapFrog dispatches Fraser, Wang, Seo and LSDyna
+      // (Source/SolverLeapfrog.cpp:234-239); Zhan's law is never called
+      if (contact_alg == Zhan) throw new Fatal("contact_alg Zhan is not dispatched by the solvers");
+      if (friction_function == Multilinear) throw new Fatal("friction_function Multilinear is not implemented");
       // a mesh's m_w comes from the constant velocity BC of its zone (UserAcc
       // sets it every call); one set directly on a mesh without such a BC is
       // not carried to the device
@@ -826,6 +840,8 @@ class Domain {
     }
     if (contact) {
       check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));
+      check(wf_set_contact_law(ctx_, (int32_t)contact_alg, PFAC, fric_type == Fr_Bound ? 1 : 0));
+      if (friction_function == Linear) check(wf_set_friction_function(ctx_, 1, friction_m, friction_b));
       if (cont_heat_cond || cont_heat_fric) check(wf_set_contact_heat(ctx_, cont_heat_cond, cont_heat_fric, contact_hc));
       // the rigid particles' temperatures as the initial conditions and
       // temperature BCs of their zones left them

@@ -283,6 +283,27 @@ int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double d
  * would).  May be called before or after wf_init; the adjoint refuses
  * contexts that use it. */
 int wf_set_material_friction(void *ctx, int32_t index, double friction_sta, double friction_dyn);
+/* wf_set_contact_law <- Domain::contact_alg / PFAC / fric_type
+ * (Source/Domain.h:64,298,369; dispatched by SolveDiffUpdateLeapFrog,
+ * Source/SolverLeapfrog.cpp:234-239; scene key "contAlgorithm",
+ * Source/WeldForm.cpp:695-705): law 0 Fraser (Domain::CalcContactForces,
+ * Source/Contact.cpp:223-554), 1 Wang (the default), 2 Seo
+ * (CalcContactForces2, Source/Contact.cpp:562-770), 4 LSDyna
+ * (CalcContactForcesLS, Source/Contact_LS.cpp:15-292); pfac > 0 (default
+ * 0.5) scales the penalty of the Fraser, Seo and LS-Dyna laws (Fraser and
+ * Seo: PFAC cont_stiff, with Particle::cont_stiff formed at wf_init from
+ * the initial state as SolveDiffUpdateKickDrift forms it, Source/
+ * SolverKickDrift.cpp:79-84; LS-Dyna: PFAC 2 m / dt^2); fric_bound selects
+ * fric_type Fr_Bound in Fraser's law.  Under Leapfrog these three laws
+ * leave a unchanged (their contforce and sums are outputs) except Fraser's
+ * Fr_Bound; KickDrift always uses Wang's.  Before wf_init.
+ * wf_set_friction_function <- Domain::friction_function Linear with
+ * friction_m, friction_b (Source/Domain.h:293-295): Wang's and LS-Dyna's
+ * laws take friction_sta = friction_dyn = m T + b of the contacting
+ * particle (Source/Contact_Wang.cpp:241-244, Source/Contact_LS.cpp:217-220);
+ * linear 0 restores the constant coefficients.  Before wf_init. */
+int wf_set_contact_law(void *ctx, int32_t law, double pfac, int32_t fric_bound);
+int wf_set_friction_function(void *ctx, int32_t linear, double m, double b);
 /* wf_set_contact_heat <- Domain::cont_heat_cond / contact_hc / cont_heat_fric
  * (Source/Domain.h:339-347, set from a scene's Contact "heatConductance",
  * "heatCondCoeff" and "heatFriction", Source/WeldForm.cpp:711-723): every

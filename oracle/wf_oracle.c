@@ -96,6 +96,13 @@ struct wfo_domain {
   int contact, nmesh, id_free_surf;
   mesh_t mesh[WFO_MAX_MESH];
   double friction_sta, friction_dyn, DFAC;
+  /* Domain::contact_alg (Source/Domain.h:64: 0 Fraser, 1 Wang, 2 Seo,
+   * 4 LSDyna), PFAC, fric_type == Fr_Bound, friction_function == Linear
+   * with friction_m / friction_b, and Particle::cont_stiff (Fraser
+   * 3-153, formed at wfo_solve_init for the Fraser and Seo laws) */
+  int contact_alg, fric_bound, friction_linear;
+  double PFAC, friction_m, friction_b;
+  double *cont_stiff;
   /* contact heat (Source/Domain.h:339-347): conduction through the contact
    * with conductance contact_hc, friction work as heat; their totals */
   int cont_heat_cond, cont_heat_fric;
@@ -142,6 +149,8 @@ wfo_domain *wfo_create(int n) {
     d->f[WFO_TIN][i] = 4.0;
   }
   for (int k = 0; k < 256; ++k) d->mat_fr[k][0] = d->mat_fr[k][1] = -1.;
+  d->contact_alg = 1;  /* Wang: the JSON driver's default */
+  d->PFAC = 0.5;       /* the JSON driver's penaltyFactor default (Source/WeldForm.cpp:685) */
   /* Domain constructor defaults (Source/Domain.cpp:78-175) */
   d->Cellfac = 2.0;
   for (int k = 0; k < 3; ++k) { d->DomMax[k] = -100000000000.0; d->DomMin[k] = 100000000000.0; }
@@ -176,7 +185,7 @@ void wfo_destroy(wfo_domain *d) {
     free(M->node); free(M->node_v); free(M->el); free(M->centroid); free(M->normal);
     free(M->pplane); free(M->radius); free(M->nfar);
   }
-  free(d->rig); free(d->cont);
+  free(d->rig); free(d->cont); free(d->cont_stiff);
   free(d);
 }
 
@@ -215,6 +224,12 @@ int wfo_set_param(wfo_domain *d, const char *s, double v) {
   else if (!strcmp(s, "contact")) d->contact = (int)v;
   else if (!strcmp(s, "friction_sta")) d->friction_sta = v;
   else if (!strcmp(s, "friction_dyn")) d->friction_dyn = v;
+  else if (!strcmp(s, "contact_alg")) d->contact_alg = (int)v;
+  else if (!strcmp(s, "PFAC")) d->PFAC = v;
+  else if (!strcmp(s, "fric_bound")) d->fric_bound = (int)v;
+  else if (!strcmp(s, "friction_linear")) d->friction_linear = (int)v;
+  else if (!strcmp(s, "friction_m")) d->friction_m = v;
+  else if (!strcmp(s, "friction_b")) d->friction_b = v;
   else if (!strcmp(s, "DFAC")) d->DFAC = v;
   else if (!strcmp(s, "cont_heat_cond")) d->cont_heat_cond = (int)v;
   else if (!strcmp(s, "cont_heat_fric")) d->cont_heat_fric = (int)v;
@@ -1655,6 +1670,234 @@ static int contact_nb_update(wfo_domain *d) {
   return 0;
 }
 
+/* The Coulomb coefficients of the particle's pairs: the domain's
+ * friction_sta / friction_dyn, or its material's; with friction_function
+ * Linear both are friction_m T + friction_b (Source/Contact_Wang.cpp:241-244,
+ * Source/Contact_LS.cpp:217-220) */
+static void pair_friction(const wfo_domain *d, int P1, double *fs, double *fd) {
+  const double *mfr = d->mat_fr[d->fi[WFO_MATIDX][P1]];
+  *fs = mfr[0] >= 0. ? mfr[0] : d->friction_sta;
+  *fd = mfr[0] >= 0. ? mfr[1] : d->friction_dyn;
+  if (d->friction_linear) *fs = *fd = d->friction_m * d->f[WFO_T][P1] + d->friction_b;
+}
+
+/* the inside test of the contact point qj against triangle e of mesh M
+ * (Fraser 3-147: every edge's cross product with qj - node_i along the
+ * normal is >= 0) */
+static int inside_tri(const mesh_t *M, int e, const double *qj, const double *n2) {
+  for (int ii = 0; ii < 3; ii++) {
+    int jj = ii + 1;
+    if (jj > 2) jj = 0;
+    const double *ni = M->node + 3 * M->el[3 * e + ii], *nj = M->node + 3 * M->el[3 * e + jj];
+    double ea[3], eb[3], c[3];
+    for (int k = 0; k < 3; k++) { ea[k] = nj[k] - ni[k]; eb[k] = qj[k] - ni[k]; }
+    cross3(ea, eb, c);
+    if (dot3(c, n2) < 0.0) return 0;
+  }
+  return 1;
+}
+
+/* Domain::CalcContactForces, Fraser's law (Source/Contact.cpp:223-554),
+ * 3D meshes.  Only pairs approaching (delta_ = -n.v_r > 0) whose time to
+ * contact deltat_cont = (h + pplane - n.x) / (-delta_) is below the step
+ * and whose contact point x + v deltat_cont - h n lies inside the triangle
+ * count; the penetration is (deltat - deltat_cont) delta_ and the normal
+ * force (PFAC cont_stiff delta - psi delta_) n is ASSIGNED to contforce.
+ * Under Leapfrog the force is not added to a (the solver's loop that would
+ * add it is commented out, Source/SolverLeapfrog.cpp:244-247): contforce
+ * and contact_force_sum are outputs.  Friction: with fric_type Fr_Bound the
+ * tangential part of a is removed; else a sliding pair (|v_t| > 1e-3)
+ * ASSIGNS contforce = -friction_dyn |contforce| t / m.  The static branch
+ * reads a tgforce that the reference never sets before it (an
+ * uninitialised variable carried across pairs); it is taken as zero here,
+ * which makes that branch add nothing.  Domain::min_force_ts takes the
+ * running minimum of dt_fext whenever it drops to a positive value. */
+static void contact_forces_fraser(wfo_domain *d) {
+  const double dt = d->deltat;
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V], *h = d->f[WFO_H], *ms = d->f[WFO_MASS], *nrmv = d->f[WFO_NORMAL];
+  double *a = d->f[WFO_A], *cf = d->f[WFO_CONTFORCE];
+  for (int i = 0; i < 3 * d->n; i++) cf[i] = 0.;
+  double min_force_ts_ = 1000.;
+  d->contact_force_sum = 0.;
+  for (int64_t q = 0; q < d->ncont; q++) {
+    const int P1 = d->cont[2 * q], P2 = d->cont[2 * q + 1];
+    const double *n2 = nrmv + 3 * P2;
+    double vr[3];
+    for (int k = 0; k < 3; k++) vr[k] = v[3 * P1 + k] - v[3 * P2 + k];
+    const double delta_ = -dot3(n2, vr);
+    if (!(delta_ > 0)) continue;
+    const mesh_t *M = &d->mesh[d->fi[WFO_MESHIDX][P2]];
+    const int e = d->fi[WFO_ELEMENT][P2];
+    const double deltat_cont = (h[P1] + M->pplane[e] - dot3(n2, x + 3 * P1)) / (-delta_);
+    if (!(deltat_cont < dt)) continue;
+    double qj[3];
+    for (int k = 0; k < 3; k++) qj[k] = x[3 * P1 + k] + (v[3 * P1 + k] * deltat_cont) - (h[P1] * n2[k]);
+    if (!inside_tri(M, e, qj, n2)) continue;
+    const double m1 = ms[P1];
+    const double delta = (dt - deltat_cont) * delta_;
+    const double kij = d->PFAC * d->cont_stiff[P1];
+    const double omega = sqrt(kij / m1);
+    const double psi_cont = 2. * m1 * omega * d->DFAC;
+    const double normal_cf = (kij * delta - psi_cont * delta_);
+    for (int k = 0; k < 3; k++) cf[3 * P1 + k] = normal_cf * n2[k];
+    d->contact_force_sum += norm3(cf + 3 * P1);
+    double tgvr[3], atg[3];
+    for (int k = 0; k < 3; k++) tgvr[k] = vr[k] + delta_ * n2[k];
+    const double norm_tgvr = norm3(tgvr);
+    const double an = dot3(a + 3 * P1, n2);
+    for (int k = 0; k < 3; k++) atg[k] = a[3 * P1 + k] - an * n2[k];
+    const double dt_fext = 1. * (m1 * 2. * norm3(v + 3 * P1) / norm3(cf + 3 * P1));
+    if (dt_fext < min_force_ts_) {
+      min_force_ts_ = dt_fext;
+      if (dt_fext > 0) d->min_force_ts = min_force_ts_;
+    }
+    if (d->fric_bound) {
+      for (int k = 0; k < 3; k++) a[3 * P1 + k] -= atg[k];
+    } else if (d->friction_dyn > 0. && norm_tgvr > 1.e-3) {
+      const double ncf = norm3(cf + 3 * P1);
+      for (int k = 0; k < 3; k++) {
+        const double tgforce = d->friction_dyn * ncf * (tgvr[k] / norm_tgvr);
+        cf[3 * P1 + k] = -tgforce / m1;
+      }
+    }
+  }
+}
+
+/* Domain::CalcContactForces2, Seo's law (Source/Contact.cpp:562-770):
+ * penetration of the current position, dist = n.x - pplane < h, contact
+ * point x - dist n inside the triangle, normal force
+ * (PFAC cont_stiff (h - dist) - psi delta_) n ASSIGNED to contforce (not
+ * added to a).  Static friction (ref_accel): if |m a_t| < friction_sta
+ * |contforce|, contforce -= PFAC cont_stiff (v_r dt)_t / m.  No dynamic
+ * branch.  min_force_ts as in Fraser's law. */
+static void contact_forces_seo(wfo_domain *d) {
+  const double dt = d->deltat;
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V], *h = d->f[WFO_H], *ms = d->f[WFO_MASS], *nrmv = d->f[WFO_NORMAL];
+  const double *a = d->f[WFO_A];
+  double *cf = d->f[WFO_CONTFORCE];
+  for (int i = 0; i < 3 * d->n; i++) cf[i] = 0.;
+  double min_force_ts_ = 1000.;
+  d->contact_force_sum = 0.;
+  for (int64_t q = 0; q < d->ncont; q++) {
+    const int P1 = d->cont[2 * q], P2 = d->cont[2 * q + 1];
+    const double *n2 = nrmv + 3 * P2;
+    double vr[3];
+    for (int k = 0; k < 3; k++) vr[k] = v[3 * P1 + k] - v[3 * P2 + k];
+    const double delta_ = -dot3(n2, vr);
+    const mesh_t *M = &d->mesh[d->fi[WFO_MESHIDX][P2]];
+    const int e = d->fi[WFO_ELEMENT][P2];
+    const double dist = dot3(n2, x + 3 * P1) - M->pplane[e];
+    if (!(dist < h[P1])) continue;
+    double qj[3];
+    for (int k = 0; k < 3; k++) qj[k] = x[3 * P1 + k] - dist * n2[k];
+    if (!inside_tri(M, e, qj, n2)) continue;
+    const double m1 = ms[P1];
+    const double delta = h[P1] - dist;
+    const double kij = d->PFAC * d->cont_stiff[P1];
+    const double omega = sqrt(kij / m1);
+    const double psi_cont = 2. * m1 * omega * d->DFAC;
+    for (int k = 0; k < 3; k++) cf[3 * P1 + k] = (kij * delta - psi_cont * delta_) * n2[k];
+    d->contact_force_sum += norm3(cf + 3 * P1);
+    double atg[3];
+    const double an = dot3(a + 3 * P1, n2);
+    for (int k = 0; k < 3; k++) atg[k] = a[3 * P1 + k] - an * n2[k];
+    const double dt_fext = 1. * (m1 * 2. * norm3(v + 3 * P1) / norm3(cf + 3 * P1));
+    if (dt_fext < min_force_ts_) {
+      min_force_ts_ = dt_fext;
+      if (dt_fext > 0) d->min_force_ts = min_force_ts_;
+    }
+    double du[3], dtg[3], tgforce[3], ref_tg[3];
+    for (int k = 0; k < 3; k++) du[k] = vr[k] * dt;
+    const double dn = dot3(du, n2);
+    for (int k = 0; k < 3; k++) dtg[k] = du[k] - dn * n2[k];
+    for (int k = 0; k < 3; k++) tgforce[k] = kij * dtg[k];
+    for (int k = 0; k < 3; k++) ref_tg[k] = atg[k] * m1;
+    if (d->friction_sta > 0.)
+      if (norm3(ref_tg) < d->friction_sta * norm3(cf + 3 * P1))
+        for (int k = 0; k < 3; k++) cf[3 * P1 + k] -= tgforce[k] / m1;
+  }
+}
+
+/* Domain::CalcContactForcesLS (Source/Contact_LS.cpp:15-292), 3D meshes:
+ * the Wang geometry (predicted position, contact point from the current
+ * one), penalty 2 m / dt^2 PFAC, normal force ASSIGNED to contforce (not
+ * added to a); the contact heat fields are reset.  Friction: the trial
+ * force fstar = tgforce_old - PFAC m v_r / dt, returned to the cone
+ * friction_sta |contforce| and added; Particle::tgforce, the old force,
+ * is never assigned by the reference (read uninitialised), taken as zero
+ * here.  contact_reaction_sum and the external work as in Wang's law;
+ * Domain::min_force_ts follows the running minimum of dt_fext. */
+static void contact_forces_ls(wfo_domain *d) {
+  const double dt = d->deltat;
+  const double *x = d->f[WFO_X], *v = d->f[WFO_V], *h = d->f[WFO_H], *ms = d->f[WFO_MASS], *nrmv = d->f[WFO_NORMAL];
+  const double *a = d->f[WFO_A];
+  double *cf = d->f[WFO_CONTFORCE];
+  double *qfric = d->f[WFO_QFRIC], *cshear = d->f[WFO_CSHEAR], *fhfl = d->f[WFO_FHFL], *qcont = d->f[WFO_QCONT];
+  for (int i = 0; i < d->n; i++) {
+    for (int k = 0; k < 3; k++) cf[3 * i + k] = 0.;
+    qfric[i] = cshear[i] = fhfl[i] = qcont[i] = 0.;
+  }
+  double min_force_ts_ = 1000.;
+  d->contact_force_sum = 0.;
+  d->contact_reaction_sum = 0.;
+  double ext_step = 0.;
+  for (int64_t q = 0; q < d->ncont; q++) {
+    const int P1 = d->cont[2 * q], P2 = d->cont[2 * q + 1];
+    const double *n2 = nrmv + 3 * P2;
+    double vr[3];
+    for (int k = 0; k < 3; k++) vr[k] = v[3 * P1 + k] - v[3 * P2 + k];
+    const double delta_ = -dot3(n2, vr);
+    const mesh_t *M = &d->mesh[d->fi[WFO_MESHIDX][P2]];
+    const int e = d->fi[WFO_ELEMENT][P2];
+    double x_pred[3];
+    for (int k = 0; k < 3; k++) x_pred[k] = x[3 * P1 + k] + v[3 * P1 + k] * dt + a[3 * P1 + k] * dt * dt / 2.0;
+    const double dist = dot3(n2, x_pred) - M->pplane[e];
+    if (!(dist < h[P1])) continue;
+    double qj[3];
+    for (int k = 0; k < 3; k++) qj[k] = x[3 * P1 + k] - dist * n2[k];
+    if (!inside_tri(M, e, qj, n2)) continue;
+    const double m1 = ms[P1];
+    const double delta = h[P1] - dist;
+    const double kij = 2.0 * m1 / (dt * dt) * d->PFAC;
+    const double omega = sqrt(kij / m1);
+    const double psi_cont = 2. * m1 * omega * d->DFAC;
+    for (int k = 0; k < 3; k++) cf[3 * P1 + k] = (kij * delta - psi_cont * delta_) * n2[k];
+    const double dt_fext = 1. * (m1 * 2. * norm3(v + 3 * P1) / norm3(cf + 3 * P1));
+    if (dt_fext < min_force_ts_) {
+      min_force_ts_ = dt_fext;
+      d->min_force_ts = min_force_ts_;
+    }
+    double fr_sta, fr_dyn;
+    pair_friction(d, P1, &fr_sta, &fr_dyn);
+    if (fr_sta > 0.) {
+      double kdeltae[3], fstar[3], fT[3];
+      for (int k = 0; k < 3; k++) kdeltae[k] = d->PFAC * m1 * vr[k] / dt;
+      const double fy = fr_sta * norm3(cf + 3 * P1);
+      for (int k = 0; k < 3; k++) fstar[k] = 0.0 - kdeltae[k];
+      const double nfs = norm3(fstar);
+      if (nfs > fy)
+        for (int k = 0; k < 3; k++) fT[k] = fy * fstar[k] / nfs;
+      else
+        for (int k = 0; k < 3; k++) fT[k] = fstar[k];
+      for (int k = 0; k < 3; k++) cf[3 * P1 + k] += fT[k];
+    }
+    d->contact_force_sum += norm3(cf + 3 * P1);
+    d->contact_reaction_sum += dot3(a + 3 * P1, n2) * m1;
+    ext_step += dot3(cf + 3 * P1, v + 3 * P2);
+  }
+  d->ext_forces_work += ext_step * dt;
+}
+
+/* the law Domain::contact_alg selects (Source/SolverLeapfrog.cpp:234-239;
+ * SolveDiffUpdateKickDrift always calls Wang's, Source/SolverKickDrift.cpp:211) */
+static void contact_forces(wfo_domain *d);
+static void contact_law(wfo_domain *d, int leapfrog) {
+  if (!leapfrog || d->contact_alg == 1) contact_forces(d);
+  else if (d->contact_alg == 0) contact_forces_fraser(d);
+  else if (d->contact_alg == 2) contact_forces_seo(d);
+  else if (d->contact_alg == 4) contact_forces_ls(d);
+}
+
 /* Domain::CalcContactForcesWang (Source/Contact_Wang.cpp:15-348), 3D,
  * no contact heat: penalty kij = 2 m / dt^2 on the penetration of the
  * predicted position into the triangle's plane, damping DFAC, Coulomb
@@ -1724,9 +1967,8 @@ static void contact_forces(wfo_domain *d) {
     /* Fraser 3-119 (Source/Contact_Wang.cpp:245-247): the contact area */
     const double dens = rho[P1];
     const double dS2 = pow(m1 / dens, 2.0 / 3.0);
-    const double *mfr = d->mat_fr[d->fi[WFO_MATIDX][P1]];
-    const double fr_sta = mfr[0] >= 0. ? mfr[0] : d->friction_sta;
-    const double fr_dyn = mfr[0] >= 0. ? mfr[1] : d->friction_dyn;
+    double fr_sta, fr_dyn;
+    pair_friction(d, P1, &fr_sta, &fr_dyn);
     if (fr_sta > 0.) {
       double du[3], dtg[3], tgf[3];
       for (int k = 0; k < 3; k++) du[k] = x_pred[k] - x[3 * P1 + k] - v[3 * P2 + k] * dt;
@@ -1886,6 +2128,19 @@ int wfo_solve_init(wfo_domain *d, double dt) {
      * and the free surface before the loop */
     for (int i = 0; i < d->n; i++) d->fi[WFO_IDORIG][i] = d->fi[WFO_ID][i];
     d->min_force_ts = dt;   /* Source/SolverLeapfrog.cpp:95 */
+    if (d->contact_alg == 0 || d->contact_alg == 2) {
+      /* Particle::cont_stiff: SolveDiffUpdateLeapFrog never sets it; the
+       * formula SolveDiffUpdateKickDrift applies before its loop
+       * (Source/SolverKickDrift.cpp:79-84, Fraser 3-153) */
+      free(d->cont_stiff);
+      d->cont_stiff = (double *)xcalloc((size_t)d->n, sizeof(double));
+      for (int i = 0; i < d->n; i++) {
+        const double rho = d->f[WFO_RHO][i], Cs = d->f[WFO_CS][i], G = d->f[WFO_G][i];
+        const double bulk = Cs * Cs * rho;
+        const double dS = pow(d->f[WFO_MASS][i] / rho, 0.33333);
+        d->cont_stiff[i] = 9. * bulk * G / (3. * bulk + G) * dS;
+      }
+    }
     clear_nb_data(d);
     main_neighbour_search(d);
     if (calc_pair_pos_list(d)) return -1;
@@ -1973,7 +2228,7 @@ int wfo_step_leapfrog(wfo_domain *d) {
   wfo_calc_accel(d);
   d->react_pending = 1;
   wfo_general_after(d);
-  if (d->contact) contact_forces(d);
+  if (d->contact) contact_law(d, 1);
   double *v = d->f[WFO_V], *a = d->f[WFO_A];
   if (d->isfirst)
     for (int i = 0; i < 3 * d->n; i++) v[i] -= a[i] * 0.5 * d->deltat;

@@ -82,6 +82,15 @@ def oracle_from_case(case, threads=1):
         d.set_param("friction_dyn", case.contact[1])
         for idx, (fs, fd) in case.mat_friction.items():
             d.set_material_friction(idx, fs, fd)
+        if case.contact_law is not None:
+            law, pfac, bound = case.contact_law
+            d.set_param("contact_alg", law)
+            d.set_param("PFAC", pfac)
+            d.set_param("fric_bound", int(bool(bound)))
+        if case.friction_linear is not None:
+            d.set_param("friction_linear", 1)
+            d.set_param("friction_m", case.friction_linear[0])
+            d.set_param("friction_b", case.friction_linear[1])
         d.set_param("DFAC", case.contact[2])
         d.set_param("cont_heat_cond", int(bool(case.contact_heat[0])))
         d.set_param("cont_heat_fric", int(bool(case.contact_heat[1])))

@@ -209,3 +209,149 @@ def test_per_material_friction_and_group_contact_sums():
         for m in range(2):
             fz, gz = lone.contact_force(m)[2], d.group_contact_force(i, m)[2]
             assert abs(fz - gz) < 1e-6 * abs(gz)
+
+
+def _cont_stiff():
+    """Fraser 3-153 as SolveDiffUpdateKickDrift forms it: 9 K G / (3 K + G) dS,
+    K = Cs^2 rho, dS = (m / rho)^0.33333."""
+    K = E / (3 * (1 - 2 * NU))
+    Cs = math.sqrt(K / RHO)
+    G = E / (2 * (1 + NU))
+    bulk = Cs * Cs * RHO
+    return 9.0 * bulk * G / (3.0 * bulk + G) * (DX ** 3) ** 0.33333
+
+
+def _law_block(law, pfac, **kw):
+    d, x, ns, z0 = _block(**kw)
+    d.set_param("contact_alg", law)
+    d.set_param("PFAC", pfac)
+    return d, x, ns, z0
+
+
+@pytest.mark.parametrize("mu_d", [0.0, 0.3])
+def test_fraser_law_closed_forms(mu_d):
+    """Fraser's law (Source/Contact.cpp:223-554) on a block moving up at vz
+    under a resting plane (gap 0.4 h): the top row approaches
+    (delta_ = vz), its time to contact t_c = (h - gap) / (-vz) is below dt,
+    penetration (dt - t_c) vz = dt vz + h - gap, normal force
+    PFAC cont_stiff (dt vz + h - gap); with a tangential velocity above
+    1e-3 the dynamic branch ASSIGNS contforce = -mu_d |F| t / m.  The force
+    is not applied: the block moves as without the plane."""
+    dt, vx, vz, pfac = 1e-8, 0.05, 1.0, 0.7
+    d, x, ns, z0 = _law_block(0, pfac, v0=(vx, 0.0, vz), mu=(0.2, mu_d))
+    d.solve_init(dt)
+    d.step()
+    m = RHO * DX ** 3
+    top = np.isclose(x[:, 2], x[:, 2].max())
+    Fn = pfac * _cont_stiff() * (dt * vz + H - (z0 - x[top, 2]))
+    cf = d.f("CONTFORCE")[:ns]
+    if mu_d == 0.0:
+        assert np.allclose(cf[top, 2], -Fn, rtol=1e-9) and np.all(cf[top, :2] == 0)
+    else:
+        assert np.allclose(cf[top, 0], -mu_d * Fn / m, rtol=1e-9) and np.all(cf[top, 1:] == 0)
+    assert np.all(cf[~top] == 0)
+    assert d.param("contact_force_sum") == pytest.approx(Fn.sum(), rel=1e-9)
+    far, _, _, _ = _law_block(0, pfac, v0=(vx, 0.0, vz), mu=(0.2, mu_d), plane_gap=3 * H)
+    far.solve_init(dt)
+    far.step()
+    assert np.array_equal(d.f("DISP")[:ns], far.f("DISP")[:ns])
+
+
+def test_fraser_bound_friction_removes_tangential_acceleration():
+    """fric_type Fr_Bound: a contacting particle's acceleration loses its
+    tangential part (a -= a - (a.n) n), so a sheared block's top row keeps
+    its tangential velocity through the first step (v += a dt / 2), while
+    without the bound it changes."""
+    dt, pfac = 1e-8, 0.7
+    g = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(4), indexing="ij"), -1).reshape(-1, 3)
+    zc = (g[:, 2] + 0.5) * DX
+    v0 = np.stack([200.0 * (zc - zc.mean()), np.zeros_like(zc), np.full_like(zc, 1.0)], axis=1)
+    out = {}
+    for bound in (0, 1):
+        d, x, ns, z0 = _law_block(0, pfac)
+        d.f("V")[:ns] = v0
+        d.f("ALPHA")[:ns] = 1.0  # Monaghan viscosity on the shear flow: a_t != 0
+        d.set_param("fric_bound", bound)
+        d.solve_init(dt)
+        d.step()
+        top = np.isclose(x[:, 2], x[:, 2].max())
+        out[bound] = d.f("V")[:ns][top, :2] - v0[top, :2]
+    assert np.abs(out[0]).max() > 0
+    assert np.all(out[1] == 0)
+
+
+@pytest.mark.parametrize("vx", [0.0, 0.5])
+def test_seo_law_closed_forms(vx):
+    """Seo's law (Source/Contact.cpp:562-770): a block under the plane (gap
+    0.4 h), normal force PFAC cont_stiff (h - gap) on the top row; with a
+    sliding velocity vx and no tangential acceleration the static branch
+    holds and contforce -= PFAC cont_stiff vx dt / m.  Not applied."""
+    dt, pfac, mu_s = 1e-8, 0.7, 0.2
+    d, x, ns, z0 = _law_block(2, pfac, v0=(vx, 0.0, 0.0), mu=(mu_s, mu_s))
+    d.solve_init(dt)
+    d.step()
+    m = RHO * DX ** 3
+    ks = pfac * _cont_stiff()
+    top = np.isclose(x[:, 2], x[:, 2].max())
+    Fn = ks * (H - (z0 - x[top, 2]))
+    cf = d.f("CONTFORCE")[:ns]
+    assert np.allclose(cf[top, 2], -Fn, rtol=1e-12)
+    if vx:
+        assert np.allclose(cf[top, 0], -ks * vx * dt / m, rtol=1e-9)
+    else:
+        assert np.all(cf[top, :2] == 0)
+    assert np.all(cf[~top] == 0)
+    far, _, _, _ = _law_block(2, pfac, v0=(vx, 0.0, 0.0), mu=(mu_s, mu_s), plane_gap=3 * H)
+    far.solve_init(dt)
+    far.step()
+    assert np.array_equal(d.f("DISP")[:ns], far.f("DISP")[:ns])
+
+
+@pytest.mark.parametrize("vx,mu_s", [(0.0, 0.2), (0.5, 0.2), (0.5, 1e-9)])
+def test_lsdyna_law_closed_forms(vx, mu_s):
+    """LS-Dyna's law (Source/Contact_LS.cpp:15-292): penalty
+    2 m / dt^2 PFAC on the predicted penetration h - gap; friction: the trial
+    force -PFAC m v_r / dt, returned to the cone mu_s |F_n|.  Not applied;
+    contact_reaction_sum = sum m a.n = 0 here."""
+    dt, pfac = 1e-8, 0.7
+    d, x, ns, z0 = _law_block(4, pfac, v0=(vx, 0.0, 0.0), mu=(mu_s, mu_s))
+    d.solve_init(dt)
+    d.step()
+    m = RHO * DX ** 3
+    kij = 2.0 * m / (dt * dt) * pfac
+    top = np.isclose(x[:, 2], x[:, 2].max())
+    Fn = kij * (H - (z0 - x[top, 2]))
+    cf = d.f("CONTFORCE")[:ns]
+    trial = pfac * m * vx / dt
+    assert np.allclose(cf[top, 2], -Fn, rtol=1e-12)
+    if vx == 0.0:
+        assert np.all(cf[top, :2] == 0)
+    elif trial > mu_s * Fn.min():
+        assert np.allclose(cf[top, 0], -mu_s * Fn, rtol=1e-9)
+    else:
+        assert np.allclose(cf[top, 0], -trial, rtol=1e-12)
+    assert np.all(cf[~top] == 0)
+    assert d.param("contact_reaction_sum") == 0.0
+    assert d.param("contact_force_sum") == pytest.approx(np.linalg.norm(cf[top], axis=1).sum(), rel=1e-12)
+
+
+def test_linear_friction_function():
+    """friction_function Linear: friction_sta = friction_dyn = m T + b of the
+    contacting particle (Source/Contact_Wang.cpp:241-244), here in the Wang
+    dynamic branch: contforce_t = -mu(T) |F_n|."""
+    dt, vx, T0, fm, fb = 1e-8, 0.5, 300.0, 1e-12, 1e-10
+    d, x, ns, z0 = _block(v0=(vx, 0.0, 0.0), mu=(5.0, 5.0))
+    d.f("T")[:ns] = T0
+    d.set_param("friction_linear", 1)
+    d.set_param("friction_m", fm)
+    d.set_param("friction_b", fb)
+    d.solve_init(dt)
+    d.step()
+    m = RHO * DX ** 3
+    kij = 2.0 * m / (dt * dt)
+    top = np.isclose(x[:, 2], x[:, 2].max())
+    Fn = kij * (H - (z0 - x[top, 2]))
+    mu = fm * T0 + fb
+    assert kij * vx * dt > mu * Fn.max()  # sliding at mu(T) = 4e-10, static at 5
+    cf = d.f("CONTFORCE")[:ns][top]
+    assert np.allclose(cf[:, 0], -mu * Fn, rtol=1e-12)

@@ -98,7 +98,11 @@ EXPORTS = [
     "wf_set_contact_heat", "wf_set_rigid_temperature", "wf_prepare", "wf_step_part",
     "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max", "wf_get_cell_grids",
     "wf_set_material_friction", "wf_set_group_sums", "wf_get_group_history",
+    "wf_set_contact_law", "wf_set_friction_function",
 ]
+
+# contact laws (Domain::contact_alg, Source/Domain.h:64)
+FRASER, WANG, SEO, LSDYNA = 0, 1, 2, 4
 
 # output tables (wf_pack_xdmf / wf_pack_csv): floats and ints per particle,
 # CSV columns
@@ -149,6 +153,8 @@ def lib():
         "wf_get_neighbour_stats": ([vp, I64P, ip], ctypes.c_int),
         "wf_get_cell_grids": ([vp, ip, ip], ctypes.c_int),
         "wf_set_material_friction": ([vp, i32, d, d], ctypes.c_int),
+        "wf_set_contact_law": ([vp, i32, d, i32], ctypes.c_int),
+        "wf_set_friction_function": ([vp, i32, d, d], ctypes.c_int),
         "wf_set_group_sums": ([vp, i32, i64], ctypes.c_int),
         "wf_get_group_history": ([vp, i32, i32, i32, dp, i64, I64P], ctypes.c_int),
         "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
@@ -428,6 +434,13 @@ class Solver:
 
     def set_material_friction(self, index, friction_sta, friction_dyn):
         self._check(lib().wf_set_material_friction(self.ctx, index, friction_sta, friction_dyn))
+
+    def set_contact_law(self, law, pfac=0.5, fric_bound=False):
+        """law: 0 Fraser, 1 Wang, 2 Seo, 4 LSDyna (Domain::contact_alg)."""
+        self._check(lib().wf_set_contact_law(self.ctx, law, pfac, int(bool(fric_bound))))
+
+    def set_friction_function(self, linear, m=0.0, b=0.0):
+        self._check(lib().wf_set_friction_function(self.ctx, int(bool(linear)), m, b))
 
     def set_group_sums(self, zone=-1, cap=4096):
         self._check(lib().wf_set_group_sums(self.ctx, zone, cap))

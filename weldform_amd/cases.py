@@ -49,6 +49,8 @@ class Case:
     sqrt_h_a: float = 0.0025
     mat_friction: dict = field(default_factory=dict)  # material idx -> (friction_sta, friction_dyn)
     group_sums: tuple = None    # (zone, cap) of wf_set_group_sums, or None
+    contact_law: tuple = None   # (law, PFAC, fric_bound) of wf_set_contact_law, or None (Wang)
+    friction_linear: tuple = None  # (friction_m, friction_b) of wf_set_friction_function, or None
     meta: dict = field(default_factory=dict)
 
     @property
@@ -407,6 +409,10 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
             s.set_rigid_temperature(case.rigid_T)
     for idx, (fs, fd) in case.mat_friction.items():
         s.set_material_friction(idx, fs, fd)
+    if case.contact_law is not None:
+        s.set_contact_law(*case.contact_law)
+    if case.friction_linear is not None:
+        s.set_friction_function(True, *case.friction_linear)
     if case.group_sums is not None:
         s.set_group_sums(*case.group_sums)
     s.init(case.dt)

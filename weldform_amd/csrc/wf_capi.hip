@@ -70,6 +70,10 @@ struct Ctx {
   wf_config cfg{};
   std::vector<MeshHost> meshes;
   double fr_sta = 0.0, fr_dyn = 0.0, dfac = 0.0;
+  // wf_set_contact_law / wf_set_friction_function (PFAC defaults to the JSON
+  // driver's penaltyFactor default, Source/WeldForm.cpp:685)
+  int contact_law = 1, fric_bound = 0, fric_linear = 0;
+  double pfac = 0.5, fr_m = 0.0, fr_b = 0.0;
   int heat_cond = 0, heat_fric = 0;
   double heat_hc = 0.0;
   std::vector<double> rigid_T;     // one per rigid particle (mesh order), 0 unless set
@@ -513,6 +517,13 @@ void allocate(Ctx *c) {
     d.fr_sta = c->fr_sta;
     d.fr_dyn = c->fr_dyn;
     d.dfac = c->dfac;
+    d.contact_law = c->contact_law;
+    d.pfac = c->pfac;
+    d.fric_bound = c->fric_bound;
+    d.fric_linear = c->fric_linear;
+    d.fr_m = c->fr_m;
+    d.fr_b = c->fr_b;
+    if (d.contact_law == 0 || d.contact_law == 2) d.cstiff = c->alloc<double>(np);
     d.rT = c->alloc<double>(ne);
     d.heat_cond = c->heat_cond;
     d.heat_fric = c->heat_fric;
@@ -634,6 +645,22 @@ void upload_all(Ctx *c) {
   dput(c, d.vol[1], vol);
   dput(c, d.pres, pres);
   dput(c, d.srec, rec);
+  if (d.cstiff) {
+    // Particle::cont_stiff: SolveDiffUpdateLeapFrog never sets it; the
+    // formula SolveDiffUpdateKickDrift applies before its loop
+    // (Source/SolverKickDrift.cpp:79-84, Fraser 3-153), from the initial
+    // density
+    std::vector<double> ks(np, 0.0);
+    for (int64_t k = 0; k < n; ++k) {
+      const int o = c->perm[k];
+      const wf_material &M = c->mats[mt[o]];
+      const double r = F[WF_F_DENSITY][o];
+      const double bulk = M.Cs * M.Cs * r;
+      const double dS = std::pow(F[WF_F_MASS][o] / r, 0.33333);
+      ks[k] = 9. * bulk * M.G / (3. * bulk + M.G) * dS;
+    }
+    dput(c, d.cstiff, ks);
+  }
   if (d.contact) {
     std::vector<double4> node(d.nnode), en(d.nrig), rx(d.nrig), z(d.nrig);
     std::vector<int4> el(d.nrig);
@@ -910,7 +937,9 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int
     // contact acts after AccelReduction and the first GeneralAfter
     // (Source/SolverLeapfrog.cpp:228-240); free-surface particles carry ID 1,
     // so no BC zone applies to them
-    if (d.contact) launch_contact(d, c->st, c->bc, cur, dt);
+    // SolveDiffUpdateKickDrift calls Wang's law whatever contact_alg says
+    // (Source/SolverKickDrift.cpp:211)
+    if (d.contact) launch_contact(d, c->st, c->bc, cur, dt, scheme == WF_LEAPFROG ? d.contact_law : 1);
     if (d.contact && d.ngroups > 0) launch_group_sums(d, c->st, 1);
     if (tp && d.contact) {
       d2d(c, tp->s.node, d.node, (size_t)d.nnode * sizeof(double4));
@@ -1013,6 +1042,8 @@ void adj_check(Ctx *c) {
   for (int m = 0; m < kMaxMat; ++m)
     if (c->mat_fr[2 * m] >= 0.0)
       throw Error(WF_EUNSUPPORTED, "the adjoint covers the context's friction, not per-material friction");
+  if (c->d.contact && (c->contact_law != 1 || c->fric_linear))
+    throw Error(WF_EUNSUPPORTED, "the adjoint covers Wang's contact law with constant friction");
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1952,6 +1983,34 @@ int wf_set_contact(void *ctx, double friction_sta, double friction_dyn, double d
       c->d.dfac = damping;
       c->drop_graphs();
     }
+    return WF_OK;
+  });
+}
+
+int wf_set_contact_law(void *ctx, int32_t law, double pfac, int32_t fric_bound) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_contact_law after wf_init");
+    if (law != 0 && law != 1 && law != 2 && law != 4)
+      throw Error(WF_EINVAL, "contact law: 0 Fraser, 1 Wang, 2 Seo or 4 LSDyna");
+    if (!(pfac > 0) || !std::isfinite(pfac)) throw Error(WF_EINVAL, "PFAC must be positive");
+    c->contact_law = law;
+    c->pfac = pfac;
+    c->fric_bound = fric_bound ? 1 : 0;
+    return WF_OK;
+  });
+}
+
+int wf_set_friction_function(void *ctx, int32_t linear, double m, double b) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_friction_function after wf_init");
+    if (linear && !(std::isfinite(m) && std::isfinite(b))) throw Error(WF_EINVAL, "non-finite friction coefficients");
+    c->fric_linear = linear ? 1 : 0;
+    c->fr_m = linear ? m : 0.0;
+    c->fr_b = linear ? b : 0.0;
     return WF_OK;
   });
 }

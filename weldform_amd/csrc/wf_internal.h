@@ -246,6 +246,13 @@ struct Dev {
   int *rmesh;
   double totmass;              // mean particle mass over solid and rigid particles
   double fr_sta, fr_dyn, dfac;
+  // the contact law (Domain::contact_alg, Source/Domain.h:64: 0 Fraser,
+  // 1 Wang, 2 Seo, 4 LSDyna), PFAC, fric_type == Fr_Bound, friction
+  // function Linear (fr = fr_m T + fr_b) and, for Fraser's and Seo's laws,
+  // Particle::cont_stiff per particle (Fraser 3-153, formed at set-up)
+  int contact_law, fric_bound, fric_linear;
+  double pfac, fr_m, fr_b;
+  double *cstiff;
   // contact heat (Source/Contact_Wang.cpp:245-324): conduction with
   // conductance hc from the rigid particles' temperatures rT, friction work
   // as heat; per particle q_fric_work, friction_hfl, cshearabs and
@@ -340,7 +347,8 @@ void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz);
 void launch_max_pl(const Dev &d, hipStream_t st);
 void launch_shadows(const Dev &d, hipStream_t st, int cur);
 void launch_surface(const Dev &d, hipStream_t st, int cur, int setup);
-void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt);
+// law: Domain::contact_alg (0 Fraser, 1 Wang, 2 Seo, 4 LSDyna)
+void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt, int law);
 // per-group sums of the step: what 0 the zone reaction (after the
 // acceleration pass), 1 the contact forces per mesh (after contact)
 void launch_group_sums(const Dev &d, hipStream_t st, int what);

@@ -2195,8 +2195,10 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
       double a[3] = {a4.x, a4.y, a4.z};
       const double h = x4.w, m1 = v4.w;
       const DevMat &Mi = d.mats[d.mat[i]];
-      const double fr_sta = Mi.fr_sta >= 0.0 ? Mi.fr_sta : d.fr_sta;
-      const double fr_dyn = Mi.fr_sta >= 0.0 ? Mi.fr_dyn : d.fr_dyn;
+      double fr_sta = Mi.fr_sta >= 0.0 ? Mi.fr_sta : d.fr_sta;
+      double fr_dyn = Mi.fr_sta >= 0.0 ? Mi.fr_dyn : d.fr_dyn;
+      // friction_function Linear (Source/Contact_Wang.cpp:241-244)
+      if (d.fric_linear) fr_sta = fr_dyn = d.fr_m * d.T[i] + d.fr_b;
       for (int q = 0; q < cnt; ++q) {
         const int r = d.cidx[(long long)q * d.npad + i];
         const int mi = d.rmesh[r];
@@ -2313,6 +2315,156 @@ WF_KERNEL void k_contact(Dev d, int cur, double dt) {
   }
 }
 
+// Fraser's (LAW 0, Source/Contact.cpp:223-554), Seo's (LAW 2,
+// Source/Contact.cpp:562-770) and LS-Dyna's (LAW 4, Source/Contact_LS.cpp:
+// 15-292) contact laws, per particle over its partners in index order.  Under
+// Leapfrog none of them adds its force to a: contforce (assigned per pair,
+// friction terms added or assigned after it) and the sums are outputs; only
+// Fraser's Fr_Bound friction changes a, removing its tangential part.
+//   Fraser: pairs approaching (delta_ = -n.v_r > 0) whose time to contact
+//     t_c = (h + pplane - n.x) / (-delta_) is below dt, contact point
+//     x + v t_c - h n; penetration (dt - t_c) delta_; stiffness
+//     PFAC cont_stiff; a sliding pair (|v_t| > 1e-3) assigns
+//     contforce = -friction_dyn |contforce| t / m.  The static branch reads
+//     a force the reference never sets before it and adds nothing here.
+//   Seo: dist = n.x - pplane < h, contact point x - dist n; stiffness
+//     PFAC cont_stiff; static friction when |m a_t| < friction_sta
+//     |contforce|: contforce -= PFAC cont_stiff (v_r dt)_t / m.
+//   LS-Dyna: Wang's geometry (predicted position), stiffness
+//     2 m / dt^2 PFAC; friction: the trial force -PFAC m v_r / dt returned
+//     to the cone friction_sta |contforce| (the reference's old tangential
+//     force is never set: zero) and added; contact_reaction_sum and the
+//     external work as Wang's; the contact heat fields are reset.
+// dt_fext = 2 m |v| / |contforce| per pair: Domain::min_force_ts takes the
+// least positive one (Fraser, Seo) or the least (LS-Dyna) below 1000.
+template <int LAW>
+WF_KERNEL void k_contact_law(Dev d, int cur, double dt) {
+  if (dt < 0) dt = d.flags->dt;
+  __shared__ double sh[kBlock / 64];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double cfsum = 0.0, creact = 0.0, extw = 0.0, fts = INFINITY;
+  if (i < d.n) {
+    double cf[3] = {0, 0, 0};
+    const int cnt = d.ccnt[i];
+    if (cnt > 0) {
+      const double4 x4 = d.xh[cur][i];
+      const double4 v4 = d.vm[i];
+      const double4 a4 = d.acc[i];
+      const double x[3] = {x4.x, x4.y, x4.z}, v[3] = {v4.x, v4.y, v4.z};
+      double a[3] = {a4.x, a4.y, a4.z};
+      const double h = x4.w, m1 = v4.w;
+      const DevMat &Mi = d.mats[d.mat[i]];
+      double fr_sta = Mi.fr_sta >= 0.0 ? Mi.fr_sta : d.fr_sta;
+      double fr_dyn = Mi.fr_sta >= 0.0 ? Mi.fr_dyn : d.fr_dyn;
+      // friction_function Linear (Source/Contact_LS.cpp:217-220)
+      if (LAW == 4 && d.fric_linear) fr_sta = fr_dyn = d.fr_m * d.T[i] + d.fr_b;
+      const double ks = (LAW == 0 || LAW == 2) ? d.cstiff[i] : 0.0;
+      const double nv = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      for (int q = 0; q < cnt; ++q) {
+        const int r = d.cidx[(long long)q * d.npad + i];
+        const double4 en = d.enorm[r];
+        const double n2[3] = {en.x, en.y, en.z};
+        const double4 v24 = d.rv[r];
+        const double v2[3] = {v24.x, v24.y, v24.z};
+        const double vr[3] = {v[0] - v2[0], v[1] - v2[1], v[2] - v2[2]};
+        const double delta_ = -(n2[0] * vr[0] + n2[1] * vr[1] + n2[2] * vr[2]);
+        double qj[3], delta;
+        if (LAW == 0) {
+          if (!(delta_ > 0)) continue;
+          const double tc = (h + en.w - (n2[0] * x[0] + n2[1] * x[1] + n2[2] * x[2])) / (-delta_);
+          if (!(tc < dt)) continue;
+          for (int k = 0; k < 3; ++k) qj[k] = x[k] + (v[k] * tc) - (h * n2[k]);
+          delta = (dt - tc) * delta_;
+        } else {
+          double xp[3];
+          for (int k = 0; k < 3; ++k) xp[k] = LAW == 4 ? x[k] + v[k] * dt + a[k] * dt * dt / 2.0 : x[k];
+          const double dist = (n2[0] * xp[0] + n2[1] * xp[1] + n2[2] * xp[2]) - en.w;
+          if (!(dist < h)) continue;
+          for (int k = 0; k < 3; ++k) qj[k] = x[k] - dist * n2[k];
+          delta = h - dist;
+        }
+        const int4 el = d.elem[r];
+        const int nd[3] = {el.x, el.y, el.z};
+        bool inside = true;
+        for (int ii = 0; ii < 3 && inside; ++ii) {
+          const int jj = ii + 1 > 2 ? 0 : ii + 1;
+          const double4 ni = d.node[nd[ii]], nj = d.node[nd[jj]];
+          const double ea[3] = {nj.x - ni.x, nj.y - ni.y, nj.z - ni.z};
+          const double eb[3] = {qj[0] - ni.x, qj[1] - ni.y, qj[2] - ni.z};
+          double c[3];
+          cross_d(ea, eb, c);
+          if (c[0] * n2[0] + c[1] * n2[1] + c[2] * n2[2] < 0.0) inside = false;
+        }
+        if (!inside) continue;
+        const double kij = LAW == 4 ? 2.0 * m1 / (dt * dt) * d.pfac : d.pfac * ks;
+        const double omega = sqrt(kij / m1);
+        const double psi = 2. * m1 * omega * d.dfac;
+        const double ncf0 = kij * delta - psi * delta_;
+        for (int k = 0; k < 3; ++k) cf[k] = ncf0 * n2[k];
+        const double nc = sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
+        if (LAW != 4) cfsum += nc;
+        const double dt_fext = 1. * (m1 * 2. * nv / nc);
+        if (dt_fext < 1000. && dt_fext < fts && (LAW == 4 || dt_fext > 0)) fts = dt_fext;
+        const double an = a[0] * n2[0] + a[1] * n2[1] + a[2] * n2[2];
+        const double atg[3] = {a[0] - an * n2[0], a[1] - an * n2[1], a[2] - an * n2[2]};
+        if (LAW == 0) {
+          if (d.fric_bound) {
+            for (int k = 0; k < 3; ++k) a[k] -= atg[k];
+          } else if (fr_dyn > 0.) {
+            double tg[3];
+            for (int k = 0; k < 3; ++k) tg[k] = vr[k] + delta_ * n2[k];
+            const double ntg = sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+            if (ntg > 1.e-3)
+              for (int k = 0; k < 3; ++k) cf[k] = -(fr_dyn * nc * (tg[k] / ntg)) / m1;
+          }
+        } else if (LAW == 2) {
+          double du[3], tg[3], rt[3];
+          for (int k = 0; k < 3; ++k) du[k] = vr[k] * dt;
+          const double dn = du[0] * n2[0] + du[1] * n2[1] + du[2] * n2[2];
+          for (int k = 0; k < 3; ++k) tg[k] = kij * (du[k] - dn * n2[k]);
+          for (int k = 0; k < 3; ++k) rt[k] = atg[k] * m1;
+          if (fr_sta > 0. && sqrt(rt[0] * rt[0] + rt[1] * rt[1] + rt[2] * rt[2]) < fr_sta * nc)
+            for (int k = 0; k < 3; ++k) cf[k] -= tg[k] / m1;
+        } else {
+          if (fr_sta > 0.) {
+            double fs[3];
+            for (int k = 0; k < 3; ++k) fs[k] = 0.0 - d.pfac * m1 * vr[k] / dt;
+            const double fy = fr_sta * nc;
+            const double nfs = sqrt(fs[0] * fs[0] + fs[1] * fs[1] + fs[2] * fs[2]);
+            for (int k = 0; k < 3; ++k) cf[k] += nfs > fy ? fy * fs[k] / nfs : fs[k];
+          }
+          cfsum += sqrt(cf[0] * cf[0] + cf[1] * cf[1] + cf[2] * cf[2]);
+          creact += an * m1;
+          extw += cf[0] * v2[0] + cf[1] * v2[1] + cf[2] * v2[2];
+        }
+      }
+      if (LAW == 0 && d.fric_bound) d.acc[i] = make_double4(a[0], a[1], a[2], a4.w);
+    }
+    d.cforce[i] = make_double4(cf[0], cf[1], cf[2], 0.0);
+    // the laws leave m_contact_force alone: no per-group contact sums
+    if (d.gcf && cnt > 0)
+      for (int m = 0; m < d.nmesh; ++m) d.gcf[(long long)m * d.npad + i] = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (d.qfric) {
+      d.qfric[i] = 0.0;
+      d.fhfl[i] = 0.0;
+      d.cshear[i] = 0.0;
+      d.qcont[i] = 0.0;
+    }
+  }
+  const double r1 = block_reduce<0>(cfsum, sh);
+  const double r2 = block_reduce<0>(creact, sh);
+  const double r3 = block_reduce<0>(extw, sh);
+  const double r4 = block_reduce<2>(fts, sh);
+  if (threadIdx.x == 0) {
+    for (int m = 0; m < d.nmesh; ++m)
+      for (int k = 0; k < 3; ++k) part(d, P_CFORCE + 3 * m + k, blockIdx.x) = 0.0;
+    part(d, P_FTS, blockIdx.x) = r4;
+    part(d, P_CFSUM, blockIdx.x) = r1;
+    part(d, P_CREACT, blockIdx.x) = r2;
+    part(d, P_EXTW, blockIdx.x) = r3 * dt;
+  }
+}
+
 // Per-group sums (wf_set_group_sums): one block per group walks the group's
 // particles in internal order (thread t takes every kGroupThreads-th) and
 // reduces in a fixed order, so the sums are deterministic.  what 0: the
@@ -2417,9 +2569,14 @@ __global__ __launch_bounds__(1024) void k_mesh_update(Dev d, double dt) {
 void launch_surface(const Dev &d, hipStream_t st, int cur, int setup) {
   k_surface<<<nblk(d.n), kBlock, 0, st>>>(d, cur, setup);
 }
-void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt) {
+void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt, int law) {
   k_rigid_bc<<<(d.nrig + kBlock - 1) / kBlock, kBlock, 0, st>>>(d, bc);
-  k_contact<<<d.nblocks, kBlock, 0, st>>>(d, cur, dt);
+  switch (law) {
+    case 0: k_contact_law<0><<<d.nblocks, kBlock, 0, st>>>(d, cur, dt); break;
+    case 2: k_contact_law<2><<<d.nblocks, kBlock, 0, st>>>(d, cur, dt); break;
+    case 4: k_contact_law<4><<<d.nblocks, kBlock, 0, st>>>(d, cur, dt); break;
+    default: k_contact<<<d.nblocks, kBlock, 0, st>>>(d, cur, dt);
+  }
 }
 void launch_rigid_drift(const Dev &d, hipStream_t st, double dt) {
   k_rigid_drift<<<(d.nrig + kBlock - 1) / kBlock, kBlock, 0, st>>>(d, dt);

@@ -261,8 +261,10 @@ int main(int argc, char **argv) try {
     readValue(contact_[0]["fricCoeffDynamic"], fric_dyn);
     readValue(contact_[0]["heatCondCoeff"], heat_cond);
     readValue(contact_[0]["penaltyFactor"], penaltyfac);
-    if (cont_alg == "Seo") dom.contact_alg = 1;
-    else if (cont_alg == "LSDyna") dom.contact_alg = 2;
+    // Source/WeldForm.cpp:695-705: "Fraser" (disabled there) and any other
+    // value keep Wang's law
+    if (cont_alg == "Seo") dom.contact_alg = Seo;
+    else if (cont_alg == "LSDyna") dom.contact_alg = LSDyna;
     bool heat_cond_ = false, heat_fric_ = false;
     readValue(contact_[0]["heatConductance"], heat_cond_);
     readValue(contact_[0]["heatFriction"], heat_fric_);
