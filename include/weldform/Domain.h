@@ -102,6 +102,7 @@ class Domain {
   bool auto_ts = true, auto_ts_acc = false, auto_ts_cont = false;
   int ts_nb_inc = 5;
   bool nonlock_sum = true;           // false: the Locking sums (float m/rho weights in the rates)
+  bool model_damage = false;         // damage of the materials' DamageModel (Johnson-Cook)
   bool gradKernelCorr = false;       // Bonet kernel-gradient correction
   bool thermal_solver = false;
   bool pl_heating = false;
@@ -556,6 +557,7 @@ class Domain {
     put9(WF_F_STRAIN_PL, &Particle::Strain_pl);
     put9(WF_F_STRAIN_PL_INCR, &Particle::Strain_pl_incr);
     put1(WF_F_PL_STRAIN, &Particle::pl_strain);
+    if (model_damage) put1(WF_F_DAMAGE, &Particle::dam_D);
     put1(WF_F_DPL_STRAIN, &Particle::delta_pl_strain);
     put1(WF_F_SIGMAY, &Particle::Sigmay);
     put1(WF_F_T, &Particle::T);
@@ -605,6 +607,7 @@ class Domain {
     get9(WF_F_STRAIN_PL, &Particle::Strain_pl);
     get9(WF_F_STRAIN_PL_INCR, &Particle::Strain_pl_incr);
     get1(WF_F_PL_STRAIN, &Particle::pl_strain);
+    if (model_damage) get1(WF_F_DAMAGE, &Particle::dam_D);
     get1(WF_F_DPL_STRAIN, &Particle::delta_pl_strain);
     get1(WF_F_SIGMAY, &Particle::Sigmay);
     get1(WF_F_EFF_SR, &Particle::eff_strain_rate);
@@ -800,6 +803,19 @@ class Domain {
     }
     if (wf_create(&cfg, (int64_t)n, &ctx_) != WF_OK) throw new Fatal(std::string("weldform_hip: ") + wf_last_error(nullptr));
     for (size_t k = 0; k < table.size(); ++k) check(wf_set_material(ctx_, (int32_t)k, &table[k]));
+    if (model_damage) {
+      // each material's JohnsonCookDamage (Source/Damage.cpp:9-46)
+      std::vector<char> done(table.size(), 0);
+      for (size_t i = 0; i < n; ++i) {
+        const Particle *P = Particles[i];
+        if (done[mi[i]] || !P->mat || !P->mat->damage) continue;
+        const JohnsonCookDamage *J = dynamic_cast<const JohnsonCookDamage *>(P->mat->damage);
+        if (!J) throw new Fatal("only the Johnson-Cook damage criterion is implemented on the device path");
+        const double D[5] = {J->m_D1, J->m_D2, J->m_D3, J->m_D4, J->m_D5};
+        check(wf_set_material_damage(ctx_, mi[i], 1, D));
+        done[mi[i]] = 1;
+      }
+    }
     check(wf_set_int_field(ctx_, WF_I_MATERIAL, mi.data()));
     upload();
     for (const auto &a : amps)

@@ -4,6 +4,7 @@
 // Yield and tangent laws themselves run on the device.
 #pragma once
 #include <cmath>
+#include <string>
 
 #include "matvec.h"
 
@@ -16,6 +17,40 @@ class Elastic_ {
 
  private:
   double E_m = 0.0, nu_m = 0.0;
+};
+
+class Material_;
+
+// damage models (Source/Material.h:14-62): the Johnson-Cook criterion runs
+// on the device path (Domain::model_damage); Rankine's pair criterion reads
+// an unset strain threshold in the reference (Source/Damage.cpp:17,65) and
+// is refused by the solver
+class DamageModel {
+ public:
+  DamageModel() {}
+  DamageModel(const double &smax_, const double &Gf_) : Gf(Gf_), sigma_max(smax_) {}
+  virtual ~DamageModel() {}
+  virtual std::string getDamageCriterion() { return std::string(""); }
+  double Gf = 0.0, sigma_max = 0.0, delta_max = 0.0;
+  Material_ *mat = nullptr;
+};
+
+class RankineDamage : public DamageModel {
+ public:
+  RankineDamage(const double &smax_, const double &Gf_) : DamageModel(smax_, Gf_) {}
+  std::string getDamageCriterion() override { return std::string("Rankine"); }
+};
+
+// eps_f = (D1 + D2 exp(D3 s)) (1 + D4 ln(rate / eps_0)) (1 + D5 T_h), on the
+// device (Source/Material.cpp:76-92)
+class JohnsonCookDamage : public DamageModel {
+ public:
+  JohnsonCookDamage(const double &D1, const double &D2, const double &D3, const double &D4, const double &D5,
+                    const double &eps0_)
+      : m_D1(D1), m_D2(D2), m_D3(D3), m_D4(D4), m_D5(D5), m_eps0(eps0_) {}
+  std::string getDamageCriterion() override { return std::string("JohnsonCook"); }
+  double m_D1, m_D2, m_D3, m_D4, m_D5;
+  double m_eps0;  // the material's reference strain rate (the device takes the material's)
 };
 
 class Material_ {
@@ -31,6 +66,7 @@ class Material_ {
   virtual double CalcYieldStress(const double &, const double &, const double &) { return 0.0; }
   double &getT_t() { return T_t; }
   double &getT_m() { return T_m; }
+  DamageModel *damage = nullptr;
 
  protected:
   Elastic_ elastic_m;

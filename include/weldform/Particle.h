@@ -32,6 +32,7 @@ class Particle {
   size_t PresEq = 0;
   Mat3_t Sigma, ShearStress, StrainRate, RotationRate, Strain, Strain_pl, Strain_pl_incr, TIR;
   double pl_strain = 0.0, delta_pl_strain = 0.0, Sigmay = 0.0, eff_strain_rate = 0.0, Sigma_eq = 0.0;
+  double dam_D = 0.0;  // damage (Domain::model_damage)
   double G = 0.0, K = 0.0, Ep = 0.0, Et = 0.0;
   double Alpha = 0.0, Beta = 0.0, TI = 0.0, TIn = 4.0, TIInitDist = 0.0;
   int ID, Material = 0, Material_model = BILINEAR;

@@ -108,7 +108,8 @@ extern "C" {
                                step's contact (Source/Contact_Wang.cpp)      */
 #define WF_F_GRADCORR 29    /* [9] read-only: Particle::gradCorrM (identity
                                unless wf_config.grad_corr)                  */
-#define WF_NFIELDS 30
+#define WF_F_DAMAGE 30      /* [1] Particle::dam_D (damage, 0..1)             */
+#define WF_NFIELDS 31
 
 /* rigid-surface particle fields (wf_get_rigid_field), 3 doubles each, in
  * the order the planes were added, element order within a plane */
@@ -303,6 +304,23 @@ int wf_set_material_friction(void *ctx, int32_t index, double friction_sta, doub
  * particle (Source/Contact_Wang.cpp:241-244, Source/Contact_LS.cpp:217-220);
  * linear 0 restores the constant coefficients.  Before wf_init. */
 int wf_set_contact_law(void *ctx, int32_t law, double pfac, int32_t fric_bound);
+/* wf_set_material_damage <- Material_::damage = new JohnsonCookDamage(D1,
+ * D2, D3, D4, D5, eps_0) and Domain::model_damage (Source/Material.h:47-62,
+ * scene keys "damageModel": "JohnsonCook" and "damageParams",
+ * Source/WeldForm.cpp:293-321): model 1 Johnson-Cook, D[5] = D1..D5; model
+ * 0 none.  With damage on any material the Leapfrog step adds
+ * Domain::CalcDamage after the stress update (Source/Damage.cpp:9-46):
+ * dam_D += delta_pl_strain / eps_f(delta_pl_strain / dt, -P / Sigma_eq, T)
+ * with eps_f = (D1 + D2 exp(D3 s)) (1 + D4 ln(rate / eps_0)) (1 + D5 T_h)
+ * (Source/Material.cpp:76-92), capped at 1; the acceleration and density
+ * passes weight each pair by 1 - (D_i + D_j) / 2 (Source/InteractionAlt.cpp:
+ * 54-60,558-572); the free surface is refreshed every step; the neighbour
+ * list of the first step is kept (Source/SolverLeapfrog.cpp:160-206).  The
+ * material must be Johnson-Cook (its T_t, T_m, eps_0 enter eps_f), the
+ * context FP64 with the Locking sums (wf_config.locking 1, which the scene
+ * driver sets with damage), and only WF_LEAPFROG steps (KickDrift has no
+ * damage).  Before wf_init. */
+int wf_set_material_damage(void *ctx, int32_t index, int32_t model, const double D[5]);
 int wf_set_friction_function(void *ctx, int32_t linear, double m, double b);
 /* wf_set_contact_heat <- Domain::cont_heat_cond / contact_hc / cont_heat_fric
  * (Source/Domain.h:339-347, set from a scene's Contact "heatConductance",

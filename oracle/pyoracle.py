@@ -21,7 +21,7 @@ F64 = ["X", "V", "A", "XPREV", "DISP", "RHO", "DRHO", "MASS", "H", "REFRHO", "P"
        "EPL", "EPLINC", "PLSTRAIN", "DPLSTRAIN", "SIGMAY", "EFFSR", "EP", "ET",
        "T", "DTDT", "KT", "CPT", "THEXP", "QCONV", "QSRC", "QPL", "QCONT",
        "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR", "GCM",
-       "QFRIC", "CSHEAR", "FHFL"]
+       "QFRIC", "CSHEAR", "FHFL", "DAMD"]
 F64_DIM = dict(X=3, V=3, A=3, XPREV=3, DISP=3, SIGMA=9, SHEAR=9, SR=9, RR=9,
                STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9, GCM=9)
 I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB", "ELEMENT", "MESHIDX", "IDORIG"]
@@ -58,6 +58,7 @@ def lib():
         L.wfo_get_param.argtypes = [vp, ctypes.c_char_p]
         L.wfo_set_material.argtypes = [vp, i32, ctypes.POINTER(Material)]
         L.wfo_set_material_friction.argtypes = [vp, i32, d, d]
+        L.wfo_set_material_damage.argtypes = [vp, i32, i32, dp]
         L.wfo_set_threads.argtypes = [i32]
         L.wfo_last_error.restype = ctypes.c_char_p
         L.wfo_add_velocity_bc.argtypes = [vp, i32, i32, dp, i32, d]
@@ -169,6 +170,10 @@ class Domain:
     def set_material(self, idx, **kw):
         m = Material(**kw)
         self._check(lib().wfo_set_material(self.h, int(idx), ctypes.byref(m)))
+
+    def set_material_damage(self, idx, model, D=(0.0, 0.0, 0.0, 0.0, 0.0)):
+        Dv = np.ascontiguousarray(D, dtype=np.float64)
+        self._check(lib().wfo_set_material_damage(self.h, int(idx), int(model), _dp(Dv)))
 
     def set_material_friction(self, idx, sta, dyn):
         self._check(lib().wfo_set_material_friction(self.h, int(idx), float(sta), float(dyn)))

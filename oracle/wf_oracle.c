@@ -41,7 +41,8 @@ static const int k_fdim[WFO_NFIELDS] = {
     1, 1, 1,              /* TI TIN TIDIST */
     9,                    /* TIR */
     9,                    /* GCM */
-    1, 1, 1               /* QFRIC CSHEAR FHFL */
+    1, 1, 1,              /* QFRIC CSHEAR FHFL */
+    1                     /* DAMD */
 };
 
 typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T, value_ang[3]; } bc_t;
@@ -66,6 +67,11 @@ struct wfo_domain {
   /* per-material Coulomb friction (static, dynamic); a negative static
    * value means the domain's friction_sta / friction_dyn */
   double mat_fr[256][2];
+  /* Domain::model_damage and each material's DamageModel (0 none,
+   * 1 JohnsonCookDamage D1..D5, Source/Material.h:47-62) */
+  int model_damage;
+  int dam_model[256];
+  double dam_par[256][5];
   /* domain flags / parameters (Source/Domain.cpp:78-175) */
   double Cellfac, DomMax[3], DomMin[3], Time, deltat, pl_work_heat_frac;
   int ts_nb_inc, thermal_solver, pl_heating;
@@ -214,6 +220,14 @@ int wfo_set_material_friction(wfo_domain *d, int idx, double sta, double dyn) {
   return 0;
 }
 
+int wfo_set_material_damage(wfo_domain *d, int idx, int model, const double D[5]) {
+  if (idx < 0 || idx >= 256) return fail("material index out of range");
+  if (model != 0 && model != 1) return fail("damage model: 0 none, 1 Johnson-Cook");
+  d->dam_model[idx] = model;
+  for (int k = 0; k < 5; k++) d->dam_par[idx][k] = model ? D[k] : 0.0;
+  return 0;
+}
+
 int wfo_set_param(wfo_domain *d, const char *s, double v) {
   if (!strcmp(s, "Cellfac")) d->Cellfac = v;
   else if (!strcmp(s, "ts_nb_inc")) d->ts_nb_inc = (int)v;
@@ -225,6 +239,7 @@ int wfo_set_param(wfo_domain *d, const char *s, double v) {
   else if (!strcmp(s, "friction_sta")) d->friction_sta = v;
   else if (!strcmp(s, "friction_dyn")) d->friction_dyn = v;
   else if (!strcmp(s, "contact_alg")) d->contact_alg = (int)v;
+  else if (!strcmp(s, "model_damage")) d->model_damage = (int)v;
   else if (!strcmp(s, "PFAC")) d->PFAC = v;
   else if (!strcmp(s, "fric_bound")) d->fric_bound = (int)v;
   else if (!strcmp(s, "friction_linear")) d->friction_linear = (int)v;
@@ -876,6 +891,24 @@ static inline void pair_geo_locking(const wfo_domain *d, int64_t p, pairgeo_t *g
   g->GK = wfo_grad_kernel(g->rij / g->h, g->h);
 }
 
+/* The damage weight of a pair in the Locking passes
+ * (Source/InteractionAlt.cpp:54-60,558-563): 1 - D_pair when D_pair > 0,
+ * clamped to 1, the pair skipped when it is not positive.  With the
+ * Johnson-Cook criterion Domain::CalcDamage sets D_pair = (D_1 + D_2) / 2
+ * after every step's particle update while it is below 1 (and keeps 1
+ * after, which the mean of two capped, non-decreasing values reaches only
+ * when both are 1), so it is formed here from the particles' dam_D
+ * (Source/Damage.cpp:141-150); no pair is added after the first step's
+ * search when damage is on. */
+static inline double pair_damage_factor(const wfo_domain *d, int p1, int p2) {
+  if (!d->model_damage) return 1.0;
+  const double Dp = (d->f[WFO_DAMD][p1] + d->f[WFO_DAMD][p2]) / 2.0;
+  double dam_f = 1.0;
+  if (Dp > 0.0) dam_f = 1.0 - Dp;
+  if (dam_f > 1.0) dam_f = 1.0;
+  return dam_f;
+}
+
 /* CalcAccel, Locking branch (nonlock_sum false), GradientType 0
  * (Source/InteractionAlt.cpp:12-235): each pair in pair order adds
  * mj (g1 . M) to particle 1 and subtracts mi (g2 . M) from particle 2, with
@@ -920,11 +953,13 @@ static void calc_accel_locking(wfo_domain *d) {
     } else {
       for (int k = 0; k < 3; k++) g1[k] = g2[k] = g.GK * g.xij[k];
     }
+    const double dam_f = pair_damage_factor(d, p1, p2);
+    if (!(dam_f > 0.0)) continue;
     for (int c = 0; c < 3; c++) {
       double t1 = g1[0] * M[c] + g1[1] * M[3 + c] + g1[2] * M[6 + c];
       double t2 = g2[0] * M[c] + g2[1] * M[3 + c] + g2[2] * M[6 + c];
-      acc[3 * p1 + c] += m[p2] * t1;
-      acc[3 * p2 + c] -= m[p1] * t2;
+      acc[3 * p1 + c] += dam_f * m[p2] * t1;
+      acc[3 * p2 + c] -= dam_f * m[p1] * t2;
     }
   }
 }
@@ -1008,8 +1043,10 @@ static void calc_dens_locking(wfo_domain *d) {
     }
     double t1 = dot3(g.vij, g1), t2 = dot3(g.vij, g2);
     double di = rho[g.p1], dj = rho[g.p2];
-    drho[g.p1] += m[g.p2] * (di / dj) * t1;
-    drho[g.p2] += m[g.p1] * (dj / di) * t2;
+    const double dam_f = pair_damage_factor(d, g.p1, g.p2);
+    if (!(dam_f > 0.0)) continue;
+    drho[g.p1] += dam_f * m[g.p2] * (di / dj) * t1;
+    drho[g.p2] += dam_f * m[g.p1] * (dj / di) * t2;
   }
 }
 
@@ -1264,6 +1301,49 @@ int wfo_stress_all(wfo_domain *d, double dt) {
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int i = 0; i < d->solid_count; i++) stress_strain(d, i, dt);
   return 0;
+}
+
+/* JohnsonCookDamage::CalcFractureStrain (Source/Material.cpp:76-92): the
+ * strain at fracture at plastic strain rate `sr`, triaxiality `sig_as`
+ * and temperature T, with the material's T_t, T_m and eps_0 */
+static double jc_fracture_strain(const wfo_material *M, const double *D, double sr, double sig_as, double T) {
+  double T_h = (T - M->Tt) / (M->Tm - M->Tt);
+  double f_sr = 1.0;
+  double f_triax = D[0];
+  if (sr > M->eps_0) f_sr = 1.0 + log(sr / M->eps_0) * D[3];
+  if (sig_as < 1.5) f_triax = D[0] + D[1] * exp(D[2] * sig_as);
+  double ef = f_triax * f_sr * (1.0 + D[4] * T_h);
+  if (ef > 10.0) ef = 10.0;
+  return ef;
+}
+
+/* Domain::CalcDamage, Johnson-Cook criterion (Source/Damage.cpp:9-46):
+ * after the stress update, a solid particle whose (never reset)
+ * delta_pl_strain is positive and whose von Mises stress
+ * (Particle::CalculateEquivalentStress, Source/Particle.cpp:623-633) is
+ * above 1e-3 adds delta_pl_strain / eps_f(delta_pl_strain / dt,
+ * -P / Sigma_eq, T) to dam_D, capped at 1.  T is the particle's with the
+ * thermal solver; the reference's T is otherwise an uninitialised private
+ * copy of a local set to 0, taken as 0 here.  The pair values follow from
+ * the particles' (pair_damage_factor). */
+static void calc_damage(wfo_domain *d) {
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int i = 0; i < d->solid_count; i++) {
+    const int mi = d->fi[WFO_MATIDX][i];
+    if (d->dam_model[mi] != 1) continue;
+    const double dpl = d->f[WFO_DPLSTRAIN][i];
+    if (!(dpl > 0.0)) continue;
+    const double *S = d->f[WFO_SHEAR] + 9 * (int64_t)i;
+    const double J2 = 0.5 * (S[0] * S[0] + 2.0 * S[1] * S[3] + 2.0 * S[2] * S[6] + S[4] * S[4] + 2.0 * S[5] * S[7] +
+                             S[8] * S[8]);
+    const double seq = sqrt(3.0 * J2);
+    if (!(seq > 1.0e-3)) continue;
+    const double T = d->thermal_solver ? d->f[WFO_T][i] : 0.0;
+    const double ef = jc_fracture_strain(&d->mats[mi], d->dam_par[mi], dpl / d->deltat, -d->f[WFO_P][i] / seq, T);
+    double D = d->f[WFO_DAMD][i] + dpl / ef;
+    if (D > 1.0) D = 1.0;
+    d->f[WFO_DAMD][i] = D;
+  }
 }
 
 /* Domain::CalcPlasticWorkHeat (Source/ThermalStuct.cpp:14-39) with
@@ -2146,6 +2226,11 @@ int wfo_solve_init(wfo_domain *d, double dt) {
     if (calc_pair_pos_list(d)) return -1;
     if (calculate_surface(d, 1)) return -1;
   }
+  /* CalculateSurface resets ID to ID_orig, which SolveDiffUpdateLeapFrog
+   * sets only with contact; with damage (and no contact) it is taken from
+   * ID here as well */
+  if (d->model_damage && !d->contact)
+    for (int i = 0; i < d->n; i++) d->fi[WFO_IDORIG][i] = d->fi[WFO_ID][i];
   clear_nb_data(d);
   d->isfirst = 1; d->isyielding = 0; d->ts_i = 0;
   d->initialized = 1;
@@ -2161,7 +2246,10 @@ static double max_pl_strain(const wfo_domain *d) {
 
 /* neighbour schedule at the top of a step (SolverLeapfrog.cpp:160-201) */
 static int schedule_search(wfo_domain *d, double mx) {
-  if (mx > MIN_PS_FOR_NBSEARCH && !d->isyielding) {
+  /* with damage: no search on first yielding, and ts_i = 1 after the first
+   * step, so the first step's search is the last (SolverLeapfrog.cpp:160-171) */
+  if (d->model_damage && !d->isfirst) d->ts_i = 1;
+  if (!d->model_damage && mx > MIN_PS_FOR_NBSEARCH && !d->isyielding) {
     clear_nb_data(d);
     main_neighbour_search(d);
     if (calc_pair_pos_list(d)) return -1;
@@ -2176,6 +2264,9 @@ static int schedule_search(wfo_domain *d, double mx) {
       if (d->contact && contact_nb_update(d)) return -1;
     }
   }
+  /* with damage the free surface is refreshed every step, after the
+   * search (SolverLeapfrog.cpp:204-206) */
+  if (d->model_damage && calculate_surface(d, 1)) return -1;
   /* SolverLeapfrog.cpp:207-212: the correction again on the first step */
   if (d->isfirst && d->grad_corr) wfo_calc_grad_corr(d);
   return 0;
@@ -2241,6 +2332,8 @@ int wfo_step_leapfrog(wfo_domain *d) {
   drift(d);
   wfo_calc_rates(d);
   wfo_stress_all(d, d->deltat);
+  /* SolverLeapfrog.cpp:315 */
+  if (d->model_damage) calc_damage(d);
   step_tail(d, mx);
   return 0;
 }
@@ -2249,6 +2342,7 @@ int wfo_step_leapfrog(wfo_domain *d) {
  * (Source/SolverKickDrift.cpp:117-330), fixed time step, no contact. */
 int wfo_step_kickdrift(wfo_domain *d) {
   if (!d->initialized) return fail("wfo_solve_init not called");
+  if (d->model_damage) return fail("damage is modelled by SolveDiffUpdateLeapFrog only");
   wfo_start_acceleration(d);
   double mx = max_pl_strain(d);
   d->max_pl_strain = mx;

@@ -53,6 +53,7 @@ enum {
   WFO_TIR,                                                 /* 9 */
   WFO_GCM,                                                 /* 9: gradCorrM */
   WFO_QFRIC, WFO_CSHEAR, WFO_FHFL,                         /* 1: q_fric_work, cshearabs, friction_hfl */
+  WFO_DAMD,                                                /* 1: dam_D */
   WFO_NFIELDS
 };
 /* int fields */
@@ -77,6 +78,8 @@ int wfo_set_param(wfo_domain *d, const char *name, double v);
 double wfo_get_param(const wfo_domain *d, const char *name);
 int wfo_set_material(wfo_domain *d, int idx, const wfo_material *m);
 int wfo_set_material_friction(wfo_domain *d, int idx, double sta, double dyn);
+/* material idx's damage model: 0 none, 1 JohnsonCookDamage with D1..D5 */
+int wfo_set_material_damage(wfo_domain *d, int idx, int model, const double D[5]);
 void wfo_set_threads(int n);
 const char *wfo_last_error(void);
 

@@ -55,6 +55,11 @@ def oracle_from_case(case, threads=1):
     for k in range(3):
         d.set_param(f"DomMax{k}", case.dom_max[k])
         d.set_param(f"DomMin{k}", case.dom_min[k])
+    if case.damage:
+        d.set_param("model_damage", 1)
+        d.set_param("nonlock_sum", 0)  # the scene driver's choice with damage (Source/WeldForm.cpp:303,314)
+        for idx, D in case.damage.items():
+            d.set_material_damage(idx, 1, D)
     d.set_param("thermal_solver", int(case.thermal))
     d.set_param("pl_heating", int(case.pl_heating))
     d.set_param("auto_ts", int(case.auto_ts))

@@ -222,3 +222,57 @@ def test_json_contact_heat_scene_runs(tmp_path):
     fric = float(r.stdout.split("Contact friction work")[1].split()[0])
     print(f"json driver: contact heat {flux:.6g} J, friction heat {fric:.6g} J", flush=True)
     assert flux > 0 and np.isfinite(flux) and fric >= 0 and np.isfinite(fric)
+
+
+def test_json_damage_scene_matches_oracle(tmp_path):
+    """The driver on config 1's scene with a Johnson-Cook material and
+    "damageModel": "JohnsonCook" (the reference's Compression_damage.json
+    form; T_t = 0 since the mechanical scene leaves T at 0), zones three rows
+    deep, top driven at -20 m/s, 200 Leapfrog steps: the driver switches to
+    the Locking sums (Source/WeldForm.cpp:303,314) and runs the damage on
+    the device; displacement and plastic strain against the oracle's run of
+    the same case."""
+    steps = 200
+    sc = json.load(open(os.path.join(os.path.dirname(__file__), "json", "upsetting_bilinear.json")))
+    jc = [175e6, 380e6, 0.32, 0.0015, 1.0, 1.0, 775.0, 0.0]
+    D = [0.2, 0.3, -2.0, 0.01, 0.4]
+    sc["Materials"][0].update({"type": "JohnsonCook", "const": jc, "damageModel": "JohnsonCook",
+                               "damageParams": D})
+    c = cases.config1(steps=steps)
+    dx = c.meta["dx"]
+    z = c.x[:, 2]
+    sc["DomainZones"] = [{"id": 2, "start": [-1.0, -1.0, -1.0], "end": [1.0, 1.0, z.min() + 2.5 * dx]},
+                         {"id": 3, "start": [-1.0, -1.0, z.max() - 2.5 * dx], "end": [1.0, 1.0, 1.0]}]
+    sc["BoundaryConditions"][1]["value"] = [0.0, 0.0, -20.0]
+    sc["Configuration"]["simTime"] = (steps - 0.5) * c.dt
+    sc["Configuration"]["outTime"] = 1.0
+    scene = str(tmp_path / "upsetting_damage.json")
+    with open(scene, "w") as f:
+        json.dump(sc, f)
+    if not os.path.exists(wfbuild.JSON_DRIVER):
+        wfbuild.build_json_driver()
+    dump = str(tmp_path / "final.txt")
+    print("json driver: JC damage scene", flush=True)
+    r = subprocess.run([wfbuild.JSON_DRIVER, scene, "--out", str(tmp_path / "run"), "--dump", dump],
+                       capture_output=True, text=True, timeout=100, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Assigned Johnson Cook Damage Model" in r.stdout
+    final = np.loadtxt(dump)
+    mat = dict(cases.JC_AL, jc_Tt=0.0, E=c.materials[0]["E"], nu=c.materials[0]["nu"], G=c.materials[0]["G"],
+               Cs=c.materials[0]["Cs"], ref_density=c.materials[0]["ref_density"])
+    c.materials = {0: mat}
+    c.sigmay = np.full(c.n, jc[0])
+    c.ids = np.where(z < z.min() + 2.5 * dx, 2, np.where(z > z.max() - 2.5 * dx, 3, 0)).astype(np.int32)
+    c.bcs = [dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, 0.0, -20.0))]
+    c.damage = {0: tuple(D)}
+    d = oracle_from_case(c, threads=8)
+    d.solve_init(c.dt)
+    for _ in range(steps):
+        d.step()
+    assert d.f("DAMD").max() > 0.01
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    e_p = normwise_err(final[:, 3], d.f("PLSTRAIN"))
+    print(f"json damage driver: displacement {e_u:.3e} plastic strain {e_p:.3e}", flush=True)
+    # the oracle's own sensitivity to a 1e-15 perturbation of the initial
+    # positions: 7e-11 displacement, 3e-9 plastic strain
+    assert final.shape == (c.n, 4) and e_u < 1e-8 and e_p < 1e-7

@@ -864,3 +864,98 @@ def test_grad_corr_reproduces_linear_fields_everywhere():
     c, d = _jittered(grad_corr=False, nonlock=False, A=A)
     d.calc_rates()
     assert np.abs(d.f("SR") - 0.5 * (A + A.T)).max() > 0.1 * scale
+
+
+# ---------------------------------------------------------------- damage
+def test_damage_pair_weights_scale_the_locking_passes():
+    """Damage enters the Locking acceleration and density passes as the pair
+    weight 1 - (D_i + D_j) / 2 (Source/InteractionAlt.cpp:54-60,558-572).
+    D = 0 leaves them unchanged, D = 0.5 everywhere halves them exactly (a
+    power-of-two weight commutes with rounding), D = 1 removes every pair;
+    the rates carry no weight."""
+    A = np.array([[1.0, 2.0, -0.5], [0.3, -1.5, 0.7], [-0.2, 0.4, 0.8]]) * 100.0
+    out = {}
+    for dam in (None, 0.0, 0.5, 1.0):
+        c, d = _jittered(nonlock=False, A=A)
+        if dam is not None:
+            d.set_param("model_damage", 1)
+            d.f("DAMD")[:] = dam
+        d.calc_accel()
+        d.calc_dens()
+        d.calc_rates()
+        out[dam] = {k: d.f(k).copy() for k in ("A", "DRHO", "SR", "RR")}
+    base = out[None]
+    assert np.abs(base["A"]).max() > 0 and np.abs(base["DRHO"]).max() > 0
+    for k in base:
+        assert np.array_equal(out[0.0][k], base[k])
+    for k in ("A", "DRHO"):
+        assert np.array_equal(out[0.5][k], 0.5 * base[k])
+        assert np.all(out[1.0][k] == 0)
+    for k in ("SR", "RR"):
+        assert np.array_equal(out[0.5][k], base[k]) and np.array_equal(out[1.0][k], base[k])
+
+
+def _damage_run(D, steps, thermal=False):
+    c = cases.damage_small(D, steps)
+    d = oracle_from_case(c, threads=4)
+    if thermal:
+        d.set_param("thermal_solver", 1)
+    d.solve_init(c.dt)
+    hist = []
+    for _ in range(steps):
+        d.step()
+        S = d.f("SHEAR")[:c.n]
+        J2 = 0.5 * (S[:, 0, 0] ** 2 + 2 * S[:, 0, 1] ** 2 + 2 * S[:, 0, 2] ** 2 + S[:, 1, 1] ** 2
+                    + 2 * S[:, 1, 2] ** 2 + S[:, 2, 2] ** 2)
+        hist.append((d.f("DPLSTRAIN")[:c.n].copy(), np.sqrt(3 * J2), d.f("P")[:c.n].copy(),
+                     d.f("PLSTRAIN")[:c.n].copy(), d.f("DAMD")[:c.n].copy()))
+    return c, d, hist
+
+
+def test_damage_accumulates_delta_pl_over_the_fracture_strain():
+    """Johnson-Cook damage (Source/Damage.cpp:9-46, Source/Material.cpp:
+    76-92) with D2 = D4 = 0: eps_f = D1 (1 + D5 T_h), T = 0 without the
+    thermal solver, so T_h = -T_t / (T_m - T_t); every step each particle
+    with a positive delta_pl_strain (the last increment, never reset) and
+    Sigma_eq > 1e-3 adds delta_pl_strain / eps_f, capped at 1.  The stale
+    increments count: a particle keeps gaining damage on steps where its
+    plastic strain does not grow.  With damage the neighbour list of the
+    first step is never rebuilt, although particles yield."""
+    D = (0.05, 0.0, -3.0, 0.0, 0.4)
+    c, d, hist = _damage_run(D, 150)
+    M = c.materials[0]
+    Th = (0.0 - M["jc_Tt"]) / (M["jc_Tm"] - M["jc_Tt"])
+    ef = D[0] * (1.0 + D[4] * Th)
+    want = np.zeros(c.n)
+    stale = 0
+    prev_pl = np.zeros(c.n)
+    for dpl, seq, _, pl, dam in hist:
+        on = (dpl > 0) & (seq > 1e-3)
+        want = np.where(on, np.minimum(want + dpl / ef, 1.0), want)
+        stale += int(np.sum(on & (pl == prev_pl)))
+        prev_pl = pl
+        assert np.array_equal(dam, want)
+    assert want.max() > 0.01 and stale > 0
+    assert np.abs(d.f("PLSTRAIN")).max() > 1e-6 and d.param("nb_searches") == 1
+
+
+def test_damage_triaxiality_and_rate_terms():
+    """The full criterion eps_f = (D1 + D2 exp(D3 s)) (1 + D4 ln(rate/eps0))
+    (1 + D5 T_h) with s = -P / Sigma_eq (the triaxiality, the exponential
+    only below 1.5) and rate = delta_pl_strain / dt, evaluated from each
+    step's fields, and capped at 10."""
+    D = (0.05, 0.3, -2.0, 0.02, 0.4)
+    c, d, hist = _damage_run(D, 120)
+    M = c.materials[0]
+    Th = (0.0 - M["jc_Tt"]) / (M["jc_Tm"] - M["jc_Tt"])
+    want = np.zeros(c.n)
+    for dpl, seq, P, _, dam in hist:
+        on = (dpl > 0) & (seq > 1e-3)
+        s = np.where(on, -P / np.where(on, seq, 1.0), 0.0)
+        rate = dpl / c.dt
+        f_sr = np.where(rate > M["jc_eps0"], 1.0 + np.log(np.where(rate > 0, rate, 1.0) / M["jc_eps0"]) * D[3], 1.0)
+        f_tr = np.where(s < 1.5, D[0] + D[1] * np.exp(D[2] * s), D[0])
+        ef = np.minimum(f_tr * f_sr * (1.0 + D[4] * Th), 10.0)
+        want = np.where(on, np.minimum(want + dpl / np.where(on, ef, 1.0), 1.0), want)
+        assert np.allclose(dam, want, rtol=1e-13, atol=0)
+    assert want.max() > 0.005

@@ -51,6 +51,7 @@ class Case:
     group_sums: tuple = None    # (zone, cap) of wf_set_group_sums, or None
     contact_law: tuple = None   # (law, PFAC, fric_bound) of wf_set_contact_law, or None (Wang)
     friction_linear: tuple = None  # (friction_m, friction_b) of wf_set_friction_function, or None
+    damage: dict = field(default_factory=dict)  # material idx -> Johnson-Cook damage (D1..D5)
     meta: dict = field(default_factory=dict)
 
     @property
@@ -144,6 +145,21 @@ def small_jc(steps=200, dx=1.0e-3, R=7.5e-3, L=30e-3, seed=7):
     """Config-2 physics (JC, jitter) at config-1 size, for parity tests."""
     return cylinder_case("cyl_upset_4k_jc", R, L, dx, AL["rho"], AL["E"], AL["nu"], JC_AL,
                          steps=steps, jitter=1e-3, seed=seed, T0=300.0)
+
+
+def damage_small(D=(0.2, 0.0, -3.0, 0.0, 0.4), steps=200, v_top=-20.0):
+    """small_jc with Johnson-Cook damage (D1..D5): with damage the free
+    surface is refreshed every step and its particles take ID 1, leaving
+    their BC zone (Source/SolverLeapfrog.cpp:204-206, Source/Domain.cpp:
+    1415-1491), so the BC zones span three rows at each end (bottom held,
+    top driven at v_top) and drive the particles inside them."""
+    c = small_jc(steps=steps)
+    z, dx = c.x[:, 2], c.meta["dx"]
+    c.ids = np.where(z < z.min() + 2.5 * dx, 2, np.where(z > z.max() - 2.5 * dx, 3, 0)).astype(np.int32)
+    c.bcs = [dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, 0.0, v_top))]
+    c.damage = {0: tuple(D)}
+    c.name = "cyl_upset_4k_jc_damage"
+    return c
 
 
 STEEL_JC = dict(model=capi.JOHNSON_COOK, alpha=1.0, beta=0.0, jc_A=792e6, jc_B=510e6, jc_n=0.26,
@@ -366,6 +382,8 @@ def config5_zone(steps=400, dx=0.4675e-3, R=15e-3, L=30e-3, v_top=-5.0, seed=202
 
 
 def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
+    if case.damage:
+        kw.setdefault("locking", True)  # the scene driver's choice with damage
     s = capi.Solver(case.n, precision=precision, max_nb=max_nb, ts_nb_inc=ts_nb_inc,
                     thermal_solver=case.thermal, pl_heating=case.pl_heating,
                     dom_max=case.dom_max, dom_min=case.dom_min, auto_ts=case.auto_ts, cfl=case.cfl,
@@ -411,6 +429,8 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
         s.set_material_friction(idx, fs, fd)
     if case.contact_law is not None:
         s.set_contact_law(*case.contact_law)
+    for idx, D in case.damage.items():
+        s.set_material_damage(idx, 1, D)
     if case.friction_linear is not None:
         s.set_friction_function(True, *case.friction_linear)
     if case.group_sums is not None:

@@ -34,7 +34,7 @@ struct Error : std::runtime_error {
   } while (0)
 
 const int kFieldDim[WF_NFIELDS] = {3, 3, 3, 3, 1, 1, 1, 9, 9, 9, 9, 9, 9, 1, 1,
-                                   1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1, 3, 9};
+                                   1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1, 3, 9, 1};
 
 struct BcHost {
   int zone, type, value_type, amp_id;
@@ -91,6 +91,8 @@ struct Ctx {
   std::vector<int32_t> perm, inv;  // perm[internal] = original
   std::vector<double> mat_xc;      // per-material FP32 shadow origin [kMaxMat][3]
   std::vector<double> mat_fr;      // per-material friction (static, dynamic) [kMaxMat][2]; < 0: unset
+  std::vector<int> mat_dam;        // per-material damage model (0 none, 1 Johnson-Cook) [kMaxMat]
+  std::vector<double> mat_dpar;    // its D1..D5 [kMaxMat][5]
   int gsum_zone = -2;              // wf_set_group_sums: -2 off, -1 contact forces only, else the zone
   long long gsum_cap = 0;
   Dev d{};
@@ -535,6 +537,14 @@ void allocate(Ctx *c) {
       d.qcont = c->alloc<double>(np);
     }
   }
+  // Domain::model_damage: some particle's material has a damage model
+  d.damage = 0;
+  for (int64_t i = 0; i < c->n; ++i)
+    if (c->mat_dam[c->hi[WF_I_MATERIAL][i]]) d.damage = 1;
+  if (d.damage) {
+    d.damD = c->alloc<double>(np);
+    if (!d.id_orig) d.id_orig = c->alloc<int>(np);
+  }
   d.ntiles = (int)(np / kBlock);
   d.stage_max = 0;
   d.lanes_mx = c->cfg.lanes ? c->cfg.lanes : kLanesMixed;
@@ -645,6 +655,19 @@ void upload_all(Ctx *c) {
   dput(c, d.vol[1], vol);
   dput(c, d.pres, pres);
   dput(c, d.srec, rec);
+  if (d.damD && field(WF_F_DAMAGE)) put_scalar(c, d.damD, field(WF_F_DAMAGE));
+  if (d.damage && !d.contact) {
+    // CalculateSurface resets ID to ID_orig, which SolveDiffUpdateLeapFrog
+    // sets only with contact (Source/SolverLeapfrog.cpp:80-83); with damage
+    // it is taken from ID here too.  The mean mass of the surface test runs
+    // over the solid particles.
+    std::vector<int> io(np, 0);
+    for (int64_t k = 0; k < n; ++k) io[k] = c->hi[WF_I_ID][c->perm[k]];
+    dput(c, d.id_orig, io);
+    double tm = 0.0;
+    for (int64_t o = 0; o < n; ++o) tm += F[WF_F_MASS][o];
+    d.totmass = tm / (double)n;
+  }
   if (d.cstiff) {
     // Particle::cont_stiff: SolveDiffUpdateLeapFrog never sets it; the
     // formula SolveDiffUpdateKickDrift applies before its loop
@@ -752,6 +775,8 @@ void upload_all(Ctx *c) {
     for (int q = 0; q < 3; ++q) dm[m].xc[q] = c->mat_xc[3 * m + q];
     dm[m].fr_sta = c->mat_fr[2 * m];
     dm[m].fr_dyn = c->mat_fr[2 * m + 1];
+    dm[m].dam_model = c->mat_dam[m];
+    for (int q = 0; q < 5; ++q) dm[m].dam[q] = c->mat_dpar[5 * m + q];
   }
   dput(c, c->dmats, dm);
   // device schedule state: the pre-loop ClearNbData leaves the data cleared
@@ -919,6 +944,9 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int
       launch_neighbours(d, c->st, cur);
       if (d.gc_on) launch_gradcorr(d, c->st, cur);
       if (d.contact) launch_surface(d, c->st, cur, 0);
+      // with damage the free surface is refreshed every step
+      // (Source/SolverLeapfrog.cpp:204-206)
+      if (d.damage) launch_surface(d, c->st, cur, 2);
     }
     if (tp) launch_tape_lists(d, c->st, tp->s, tp->force_lists, tp->ver, tp->slot);
   }
@@ -1044,6 +1072,7 @@ void adj_check(Ctx *c) {
       throw Error(WF_EUNSUPPORTED, "the adjoint covers the context's friction, not per-material friction");
   if (c->d.contact && (c->contact_law != 1 || c->fric_linear))
     throw Error(WF_EUNSUPPORTED, "the adjoint covers Wang's contact law with constant friction");
+  if (c->d.damage) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover damage");
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1179,6 +1208,8 @@ int wf_create(const wf_config *cfg, int64_t n, void **out) {
     c->mats.resize(kMaxMat);
     c->mat_set.assign(kMaxMat, 0);
     c->mat_fr.assign(2 * kMaxMat, -1.0);
+    c->mat_dam.assign(kMaxMat, 0);
+    c->mat_dpar.assign(5 * kMaxMat, 0.0);
     for (int f = 0; f < WF_NIFIELDS; ++f) c->hi[f].assign(n, 0);
     c->hi[WF_I_FREE].assign(n, 1);
     try {
@@ -1227,6 +1258,8 @@ int wf_set_material(void *ctx, int32_t index, const wf_material *m) {
       for (int q = 0; q < 3; ++q) one[0].xc[q] = c->mat_xc[3 * index + q];
       one[0].fr_sta = c->mat_fr[2 * index];
       one[0].fr_dyn = c->mat_fr[2 * index + 1];
+      one[0].dam_model = c->mat_dam[index];
+      for (int q = 0; q < 5; ++q) one[0].dam[q] = c->mat_dpar[5 * index + q];
       HIPCHK(hipMemcpyAsync(c->dmats + index, one.data(), sizeof(DevMat), hipMemcpyHostToDevice, c->st));
       HIPCHK(hipStreamSynchronize(c->st));
       // the acceleration kernel is chosen by whether a material has beta != 0
@@ -1327,6 +1360,10 @@ int wf_set_field(void *ctx, int32_t field, const double *h) {
       case WF_F_T_INF: put_scalar(c, d.tinf, h); break;
       case WF_F_Q_SOURCE: put_scalar(c, d.qsrc, h); break;
       case WF_F_PS_ENERGY: put_scalar(c, d.psen, h); break;
+      case WF_F_DAMAGE:
+        if (!d.damD) throw Error(WF_EINVAL, "no damage model set (wf_set_material_damage)");
+        put_scalar(c, d.damD, h);
+        break;
       default: throw Error(WF_EINVAL, "field not settable");
     }
     launch_shadows(d, c->st, c->cur);
@@ -1393,6 +1430,12 @@ int wf_get_field(void *ctx, int32_t field, double *h) {
       case WF_F_T_INF: get_scalar(c, d.tinf, h); break;
       case WF_F_Q_SOURCE: get_scalar(c, d.qsrc, h); break;
       case WF_F_PS_ENERGY: get_scalar(c, d.psen, h); break;
+      case WF_F_DAMAGE:
+        if (d.damD)
+          get_scalar(c, d.damD, h);
+        else
+          std::memset(h, 0, sizeof(double) * (size_t)c->n);
+        break;
       case WF_F_CONTFORCE:
         if (d.contact)
           get_d4(c, d.cforce, h, 0, 3, 3);
@@ -1520,9 +1563,10 @@ int wf_get_int_field(void *ctx, int32_t field, int32_t *h) {
       std::memset(h, 0, (size_t)c->n * sizeof(int32_t));
       return WF_OK;
     }
-    if (field == WF_I_NB || field == WF_I_CONTNB || (field == WF_I_ID && c->inited && c->d.contact)) {
+    if (field == WF_I_NB || field == WF_I_CONTNB ||
+        (field == WF_I_ID && c->inited && (c->d.contact || c->d.damage))) {
       // neighbour and contact-pair counts; zone ids once free-surface
-      // particles took ID 1
+      // particles took ID 1 (contact, damage)
       if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
       const int *src = field == WF_I_NB ? c->d.nnb : (field == WF_I_CONTNB ? c->d.ccnt : c->d.id);
       auto t = dget(c, src, (size_t)c->npad);
@@ -1636,6 +1680,19 @@ int wf_init(void *ctx, double dt) {
     }
     if ((c->heat_cond || c->heat_fric) && c->meshes.empty())
       throw Error(WF_EINVAL, "contact heat (wf_set_contact_heat) needs a rigid mesh (wf_add_rigid_plane / _mesh)");
+    {
+      bool dam = false;
+      for (int64_t i = 0; i < c->n; ++i) {
+        const int m = c->hi[WF_I_MATERIAL][i];
+        if (!c->mat_dam[m]) continue;
+        dam = true;
+        if (c->mats[m].model != WF_JOHNSON_COOK)
+          throw Error(WF_EINVAL, "Johnson-Cook damage needs a Johnson-Cook material (T_t, T_m, eps_0)");
+      }
+      if (dam && c->cfg.precision != WF_FP64) throw Error(WF_EUNSUPPORTED, "damage needs precision WF_FP64");
+      if (dam && !c->cfg.locking)
+        throw Error(WF_EUNSUPPORTED, "damage needs the Locking sums (wf_config.locking 1, as the scene driver sets)");
+    }
     if (c->heat_cond && c->rigid_T.empty())
       throw Error(WF_EINVAL,
                   "contact conduction needs the rigid particles' temperatures (wf_set_rigid_temperature)");
@@ -1668,6 +1725,7 @@ int wf_step(void *ctx, int32_t scheme, int64_t nsteps) {
     as_ctx(ctx);
     if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
     if (scheme != WF_LEAPFROG && scheme != WF_KICKDRIFT) throw Error(WF_EINVAL, "unknown scheme");
+    if (scheme == WF_KICKDRIFT && c->d.damage) throw Error(WF_EUNSUPPORTED, "damage is modelled by the Leapfrog step only");
     if (c->next_part != 0) throw Error(WF_ESTATE, "wf_step inside a step run by wf_step_part");
     for (int64_t s = 0; s < nsteps; ++s) one_step(c, scheme);
     return WF_OK;
@@ -2011,6 +2069,23 @@ int wf_set_friction_function(void *ctx, int32_t linear, double m, double b) {
     c->fric_linear = linear ? 1 : 0;
     c->fr_m = linear ? m : 0.0;
     c->fr_b = linear ? b : 0.0;
+    return WF_OK;
+  });
+}
+
+int wf_set_material_damage(void *ctx, int32_t index, int32_t model, const double D[5]) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_material_damage after wf_init");
+    if (index < 0 || index >= kMaxMat) throw Error(WF_EINVAL, "material index out of range");
+    if (model != 0 && model != 1) throw Error(WF_EINVAL, "damage model: 0 none, 1 Johnson-Cook");
+    if (model && !D) throw Error(WF_EINVAL, "null damage parameters");
+    if (model)
+      for (int q = 0; q < 5; ++q)
+        if (!std::isfinite(D[q])) throw Error(WF_EINVAL, "non-finite damage parameter");
+    c->mat_dam[index] = model;
+    for (int q = 0; q < 5; ++q) c->mat_dpar[5 * index + q] = model ? D[q] : 0.0;
     return WF_OK;
   });
 }

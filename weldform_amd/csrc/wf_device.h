@@ -194,6 +194,32 @@ __device__ inline double jc_yield(const DevMat &M, double strain, double sr, dou
   return (M.jc_A + M.jc_B * pow(strain, M.jc_n)) * f * (1.0 - pow(T_h, M.jc_m));
 }
 
+// JohnsonCookDamage::CalcFractureStrain (Source/Material.cpp:76-92): the
+// strain at fracture at plastic strain rate sr, triaxiality sig_as and
+// temperature T, with the material's T_t, T_m and eps_0; capped at 10
+__device__ inline double jc_fracture_strain(const DevMat &M, double sr, double sig_as, double T) {
+  const double T_h = (T - M.jc_Tt) / (M.jc_Tm - M.jc_Tt);
+  double f_sr = 1.0;
+  double f_triax = M.dam[0];
+  if (sr > M.jc_eps0) f_sr = 1.0 + log(sr / M.jc_eps0) * M.dam[3];
+  if (sig_as < 1.5) f_triax = M.dam[0] + M.dam[1] * exp(M.dam[2] * sig_as);
+  double ef = f_triax * f_sr * (1.0 + M.dam[4] * T_h);
+  if (ef > 10.0) ef = 10.0;
+  return ef;
+}
+
+// the damage weight of a pair: 1 - D_pair with D_pair = (D_i + D_j) / 2,
+// clamped to 1 (the Johnson-Cook criterion's pair value, Source/Damage.cpp:
+// 141-150; Source/InteractionAlt.cpp:54-60); not positive: the pair is
+// skipped
+__device__ inline double dam_weight(double Di, double Dj) {
+  const double Dp = (Di + Dj) / 2.0;
+  double f = 1.0;
+  if (Dp > 0.0) f = 1.0 - Dp;
+  if (f > 1.0) f = 1.0;
+  return f;
+}
+
 __device__ inline double hol_yield(const DevMat &M, double strain) {
   if (strain + M.hol_eps0 > M.hol_eps1) return M.hol_K * pow(strain + M.hol_eps0, M.hol_m);
   return M.hol_sy0;

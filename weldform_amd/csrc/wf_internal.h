@@ -80,7 +80,7 @@ struct __align__(32) StressRecF {
 
 struct DevMat {
   int model;
-  int pad;
+  int dam_model;               // damage: 0 none, 1 Johnson-Cook (dam: D1..D5)
   double E, G, Cs, ref_density, P0, alpha, beta, Ep;
   double hol_K, hol_m, hol_sy0, hol_eps0, hol_eps1;
   double jc_A, jc_B, jc_n, jc_C, jc_eps0, jc_m, jc_Tm, jc_Tt;
@@ -89,6 +89,7 @@ struct DevMat {
   // Coulomb friction of the material's particles against the rigid
   // surfaces (wf_set_material_friction); negative: the context's
   double fr_sta, fr_dyn;
+  double dam[5];
   // origin of the material's FP32 position shadows: the centre of its
   // particles' initial bounding box (pairs form within one material, so the
   // pair differences see one origin; a batch of separated specimens keeps
@@ -251,6 +252,9 @@ struct Dev {
   // function Linear (fr = fr_m T + fr_b) and, for Fraser's and Seo's laws,
   // Particle::cont_stiff per particle (Fraser 3-153, formed at set-up)
   int contact_law, fric_bound, fric_linear;
+  // Domain::model_damage and Particle::dam_D (Johnson-Cook damage)
+  int damage;
+  double *damD;
   double pfac, fr_m, fr_b;
   double *cstiff;
   // contact heat (Source/Contact_Wang.cpp:245-324): conduction with

@@ -87,7 +87,11 @@ WF_KERNEL void k_schedule(Dev d, int force) {
   if (force) {
     search = periodic = 1;
   } else {
-    if (mx && !f->isyielding) {
+    // with damage: no search when yielding starts, and ts_i = 1 after the
+    // first step, so the first step's search is the last
+    // (Source/SolverLeapfrog.cpp:160-171)
+    if (d.damage && !isfirst) f->ts_i = 1;
+    if (!d.damage && mx && !f->isyielding) {
       search = 1;
       f->isyielding = 1;
       f->cleared = 0;
@@ -834,7 +838,14 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
         Mxz += s * (tri[XZ] + trj[XZ]);
         Myz += s * (tri[YZ] + trj[YZ]);
       }
-      const R w = vj.w * GK;
+      R w = vj.w * GK;
+      if constexpr (!MX) {
+        // damage weight of the pair (Source/InteractionAlt.cpp:54-60)
+        if (d.damage) {
+          const R df = dam_weight(d.damD[i], d.damD[j]);
+          w = df > (R)0 ? (df * vj.w) * GK : (R)0;
+        }
+      }
       R ex, ey, ez;
       corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
       ax += w * (ex * Mxx2 + ey * Mxy + ez * Mxz);
@@ -1405,7 +1416,13 @@ __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, co
         s += (volj * GK) * ((vi.x - vj.x) * ex + (vi.y - vj.y) * ey + (vi.z - vj.z) * ez);
       } else {
         const R t = (vi.x - vj.x) * (GK * ex) + (vi.y - vj.y) * (GK * ey) + (vi.z - vj.z) * (GK * ez);
-        s += volj * t;
+        R wv = volj;
+        // damage weight of the pair (Source/InteractionAlt.cpp:558-572)
+        if (d.damage) {
+          const R df = dam_weight(d.damD[i], d.damD[j]);
+          wv = df > (R)0 ? df * volj : (R)0;
+        }
+        s += wv * t;
       }
     });
   }
@@ -1810,6 +1827,26 @@ WF_KERNEL void k_stress(Dev d, int buf, double dt) {
       d.sigy[i] = Sy;
       d.pl[i] = pl;
       d.pres[i] = P;
+      // Domain::CalcDamage, Johnson-Cook criterion (Source/Damage.cpp:9-46),
+      // after the stress update: the (never reset) delta_pl_strain over the
+      // fracture strain at the plastic rate, the triaxiality -P / Sigma_eq
+      // (Particle::CalculateEquivalentStress, Source/Particle.cpp:623-633)
+      // and T (0 without the thermal solver: the reference's is an
+      // uninitialised private copy of a local set to 0)
+      if (d.damage && M.dam_model == 1) {
+        const double dpl = d.dpl[i];
+        if (dpl > 0.0) {
+          const double J2e = 0.5 * (Sn[XX] * Sn[XX] + 2.0 * Sn[XY] * Sn[XY] + 2.0 * Sn[XZ] * Sn[XZ] + Sn[YY] * Sn[YY] +
+                                    2.0 * Sn[YZ] * Sn[YZ] + Sn[ZZ] * Sn[ZZ]);
+          const double seq = sqrt(3.0 * J2e);
+          if (seq > 1.0e-3) {
+            const double ef = jc_fracture_strain(M, dpl / dt, -P / seq, d.thermal ? T : 0.0);
+            double D = d.damD[i] + dpl / ef;
+            if (D > 1.0) D = 1.0;
+            d.damD[i] = D;
+          }
+        }
+      }
       if (pl > 1.e-6) d.flags->yielded = 1;
       StressRec rec;
       for (int k = 0; k < 6; ++k) rec.s[k] = sig[k];
@@ -2080,8 +2117,10 @@ void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
 // partners are the rigid particles, in index order, of an adjacent cell
 // closer than h_i + h_r, for particles on the surface before and after.
 // setup = 1: the search before the first step only sets the surface.
+// setup = 2: with damage, every step after the search (CalculateSurface(1),
+// Source/SolverLeapfrog.cpp:204-206): the surface ids only.
 WF_KERNEL void k_surface(Dev d, int cur, int setup) {
-  if (!d.flags->do_search) return;
+  if (setup != 2 && !d.flags->do_search) return;
   const long long np = d.npad;
   {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -2104,6 +2143,7 @@ WF_KERNEL void k_surface(Dev d, int cur, int setup) {
     nz *= f;
     const int surf = sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * xi.w && cnt <= 46;
     d.id[i] = surf ? 1 : d.id_orig[i];
+    if (setup == 2) return;
     int c = 0;
     if (!setup && old_surf && surf) {
       // ContactNbSearch: the rigid particles of the 27 cells around i's own

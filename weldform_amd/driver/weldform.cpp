@@ -9,9 +9,9 @@
 // device; --mixed selects the mixed-precision step; --out sets the output
 // file key (the reference writes "test06_NNNN"); --dump writes the final
 // displacement and plastic strain of every particle at full precision.  Scene features off the
-// device path (2D / axisymmetric blocks, LS-Dyna particle files, Nastran
-// meshes, damage, GMT, the Fraser and thermal-only solvers, smoothing-length
-// update) stop with a Fatal naming the key.
+// device path (2D / axisymmetric blocks, LS-Dyna particle files, Rankine
+// damage, GMT, the Fraser and thermal-only solvers, smoothing-length update)
+// stop with a Fatal naming the key.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -104,9 +104,24 @@ int main(int argc, char **argv) try {
   } else {
     throw new Fatal("Invalid material type.");
   }
+  // Source/WeldForm.cpp:293-321: the damage model switches the Locking sums on
   std::string damage_mod;
   readValue(m0["damageModel"], damage_mod);
-  if (!damage_mod.empty()) throw new Fatal("damage models are not implemented on the device path");
+  if (damage_mod == "JohnsonCook") {
+    std::vector<double> D(5, 0.0);
+    readArray(m0["damageParams"], D);
+    if (D.size() < 5) throw new Fatal("JohnsonCook damage needs damageParams [D1, D2, D3, D4, D5]");
+    if (model != JOHNSON_COOK) throw new Fatal("JohnsonCook damage needs a JohnsonCook material on the device path");
+    mat->damage = new JohnsonCookDamage(D[0], D[1], D[2], D[3], D[4], ((JohnsonCook *)mat)->getRefStrainRate());
+    mat->damage->mat = mat;
+    std::cout << "Assigned Johnson Cook Damage Model" << std::endl;
+    dom.model_damage = true;
+    dom.nonlock_sum = false;
+  } else if (damage_mod == "Rankine") {
+    throw new Fatal("the Rankine damage criterion is not implemented on the device path");
+  } else if (!damage_mod.empty()) {
+    throw new Fatal("unknown damageModel " + damage_mod);
+  }
   double k_T = 0.0, cp_T = 0.0, th_ex = 0.0;
   readValue(m0["thermalCond"], k_T);
   readValue(m0["thermalHeatCap"], cp_T);
