@@ -1179,6 +1179,7 @@ const char *wf_last_error(const void *ctx) {
 
 int wf_device_count(int32_t *count) {
   return guard(nullptr, [&] {
+    if (!count) throw Error(WF_EINVAL, "null argument");
     int n = 0;
     HIPCHK(hipGetDeviceCount(&n));
     *count = n;
