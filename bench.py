@@ -113,6 +113,53 @@ def timed_run(case, precision, steps, warmup):
     e.close()
     return s, wall, wall_eager, times, init_s
 
+def yield_run(steps, chunk=50):
+    """The headline billet (config 2 geometry and material, 976,320
+    particles) upset at -20 m/s so that it yields: once a particle's plastic
+    strain passes 1e-6 the reference rebuilds every neighbour list every
+    ts_nb_inc = 5 steps (Source/SolverLeapfrog.cpp:160-201,347-356).  Mixed
+    precision, graph replay, timed in chunks: the ms/step before and after
+    the first yield show what the rebuilds cost."""
+    import numpy as np
+
+    from weldform_amd import capi, cases
+
+    case = cases.config2(steps=steps)
+    case.bcs = [dict(b) for b in case.bcs]
+    for b in case.bcs:
+        if b["zone"] == case.top_zone:
+            b["value"] = (0.0, 0.0, -20.0)
+    log(f"bench: yielding {case.name} n={case.n} v_top=-20 m/s steps={steps}")
+    s = cases.make_solver(case, precision=capi.MIXED)
+    s.prepare()
+    s.sync()
+    walls, searches, yielded_at = [], [], None
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        t0 = time.perf_counter()
+        s.step(k)
+        s.sync()
+        walls.append((k, time.perf_counter() - t0))
+        sc = s.scalars()
+        searches.append(sc["nb_searches"])
+        if yielded_at is None and sc["max_pl_strain"] > 1e-6:
+            yielded_at = done + k
+        done += k
+    per = [searches[0]] + [b - a for a, b in zip(searches[:-1], searches[1:])]  # searches in each chunk
+    pre = [w / k for (k, w), m in list(zip(walls, per))[1:] if m == 0]
+    post = [w / k for (k, w), m in zip(walls, per) if m >= k // 5]
+    s.close()
+    res = {"workload": f"{case.name} at v_top=-20 m/s: 976,320 particles, mixed, {steps} Leapfrog steps, "
+                       f"timed in chunks of {chunk}",
+           "steps": steps, "nb_searches": int(searches[-1]), "yielded_by_step": yielded_at,
+           "ms_per_step_chunks": [1e3 * w / k for k, w in walls], "searches_per_chunk": [int(m) for m in per],
+           "ms_per_step_before_yield": 1e3 * float(np.median(pre)) if pre else None,
+           "ms_per_step_yielding": 1e3 * float(np.median(post)) if post else None}
+    log("bench: yielding " + json.dumps(res))
+    return res
+
+
 def contact_run(steps, warmup=10):
     """Thermo-mechanical contact compression (weldform_amd.cases.config3,
     384,256 particles, two rigid planes of 2,048 triangles, conduction and
@@ -440,6 +487,8 @@ def main():
                     help="steps of the batched-ensemble leg (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=400,
                     help="window of the adjoint calibration leg (0: skip)")
+    ap.add_argument("--yield-steps", type=int, default=500,
+                    help="steps of the yielding headline-billet leg (-20 m/s; 0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of 1 GiB (for scaling FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -552,6 +601,8 @@ def main():
     }
     if not args.no_output:
         out["config"]["output_976k"] = output_run(mx["solver"], case.n)
+    if args.yield_steps > 0:
+        out["config"]["yielding_976k"] = yield_run(args.yield_steps)
     if args.config3_steps > 0:
         out["config"]["config3_contact_thermal"] = contact_run(args.config3_steps)
     if args.batch_steps > 0:
