@@ -109,7 +109,11 @@ extern "C" {
 #define WF_F_GRADCORR 29    /* [9] read-only: Particle::gradCorrM (identity
                                unless wf_config.grad_corr)                  */
 #define WF_F_DAMAGE 30      /* [1] Particle::dam_D (damage, 0..1)             */
-#define WF_NFIELDS 31
+#define WF_F_ZWAB 31        /* [1] read-only: Particle::ZWab of the last
+                               wf_calc_force2233 (0 before it)              */
+#define WF_F_VXSPH 32       /* [3] read-only: Particle::VXSPH, likewise        */
+#define WF_F_SUMDEN 33      /* [1] read-only: Particle::SumDen, likewise       */
+#define WF_NFIELDS 34
 
 /* rigid-surface particle fields (wf_get_rigid_field), 3 doubles each, in
  * the order the planes were added, element order within a plane */
@@ -400,6 +404,16 @@ int wf_get_neighbour_pairs(void *ctx, int32_t *pairs2, int64_t cap, int64_t *cou
 
 /* single passes for parity checks (no time advance) */
 int wf_eval(void *ctx, int32_t what, double dt);
+/* wf_calc_force2233 <- Domain::LastComputeAcceleration's pair loop, i.e.
+ * Domain::CalcForce2233 over every pair (Source/Domain.cpp:1700-1709,
+ * Source/Interaction.cpp:25-333): one fused FP64 pass over each particle's
+ * row at the current state (after wf_rebuild_neighbours), no time advance.
+ * It sets a, dDensity, StrainRate and RotationRate (the Locking form, float
+ * m_j/rho_j weights in the rates), and WF_F_ZWAB, WF_F_VXSPH (with the XSPH
+ * coefficient `xsph`) and, when `shepard` != 0, WF_F_SUMDEN (the Shepard
+ * sum due for every particle).  GradientType 0; refused with the
+ * tensile-instability term.  Leapfrog and KickDrift do not call it. */
+int wf_calc_force2233(void *ctx, double xsph, int32_t shepard);
 
 /* kernel timing with HIP events on the context's stream.
  * kernel: 0 accel, 1 density+drift, 2 rates+stress, 3 neighbour build,

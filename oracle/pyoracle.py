@@ -21,9 +21,9 @@ F64 = ["X", "V", "A", "XPREV", "DISP", "RHO", "DRHO", "MASS", "H", "REFRHO", "P"
        "EPL", "EPLINC", "PLSTRAIN", "DPLSTRAIN", "SIGMAY", "EFFSR", "EP", "ET",
        "T", "DTDT", "KT", "CPT", "THEXP", "QCONV", "QSRC", "QPL", "QCONT",
        "HCONV", "TINF", "PSENERGY", "NORMAL", "CONTFORCE", "TI", "TIN", "TIDIST", "TIR", "GCM",
-       "QFRIC", "CSHEAR", "FHFL", "DAMD"]
+       "QFRIC", "CSHEAR", "FHFL", "DAMD", "ZWAB", "VXSPH", "SUMDEN"]
 F64_DIM = dict(X=3, V=3, A=3, XPREV=3, DISP=3, SIGMA=9, SHEAR=9, SR=9, RR=9,
-               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9, GCM=9)
+               STRAIN=9, EPL=9, EPLINC=9, NORMAL=3, CONTFORCE=3, TIR=9, GCM=9, VXSPH=3)
 I32 = ["ID", "MATERIAL", "MATIDX", "MODEL", "FREE", "THBC", "NB", "ELEMENT", "MESHIDX", "IDORIG"]
 
 
@@ -59,6 +59,7 @@ def lib():
         L.wfo_set_material.argtypes = [vp, i32, ctypes.POINTER(Material)]
         L.wfo_set_material_friction.argtypes = [vp, i32, d, d]
         L.wfo_set_material_damage.argtypes = [vp, i32, i32, dp]
+        L.wfo_calc_force2233.argtypes = [vp, d, i32]
         L.wfo_set_threads.argtypes = [i32]
         L.wfo_last_error.restype = ctypes.c_char_p
         L.wfo_add_velocity_bc.argtypes = [vp, i32, i32, dp, i32, d]
@@ -229,6 +230,9 @@ class Domain:
 
     def calc_rates(self):
         self._check(lib().wfo_calc_rates(self.h))
+
+    def calc_force2233(self, xsph=0.0, shepard=False):
+        self._check(lib().wfo_calc_force2233(self.h, float(xsph), int(bool(shepard))))
 
     def stress_all(self, dt):
         self._check(lib().wfo_stress_all(self.h, float(dt)))

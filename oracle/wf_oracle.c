@@ -42,7 +42,8 @@ static const int k_fdim[WFO_NFIELDS] = {
     9,                    /* TIR */
     9,                    /* GCM */
     1, 1, 1,              /* QFRIC CSHEAR FHFL */
-    1                     /* DAMD */
+    1,                    /* DAMD */
+    1, 3, 1               /* ZWAB VXSPH SUMDEN */
 };
 
 typedef struct { int zone, type, value_type, amp_id; double value[3], amp_factor, T, value_ang[3]; } bc_t;
@@ -972,6 +973,52 @@ int wfo_calc_accel(wfo_domain *d) {
   }
   calc_accel_pairs(d);
   accel_reduction(d);
+  return 0;
+}
+
+/* Domain::LastComputeAcceleration's pair loop (Source/Domain.cpp:1700-1709):
+ * Domain::CalcForce2233 (Source/Interaction.cpp:25-333) for every pair in
+ * pair order, P1 = first, P2 = second, free-free pairs, GradientType 0.
+ * a, dDensity, StrainRate and RotationRate are the Locking passes' sums
+ * (the same terms, added pair by pair in the same order: m_j g1.M into a,
+ * m_j (d_i/d_j) v_ij.g1 into dDensity, (float)(m_j/d_j) times the rate
+ * tensors), so they are formed by those functions; the fused loop's own
+ * terms follow: ZWab += (float)(m_j/d_j) K, VXSPH += XSPH m_j /
+ * ((d_i + d_j)/2) K (-v_ij) (v_ij for particle 2), and SumDen += m_j K when
+ * the Shepard sum is due.  Everything starts from StartAcceleration's
+ * zeros. */
+static void calc_dens_locking(wfo_domain *d);
+static void calc_rates_locking(wfo_domain *d);
+int wfo_calc_force2233(wfo_domain *d, double xsph, int shepard) {
+  for (int i = 0; i < 3 * d->n; i++) d->f[WFO_A][i] = 0.0;
+  calc_accel_locking(d);
+  calc_dens_locking(d);
+  calc_rates_locking(d);
+  const double *m = d->f[WFO_MASS], *rho = d->f[WFO_RHO];
+  double *zw = d->f[WFO_ZWAB], *vx = d->f[WFO_VXSPH], *sd = d->f[WFO_SUMDEN];
+  for (int i = 0; i < d->n; i++) {
+    zw[i] = sd[i] = 0.0;
+    for (int k = 0; k < 3; k++) vx[3 * i + k] = 0.0;
+  }
+  for (int64_t p = 0; p < d->npairs; p++) {
+    pairgeo_t g;
+    pair_geo_locking(d, p, &g);
+    const int p1 = g.p1, p2 = g.p2;
+    const double K = wfo_kernel(g.rij / g.h, g.h);
+    const double di = rho[p1], dj = rho[p2], mi = m[p1], mj = m[p2];
+    const float mj_dj = (float)(mj / dj), mi_di = (float)(mi / di);
+    zw[p1] += mj_dj * K;
+    zw[p2] += mi_di * K;
+    if (xsph != 0.0)
+      for (int k = 0; k < 3; k++) {
+        vx[3 * p1 + k] += xsph * mj / (0.5 * (di + dj)) * K * (-g.vij[k]);
+        vx[3 * p2 + k] += xsph * mi / (0.5 * (di + dj)) * K * g.vij[k];
+      }
+    if (shepard) {
+      sd[p1] += mj * K;
+      sd[p2] += mi * K;
+    }
+  }
   return 0;
 }
 

@@ -54,6 +54,7 @@ enum {
   WFO_GCM,                                                 /* 9: gradCorrM */
   WFO_QFRIC, WFO_CSHEAR, WFO_FHFL,                         /* 1: q_fric_work, cshearabs, friction_hfl */
   WFO_DAMD,                                                /* 1: dam_D */
+  WFO_ZWAB, WFO_VXSPH, WFO_SUMDEN,                         /* 1, 3, 1: ZWab, VXSPH, SumDen */
   WFO_NFIELDS
 };
 /* int fields */
@@ -111,6 +112,8 @@ int wfo_calc_grad_corr(wfo_domain *d);   /* CalcGradCorrMatrix: gradCorrM of eve
 int wfo_calc_accel(wfo_domain *d);                 /* CalcAccel+Reduction  */
 int wfo_calc_dens(wfo_domain *d);                  /* CalcDensInc+Reduction*/
 int wfo_calc_rates(wfo_domain *d);                 /* CalcRateTensors+Red. */
+/* LastComputeAcceleration's pair loop: CalcForce2233 over the pairs */
+int wfo_calc_force2233(wfo_domain *d, double xsph, int shepard);
 int wfo_stress_all(wfo_domain *d, double dt);      /* CalcStressStrain     */
 int wfo_thermal_calcs(wfo_domain *d, double dt);   /* ThermalCalcs         */
 int wfo_general_after(wfo_domain *d);              /* UserAcc BC hook      */

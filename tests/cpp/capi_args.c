@@ -87,6 +87,7 @@ int main(void) {
   EXPECT(wf_rebuild_neighbours(NULL), WF_EINVAL);
   EXPECT(wf_get_neighbour_pairs(NULL, ib, 1, &cnt), WF_EINVAL);
   EXPECT(wf_eval(NULL, 0, 1e-8), WF_EINVAL);
+  EXPECT(wf_calc_force2233(NULL, 0.5, 1), WF_EINVAL);
   EXPECT(wf_set_timing(NULL, 1), WF_EINVAL);
   EXPECT(wf_get_cell_grids(NULL, g3, g3), WF_EINVAL);
   EXPECT(wf_pack_xdmf(NULL, fb, ib), WF_EINVAL);

@@ -959,3 +959,72 @@ def test_damage_triaxiality_and_rate_terms():
         want = np.where(on, np.minimum(want + dpl / np.where(on, ef, 1.0), 1.0), want)
         assert np.allclose(dam, want, rtol=1e-13, atol=0)
     assert want.max() > 0.005
+
+
+# ---------------------------------------------------------------- CalcForce2233
+def test_force2233_equals_split_locking_passes():
+    """CalcForce2233's acceleration, density rate and rate tensors are the
+    Locking passes' terms added pair by pair in the same order: bit for bit
+    the split passes' sums (with and without the gradient correction)."""
+    A = np.array([[1.0, 2.0, -0.5], [0.3, -1.5, 0.7], [-0.2, 0.4, 0.8]]) * 100.0
+    for gc in (False, True):
+        c, d = _jittered(nonlock=False, A=A, grad_corr=gc)
+        d.calc_accel()
+        d.calc_dens()
+        d.calc_rates()
+        split = {k: d.f(k).copy() for k in ("A", "DRHO", "SR", "RR")}
+        d.start_acceleration()
+        d.calc_force2233(xsph=0.5, shepard=True)
+        for k in split:
+            assert np.array_equal(d.f(k), split[k]), (gc, k)
+
+
+def test_force2233_shepard_xsph_closed_forms():
+    """Two particles at distance r: ZWab_1 = (float)(m_2/rho_2) W(r),
+    SumDen_1 = m_2 W(r), VXSPH_1 = XSPH m_2 / ((rho_1 + rho_2)/2) W(r)
+    (v_2 - v_1); on a jittered billet sum_i m_i VXSPH_i = 0 (each pair's two
+    terms cancel) and a uniform velocity gives VXSPH = 0."""
+    h, r = 1.2e-3, 1.5e-3
+    d = po.Domain(2)
+    d.f("X")[:] = [[0.0, 0.0, 0.0], [r, 0.0, 0.0]]
+    d.f("H")[:] = h
+    d.f("MASS")[:] = [2.0e-6, 3.0e-6]
+    d.f("RHO")[:] = [2700.0, 2650.0]
+    d.f("REFRHO")[:] = 2700.0
+    d.f("V")[:] = [[0.1, -0.2, 0.3], [-0.4, 0.5, 0.2]]
+    d.set_material(0, model=po.BILINEAR, E=70e9, nu=0.3)
+    d.set_param("nonlock_sum", 0)
+    for k in range(3):
+        d.set_param(f"DomMax{k}", 0.01)
+        d.set_param(f"DomMin{k}", -0.01)
+    d.neighbour_search()
+    d.start_acceleration()
+    xs = 0.7
+    d.calc_force2233(xsph=xs, shepard=True)
+    W = po.kernel(r / h, h)
+    m, rho, v = d.f("MASS"), d.f("RHO"), d.f("V")
+    assert d.f("ZWAB")[0] == float(np.float32(m[1] / rho[1])) * W
+    assert d.f("ZWAB")[1] == float(np.float32(m[0] / rho[0])) * W
+    assert d.f("SUMDEN")[0] == m[1] * W and d.f("SUMDEN")[1] == m[0] * W
+    want = xs * m[1] / (0.5 * (rho[0] + rho[1])) * W * (v[1] - v[0])
+    assert np.allclose(d.f("VXSPH")[0], want, rtol=1e-15, atol=0)
+    A = np.array([[1.0, 2.0, -0.5], [0.3, -1.5, 0.7], [-0.2, 0.4, 0.8]]) * 100.0
+    c, d = _jittered(nonlock=False, A=A)
+    d.calc_force2233(xsph=0.5, shepard=False)
+    mv = (d.f("MASS")[:, None] * d.f("VXSPH")).sum(0)
+    assert np.abs(mv).max() < 1e-12 * np.abs(d.f("MASS")[:, None] * d.f("VXSPH")).sum(0).max()
+    assert np.all(d.f("SUMDEN") == 0)
+    c, d = _jittered(nonlock=False)
+    d.f("V")[:] = [0.3, -1.2, 2.0]
+    d.calc_force2233(xsph=0.5, shepard=True)
+    assert np.all(d.f("VXSPH") == 0)
+    # on the (slightly jittered) lattice interior the Shepard sum plus the
+    # particle's own term is V sum_k W(|k| dx) over the lattice offsets
+    dx, hh = c.meta["dx"], c.h[0]
+    rr = np.arange(-3, 4)
+    kk = np.stack(np.meshgrid(rr, rr, rr, indexing="ij"), -1).reshape(-1, 3)
+    dist = np.sqrt((kk * kk).sum(1)) * dx
+    lat = (c.mass[0] / c.density[0]) * sum(po.kernel(q / hh, hh) for q in dist if q < 2 * hh)
+    own = d.f("MASS") * np.array([po.kernel(0.0, x) for x in d.f("H")])
+    interior = d.i("NB") == d.i("NB").max()
+    assert np.allclose((d.f("SUMDEN") + own)[interior] / d.f("RHO")[interior], lat, rtol=1e-2)

@@ -29,7 +29,7 @@ FIELDS = {
     "dTdt": (19, 1), "ddensity": (20, 1), "q_plheat": (21, 1),
     "strain_pl_incr": (22, 9), "q_conv": (23, 1), "h_conv": (24, 1),
     "T_inf": (25, 1), "q_source": (26, 1), "ps_energy": (27, 1), "contforce": (28, 3), "gradcorr": (29, 9),
-    "dam_D": (30, 1),
+    "dam_D": (30, 1), "zwab": (31, 1), "vxsph": (32, 3), "sumden": (33, 1),
 }
 RIGID_FIELDS = {"x": 0, "v": 1, "normal": 2}
 INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4, "contnb": 5}
@@ -99,7 +99,7 @@ EXPORTS = [
     "wf_set_contact_heat", "wf_set_rigid_temperature", "wf_prepare", "wf_step_part",
     "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max", "wf_get_cell_grids",
     "wf_set_material_friction", "wf_set_group_sums", "wf_get_group_history",
-    "wf_set_contact_law", "wf_set_friction_function", "wf_set_material_damage",
+    "wf_set_contact_law", "wf_set_friction_function", "wf_set_material_damage", "wf_calc_force2233",
 ]
 
 # contact laws (Domain::contact_alg, Source/Domain.h:64)
@@ -157,6 +157,7 @@ def lib():
         "wf_set_contact_law": ([vp, i32, d, i32], ctypes.c_int),
         "wf_set_friction_function": ([vp, i32, d, d], ctypes.c_int),
         "wf_set_material_damage": ([vp, i32, i32, dp], ctypes.c_int),
+        "wf_calc_force2233": ([vp, d, i32], ctypes.c_int),
         "wf_set_group_sums": ([vp, i32, i64], ctypes.c_int),
         "wf_get_group_history": ([vp, i32, i32, i32, dp, i64, I64P], ctypes.c_int),
         "wf_gen_cylinder": ([dp, d, d, d, dp, i64], i64),
@@ -445,6 +446,10 @@ class Solver:
         """model 1: Johnson-Cook damage with D = (D1..D5); 0: none."""
         Dv = np.ascontiguousarray(D, dtype=np.float64)
         self._check(lib().wf_set_material_damage(self.ctx, index, model, _dp(Dv)))
+
+    def calc_force2233(self, xsph=0.0, shepard=False):
+        """CalcForce2233 over every pair at the current state (no time advance)."""
+        self._check(lib().wf_calc_force2233(self.ctx, float(xsph), int(bool(shepard))))
 
     def set_friction_function(self, linear, m=0.0, b=0.0):
         self._check(lib().wf_set_friction_function(self.ctx, int(bool(linear)), m, b))

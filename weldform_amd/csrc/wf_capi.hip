@@ -34,7 +34,8 @@ struct Error : std::runtime_error {
   } while (0)
 
 const int kFieldDim[WF_NFIELDS] = {3, 3, 3, 3, 1, 1, 1, 9, 9, 9, 9, 9, 9, 1, 1,
-                                   1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1, 3, 9, 1};
+                                   1, 1, 1, 1, 1, 1, 1, 9, 1, 1, 1, 1, 1, 3, 9, 1,
+                                   1, 3, 1};
 
 struct BcHost {
   int zone, type, value_type, amp_id;
@@ -1281,7 +1282,9 @@ int wf_set_field(void *ctx, int32_t field, const double *h) {
   return guard(c, [&] {
     as_ctx(ctx);
     if (field < 0 || field >= WF_NFIELDS || !h) throw Error(WF_EINVAL, "bad field");
-    if (field == WF_F_CONTFORCE || field == WF_F_GRADCORR) throw Error(WF_EINVAL, "read-only field");
+    if (field == WF_F_CONTFORCE || field == WF_F_GRADCORR || field == WF_F_ZWAB || field == WF_F_VXSPH ||
+        field == WF_F_SUMDEN)
+      throw Error(WF_EINVAL, "read-only field");
     const size_t cnt = (size_t)kFieldDim[field] * c->n;
     c->hf[field].assign(h, h + cnt);
     if (!c->inited) return WF_OK;
@@ -1436,6 +1439,19 @@ int wf_get_field(void *ctx, int32_t field, double *h) {
           get_scalar(c, d.damD, h);
         else
           std::memset(h, 0, sizeof(double) * (size_t)c->n);
+        break;
+      case WF_F_ZWAB:
+      case WF_F_SUMDEN:
+        if (d.zwab)
+          get_scalar(c, field == WF_F_ZWAB ? d.zwab : d.sumden, h);
+        else
+          std::memset(h, 0, sizeof(double) * (size_t)c->n);
+        break;
+      case WF_F_VXSPH:
+        if (d.vxsph)
+          get_d4(c, d.vxsph, h, 0, 3, 3);
+        else
+          std::memset(h, 0, sizeof(double) * 3 * (size_t)c->n);
         break;
       case WF_F_CONTFORCE:
         if (d.contact)
@@ -2386,6 +2402,28 @@ int wf_eval(void *ctx, int32_t what, double dt) {
       case WF_EVAL_STRESS: launch_stress(c->d, c->st, c->cur, dt); break;
       default: throw Error(WF_EINVAL, "unknown evaluation");
     }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->st));
+    return WF_OK;
+  });
+}
+
+int wf_calc_force2233(void *ctx, double xsph, int32_t shepard) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    if (!c->built) throw Error(WF_ESTATE, "call wf_rebuild_neighbours first");
+    if (!std::isfinite(xsph)) throw Error(WF_EINVAL, "non-finite XSPH coefficient");
+    if (c->d.ti_on) throw Error(WF_EUNSUPPORTED, "CalcForce2233 with the tensile-instability term");
+    Dev &d = c->d;
+    if (!d.zwab) {
+      // the step graphs hold their own copy of the context and never read these
+      d.zwab = c->alloc<double>((size_t)c->npad);
+      d.sumden = c->alloc<double>((size_t)c->npad);
+      d.vxsph = c->alloc<double4>((size_t)c->npad);
+    }
+    launch_force2233(d, c->st, c->cur, xsph, shepard ? 1 : 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->st));
     return WF_OK;

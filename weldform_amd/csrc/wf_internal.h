@@ -255,6 +255,10 @@ struct Dev {
   // Domain::model_damage and Particle::dam_D (Johnson-Cook damage)
   int damage;
   double *damD;
+  // CalcForce2233's own outputs (allocated by the first wf_calc_force2233):
+  // Particle::ZWab, VXSPH, SumDen
+  double *zwab, *sumden;
+  double4 *vxsph;
   double pfac, fr_m, fr_b;
   double *cstiff;
   // contact heat (Source/Contact_Wang.cpp:245-324): conduction with
@@ -351,6 +355,8 @@ void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz);
 void launch_max_pl(const Dev &d, hipStream_t st);
 void launch_shadows(const Dev &d, hipStream_t st, int cur);
 void launch_surface(const Dev &d, hipStream_t st, int cur, int setup);
+// Domain::CalcForce2233 over every particle's row (FP64)
+void launch_force2233(const Dev &d, hipStream_t st, int cur, double xsph, int shepard);
 // law: Domain::contact_alg (0 Fraser, 1 Wang, 2 Seo, 4 LSDyna)
 void launch_contact(const Dev &d, hipStream_t st, const BcTable &bc, int cur, double dt, int law);
 // per-group sums of the step: what 0 the zone reaction (after the

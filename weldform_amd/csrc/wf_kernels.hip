@@ -2505,6 +2505,105 @@ WF_KERNEL void k_contact_law(Dev d, int cur, double dt) {
   }
 }
 
+// Domain::CalcForce2233 (Source/Interaction.cpp:25-333) as
+// LastComputeAcceleration applies it to every pair (Source/Domain.cpp:
+// 1700-1709), as one FP64 gather per particle over its row: the pair's
+// terms for particle i are the reference's for P1 with (P1, P2) = (i, j)
+// (each term is odd or even under the swap, so the second particle's terms
+// are the same formula from its side).  Per neighbour: a += m_j g.(sigma_i/
+// rho_i^2 + sigma_j/rho_j^2 + Pi), dDensity from (m_j/rho_j) v_ij.g (times
+// rho_i), StrainRate / RotationRate with the float weight (float)(m_j/rho_j)
+// (`float mj_dj`, Source/Interaction.cpp:262), ZWab += (float)(m_j/rho_j) W,
+// VXSPH += XSPH m_j / ((rho_i + rho_j)/2) W (v_j - v_i) and, when shepard,
+// SumDen += m_j W; g = GK x_ij or (GK L_i) x_ij under the gradient
+// correction.  GradientType 0, free-free pairs, no tensile instability.
+template <bool GC>
+__global__ __launch_bounds__(kBlock) void k_force2233(Dev d, int cur, double xsph, int shepard) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= d.n) return;
+  using S = Src<false>;
+  const long long np = d.npad;
+  double a[3] = {0, 0, 0}, s = 0, sr[6] = {0, 0, 0, 0, 0, 0}, w[3] = {0, 0, 0}, zw = 0, sd = 0;
+  double vx[3] = {0, 0, 0};
+  const double rhoi = d.rho[cur][i];
+  if (d.group[i] >= 0) {
+    double Li[6];
+    own_corr<GC>(d, i, Li);
+    const DevMat &M = d.mats[d.mat[i]];
+    const double alpha = M.alpha, beta = M.beta, cs = M.Cs;
+    const bool av = (alpha != 0.0 || beta != 0.0);
+    const double4 xi = S::x(d, cur, i), vi = S::v(d, i);
+    double ti[6], rho_i;
+    S::srec(d, i, ti, rho_i);
+    const KH<double> ki = kern_h_t<double>(xi.w);
+    const HbmJ<false> g{d, cur, i};
+    for_each_nb(g, d.nnb[i], 0, 1, [&](int j) {
+      const double4 xj = g.x(j), vj = g.v(j);
+      double tj[6], rho_j;
+      g.srec(j, tj, rho_j);
+      const KH<double> kh = S::kh(ki, xi, xj);
+      const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const double GK = gk_t<double>(r2, kh);
+      const double q = r2 > 0.0 ? sqrt(r2) * kh.ih : 0.0;
+      const double K = w_shape<double>(q) * (kh.ih * kh.ih * kh.ih * (1.0 / M_PI));
+      const double vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
+      const double vdot = vdx * dx + vdy * dy + vdz * dz;
+      double pi = 0.0;
+      if (av && vdot < 0.0) {
+        const double mu = kh.h * vdot / (r2 + 0.01 * kh.h * kh.h);
+        pi = (alpha * cs * mu + beta * mu * mu) / (0.5 * (rho_i + rho_j));
+      }
+      double ex, ey, ez;
+      corr_dir<GC>(Li, dx, dy, dz, ex, ey, ez);
+      const double gx = GK * ex, gy = GK * ey, gz = GK * ez;
+      const double Mxx = ti[XX] + tj[XX] + pi, Myy = ti[YY] + tj[YY] + pi, Mzz = ti[ZZ] + tj[ZZ] + pi;
+      const double Mxy = ti[XY] + tj[XY], Mxz = ti[XZ] + tj[XZ], Myz = ti[YZ] + tj[YZ];
+      const double mj = vj.w;
+      a[0] += mj * (gx * Mxx + gy * Mxy + gz * Mxz);
+      a[1] += mj * (gx * Mxy + gy * Myy + gz * Myz);
+      a[2] += mj * (gx * Mxz + gy * Myz + gz * Mzz);
+      const double volj = S::vol(d, cur, j, xj);
+      s += volj * (vdx * gx + vdy * gy + vdz * gz);
+      const double fw = (double)(float)volj;
+      const double c = fw * (-0.5 * GK);
+      sr[XX] += c * (2.0 * vdx * ex);
+      sr[YY] += c * (2.0 * vdy * ey);
+      sr[ZZ] += c * (2.0 * vdz * ez);
+      sr[XY] += c * (vdx * ey + vdy * ex);
+      sr[XZ] += c * (vdx * ez + vdz * ex);
+      sr[YZ] += c * (vdy * ez + vdz * ey);
+      w[0] += c * (vdx * ey - vdy * ex);
+      w[1] += c * (vdx * ez - vdz * ex);
+      w[2] += c * (vdy * ez - vdz * ey);
+      // rows are padded with the particle itself: its pair terms above are
+      // exact zeros, the kernel sums must skip it
+      const bool real = j != i;
+      zw += real ? fw * K : 0.0;
+      const double xf = xsph * mj / (0.5 * (rhoi + d.rho[cur][j])) * K;
+      vx[0] += xf * -vdx;
+      vx[1] += xf * -vdy;
+      vx[2] += xf * -vdz;
+      if (shepard && real) sd += mj * K;
+    });
+  }
+  const double4 a4 = d.acc[i];
+  d.acc[i] = make_double4(a[0], a[1], a[2], a4.w);
+  d.drho[i] = s * rhoi;
+  for (int k = 0; k < 6; ++k) d.sr[k * np + i] = sr[k];
+  for (int k = 0; k < 3; ++k) d.rr[k * np + i] = w[k];
+  d.zwab[i] = zw;
+  d.vxsph[i] = make_double4(vx[0], vx[1], vx[2], 0.0);
+  d.sumden[i] = sd;
+}
+
+void launch_force2233(const Dev &d, hipStream_t st, int cur, double xsph, int shepard) {
+  if (d.gc_on)
+    k_force2233<true><<<d.nblocks, kBlock, 0, st>>>(d, cur, xsph, shepard);
+  else
+    k_force2233<false><<<d.nblocks, kBlock, 0, st>>>(d, cur, xsph, shepard);
+}
+
 // Per-group sums (wf_set_group_sums): one block per group walks the group's
 // particles in internal order (thread t takes every kGroupThreads-th) and
 // reduces in a fixed order, so the sums are deterministic.  what 0: the
