@@ -462,8 +462,11 @@ void allocate(Ctx *c) {
   d.cell_start = c->alloc<int>((size_t)d.sncells + 1);
   d.cell_cursor = c->alloc<int>((size_t)d.sncells + 1);
   d.scell = c->alloc<int>(np);
-  d.sxyz = c->alloc<float4>(np);
-  d.sinfo = c->alloc<int2>(np);
+  d.sxyz = c->alloc<float4>(np + 4);
+  d.sinfo = c->alloc<int2>(np + 4);
+  d.soff = c->alloc<int4>((size_t)d.sncells);
+  d.sslice = c->alloc<unsigned char>(np);
+  d.hblk = c->alloc<unsigned>((size_t)d.nblocks);
   c->dmats = c->alloc<DevMat>(kMaxMat);
   d.mats = c->dmats;
   c->damps = c->alloc<DevAmp>(kMaxAmp);

@@ -118,6 +118,8 @@ struct DevFlags {
   double dt_prev;    // prev_deltat of the step
   double dtint;      // Domain::deltatint (the initial dt)
   double min_force_ts;  // Domain::min_force_ts (the contact step criterion)
+  unsigned hmax_bits;   // largest FP32 h of the search in progress (bits of a positive float)
+  unsigned fin_ticket;  // blocks of the end-of-step reduction done (0 between steps)
 };
 
 // partial-sum slots per block
@@ -214,6 +216,12 @@ struct Dev {
   // FP32, and (particle index, group)
   float4 *sxyz;
   int2 *sinfo;
+  // per search cell: where its z-slices 1, 2, 3 start in the cell order (a
+  // cell's records are ordered by the quarter of the cell height they lie
+  // in, then by index), and the cell's end
+  int4 *soff;
+  unsigned char *sslice;  // each particle's z-slice in its search cell
+  unsigned *hblk;         // per block of particles: the largest FP32 h (bits)
   // rigid (contact) particles binned in the same cells: counts, starts,
   // cursors [ncells + 1], cell of each rigid particle and the cell order
   int *rcell, *rcount, *rstart, *rcursor, *rsorted;

@@ -136,20 +136,47 @@ __device__ inline int scell_of(const Dev &d, const double4 x) {
 // threads 0..n-1: the SPH particles; n..n+nrig-1: the rigid particles.
 // The counts and cursors are zero on entry (allocation, then cleared by
 // k_cell_sort once a search has consumed them).
+// z-slice (quarter of the cell height) of a record with FP32 offset z from
+// its cell's lower face: the floor of z / (cell height / 4), clamped into
+// the cell (a particle clamped into a boundary cell, or one that rounds
+// onto a face, counts in the nearest slice)
+__device__ inline int z_slice(float z, float inv_hz) {
+  const float q = z * inv_hz;
+  return q < 1.f ? 0 : (q < 2.f ? 1 : (q < 3.f ? 2 : 3));
+}
+
+// Also records each SPH particle's z-slice in its search cell (offset
+// rounded to FP32 exactly as k_cell_sort's records are) and, per block, the
+// largest SPH h (its FP32 bits; h > 0, so the bits order as the values)
+// for the candidate pruning of k_build_nbr.
 WF_KERNEL void k_cell_hist(Dev d, int cur) {
   if (!d.flags->do_search) return;
+  __shared__ unsigned hw[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  unsigned hb = 0;
   if (i < d.n) {
     const double4 x = d.xh[cur][i];
+    hb = __float_as_uint((float)x.w);
     d.cell[i] = cell_of(d, x);
     const int c = scell_of(d, x);
     d.scell[i] = c;
+    const int cz = c % d.scellno[2];
+    const double oz = d.blpf[2] + cz * d.scellsize[2];
+    d.sslice[i] = (unsigned char)z_slice((float)(x.z - oz), 4.f / (float)d.scellsize[2]);
     atomicAdd(&d.cell_count[c], 1);
   } else if (i - d.n < d.nrig) {
     const int r = i - d.n;
     const int c = cell_of(d, d.rx[r]);
     d.rcell[r] = c;
     atomicAdd(&d.rcount[c], 1);
+  }
+  for (int off = 32; off > 0; off >>= 1) hb = max(hb, (unsigned)__shfl_xor((int)hb, off, 64));
+  if ((threadIdx.x & 63) == 0) hw[threadIdx.x >> 6] = hb;
+  __syncthreads();
+  if (threadIdx.x == 0 && blockIdx.x < d.nblocks) {
+    unsigned m = hw[0];
+    for (int w = 1; w < kBlock / 64; ++w) m = max(m, hw[w]);
+    d.hblk[blockIdx.x] = m;
   }
 }
 
@@ -239,6 +266,18 @@ __global__ __launch_bounds__(1024) void k_scan(Dev d) {
   if (!d.flags->do_search) return;
   __shared__ int sh[34];
   __shared__ int buf[kScanRound + kScanRound / 16];
+  // the largest SPH h of this search from k_cell_hist's block maxima
+  unsigned hb = 0;
+  for (int b = threadIdx.x; b < d.nblocks; b += 1024) hb = max(hb, d.hblk[b]);
+  for (int off = 32; off > 0; off >>= 1) hb = max(hb, (unsigned)__shfl_xor((int)hb, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = (int)hb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned m = 0;
+    for (int w = 0; w < 16; ++w) m = max(m, (unsigned)sh[w]);
+    d.flags->hmax_bits = m;
+  }
+  __syncthreads();
   scan_cells(d.cell_count, d.cell_start, d.sncells, sh, buf);
   if (d.nrig) scan_cells(d.rcount, d.rstart, d.ncells, sh, buf);
 }
@@ -255,11 +294,11 @@ WF_KERNEL void k_scatter(Dev d) {
   }
 }
 
-// per cell: clear the counters for the next search, sort the cell's
-// particles by index (cells hold ~10-30 particles; the order, and with it
-// every neighbour row, becomes independent of the atomic order) and write
-// their candidate records (FP32 offsets from the cell's lower corner, h,
-// index, group) in cell order
+// per cell: clear the counters for the next search, order the cell's
+// particles by z-slice, then by index (cells hold ~10-30 particles; the
+// order, and with it every neighbour row, becomes independent of the atomic
+// order), record where slices 1-3 start, and write the candidate records
+// (FP32 offsets from the cell's lower corner, h, index, group) in cell order
 WF_KERNEL void k_cell_sort(Dev d, int cur) {
   if (!d.flags->do_search) return;
   const int c = blockIdx.x * kBlock + threadIdx.x;
@@ -271,33 +310,48 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
   d.cell_count[c] = 0;
   d.cell_cursor[c] = 0;
   const int s = d.cell_start[c], e = d.cell_start[c + 1];
-  for (int a = s + 1; a < e; ++a) {
-    const int v = d.sorted[a];
-    int b = a - 1;
-    while (b >= s && d.sorted[b] > v) {
-      d.sorted[b + 1] = d.sorted[b];
-      --b;
-    }
-    d.sorted[b + 1] = v;
-  }
   const int nz = d.scellno[2], ny = d.scellno[1];
   const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
   const double ox = d.blpf[0] + cx * d.scellsize[0], oy = d.blpf[1] + cy * d.scellsize[1],
                oz = d.blpf[2] + cz * d.scellsize[2];
+  // key (slice, index) of each entry, held in sinfo while sorting
   for (int p = s; p < e; ++p) {
     const int j = d.sorted[p];
+    d.sinfo[p] = make_int2(j, d.sslice[j]);
+  }
+  for (int a = s + 1; a < e; ++a) {
+    const int2 v = d.sinfo[a];
+    int b = a - 1;
+    for (; b >= s; --b) {
+      const int2 u = d.sinfo[b];
+      if (u.y < v.y || (u.y == v.y && u.x < v.x)) break;
+      d.sinfo[b + 1] = u;
+    }
+    d.sinfo[b + 1] = v;
+  }
+  int st[3] = {e, e, e};
+  for (int p = e - 1; p >= s; --p) {
+    const int2 k = d.sinfo[p];
+    if (k.y >= 1) st[0] = p;
+    if (k.y >= 2) st[1] = p;
+    if (k.y >= 3) st[2] = p;
+    const int j = k.x;
     const double4 x = d.xh[cur][j];
+    d.sorted[p] = j;
     d.sxyz[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
     d.sinfo[p] = make_int2(j, d.group[j]);
   }
+  d.soff[c] = make_int4(st[0], st[1], st[2], e);
 }
 
 // One lane per particle (lanes in cell order) over the 27 cells around the
-// particle's own, in cell order (each cell's particles by index), so a row
-// lists its neighbours in a fixed order (cells of the SPH search grid).  The
-// 27 cells are 9 columns of up to three consecutive cell ids, whose
-// candidates are one contiguous range of the cell-ordered records; a
-// range is walked four candidates at a time, their loads issued together.
+// particle's own, in cell order (each cell's particles by z-slice, then by
+// index), so a row lists its neighbours in a fixed order (cells of the SPH
+// search grid).  The 27 cells are 9 columns of up to three consecutive cell
+// ids, whose candidates are one contiguous range of the cell-ordered
+// records; of it only the slices within reach of the particle are walked
+// (columns out of reach are skipped), four candidates at a time, their
+// loads issued together.
 // The radius test runs in FP32 on offsets from the cell corners (|error| <
 // 1e-6 relative at the radius) and is trusted outside a band of 1e-4
 // around it; inside the band, and for a particle lying far outside the grid
@@ -313,7 +367,8 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
   const int i = ps < d.n ? d.sorted[ps] : ps;
   const long long np = d.npad;
   int4 *row = reinterpret_cast<int4 *>(d.nbr) + i;
-  int4 pend = make_int4(i, i, i, i);  // the row's group being filled
+  int4 pend = make_int4(i, i, i, i);  // the row's last four entries (the group being filled)
+  int4 *wp = row;                     // where the group being filled goes
   int cnt = 0;
   if (i < d.n && d.group[i] >= 0) {
     const int g = d.group[i];
@@ -328,8 +383,18 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
     const float hi = (float)xi.w, hcf = 0.5f * (float)d.cellfac;
     const bool exact_all = fabsf(oxi) > 64.f * csx || fabsf(oyi) > 64.f * csy || fabsf(ozi) > 64.f * csz;
     const int az0 = cz > 0 ? cz - 1 : 0, az1 = cz + 1 < nz ? cz + 1 : nz - 1, m = az1 - az0;
+    // every neighbour lies within ri of particle i: the pair radius
+    // Cellfac (h_i + h_j) / 2 with h_j at most the largest h of the search,
+    // widened by 1e-4 (FP32 rounding of these offsets is ~1e-7 of a cell)
+    const float ri = hcf * (hi + __uint_as_float(d.flags->hmax_bits)) * 1.0001f, ri2 = ri * ri;
+    const float inv_hz = 4.f / csz;  // the z-slices of k_cell_sort
+    const int gmax = 4 * m + 3;      // last slice of the column
+    // z of i from the lower faces of cells az0, az0 + 1, az0 + 2
+    const float bz0 = (float)(cz - az0) * csz + ozi, bz1 = (float)(cz - az0 - 1) * csz + ozi,
+                bz2 = (float)(cz - az0 - 2) * csz + ozi;
     for (int col = 0; col < 9; ++col) {
-      const int ax = cx + col / 3 - 1, ay = cy + col % 3 - 1;
+      const int dcx = col / 3 - 1, dcy = col % 3 - 1;
+      const int ax = cx + dcx, ay = cy + dcy;
       if (ax < 0 || ax >= nx || ay < 0 || ay >= ny) continue;
       const float bx = (float)(cx - ax) * csx + oxi, by = (float)(cy - ay) * csy + oyi;
       const int cc0 = (ax * ny + ay) * nz + az0;
@@ -337,42 +402,72 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
       const int s0 = d.cell_start[cc0], s1 = d.cell_start[cc0 + 1];
       const int s2 = m >= 1 ? d.cell_start[cc0 + 2] : s1;
       const int se = d.cell_start[cc0 + m + 1];
-      for (int pb = s0; pb < se; pb += 4) {
+      // the column's records within reach: the column lies at least rho from
+      // i across x and y (a face of i's own cell bounds it, clamped records
+      // included), so only |z - z_i| <= sqrt(ri^2 - rho^2) can pair, which
+      // is slices glo..ghi of the column (z from the bottom of cell az0)
+      int lo = s0, hi_end = se;
+      if (!exact_all) {
+        const float gx = fmaxf(dcx < 0 ? oxi : (dcx > 0 ? csx - oxi : 0.f), 0.f);
+        const float gy = fmaxf(dcy < 0 ? oyi : (dcy > 0 ? csy - oyi : 0.f), 0.f);
+        const float rho2 = gx * gx + gy * gy;
+        if (rho2 > ri2) continue;
+        const float w = sqrtf(ri2 - rho2);
+        const int glo = min(max((int)floorf((bz0 - w) * inv_hz), 0), gmax);
+        const int ghi = min(max((int)floorf((bz0 + w) * inv_hz), 0), gmax);
+        const int alo = glo >> 2, klo = glo & 3, ahi = ghi >> 2, khi = ghi & 3;
+        const int4 ol = d.soff[cc0 + alo], oh = d.soff[cc0 + ahi];
+        lo = klo == 0 ? (alo == 0 ? s0 : (alo == 1 ? s1 : s2)) : (klo == 1 ? ol.x : (klo == 2 ? ol.y : ol.z));
+        hi_end = khi == 3 ? oh.w : (khi == 0 ? oh.x : (khi == 1 ? oh.y : oh.z));
+      }
+      // (the record arrays hold 4 spare entries, so a batch may read past
+      // the range's end; those candidates are not valid)
+      const int2 *fp = d.sinfo + lo;
+      const float4 *qp = d.sxyz + lo;
+      for (int pb = lo; pb < hi_end; pb += 4, fp += 4, qp += 4) {
         int2 f[4];
         float4 q[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const int p = pb + u < se ? pb + u : se - 1;
-          f[u] = d.sinfo[p];
-          q[u] = d.sxyz[p];
+          f[u] = fp[u];
+          q[u] = qp[u];
         }
         // compiler-only barrier: the four candidates' loads stay issued
         // together (not sunk into the accept branches); emits no instruction
         asm volatile("" ::: "memory");
+        bool in[4], near[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
           const int p = pb + u;
-          const bool valid = p < se && f[u].x != i && f[u].y == g;
-          const int az = az0 + (p >= s1 ? 1 : 0) + (p >= s2 ? 1 : 0);
-          const float bz = (float)(cz - az) * csz + ozi;
+          const bool valid = (p < hi_end) & (f[u].x != i) & (f[u].y == g);
+          const float bz = p >= s2 ? bz2 : (p >= s1 ? bz1 : bz0);
           const float dx = bx - q[u].x, dy = by - q[u].y, dz = bz - q[u].z;
           const float r2 = dx * dx + dy * dy + dz * dz;
           const float t = hcf * (hi + q[u].w);
           const float t2 = t * t;
-          bool in = valid && r2 < t2;
-          if (valid && (exact_all || fabsf(r2 - t2) <= 1e-4f * t2)) in = in_radius(xi, d.xh[cur][f[u].x], d.cellfac);
-          if (in) {
-            // entries are gathered four at a time and stored as one group
-            // (past max_nb they are only counted)
-            if (cnt < d.max_nb) {
-              const int k = cnt & 3;
-              pend.x = k == 0 ? f[u].x : pend.x;
-              pend.y = k == 1 ? f[u].x : pend.y;
-              pend.z = k == 2 ? f[u].x : pend.z;
-              pend.w = k == 3 ? f[u].x : pend.w;
-              if (k == 3) row[(long long)(cnt >> 2) * np] = pend;
-            }
-            ++cnt;
+          in[u] = valid & (r2 < t2);
+          near[u] = valid & (exact_all | (fabsf(r2 - t2) <= 1e-4f * t2));
+        }
+        // the exact predicate decides the (rare) candidates in the band
+        if (near[0] | near[1] | near[2] | near[3]) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (near[u]) in[u] = in_radius(xi, d.xh[cur][f[u].x], d.cellfac);
+        }
+        // append without branches: the group being filled shifts left and
+        // the entry enters at w; a full group is stored (entries past max_nb
+        // are only counted)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool take = in[u] && cnt < d.max_nb;
+          pend.x = take ? pend.y : pend.x;
+          pend.y = take ? pend.z : pend.y;
+          pend.z = take ? pend.w : pend.z;
+          pend.w = take ? f[u].x : pend.w;
+          cnt += in[u] ? 1 : 0;
+          if (take & ((cnt & 3) == 0)) {
+            *wp = pend;
+            wp += np;
           }
         }
       }
@@ -384,12 +479,12 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
     if (cnt > d.max_nb) d.flags->overflow = 1;
     // the pair passes read rows in groups of four: the rest of the last
     // group holds the particle itself (a pair term of exact zeros)
+    // (the k = kept & 3 pending entries sit in the last k places of pend)
     if (kept & 3) {
       const int k = kept & 3;
-      pend.y = k <= 1 ? i : pend.y;
-      pend.z = k <= 2 ? i : pend.z;
-      pend.w = i;
-      row[(long long)(kept >> 2) * np] = pend;
+      const int4 last = k == 1 ? make_int4(pend.w, i, i, i)
+                               : (k == 2 ? make_int4(pend.z, pend.w, i, i) : make_int4(pend.y, pend.z, pend.w, i));
+      *wp = last;
     }
   }
   // deterministic totals via block reduction + integer atomics
@@ -406,7 +501,8 @@ WF_KERNEL void k_build_nbr(Dev d, int cur) {
 // and all their neighbour entries go into an LDS hash set (open
 // addressing); its distinct values (the staged set, ~6 % of the ~7000
 // entries: each neighbour appears in many rows) are sorted (bitonic, in
-// LDS) so a pass loads the set in long contiguous runs; each value's hash
+// registers and lane shuffles) so a pass loads the set in long contiguous
+// runs; each value's hash
 // slot then records its position in the set, and every row entry is
 // rewritten as that position.  The hash table's layout depends on insertion
 // order, the sorted set does not.  Tiles whose set exceeds stage_max keep
@@ -418,6 +514,56 @@ static_assert(kHashSlots >= 2 * kStageMax && (kHashSlots & (kHashSlots - 1)) == 
 static_assert(kStagePow2 >= kStageMax && (kStagePow2 & (kStagePow2 - 1)) == 0, "sort padding");
 static_assert(kStageThreads == 2 * kBlock, "two threads per particle of a tile");
 __device__ inline unsigned stage_hash(int v) { return ((unsigned)v * 2654435761u) >> 21; }  // 11 bits
+static_assert(kStagePow2 == 4 * kStageThreads, "the set sort holds four values per thread");
+
+// Bitonic sort (ascending) of the first P values (P a power of two <=
+// kStagePow2) of the block, element e = 4 tid + r held in v[r]; elements at
+// e >= P must hold values >= every value below P.  Compare-exchanges within
+// a thread (j = 1, 2) stay in registers, those within a wavefront
+// (4 <= j < 256) exchange through lane shuffles, and only j >= 256 goes
+// through LDS (buf: kStagePow2 ints), two barriers each.  Every thread of
+// the block calls this with the same P.
+__device__ inline void stage_sort(int (&v)[4], int P, int *buf) {
+  const int tid = threadIdx.x;
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 4) {
+        int o[4];
+        if (j >= 256) {
+          reinterpret_cast<int4 *>(buf)[tid] = make_int4(v[0], v[1], v[2], v[3]);
+          __syncthreads();
+          const int4 w = reinterpret_cast<const int4 *>(buf)[tid ^ (j >> 2)];
+          __syncthreads();
+          o[0] = w.x;
+          o[1] = w.y;
+          o[2] = w.z;
+          o[3] = w.w;
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) o[r] = __shfl_xor(v[r], j >> 2, 64);
+        }
+        // the element at the lower index of its pair keeps the minimum when
+        // the run (e & k) sorts ascending
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = 4 * tid + r;
+          const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
+          v[r] = keep_min ? min(v[r], o[r]) : max(v[r], o[r]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r & j) continue;
+          const int e = 4 * tid + r;
+          const int a = v[r], b = v[r ^ j];
+          const bool asc = (e & k) == 0;
+          v[r] = asc ? min(a, b) : max(a, b);
+          v[r ^ j] = asc ? max(a, b) : min(a, b);
+        }
+      }
+    }
+  }
+}
 
 __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
   if (!d.flags->do_search) return;
@@ -440,18 +586,30 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
     __syncthreads();
     // a value is counted by the thread whose CAS claims its slot; probing
     // stops once the set has more than stage_max values (the tile then
-    // gathers from HBM), and every probe sequence ends within kHashSlots
-    auto insert = [&](int v) {
-      unsigned s = stage_hash(v) & mask;
-      for (int probe = 0; probe < kHashSlots; ++probe) {
-        if (*(volatile int *)&full) break;
-        const int old = atomicCAS(&ht[s], -1, v);
-        if (old == -1) {
-          if (atomicAdd(&nset, 1) >= d.stage_max) full = 1;
-          break;
+    // gathers from HBM), and every probe sequence ends within kHashSlots.
+    // A group's four home-slot CAS are issued together (they are
+    // independent; a lane's atomics on one slot apply in order, so the
+    // repeated padding entries find their own value); only collisions
+    // probe on, one value at a time.
+    auto insert4 = [&](const int4 v4) {
+      if (*(volatile int *)&full) return;
+      const int vv[4] = {v4.x, v4.y, v4.z, v4.w};
+      unsigned s[4];
+      int old[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] = stage_hash(vv[u]) & mask;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) old[u] = atomicCAS(&ht[s[u]], -1, vv[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        int o = old[u];
+        unsigned sl = s[u];
+        for (int probe = 1; probe < kHashSlots && o != -1 && o != vv[u]; ++probe) {
+          if (*(volatile int *)&full) break;
+          sl = (sl + 1) & mask;
+          o = atomicCAS(&ht[sl], -1, vv[u]);
         }
-        if (old == v) break;
-        s = (s + 1) & mask;
+        if (o == -1 && atomicAdd(&nset, 1) >= d.stage_max) full = 1;
       }
     };
     // a row's groups of four entries (the last padded with i itself), the
@@ -460,7 +618,7 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
     const int4 *row = reinterpret_cast<const int4 *>(d.nbr) + i;
     const int ng = (c + 3) >> 2;
     if (i < d.n) {
-      if (half == 0) insert(i);
+      if (half == 0) insert4(make_int4(i, i, i, i));
       for (int g0 = half; g0 < ng; g0 += 8) {
         int4 v[4];
 #pragma unroll
@@ -468,10 +626,7 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           if (g0 + 2 * b >= ng) break;
-          insert(v[b].x);
-          insert(v[b].y);
-          insert(v[b].z);
-          insert(v[b].w);
+          insert4(v[b]);
         }
       }
     }
@@ -481,43 +636,40 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
       __syncthreads();
       continue;
     }
-    // compact the table (slot order), pad to a power of two, sort
-    int carry = 0;
-    for (int base = 0; base < kHashSlots; base += kStageThreads) {
-      const int v = ht[base + tid];
-      const int f = v >= 0 ? 1 : 0;
-      int sub;
-      const int e = block_scan_excl(f, sh, &sub) + carry;
-      if (f) uniq[e] = v;
-      carry += sub;
-    }
-    const int U = carry;
+    // compact the table: thread tid takes slots [8 tid, 8 tid + 8), one
+    // block scan places its values (the order is irrelevant: sorted next)
+    constexpr int kSlotsPer = kHashSlots / kStageThreads;
+    static_assert(kSlotsPer == 8, "two int4 loads of slots per thread");
+    const int4 h0 = reinterpret_cast<const int4 *>(ht)[2 * tid];
+    const int4 h1 = reinterpret_cast<const int4 *>(ht)[2 * tid + 1];
+    const int hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+    int f = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) f += hv[q] >= 0 ? 1 : 0;
+    int U;
+    int e = block_scan_excl(f, sh, &U);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (hv[q] >= 0) uniq[e++] = hv[q];
     int P = 1;
     while (P < U) P <<= 1;
-    for (int p = U + tid; p < P; p += kStageThreads) uniq[p] = 0x7fffffff;
     __syncthreads();
-    for (int k = 2; k <= P; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        for (int p = tid; p < P; p += kStageThreads) {
-          const int q = p ^ j;
-          if (q > p) {
-            const int a = uniq[p], b = uniq[q];
-            if ((a > b) == ((p & k) == 0)) {
-              uniq[p] = b;
-              uniq[q] = a;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
+    int sv[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sv[r] = 4 * tid + r < U ? uniq[4 * tid + r] : 0x7fffffff;
+    __syncthreads();
+    stage_sort(sv, P, uniq);
     // each value's slot records its position in the sorted set
-    for (int p = tid; p < U; p += kStageThreads) {
-      const int v = uniq[p];
-      unsigned s = stage_hash(v) & mask;
-      while (ht[s] != v) s = (s + 1) & mask;
-      hp[s] = (unsigned short)p;
-      d.stage[(long long)t * kStageMax + p] = v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int p = 4 * tid + r;
+      if (p < U) {
+        const int v = sv[r];
+        unsigned s = stage_hash(v) & mask;
+        while (ht[s] != v) s = (s + 1) & mask;
+        hp[s] = (unsigned short)p;
+        d.stage[(long long)t * kStageMax + p] = v;
+      }
     }
     __syncthreads();
     auto pos = [&](int v) -> unsigned short {
@@ -2004,8 +2156,10 @@ void launch_thermal(const Dev &d, hipStream_t st, int buf, double dt) {
 // end of step: fixed-order reduction of the block partials, scalar updates,
 // reaction history, and the end-of-step half of the neighbour schedule.
 // One block per active slot sums that slot's partials (each thread its
-// blocks in order, the wavefront butterfly, then the wavefronts in order),
-// so the slots reduce in parallel across CUs; one thread then applies them.
+// blocks in order, four loads in flight, then the wavefront butterfly and
+// the wavefronts in order), so the slots reduce in parallel across CUs; the
+// block that finishes last (a ticket counter, reset by that block for the
+// next step) then applies them with one thread.
 
 struct FinList {
   int n;
@@ -2013,23 +2167,11 @@ struct FinList {
   int op[P_COUNT];  // 0 sum, 1 max, 2 min
 };
 
-__global__ __launch_bounds__(256) void k_fin_sum(Dev d, FinList L) {
-  __shared__ double sh[4];
-  const int s = L.slot[blockIdx.x], op = L.op[blockIdx.x];
-  double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-  for (int b = threadIdx.x; b < d.nblocks; b += blockDim.x) {
-    const double x = part(d, s, b);
-    v = op == 0 ? v + x : (op == 1 ? fmax(v, x) : fmin(v, x));
-  }
-  double r;
-  if (op == 0) r = block_reduce<0>(v, sh);
-  else if (op == 1) r = block_reduce<1>(v, sh);
-  else r = block_reduce<2>(v, sh);
-  if (threadIdx.x == 0) d.fin[s] = r;
+__device__ inline double fin_op(int op, double v, double x) {
+  return op == 0 ? v + x : (op == 1 ? fmax(v, x) : fmin(v, x));
 }
 
-__global__ __launch_bounds__(64) void k_fin_apply(Dev d, double dt, FinList L) {
-  if (threadIdx.x != 0) return;
+__device__ void fin_apply(const Dev &d, double dt, const FinList &L) {
   if (dt < 0) dt = d.flags->dt;
   DevFlags *f = d.flags;
   const long long step = f->step;
@@ -2037,7 +2179,8 @@ __global__ __launch_bounds__(64) void k_fin_apply(Dev d, double dt, FinList L) {
   double e_kin = 0.0, e_int = 0.0, plw = 0.0, tmax = 0.0, tmin = 0.0, cfsum = 0.0, creact = 0.0, extw = 0.0;
   for (int k = 0; k < L.n; ++k) {
     const int s = L.slot[k];
-    const double r = d.fin[s];
+    // written by the other blocks of this launch: read past the L1 cache
+    const double r = __hip_atomic_load(&d.fin[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (s == P_KIN) e_kin = r;
     if (s == P_INT) e_int = r;
     if (s == P_PLW) plw = r;
@@ -2073,6 +2216,40 @@ __global__ __launch_bounds__(64) void k_fin_apply(Dev d, double dt, FinList L) {
   f->time += dt;
 }
 
+constexpr int kFinThreads = 1024;
+__global__ __launch_bounds__(kFinThreads) void k_finalize(Dev d, double dt, FinList L) {
+  __shared__ double sh[kFinThreads / 64];
+  __shared__ int last;
+  const int s = L.slot[blockIdx.x], op = L.op[blockIdx.x];
+  const int nb = d.nblocks;
+  double v = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
+  for (int b = threadIdx.x; b < nb; b += 4 * blockDim.x) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int bu = b + u * (int)blockDim.x;
+      x[u] = bu < nb ? part(d, s, bu) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (b + u * (int)blockDim.x < nb) v = fin_op(op, v, x[u]);
+  }
+  double r;
+  if (op == 0) r = block_reduce<0>(v, sh);
+  else if (op == 1) r = block_reduce<1>(v, sh);
+  else r = block_reduce<2>(v, sh);
+  if (threadIdx.x == 0) {
+    d.fin[s] = r;
+    __threadfence();
+    last = atomicAdd(&d.flags->fin_ticket, 1u) == (unsigned)(L.n - 1);
+  }
+  __syncthreads();
+  if (!last || threadIdx.x != 0) return;
+  __threadfence();
+  fin_apply(d, dt, L);
+  d.flags->fin_ticket = 0;
+}
+
 void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
   FinList L{};
   auto add = [&](int s, int op) {
@@ -2098,8 +2275,7 @@ void launch_finalize(const Dev &d, hipStream_t st, double dt, int nz) {
       add(P_CHC, 0);
     }
   }
-  k_fin_sum<<<L.n, 256, 0, st>>>(d, L);
-  k_fin_apply<<<1, 64, 0, st>>>(d, dt, L);
+  k_finalize<<<L.n, kFinThreads, 0, st>>>(d, dt, L);
 }
 
 // ---------------------------------------------------------------------------
