@@ -460,7 +460,16 @@ void allocate(Ctx *c) {
   d.nbr = c->alloc<int>((size_t)nb_rows(c->cfg.max_nb) * np);
   d.cell_count = c->alloc<int>((size_t)d.sncells + 1);
   d.cell_start = c->alloc<int>((size_t)d.sncells + 1);
-  d.cell_cursor = c->alloc<int>((size_t)d.sncells + 1);
+  {
+    // chunks of cells per block of the cell-start scan
+    const size_t nsch = ((size_t)d.sncells + kScanChunk - 1) / kScanChunk;
+    const size_t nrch = ((size_t)d.ncells + kScanChunk - 1) / kScanChunk;
+    d.srank = c->alloc<int>(np);
+    d.cell_loc = c->alloc<int>((size_t)d.sncells + 1);
+    d.ctot = c->alloc<int>(nsch);
+    d.coff = c->alloc<int>(nsch);
+    d.hred = c->alloc<unsigned>(nsch + nrch);
+  }
   d.scell = c->alloc<int>(np);
   d.sxyz = c->alloc<float4>(np + 4);
   d.sinfo = c->alloc<int2>(np + 4);
@@ -514,7 +523,11 @@ void allocate(Ctx *c) {
     d.rsorted = c->alloc<int>(ne);
     d.rcount = c->alloc<int>((size_t)d.ncells + 1);
     d.rstart = c->alloc<int>((size_t)d.ncells + 1);
-    d.rcursor = c->alloc<int>((size_t)d.ncells + 1);
+    const size_t nrch = ((size_t)d.ncells + kScanChunk - 1) / kScanChunk;
+    d.rrank = c->alloc<int>(ne);
+    d.rloc = c->alloc<int>((size_t)d.ncells + 1);
+    d.rtot = c->alloc<int>(nrch);
+    d.roff = c->alloc<int>(nrch);
     d.id_orig = c->alloc<int>(np);
     d.ccnt = c->alloc<int>(np);
     d.cidx = c->alloc<int>((size_t)kMaxContact * np);

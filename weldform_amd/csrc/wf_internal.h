@@ -26,6 +26,7 @@
 namespace wf {
 
 constexpr int kBlock = 128;
+constexpr int kScanChunk = 1024;  // cells per block of the counting sort's cell-start scan
 // LDS staging of the mixed-precision pair passes: a tile is one block of
 // kBlock consecutive particles; its staged set (the tile's particles and all
 // their neighbours, sorted by index) holds at most kStageMax particles.
@@ -120,6 +121,7 @@ struct DevFlags {
   double min_force_ts;  // Domain::min_force_ts (the contact step criterion)
   unsigned hmax_bits;   // largest FP32 h of the search in progress (bits of a positive float)
   unsigned fin_ticket;  // blocks of the end-of-step reduction done (0 between steps)
+  unsigned scan_ticket; // blocks of the cell-start scan done (0 between searches)
 };
 
 // partial-sum slots per block
@@ -206,7 +208,12 @@ struct Dev {
   // partners and the thermal pair orientation use; scell: its cell in the
   // search grid (same box, scellsize >= Cellfac x the largest SPH h), which
   // the SPH neighbour build uses and the counting sort below is over
-  int *cell, *cell_count, *cell_start, *cell_cursor, *sorted;
+  int *cell, *cell_count, *cell_start, *sorted;
+  // counting sort: each particle's rank within its cell (atomic order), the
+  // cells' in-chunk offsets, the chunks' totals and offsets (chunks of cells
+  // scanned by one block each) and the blocks' shares of the largest h
+  int *srank, *cell_loc, *ctot, *coff;
+  unsigned *hred;
   int *scell;
   int scellno[3];
   double scellsize[3];
@@ -224,7 +231,8 @@ struct Dev {
   unsigned *hblk;         // per block of particles: the largest FP32 h (bits)
   // rigid (contact) particles binned in the same cells: counts, starts,
   // cursors [ncells + 1], cell of each rigid particle and the cell order
-  int *rcell, *rcount, *rstart, *rcursor, *rsorted;
+  int *rcell, *rcount, *rstart, *rsorted;
+  int *rrank, *rloc, *rtot, *roff;  // the rigid grid's counting sort, as above
   int *nnb, *nbr;
   // LDS staging (mixed precision): per tile the sorted staged particle list
   // stage[t][0..stage_cnt[t]) (stage_cnt = -1: tile gathers from HBM), each

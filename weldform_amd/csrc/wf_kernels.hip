@@ -134,8 +134,8 @@ __device__ inline int scell_of(const Dev &d, const double4 x) {
 }
 
 // threads 0..n-1: the SPH particles; n..n+nrig-1: the rigid particles.
-// The counts and cursors are zero on entry (allocation, then cleared by
-// k_cell_sort once a search has consumed them).
+// The counts are zero on entry (allocation, then cleared by k_cell_sort
+// once a search has consumed them).
 // z-slice (quarter of the cell height) of a record with FP32 offset z from
 // its cell's lower face: the floor of z / (cell height / 4), clamped into
 // the cell (a particle clamped into a boundary cell, or one that rounds
@@ -145,7 +145,9 @@ __device__ inline int z_slice(float z, float inv_hz) {
   return q < 1.f ? 0 : (q < 2.f ? 1 : (q < 3.f ? 2 : 3));
 }
 
-// Also records each SPH particle's z-slice in its search cell (offset
+// The counter's old value is the particle's place among its cell's
+// particles (in atomic order; k_cell_sort fixes the order).  Also records
+// each SPH particle's z-slice in its search cell (offset
 // rounded to FP32 exactly as k_cell_sort's records are) and, per block, the
 // largest SPH h (its FP32 bits; h > 0, so the bits order as the values)
 // for the candidate pruning of k_build_nbr.
@@ -163,12 +165,12 @@ WF_KERNEL void k_cell_hist(Dev d, int cur) {
     const int cz = c % d.scellno[2];
     const double oz = d.blpf[2] + cz * d.scellsize[2];
     d.sslice[i] = (unsigned char)z_slice((float)(x.z - oz), 4.f / (float)d.scellsize[2]);
-    atomicAdd(&d.cell_count[c], 1);
+    d.srank[i] = atomicAdd(&d.cell_count[c], 1);
   } else if (i - d.n < d.nrig) {
     const int r = i - d.n;
     const int c = cell_of(d, d.rx[r]);
     d.rcell[r] = c;
-    atomicAdd(&d.rcount[c], 1);
+    d.rrank[r] = atomicAdd(&d.rcount[c], 1);
   }
   for (int off = 32; off > 0; off >>= 1) hb = max(hb, (unsigned)__shfl_xor((int)hb, off, 64));
   if ((threadIdx.x & 63) == 0) hw[threadIdx.x >> 6] = hb;
@@ -210,110 +212,145 @@ __device__ inline int block_scan_excl(int v, int *sh, int *tot) {
   return r;
 }
 
-// exclusive scan of cnt[0..ncells) into start[0..ncells] by one block of
-// 1024 threads, in rounds of K * 1024 cells: a round is loaded coalesced
-// (the next round's counts prefetched into registers) into LDS, each thread
-// scans K consecutive cells there (the buffer padded by one word per 16 so
-// the per-thread runs fall in distinct banks), the threads' sums are
-// block-scanned, and the offsets leave LDS coalesced.
-constexpr int kScanK = 16;
-constexpr int kScanRound = kScanK * 1024;
-__device__ inline int scan_pad(int p) { return p + (p >> 4); }
-__device__ inline void scan_cells(const int *cnt, int *start, int ncells, int *sh, int *buf) {
-  const int tid = threadIdx.x;
-  int nv[kScanK];
+// Cell starts of the counting sort in two levels: each block scans one
+// chunk of kScanChunk cells (four per thread, one block scan) into in-chunk
+// offsets (loc) and the chunk's total (tot); the block that finishes last (a
+// ticket counter, reset by it) scans the chunk totals into chunk offsets
+// (off) and the grand total (start[ncells]).  A cell starts at loc[c] +
+// off[c / kScanChunk]; k_cell_sort writes that into start[c].
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 4;
+static_assert(kScanChunk == kScanThreads * kScanPer, "a chunk is one block's four cells per thread");
+__host__ __device__ inline int scan_chunks(long long ncells) { return (int)((ncells + kScanChunk - 1) / kScanChunk); }
+
+__device__ inline void scan_chunk(const int *cnt, int *loc, int *tot, int ncells, int b, int *sh) {
+  const int c0 = b * kScanChunk + threadIdx.x * kScanPer;
+  int v[kScanPer], sum = 0;
 #pragma unroll
-  for (int k = 0; k < kScanK; ++k) {
-    const int c = k * 1024 + tid;
-    nv[k] = c < ncells ? cnt[c] : 0;
+  for (int k = 0; k < kScanPer; ++k) {
+    v[k] = c0 + k < ncells ? cnt[c0 + k] : 0;
+    sum += v[k];
   }
-  int carry = 0;
-  for (int base = 0; base < ncells; base += kScanRound) {
+  int t;
+  int off = block_scan_excl(sum, sh, &t);
 #pragma unroll
-    for (int k = 0; k < kScanK; ++k) buf[scan_pad(k * 1024 + tid)] = nv[k];
-#pragma unroll
-    for (int k = 0; k < kScanK; ++k) {
-      const int c = base + kScanRound + k * 1024 + tid;
-      nv[k] = c < ncells ? cnt[c] : 0;
-    }
-    __syncthreads();
-    int v[kScanK], sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanK; ++k) {
-      v[k] = buf[scan_pad(tid * kScanK + k)];
-      sum += v[k];
-    }
-    int tot;
-    int off = block_scan_excl(sum, sh, &tot) + carry;
-#pragma unroll
-    for (int k = 0; k < kScanK; ++k) {
-      buf[scan_pad(tid * kScanK + k)] = off;
-      off += v[k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kScanK; ++k) {
-      const int c = base + k * 1024 + tid;
-      if (c < ncells) start[c] = buf[scan_pad(k * 1024 + tid)];
-    }
-    carry += tot;
-    __syncthreads();
+  for (int k = 0; k < kScanPer; ++k) {
+    if (c0 + k < ncells) loc[c0 + k] = off;
+    off += v[k];
   }
-  if (tid == 0) start[ncells] = carry;
+  if (threadIdx.x == 0) tot[b] = t;
 }
 
-__global__ __launch_bounds__(1024) void k_scan(Dev d) {
+// exclusive scan of the chunk totals written by the other blocks of the
+// launch (read past the L1 cache) into off[0..nch), the sum into *end
+__device__ inline void scan_totals(const int *tot, int *off, int nch, int *end, int *sh) {
+  int carry = 0;
+  for (int base = 0; base < nch; base += kScanThreads) {
+    const int q = base + threadIdx.x;
+    const int v = q < nch ? __hip_atomic_load(&tot[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+    int t;
+    const int e = block_scan_excl(v, sh, &t);
+    if (q < nch) off[q] = carry + e;
+    carry += t;
+  }
+  if (threadIdx.x == 0) *end = carry;
+}
+
+// blocks [0, ns): chunks of the search grid's counts; [ns, ns + nr): chunks
+// of the rigid particles' counts (reference grid).  Each block also reduces
+// a share of k_cell_hist's per-block largest h; the last block combines the
+// shares into the search's largest h.
+__global__ __launch_bounds__(kScanThreads) void k_scan(Dev d) {
   if (!d.flags->do_search) return;
   __shared__ int sh[34];
-  __shared__ int buf[kScanRound + kScanRound / 16];
-  // the largest SPH h of this search from k_cell_hist's block maxima
+  __shared__ int last;
+  const int ns = scan_chunks(d.sncells), nr = d.nrig ? scan_chunks(d.ncells) : 0, nb = ns + nr;
+  const int b = blockIdx.x;
+  if (b < ns) scan_chunk(d.cell_count, d.cell_loc, d.ctot, d.sncells, b, sh);
+  else scan_chunk(d.rcount, d.rloc, d.rtot, d.ncells, b - ns, sh);
+  const int per = (d.nblocks + nb - 1) / nb;
   unsigned hb = 0;
-  for (int b = threadIdx.x; b < d.nblocks; b += 1024) hb = max(hb, d.hblk[b]);
-  for (int off = 32; off > 0; off >>= 1) hb = max(hb, (unsigned)__shfl_xor((int)hb, off, 64));
+  for (int q = b * per + threadIdx.x; q < min(d.nblocks, (b + 1) * per); q += kScanThreads) hb = max(hb, d.hblk[q]);
+  for (int o = 32; o > 0; o >>= 1) hb = max(hb, (unsigned)__shfl_xor((int)hb, o, 64));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = (int)hb;
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned m = 0;
-    for (int w = 0; w < 16; ++w) m = max(m, (unsigned)sh[w]);
-    d.flags->hmax_bits = m;
+    for (int w = 0; w < kScanThreads / 64; ++w) m = max(m, (unsigned)sh[w]);
+    d.hred[b] = m;
+    __threadfence();
+    last = atomicAdd(&d.flags->scan_ticket, 1u) == (unsigned)(nb - 1);
   }
   __syncthreads();
-  scan_cells(d.cell_count, d.cell_start, d.sncells, sh, buf);
-  if (d.nrig) scan_cells(d.rcount, d.rstart, d.ncells, sh, buf);
+  if (!last) return;
+  __threadfence();
+  scan_totals(d.ctot, d.coff, ns, d.cell_start + d.sncells, sh);
+  if (nr) scan_totals(d.rtot, d.roff, nr, d.rstart + d.ncells, sh);
+  hb = 0;
+  for (int q = threadIdx.x; q < nb; q += kScanThreads)
+    hb = max(hb, __hip_atomic_load(&d.hred[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  for (int o = 32; o > 0; o >>= 1) hb = max(hb, (unsigned)__shfl_xor((int)hb, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = (int)hb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned m = 0;
+    for (int w = 0; w < kScanThreads / 64; ++w) m = max(m, (unsigned)sh[w]);
+    d.flags->hmax_bits = m;
+    d.flags->scan_ticket = 0;
+  }
 }
 
+// each particle to its place: its cell's start plus its in-cell rank
 WF_KERNEL void k_scatter(Dev d) {
   if (!d.flags->do_search) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < d.n) {
     const int c = d.scell[i];
-    d.sorted[d.cell_start[c] + atomicAdd(&d.cell_cursor[c], 1)] = i;
+    d.sorted[d.cell_loc[c] + d.coff[c / kScanChunk] + d.srank[i]] = i;
   } else if (i - d.n < d.nrig) {
     const int r = i - d.n, c = d.rcell[r];
-    d.rsorted[d.rstart[c] + atomicAdd(&d.rcursor[c], 1)] = r;
+    d.rsorted[d.rloc[c] + d.roff[c / kScanChunk] + d.rrank[r]] = r;
   }
 }
 
-// per cell: clear the counters for the next search, order the cell's
-// particles by z-slice, then by index (cells hold ~10-30 particles; the
-// order, and with it every neighbour row, becomes independent of the atomic
-// order), record where slices 1-3 start, and write the candidate records
-// (FP32 offsets from the cell's lower corner, h, index, group) in cell order
-WF_KERNEL void k_cell_sort(Dev d, int cur) {
-  if (!d.flags->do_search) return;
-  const int c = blockIdx.x * kBlock + threadIdx.x;
-  if (d.nrig && c < d.ncells) {
-    d.rcount[c] = 0;
-    d.rcursor[c] = 0;
-  }
-  if (c >= d.sncells) return;
-  d.cell_count[c] = 0;
-  d.cell_cursor[c] = 0;
-  const int s = d.cell_start[c], e = d.cell_start[c + 1];
+// one cell of k_cell_sort, by the 64 lanes of a wavefront (lane = its lane)
+__device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
+  const int s = d.cell_loc[c] + d.coff[c / kScanChunk];
+  const int e = c + 1 < d.sncells ? d.cell_loc[c + 1] + d.coff[(c + 1) / kScanChunk] : d.cell_start[d.sncells];
+  const int cnt = e - s;
   const int nz = d.scellno[2], ny = d.scellno[1];
   const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
   const double ox = d.blpf[0] + cx * d.scellsize[0], oy = d.blpf[1] + cy * d.scellsize[1],
                oz = d.blpf[2] + cz * d.scellsize[2];
+  if (lane == 0) {
+    d.cell_start[c] = s;
+    d.cell_count[c] = 0;
+  }
+  if (cnt <= 64) {
+    int j = 0, sl = 0;
+    if (lane < cnt) {
+      j = d.sorted[s + lane];
+      sl = d.sslice[j];
+    }
+    const long long key = lane < cnt ? ((long long)sl << 32) | (long long)j : 0x7fffffffffffffffLL;
+    int rank = 0;
+    for (int l = 0; l < cnt; ++l) rank += __shfl(key, l, 64) < key ? 1 : 0;
+    const unsigned long long b0 = __ballot(lane < cnt && sl == 0);
+    const unsigned long long b1 = __ballot(lane < cnt && sl <= 1);
+    const unsigned long long b2 = __ballot(lane < cnt && sl <= 2);
+    if (lane < cnt) {
+      const int p = s + rank;
+      const double4 x = d.xh[cur][j];
+      d.sorted[p] = j;
+      d.sxyz[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
+      d.sinfo[p] = make_int2(j, d.group[j]);
+    }
+    if (lane == 0)
+      d.soff[c] = make_int4(s + __popcll(b0), s + __popcll(b1), s + __popcll(b2), e);
+    return;
+  }
+  if (lane != 0) return;
   // key (slice, index) of each entry, held in sinfo while sorting
   for (int p = s; p < e; ++p) {
     const int j = d.sorted[p];
@@ -342,6 +379,28 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
     d.sinfo[p] = make_int2(j, d.group[j]);
   }
   d.soff[c] = make_int4(st[0], st[1], st[2], e);
+}
+
+// per cell: write the cell's start, clear the counters for the next search,
+// order the cell's particles by z-slice, then by index (cells hold ~10-30
+// particles; the order, and with it every neighbour row, becomes
+// independent of the atomic order), record where slices 1-3 start, and write
+// the candidate records (FP32 offsets from the cell's lower corner, h,
+// index, group) in cell order.  One wavefront per cell: each lane holds one
+// particle and counts the lanes with smaller keys (its rank); a cell of more
+// than 64 particles is sorted by its first lane alone.  The wavefronts, and
+// the threads writing the rigid grid's starts, stride over the cells (a
+// modest grid, so a step without a search exits early at little cost).
+WF_KERNEL void k_cell_sort(Dev d, int cur) {
+  if (!d.flags->do_search) return;
+  if (d.nrig)
+    for (int c = blockIdx.x * kBlock + threadIdx.x; c < d.ncells; c += gridDim.x * kBlock) {
+      d.rstart[c] = d.rloc[c] + d.roff[c / kScanChunk];
+      d.rcount[c] = 0;
+    }
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (kBlock / 64);
+  for (int c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); c < d.sncells; c += nw) sort_cell(d, cur, c, lane);
 }
 
 // One lane per particle (lanes in cell order) over the 27 cells around the
@@ -751,9 +810,15 @@ void launch_schedule(const Dev &d, hipStream_t st, int force) {
 
 void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
   k_cell_hist<<<nblk((long long)d.n + d.nrig), kBlock, 0, st>>>(d, cur);
-  k_scan<<<1, 1024, 0, st>>>(d);
+  k_scan<<<scan_chunks(d.sncells) + (d.nrig ? scan_chunks(d.ncells) : 0), kScanThreads, 0, st>>>(d);
   k_scatter<<<nblk((long long)d.n + d.nrig), kBlock, 0, st>>>(d);
-  k_cell_sort<<<nblk(d.sncells > d.ncells ? d.sncells : d.ncells), kBlock, 0, st>>>(d, cur);
+  // (at most 8192 blocks: an idle launch of that size costs about as much
+  // as the other search kernels', and 16k wavefronts keep the cells' loads
+  // in flight)
+  {
+    const int g = nblk((long long)d.sncells * 64);
+    k_cell_sort<<<g < 8192 ? g : 8192, kBlock, 0, st>>>(d, cur);
+  }
   k_build_nbr<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
   if (d.stage_max > 0) k_stage<<<d.ntiles < 2048 ? d.ntiles : 2048, kStageThreads, 0, st>>>(d);
 }
