@@ -133,7 +133,7 @@ def yield_run(steps, chunk=50):
     s = cases.make_solver(case, precision=capi.MIXED)
     s.prepare()
     s.sync()
-    walls, searches, yielded_at = [], [], None
+    walls, searches, yielded_at, staging = [], [], None, []
     done = 0
     while done < steps:
         k = min(chunk, steps - done)
@@ -146,6 +146,11 @@ def yield_run(steps, chunk=50):
         if yielded_at is None and sc["max_pl_strain"] > 1e-6:
             yielded_at = done + k
         done += k
+        if done == k or done == steps:
+            # tiles whose staged set still fits LDS after this chunk's last build
+            tiles, staged, values = s.stage_stats()
+            staging.append({"after_step": done, "staged_tile_fraction": staged / tiles,
+                            "mean_staged_set": values / max(staged, 1)})
     per = [searches[0]] + [b - a for a, b in zip(searches[:-1], searches[1:])]  # searches in each chunk
     pre = [w / k for (k, w), m in list(zip(walls, per))[1:] if m == 0]
     post = [w / k for (k, w), m in zip(walls, per) if m >= k // 5]
@@ -155,7 +160,8 @@ def yield_run(steps, chunk=50):
            "steps": steps, "nb_searches": int(searches[-1]), "yielded_by_step": yielded_at,
            "ms_per_step_chunks": [1e3 * w / k for k, w in walls], "searches_per_chunk": [int(m) for m in per],
            "ms_per_step_before_yield": 1e3 * float(np.median(pre)) if pre else None,
-           "ms_per_step_yielding": 1e3 * float(np.median(post)) if post else None}
+           "ms_per_step_yielding": 1e3 * float(np.median(post)) if post else None,
+           "lds_staging": staging}
     log("bench: yielding " + json.dumps(res))
     return res
 
@@ -326,12 +332,33 @@ def batch_run(steps, n_sample=16):
     s = cases.make_solver(case, precision=capi.MIXED, **best)
     s.prepare()
     s.sync()
+    # timed in chunks of 50 steps: the chunks without a search (before the
+    # specimens yield) against those with a rebuild every 5 steps
+    walls, searches, done = [], [], 0
     t0 = time.perf_counter()
-    s.step(steps)
-    s.sync()
+    while done < steps:
+        k = min(50, steps - done)
+        t1 = time.perf_counter()
+        s.step(k)
+        s.sync()
+        walls.append((k, time.perf_counter() - t1))
+        searches.append(s.scalars()["nb_searches"])
+        done += k
     wall = time.perf_counter() - t0
+    per = [searches[0]] + [b - a for a, b in zip(searches[:-1], searches[1:])]
+    pre = [w / k for (k, w), m in list(zip(walls, per))[1:] if m == 0]
+    post = [w / k for (k, w), m in zip(walls, per) if m >= k // 5]
     top = np.array([s.group_history(1, i, 0)[:, 2] for i in range(nspec)])  # (specimen, step)
     yielded = float((s.get("pl_strain") > 0).mean())
+    # 20 more (yielding) steps launched one by one with HIP events around
+    # each part of the step: where a yielding batch step goes
+    nb0 = s.scalars()["nb_searches"]
+    s.set_timing(True)
+    s.step(20)
+    s.sync()
+    kt = {k: s.kernel_time(k)[0] for k in ("accel", "density", "rates", "neighbours", "surface", "step")}
+    kt["searches"] = int(s.scalars()["nb_searches"] - nb0)
+    s.set_timing(False)
     s.close()
     spec_n = case.meta["spec_n"]
     order = np.argsort(spec_n)
@@ -359,6 +386,10 @@ def batch_run(steps, n_sample=16):
                        f"{steps} mixed Leapfrog steps, one context",
            "particles": case.n, "specimens": nspec, "steps": steps, "ms_per_step": ms,
            "particle_steps_per_s": case.n * steps / wall, "yielded_fraction": yielded,
+           "ms_per_step_chunks": [1e3 * w / k for k, w in walls], "searches_per_chunk": [int(m) for m in per],
+           "ms_per_step_before_yield": 1e3 * float(np.median(pre)) if pre else None,
+           "ms_per_step_yielding": 1e3 * float(np.median(post)) if post else None,
+           "kernel_ms_yielding_eager_20_steps": kt,
            "autotune": {"best": best, "table": table},
            "per_specimen_share_of_batch_step_ms": {"p50": float(np.percentile(share, 50)),
                                                    "p99": float(np.percentile(share, 99))},

@@ -417,10 +417,19 @@ int wf_calc_force2233(void *ctx, double xsph, int32_t shepard);
 
 /* kernel timing with HIP events on the context's stream.
  * kernel: 0 accel, 1 density+drift, 2 rates+stress, 3 neighbour build,
- *         4 whole step.  avg_ms is the mean duration per launch. */
+ *         4 whole step, 5 after-search set-up (kernel-gradient correction,
+ *         free surface and contact partners).  avg_ms is the mean duration
+ *         per launch. */
 int wf_set_timing(void *ctx, int32_t enable);
 int wf_get_kernel_time(void *ctx, int32_t kernel, double *avg_ms, int64_t *launches);
 int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_count);
+/* LDS staging of the mixed-precision pair passes after the last neighbour
+ * build (no counterpart in the reference: a diagnostic of this library's
+ * tiles of consecutive particles): the number of tiles, how many of them
+ * gather their neighbours from LDS (their staged set fits; the others gather
+ * from HBM) and the total size of the staged sets.  FP64 contexts report no
+ * staged tiles. */
+int wf_get_stage_stats(void *ctx, int64_t *tiles, int64_t *staged_tiles, int64_t *staged_values);
 /* cells per axis of the reference grid (CellInitiate, Source/Neighbour.cpp:
  * 10-139: sized by h_max over all particles, rigid ones included; the
  * contact partners and the thermal pair orientation use it) and of the grid

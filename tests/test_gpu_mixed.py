@@ -106,3 +106,14 @@ def test_lds_staging_is_transparent(stage_max):
         assert np.array_equal(got.get(f), ref.get(f)), f
     assert np.array_equal(got.reaction_history(c.top_zone), ref.reaction_history(c.top_zone))
     assert got.scalars()["kin_energy_sum"] == ref.scalars()["kin_energy_sum"]
+    # which tiles the last build staged: none with staging off; all with the
+    # default bound (every set of this small billet fits); with 520 about
+    # half, each staged set within the bound
+    tiles, staged, values = got.stage_stats()
+    assert tiles == -(-c.n // 128)
+    assert ref.stage_stats()[1] == 0
+    if stage_max == 0:
+        assert staged == tiles
+    else:
+        assert 0 < staged < tiles
+        assert values <= 520 * staged

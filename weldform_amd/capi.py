@@ -34,7 +34,7 @@ FIELDS = {
 RIGID_FIELDS = {"x": 0, "v": 1, "normal": 2}
 INT_FIELDS = {"id": 0, "material": 1, "free": 2, "thermal_bc": 3, "nb": 4, "contnb": 5}
 EVAL = {"accel": 0, "density": 1, "rates": 2, "stress": 3}
-KERNELS = {"accel": 0, "density": 1, "rates": 2, "neighbours": 3, "step": 4}
+KERNELS = {"accel": 0, "density": 1, "rates": 2, "neighbours": 3, "step": 4, "surface": 5}
 
 
 class WFError(RuntimeError):
@@ -92,7 +92,7 @@ EXPORTS = [
     "wf_get_int_field", "wf_add_amplitude", "wf_add_velocity_bc",
     "wf_add_temperature_bc", "wf_init", "wf_step", "wf_sync", "wf_get_scalars",
     "wf_get_reaction_history", "wf_rebuild_neighbours", "wf_get_neighbour_pairs",
-    "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats",
+    "wf_eval", "wf_set_timing", "wf_get_kernel_time", "wf_get_neighbour_stats", "wf_get_stage_stats",
     "wf_gen_cylinder", "wf_add_rigid_plane", "wf_set_contact", "wf_get_contact_force_history",
     "wf_get_rigid_field", "wf_adj_forward", "wf_adj_backward", "wf_adj_get_field", "wf_adj_restore",
     "wf_pack_xdmf", "wf_pack_csv", "wf_add_rigid_mesh", "wf_set_bc_angular_velocity",
@@ -152,6 +152,7 @@ def lib():
         "wf_set_timing": ([vp, i32], ctypes.c_int),
         "wf_get_kernel_time": ([vp, i32, dp, I64P], ctypes.c_int),
         "wf_get_neighbour_stats": ([vp, I64P, ip], ctypes.c_int),
+        "wf_get_stage_stats": ([vp, I64P, I64P, I64P], ctypes.c_int),
         "wf_get_cell_grids": ([vp, ip, ip], ctypes.c_int),
         "wf_set_material_friction": ([vp, i32, d, d], ctypes.c_int),
         "wf_set_contact_law": ([vp, i32, d, i32], ctypes.c_int),
@@ -471,6 +472,13 @@ class Solver:
         g = np.zeros(3, dtype=np.int32)
         self._check(lib().wf_get_cell_grids(self.ctx, _ip(r), _ip(g)))
         return r, g
+
+    def stage_stats(self):
+        """(tiles, tiles staged in LDS, total size of the staged sets) after
+        the last neighbour build; FP64 contexts stage nothing."""
+        t, st, sv = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        self._check(lib().wf_get_stage_stats(self.ctx, ctypes.byref(t), ctypes.byref(st), ctypes.byref(sv)))
+        return t.value, st.value, sv.value
 
     def neighbour_stats(self):
         p = ctypes.c_int64(0)

@@ -56,7 +56,7 @@ struct AmpHost {
   }
 };
 
-enum { T_ACCEL = 0, T_DENS, T_RATES, T_NBR, T_STEP, T_COUNT };
+enum { T_ACCEL = 0, T_DENS, T_RATES, T_NBR, T_STEP, T_SURF, T_COUNT };
 
 // rigid plane mesh (TriMesh::AxisPlaneMesh + CalcSpheres, Source/Mesh.cpp:82-220)
 struct MeshHost {
@@ -443,6 +443,7 @@ void allocate(Ctx *c) {
     d.vol[b] = c->alloc<double>(np);
   }
   d.vm = c->alloc<double4>(np);
+  d.pmass = c->alloc<double>(np);
   d.acc = c->alloc<double4>(np);
   d.disp = c->alloc<double4>(np);
   d.srec = c->alloc<StressRec>(np);
@@ -645,6 +646,11 @@ void upload_all(Ctx *c) {
   dput(c, d.xh[0], xh);
   dput(c, d.xh[1], xh);
   dput(c, d.vm, vm);
+  {
+    std::vector<double> pm(vm.size());
+    for (size_t k = 0; k < vm.size(); ++k) pm[k] = vm[k].w;
+    dput(c, d.pmass, pm);
+  }
   if (field(WF_F_A)) put_d4(c, d.acc, field(WF_F_A), 0, 3, 3);
   if (field(WF_F_DISP)) put_d4(c, d.disp, field(WF_F_DISP), 0, 3, 3);
   // density, volumes, stress records, pressure (InitialChecks,
@@ -959,6 +965,9 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int
     {
       Timed t(c, T_NBR);
       launch_neighbours(d, c->st, cur);
+    }
+    {
+      Timed t(c, T_SURF);
       if (d.gc_on) launch_gradcorr(d, c->st, cur);
       if (d.contact) launch_surface(d, c->st, cur, 0);
       // with damage the free surface is refreshed every step
@@ -1328,6 +1337,11 @@ int wf_set_field(void *ctx, int32_t field, const double *h) {
         // mass, density and stress feed vol = m/rho and the stress record
         if (field == WF_F_MASS) put_d4(c, d.vm, h, 3, 1, 1);
         auto vm = dget(c, d.vm, (size_t)c->npad);
+        if (field == WF_F_MASS) {
+          std::vector<double> pm(vm.size());
+          for (size_t k = 0; k < vm.size(); ++k) pm[k] = vm[k].w;
+          dput(c, d.pmass, pm);
+        }
         std::vector<double> rho((size_t)c->npad);
         if (field == WF_F_DENSITY)
           for (int64_t k = 0; k < c->n; ++k) rho[k] = h[c->perm[k]];
@@ -2480,6 +2494,27 @@ int wf_get_neighbour_stats(void *ctx, int64_t *directed_pairs, int32_t *max_coun
     auto f = dget(c, c->d.flags, 1)[0];
     if (directed_pairs) *directed_pairs = f.directed_pairs;
     if (max_count) *max_count = f.max_count;
+    return WF_OK;
+  });
+}
+
+int wf_get_stage_stats(void *ctx, int64_t *tiles, int64_t *staged_tiles, int64_t *staged_values) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (!c->inited) throw Error(WF_ESTATE, "wf_init not called");
+    int64_t t = c->d.ntiles, st = 0, sv = 0;
+    if (c->d.mixed && c->d.stage_cnt) {
+      const auto cnt = dget(c, c->d.stage_cnt, (size_t)c->d.ntiles);
+      for (int v : cnt)
+        if (v >= 0) {
+          ++st;
+          sv += v;
+        }
+    }
+    if (tiles) *tiles = t;
+    if (staged_tiles) *staged_tiles = st;
+    if (staged_values) *staged_values = sv;
     return WF_OK;
   });
 }

@@ -286,6 +286,7 @@ struct Dev {
   double *rT;
   double *qfric, *fhfl, *cshear, *qcont;
   int *id_orig, *ccnt, *cidx;  // contact partners [kMaxContact][npad]
+  double *pmass;               // particle mass (vm.w), for the free-surface sums
   double4 *cforce;
   double *cf_hist;             // [hist_cap][kMaxMesh][3]
   // sums per particle group (material index, wf_set_group_sums): each

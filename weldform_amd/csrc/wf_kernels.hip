@@ -314,19 +314,32 @@ WF_KERNEL void k_scatter(Dev d) {
   }
 }
 
-// one cell of k_cell_sort, by the 64 lanes of a wavefront (lane = its lane)
-__device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
-  const int s = d.cell_loc[c] + d.coff[c / kScanChunk];
-  const int e = c + 1 < d.sncells ? d.cell_loc[c + 1] + d.coff[(c + 1) / kScanChunk] : d.cell_start[d.sncells];
-  const int cnt = e - s;
+// where search cell c's records start (its in-chunk offset plus its chunk's)
+// and end
+__device__ inline int cell_first(const Dev &d, int c) { return d.cell_loc[c] + d.coff[c / kScanChunk]; }
+__device__ inline int cell_end(const Dev &d, int c) {
+  return c + 1 < d.sncells ? cell_first(d, c + 1) : d.cell_start[d.sncells];
+}
+// lower corner of search cell c
+__device__ inline double3 cell_corner(const Dev &d, int c) {
   const int nz = d.scellno[2], ny = d.scellno[1];
   const int cz = c % nz, cy = (c / nz) % ny, cx = c / (nz * ny);
-  const double ox = d.blpf[0] + cx * d.scellsize[0], oy = d.blpf[1] + cy * d.scellsize[1],
-               oz = d.blpf[2] + cz * d.scellsize[2];
-  if (lane == 0) {
-    d.cell_start[c] = s;
-    d.cell_count[c] = 0;
-  }
+  return make_double3(d.blpf[0] + cx * d.scellsize[0], d.blpf[1] + cy * d.scellsize[1],
+                      d.blpf[2] + cz * d.scellsize[2]);
+}
+// the candidate record of particle j at position p, offsets from corner o
+__device__ inline void put_record(const Dev &d, int cur, int p, int j, const double3 &o) {
+  const double4 x = d.xh[cur][j];
+  d.sxyz[p] = make_float4((float)(x.x - o.x), (float)(x.y - o.y), (float)(x.z - o.z), (float)x.w);
+  d.sinfo[p] = make_int2(j, d.group[j]);
+}
+
+// one cell of two or more particles, by the 64 lanes of a wavefront (lane =
+// its lane)
+__device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
+  const int s = cell_first(d, c), e = cell_end(d, c);
+  const int cnt = e - s;
+  const double3 o = cell_corner(d, c);
   if (cnt <= 64) {
     int j = 0, sl = 0;
     if (lane < cnt) {
@@ -340,11 +353,8 @@ __device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
     const unsigned long long b1 = __ballot(lane < cnt && sl <= 1);
     const unsigned long long b2 = __ballot(lane < cnt && sl <= 2);
     if (lane < cnt) {
-      const int p = s + rank;
-      const double4 x = d.xh[cur][j];
-      d.sorted[p] = j;
-      d.sxyz[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
-      d.sinfo[p] = make_int2(j, d.group[j]);
+      d.sorted[s + rank] = j;
+      put_record(d, cur, s + rank, j, o);
     }
     if (lane == 0)
       d.soff[c] = make_int4(s + __popcll(b0), s + __popcll(b1), s + __popcll(b2), e);
@@ -372,11 +382,8 @@ __device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
     if (k.y >= 1) st[0] = p;
     if (k.y >= 2) st[1] = p;
     if (k.y >= 3) st[2] = p;
-    const int j = k.x;
-    const double4 x = d.xh[cur][j];
-    d.sorted[p] = j;
-    d.sxyz[p] = make_float4((float)(x.x - ox), (float)(x.y - oy), (float)(x.z - oz), (float)x.w);
-    d.sinfo[p] = make_int2(j, d.group[j]);
+    d.sorted[p] = k.x;
+    put_record(d, cur, p, k.x, o);
   }
   d.soff[c] = make_int4(st[0], st[1], st[2], e);
 }
@@ -386,11 +393,14 @@ __device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
 // particles; the order, and with it every neighbour row, becomes
 // independent of the atomic order), record where slices 1-3 start, and write
 // the candidate records (FP32 offsets from the cell's lower corner, h,
-// index, group) in cell order.  One wavefront per cell: each lane holds one
-// particle and counts the lanes with smaller keys (its rank); a cell of more
-// than 64 particles is sorted by its first lane alone.  The wavefronts, and
-// the threads writing the rigid grid's starts, stride over the cells (a
-// modest grid, so a step without a search exits early at little cost).
+// index, group) in cell order.  A wavefront takes cells in groups of
+// kSortGroup: one lane per cell writes the start, clears the counter and
+// finishes a cell of at most one particle; then the wavefront sorts the
+// group's larger cells one after another (sort_cell), so grids of mostly
+// empty cells cost little.  The wavefronts, and the threads writing the
+// rigid grid's starts, stride over the cells (a modest grid, so a step
+// without a search exits early at little cost).
+constexpr int kSortGroup = 16;
 WF_KERNEL void k_cell_sort(Dev d, int cur) {
   if (!d.flags->do_search) return;
   if (d.nrig)
@@ -400,7 +410,33 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
     }
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * (kBlock / 64);
-  for (int c = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); c < d.sncells; c += nw) sort_cell(d, cur, c, lane);
+  for (int base = (blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * kSortGroup; base < d.sncells;
+       base += nw * kSortGroup) {
+    const int c = base + lane;
+    const bool mine = lane < kSortGroup && c < d.sncells;
+    int cnt = 0;
+    if (mine) {
+      const int s = cell_first(d, c), e = cell_end(d, c);
+      cnt = e - s;
+      d.cell_start[c] = s;
+      d.cell_count[c] = 0;
+      if (cnt <= 1) {
+        int4 so = make_int4(s, s, s, e);
+        if (cnt == 1) {
+          const int j = d.sorted[s], sl = d.sslice[j];
+          put_record(d, cur, s, j, cell_corner(d, c));
+          so = make_int4(s + (sl <= 0 ? 1 : 0), s + (sl <= 1 ? 1 : 0), s + (sl <= 2 ? 1 : 0), e);
+        }
+        d.soff[c] = so;
+      }
+    }
+    unsigned long long big = __ballot(mine && cnt >= 2);
+    while (big) {
+      const int l = __ffsll((long long)big) - 1;
+      big &= big - 1;
+      sort_cell(d, cur, base + l, lane);
+    }
+  }
 }
 
 // One lane per particle (lanes in cell order) over the 27 cells around the
@@ -816,7 +852,7 @@ void launch_neighbours(const Dev &d, hipStream_t st, int cur) {
   // as the other search kernels', and 16k wavefronts keep the cells' loads
   // in flight)
   {
-    const int g = nblk((long long)d.sncells * 64);
+    const int g = nblk((long long)(d.sncells + kSortGroup - 1) / kSortGroup * 64);
     k_cell_sort<<<g < 8192 ? g : 8192, kBlock, 0, st>>>(d, cur);
   }
   k_build_nbr<<<nblk(d.n), kBlock, 0, st>>>(d, cur);
@@ -2370,13 +2406,26 @@ WF_KERNEL void k_surface(Dev d, int cur, int setup) {
     const double4 xi = d.xh[cur][i];
     double nx = 0.0, ny = 0.0, nz = 0.0;
     const int cnt = d.nnb[i];
-    for (int k = 0; k < cnt; ++k) {
-      const int j = d.nbr[nb_slot(np, k, i)];
-      const double4 xj = d.xh[cur][j];
-      const double mj = d.vm[j].w;
-      nx += mj * (xi.x - xj.x);
-      ny += mj * (xi.y - xj.y);
-      nz += mj * (xi.z - xj.z);
+    // the row's groups of four entries, their gathers issued together; the
+    // sum runs over the entries in row order
+    const int4 *row = reinterpret_cast<const int4 *>(d.nbr) + i;
+    for (int g = 0; 4 * g < cnt; ++g) {
+      const int4 e = row[(long long)g * np];
+      const int jj[4] = {e.x, e.y, e.z, e.w};
+      double4 xj[4];
+      double mj[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        xj[u] = d.xh[cur][jj[u]];
+        mj[u] = d.pmass[jj[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (4 * g + u >= cnt) break;
+        nx += mj[u] * (xi.x - xj[u].x);
+        ny += mj[u] * (xi.y - xj[u].y);
+        nz += mj[u] * (xi.z - xj[u].z);
+      }
     }
     const double f = 1. / d.totmass;
     nx *= f;
