@@ -308,6 +308,19 @@ int wf_set_material_friction(void *ctx, int32_t index, double friction_sta, doub
  * particle (Source/Contact_Wang.cpp:241-244, Source/Contact_LS.cpp:217-220);
  * linear 0 restores the constant coefficients.  Before wf_init. */
 int wf_set_contact_law(void *ctx, int32_t law, double pfac, int32_t fric_bound);
+/* wf_set_dimension <- Domain::Dimension (Source/Domain.h:331): 3, or 2 for
+ * plane strain, every particle in one z-plane.  In 2D the cubic spline
+ * takes its 2D constants (Source/Functions.cpp:32,81), StartAcceleration
+ * zeroes each z velocity at the top of the step (Source/Domain.cpp:1592),
+ * the pair forces lose their z part (Source/InteractionAlt.cpp:184,
+ * Source/Interaction.cpp:237), the cell grid has one z layer
+ * (Source/Neighbour.cpp:56-75), the free surface takes at most 12
+ * neighbours (Source/Domain.cpp:1474), the kernel-gradient correction sets
+ * the sum's zz entry to 1 (Source/Domain.cpp:1911-1915) and convection uses
+ * (m/rho)^(1/3) (Source/Thermal.cpp:354-366).  Rigid contact, the
+ * tensile-instability term, the adjoint and the axisymmetric branches are
+ * refused in 2D (WF_EUNSUPPORTED).  Before wf_init. */
+int wf_set_dimension(void *ctx, int32_t dim);
 /* wf_set_material_damage <- Material_::damage = new JohnsonCookDamage(D1,
  * D2, D3, D4, D5, eps_0) and Domain::model_damage (Source/Material.h:47-62,
  * scene keys "damageModel": "JohnsonCook" and "damageParams",

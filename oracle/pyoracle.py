@@ -95,6 +95,10 @@ def lib():
         L.wfo_kernel.argtypes = [d, d]
         L.wfo_grad_kernel.restype = d
         L.wfo_grad_kernel.argtypes = [d, d]
+        L.wfo_kernel_dim.restype = d
+        L.wfo_kernel_dim.argtypes = [d, d, ctypes.c_int]
+        L.wfo_grad_kernel_dim.restype = d
+        L.wfo_grad_kernel_dim.argtypes = [d, d, ctypes.c_int]
         L.wfo_sym3_eig.restype = None
         L.wfo_sym3_eig.argtypes = [dp, dp, dp]
         _lib = L
@@ -109,12 +113,12 @@ def set_threads(n):
     lib().wfo_set_threads(int(n))
 
 
-def kernel(q, h):
-    return lib().wfo_kernel(float(q), float(h))
+def kernel(q, h, dim=3):
+    return lib().wfo_kernel_dim(float(q), float(h), int(dim))
 
 
-def grad_kernel(q, h):
-    return lib().wfo_grad_kernel(float(q), float(h))
+def grad_kernel(q, h, dim=3):
+    return lib().wfo_grad_kernel_dim(float(q), float(h), int(dim))
 
 
 def sym3_eig(A):

@@ -73,6 +73,9 @@ struct wfo_domain {
   int model_damage;
   int dam_model[256];
   double dam_par[256][5];
+  /* Domain::Dimension: 3, or 2 for plane strain (all particles in one
+   * z-plane; Source/Domain.h:331) */
+  int dim;
   /* domain flags / parameters (Source/Domain.cpp:78-175) */
   double Cellfac, DomMax[3], DomMin[3], Time, deltat, pl_work_heat_frac;
   int ts_nb_inc, thermal_solver, pl_heating;
@@ -160,6 +163,7 @@ wfo_domain *wfo_create(int n) {
   d->PFAC = 0.5;       /* the JSON driver's penaltyFactor default (Source/WeldForm.cpp:685) */
   /* Domain constructor defaults (Source/Domain.cpp:78-175) */
   d->Cellfac = 2.0;
+  d->dim = 3;
   for (int k = 0; k < 3; ++k) { d->DomMax[k] = -100000000000.0; d->DomMin[k] = 100000000000.0; }
   d->ts_nb_inc = 5;
   d->pl_work_heat_frac = 0.9;
@@ -231,6 +235,10 @@ int wfo_set_material_damage(wfo_domain *d, int idx, int model, const double D[5]
 
 int wfo_set_param(wfo_domain *d, const char *s, double v) {
   if (!strcmp(s, "Cellfac")) d->Cellfac = v;
+  else if (!strcmp(s, "Dimension")) {
+    if (v != 2.0 && v != 3.0) return fail("Dimension must be 2 or 3");
+    d->dim = (int)v;
+  }
   else if (!strcmp(s, "ts_nb_inc")) d->ts_nb_inc = (int)v;
   else if (!strcmp(s, "thermal_solver")) d->thermal_solver = (int)v;
   else if (!strcmp(s, "pl_heating")) d->pl_heating = (int)v;
@@ -282,6 +290,7 @@ double wfo_get_param(const wfo_domain *d, const char *s) {
   if (!strcmp(s, "steps")) return (double)d->steps;
   if (!strcmp(s, "nb_searches")) return (double)d->nb_searches;
   if (!strcmp(s, "hmax")) return d->hmax;
+  if (!strcmp(s, "Dimension")) return (double)d->dim;
   if (!strcmp(s, "m_maxT")) return d->m_maxT;
   if (!strcmp(s, "m_minT")) return d->m_minT;
   if (!strcmp(s, "n_other_pairs")) return (double)d->n_other_pairs;
@@ -295,20 +304,24 @@ double wfo_get_param(const wfo_domain *d, const char *s) {
 }
 
 /* ------------------------------------------------------------------ */
-/* Cubic-spline kernel, 3D (Source/Functions.cpp:25-37, 74-87)         */
-double wfo_kernel(double q, double h) {
-  double C = 1.0 / (h * h * h * M_PI);
+/* Cubic-spline kernel (Source/Functions.cpp:25-37, 74-87): C = 1/(pi h^3)
+ * for W and 1/(pi h^4) for the gradient in 3D, 10/(7 pi h^2) and
+ * 10/(7 pi h^3) in 2D */
+double wfo_kernel_dim(double q, double h, int dim) {
+  double C = dim == 2 ? 10.0 / (7.0 * h * h * M_PI) : 1.0 / (h * h * h * M_PI);
   if (q < 1.0) return C * (1.0 - (3.0 / 2.0) * q * q + (3.0 / 4.0) * q * q * q);
   else if (q < 2.0) return C * ((1.0 / 4.0) * (2.0 - q) * (2.0 - q) * (2.0 - q));
   return 0.0;
 }
-double wfo_grad_kernel(double q, double h) {
-  double C = 1.0 / (h * h * h * h * M_PI);
+double wfo_grad_kernel_dim(double q, double h, int dim) {
+  double C = dim == 2 ? 10.0 / (7.0 * h * h * h * M_PI) : 1.0 / (h * h * h * h * M_PI);
   if (q == 0.0) return C / h * (-3.0 + (9.0 / 2.0) * q);
   else if (q < 1.0) return C / (q * h) * (-3.0 * q + (9.0 / 4.0) * q * q);
   else if (q < 2.0) return C / (q * h) * ((-3.0 / 4.0) * (2.0 - q) * (2.0 - q));
   return 0.0;
 }
+double wfo_kernel(double q, double h) { return wfo_kernel_dim(q, h, 3); }
+double wfo_grad_kernel(double q, double h) { return wfo_grad_kernel_dim(q, h, 3); }
 
 static inline double norm3(const double *a) {
   return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
@@ -495,7 +508,8 @@ static void initial_checks(wfo_domain *d) {
     d->f[WFO_P][i] = d->f[WFO_P0][i] + (d->f[WFO_CS][i] * d->f[WFO_CS][i]) * (d->f[WFO_RHO][i] - d->f[WFO_REFRHO][i]);
 }
 
-/* CellInitiate, 3D non-periodic (Source/Neighbour.cpp:10-139) */
+/* CellInitiate, non-periodic (Source/Neighbour.cpp:10-139); in 2D one
+ * layer of cells along z (CellNo[2] = 1, CellSize[2] = 0, :56-75) */
 static int cell_initiate(wfo_domain *d) {
   const double *x = d->f[WFO_X];
   if (!(norm3(d->TRPR) > 0.0) && !(norm3(d->BLPF) > 0.0)) {
@@ -514,13 +528,15 @@ static int cell_initiate(wfo_domain *d) {
     if (d->DomMin[k] < d->BLPF[k]) d->BLPF[k] = d->DomMin[k];
   }
   for (int k = 0; k < 3; k++) { d->TRPR[k] += d->hmax / 2; d->BLPF[k] -= d->hmax / 2; }
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < d->dim; k++) {
     double e = (d->TRPR[k] - d->BLPF[k]) / (d->Cellfac * d->hmax);
     if ((double)(ceil(e) - e) < (d->hmax / 10.0)) d->CellNo[k] = (int)ceil(e);
     else d->CellNo[k] = (int)floor(e);
     if (d->CellNo[k] <= 0) return fail("CellInitiate: zero cells along an axis");
   }
+  if (d->dim == 2) d->CellNo[2] = 1;
   for (int k = 0; k < 3; k++) d->CellSize[k] = (d->TRPR[k] - d->BLPF[k]) / d->CellNo[k];
+  if (d->dim == 2) d->CellSize[2] = 0.0;
   int64_t nc = (int64_t)d->CellNo[0] * d->CellNo[1] * d->CellNo[2];
   if (nc > (int64_t)1 << 30) return fail("CellInitiate: too many cells");
   free(d->HOC);
@@ -531,7 +547,9 @@ static int cell_initiate(wfo_domain *d) {
 
 #define HOC_AT(d, i, j, k) (d)->HOC[((int64_t)(i) * (d)->CellNo[1] + (j)) * (d)->CellNo[2] + (k)]
 
-/* CellReset + ListGenerate, 3D (Source/Neighbour.cpp:141-284) */
+/* CellReset + ListGenerate (Source/Neighbour.cpp:141-284); in 2D every
+ * particle is in the one z layer.  The cell pairs below are the 3D
+ * stencil, which over one layer visits the same pairs of cells. */
 static void list_generate(wfo_domain *d) {
   int64_t nc = (int64_t)d->CellNo[0] * d->CellNo[1] * d->CellNo[2];
   for (int64_t c = 0; c < nc; c++) d->HOC[c] = -1;
@@ -539,6 +557,10 @@ static void list_generate(wfo_domain *d) {
   for (int a = 0; a < d->n; a++) {
     int c[3];
     for (int k = 0; k < 3; k++) {
+      if (k == 2 && d->dim == 2) {
+        c[k] = 0;
+        continue;
+      }
       c[k] = (int)(floor((x[3 * a + k] - d->BLPF[k]) / d->CellSize[k]));
       if (c[k] < 0) c[k] = 0;
       if (c[k] >= d->CellNo[k]) c[k] = d->CellNo[k] - 1;
@@ -778,9 +800,12 @@ static void tensile_instability(wfo_domain *d, int i) {
     }
 }
 
-/* StartAcceleration with a = 0 (Source/Domain.cpp:1519-1597), 3D */
+/* StartAcceleration with a = 0 (Source/Domain.cpp:1519-1597); in 2D the
+ * velocity's z part is reset too (:1592; the 2D tensile-instability branch
+ * is not restated: TI is refused in 2D) */
 int wfo_start_acceleration(wfo_domain *d) {
   for (int i = 0; i < d->n; i++) {
+    if (d->dim == 2) d->f[WFO_V][3 * i + 2] = 0.0;
     if (d->fi[WFO_FREE][i] && d->f[WFO_TI][i] > 0.0) tensile_instability(d, i);
     if (!d->fi[WFO_FREE][i]) {
       d->f[WFO_P][i] = 0.0;
@@ -808,7 +833,7 @@ static inline void pair_geo(const wfo_domain *d, int64_t p, pairgeo_t *g) {
     g->vij[k] = v[3 * g->p1 + k] - v[3 * g->p2 + k];
   }
   g->rij = norm3(g->xij);
-  g->GK = wfo_grad_kernel(g->rij / g->h, g->h);
+  g->GK = wfo_grad_kernel_dim(g->rij / g->h, g->h, d->dim);
 }
 
 /* CalcAccel, Nishimura branch, GradientType 0
@@ -837,8 +862,8 @@ static void calc_accel_pairs(wfo_domain *d) {
     const double *ti = d->f[WFO_TI], *tir = d->f[WFO_TIR];
     int ti_on = ti[p1] > 0.0 || ti[p2] > 0.0;
     if (ti_on) {
-      double K = wfo_kernel(g.rij / g.h, g.h);
-      double K0 = wfo_kernel((d->f[WFO_TIDIST][p1] + d->f[WFO_TIDIST][p2]) / (2.0 * g.h), g.h);
+      double K = wfo_kernel_dim(g.rij / g.h, g.h, d->dim);
+      double K0 = wfo_kernel_dim((d->f[WFO_TIDIST][p1] + d->f[WFO_TIDIST][p2]) / (2.0 * g.h), g.h, d->dim);
       tis = pow((K / K0), (d->f[WFO_TIN][p1] + d->f[WFO_TIN][p2]) / 2.0);
     }
     double M[9];
@@ -851,6 +876,8 @@ static void calc_accel_pairs(wfo_domain *d) {
     double gv[3] = {g.GK * g.xij[0], g.GK * g.xij[1], g.GK * g.xij[2]};
     for (int c = 0; c < 3; c++)
       d->pair_force[3 * p + c] = gv[0] * M[c] + gv[1] * M[3 + c] + gv[2] * M[6 + c];
+    /* plane strain: the pair force has no z part (:184) */
+    if (d->dim == 2) d->pair_force[3 * p + 2] = 0.0;
   }
 }
 
@@ -889,7 +916,7 @@ static inline void pair_geo_locking(const wfo_domain *d, int64_t p, pairgeo_t *g
     g->vij[k] = v[3 * g->p1 + k] - v[3 * g->p2 + k];
   }
   g->rij = norm3(g->xij);
-  g->GK = wfo_grad_kernel(g->rij / g->h, g->h);
+  g->GK = wfo_grad_kernel_dim(g->rij / g->h, g->h, d->dim);
 }
 
 /* The damage weight of a pair in the Locking passes
@@ -936,8 +963,8 @@ static void calc_accel_locking(wfo_domain *d) {
     const double *ti = d->f[WFO_TI], *tir = d->f[WFO_TIR];
     int ti_on = ti[p1] > 0.0 || ti[p2] > 0.0;
     if (ti_on) {
-      double K = wfo_kernel(g.rij / g.h, g.h);
-      double K0 = wfo_kernel((d->f[WFO_TIDIST][p1] + d->f[WFO_TIDIST][p2]) / (2.0 * g.h), g.h);
+      double K = wfo_kernel_dim(g.rij / g.h, g.h, d->dim);
+      double K0 = wfo_kernel_dim((d->f[WFO_TIDIST][p1] + d->f[WFO_TIDIST][p2]) / (2.0 * g.h), g.h, d->dim);
       tis = pow((K / K0), (d->f[WFO_TIN][p1] + d->f[WFO_TIN][p2]) / 2.0);
     }
     double M[9];
@@ -957,6 +984,9 @@ static void calc_accel_locking(wfo_domain *d) {
     const double dam_f = pair_damage_factor(d, p1, p2);
     if (!(dam_f > 0.0)) continue;
     for (int c = 0; c < 3; c++) {
+      /* plane strain: temp(2) = 0 (Source/InteractionAlt.cpp:184), which
+       * the corrected-gradient terms do not see */
+      if (c == 2 && d->dim == 2 && !d->grad_corr) continue;
       double t1 = g1[0] * M[c] + g1[1] * M[3 + c] + g1[2] * M[6 + c];
       double t2 = g2[0] * M[c] + g2[1] * M[3 + c] + g2[2] * M[6 + c];
       acc[3 * p1 + c] += dam_f * m[p2] * t1;
@@ -1004,7 +1034,7 @@ int wfo_calc_force2233(wfo_domain *d, double xsph, int shepard) {
     pairgeo_t g;
     pair_geo_locking(d, p, &g);
     const int p1 = g.p1, p2 = g.p2;
-    const double K = wfo_kernel(g.rij / g.h, g.h);
+    const double K = wfo_kernel_dim(g.rij / g.h, g.h, d->dim);
     const double di = rho[p1], dj = rho[p2], mi = m[p1], mj = m[p2];
     const float mj_dj = (float)(mj / dj), mi_di = (float)(mi / di);
     zw[p1] += mj_dj * K;
@@ -1022,11 +1052,12 @@ int wfo_calc_force2233(wfo_domain *d, double xsph, int shepard) {
   return 0;
 }
 
-/* Domain::CalcGradCorrMatrix (Source/Domain.cpp:1832-1940), 3D: for every
+/* Domain::CalcGradCorrMatrix (Source/Domain.cpp:1832-1940): for every
  * pair in pair order, m = (GK x_ij) (x) x_ij; particle 1 subtracts
- * (mj/dj) m, particle 2 (mi/di) m; then each free particle's sum is
- * inverted (InvLog, External/matvec.h:673-693: |det| < 1e-10 fails and the
- * identity is taken); fixed particles get the identity */
+ * (mj/dj) m, particle 2 (mi/di) m; in 2D the sum's zz entry is set to 1
+ * (:1911-1915); then each free particle's sum is inverted (InvLog,
+ * External/matvec.h:673-693: |det| < 1e-10 fails and the identity is
+ * taken); fixed particles get the identity */
 int wfo_calc_grad_corr(wfo_domain *d) {
   double *T = (double *)xcalloc((size_t)d->n * 9, sizeof(double));
   const double *rho = d->f[WFO_RHO], *m = d->f[WFO_MASS];
@@ -1045,6 +1076,7 @@ int wfo_calc_grad_corr(wfo_domain *d) {
   double *L = d->f[WFO_GCM];
   for (int i = 0; i < d->solid_count; i++) {
     double *Li = L + 9 * (int64_t)i;
+    if (d->dim == 2) T[9 * (int64_t)i + 8] = 1.0;
     const double *M = T + 9 * (int64_t)i;
     int ok = d->fi[WFO_FREE][i];
     if (ok) {
@@ -1427,12 +1459,13 @@ static void plastic_work_heat(wfo_domain *d, double dt) {
     }
 #define MERGE_END }
 
-/* CalcConvHeat, 3D (Source/Thermal.cpp:339-389) */
+/* CalcConvHeat (Source/Thermal.cpp:339-389): dS2 = (m/rho)^(2/3) in 3D,
+ * (m/rho)^(1/3) in plane strain (:354-366) */
 static void conv_heat(wfo_domain *d) {
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (int i = 0; i < d->n; i++) {
     double dens = d->f[WFO_RHO][i];
-    double dS2 = pow(d->f[WFO_MASS][i] / dens, 2.0 / 3.0);
+    double dS2 = pow(d->f[WFO_MASS][i] / dens, d->dim == 2 ? 1.0 / 3.0 : 2.0 / 3.0);
     if (d->fi[WFO_THBC][i] == 1)
       d->f[WFO_QCONV][i] = dens * d->f[WFO_HCONV][i] * dS2 * (d->f[WFO_TINF][i] - d->f[WFO_T][i]) / d->f[WFO_MASS][i];
   }
@@ -1451,7 +1484,7 @@ static void temp_inc(wfo_domain *d) {
       int p1 = d->pairs[2 * p_], p2 = d->pairs[2 * p_ + 1];
       double xij[3] = {x[3 * p1] - x[3 * p2], x[3 * p1 + 1] - x[3 * p2 + 1], x[3 * p1 + 2] - x[3 * p2 + 2]};
       double h = (hh[p1] + hh[p2]) / 2.0;
-      double GK = wfo_grad_kernel(norm3(xij) / h, h);
+      double GK = wfo_grad_kernel_dim(norm3(xij) / h, h, d->dim);
       double mj = ms[p2], dj = rho[p2];
       double gv[3] = {GK * xij[0], GK * xij[1], GK * xij[2]};
       double m = mj / dj * 4. * (kT[p1] * kT[p2]) / (kT[p1] + kT[p2]) * (T[p1] - T[p2]) * dot3(xij, gv) /
@@ -1515,7 +1548,7 @@ static void kin_energy(wfo_domain *d) {
       double xij[3], vij[3];
       for (int k = 0; k < 3; k++) { xij[k] = x[3 * p1 + k] - x[3 * p2 + k]; vij[k] = v[3 * p1 + k] - v[3 * p2 + k]; }
       double h = (hh[p1] + hh[p2]) / 2.0;
-      double GK = wfo_grad_kernel(norm3(xij) / h, h);
+      double GK = wfo_grad_kernel_dim(norm3(xij) / h, h, d->dim);
       double di = rho[p1], dj = rho[p2], mj = ms[p2];
       double gv[3] = {GK * xij[0], GK * xij[1], GK * xij[2]};
       double temp = 0.5 * mj * (P[p1] / (di * di) + P[p2] / (dj * dj)) * dot3(vij, gv);
@@ -1726,8 +1759,8 @@ static int mesh_finish(wfo_domain *d, mesh_t *M, int first, int id, double hfac)
 
 /* Domain::CalculateSurface (Source/Domain.cpp:1415-1491): mass-weighted
  * sum of x_ij over SMPairs, scaled by the mean particle mass over ALL
- * particles (rigid ones count with mass 0); |n| >= h/4 and Nb <= 46 marks
- * a free-surface particle, which takes ID 1. */
+ * particles (rigid ones count with mass 0); |n| >= h/4 and Nb <= 46 (12
+ * in 2D, :1472-1474) marks a free-surface particle, which takes ID 1. */
 static int calculate_surface(wfo_domain *d, int id) {
   d->id_free_surf = id;
   double totmass = 0.;
@@ -1751,7 +1784,7 @@ static int calculate_surface(wfo_domain *d, int id) {
   int surf = 0;
   for (int i = 0; i < maxid; i++) {
     for (int k = 0; k < 3; k++) nrm[3 * i + k] *= 1. / totmass;
-    if (norm3(nrm + 3 * i) >= 0.25 * d->f[WFO_H][i] && d->fi[WFO_NB][i] <= 46) {
+    if (norm3(nrm + 3 * i) >= 0.25 * d->f[WFO_H][i] && d->fi[WFO_NB][i] <= (d->dim == 2 ? 12 : 46)) {
       d->fi[WFO_ID][i] = id;
       surf++;
     }
@@ -2243,6 +2276,14 @@ static double update_time_step(wfo_domain *d) {
 
 int wfo_solve_init(wfo_domain *d, double dt) {
   if (d->n <= 0) return fail("Particle Count is Null");
+  if (d->dim == 2) {
+    /* plane strain: the 2D tensile-instability branch and the 2D contact
+     * (whose dS2 reads another particle, Source/Contact_Wang.cpp:259) are
+     * not restated */
+    if (d->contact) return fail("2D: rigid contact is not restated");
+    for (int i = 0; i < d->n; i++)
+      if (d->f[WFO_TI][i] > 0.0) return fail("2D: the tensile-instability term is not restated");
+  }
   d->deltat = dt;
   d->deltatint = d->deltatmin = dt;
   initial_checks(d);

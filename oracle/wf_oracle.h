@@ -132,8 +132,11 @@ const int *wfo_contact_pairs(const wfo_domain *d);          /* (SPH, rigid) */
 /* reaction observed at the first BC call of the last step:
  * sum of m*a over particles with ID == zone (before BCs zero a) */
 int wfo_reaction(const wfo_domain *d, int zone, double out[3]);
-/* kernel functions, exported for closed-form pins */
+/* kernel functions, exported for closed-form pins (3D; the _dim forms take
+ * Domain::Dimension, 2 or 3) */
 double wfo_kernel(double q, double h);
+double wfo_kernel_dim(double q, double h, int dim);
+double wfo_grad_kernel_dim(double q, double h, int dim);
 /* symmetric 3x3 eigen-decomposition (cyclic Jacobi): A = V diag(w) V^T */
 void wfo_sym3_eig(const double *A, double *w, double *V);
 double wfo_grad_kernel(double q, double h);

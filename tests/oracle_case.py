@@ -55,6 +55,8 @@ def oracle_from_case(case, threads=1):
     for k in range(3):
         d.set_param(f"DomMax{k}", case.dom_max[k])
         d.set_param(f"DomMin{k}", case.dom_min[k])
+    if case.dim != 3:
+        d.set_param("Dimension", case.dim)
     if case.damage:
         d.set_param("model_damage", 1)
         d.set_param("nonlock_sum", 0)  # the scene driver's choice with damage (Source/WeldForm.cpp:303,314)

@@ -100,6 +100,7 @@ EXPORTS = [
     "wf_pack_xdmf_noalloc", "wf_pack_xdmf_noalloc_max", "wf_get_cell_grids",
     "wf_set_material_friction", "wf_set_group_sums", "wf_get_group_history",
     "wf_set_contact_law", "wf_set_friction_function", "wf_set_material_damage", "wf_calc_force2233",
+    "wf_set_dimension",
 ]
 
 # contact laws (Domain::contact_alg, Source/Domain.h:64)
@@ -157,6 +158,7 @@ def lib():
         "wf_set_material_friction": ([vp, i32, d, d], ctypes.c_int),
         "wf_set_contact_law": ([vp, i32, d, i32], ctypes.c_int),
         "wf_set_friction_function": ([vp, i32, d, d], ctypes.c_int),
+        "wf_set_dimension": ([vp, i32], ctypes.c_int),
         "wf_set_material_damage": ([vp, i32, i32, dp], ctypes.c_int),
         "wf_calc_force2233": ([vp, d, i32], ctypes.c_int),
         "wf_set_group_sums": ([vp, i32, i64], ctypes.c_int),
@@ -451,6 +453,10 @@ class Solver:
     def calc_force2233(self, xsph=0.0, shepard=False):
         """CalcForce2233 over every pair at the current state (no time advance)."""
         self._check(lib().wf_calc_force2233(self.ctx, float(xsph), int(bool(shepard))))
+
+    def set_dimension(self, dim):
+        """Domain::Dimension: 3, or 2 for plane strain (before init)."""
+        self._check(lib().wf_set_dimension(self.ctx, int(dim)))
 
     def set_friction_function(self, linear, m=0.0, b=0.0):
         self._check(lib().wf_set_friction_function(self.ctx, int(bool(linear)), m, b))

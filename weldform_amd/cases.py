@@ -52,6 +52,7 @@ class Case:
     contact_law: tuple = None   # (law, PFAC, fric_bound) of wf_set_contact_law, or None (Wang)
     friction_linear: tuple = None  # (friction_m, friction_b) of wf_set_friction_function, or None
     damage: dict = field(default_factory=dict)  # material idx -> Johnson-Cook damage (D1..D5)
+    dim: int = 3                # Domain::Dimension (2: plane strain, every particle at one z)
     meta: dict = field(default_factory=dict)
 
     @property
@@ -118,6 +119,50 @@ def cylinder_case(name, R, L, dx, rho, E, nu, mat, dt_cfl=0.4, hfac=1.2, steps=1
 
 
 AL = dict(rho=2700.0, E=68.9e9, nu=0.3)
+
+
+def plate2d_case(name, Lx, Ly, dx, rho, E, nu, mat, dt_cfl=0.4, hfac=1.2, steps=200, jitter=0.0, seed=0,
+                 v_top=-1.0, T0=None):
+    """Plane-strain upsetting (Domain::Dimension 2): a square lattice of
+    pitch dx over [0, Lx] x [0, Ly] in the plane z = 0, mass rho dx^2 (per
+    unit thickness), h = hfac dx; the bottom row (zone 2) held, the top row
+    (zone 3) driven along y at v_top; in-plane lattice jitter of +-jitter dx."""
+    nx, ny = int(round(Lx / dx)) + 1, int(round(Ly / dx)) + 1
+    gx, gy = np.meshgrid(np.arange(nx) * dx, np.arange(ny) * dx, indexing="ij")
+    x = np.stack([gx.ravel(), gy.ravel(), np.zeros(nx * ny)], axis=1)
+    if jitter > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        x[:, :2] += rng.uniform(-jitter * dx, jitter * dx, size=(len(x), 2))
+    n = len(x)
+    K, G, Cs = elastic(E, nu, rho)
+    h = hfac * dx
+    dt = dt_cfl * h / Cs
+    ids = np.zeros(n, dtype=np.int32)
+    ids[x[:, 1] <= 0.5 * dx] = 2
+    ids[x[:, 1] >= (ny - 1) * dx - 0.5 * dx] = 3
+    mat = dict(mat)
+    mat.update(E=E, nu=nu, G=G, Cs=Cs, ref_density=rho, P0=0.0)
+    if mat["model"] == capi.JOHNSON_COOK:
+        Th = (T0 - mat["jc_Tt"]) / (mat["jc_Tm"] - mat["jc_Tt"])
+        sy0 = mat["jc_A"] * (1.0 - Th ** mat["jc_m"])
+    elif mat["model"] == capi.HOLLOMON:
+        sy0 = mat["hol_sy0"]
+    else:
+        sy0 = mat.pop("sy0")
+    mat.pop("sy0", None)
+    return Case(name=name, x=x, h=np.full(n, h), mass=np.full(n, rho * dx * dx), density=np.full(n, rho),
+                ids=ids, material=np.zeros(n, dtype=np.int32), materials={0: mat},
+                sigmay=np.full(n, sy0), dt=dt, steps=steps,
+                T=None if T0 is None else np.full(n, float(T0)),
+                bcs=[dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, v_top, 0.0))],
+                dim=2, meta=dict(Lx=Lx, Ly=Ly, dx=dx, h=h, K=K, jitter=jitter, seed=seed))
+
+
+def plate2d_small(steps=200, v_top=-5.0, jitter=1e-3, seed=11, dx=1.0e-3):
+    """A 30 mm x 60 mm aluminium plate in plane strain (1,891 particles),
+    Johnson-Cook, upset at v_top along y."""
+    return plate2d_case("plate2d_upset_jc", 30e-3, 60e-3, dx, AL["rho"], AL["E"], AL["nu"], JC_AL,
+                        steps=steps, jitter=jitter, seed=seed, v_top=v_top, T0=300.0)
 
 
 def config1(steps=1000):
@@ -435,5 +480,7 @@ def make_solver(case, precision=capi.FP64, max_nb=128, ts_nb_inc=5, **kw):
         s.set_friction_function(True, *case.friction_linear)
     if case.group_sums is not None:
         s.set_group_sums(*case.group_sums)
+    if case.dim != 3:
+        s.set_dimension(case.dim)
     s.init(case.dt)
     return s

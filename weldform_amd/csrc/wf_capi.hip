@@ -74,6 +74,7 @@ struct Ctx {
   // wf_set_contact_law / wf_set_friction_function (PFAC defaults to the JSON
   // driver's penaltyFactor default, Source/WeldForm.cpp:685)
   int contact_law = 1, fric_bound = 0, fric_linear = 0;
+  int dim = 3;  // Domain::Dimension (wf_set_dimension)
   double pfac = 0.5, fr_m = 0.0, fr_b = 0.0;
   int heat_cond = 0, heat_fric = 0;
   double heat_hc = 0.0;
@@ -308,6 +309,9 @@ void cell_initiate(Ctx *c) {
   for (int k = 0; k < 3; ++k) {
     const double e = (TRPR[k] - BLPF[k]) / (c->cfg.cellfac * hmax);
     int cn = ((double)(std::ceil(e) - e) < (hmax / 10.0)) ? (int)std::ceil(e) : (int)std::floor(e);
+    // 2D: one layer of cells along z (Source/Neighbour.cpp:56-75); its
+    // height here is the box's, so every particle falls in it
+    if (k == 2 && c->dim == 2) cn = 1;
     if (cn <= 0) throw Error(WF_EINVAL, "CellInitiate: zero cells along an axis");
     c->d.cellno[k] = cn;
     c->d.blpf[k] = BLPF[k];
@@ -429,6 +433,7 @@ void allocate(Ctx *c) {
   Dev &d = c->d;
   const size_t np = (size_t)c->npad;
   d.n = (int)c->n;
+  d.dim = c->dim;
   d.npad = c->npad;
   d.max_nb = c->cfg.max_nb;
   d.cellfac = c->cfg.cellfac;
@@ -960,6 +965,9 @@ void enqueue_step(Ctx *c, int scheme, int cur, const TapeHook *tp = nullptr, int
   const double dt = d.auto_ts ? -1.0 : c->dt;
   const int nxt = cur ^ 1;
   if (in(P_SEARCH)) {
+    // StartAcceleration's plane-strain reset at the top of the step
+    // (Source/SolverLeapfrog.cpp:126, Source/SolverKickDrift.cpp:119)
+    if (d.dim == 2) launch_plane_strain(d, c->st);
     if (tp) tape_state(c, tp->s, cur);
     launch_schedule(d, c->st, 0);
     {
@@ -1099,6 +1107,7 @@ void adj_check(Ctx *c) {
   if (c->d.contact && (c->contact_law != 1 || c->fric_linear))
     throw Error(WF_EUNSUPPORTED, "the adjoint covers Wang's contact law with constant friction");
   if (c->d.damage) throw Error(WF_EUNSUPPORTED, "the adjoint does not cover damage");
+  if (c->d.dim == 2) throw Error(WF_EUNSUPPORTED, "the adjoint covers the 3D step");
 }
 
 inline size_t round256(size_t b) { return (b + 255) / 256 * 256; }
@@ -1727,6 +1736,19 @@ int wf_init(void *ctx, double dt) {
     }
     if ((c->heat_cond || c->heat_fric) && c->meshes.empty())
       throw Error(WF_EINVAL, "contact heat (wf_set_contact_heat) needs a rigid mesh (wf_add_rigid_plane / _mesh)");
+    if (c->dim == 2) {
+      // plane strain covers the solid passes, conduction and convection;
+      // the reference's 2D contact reads another particle's mass in its
+      // dS2 (Source/Contact_Wang.cpp:259) and its 2D tensile-instability
+      // branch is not modelled here
+      if (!c->meshes.empty()) throw Error(WF_EUNSUPPORTED, "2D: rigid contact is not supported");
+      for (int m = 0; m < kMaxMat; ++m)
+        if (c->mat_set[m] && c->mats[m].ti > 0.0)
+          throw Error(WF_EUNSUPPORTED, "2D: the tensile-instability term is not supported");
+      const std::vector<double> &x = c->hf[WF_F_X];
+      for (int64_t i = 1; i < c->n; ++i)
+        if (x[3 * i + 2] != x[2]) throw Error(WF_EINVAL, "2D: every particle must have the same z");
+    }
     {
       bool dam = false;
       for (int64_t i = 0; i < c->n; ++i) {
@@ -2103,6 +2125,17 @@ int wf_set_contact_law(void *ctx, int32_t law, double pfac, int32_t fric_bound) 
     c->contact_law = law;
     c->pfac = pfac;
     c->fric_bound = fric_bound ? 1 : 0;
+    return WF_OK;
+  });
+}
+
+int wf_set_dimension(void *ctx, int32_t dim) {
+  Ctx *c = (Ctx *)ctx;
+  return guard(c, [&] {
+    as_ctx(ctx);
+    if (c->inited) throw Error(WF_ESTATE, "wf_set_dimension after wf_init");
+    if (dim != 2 && dim != 3) throw Error(WF_EINVAL, "dimension must be 2 (plane strain) or 3");
+    c->dim = dim;
     return WF_OK;
   });
 }

@@ -107,21 +107,26 @@ __device__ inline double rsqrt_d(double x) {
   return fma(0.5 * y, e, y);
 }
 
+// kernel constants of smoothing length h in dimension dim: the gradient
+// factor's c5 = 1/(pi h^5) in 3D, 10/(7 pi h^4) in 2D (plane strain; the
+// cubic spline's C/h with C from Source/Functions.cpp:81)
 struct KernH {
-  double h, ih, c5;  // pair smoothing length, 1/h, 1/(pi h^5)
+  double h, ih, c5;  // pair smoothing length, 1/h, c5
+  int dim;
 };
-__device__ inline KernH kern_h(double h) {
+__device__ inline KernH kern_h(double h, int dim = 3) {
   KernH k;
   k.h = h;
   k.ih = rcp_d(h);
+  k.dim = dim;
   const double ih2 = k.ih * k.ih;
-  k.c5 = ih2 * ih2 * k.ih * (1.0 / M_PI);
+  k.c5 = dim == 2 ? ih2 * ih2 * (10.0 / (7.0 * M_PI)) : ih2 * ih2 * k.ih * (1.0 / M_PI);
   return k;
 }
 // h of the pair (h_i + h_j)/2; reuses particle i's constants when h_j == h_i
 __device__ inline KernH pair_h(const KernH &ki, double hi, double hj) {
   if (hj == hi) return ki;
-  return kern_h((hi + hj) / 2);
+  return kern_h((hi + hj) / 2, ki.dim);
 }
 __device__ inline double grad_kernel_r2(double r2, const KernH &k) {
   if (r2 == 0.0) return -3.0 * k.c5;
@@ -145,20 +150,22 @@ __device__ inline float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
 template <typename R>
 struct KH {
   R h, ih, c5;
+  int dim;
 };
 template <typename R>
-__device__ inline KH<R> kern_h_t(R h) {
+__device__ inline KH<R> kern_h_t(R h, int dim = 3) {
   KH<R> k;
   k.h = h;
   k.ih = rcp_t(h);
+  k.dim = dim;
   const R ih2 = k.ih * k.ih;
-  k.c5 = ih2 * ih2 * k.ih * (R)(1.0 / M_PI);
+  k.c5 = dim == 2 ? ih2 * ih2 * (R)(10.0 / (7.0 * M_PI)) : ih2 * ih2 * k.ih * (R)(1.0 / M_PI);
   return k;
 }
 template <typename R>
 __device__ inline KH<R> pair_h_t(const KH<R> &ki, R hi, R hj) {
   if (hj == hi) return ki;
-  return kern_h_t<R>((hi + hj) / (R)2);
+  return kern_h_t<R>((hi + hj) / (R)2, ki.dim);
 }
 // both branches evaluated and selected (no divergent branch in the pair
 // loops, so loads of later neighbours can be scheduled early); each branch

@@ -172,6 +172,7 @@ struct BcTable {
 
 struct Dev {
   int n;
+  int dim;  // Domain::Dimension: 3, or 2 (plane strain, one z-plane)
   long long npad;
   int max_nb;
   int ncells;
@@ -348,6 +349,7 @@ void launch_tape_lists(const Dev &d, hipStream_t st, const AdjSlot &s, int force
 
 // launchers (wf_kernels.hip); all enqueue on `st`
 void launch_schedule(const Dev &d, hipStream_t st, int force);
+void launch_plane_strain(const Dev &d, hipStream_t st);
 void launch_neighbours(const Dev &d, hipStream_t st, int cur);
 void launch_gradcorr(const Dev &d, hipStream_t st, int cur);
 void launch_accel(const Dev &d, hipStream_t st, int cur);

@@ -792,9 +792,10 @@ __global__ __launch_bounds__(kStageThreads) void k_stage(Dev d) {
   }
 }
 
-// Domain::CalcGradCorrMatrix (Source/Domain.cpp:1832-1940), 3D:
+// Domain::CalcGradCorrMatrix (Source/Domain.cpp:1832-1940):
 // L_i = (-sum_j V_j GK_ij x_ij (x) x_ij)^-1 over i's neighbour row (FP64
-// gather, V_j = m_j / rho_j, x_ij = x_i - x_j), symmetric; the identity
+// gather, V_j = m_j / rho_j, x_ij = x_i - x_j; in 2D the sum's zz entry is
+// set to 1 before the inversion), symmetric; the identity
 // where |det| < 1e-10 (InvLog, External/matvec.h:673-693) and for fixed
 // particles.  Runs when the schedule set gc_due.
 WF_KERNEL void k_gradcorr(Dev d, int cur) {
@@ -804,7 +805,7 @@ WF_KERNEL void k_gradcorr(Dev d, int cur) {
     double L[6] = {1.0, 1.0, 1.0, 0.0, 0.0, 0.0};
     if (d.group[i] >= 0) {
       const double4 xi = d.xh[cur][i];
-      const KH<double> ki = kern_h_t<double>(xi.w);
+      const KH<double> ki = kern_h_t<double>(xi.w, d.dim);
       double m[6] = {0, 0, 0, 0, 0, 0};
       const int cnt = d.nnb[i];
       for (int k = 0; k < cnt; ++k) {
@@ -820,6 +821,7 @@ WF_KERNEL void k_gradcorr(Dev d, int cur) {
         m[XZ] -= w * dx * dz;
         m[YZ] -= w * dy * dz;
       }
+      if (d.dim == 2) m[ZZ] = 1.0;  // the 2D sum's zz entry (Source/Domain.cpp:1911-1915)
       const double c00 = m[YY] * m[ZZ] - m[YZ] * m[YZ], c01 = m[XZ] * m[YZ] - m[XY] * m[ZZ];
       const double c02 = m[XY] * m[YZ] - m[XZ] * m[YY];
       const double det = m[XX] * c00 + m[XY] * c01 + m[XZ] * c02;
@@ -839,6 +841,16 @@ WF_KERNEL void k_gradcorr(Dev d, int cur) {
 void launch_gradcorr(const Dev &d, hipStream_t st, int cur) {
   k_gradcorr<<<nblk_small(d.n), kBlock, 0, st>>>(d, cur);
 }
+
+// plane strain: StartAcceleration resets each particle's z velocity at the
+// top of every step (Source/Domain.cpp:1592); the FP32 shadow with it
+WF_KERNEL void k_plane_strain(Dev d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= d.n) return;
+  d.vm[i].z = 0.0;
+  if (d.mixed) d.vmf[i].z = 0.0f;
+}
+void launch_plane_strain(const Dev &d, hipStream_t st) { k_plane_strain<<<d.nblocks, kBlock, 0, st>>>(d); }
 
 void launch_schedule(const Dev &d, hipStream_t st, int force) {
   k_schedule<<<1, 64, 0, st>>>(d, force);
@@ -1049,7 +1061,7 @@ __device__ inline void accel_lane(const Dev &d, int cur, int i, const G &g, int 
     const auto vi = S::v(d, i);
     R ti[6], rhoi;
     S::srec(d, i, ti, rhoi);
-    const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
+    const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi), d.dim);
     const R acs = alpha * cs;
     R tri[6] = {0, 0, 0, 0, 0, 0};
     const R tin = (R)M.ti_n, tid = (R)M.ti_dist;
@@ -1189,8 +1201,8 @@ __device__ inline void walk_groups(const Rows &rows, int cnt, int sub, const typ
 struct GkF {
   float ih, gin_a, gin_b, gout;  // 1/h, 2.25 c5, -3 c5, -0.75 c5 h
 };
-__device__ inline GkF gk_consts(float h) {
-  const KH<float> k = kern_h_t<float>(h);
+__device__ inline GkF gk_consts(float h, int dim) {
+  const KH<float> k = kern_h_t<float>(h, dim);
   return GkF{k.ih, 2.25f * k.c5, -3.0f * k.c5, -0.75f * k.c5 * h};
 }
 // GK of the cubic spline (Source/Functions.cpp:74-87) for r2 = |x_ij|^2:
@@ -1247,7 +1259,7 @@ __device__ inline void accel_mx_lane(const Dev &d, int cur, int i, int sub, cons
   const float4 xi = d.xhf[cur][i], vi = d.vmf[i];
   const StressRecF ri = d.srecf[i];
   const float h = (float)d.xh[cur][i].w;
-  const GkF k = gk_consts(h);
+  const GkF k = gk_consts(h, d.dim);
   const float h2c = 0.01f * h * h, hh = 2.0f * h;
   const float acs = (float)(M.alpha * M.Cs), bh = 0.5f * (float)M.beta, hha = hh * acs;
   walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
@@ -1310,6 +1322,9 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_f64(Dev d, int cur) {
     }
   }
   combine_lanes<LANES>(a, red);
+  // plane strain: the pair forces have no z part (Source/InteractionAlt.cpp:
+  // 184; the corrected-gradient terms keep theirs)
+  if (d.dim == 2 && !d.gc_on) a[2] = 0.0;
   if (i < d.n && sub == 0) d.acc[i] = make_double4(a[0], a[1], a[2], 0.0);
 }
 
@@ -1338,6 +1353,7 @@ __device__ inline void k_accel_mx_body(const Dev &d, int cur) {
     }
   }
   combine_lanes<LANES>(a, red);
+  if (d.dim == 2 && !d.gc_on) a[2] = 0.0f;  // plane strain, as the FP64 pass
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
 }
 
@@ -1370,6 +1386,7 @@ __global__ __launch_bounds__(kBlock *LANES) void k_accel_mx_ti(Dev d, int cur) {
     else accel_lane<true, true, false>(d, cur, i, g, sub, LANES, a);
   }
   combine_lanes<LANES>(a, red);
+  if (d.dim == 2 && !d.gc_on) a[2] = 0.0f;  // plane strain, as the FP64 pass
   if (i < d.n && sub == 0) d.acc[i] = make_double4((double)a[0], (double)a[1], (double)a[2], 0.0);
 }
 
@@ -1656,7 +1673,7 @@ __device__ inline typename Src<MX>::R dens_lane(const Dev &d, int cur, int i, co
     own_corr<GC>(d, i, Li);
     const auto xi = S::x(d, cur, i);
     const auto vi = S::v(d, i);
-    const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi));
+    const KH<R> ki = kern_h_t<R>(S::h_own(d, cur, i, xi), d.dim);
     for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
@@ -1765,7 +1782,7 @@ __device__ inline float dens_mx_lane(const Dev &d, int cur, int i, int sub, cons
   float Li[6];
   own_corr<GC>(d, i, Li);
   const float4 xi = d.xhf[cur][i], vi = d.vmf[i];
-  const GkF k = gk_consts((float)d.xh[cur][i].w);
+  const GkF k = gk_consts((float)d.xh[cur][i].w, d.dim);
   walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
     float4 X[4], V[4];
 #pragma unroll
@@ -1848,7 +1865,7 @@ __device__ inline void rates_lane(const Dev &d, int buf, int i, const G &g, int 
     const R P0 = (R)M.P0, cs2 = (R)(M.Cs * M.Cs), rho0 = (R)M.ref_density;
     const R pri = g.pr(MX ? g.self() : i, MX ? (R)0 : Sr::rho(d, buf, i, xi, vi), P0, cs2, rho0);
     const R mi = vi.w;
-    const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi));
+    const KH<R> ki = kern_h_t<R>(Sr::h_own(d, buf, i, xi), d.dim);
     for_each_nb(g, d.nnb[i], sub, lanes, [&](int j) {
       const auto xj = g.x(j);
       const auto vj = g.v(j);
@@ -1932,7 +1949,7 @@ __device__ inline void rates_mx_lane(const Dev &d, int buf, int i, int own, int 
   own_corr<GC>(d, i, Li);
   const float4 xi = d.xhf[buf][i], vi = d.vmf[i];
   const float pri = d.prf[i], mneg = -vi.w;
-  const GkF k = gk_consts((float)d.xh[buf][i].w);
+  const GkF k = gk_consts((float)d.xh[buf][i].w, d.dim);
   walk_groups<LANES>(rows, d.nnb[i], sub, rows.self(), [&](const typename Rows::G &c) {
     float4 X[4], V[4];
     float P[4];
@@ -2196,13 +2213,14 @@ WF_KERNEL void k_thermal(Dev d, int buf, double dt) {
     const double Ti = d.T[i];
     const double voli = d.vol[buf][i];
     if (d.thbc[i] == 1) {
-      const double dS2 = pow(m / rho, 2.0 / 3.0);
+      // (m/rho)^(2/3), (m/rho)^(1/3) in plane strain (Source/Thermal.cpp:354-366)
+      const double dS2 = pow(m / rho, d.dim == 2 ? 1.0 / 3.0 : 2.0 / 3.0);
       d.qconv[i] = rho * d.hconv[i] * dS2 * (d.tinf[i] - Ti) / m;
     }
     const double kk = 4. * (M.k_T * M.k_T) / (M.k_T + M.k_T);
     const int ci = d.cell[i], oi = d.perm[i];
     double temp = 0.0;
-    const KernH ki = kern_h(xi.w);
+    const KernH ki = kern_h(xi.w, d.dim);
     const int cnt = d.nnb[i];
     for (int k = 0; k < cnt; ++k) {
       const int j = d.nbr[nb_slot(np, k, i)];
@@ -2431,7 +2449,8 @@ WF_KERNEL void k_surface(Dev d, int cur, int setup) {
     nx *= f;
     ny *= f;
     nz *= f;
-    const int surf = sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * xi.w && cnt <= 46;
+    // at most 46 neighbours (12 in 2D, Source/Domain.cpp:1472-1474)
+    const int surf = sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * xi.w && cnt <= (d.dim == 2 ? 12 : 46);
     d.id[i] = surf ? 1 : d.id_orig[i];
     if (setup == 2) return;
     int c = 0;
@@ -2825,7 +2844,7 @@ __global__ __launch_bounds__(kBlock) void k_force2233(Dev d, int cur, double xsp
     const double4 xi = S::x(d, cur, i), vi = S::v(d, i);
     double ti[6], rho_i;
     S::srec(d, i, ti, rho_i);
-    const KH<double> ki = kern_h_t<double>(xi.w);
+    const KH<double> ki = kern_h_t<double>(xi.w, d.dim);
     const HbmJ<false> g{d, cur, i};
     for_each_nb(g, d.nnb[i], 0, 1, [&](int j) {
       const double4 xj = g.x(j), vj = g.v(j);
@@ -2836,7 +2855,9 @@ __global__ __launch_bounds__(kBlock) void k_force2233(Dev d, int cur, double xsp
       const double r2 = dx * dx + dy * dy + dz * dz;
       const double GK = gk_t<double>(r2, kh);
       const double q = r2 > 0.0 ? sqrt(r2) * kh.ih : 0.0;
-      const double K = w_shape<double>(q) * (kh.ih * kh.ih * kh.ih * (1.0 / M_PI));
+      // W's constant C (Source/Functions.cpp:32): 1/(pi h^3), 10/(7 pi h^2) in 2D
+      const double K = w_shape<double>(q) * (d.dim == 2 ? kh.ih * kh.ih * (10.0 / (7.0 * M_PI))
+                                                        : kh.ih * kh.ih * kh.ih * (1.0 / M_PI));
       const double vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
       const double vdot = vdx * dx + vdy * dy + vdz * dz;
       double pi = 0.0;
@@ -2877,6 +2898,7 @@ __global__ __launch_bounds__(kBlock) void k_force2233(Dev d, int cur, double xsp
       if (shepard && real) sd += mj * K;
     });
   }
+  if (d.dim == 2 && !GC) a[2] = 0.0;  // plane strain (Source/Interaction.cpp:237)
   const double4 a4 = d.acc[i];
   d.acc[i] = make_double4(a[0], a[1], a[2], a4.w);
   d.drho[i] = s * rhoi;
