@@ -180,13 +180,20 @@ class Domain {
   }
 
   // AddBoxLength, 3D cubic (type 1) and hexagonal close packing (type 0),
-  // Source/Domain.cpp:326-460; mass = box volume * rho / N
+  // Source/Domain.cpp:326-460; mass = box volume * rho / N.  In 2D the
+  // plane z = 0 (Source/Domain.cpp:494-571): hexagonal rows (rotation 0) or
+  // columns (rotation 90) with mass sqrt(3) r^2 rho each, or a square
+  // lattice with mass 4 r^2 rho
   void AddBoxLength(int tag, Vec3_t const &V, double Lx, double Ly, double Lz, double r, double Density, double h,
                     int type, int rotation, bool random, bool Fixed) {
-    if (Dimension != 3) throw new Fatal("AddBoxLength: only the 3D generator is available");
+    if (Dimension != 3 && Dimension != 2) throw new Fatal("AddBoxLength: Dimension must be 2 or 3");
     if (!(type == 0 || type == 1) || !(rotation == 0 || rotation == 90))
       throw new Fatal("AddBoxLength: packing type must be 0 or 1 and rotation 0 or 90");
     if (random) throw new Fatal("AddBoxLength: randomised packing is not supported on the device path");
+    if (Dimension == 2) {
+      add_box_2d(tag, V, Lx, Ly, r, Density, h, type, rotation, Fixed);
+      return;
+    }
     const size_t pre = Particles.Size();
     size_t k = 0;
     double zp = V(2);
@@ -231,6 +238,51 @@ class Domain {
     const Vec3_t ext = Max - V;
     const double Mass = ext(0) * ext(1) * ext(2) * Density / (double)(Particles.Size() - pre);
     for (size_t i = pre; i < Particles.Size(); ++i) Particles[i]->Mass = Mass;
+    solid_part_count = (int)Particles.Size();
+    R = r;
+  }
+
+  // the 2D branches of AddBoxLength (Source/Domain.cpp:494-571)
+  void add_box_2d(int tag, Vec3_t const &V, double Lx, double Ly, double r, double Density, double h, int type,
+                  int rotation, bool Fixed) {
+    const double s3 = std::sqrt(3.0);
+    auto push = [&](double x, double y, double m) {
+      Particles.Push(new Particle(tag, Vec3_t(x, y, 0.0), Vec3_t(0, 0, 0), m, Density, h, Fixed));
+    };
+    if (type == 1) {
+      // square lattice of pitch 2r
+      for (size_t j = 0;; ++j) {
+        const double yp = j == 0 ? V(1) : V(1) + (2.0 * j + 1) * r;
+        if (!(yp <= V(1) + Ly - r)) break;
+        for (size_t i = 0;; ++i) {
+          const double xp = i == 0 ? V(0) : V(0) + (2.0 * i + 1) * r;
+          if (!(xp <= V(0) + Lx - r)) break;
+          push(V(0) + (2.0 * i + 1) * r, V(1) + (2.0 * j + 1) * r, 2.0 * r * 2.0 * r * Density);
+        }
+      }
+    } else if (rotation == 0) {
+      // hexagonal rows: odd rows shifted by r
+      for (size_t j = 0;; ++j) {
+        const double yp = j == 0 ? V(1) : V(1) + (s3 * j + 1) * r;
+        if (!(yp <= V(1) + Ly - r)) break;
+        for (size_t i = 0;; ++i) {
+          const double xp = i == 0 ? V(0) : V(0) + (2 * i + (j % 2) + 1) * r;
+          if (!(xp <= V(0) + Lx - r)) break;
+          push(V(0) + (2 * i + (j % 2) + 1) * r, V(1) + (s3 * j + 1) * r, s3 * r * r * Density);
+        }
+      }
+    } else {
+      // hexagonal columns: odd columns shifted by r
+      for (size_t i = 0;; ++i) {
+        const double xp = i == 0 ? V(0) : V(0) + (s3 * i + 1) * r;
+        if (!(xp <= V(0) + Lx - r)) break;
+        for (size_t j = 0;; ++j) {
+          const double yp = j == 0 ? V(1) : V(1) + (2 * j + (i % 2) + 1) * r;
+          if (!(yp <= V(1) + Ly - r)) break;
+          push(V(0) + (s3 * i + 1) * r, V(1) + (2 * j + (i % 2) + 1) * r, s3 * r * r * Density);
+        }
+      }
+    }
     solid_part_count = (int)Particles.Size();
     R = r;
   }
@@ -370,7 +422,7 @@ class Domain {
   void check_supported() {
     const size_t n = Particles.Size();
     if (n == 0) throw new Fatal("Particle Count is Null. Please Check Radius and Domain Dimensions.");
-    if (Dimension != 3) throw new Fatal("the device path is 3D only (set Dimension = 3)");
+    if (Dimension != 3 && Dimension != 2) throw new Fatal("Dimension must be 2 (plane strain) or 3");
     if (KernelType != Qubic_Spline) throw new Fatal("the device path implements the cubic spline kernel only");
     if (GradientType != Squared_density) throw new Fatal("the device path implements GradientType 0 only");
     if (contact) {
@@ -854,6 +906,7 @@ class Domain {
       }
       if (rc < 0) check(rc);
     }
+    if (Dimension == 2) check(wf_set_dimension(ctx_, 2));
     if (contact) {
       check(wf_set_contact(ctx_, friction_sta, friction_dyn, DFAC));
       check(wf_set_contact_law(ctx_, (int32_t)contact_alg, PFAC, fric_type == Fr_Bound ? 1 : 0));

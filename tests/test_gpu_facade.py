@@ -17,7 +17,7 @@ import pytest
 import h5read
 from oracle_case import normwise_err, oracle_from_case
 from weldform_amd import build as wfbuild
-from weldform_amd import cases, output
+from weldform_amd import capi, cases, output
 
 pytestmark = pytest.mark.gpu
 
@@ -276,3 +276,65 @@ def test_json_damage_scene_matches_oracle(tmp_path):
     # the oracle's own sensitivity to a 1e-15 perturbation of the initial
     # positions: 7e-11 displacement, 3e-9 plastic strain
     assert final.shape == (c.n, 4) and e_u < 1e-8 and e_p < 1e-7
+
+
+def test_json_plane_strain_box_scene_matches_oracle(tmp_path):
+    """A Box block thinner than h makes the scene plane strain (Source/
+    WeldForm.cpp:420-424): the driver's 2D AddBoxLength square lattice
+    (pitch 2r, mass 4 r^2 rho, Source/Domain.cpp:550-571) upset along y for
+    200 Leapfrog steps, against the oracle's 2D run of the same lattice."""
+    steps = 200
+    sc = json.load(open(os.path.join(os.path.dirname(__file__), "json", "upsetting_bilinear.json")))
+    r, rho = 0.0005, 2700.0
+    Lx, Ly = 0.015, 0.030
+    sc["DomainBlocks"] = [{"type": "Box", "zoneId": 0, "start": [0.0, 0.0, 0.0], "dim": [Lx, Ly, 0.0]}]
+    sc["DomainZones"] = [{"id": 2, "start": [-1.0, -1.0, -1.0], "end": [1.0, 0.00075, 1.0]},
+                         {"id": 3, "start": [-1.0, Ly - 0.00175, -1.0], "end": [1.0, 1.0, 1.0]}]
+    sc["BoundaryConditions"] = [{"zoneId": 2, "valueType": 0, "value": [0.0, 0.0, 0.0]},
+                                {"zoneId": 3, "valueType": 0, "value": [0.0, -1.0, 0.0]}]
+    E, nu, Et = 68.9e9, 0.3, 1.5e9
+    K, G, Cs = cases.elastic(E, nu, rho)
+    h = 1.2 * 2 * r
+    dt = 0.4 * h / Cs
+    sc["Configuration"]["simTime"] = (steps - 0.5) * dt
+    sc["Configuration"]["outTime"] = 1.0
+    scene = str(tmp_path / "plate2d.json")
+    with open(scene, "w") as f:
+        json.dump(sc, f)
+    if not os.path.exists(wfbuild.JSON_DRIVER):
+        wfbuild.build_json_driver()
+    dump = str(tmp_path / "final.txt")
+    print("json driver: plane-strain box scene", flush=True)
+    run = subprocess.run([wfbuild.JSON_DRIVER, scene, "--out", str(tmp_path / "run"), "--dump", dump],
+                         capture_output=True, text=True, timeout=100, cwd=str(tmp_path))
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "Dimension is set to 2" in run.stdout
+    final = np.loadtxt(dump)
+    # the same lattice, in the generator's order (x fastest) and with its
+    # loop bounds evaluated as it evaluates them (xp <= Lx - r in doubles)
+    def count(L):
+        k = 0
+        while (0.0 if k == 0 else 0.0 + (2.0 * k + 1) * r) <= 0.0 + L - r:
+            k += 1
+        return k
+    nx, ny = count(Lx), count(Ly)
+    gy, gx = np.meshgrid((2 * np.arange(ny) + 1) * r, (2 * np.arange(nx) + 1) * r, indexing="ij")
+    x = np.stack([gx.ravel(), gy.ravel(), np.zeros(nx * ny)], axis=1)
+    n = len(x)
+    assert final.shape[0] == n
+    ids = np.where(x[:, 1] <= 0.00075, 2, np.where(x[:, 1] >= Ly - 0.00175, 3, 0)).astype(np.int32)
+    mat = dict(model=capi.BILINEAR, alpha=1.0, beta=0.0, Ep=E * Et / (E - Et), E=E, nu=nu, G=G, Cs=Cs,
+               ref_density=rho, P0=0.0)
+    c = cases.Case(name="plate2d_json", x=x, h=np.full(n, h), mass=np.full(n, 4 * r * r * rho),
+                   density=np.full(n, rho), ids=ids, material=np.zeros(n, dtype=np.int32), materials={0: mat},
+                   sigmay=np.full(n, 300e6), dt=dt, steps=steps,
+                   bcs=[dict(zone=2, value=(0.0, 0.0, 0.0)), dict(zone=3, value=(0.0, -1.0, 0.0))], dim=2,
+                   dom_max=(0.0, -1e11, -1e11), dom_min=(0.0, 1e11, 1e11))
+    d = oracle_from_case(c, threads=8)
+    d.solve_init(c.dt)
+    for _ in range(steps):
+        d.step()
+    e_u = normwise_err(final[:, :3], d.f("DISP"))
+    print(f"json plane-strain driver: displacement {e_u:.3e}", flush=True)
+    assert e_u < 1e-8
+    assert np.all(final[:, 2] == 0.0)

@@ -193,7 +193,11 @@ int main(int argc, char **argv) try {
     dom.DomMax(0) = L[i];
     dom.DomMin(0) = -L[i];
   }
-  if (std::fabs(L[2]) < h) throw new Fatal("a block thinner than h is 2D; the device path is 3D only");
+  // a block thinner than h is plane strain (Source/WeldForm.cpp:420-424)
+  if (std::fabs(L[2]) < h) {
+    dom.Dimension = 2;
+    std::cout << "Z Value is less than h. Dimension is set to 2." << std::endl;
+  }
   if (domtype == "Box") {
     if (gridCS == "AxiSymmetric") throw new Fatal("axisymmetric blocks are not implemented on the device path");
     dom.AddBoxLength(id, start, L[0], L[1], L[2], r, rho, h, 1, 0, false, false);
