@@ -2424,10 +2424,14 @@ WF_KERNEL void k_surface(Dev d, int cur, int setup) {
     const double4 xi = d.xh[cur][i];
     double nx = 0.0, ny = 0.0, nz = 0.0;
     const int cnt = d.nnb[i];
-    // the row's groups of four entries, their gathers issued together; the
-    // sum runs over the entries in row order
+    // a free-surface particle has at most 46 neighbours (12 in 2D,
+    // Source/Domain.cpp:1472-1474); with more the test fails whatever the
+    // normal, so the row is not walked.  Otherwise the row's groups of four
+    // entries, their gathers issued together; the sum runs over the entries
+    // in row order
+    const int nb_max = d.dim == 2 ? 12 : 46;
     const int4 *row = reinterpret_cast<const int4 *>(d.nbr) + i;
-    for (int g = 0; 4 * g < cnt; ++g) {
+    for (int g = 0; cnt <= nb_max && 4 * g < cnt; ++g) {
       const int4 e = row[(long long)g * np];
       const int jj[4] = {e.x, e.y, e.z, e.w};
       double4 xj[4];
@@ -2449,8 +2453,7 @@ WF_KERNEL void k_surface(Dev d, int cur, int setup) {
     nx *= f;
     ny *= f;
     nz *= f;
-    // at most 46 neighbours (12 in 2D, Source/Domain.cpp:1472-1474)
-    const int surf = sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * xi.w && cnt <= (d.dim == 2 ? 12 : 46);
+    const int surf = cnt <= nb_max && sqrt(nx * nx + ny * ny + nz * nz) >= 0.25 * xi.w;
     d.id[i] = surf ? 1 : d.id_orig[i];
     if (setup == 2) return;
     int c = 0;
