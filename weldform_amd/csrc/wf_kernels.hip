@@ -155,18 +155,36 @@ WF_KERNEL void k_cell_hist(Dev d, int cur) {
   if (!d.flags->do_search) return;
   __shared__ unsigned hw[kBlock / 64];
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   unsigned hb = 0;
+  int c = -1;
   if (i < d.n) {
     const double4 x = d.xh[cur][i];
     hb = __float_as_uint((float)x.w);
     d.cell[i] = cell_of(d, x);
-    const int c = scell_of(d, x);
+    c = scell_of(d, x);
     d.scell[i] = c;
     const int cz = c % d.scellno[2];
     const double oz = d.blpf[2] + cz * d.scellsize[2];
     d.sslice[i] = (unsigned char)z_slice((float)(x.z - oz), 4.f / (float)d.scellsize[2]);
-    d.srank[i] = atomicAdd(&d.cell_count[c], 1);
-  } else if (i - d.n < d.nrig) {
+  }
+  // one atomic per run of consecutive lanes in the same cell (the particles
+  // are in Morton order, so a wavefront's lanes share few cells): the run's
+  // first lane adds the run length, each lane takes the old count plus its
+  // place in the run
+  {
+    const int cprev = __shfl_up(c, 1, 64);
+    const unsigned long long heads = __ballot(lane == 0 || cprev != c);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+    const int start = 63 - __clzll(heads & upto);
+    const unsigned long long above = heads & ~upto;
+    const int end = above ? __ffsll((long long)above) - 1 : 64;
+    int base = 0;
+    if (c >= 0 && lane == start) base = atomicAdd(&d.cell_count[c], end - start);
+    base = __shfl(base, start, 64);
+    if (c >= 0) d.srank[i] = base + (lane - start);
+  }
+  if (i >= d.n && i - d.n < d.nrig) {
     const int r = i - d.n;
     const int c = cell_of(d, d.rx[r]);
     d.rcell[r] = c;
