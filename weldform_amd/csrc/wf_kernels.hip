@@ -352,6 +352,37 @@ __device__ inline void put_record(const Dev &d, int cur, int p, int j, const dou
   d.sinfo[p] = make_int2(j, d.group[j]);
 }
 
+// cell c (or none, c < 0) of 2-16 particles by the 16-lane segment of the
+// wavefront that lane belongs to: each lane holds one particle and counts
+// the segment's smaller keys; every lane of the wavefront calls this
+__device__ inline void sort_small_cell(const Dev &d, int cur, int c, int lane) {
+  const int seg = lane >> 4, sl = lane & 15;
+  int s = 0, e = 0;
+  if (c >= 0) {
+    s = cell_first(d, c);
+    e = cell_end(d, c);
+  }
+  const bool act = sl < e - s;
+  int j = 0, zs = 0;
+  if (act) {
+    j = d.sorted[s + sl];
+    zs = d.sslice[j];
+  }
+  const long long key = act ? ((long long)zs << 32) | (long long)j : 0x7fffffffffffffffLL;
+  int rank = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) rank += __shfl(key, (seg << 4) + q, 64) < key ? 1 : 0;
+  const int sh = seg << 4;
+  const unsigned b0 = (unsigned)((__ballot(act && zs == 0) >> sh) & 0xffffu);
+  const unsigned b1 = (unsigned)((__ballot(act && zs <= 1) >> sh) & 0xffffu);
+  const unsigned b2 = (unsigned)((__ballot(act && zs <= 2) >> sh) & 0xffffu);
+  if (act) {
+    d.sorted[s + rank] = j;
+    put_record(d, cur, s + rank, j, cell_corner(d, c));
+  }
+  if (c >= 0 && sl == 0) d.soff[c] = make_int4(s + __popc(b0), s + __popc(b1), s + __popc(b2), e);
+}
+
 // one cell of two or more particles, by the 64 lanes of a wavefront (lane =
 // its lane)
 __device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
@@ -414,8 +445,9 @@ __device__ inline void sort_cell(const Dev &d, int cur, int c, int lane) {
 // index, group) in cell order.  A wavefront takes cells in groups of
 // kSortGroup: one lane per cell writes the start, clears the counter and
 // finishes a cell of at most one particle; then the wavefront sorts the
-// group's larger cells one after another (sort_cell), so grids of mostly
-// empty cells cost little.  The wavefronts, and the threads writing the
+// group's cells of 2-16 particles four at a time (sort_small_cell) and the
+// larger ones one after another (sort_cell), so grids of mostly empty
+// cells cost little.  The wavefronts, and the threads writing the
 // rigid grid's starts, stride over the cells (a modest grid, so a step
 // without a search exits early at little cost).
 constexpr int kSortGroup = 16;
@@ -448,7 +480,20 @@ WF_KERNEL void k_cell_sort(Dev d, int cur) {
         d.soff[c] = so;
       }
     }
-    unsigned long long big = __ballot(mine && cnt >= 2);
+    // cells of 2-16 particles four at a time, one per 16-lane segment;
+    // larger cells by the whole wavefront
+    unsigned long long small = __ballot(mine && cnt >= 2 && cnt <= 16);
+    unsigned long long big = __ballot(mine && cnt > 16);
+    while (small) {
+      int cq[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        cq[q] = small ? base + __ffsll((long long)small) - 1 : -1;
+        small &= small - 1;
+      }
+      const int seg = lane >> 4;
+      sort_small_cell(d, cur, seg == 0 ? cq[0] : (seg == 1 ? cq[1] : (seg == 2 ? cq[2] : cq[3])), lane);
+    }
     while (big) {
       const int l = __ffsll((long long)big) - 1;
       big &= big - 1;
