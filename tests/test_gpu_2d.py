@@ -213,3 +213,28 @@ def test_2d_refusals():
         g.set_dimension(3)
     assert e.value.code == capi.ESTATE
     g.close()
+
+
+def test_2d_damage_surface_matches_oracle():
+    """Damage refreshes the free surface every step (Source/SolverLeapfrog.
+    cpp:204-206); in 2D a surface particle has at most 12 neighbours (Source/
+    Domain.cpp:1474), so on the plain lattice exactly the outer ring takes
+    ID 1.  FP64 with the Locking sums, against the oracle."""
+    c = cases.plate2d_small(steps=20, v_top=-5.0, jitter=0.0)
+    c.damage = {0: (10.0, 0.0, 0.0, 0.0, 0.0)}
+    d = oracle_from_case(c, threads=8)
+    d.solve_init(c.dt)
+    for _ in range(c.steps):
+        d.step()
+    g = cases.make_solver(c)
+    g.step(c.steps)
+    g.sync()
+    ids = g.get_int("id")
+    assert np.array_equal(ids, d.i("ID")[:c.n])
+    x = c.x
+    Lx, Ly = c.meta["Lx"], c.meta["Ly"]
+    ring = (x[:, 0] < 1e-9) | (x[:, 0] > Lx - 1e-9) | (x[:, 1] < 1e-9) | (x[:, 1] > Ly - 1e-9)
+    assert np.all(ids[ring] == 1) and not np.any(ids[~ring] == 1)
+    assert normwise_err(g.get("disp"), d.f("DISP")) < 1e-8
+    _assert_plane(g)
+    g.close()
