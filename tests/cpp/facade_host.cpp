@@ -2,6 +2,7 @@
 // generators, AssignZone and the errors raised for settings the device path
 // does not implement.  Prints one "key value..." line per check; the pytest
 // side (tests/test_capi_host.py) compares them with numpy and the oracle.
+#include <cmath>
 #include <cstdio>
 #include <functional>
 #include <string>
@@ -42,6 +43,21 @@ int main() {
       sz += dom.Particles[a]->x(2);
     }
     std::printf("boxsum %d %.17g %.17g %.17g\n", type, sx, sy, sz);
+  }
+  // plane strain: the 2D packings (type, rotation) of AddBoxLength
+  const int p2[3][2] = {{0, 0}, {0, 90}, {1, 0}};
+  for (const auto &tr : p2) {
+    SPH::Domain dom;
+    dom.Dimension = 2;
+    dom.AddBoxLength(1, Vec3_t(0.1, -0.2, 0.0), 3e-3, 2e-3, 0.0, 0.25e-3, 7850.0, 0.6e-3, tr[0], tr[1], false, false);
+    double sx = 0, sy = 0, sz = 0;
+    for (size_t a = 0; a < dom.Particles.Size(); ++a) {
+      sx += dom.Particles[a]->x(0);
+      sy += dom.Particles[a]->x(1);
+      sz += std::fabs(dom.Particles[a]->x(2));
+    }
+    std::printf("box2d %d %d %zu %.17g %.17g %.17g %.17g\n", tr[0], tr[1], dom.Particles.Size(),
+                dom.Particles[0]->Mass, sx, sy, sz);
   }
   // AddCylinderLength and AssignZone
   {

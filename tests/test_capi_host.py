@@ -148,6 +148,48 @@ def test_cpp_facade_generators_and_errors(tmp_path):
     assert np.allclose(boxsum[1], sums, rtol=1e-13)
     # hexagonal close packing: same generator count, mass from the same rule
     assert int(box[0][0]) == 264 and float(box[0][1]) > 0
+    # 2D packings in the plane z = 0 (Source/Domain.cpp:494-571), restated
+    # here from their loops: rows of pitch sqrt(3) r shifted by r on odd rows
+    # (rotation 0), the same as columns (rotation 90), the square lattice;
+    # masses sqrt(3) r^2 rho, sqrt(3) r^2 rho, 4 r^2 rho
+    V2, L2, s3 = (0.1, -0.2), (3e-3, 2e-3), np.sqrt(3.0)
+
+    def pts(kind):
+        out = []
+        if kind == (1, 0):
+            j = 0
+            while (V2[1] if j == 0 else V2[1] + (2.0 * j + 1) * r) <= V2[1] + L2[1] - r:
+                i = 0
+                while (V2[0] if i == 0 else V2[0] + (2.0 * i + 1) * r) <= V2[0] + L2[0] - r:
+                    out.append((V2[0] + (2.0 * i + 1) * r, V2[1] + (2.0 * j + 1) * r))
+                    i += 1
+                j += 1
+        elif kind == (0, 0):
+            j = 0
+            while (V2[1] if j == 0 else V2[1] + (s3 * j + 1) * r) <= V2[1] + L2[1] - r:
+                i = 0
+                while (V2[0] if i == 0 else V2[0] + (2 * i + (j % 2) + 1) * r) <= V2[0] + L2[0] - r:
+                    out.append((V2[0] + (2 * i + (j % 2) + 1) * r, V2[1] + (s3 * j + 1) * r))
+                    i += 1
+                j += 1
+        else:
+            i = 0
+            while (V2[0] if i == 0 else V2[0] + (s3 * i + 1) * r) <= V2[0] + L2[0] - r:
+                j = 0
+                while (V2[1] if j == 0 else V2[1] + (2 * j + (i % 2) + 1) * r) <= V2[1] + L2[1] - r:
+                    out.append((V2[0] + (s3 * i + 1) * r, V2[1] + (2 * j + (i % 2) + 1) * r))
+                    j += 1
+                i += 1
+        return np.array(out)
+
+    got2 = {(int(s.split()[0]), int(s.split()[1])): [float(v) for v in s.split()[2:]] for s in out["box2d"]}
+    for kind, m in (((0, 0), s3 * r * r * rho), ((0, 90), s3 * r * r * rho), ((1, 0), 4 * r * r * rho)):
+        P = pts(kind)
+        g = got2[kind]
+        assert int(g[0]) == len(P) > 20, kind
+        assert g[1] == pytest.approx(m, rel=1e-15), kind
+        assert g[2] == pytest.approx(P[:, 0].sum(), rel=1e-13) and g[3] == pytest.approx(P[:, 1].sum(), rel=1e-13)
+        assert g[4] == 0.0
     # cylinder: the 4,524-particle lattice, mass pi R^2 L rho / N, one bottom row
     c = out["cyl"][0].split()
     assert int(c[0]) == 4524 and int(c[2]) == 156

@@ -75,7 +75,7 @@ def test_material_initial_yield_quirks(tmp_path):
     (lambda s: s["Configuration"].update(solver="Mech-Fraser"), "Fraser"),
     (lambda s: s["Configuration"].update(solver="Thermal"), "thermal-only"),
     (lambda s: s["DomainBlocks"][0].update(gridCoordSys="Cylindrical"), "Cylindrical"),
-    (lambda s: s["DomainBlocks"][0].update(type="Box", dim=[0.01, 0.01, 0.0005]), "3D only"),
+    (lambda s: s["DomainBlocks"][0].update(dim=[0.015, 0.0, 0.0005]), "only the 3D generator"),
     (lambda s: s["Materials"][0].update(type="GMT"), "GMT"),
     (lambda s: s["Materials"][0].update(damageModel="Rankine"), "damage"),
     (lambda s: s["Configuration"].update(smoothlenUpdate=True), "smoothlenUpdate"),
@@ -87,6 +87,17 @@ def test_scene_features_off_the_path_stop(tmp_path, edit, message):
     edit(s)
     rc, out, err = _dry(tmp_path, s)
     assert rc == 1 and message in err, (out, err)
+
+
+def test_thin_box_scene_is_plane_strain(tmp_path):
+    """A Box block thinner than h is 2D (Source/WeldForm.cpp:420-424): the
+    square packing of pitch 2r in the plane, 14 x 14 particles for a 15 mm
+    square at r = 0.5 mm (the generator stops a centre short of L - r)."""
+    s = json.load(open(SCENE))
+    s["DomainBlocks"][0].update(type="Box", dim=[0.015, 0.015, 0.0])
+    rc, out, err = _dry(tmp_path, s)
+    assert rc == 0, err
+    assert "Dimension is set to 2" in out and "196 particles (196 solid)" in out
 
 
 def test_contact_scene_adds_rigid_particles(tmp_path):
